@@ -1,0 +1,258 @@
+"""Deterministic synthetic soft-body scenes (meshes, attributes, colliders).
+
+A scene generator returns *mesh specs*: vertex records plus faces / explicit
+constraints and the per-vertex attributes that the settings builder
+(CreateConstraints / Calculate* / Optimize, SoftBodySharedSettings.cpp:189-1182)
+turns into the constraint arrays.  The same spec is fed to the product's host
+builder and, in the tests, to the reference's own builder, so both sides run
+on identical inputs.
+
+Random numbers come from numpy's PCG64 with fixed seeds and are drawn as
+float64 then rounded once to float32, so the scenes are reproducible across
+machines.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .records import (BEND_DIHEDRAL, EDGE, LRA_NONE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_PLANE,
+                      SHAPE_SPHERE, VERTEX, VOLUME, BodyParams, Collider)
+
+# Build flags (same bit meaning on the product and oracle builders)
+CREATE_CONSTRAINTS = 1
+CALC_EDGE_LENGTHS = 2
+CALC_VOLUMES = 4
+CALC_LRA_LENGTHS = 8
+OPTIMIZE = 16
+CALC_BEND_CONSTANTS = 32
+
+
+@dataclass
+class MeshSpec:
+    vertices: np.ndarray                       # VERTEX records
+    faces: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.uint32))
+    edges: np.ndarray = field(default_factory=lambda: np.zeros(0, EDGE))
+    volumes: np.ndarray = field(default_factory=lambda: np.zeros(0, VOLUME))
+    attrs: np.ndarray = field(default_factory=lambda: np.zeros((0, 5), np.float32))
+    bend_type: int = BEND_DIHEDRAL
+    angle_tolerance: float = float(np.float32(np.deg2rad(8.0)))
+    lra_multiplier: float = 1.0
+    flags: int = CREATE_CONSTRAINTS | OPTIMIZE
+
+
+@dataclass
+class BodySpec:
+    mesh: MeshSpec
+    position: tuple = (0.0, 0.0, 0.0)
+    rotation: tuple = (0.0, 0.0, 0.0, 1.0)
+    params: BodyParams = field(default_factory=BodyParams)
+    # optional per-constraint compliance override applied after the build
+    # (callable(settings_arrays) -> None)
+    post: object = None
+
+
+@dataclass
+class Scene:
+    name: str
+    bodies: list
+    colliders: list
+    gravity: tuple = (0.0, 0.0, -9.81)
+    dt: float = 1.0 / 60.0
+    collision_steps: int = 1
+    frames: int = 1
+
+
+def attrs(compliance, shear, bend, lra_type=LRA_NONE, lra_mult=1.0, n=1):
+    a = np.zeros((n, 5), np.float32)
+    a[:, 0], a[:, 1], a[:, 2], a[:, 3], a[:, 4] = compliance, shear, bend, lra_type, lra_mult
+    return a
+
+
+# ------------------------------------------------------------------ meshes
+def cloth(n, spacing, inv_mass=100.0, pinned=(), jitter=0.0, seed=0, nz=None):
+    """n x nz vertex cloth in the x-y plane, centred on the origin.
+
+    Vertex (x, y) has index x + y * n.  Each quad is split along its (x,y)-(x+1,y+1)
+    diagonal into two counter-clockwise (seen from +z) triangles.
+    """
+    nz = n if nz is None else nz
+    xs = (np.arange(n, dtype=np.float64) - 0.5 * (n - 1)) * spacing
+    ys = (np.arange(nz, dtype=np.float64) - 0.5 * (nz - 1)) * spacing
+    X, Y = np.meshgrid(xs, ys, indexing="xy")
+    pos = np.stack([X.ravel(), Y.ravel(), np.zeros(n * nz)], axis=1)
+    if jitter > 0:
+        rng = np.random.Generator(np.random.PCG64(seed))
+        pos = pos + rng.uniform(-jitter, jitter, size=pos.shape)
+    v = np.zeros(n * nz, VERTEX)
+    v["pos"] = pos.astype(np.float32)
+    v["inv_mass"] = np.float32(inv_mass)
+    for p in pinned:
+        v["inv_mass"][p] = 0.0
+    idx = np.arange(n * nz, dtype=np.uint32).reshape(nz, n)
+    a, b = idx[:-1, :-1].ravel(), idx[:-1, 1:].ravel()
+    c, d = idx[1:, :-1].ravel(), idx[1:, 1:].ravel()
+    faces = np.empty((2 * a.size, 3), np.uint32)
+    faces[0::2] = np.stack([a, b, d], 1)
+    faces[1::2] = np.stack([a, d, c], 1)
+    return v, faces
+
+
+_CORNERS = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0],
+                     [0, 0, 1], [1, 0, 1], [0, 1, 1], [1, 1, 1]], dtype=np.int64)
+# Five-tetrahedron split of a cell; the two mirror variants alternate with the
+# parity of x+y+z so the face diagonals of neighbouring cells agree.
+_TETS_EVEN = np.array([[1, 2, 4, 7], [0, 1, 2, 4], [3, 1, 2, 7], [5, 1, 4, 7], [6, 2, 4, 7]])
+_TETS_ODD = np.array([[0, 3, 5, 6], [1, 0, 3, 5], [2, 0, 3, 6], [4, 0, 5, 6], [7, 3, 5, 6]])
+
+
+def tet_cube(cells, spacing, inv_mass=1.0, compliance=1e-6, seed=None, jitter=0.0):
+    """(cells+1)^3-vertex block of 5 tets per cell, centred on the origin.
+
+    Returns vertex records, edge records (unique tet edges sorted by vertex
+    pair, rest length to be computed) and volume records (rest volume to be
+    computed).
+    """
+    g = cells + 1
+    r = (np.arange(g, dtype=np.float64) - 0.5 * cells) * spacing
+    Z, Y, X = np.meshgrid(r, r, r, indexing="ij")
+    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    if jitter > 0:
+        rng = np.random.Generator(np.random.PCG64(seed))
+        pos = pos + rng.uniform(-jitter, jitter, size=pos.shape)
+    v = np.zeros(g ** 3, VERTEX)
+    v["pos"] = pos.astype(np.float32)
+    v["inv_mass"] = np.float32(inv_mass)
+
+    cz, cy, cx = np.meshgrid(np.arange(cells), np.arange(cells), np.arange(cells), indexing="ij")
+    cx, cy, cz = cx.ravel(), cy.ravel(), cz.ravel()
+    corner_idx = ((cx[:, None] + _CORNERS[None, :, 0]) + (cy[:, None] + _CORNERS[None, :, 1]) * g
+                  + (cz[:, None] + _CORNERS[None, :, 2]) * g * g)          # [cells^3, 8]
+    odd = ((cx + cy + cz) & 1).astype(bool)
+    tets = np.where(odd[:, None, None], corner_idx[:, _TETS_ODD], corner_idx[:, _TETS_EVEN])
+    tets = tets.reshape(-1, 4).astype(np.uint32)
+
+    vol = np.zeros(len(tets), VOLUME)
+    vol["v"] = tets
+    vol["compliance"] = np.float32(compliance)
+
+    pairs = np.concatenate([tets[:, [i, j]] for i in range(4) for j in range(i + 1, 4)])
+    pairs = np.sort(pairs, axis=1)
+    pairs = np.unique(pairs, axis=0)
+    e = np.zeros(len(pairs), EDGE)
+    e["v"] = pairs
+    e["compliance"] = np.float32(compliance)
+    return v, e, vol
+
+
+def icosphere(subdivisions, radius=1.0, inv_mass=1.0):
+    """Icosahedron subdivided `subdivisions` times and projected to a sphere
+    (outward counter-clockwise faces)."""
+    t = (1.0 + 5.0 ** 0.5) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t),
+             (0, -1, -t), (0, 1, -t), (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+             (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8),
+             (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    verts = [np.array(p, np.float64) / np.linalg.norm(p) for p in verts]
+    for _ in range(subdivisions):
+        cache = {}
+
+        def mid(i, j):
+            key = (min(i, j), max(i, j))
+            if key not in cache:
+                m = verts[i] + verts[j]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    v = np.zeros(len(verts), VERTEX)
+    v["pos"] = (np.array(verts) * radius).astype(np.float32)
+    v["inv_mass"] = np.float32(inv_mass)
+    return v, np.array(faces, np.uint32)
+
+
+# ------------------------------------------------------------------ colliders
+def _rot_matrix(q):
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def collider_world_bounds(c: Collider):
+    """World-space AABB of a static collider body (conservative for planes)."""
+    p = np.array(c.pos, np.float64)
+    R = _rot_matrix(c.rot)
+    if c.type == SHAPE_SPHERE:
+        he = np.full(3, c.params[0])
+        return tuple(np.concatenate([p - he, p + he]).astype(np.float32))
+    if c.type == SHAPE_BOX:
+        he = np.abs(R) @ np.array(c.params[:3], np.float64)
+    elif c.type == SHAPE_CAPSULE:
+        hh, r = c.params[0], c.params[1]
+        he = np.abs(R) @ np.array([r, hh + r, r]) + 0.0
+    else:  # plane: square of half extent around the plane point, extruded half extent behind
+        n = np.array(c.params[:3], np.float64)
+        he = np.full(3, 2.0 * c.params[4])
+        centre = -n * c.params[3]
+        lo, hi = R @ centre + p - he, R @ centre + p + he
+        return tuple(np.concatenate([lo, hi]).astype(np.float32))
+    return tuple(np.concatenate([p - he, p + he]).astype(np.float32))
+
+
+# ------------------------------------------------------------------ configs
+def config1_cloth_on_box(frames=300):
+    """32x32 cloth (spacing 0.05 m, inv-mass 100, two top corners pinned), structural +
+    shear edges (compliance 1e-5) and dihedral bends (1e-3), jitter U(-1e-4, 1e-4) seed 1,
+    gravity -9.81 z, dt 1/60, 1 collision step x 5 iterations, over a static
+    1 x 1 x 0.2 m box whose top is 0.5 m below the cloth; sleeping off."""
+    n = 32
+    v, f = cloth(n, 0.05, 100.0, pinned=((n - 1) * n, n * n - 1), jitter=1e-4, seed=1)
+    mesh = MeshSpec(vertices=v, faces=f, attrs=attrs(1e-5, 1e-5, 1e-3), bend_type=BEND_DIHEDRAL)
+    body = BodySpec(mesh=mesh, params=BodyParams(num_iterations=5, allow_sleeping=False))
+    box = Collider(SHAPE_BOX, (0.5, 0.5, 0.1), pos=(0.0, 0.0, -0.6))
+    box.bounds = collider_world_bounds(box)
+    return Scene("cloth32_on_box", [body], [box], gravity=(0.0, 0.0, -9.81), dt=1.0 / 60.0,
+                 collision_steps=1, frames=frames)
+
+
+def _log_uniform(rng, lo, hi, size):
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), size=size))
+
+
+def config2_cloth_and_cube(cloth_n=512, cube_cells=48, num_spheres=64, seed=2, frames=1):
+    """512^2 cloth (spacing 0.02 m, per-constraint compliance log-U[1e-6, 1e-3]) plus a
+    48^3-cell x 5-tet cube (edge + volume compliance 1e-6), ground plane and 64 static
+    spheres r ~ U(0.1, 0.5); 4 collision steps x 8 iterations."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    v, f = cloth(cloth_n, 0.02, 100.0, jitter=1e-4, seed=seed)
+    mesh = MeshSpec(vertices=v, faces=f, attrs=attrs(1e-4, 1e-4, 1e-4), bend_type=BEND_DIHEDRAL)
+
+    def cloth_compliance(s):
+        r = np.random.Generator(np.random.PCG64(seed + 100))
+        s.edges["compliance"] = _log_uniform(r, 1e-6, 1e-3, len(s.edges)).astype(np.float32)
+        s.bends["compliance"] = _log_uniform(r, 1e-6, 1e-3, len(s.bends)).astype(np.float32)
+
+    half = 0.5 * 0.02 * (cloth_n - 1)
+    cloth_body = BodySpec(mesh=mesh, position=(0.0, 0.0, 3.0),
+                          params=BodyParams(num_iterations=8, allow_sleeping=False), post=cloth_compliance)
+    tv, te, tt = tet_cube(cube_cells, 0.05, inv_mass=1.0, compliance=1e-6)
+    cube_mesh = MeshSpec(vertices=tv, edges=te, volumes=tt, flags=CALC_EDGE_LENGTHS | CALC_VOLUMES | OPTIMIZE)
+    cube_body = BodySpec(mesh=cube_mesh, position=(half + 3.0, 0.0, 2.0),
+                         params=BodyParams(num_iterations=8, allow_sleeping=False))
+    colliders = [Collider(SHAPE_PLANE, (0.0, 0.0, 1.0, 0.0, 1000.0))]
+    for _ in range(num_spheres):
+        r = float(rng.uniform(0.1, 0.5))
+        p = (float(rng.uniform(-half, half)), float(rng.uniform(-half, half)), float(rng.uniform(0.5, 2.0)))
+        colliders.append(Collider(SHAPE_SPHERE, (r,), pos=p))
+    for c in colliders:
+        c.bounds = collider_world_bounds(c)
+    return Scene("cloth512_cube48", [cloth_body, cube_body], colliders, gravity=(0.0, 0.0, -9.81),
+                 dt=1.0 / 60.0, collision_steps=4, frames=frames)

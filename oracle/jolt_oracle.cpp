@@ -1,0 +1,465 @@
+// Oracle harness around the reference Jolt Physics library (compiled from the
+// reference sources by oracle/Makefile into oracle/_ref/).  TEST INFRASTRUCTURE
+// ONLY: it is loaded by tests/ and by bench.py's cpu_baseline leg as the
+// checker / CPU baseline, never by the product path.
+//
+// It exposes a flat C API so Python (ctypes) can
+//   * build a SoftBodySharedSettings with the reference's own CreateConstraints /
+//     Calculate* / Optimize (SoftBodySharedSettings.cpp:189-369, 371-379,
+//     483-491, 536-555, 613-1182) and read every array back, including the
+//     private mUpdateGroups (through the public SaveBinaryState,
+//     SoftBodySharedSettings.cpp:1203-1236);
+//   * build a SoftBodySharedSettings from explicit arrays (through the public
+//     RestoreBinaryState, SoftBodySharedSettings.cpp:1238-1268), so the
+//     reference runs on exactly the arrays the product consumes;
+//   * set up a PhysicsSystem with static plane / sphere / box / capsule bodies
+//     and soft bodies, step it with PhysicsSystem::Update
+//     (PhysicsSystem.cpp:176) and read the soft-body vertex state back.
+//
+// Record layouts are the reference's in-memory ones (SoftBodySharedSettings.h):
+//   Vertex 28 B {float pos[3], vel[3], invMass}
+//   Face 16 B {u32 v[3], material}
+//   Edge 16 B {u32 v[2]; float restLength, compliance}
+//   DihedralBend 24 B {u32 v[4]; float compliance, initialAngle}
+//   Volume 24 B {u32 v[4]; float sixRestVolume, compliance}
+//   LRA 12 B {u32 v[2]; float maxDistance}
+//   UpdateGroup 28 B {u32 edgeEnd, lraEnd, rodSSEnd, rodBTEnd, dihedralEnd, volumeEnd, skinnedEnd}
+
+#include <Jolt/Jolt.h>
+#include <Jolt/RegisterTypes.h>
+#include <Jolt/Core/Factory.h>
+#include <Jolt/Core/TempAllocator.h>
+#include <Jolt/Core/JobSystemThreadPool.h>
+#include <Jolt/Core/StreamWrapper.h>
+#include <Jolt/Physics/PhysicsSettings.h>
+#include <Jolt/Physics/PhysicsSystem.h>
+#include <Jolt/Physics/Body/BodyCreationSettings.h>
+#include <Jolt/Physics/Body/BodyLock.h>
+#include <Jolt/Physics/Collision/Shape/BoxShape.h>
+#include <Jolt/Physics/Collision/Shape/SphereShape.h>
+#include <Jolt/Physics/Collision/Shape/CapsuleShape.h>
+#include <Jolt/Physics/Collision/Shape/PlaneShape.h>
+#include <Jolt/Physics/SoftBody/SoftBodySharedSettings.h>
+#include <Jolt/Physics/SoftBody/SoftBodyCreationSettings.h>
+#include <Jolt/Physics/SoftBody/SoftBodyMotionProperties.h>
+
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <thread>
+#include <vector>
+
+using namespace JPH;
+
+static_assert(sizeof(SoftBodySharedSettings::Vertex) == 28, "vertex record");
+static_assert(sizeof(SoftBodySharedSettings::Face) == 16, "face record");
+static_assert(sizeof(SoftBodySharedSettings::Edge) == 16, "edge record");
+static_assert(sizeof(SoftBodySharedSettings::DihedralBend) == 24, "bend record");
+static_assert(sizeof(SoftBodySharedSettings::Volume) == 24, "volume record");
+static_assert(sizeof(SoftBodySharedSettings::LRA) == 12, "lra record");
+
+namespace {
+
+// ---------------------------------------------------------------- one-time init
+std::once_flag g_init_once;
+void InitJolt()
+{
+	std::call_once(g_init_once, [] {
+		RegisterDefaultAllocator();
+		Factory::sInstance = new Factory();
+		RegisterTypes();
+	});
+}
+
+// ---------------------------------------------------------------- layers
+namespace Layers { constexpr ObjectLayer STATIC = 0, MOVING = 1, NUM = 2; }
+namespace BPLayers { constexpr BroadPhaseLayer STATIC(0), MOVING(1); constexpr uint NUM = 2; }
+
+class BPLayerMap final : public BroadPhaseLayerInterface
+{
+public:
+	uint GetNumBroadPhaseLayers() const override { return BPLayers::NUM; }
+	BroadPhaseLayer GetBroadPhaseLayer(ObjectLayer inLayer) const override { return inLayer == Layers::STATIC ? BPLayers::STATIC : BPLayers::MOVING; }
+#if defined(JPH_EXTERNAL_PROFILE) || defined(JPH_PROFILE_ENABLED)
+	const char *GetBroadPhaseLayerName(BroadPhaseLayer inLayer) const override { return inLayer == BPLayers::STATIC ? "STATIC" : "MOVING"; }
+#endif
+};
+
+class ObjVsBP final : public ObjectVsBroadPhaseLayerFilter
+{
+public:
+	bool ShouldCollide(ObjectLayer inLayer, BroadPhaseLayer inBP) const override
+	{
+		// Moving things see everything, static things see only moving things
+		return inLayer == Layers::MOVING || inBP == BPLayers::MOVING;
+	}
+};
+
+class ObjVsObj final : public ObjectLayerPairFilter
+{
+public:
+	bool ShouldCollide(ObjectLayer a, ObjectLayer b) const override { return a == Layers::MOVING || b == Layers::MOVING; }
+};
+
+// ---------------------------------------------------------------- settings
+struct SettingsHandle
+{
+	Ref<SoftBodySharedSettings> mSettings;
+};
+
+struct World
+{
+	BPLayerMap mBP;
+	ObjVsBP mObjVsBP;
+	ObjVsObj mObjVsObj;
+	PhysicsSystem mSystem;
+	std::unique_ptr<TempAllocatorImpl> mTemp;
+	std::unique_ptr<JobSystemThreadPool> mJobs;
+	std::vector<BodyID> mSoftBodies;
+	std::vector<BodyID> mColliders;
+};
+
+template <class T>
+void WriteArray(std::ostream &os, const T *data, uint32 count)
+{
+	os.write((const char *)&count, 4);
+	if (count)
+		os.write((const char *)data, std::streamsize(sizeof(T)) * count);
+}
+
+template <class T>
+std::vector<T> ReadArray(std::istream &is)
+{
+	uint32 n = 0;
+	is.read((char *)&n, 4);
+	std::vector<T> v(n);
+	if (n)
+		is.read((char *)v.data(), std::streamsize(sizeof(T)) * n);
+	return v;
+}
+
+struct UpdateGroupRec { uint32 e, l, rss, rbt, d, v, s; };
+
+// All arrays of a settings object, private update groups included.
+struct Arrays
+{
+	std::vector<SoftBodySharedSettings::Vertex> v;
+	std::vector<SoftBodySharedSettings::Face> f;
+	std::vector<SoftBodySharedSettings::Edge> e;
+	std::vector<SoftBodySharedSettings::DihedralBend> d;
+	std::vector<SoftBodySharedSettings::Volume> t;
+	std::vector<SoftBodySharedSettings::LRA> l;
+	std::vector<UpdateGroupRec> g;
+};
+
+Arrays ReadBack(const SoftBodySharedSettings &s)
+{
+	std::stringstream ss;
+	StreamOutWrapper out(ss);
+	s.SaveBinaryState(out);
+	Arrays a;
+	std::istream &is = ss;
+	a.v = ReadArray<SoftBodySharedSettings::Vertex>(is);
+	a.f = ReadArray<SoftBodySharedSettings::Face>(is);
+	a.e = ReadArray<SoftBodySharedSettings::Edge>(is);
+	a.d = ReadArray<SoftBodySharedSettings::DihedralBend>(is);
+	a.t = ReadArray<SoftBodySharedSettings::Volume>(is);
+	uint32 n_skinned = 0; is.read((char *)&n_skinned, 4);
+	uint32 n_skinned_normals = 0; is.read((char *)&n_skinned_normals, 4);
+	a.l = ReadArray<SoftBodySharedSettings::LRA>(is);
+	a.g = ReadArray<UpdateGroupRec>(is);
+	return a;
+}
+
+} // namespace
+
+extern "C" {
+
+// ------------------------------------------------------------------ settings
+
+// Builds settings with the reference's own generation code.
+//   vertices: V records of 28 B; faces: F x u32[3]; edges/volumes/lras: explicit
+//   constraint records (may be 0); attrs: A x {compliance, shear, bend, lraType,
+//   lraMultiplier} as 5 floats (lraType as float 0/1/2).
+//   flags bit0 CreateConstraints, bit1 CalculateEdgeLengths, bit2
+//   CalculateVolumeConstraintVolumes, bit3 CalculateLRALengths, bit4 Optimize,
+//   bit5 CalculateBendConstraintConstants (when bends are given explicitly).
+void *jo_settings_build(const void *vertices, uint32 nv, const uint32 *faces, uint32 nf,
+						const void *edges, uint32 ne, const void *bends, uint32 nb,
+						const void *volumes, uint32 nt, const void *lras, uint32 nl,
+						const float *attrs, uint32 na, int bend_type, float angle_tolerance,
+						float lra_multiplier, uint32 flags)
+{
+	InitJolt();
+	Ref<SoftBodySharedSettings> s = new SoftBodySharedSettings;
+	s->mVertices.resize(nv);
+	if (nv) memcpy(s->mVertices.data(), vertices, 28ull * nv);
+	for (uint32 i = 0; i < nf; ++i)
+		s->AddFace(SoftBodySharedSettings::Face(faces[3 * i], faces[3 * i + 1], faces[3 * i + 2]));
+	s->mEdgeConstraints.resize(ne);
+	if (ne) memcpy(s->mEdgeConstraints.data(), edges, 16ull * ne);
+	s->mDihedralBendConstraints.resize(nb);
+	if (nb) memcpy(s->mDihedralBendConstraints.data(), bends, 24ull * nb);
+	s->mVolumeConstraints.resize(nt);
+	if (nt) memcpy(s->mVolumeConstraints.data(), volumes, 24ull * nt);
+	s->mLRAConstraints.resize(nl);
+	if (nl) memcpy(s->mLRAConstraints.data(), lras, 12ull * nl);
+
+	if (flags & 1)
+	{
+		std::vector<SoftBodySharedSettings::VertexAttributes> va(na);
+		for (uint32 i = 0; i < na; ++i)
+		{
+			va[i].mCompliance = attrs[5 * i + 0];
+			va[i].mShearCompliance = attrs[5 * i + 1];
+			va[i].mBendCompliance = attrs[5 * i + 2];
+			va[i].mLRAType = (SoftBodySharedSettings::ELRAType)int(attrs[5 * i + 3]);
+			va[i].mLRAMaxDistanceMultiplier = attrs[5 * i + 4];
+		}
+		s->CreateConstraints(va.data(), na, (SoftBodySharedSettings::EBendType)bend_type, angle_tolerance);
+	}
+	if (flags & 2) s->CalculateEdgeLengths();
+	if (flags & 4) s->CalculateVolumeConstraintVolumes();
+	if (flags & 8) s->CalculateLRALengths(lra_multiplier);
+	if (flags & 32) s->CalculateBendConstraintConstants();
+	if (flags & 16) s->Optimize();
+	return new SettingsHandle{ s };
+}
+
+// Builds settings from explicit arrays (everything already computed, including
+// the update groups) through the public RestoreBinaryState.
+void *jo_settings_from_arrays(const void *vertices, uint32 nv, const uint32 *faces4, uint32 nf,
+							  const void *edges, uint32 ne, const void *bends, uint32 nb,
+							  const void *volumes, uint32 nt, const void *lras, uint32 nl,
+							  const uint32 *groups7, uint32 ng)
+{
+	InitJolt();
+	std::stringstream ss;
+	WriteArray(ss, (const char(*)[28])vertices, nv);
+	WriteArray(ss, (const char(*)[16])faces4, nf);
+	WriteArray(ss, (const char(*)[16])edges, ne);
+	WriteArray(ss, (const char(*)[24])bends, nb);
+	WriteArray(ss, (const char(*)[24])volumes, nt);
+	uint32 zero = 0;
+	ss.write((const char *)&zero, 4);				// skinned
+	ss.write((const char *)&zero, 4);				// skinned normals
+	WriteArray(ss, (const char(*)[12])lras, nl);
+	WriteArray(ss, (const char(*)[28])groups7, ng);
+	ss.write((const char *)&zero, 4);				// rod stretch shear
+	ss.write((const char *)&zero, 4);				// rod bend twist
+	ss.write((const char *)&zero, 4);				// inv bind matrices
+	ss.seekg(0);
+	Ref<SoftBodySharedSettings> s = new SoftBodySharedSettings;
+	StreamInWrapper in(ss);
+	s->RestoreBinaryState(in);
+	if (in.IsFailed())
+		return nullptr;
+	return new SettingsHandle{ s };
+}
+
+// counts[7] = V, F, E, B, T, L, G
+void jo_settings_counts(void *h, uint32 *counts)
+{
+	Arrays a = ReadBack(*((SettingsHandle *)h)->mSettings);
+	counts[0] = (uint32)a.v.size(); counts[1] = (uint32)a.f.size(); counts[2] = (uint32)a.e.size();
+	counts[3] = (uint32)a.d.size(); counts[4] = (uint32)a.t.size(); counts[5] = (uint32)a.l.size();
+	counts[6] = (uint32)a.g.size();
+}
+
+// Copies all arrays out (buffers sized from jo_settings_counts; null skips).
+void jo_settings_read(void *h, void *vertices, void *faces4, void *edges, void *bends, void *volumes, void *lras, void *groups7)
+{
+	Arrays a = ReadBack(*((SettingsHandle *)h)->mSettings);
+	if (vertices && !a.v.empty()) memcpy(vertices, a.v.data(), 28 * a.v.size());
+	if (faces4 && !a.f.empty()) memcpy(faces4, a.f.data(), 16 * a.f.size());
+	if (edges && !a.e.empty()) memcpy(edges, a.e.data(), 16 * a.e.size());
+	if (bends && !a.d.empty()) memcpy(bends, a.d.data(), 24 * a.d.size());
+	if (volumes && !a.t.empty()) memcpy(volumes, a.t.data(), 24 * a.t.size());
+	if (lras && !a.l.empty()) memcpy(lras, a.l.data(), 12 * a.l.size());
+	if (groups7 && !a.g.empty()) memcpy(groups7, a.g.data(), 28 * a.g.size());
+}
+
+void jo_settings_free(void *h) { delete (SettingsHandle *)h; }
+
+// Reference ACosApproximate (Math/Trigonometry.h:48-65) for the math pins.
+float jo_acos_approximate(float x) { return ACosApproximate(x); }
+
+// ------------------------------------------------------------------ world
+
+// num_threads: worker threads of the JobSystemThreadPool (0 = run jobs on the
+// calling thread, as UnitTests/PhysicsTestContext.h:28 does).
+void *jo_world_create(uint32 max_bodies, int num_threads)
+{
+	InitJolt();
+	World *w = new World;
+	w->mSystem.Init(max_bodies, 0, max_bodies, max_bodies, w->mBP, w->mObjVsBP, w->mObjVsObj);
+	w->mTemp = std::make_unique<TempAllocatorImpl>(256 * 1024 * 1024);
+	w->mJobs = std::make_unique<JobSystemThreadPool>(cMaxPhysicsJobs, cMaxPhysicsBarriers, num_threads);
+	PhysicsSettings ps = w->mSystem.GetPhysicsSettings();
+	ps.mAllowSleeping = true;
+	w->mSystem.SetPhysicsSettings(ps);
+	return w;
+}
+
+void jo_world_free(void *wp) { delete (World *)wp; }
+
+void jo_world_set_gravity(void *wp, float gx, float gy, float gz)
+{
+	((World *)wp)->mSystem.SetGravity(Vec3(gx, gy, gz));
+}
+
+// Static collider. type 0 plane {nx,ny,nz,c, halfExtent}, 1 sphere {r},
+// 2 box {hx,hy,hz}, 3 capsule {halfHeight, r}.  pos[3], rot[4] (x,y,z,w).
+int jo_world_add_collider(void *wp, int type, const float *params, const float *pos, const float *rot, float friction, float restitution)
+{
+	World *w = (World *)wp;
+	Ref<Shape> shape;
+	switch (type)
+	{
+	case 0: shape = new PlaneShape(Plane(Vec3(params[0], params[1], params[2]), params[3]), nullptr, params[4]); break;
+	case 1: shape = new SphereShape(params[0]); break;
+	case 2: shape = new BoxShape(Vec3(params[0], params[1], params[2]), 0.0f); break;
+	case 3: shape = new CapsuleShape(params[0], params[1]); break;
+	default: return -1;
+	}
+	BodyCreationSettings bcs(shape, RVec3(pos[0], pos[1], pos[2]), Quat(rot[0], rot[1], rot[2], rot[3]), EMotionType::Static, Layers::STATIC);
+	bcs.mFriction = friction;
+	bcs.mRestitution = restitution;
+	BodyID id = w->mSystem.GetBodyInterface().CreateAndAddBody(bcs, EActivation::DontActivate);
+	if (id.IsInvalid())
+		return -1;
+	w->mColliders.push_back(id);
+	return int(w->mColliders.size() - 1);
+}
+
+// Soft body.  p: {numIterations, linearDamping, maxLinearVelocity, restitution,
+// friction, pressure, gravityFactor, vertexRadius, updatePosition,
+// makeRotationIdentity, allowSleeping} as 11 floats.
+int jo_world_add_soft_body(void *wp, void *settings, const float *pos, const float *rot, const float *p)
+{
+	World *w = (World *)wp;
+	SoftBodyCreationSettings cs(((SettingsHandle *)settings)->mSettings, RVec3(pos[0], pos[1], pos[2]), Quat(rot[0], rot[1], rot[2], rot[3]), Layers::MOVING);
+	cs.mNumIterations = uint32(p[0]);
+	cs.mLinearDamping = p[1];
+	cs.mMaxLinearVelocity = p[2];
+	cs.mRestitution = p[3];
+	cs.mFriction = p[4];
+	cs.mPressure = p[5];
+	cs.mGravityFactor = p[6];
+	cs.mVertexRadius = p[7];
+	cs.mUpdatePosition = p[8] != 0.0f;
+	cs.mMakeRotationIdentity = p[9] != 0.0f;
+	cs.mAllowSleeping = p[10] != 0.0f;
+	BodyID id = w->mSystem.GetBodyInterface().CreateAndAddSoftBody(cs, EActivation::Activate);
+	if (id.IsInvalid())
+		return -1;
+	w->mSoftBodies.push_back(id);
+	return int(w->mSoftBodies.size() - 1);
+}
+
+void jo_world_optimize_broadphase(void *wp) { ((World *)wp)->mSystem.OptimizeBroadPhase(); }
+
+// World-space bounds of a collider body (Body::GetWorldSpaceBounds), min xyz, max xyz.
+int jo_world_collider_bounds(void *wp, int collider, float *out6)
+{
+	World *w = (World *)wp;
+	BodyLockRead lock(w->mSystem.GetBodyLockInterface(), w->mColliders[collider]);
+	if (!lock.Succeeded())
+		return -1;
+	const AABox &b = lock.GetBody().GetWorldSpaceBounds();
+	out6[0] = b.mMin.GetX(); out6[1] = b.mMin.GetY(); out6[2] = b.mMin.GetZ();
+	out6[3] = b.mMax.GetX(); out6[4] = b.mMax.GetY(); out6[5] = b.mMax.GetZ();
+	return 0;
+}
+
+// Runs `frames` calls of PhysicsSystem::Update(dt, collision_steps).  Returns
+// the summed wall time of the Update calls in seconds (chrono around each
+// call, as PerformanceTest.cpp:302-391 does), or a negative value on error.
+double jo_world_update(void *wp, float dt, int collision_steps, int frames)
+{
+	World *w = (World *)wp;
+	double total = 0.0;
+	for (int i = 0; i < frames; ++i)
+	{
+		auto t0 = std::chrono::high_resolution_clock::now();
+		EPhysicsUpdateError err = w->mSystem.Update(dt, collision_steps, w->mTemp.get(), w->mJobs.get());
+		auto t1 = std::chrono::high_resolution_clock::now();
+		if (err != EPhysicsUpdateError::None)
+			return -1.0;
+		total += std::chrono::duration<double>(t1 - t0).count();
+	}
+	return total;
+}
+
+// Single-body public entry point SoftBodyMotionProperties::CustomUpdate
+// (SoftBodyMotionProperties.cpp:1232-1256).
+int jo_body_custom_update(void *wp, int body, float dt)
+{
+	World *w = (World *)wp;
+	BodyLockWrite lock(w->mSystem.GetBodyLockInterface(), w->mSoftBodies[body]);
+	if (!lock.Succeeded())
+		return -1;
+	Body &b = lock.GetBody();
+	SoftBodyMotionProperties *mp = static_cast<SoftBodyMotionProperties *>(b.GetMotionProperties());
+	lock.ReleaseLock();
+	mp->CustomUpdate(dt, b, w->mSystem);
+	return 0;
+}
+
+// Vertex state in body-local space.  Any output may be null.
+//   pos/prev/vel: V x 3 floats; plane: V x 4; shape_idx: V ints;
+//   penetration: V floats; contact: V ints; inv_mass: V floats.
+// body_pos[3], body_rot[4]: body position / rotation.
+int jo_body_read(void *wp, int body, float *pos, float *prev, float *vel, float *plane, int *shape_idx, float *penetration, int *contact, float *inv_mass, float *body_pos, float *body_rot)
+{
+	World *w = (World *)wp;
+	BodyLockRead lock(w->mSystem.GetBodyLockInterface(), w->mSoftBodies[body]);
+	if (!lock.Succeeded())
+		return -1;
+	const Body &b = lock.GetBody();
+	const SoftBodyMotionProperties *mp = static_cast<const SoftBodyMotionProperties *>(b.GetMotionProperties());
+	const Array<SoftBodyVertex> &verts = mp->GetVertices();
+	for (size_t i = 0; i < verts.size(); ++i)
+	{
+		const SoftBodyVertex &v = verts[i];
+		if (pos) { pos[3 * i] = v.mPosition.GetX(); pos[3 * i + 1] = v.mPosition.GetY(); pos[3 * i + 2] = v.mPosition.GetZ(); }
+		if (prev) { prev[3 * i] = v.mPreviousPosition.GetX(); prev[3 * i + 1] = v.mPreviousPosition.GetY(); prev[3 * i + 2] = v.mPreviousPosition.GetZ(); }
+		if (vel) { vel[3 * i] = v.mVelocity.GetX(); vel[3 * i + 1] = v.mVelocity.GetY(); vel[3 * i + 2] = v.mVelocity.GetZ(); }
+		if (plane)
+		{
+			Vec3 n = v.mCollisionPlane.GetNormal();
+			plane[4 * i] = n.GetX(); plane[4 * i + 1] = n.GetY(); plane[4 * i + 2] = n.GetZ(); plane[4 * i + 3] = v.mCollisionPlane.GetConstant();
+		}
+		if (shape_idx) shape_idx[i] = v.mCollidingShapeIndex;
+		if (penetration) penetration[i] = v.mLargestPenetration;
+		if (contact) contact[i] = v.mHasContact ? 1 : 0;
+		if (inv_mass) inv_mass[i] = v.mInvMass;
+	}
+	if (body_pos) { RVec3 p = b.GetPosition(); body_pos[0] = float(p.GetX()); body_pos[1] = float(p.GetY()); body_pos[2] = float(p.GetZ()); }
+	if (body_rot) { Quat q = b.GetRotation(); body_rot[0] = q.GetX(); body_rot[1] = q.GetY(); body_rot[2] = q.GetZ(); body_rot[3] = q.GetW(); }
+	return int(verts.size());
+}
+
+// Overwrites vertex positions / velocities (body-local space).  Null skips.
+int jo_body_write(void *wp, int body, const float *pos, const float *vel)
+{
+	World *w = (World *)wp;
+	BodyLockWrite lock(w->mSystem.GetBodyLockInterface(), w->mSoftBodies[body]);
+	if (!lock.Succeeded())
+		return -1;
+	Body &b = lock.GetBody();
+	SoftBodyMotionProperties *mp = static_cast<SoftBodyMotionProperties *>(b.GetMotionProperties());
+	Array<SoftBodyVertex> &verts = mp->GetVertices();
+	for (size_t i = 0; i < verts.size(); ++i)
+	{
+		if (pos) verts[i].mPosition = Vec3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+		if (vel) verts[i].mVelocity = Vec3(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
+	}
+	return 0;
+}
+
+int jo_hardware_concurrency() { return int(std::thread::hardware_concurrency()); }
+
+} // extern "C"

@@ -1,0 +1,305 @@
+"""ctypes front end of the two CPU oracles.  TEST INFRASTRUCTURE ONLY.
+
+* ``JoltReference`` drives the reference library compiled from the reference
+  sources (oracle/_ref/libjolt_oracle_{nofma,fma}.so, built by oracle/Makefile):
+  the reference's own CreateConstraints / Optimize / PhysicsSystem::Update.
+* ``Restatement`` drives the own C++ restatement (oracle/libxpbd_oracle.so) in
+  FP32 (bit-exact target) or FP64 (the error yardstick).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this
+module, and only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from joltphysics_amd.records import (BEND, EDGE, FACE, GROUP, LRA, VERTEX, VOLUME, SettingsArrays)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_DIR = os.path.join(HERE, "_ref")
+
+_vp = C.c_void_p
+_u32 = C.c_uint32
+_f = C.c_float
+_i = C.c_int
+
+
+def _ptr(a):
+    if a is None or (hasattr(a, "size") and a.size == 0):
+        return None
+    return a.ctypes.data_as(_vp)
+
+
+def ref_library_path(variant="nofma"):
+    return os.path.join(REF_DIR, f"libjolt_oracle_{variant}.so")
+
+
+def restatement_path():
+    return os.path.join(HERE, "libxpbd_oracle.so")
+
+
+def have_reference(variant="nofma"):
+    return os.path.exists(ref_library_path(variant))
+
+
+def have_restatement():
+    return os.path.exists(restatement_path())
+
+
+def _faces4(faces):
+    """Face records from either (F,3) index arrays or FACE records."""
+    if faces.dtype == FACE:
+        return np.ascontiguousarray(faces)
+    f = np.zeros(len(faces), FACE)
+    f["v"] = faces
+    return f
+
+
+class JoltReference:
+    """The reference library (compiled from its sources)."""
+
+    _libs = {}
+
+    def __init__(self, variant="nofma"):
+        if variant not in self._libs:
+            lib = C.CDLL(ref_library_path(variant))
+            lib.jo_settings_build.restype = _vp
+            lib.jo_settings_build.argtypes = [_vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32,
+                                              _vp, _u32, _i, _f, _f, _u32]
+            lib.jo_settings_from_arrays.restype = _vp
+            lib.jo_settings_from_arrays.argtypes = [_vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32,
+                                                    _vp, _u32, _vp, _u32]
+            lib.jo_settings_counts.argtypes = [_vp, _vp]
+            lib.jo_settings_read.argtypes = [_vp] * 8
+            lib.jo_settings_free.argtypes = [_vp]
+            lib.jo_acos_approximate.restype = _f
+            lib.jo_acos_approximate.argtypes = [_f]
+            lib.jo_world_create.restype = _vp
+            lib.jo_world_create.argtypes = [_u32, _i]
+            lib.jo_world_free.argtypes = [_vp]
+            lib.jo_world_set_gravity.argtypes = [_vp, _f, _f, _f]
+            lib.jo_world_add_collider.restype = _i
+            lib.jo_world_add_collider.argtypes = [_vp, _i, _vp, _vp, _vp, _f, _f]
+            lib.jo_world_collider_bounds.restype = _i
+            lib.jo_world_collider_bounds.argtypes = [_vp, _i, _vp]
+            lib.jo_world_add_soft_body.restype = _i
+            lib.jo_world_add_soft_body.argtypes = [_vp, _vp, _vp, _vp, _vp]
+            lib.jo_world_optimize_broadphase.argtypes = [_vp]
+            lib.jo_world_update.restype = C.c_double
+            lib.jo_world_update.argtypes = [_vp, _f, _i, _i]
+            lib.jo_body_custom_update.restype = _i
+            lib.jo_body_custom_update.argtypes = [_vp, _i, _f]
+            lib.jo_body_read.restype = _i
+            lib.jo_body_read.argtypes = [_vp, _i] + [_vp] * 10
+            lib.jo_body_write.restype = _i
+            lib.jo_body_write.argtypes = [_vp, _i, _vp, _vp]
+            lib.jo_hardware_concurrency.restype = _i
+            self._libs[variant] = lib
+        self.lib = self._libs[variant]
+        self.variant = variant
+
+    # ---------------------------------------------------------------- settings
+    def build_settings(self, mesh) -> SettingsArrays:
+        """Runs the reference's CreateConstraints / Calculate* / Optimize on a MeshSpec."""
+        v = np.ascontiguousarray(mesh.vertices, VERTEX)
+        f = np.ascontiguousarray(mesh.faces, np.uint32).reshape(-1, 3)
+        e = np.ascontiguousarray(mesh.edges, EDGE)
+        t = np.ascontiguousarray(mesh.volumes, VOLUME)
+        a = np.ascontiguousarray(mesh.attrs, np.float32).reshape(-1, 5)
+        h = self.lib.jo_settings_build(_ptr(v), len(v), _ptr(f), len(f), _ptr(e), len(e), None, 0,
+                                       _ptr(t), len(t), None, 0, _ptr(a), len(a), int(mesh.bend_type),
+                                       float(mesh.angle_tolerance), float(mesh.lra_multiplier), int(mesh.flags))
+        try:
+            return self._read_settings(h)
+        finally:
+            self.lib.jo_settings_free(h)
+
+    def _read_settings(self, h) -> SettingsArrays:
+        counts = np.zeros(7, np.uint32)
+        self.lib.jo_settings_counts(h, _ptr(counts))
+        nv, nf, ne, nb, nt, nl, ng = (int(c) for c in counts)
+        s = SettingsArrays(np.zeros(nv, VERTEX), np.zeros(nf, FACE), np.zeros(ne, EDGE), np.zeros(nb, BEND),
+                           np.zeros(nt, VOLUME), np.zeros(nl, LRA), np.zeros(ng, GROUP))
+        self.lib.jo_settings_read(h, _ptr(s.vertices), _ptr(s.faces), _ptr(s.edges), _ptr(s.bends),
+                                  _ptr(s.volumes), _ptr(s.lras), _ptr(s.groups))
+        return s
+
+    def settings_handle(self, s: SettingsArrays):
+        f4 = _faces4(s.faces)
+        h = self.lib.jo_settings_from_arrays(_ptr(s.vertices), len(s.vertices), _ptr(f4), len(f4),
+                                             _ptr(s.edges), len(s.edges), _ptr(s.bends), len(s.bends),
+                                             _ptr(s.volumes), len(s.volumes), _ptr(s.lras), len(s.lras),
+                                             _ptr(s.groups), len(s.groups))
+        if not h:
+            raise RuntimeError("reference rejected the settings arrays")
+        return h
+
+    def acos_approximate(self, x):
+        return self.lib.jo_acos_approximate(float(x))
+
+    def hardware_concurrency(self):
+        return self.lib.jo_hardware_concurrency()
+
+
+class JoltWorld:
+    """PhysicsSystem with static colliders and soft bodies (reference)."""
+
+    def __init__(self, ref: JoltReference, gravity, max_bodies=4096, num_threads=0):
+        self.ref = ref
+        self.lib = ref.lib
+        self.h = self.lib.jo_world_create(max_bodies, num_threads)
+        self.lib.jo_world_set_gravity(self.h, *map(float, gravity))
+        self._settings = []
+        self.sizes = []
+
+    def close(self):
+        if self.h:
+            for s in self._settings:
+                self.lib.jo_settings_free(s)
+            self.lib.jo_world_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def add_collider(self, c):
+        p = c.param_array()
+        pos = np.array(c.pos, np.float32)
+        rot = np.array(c.rot, np.float32)
+        i = self.lib.jo_world_add_collider(self.h, int(c.type), _ptr(p), _ptr(pos), _ptr(rot),
+                                           float(c.friction), float(c.restitution))
+        if i < 0:
+            raise RuntimeError("add_collider failed")
+        return i
+
+    def collider_bounds(self, i):
+        out = np.zeros(6, np.float32)
+        self.lib.jo_world_collider_bounds(self.h, i, _ptr(out))
+        return tuple(float(x) for x in out)
+
+    def add_soft_body(self, settings: SettingsArrays, position, rotation, params):
+        sh = self.ref.settings_handle(settings)
+        self._settings.append(sh)
+        pos = np.array(position, np.float32)
+        rot = np.array(rotation, np.float32)
+        p = params.as_floats()
+        i = self.lib.jo_world_add_soft_body(self.h, sh, _ptr(pos), _ptr(rot), _ptr(p))
+        if i < 0:
+            raise RuntimeError("add_soft_body failed")
+        self.sizes.append(len(settings.vertices))
+        return i
+
+    def optimize_broadphase(self):
+        self.lib.jo_world_optimize_broadphase(self.h)
+
+    def update(self, dt, collision_steps=1, frames=1):
+        t = self.lib.jo_world_update(self.h, float(dt), int(collision_steps), int(frames))
+        if t < 0:
+            raise RuntimeError("PhysicsSystem::Update reported an error")
+        return t
+
+    def custom_update(self, body, dt):
+        if self.lib.jo_body_custom_update(self.h, body, float(dt)) != 0:
+            raise RuntimeError("CustomUpdate failed")
+
+    def read(self, body):
+        n = self.sizes[body]
+        out = dict(pos=np.zeros((n, 3), np.float32), prev=np.zeros((n, 3), np.float32),
+                   vel=np.zeros((n, 3), np.float32), plane=np.zeros((n, 4), np.float32),
+                   shape_idx=np.zeros(n, np.int32), penetration=np.zeros(n, np.float32),
+                   contact=np.zeros(n, np.int32), inv_mass=np.zeros(n, np.float32),
+                   body_pos=np.zeros(3, np.float32), body_rot=np.zeros(4, np.float32))
+        self.lib.jo_body_read(self.h, body, *(_ptr(out[k]) for k in (
+            "pos", "prev", "vel", "plane", "shape_idx", "penetration", "contact", "inv_mass",
+            "body_pos", "body_rot")))
+        return out
+
+    def write(self, body, pos=None, vel=None):
+        pos = None if pos is None else np.ascontiguousarray(pos, np.float32)
+        vel = None if vel is None else np.ascontiguousarray(vel, np.float32)
+        self.lib.jo_body_write(self.h, body, _ptr(pos), _ptr(vel))
+
+
+class Restatement:
+    """Own CPU restatement of the update, FP32 (precision=32) or FP64 (64)."""
+
+    _lib = None
+
+    def __init__(self, gravity, precision=32):
+        if Restatement._lib is None:
+            lib = C.CDLL(restatement_path())
+            lib.xr_world_create.restype = _vp
+            lib.xr_world_create.argtypes = [_i]
+            lib.xr_world_free.argtypes = [_vp]
+            lib.xr_world_set_gravity.argtypes = [_vp, _f, _f, _f]
+            lib.xr_world_add_collider.restype = _i
+            lib.xr_world_add_collider.argtypes = [_vp, _i, _vp, _vp, _vp, _f, _f, _vp]
+            lib.xr_world_add_soft_body.restype = _i
+            lib.xr_world_add_soft_body.argtypes = [_vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32,
+                                                   _vp, _u32, _vp, _u32, _vp, _vp, _vp]
+            lib.xr_world_update.restype = C.c_double
+            lib.xr_world_update.argtypes = [_vp, _f, _i, _i]
+            lib.xr_body_read.restype = _i
+            lib.xr_body_read.argtypes = [_vp, _i] + [_vp] * 8
+            lib.xr_body_write.restype = _i
+            lib.xr_body_write.argtypes = [_vp, _i, _vp, _vp]
+            lib.xr_acos_approximate.restype = _f
+            lib.xr_acos_approximate.argtypes = [_f]
+            Restatement._lib = lib
+        self.lib = Restatement._lib
+        self.h = self.lib.xr_world_create(precision)
+        self.lib.xr_world_set_gravity(self.h, *map(float, gravity))
+        self.sizes = []
+
+    def close(self):
+        if self.h:
+            self.lib.xr_world_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def add_collider(self, c, bounds=None):
+        p = c.param_array()
+        pos = np.array(c.pos, np.float32)
+        rot = np.array(c.rot, np.float32)
+        b = np.array(bounds if bounds is not None else c.bounds, np.float32)
+        return self.lib.xr_world_add_collider(self.h, int(c.type), _ptr(p), _ptr(pos), _ptr(rot),
+                                              float(c.friction), float(c.restitution), _ptr(b))
+
+    def add_soft_body(self, s: SettingsArrays, position, rotation, params):
+        f4 = _faces4(s.faces)
+        pos = np.array(position, np.float32)
+        rot = np.array(rotation, np.float32)
+        p = params.as_floats()
+        i = self.lib.xr_world_add_soft_body(self.h, _ptr(s.vertices), len(s.vertices), _ptr(f4), len(f4),
+                                            _ptr(s.edges), len(s.edges), _ptr(s.bends), len(s.bends),
+                                            _ptr(s.volumes), len(s.volumes), _ptr(s.lras), len(s.lras),
+                                            _ptr(s.groups), len(s.groups), _ptr(pos), _ptr(rot), _ptr(p))
+        if i < 0:
+            raise RuntimeError("add_soft_body failed")
+        self.sizes.append(len(s.vertices))
+        return i
+
+    def update(self, dt, collision_steps=1, frames=1):
+        return self.lib.xr_world_update(self.h, float(dt), int(collision_steps), int(frames))
+
+    def read(self, body):
+        n = self.sizes[body]
+        out = dict(pos=np.zeros((n, 3)), prev=np.zeros((n, 3)), vel=np.zeros((n, 3)),
+                   plane=np.zeros((n, 4)), shape_idx=np.zeros(n, np.int32), penetration=np.zeros(n),
+                   contact=np.zeros(n, np.int32), body_pos=np.zeros(3))
+        self.lib.xr_body_read(self.h, body, *(_ptr(out[k]) for k in (
+            "pos", "prev", "vel", "plane", "shape_idx", "penetration", "contact", "body_pos")))
+        return out
+
+    def write(self, body, pos=None, vel=None):
+        pos = None if pos is None else np.ascontiguousarray(pos, np.float64)
+        vel = None if vel is None else np.ascontiguousarray(vel, np.float64)
+        self.lib.xr_body_write(self.h, body, _ptr(pos), _ptr(vel))
+
+    def acos_approximate(self, x):
+        return self.lib.xr_acos_approximate(float(x))

@@ -1,0 +1,949 @@
+// CPU restatement of Jolt Physics' soft-body XPBD update (reference v5.5.1).
+//
+// TEST INFRASTRUCTURE ONLY -- this is the oracle.  It is loaded by tests/ and
+// by bench.py's cpu_baseline leg as the checker; the product never links,
+// loads or calls it.
+//
+// Templated on the arithmetic type F:
+//   F = float  : follows the reference's operation order so that, built with
+//                -ffp-contract=off, it is bit-exact with the reference compiled
+//                the same way (pinned against oracle/_ref by the tests);
+//   F = double : the same algorithm in FP64, the "FP64 CPU re-run" used to state
+//                the FP32 error of both the reference and the GPU path.
+//
+// Every function cites the reference lines it restates.  Paths are relative to
+// the reference's Jolt/ directory.  The SSE semantics that fix the float
+// results (Math/Vec3.inl, Math/Mat44.inl) are restated in the small vector
+// helpers below:
+//   Dot(a,b)   = (a.x*b.x + a.y*b.y) + a.z*b.z            Vec3.inl:901-934 (ReduceSum)
+//   Cross      = componentwise mul/sub, no FMA             Vec3.inl:855-863
+//   -v         = 0 - v                                     Vec3.inl:669-672
+//   sMin/sMax  = (a < b ? a : b) / (a > b ? a : b)         _mm_min_ps/_mm_max_ps
+//   Mat44*Vec3 = ((c0*x + c1*y) + c2*z) + c3               Mat44.inl:295-301
+//   Multiply3x3= (c0*x + c1*y) + c2*z                      Mat44.inl:384-389
+//   Mat44*Mat44= per column ((c0*m.x + c1*m.y) + c2*m.z) + c3*m.w   Mat44.inl:241-252
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+using u32 = uint32_t;
+
+// ------------------------------------------------------------------ vectors
+template <class F> struct V3 { F x, y, z; };
+
+template <class F> inline V3<F> operator+(V3<F> a, V3<F> b) { return { a.x + b.x, a.y + b.y, a.z + b.z }; }
+template <class F> inline V3<F> operator-(V3<F> a, V3<F> b) { return { a.x - b.x, a.y - b.y, a.z - b.z }; }
+template <class F> inline V3<F> operator*(V3<F> a, F s) { return { a.x * s, a.y * s, a.z * s }; }
+template <class F> inline V3<F> operator*(F s, V3<F> a) { return { s * a.x, s * a.y, s * a.z }; }
+template <class F> inline V3<F> operator/(V3<F> a, F s) { return { a.x / s, a.y / s, a.z / s }; }
+template <class F> inline V3<F> Neg(V3<F> a) { return { F(0) - a.x, F(0) - a.y, F(0) - a.z }; }
+template <class F> inline V3<F> Mul(V3<F> a, V3<F> b) { return { a.x * b.x, a.y * b.y, a.z * b.z }; }
+template <class F> inline F Dot(V3<F> a, V3<F> b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+template <class F> inline F LenSq(V3<F> a) { return Dot(a, a); }
+template <class F> inline F Len(V3<F> a) { return std::sqrt(LenSq(a)); }
+template <class F> inline V3<F> Cross(V3<F> a, V3<F> b)
+{
+	return { a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x };
+}
+template <class F> inline F SMin(F a, F b) { return a < b ? a : b; }
+template <class F> inline F SMax(F a, F b) { return a > b ? a : b; }
+template <class F> inline V3<F> VMin(V3<F> a, V3<F> b) { return { SMin(a.x, b.x), SMin(a.y, b.y), SMin(a.z, b.z) }; }
+template <class F> inline V3<F> VMax(V3<F> a, V3<F> b) { return { SMax(a.x, b.x), SMax(a.y, b.y), SMax(a.z, b.z) }; }
+template <class F> inline F Comp(V3<F> a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+// Abs: max(0 - v, v) (Vec3.inl:817-822)
+template <class F> inline V3<F> VAbs(V3<F> a) { return VMax(Neg(a), a); }
+// GetSign: sign bit of v or'ed onto 1.0 (Vec3.inl:1161-1166)
+template <class F> inline V3<F> VSign(V3<F> a)
+{
+	return { std::signbit(a.x) ? F(-1) : F(1), std::signbit(a.y) ? F(-1) : F(1), std::signbit(a.z) ? F(-1) : F(1) };
+}
+// GetLowestComponentIndex (Vec3.inl:809-812)
+template <class F> inline int LowestIndex(V3<F> a) { return a.x < a.y ? (a.z < a.x ? 2 : 0) : (a.z < a.y ? 2 : 1); }
+
+template <class F> struct Constants;
+template <> struct Constants<float> { static constexpr float kPi = 3.14159265358979323846f; };
+template <> struct Constants<double> { static constexpr double kPi = 3.14159265358979323846; };
+
+// Sign(0) = +1 (Math/Math.h:116-119)
+template <class F> inline F SignOf(F v) { return v < F(0) ? F(-1) : F(1); }
+
+// ACosApproximate (Math/Trigonometry.h:48-65)
+template <class F> inline F ACosApprox(F x)
+{
+	F abs_x = std::min(std::abs(x), F(1));
+	F val = std::sqrt(F(1) - abs_x) * (Constants<F>::kPi / F(2) - F(0.175394f) * abs_x);
+	return x < F(0) ? Constants<F>::kPi - val : val;
+}
+
+// ------------------------------------------------------------------ matrices
+// Column major: c[col] = {x, y, z, w}
+template <class F> struct M44 { F c[4][4]; };
+
+template <class F> inline V3<F> Col3(const M44<F> &m, int i) { return { m.c[i][0], m.c[i][1], m.c[i][2] }; }
+
+// Mat44::sRotation(Quat) (Mat44.inl:85-141, scalar branch documented equal to the SSE4.1 one)
+template <class F> M44<F> RotationTranslation(const F q[4], V3<F> t)
+{
+	F x = q[0], y = q[1], z = q[2], w = q[3];
+	F tx = x + x, ty = y + y, tz = z + z;
+	F xx = tx * x, yy = ty * y, zz = tz * z;
+	F xy = tx * y, xz = tx * z, xw = tx * w;
+	F yz = ty * z, yw = ty * w, zw = tz * w;
+	M44<F> m;
+	m.c[0][0] = (F(1) - yy) - zz; m.c[0][1] = xy + zw; m.c[0][2] = xz - yw; m.c[0][3] = 0;
+	m.c[1][0] = xy - zw; m.c[1][1] = (F(1) - zz) - xx; m.c[1][2] = yz + xw; m.c[1][3] = 0;
+	m.c[2][0] = xz + yw; m.c[2][1] = yz - xw; m.c[2][2] = (F(1) - xx) - yy; m.c[2][3] = 0;
+	m.c[3][0] = t.x; m.c[3][1] = t.y; m.c[3][2] = t.z; m.c[3][3] = 1;	// sRotationTranslation (Mat44.inl:149-154)
+	return m;
+}
+
+template <class F> inline V3<F> MulPoint(const M44<F> &m, V3<F> v)
+{
+	V3<F> r;
+	for (int k = 0; k < 3; ++k)
+		(&r.x)[k] = ((m.c[0][k] * v.x + m.c[1][k] * v.y) + m.c[2][k] * v.z) + m.c[3][k];
+	return r;
+}
+
+template <class F> inline V3<F> Mul3x3(const M44<F> &m, V3<F> v)
+{
+	V3<F> r;
+	for (int k = 0; k < 3; ++k)
+		(&r.x)[k] = (m.c[0][k] * v.x + m.c[1][k] * v.y) + m.c[2][k] * v.z;
+	return r;
+}
+
+template <class F> M44<F> Mul(const M44<F> &a, const M44<F> &b)
+{
+	M44<F> r;
+	for (int i = 0; i < 4; ++i)
+		for (int k = 0; k < 4; ++k)
+			r.c[i][k] = ((a.c[0][k] * b.c[i][0] + a.c[1][k] * b.c[i][1]) + a.c[2][k] * b.c[i][2]) + a.c[3][k] * b.c[i][3];
+	return r;
+}
+
+// Transposed3x3 (Mat44.inl:612-630): rotation transposed, translation column (0,0,0,1)
+template <class F> M44<F> Transposed3x3(const M44<F> &m)
+{
+	M44<F> r;
+	for (int i = 0; i < 3; ++i)
+	{
+		for (int k = 0; k < 3; ++k)
+			r.c[i][k] = m.c[k][i];
+		r.c[i][3] = 0;
+	}
+	r.c[3][0] = r.c[3][1] = r.c[3][2] = 0; r.c[3][3] = 1;
+	return r;
+}
+
+// InversedRotationTranslation (Mat44.inl:944-949)
+template <class F> M44<F> InvRT(const M44<F> &m)
+{
+	M44<F> r = Transposed3x3(m);
+	V3<F> t = Neg(Mul3x3(r, Col3(m, 3)));
+	r.c[3][0] = t.x; r.c[3][1] = t.y; r.c[3][2] = t.z; r.c[3][3] = 1;
+	return r;
+}
+
+// GetQuaternion (Mat44.inl:998-1050)
+template <class F> void GetQuaternion(const M44<F> &m, F q[4])
+{
+	auto e = [&](int col, int row) { return m.c[col][row]; };
+	F tr = e(0, 0) + e(1, 1) + e(2, 2);
+	if (tr >= F(0))
+	{
+		F s = std::sqrt(tr + F(1));
+		F is = F(0.5) / s;
+		q[0] = (e(1, 2) - e(2, 1)) * is; q[1] = (e(2, 0) - e(0, 2)) * is; q[2] = (e(0, 1) - e(1, 0)) * is; q[3] = F(0.5) * s;
+		return;
+	}
+	int i = 0;
+	if (e(1, 1) > e(0, 0)) i = 1;
+	if (e(2, 2) > e(i, i)) i = 2;
+	if (i == 0)
+	{
+		F s = std::sqrt(e(0, 0) - (e(1, 1) + e(2, 2)) + F(1));
+		F is = F(0.5) / s;
+		q[0] = F(0.5) * s; q[1] = (e(1, 0) + e(0, 1)) * is; q[2] = (e(0, 2) + e(2, 0)) * is; q[3] = (e(1, 2) - e(2, 1)) * is;
+	}
+	else if (i == 1)
+	{
+		F s = std::sqrt(e(1, 1) - (e(2, 2) + e(0, 0)) + F(1));
+		F is = F(0.5) / s;
+		q[0] = (e(1, 0) + e(0, 1)) * is; q[1] = F(0.5) * s; q[2] = (e(2, 1) + e(1, 2)) * is; q[3] = (e(2, 0) - e(0, 2)) * is;
+	}
+	else
+	{
+		F s = std::sqrt(e(2, 2) - (e(0, 0) + e(1, 1)) + F(1));
+		F is = F(0.5) / s;
+		q[0] = (e(0, 2) + e(2, 0)) * is; q[1] = (e(2, 1) + e(1, 2)) * is; q[2] = F(0.5) * s; q[3] = (e(0, 1) - e(1, 0)) * is;
+	}
+}
+
+// ------------------------------------------------------------------ planes / boxes
+template <class F> struct PlaneT { V3<F> n; F c; };
+
+// Plane::sFromPointAndNormal (Geometry/Plane.h:21)
+template <class F> inline PlaneT<F> PlaneFromPointNormal(V3<F> p, V3<F> n) { return { n, -Dot(n, p) }; }
+// Plane::GetTransformed (Geometry/Plane.h:42-46)
+template <class F> inline PlaneT<F> PlaneTransformed(const PlaneT<F> &p, const M44<F> &m)
+{
+	V3<F> tn = Mul3x3(m, p.n);
+	return { tn, p.c - Dot(Col3(m, 3), tn) };
+}
+// Plane::SignedDistance (Geometry/Plane.h:57)
+template <class F> inline F SignedDistance(const PlaneT<F> &p, V3<F> x) { return Dot(x, p.n) + p.c; }
+
+template <class F> struct Box3 { V3<F> mn, mx; };
+template <class F> inline Box3<F> EmptyBox() { F m = F(FLT_MAX); return { { m, m, m }, { -m, -m, -m } }; }
+template <class F> inline void Encapsulate(Box3<F> &b, V3<F> p) { b.mn = VMin(b.mn, p); b.mx = VMax(b.mx, p); }
+// AABox::Transformed (Geometry/AABox.h:193-213)
+template <class F> Box3<F> Transformed(const Box3<F> &b, const M44<F> &m)
+{
+	V3<F> nmin = Col3(m, 3), nmax = Col3(m, 3);
+	for (int c = 0; c < 3; ++c)
+	{
+		V3<F> col = Col3(m, c);
+		V3<F> a = col * Comp(b.mn, c);
+		V3<F> bb = col * Comp(b.mx, c);
+		nmin = nmin + VMin(a, bb);
+		nmax = nmax + VMax(a, bb);
+	}
+	return { nmin, nmax };
+}
+// AABox::Overlaps (Geometry/AABox.h:164-167)
+template <class F> inline bool Overlaps(const Box3<F> &a, const Box3<F> &b)
+{
+	return !(a.mn.x > b.mx.x || a.mn.y > b.mx.y || a.mn.z > b.mx.z || a.mx.x < b.mn.x || a.mx.y < b.mn.y || a.mx.z < b.mn.z);
+}
+
+// ------------------------------------------------------------------ records
+struct VertexRec { float pos[3], vel[3], inv_mass; };
+struct FaceRec { u32 v[3], material; };
+struct EdgeRec { u32 v[2]; float rest, compliance; };
+struct BendRec { u32 v[4]; float compliance, initial_angle; };
+struct VolumeRec { u32 v[4]; float six_rest_volume, compliance; };
+struct LRARec { u32 v[2]; float max_distance; };
+struct GroupRec { u32 edge_end, lra_end, rod_ss_end, rod_bt_end, bend_end, volume_end, skinned_end; };
+
+// ------------------------------------------------------------------ world
+struct Collider
+{
+	int type;				// 0 plane, 1 sphere, 2 box, 3 capsule
+	float params[5];
+	float pos[3], rot[4];
+	float friction, restitution;
+	float bounds[6];		// world space bounds of the static body (min xyz, max xyz)
+};
+
+template <class F> struct Leaf
+{
+	M44<F> transform;		// leaf shape transform in soft-body local space
+	const Collider *col;
+	F friction, restitution;
+};
+
+template <class F> struct Body
+{
+	// shared settings (after Optimize)
+	std::vector<FaceRec> faces;
+	std::vector<EdgeRec> edges;
+	std::vector<BendRec> bends;
+	std::vector<VolumeRec> volumes;
+	std::vector<LRARec> lras;
+	std::vector<GroupRec> groups;
+
+	// vertex state (SoftBodyVertex.h:34-41)
+	std::vector<V3<F>> x, prev, v;
+	std::vector<F> w;
+	std::vector<PlaneT<F>> plane;
+	std::vector<int> shape_idx;
+	std::vector<F> largest;
+	std::vector<char> contact;
+
+	// creation settings (SoftBodyCreationSettings.h:61-72)
+	u32 iterations;
+	float linear_damping, max_linear_velocity, restitution, friction, pressure, gravity_factor, vertex_radius;
+	bool update_position, allow_sleeping;
+
+	// body
+	V3<F> position;
+	F rotation[4];
+	Box3<F> local_bounds, local_predicted_bounds;
+	F sleep_timer = 0;
+	bool active = true;
+	V3<F> linear_velocity{ 0, 0, 0 }, angular_velocity{ 0, 0, 0 };
+
+	// per step context (SoftBodyUpdateContext.h:56-60)
+	V3<F> gravity;
+	F dt_sub;
+	V3<F> disp_gravity;
+	std::vector<Leaf<F>> leaves;
+};
+
+template <class F> struct World
+{
+	V3<F> gravity{ 0, F(-9.81f), 0 };
+	std::vector<Collider> colliders;
+	std::vector<Body<F>> bodies;
+};
+
+// ------------------------------------------------------------------ shape collision
+// Restates Shape::CollideSoftBodyVertices for the four primitive shapes, with
+// UpdatePenetration / SetCollision of CollideSoftBodyVertexIterator.h:83-97.
+template <class F> void CollideVertices(Body<F> &b, const Leaf<F> &leaf, int shape_index)
+{
+	const Collider &c = *leaf.col;
+	const M44<F> &T = leaf.transform;
+	const size_t nv = b.x.size();
+	auto update = [&](size_t i, F pen) {
+		if (b.largest[i] >= pen)
+			return false;
+		b.largest[i] = pen;
+		return true;
+	};
+	switch (c.type)
+	{
+	case 0: // PlaneShape.cpp:243-256 with Plane::Scaled (scale 1, Plane.h:49-54) and GetTransformed
+		{
+			PlaneT<F> p{ { F(c.params[0]), F(c.params[1]), F(c.params[2]) }, F(c.params[3]) };
+			V3<F> one{ 1, 1, 1 };
+			V3<F> sn{ p.n.x / one.x, p.n.y / one.y, p.n.z / one.z };
+			F snl = Len(sn);
+			PlaneT<F> scaled{ sn / snl, p.c / snl };
+			PlaneT<F> plane = PlaneTransformed(scaled, T);
+			for (size_t i = 0; i < nv; ++i)
+				if (b.w[i] > F(0))
+				{
+					F pen = -SignedDistance(plane, b.x[i]);
+					if (update(i, pen)) { b.plane[i] = plane; b.shape_idx[i] = shape_index; }
+				}
+		}
+		break;
+	case 1: // SphereShape.cpp:279-301
+		{
+			V3<F> center = Col3(T, 3);
+			F radius = F(c.params[0]);
+			for (size_t i = 0; i < nv; ++i)
+				if (b.w[i] > F(0))
+				{
+					V3<F> delta = b.x[i] - center;
+					F distance = Len(delta);
+					F pen = radius - distance;
+					if (update(i, pen))
+					{
+						V3<F> normal = distance > F(0) ? delta / distance : V3<F>{ 0, 1, 0 };
+						V3<F> point = center + radius * normal;
+						b.plane[i] = PlaneFromPointNormal(point, normal);
+						b.shape_idx[i] = shape_index;
+					}
+				}
+		}
+		break;
+	case 2: // BoxShape.cpp:234-283
+		{
+			M44<F> inv = InvRT(T);
+			V3<F> he{ F(c.params[0]), F(c.params[1]), F(c.params[2]) };
+			he = Mul(VAbs(V3<F>{ 1, 1, 1 }), he);
+			for (size_t i = 0; i < nv; ++i)
+				if (b.w[i] > F(0))
+				{
+					V3<F> local = MulPoint(inv, b.x[i]);
+					V3<F> clamped = VMax(VMin(local, he), Neg(he));
+					if (clamped.x == local.x && clamped.y == local.y && clamped.z == local.z)
+					{
+						V3<F> delta = he - VAbs(local);
+						int index = LowestIndex(delta);
+						F pen = Comp(delta, index);
+						if (update(i, pen))
+						{
+							V3<F> axis{ F(index == 0), F(index == 1), F(index == 2) };
+							V3<F> normal = Mul(VSign(local), axis);
+							V3<F> point = Mul(normal, he);
+							b.plane[i] = PlaneTransformed(PlaneFromPointNormal(point, normal), T);
+							b.shape_idx[i] = shape_index;
+						}
+					}
+					else
+					{
+						V3<F> normal = local - clamped;
+						F nl = Len(normal);
+						F pen = -nl;
+						if (update(i, pen))
+						{
+							normal = normal / nl;
+							b.plane[i] = PlaneTransformed(PlaneFromPointNormal(clamped, normal), T);
+							b.shape_idx[i] = shape_index;
+						}
+					}
+				}
+		}
+		break;
+	case 3: // CapsuleShape.cpp:324-375 (scale 1)
+		{
+			M44<F> inv = InvRT(T);
+			F scale = std::abs(F(1));
+			F hh = scale * F(c.params[0]);
+			F radius = scale * F(c.params[1]);
+			for (size_t i = 0; i < nv; ++i)
+				if (b.w[i] > F(0))
+				{
+					V3<F> local = MulPoint(inv, b.x[i]);
+					if (std::abs(local.y) <= hh)
+					{
+						V3<F> normal{ local.x, 0, local.z };
+						F nl = Len(normal);
+						F pen = radius - nl;
+						if (update(i, pen))
+						{
+							normal = nl > F(0) ? normal / nl : V3<F>{ 1, 0, 0 };
+							V3<F> point = radius * normal;
+							b.plane[i] = PlaneTransformed(PlaneFromPointNormal(point, normal), T);
+							b.shape_idx[i] = shape_index;
+						}
+					}
+					else
+					{
+						V3<F> center{ 0, SignOf(local.y) * hh, 0 };
+						V3<F> delta = local - center;
+						F distance = Len(delta);
+						F pen = radius - distance;
+						if (update(i, pen))
+						{
+							V3<F> normal = delta / distance;
+							V3<F> point = center + radius * normal;
+							b.plane[i] = PlaneTransformed(PlaneFromPointNormal(point, normal), T);
+							b.shape_idx[i] = shape_index;
+						}
+					}
+				}
+		}
+		break;
+	}
+}
+
+// ------------------------------------------------------------------ solver pieces
+// GetVolumeTimesSix (SoftBodyMotionProperties.cpp:107-118), sequential sum
+template <class F> F VolumeTimesSix(const Body<F> &b)
+{
+	F six = 0;
+	for (const FaceRec &f : b.faces)
+		six += Dot(Cross(b.x[f.v[0]], b.x[f.v[1]]), b.x[f.v[2]]);
+	return six;
+}
+
+// ApplyPressure (SoftBodyMotionProperties.cpp:291-323)
+template <class F> void ApplyPressure(Body<F> &b)
+{
+	F dt = b.dt_sub;
+	F coef_p = F(b.pressure);
+	if (coef_p > F(0))
+	{
+		F six = VolumeTimesSix(b);
+		if (six > F(0))
+		{
+			F coefficient = coef_p * dt / six;
+			for (const FaceRec &f : b.faces)
+			{
+				V3<F> x1 = b.x[f.v[0]], x2 = b.x[f.v[1]], x3 = b.x[f.v[2]];
+				V3<F> impulse = coefficient * Cross(x2 - x1, x3 - x1);
+				for (u32 i : f.v)
+					b.v[i] = b.v[i] + b.w[i] * impulse;
+			}
+		}
+	}
+}
+
+// IntegratePositions (SoftBodyMotionProperties.cpp:325-365); accumulated force is zero
+template <class F> void Integrate(Body<F> &b)
+{
+	F dt = b.dt_sub;
+	F damping = std::max(F(0), F(1) - F(b.linear_damping) * dt);
+	V3<F> gdt = b.gravity * dt;
+	V3<F> imp = V3<F>{ 0, 0, 0 } * dt / std::max(F(b.x.size()), F(1));
+	for (size_t i = 0; i < b.x.size(); ++i)
+	{
+		if (b.w[i] > F(0))
+		{
+			b.v[i] = b.v[i] + (gdt + imp * b.w[i]);
+			b.v[i] = b.v[i] * damping;
+		}
+		V3<F> p = b.x[i];
+		b.prev[i] = p;
+		b.x[i] = p + b.v[i] * dt;
+	}
+}
+
+// ApplyDihedralBendConstraints (SoftBodyMotionProperties.cpp:367-456)
+template <class F> void ApplyBends(Body<F> &b, u32 begin, u32 end)
+{
+	F inv_dt_sq = F(1) / (b.dt_sub * b.dt_sub);
+	const F pi = Constants<F>::kPi;
+	for (u32 k = begin; k < end; ++k)
+	{
+		const BendRec &r = b.bends[k];
+		V3<F> x0 = b.x[r.v[0]], x1 = b.x[r.v[1]], x2 = b.x[r.v[2]], x3 = b.x[r.v[3]];
+		V3<F> e = x1 - x0;
+		F e_len = Len(e);
+		if (e_len < F(1.0e-6f))
+			continue;
+		V3<F> x1x2 = x2 - x1, x1x3 = x3 - x1;
+		V3<F> n1 = Cross(x2 - x0, x1x2);
+		V3<F> n2 = Cross(x1x3, x3 - x0);
+		F n1sq = LenSq(n1), n2sq = LenSq(n2);
+		F n1n2 = n1sq * n2sq;
+		if (n1n2 < F(1.0e-24f))
+			continue;
+		F sign = SignOf(Dot(Cross(n2, n1), e));
+		F d = Dot(n1, n2) / std::sqrt(n1n2);
+		F c = sign * ACosApprox(d) - F(r.initial_angle);
+		if (c > pi)
+			c -= F(2) * pi;
+		else if (c < -pi)
+			c += F(2) * pi;
+		n1 = n1 / n1sq;
+		n2 = n2 / n2sq;
+		V3<F> d0c = (Dot(x1x2, e) * n1 + Dot(x1x3, e) * n2) / e_len;
+		V3<F> d2c = e_len * n1;
+		V3<F> d3c = e_len * n2;
+		V3<F> d1c = (Neg(d0c) - d2c) - d3c;
+		F w0 = b.w[r.v[0]], w1 = b.w[r.v[1]], w2 = b.w[r.v[2]], w3 = b.w[r.v[3]];
+		F denom = (((w0 * LenSq(d0c) + w1 * LenSq(d1c)) + w2 * LenSq(d2c)) + w3 * LenSq(d3c)) + F(r.compliance) * inv_dt_sq;
+		if (denom < F(1.0e-12f))
+			continue;
+		F ml = c / denom;
+		b.x[r.v[0]] = x0 - (ml * w0) * d0c;
+		b.x[r.v[1]] = x1 - (ml * w1) * d1c;
+		b.x[r.v[2]] = x2 - (ml * w2) * d2c;
+		b.x[r.v[3]] = x3 - (ml * w3) * d3c;
+	}
+}
+
+// ApplyVolumeConstraints (SoftBodyMotionProperties.cpp:458-507)
+template <class F> void ApplyVolumes(Body<F> &b, u32 begin, u32 end)
+{
+	F inv_dt_sq = F(1) / (b.dt_sub * b.dt_sub);
+	for (u32 k = begin; k < end; ++k)
+	{
+		const VolumeRec &r = b.volumes[k];
+		V3<F> x1 = b.x[r.v[0]], x2 = b.x[r.v[1]], x3 = b.x[r.v[2]], x4 = b.x[r.v[3]];
+		V3<F> x1x2 = x2 - x1, x1x3 = x3 - x1, x1x4 = x4 - x1;
+		F c = std::abs(Dot(Cross(x1x2, x1x3), x1x4)) - F(r.six_rest_volume);
+		V3<F> d1c = Cross(x4 - x2, x3 - x2);
+		V3<F> d2c = Cross(x1x3, x1x4);
+		V3<F> d3c = Cross(x1x4, x1x2);
+		V3<F> d4c = Cross(x1x2, x1x3);
+		F w1 = b.w[r.v[0]], w2 = b.w[r.v[1]], w3 = b.w[r.v[2]], w4 = b.w[r.v[3]];
+		F denom = (((w1 * LenSq(d1c) + w2 * LenSq(d2c)) + w3 * LenSq(d3c)) + w4 * LenSq(d4c)) + F(r.compliance) * inv_dt_sq;
+		if (denom < F(1.0e-12f))
+			continue;
+		F ml = c / denom;
+		b.x[r.v[0]] = x1 - (ml * w1) * d1c;
+		b.x[r.v[1]] = x2 - (ml * w2) * d2c;
+		b.x[r.v[2]] = x3 - (ml * w3) * d3c;
+		b.x[r.v[3]] = x4 - (ml * w4) * d4c;
+	}
+}
+
+// ApplyEdgeConstraints (SoftBodyMotionProperties.cpp:573-601)
+template <class F> void ApplyEdges(Body<F> &b, u32 begin, u32 end)
+{
+	F inv_dt_sq = F(1) / (b.dt_sub * b.dt_sub);
+	for (u32 k = begin; k < end; ++k)
+	{
+		const EdgeRec &r = b.edges[k];
+		V3<F> x0 = b.x[r.v[0]], x1 = b.x[r.v[1]];
+		V3<F> delta = x1 - x0;
+		F length = Len(delta);
+		F w0 = b.w[r.v[0]], w1 = b.w[r.v[1]];
+		F denom = length * ((w0 + w1) + F(r.compliance) * inv_dt_sq);
+		if (denom < F(1.0e-12f))
+			continue;
+		V3<F> corr = (delta * (length - F(r.rest))) / denom;
+		b.x[r.v[0]] = x0 + w0 * corr;
+		b.x[r.v[1]] = x1 - w1 * corr;
+	}
+}
+
+// ApplyLRAConstraints (SoftBodyMotionProperties.cpp:679-698)
+template <class F> void ApplyLRAs(Body<F> &b, u32 begin, u32 end)
+{
+	for (u32 k = begin; k < end; ++k)
+	{
+		const LRARec &r = b.lras[k];
+		V3<F> x0 = b.x[r.v[0]];
+		V3<F> delta = b.x[r.v[1]] - x0;
+		F dsq = LenSq(delta);
+		F md = F(r.max_distance);
+		if (dsq > md * md)
+			b.x[r.v[1]] = x0 + (delta * md) / std::sqrt(dsq);
+	}
+}
+
+// ProcessGroup (SoftBodyMotionProperties.cpp:1030-1055): volume -> bend -> skin -> edge -> rods -> LRA
+template <class F> void ProcessGroup(Body<F> &b, size_t g)
+{
+	GroupRec start{ 0, 0, 0, 0, 0, 0, 0 };
+	const GroupRec &p = g > 0 ? b.groups[g - 1] : start;
+	const GroupRec &c = b.groups[g];
+	ApplyVolumes(b, p.volume_end, c.volume_end);
+	ApplyBends(b, p.bend_end, c.bend_end);
+	ApplyEdges(b, p.edge_end, c.edge_end);
+	ApplyLRAs(b, p.lra_end, c.lra_end);
+}
+
+// ApplyCollisionConstraintsAndUpdateVelocities, static / kinematic branch
+// (SoftBodyMotionProperties.cpp:700-825).
+template <class F> void CollisionAndVelocity(Body<F> &b)
+{
+	F dt = b.dt_sub;
+	F restitution_threshold = F(-2) * Len(b.gravity) * dt;
+	F r_vtx = F(b.vertex_radius);
+	for (size_t i = 0; i < b.x.size(); ++i)
+		if (b.w[i] > F(0))
+		{
+			V3<F> prev_v = b.v[i];
+			b.v[i] = (b.x[i] - b.prev[i]) / dt;
+			if (b.shape_idx[i] >= 0)
+			{
+				F pd = -SignedDistance(b.plane[i], b.x[i]) + r_vtx;
+				if (pd > F(0))
+				{
+					b.contact[i] = 1;
+					const Leaf<F> &cs = b.leaves[b.shape_idx[i]];
+					V3<F> n = b.plane[i].n;
+					b.x[i] = b.x[i] + n * pd;
+					// static body; mSoftBodyInvMassScale = 1 > 0
+					V3<F> vn = n * Dot(n, b.v[i]);
+					V3<F> vt = b.v[i] - vn;
+					F vtl = Len(vt);
+					if (vtl > F(0))
+						b.v[i] = b.v[i] - vt * std::min(cs.friction * pd / (vtl * dt), F(1));
+					b.v[i] = b.v[i] - vn;
+					F pvn = Dot(prev_v, n);
+					if (pvn < restitution_threshold)
+						b.v[i] = b.v[i] - (cs.restitution * pvn) * n;
+				}
+			}
+		}
+}
+
+// UpdateSoftBodyState (SoftBodyMotionProperties.cpp:827-921); step_dt = context mDeltaTime
+template <class F> V3<F> UpdateState(Body<F> &b, F step_dt, const M44<F> &com)
+{
+	F max_lin_sq = F(b.max_linear_velocity) * F(b.max_linear_velocity);
+	F max_v_sq = 0;
+	V3<F> lin{ 0, 0, 0 }, ang{ 0, 0, 0 };
+	b.local_bounds = EmptyBox<F>();
+	b.local_predicted_bounds = EmptyBox<F>();
+	for (size_t i = 0; i < b.x.size(); ++i)
+	{
+		F v_sq = LenSq(b.v[i]);
+		max_v_sq = std::max(max_v_sq, v_sq);
+		if (v_sq > max_lin_sq)
+			b.v[i] = b.v[i] * std::sqrt(max_lin_sq / v_sq);
+		lin = lin + b.v[i];
+		ang = ang + Cross(b.x[i], b.v[i]);
+		Encapsulate(b.local_bounds, b.x[i]);
+		Encapsulate(b.local_predicted_bounds, (b.x[i] + b.v[i] * step_dt) + b.disp_gravity);
+		b.largest[i] = F(-FLT_MAX);
+		b.shape_idx[i] = -1;
+		b.contact[i] = 0;
+	}
+	F nvd = F(std::max(int(b.x.size()), 1));
+	b.linear_velocity = Mul3x3(com, lin / nvd);
+	b.angular_velocity = Mul3x3(com, ang / nvd);
+	V3<F> delta_world{ 0, 0, 0 };
+	if (b.update_position)
+	{
+		V3<F> delta = F(0.5) * (b.local_bounds.mn + b.local_bounds.mx);
+		delta_world = Mul3x3(com, delta);
+		for (size_t i = 0; i < b.x.size(); ++i)
+			b.x[i] = b.x[i] - delta;
+		b.local_bounds.mn = b.local_bounds.mn + Neg(delta);
+		b.local_bounds.mx = b.local_bounds.mx + Neg(delta);
+		b.local_predicted_bounds.mn = b.local_predicted_bounds.mn + Neg(delta);
+		b.local_predicted_bounds.mx = b.local_predicted_bounds.mx + Neg(delta);
+	}
+	if (b.allow_sleeping)
+	{
+		if (max_v_sq > F(0.03f))
+			b.sleep_timer = 0;
+		else
+		{
+			b.sleep_timer += step_dt;
+			if (b.sleep_timer >= F(0.5f))
+				b.active = false;	// ECanSleep::CanSleep -> deactivated (PhysicsSystem.cpp:2866-2876)
+		}
+	}
+	return delta_world;
+}
+
+// One collision step of one body: InitializeUpdateContext + DetermineCollidingShapes
+// + ParallelUpdate state machine + JobSoftBodyFinalize position update
+// (SoftBodyMotionProperties.cpp:937-960, 120-255, 969-1115; PhysicsSystem.cpp:2842-2864).
+template <class F> void StepBody(World<F> &w, Body<F> &b, F step_dt)
+{
+	M44<F> com = RotationTranslation(b.rotation, b.position);
+	// InitializeUpdateContext: gravity to local space, sub step, gravity displacement
+	V3<F> g = F(b.gravity_factor) * w.gravity;
+	b.gravity = Mul3x3(Transposed3x3(com), g);
+	b.dt_sub = step_dt / F(b.iterations);
+	F ni = F(b.iterations);
+	b.disp_gravity = (((F(0.5f) * ni) * F(b.iterations + 1)) * (b.dt_sub * b.dt_sub)) * b.gravity;
+
+	// DetermineCollidingShapes (static bodies only)
+	Box3<F> lb = b.local_bounds;
+	lb.mn = VMin(lb.mn, b.local_predicted_bounds.mn);
+	lb.mx = VMax(lb.mx, b.local_predicted_bounds.mx);
+	V3<F> rv{ F(b.vertex_radius), F(b.vertex_radius), F(b.vertex_radius) };
+	lb.mn = lb.mn - rv;
+	lb.mx = lb.mx + rv;
+	Box3<F> wb = Transformed(lb, com);
+	M44<F> inv = InvRT(com);
+	b.leaves.clear();
+	for (const Collider &c : w.colliders)
+	{
+		Box3<F> cb{ { F(c.bounds[0]), F(c.bounds[1]), F(c.bounds[2]) }, { F(c.bounds[3]), F(c.bounds[4]), F(c.bounds[5]) } };
+		if (!Overlaps(wb, cb))
+			continue;
+		F q[4] = { F(c.rot[0]), F(c.rot[1]), F(c.rot[2]), F(c.rot[3]) };
+		M44<F> ccom = RotationTranslation(q, V3<F>{ F(c.pos[0]), F(c.pos[1]), F(c.pos[2]) });
+		M44<F> rel = Mul(inv, ccom);
+		F lq[4];
+		GetQuaternion(rel, lq);
+		Leaf<F> leaf;
+		leaf.transform = RotationTranslation(lq, Col3(rel, 3));
+		leaf.col = &c;
+		leaf.friction = std::sqrt(F(b.friction) * F(c.friction));		// ContactConstraintManager.h:554
+		leaf.restitution = std::max(F(b.restitution), F(c.restitution)); // ContactConstraintManager.h:555
+		b.leaves.push_back(leaf);
+	}
+
+	// DetermineCollisionPlanes (SoftBodyMotionProperties.cpp:257-265)
+	for (size_t s = 0; s < b.leaves.size(); ++s)
+		CollideVertices(b, b.leaves[s], int(s));
+
+	// Iterations: StartFirstIteration, then ParallelApplyConstraints loop
+	ApplyPressure(b);
+	Integrate(b);
+	for (u32 it = 0; it < b.iterations; ++it)
+	{
+		for (size_t g = 0; g + 1 < b.groups.size(); ++g)
+			ProcessGroup(b, g);
+		if (!b.groups.empty())
+			ProcessGroup(b, b.groups.size() - 1);
+		CollisionAndVelocity(b);
+		if (it + 1 < b.iterations)
+		{
+			ApplyPressure(b);
+			Integrate(b);
+		}
+	}
+	V3<F> delta_world = UpdateState(b, step_dt, com);
+	// JobSoftBodyFinalize: SetPositionAndRotationInternal(GetPosition() + delta)
+	b.position = b.position + delta_world;
+}
+
+template <class F> struct Api
+{
+	static void *Create() { return new World<F>; }
+};
+
+} // namespace
+
+// ====================================================================== C API
+// precision: 32 -> float restatement, 64 -> double restatement.
+struct XrWorld
+{
+	int precision;
+	World<float> f;
+	World<double> d;
+};
+
+template <class F> static World<F> &Get(XrWorld *w);
+template <> World<float> &Get<float>(XrWorld *w) { return w->f; }
+template <> World<double> &Get<double>(XrWorld *w) { return w->d; }
+
+template <class F> static int AddBody(XrWorld *xw, const VertexRec *verts, u32 nv, const FaceRec *faces, u32 nf, const EdgeRec *edges, u32 ne,
+									  const BendRec *bends, u32 nb, const VolumeRec *vols, u32 nt, const LRARec *lras, u32 nl,
+									  const GroupRec *groups, u32 ng, const float *pos, const float *rot, const float *p)
+{
+	World<F> &w = Get<F>(xw);
+	Body<F> b;
+	b.faces.assign(faces, faces + nf);
+	b.edges.assign(edges, edges + ne);
+	b.bends.assign(bends, bends + nb);
+	b.volumes.assign(vols, vols + nt);
+	b.lras.assign(lras, lras + nl);
+	b.groups.assign(groups, groups + ng);
+	b.iterations = u32(p[0]);
+	b.linear_damping = p[1]; b.max_linear_velocity = p[2]; b.restitution = p[3]; b.friction = p[4];
+	b.pressure = p[5]; b.gravity_factor = p[6]; b.vertex_radius = p[7];
+	b.update_position = p[8] != 0.0f;
+	bool make_identity = p[9] != 0.0f;
+	b.allow_sleeping = p[10] != 0.0f;
+	b.position = { F(pos[0]), F(pos[1]), F(pos[2]) };
+	F q[4] = { F(rot[0]), F(rot[1]), F(rot[2]), F(rot[3]) };
+	// SoftBodyMotionProperties::Initialize (SoftBodyMotionProperties.cpp:59-105)
+	M44<F> R;
+	F ident[4] = { 0, 0, 0, 1 };
+	R = RotationTranslation(make_identity ? q : ident, V3<F>{ 0, 0, 0 });
+	for (int k = 0; k < 4; ++k)
+		b.rotation[k] = make_identity ? ident[k] : q[k];
+	b.x.resize(nv); b.prev.resize(nv); b.v.resize(nv); b.w.resize(nv);
+	b.plane.assign(nv, PlaneT<F>{ { 0, 0, 0 }, 0 });
+	b.shape_idx.assign(nv, -1); b.largest.assign(nv, F(-FLT_MAX)); b.contact.assign(nv, 0);
+	b.local_bounds = EmptyBox<F>();
+	for (u32 i = 0; i < nv; ++i)
+	{
+		V3<F> xp{ F(verts[i].pos[0]), F(verts[i].pos[1]), F(verts[i].pos[2]) };
+		V3<F> vv{ F(verts[i].vel[0]), F(verts[i].vel[1]), F(verts[i].vel[2]) };
+		b.x[i] = b.prev[i] = MulPoint(R, xp);
+		b.v[i] = Mul3x3(R, vv);
+		b.w[i] = F(verts[i].inv_mass);
+		Encapsulate(b.local_bounds, b.x[i]);
+	}
+	b.local_predicted_bounds = b.local_bounds;
+	w.bodies.push_back(std::move(b));
+	return int(w.bodies.size() - 1);
+}
+
+template <class F> static void UpdateWorld(XrWorld *xw, float dt, int steps)
+{
+	World<F> &w = Get<F>(xw);
+	// PhysicsSystem::Update: step delta time = dt / collision steps (PhysicsSystem.cpp:215)
+	F step_dt = F(dt) / F(steps);
+	for (int s = 0; s < steps; ++s)
+		for (Body<F> &b : w.bodies)
+			if (b.active)
+				StepBody(w, b, step_dt);
+}
+
+template <class F> static int ReadBody(XrWorld *xw, int bi, double *pos, double *prev, double *vel, double *plane, int *idx, double *pen, int *contact, double *bpos)
+{
+	World<F> &w = Get<F>(xw);
+	if (bi < 0 || bi >= int(w.bodies.size()))
+		return -1;
+	const Body<F> &b = w.bodies[bi];
+	for (size_t i = 0; i < b.x.size(); ++i)
+	{
+		if (pos) { pos[3 * i] = b.x[i].x; pos[3 * i + 1] = b.x[i].y; pos[3 * i + 2] = b.x[i].z; }
+		if (prev) { prev[3 * i] = b.prev[i].x; prev[3 * i + 1] = b.prev[i].y; prev[3 * i + 2] = b.prev[i].z; }
+		if (vel) { vel[3 * i] = b.v[i].x; vel[3 * i + 1] = b.v[i].y; vel[3 * i + 2] = b.v[i].z; }
+		if (plane) { plane[4 * i] = b.plane[i].n.x; plane[4 * i + 1] = b.plane[i].n.y; plane[4 * i + 2] = b.plane[i].n.z; plane[4 * i + 3] = b.plane[i].c; }
+		if (idx) idx[i] = b.shape_idx[i];
+		if (pen) pen[i] = b.largest[i];
+		if (contact) contact[i] = b.contact[i];
+	}
+	if (bpos) { bpos[0] = b.position.x; bpos[1] = b.position.y; bpos[2] = b.position.z; }
+	return int(b.x.size());
+}
+
+template <class F> static int WriteBody(XrWorld *xw, int bi, const double *pos, const double *vel)
+{
+	World<F> &w = Get<F>(xw);
+	if (bi < 0 || bi >= int(w.bodies.size()))
+		return -1;
+	Body<F> &b = w.bodies[bi];
+	for (size_t i = 0; i < b.x.size(); ++i)
+	{
+		if (pos) b.x[i] = { F(pos[3 * i]), F(pos[3 * i + 1]), F(pos[3 * i + 2]) };
+		if (vel) b.v[i] = { F(vel[3 * i]), F(vel[3 * i + 1]), F(vel[3 * i + 2]) };
+	}
+	return 0;
+}
+
+extern "C" {
+
+void *xr_world_create(int precision)
+{
+	if (precision != 32 && precision != 64)
+		return nullptr;
+	XrWorld *w = new XrWorld;
+	w->precision = precision;
+	return w;
+}
+
+void xr_world_free(void *w) { delete (XrWorld *)w; }
+
+void xr_world_set_gravity(void *wp, float gx, float gy, float gz)
+{
+	XrWorld *w = (XrWorld *)wp;
+	w->f.gravity = { gx, gy, gz };
+	w->d.gravity = { gx, gy, gz };
+}
+
+// type 0 plane {nx,ny,nz,c,halfExtent}, 1 sphere {r}, 2 box {hx,hy,hz}, 3 capsule {halfHeight, r};
+// bounds: the static body's world-space bounds (min xyz, max xyz).
+int xr_world_add_collider(void *wp, int type, const float *params, const float *pos, const float *rot, float friction, float restitution, const float *bounds)
+{
+	XrWorld *w = (XrWorld *)wp;
+	if (type < 0 || type > 3)
+		return -1;
+	Collider c{};
+	c.type = type;
+	int np = type == 0 ? 5 : (type == 1 ? 1 : (type == 2 ? 3 : 2));
+	for (int i = 0; i < np; ++i) c.params[i] = params[i];
+	for (int i = 0; i < 3; ++i) c.pos[i] = pos[i];
+	for (int i = 0; i < 4; ++i) c.rot[i] = rot[i];
+	for (int i = 0; i < 6; ++i) c.bounds[i] = bounds[i];
+	c.friction = friction;
+	c.restitution = restitution;
+	w->f.colliders.push_back(c);
+	w->d.colliders.push_back(c);
+	return int(w->f.colliders.size() - 1);
+}
+
+// p: {numIterations, linearDamping, maxLinearVelocity, restitution, friction,
+// pressure, gravityFactor, vertexRadius, updatePosition, makeRotationIdentity, allowSleeping}
+int xr_world_add_soft_body(void *wp, const void *verts, u32 nv, const void *faces4, u32 nf, const void *edges, u32 ne,
+						   const void *bends, u32 nb, const void *vols, u32 nt, const void *lras, u32 nl,
+						   const void *groups7, u32 ng, const float *pos, const float *rot, const float *p)
+{
+	XrWorld *w = (XrWorld *)wp;
+	int a = AddBody<float>(w, (const VertexRec *)verts, nv, (const FaceRec *)faces4, nf, (const EdgeRec *)edges, ne, (const BendRec *)bends, nb,
+						   (const VolumeRec *)vols, nt, (const LRARec *)lras, nl, (const GroupRec *)groups7, ng, pos, rot, p);
+	int b = AddBody<double>(w, (const VertexRec *)verts, nv, (const FaceRec *)faces4, nf, (const EdgeRec *)edges, ne, (const BendRec *)bends, nb,
+							(const VolumeRec *)vols, nt, (const LRARec *)lras, nl, (const GroupRec *)groups7, ng, pos, rot, p);
+	return a == b ? a : -1;
+}
+
+double xr_world_update(void *wp, float dt, int collision_steps, int frames)
+{
+	XrWorld *w = (XrWorld *)wp;
+	auto t0 = std::chrono::high_resolution_clock::now();
+	for (int i = 0; i < frames; ++i)
+	{
+		if (w->precision == 32)
+			UpdateWorld<float>(w, dt, collision_steps);
+		else
+			UpdateWorld<double>(w, dt, collision_steps);
+	}
+	return std::chrono::duration<double>(std::chrono::high_resolution_clock::now() - t0).count();
+}
+
+int xr_body_read(void *wp, int body, double *pos, double *prev, double *vel, double *plane, int *idx, double *pen, int *contact, double *bpos)
+{
+	XrWorld *w = (XrWorld *)wp;
+	return w->precision == 32 ? ReadBody<float>(w, body, pos, prev, vel, plane, idx, pen, contact, bpos)
+							  : ReadBody<double>(w, body, pos, prev, vel, plane, idx, pen, contact, bpos);
+}
+
+int xr_body_write(void *wp, int body, const double *pos, const double *vel)
+{
+	XrWorld *w = (XrWorld *)wp;
+	return w->precision == 32 ? WriteBody<float>(w, body, pos, vel) : WriteBody<double>(w, body, pos, vel);
+}
+
+// ACosApproximate restated (for the closed-form pins)
+float xr_acos_approximate(float x) { return ACosApprox<float>(x); }
+
+} // extern "C"
