@@ -1,0 +1,174 @@
+/*
+ * jolt_xpbd.h -- C ABI of the MI355X (gfx950) soft-body XPBD update.
+ *
+ * A drop-in for the soft-body hot path of Jolt Physics v5.5.1: the work that
+ * SoftBodyMotionProperties::ParallelUpdate performs for every soft body inside
+ * PhysicsSystem::Update (Jolt/Physics/PhysicsSystem.cpp:2701-2897) and inside
+ * the public single-body SoftBodyMotionProperties::CustomUpdate
+ * (Jolt/Physics/SoftBody/SoftBodyMotionProperties.cpp:1232-1256).
+ *
+ * All arrays crossing this boundary use the reference's own in-memory record
+ * layouts (Jolt/Physics/SoftBody/SoftBodySharedSettings.h), so an integration
+ * passes SoftBodySharedSettings::mEdgeConstraints.data() etc. unchanged:
+ *   Vertex 28 B {float pos[3], vel[3], invMass}          (.h:127-138)
+ *   Face 16 B {uint32 v[3], materialIndex}               (.h:141-154)
+ *   Edge 16 B {uint32 v[2]; float restLength, compliance} (.h:157-171)
+ *   DihedralBend 24 B {uint32 v[4]; float compliance, initialAngle} (.h:191-205)
+ *   Volume 24 B {uint32 v[4]; float sixRestVolume, compliance}      (.h:208-222)
+ *   LRA 12 B {uint32 v[2]; float maxDistance}            (.h:289-303)
+ *   UpdateGroup 28 B {uint32 edgeEnd, lraEnd, rodStretchShearEnd,
+ *                     rodBendTwistEnd, dihedralBendEnd, volumeEnd, skinnedEnd} (.h:374-383)
+ *
+ * Conventions: every function returns XPBD_OK (0) or a negative XPBD_E*
+ * code and never aborts; xpbd_last_error() describes the last failure of the
+ * calling thread.  The context owns all device memory and one HIP stream;
+ * caller buffers are only read/written during the call.  One host thread per
+ * context.
+ */
+#ifndef JOLT_XPBD_H
+#define JOLT_XPBD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XPBD_OK 0
+#define XPBD_E_INVALID -1       /* bad argument / handle / record */
+#define XPBD_E_UNSUPPORTED -2   /* feature outside this build (skinned, rods, ...) */
+#define XPBD_E_DEVICE -3        /* HIP runtime error */
+#define XPBD_E_NOMEM -4         /* device or host allocation failed */
+#define XPBD_E_CAPACITY -5      /* a compile-time capacity limit was exceeded */
+
+/* Collider shape types */
+#define XPBD_SHAPE_PLANE 0      /* params {nx, ny, nz, c, halfExtent}  PlaneShape */
+#define XPBD_SHAPE_SPHERE 1     /* params {radius}                     SphereShape */
+#define XPBD_SHAPE_BOX 2        /* params {hx, hy, hz}                 BoxShape */
+#define XPBD_SHAPE_CAPSULE 3    /* params {halfHeightOfCylinder, radius} CapsuleShape */
+
+typedef struct xpbd_context xpbd_context;
+typedef struct xpbd_settings xpbd_settings;
+
+/* Last error message of the calling thread (never NULL). */
+const char *xpbd_last_error(void);
+
+/* Library / device management.  device: HIP ordinal. */
+int xpbd_context_create(int device, xpbd_context **out_ctx);
+int xpbd_context_destroy(xpbd_context *ctx);
+/* The context's HIP stream (hipStream_t), for callers that time or order work. */
+void *xpbd_context_stream(xpbd_context *ctx);
+
+/*
+ * Shared settings: replaces SoftBodySharedSettings after Optimize()
+ * (SoftBodySharedSettings.cpp:613-1182).  Builds the order-preserving level
+ * schedule of the update groups and uploads everything.  Settings are
+ * immutable and shared by any number of bodies (like the reference's
+ * RefConst<SoftBodySharedSettings>, SoftBodyCreationSettings.h:49).
+ */
+typedef struct xpbd_settings_desc {
+    const void *vertices;       uint32_t num_vertices;
+    const void *faces;          uint32_t num_faces;
+    const void *edges;          uint32_t num_edges;
+    const void *dihedral_bends; uint32_t num_dihedral_bends;
+    const void *volumes;        uint32_t num_volumes;
+    const void *lras;           uint32_t num_lras;
+    const void *update_groups;  uint32_t num_update_groups;
+    /* must be 0: skinned constraints and Cosserat rods are not part of this build */
+    uint32_t num_skinned;
+    uint32_t num_rod_stretch_shear;
+    uint32_t num_rod_bend_twist;
+} xpbd_settings_desc;
+
+int xpbd_settings_create(xpbd_context *ctx, const xpbd_settings_desc *desc, xpbd_settings **out_settings);
+int xpbd_settings_release(xpbd_settings *settings);
+/* Schedule statistics: out[0]=parallel groups, out[1]=max levels of a parallel
+ * group, out[2]=tail levels, out[3]=tail constraints, out[4]=total constraints. */
+int xpbd_settings_stats(const xpbd_settings *settings, uint32_t out[5]);
+
+/*
+ * Soft body: replaces BodyInterface::CreateSoftBody + SoftBodyMotionProperties::
+ * Initialize (Body.cpp:466-492, SoftBodyMotionProperties.cpp:59-105) for the
+ * fields of SoftBodyCreationSettings that the update uses (.h:61-72).
+ */
+typedef struct xpbd_body_desc {
+    xpbd_settings *settings;
+    float position[3];
+    float rotation[4];          /* quaternion x, y, z, w */
+    uint32_t num_iterations;
+    float linear_damping;
+    float max_linear_velocity;
+    float restitution;
+    float friction;
+    float pressure;
+    float gravity_factor;
+    float vertex_radius;
+    int32_t update_position;
+    int32_t make_rotation_identity;
+    int32_t allow_sleeping;
+} xpbd_body_desc;
+
+int xpbd_body_create(xpbd_context *ctx, const xpbd_body_desc *desc, int32_t *out_body);
+
+/*
+ * Static colliders of the world, in broadphase order (the order in which
+ * PhysicsSystem::GetBroadPhaseQuery().CollideAABox reports them,
+ * SoftBodyMotionProperties.cpp:253).  Each step every soft body keeps the
+ * colliders whose world_bounds overlap its predicted bounds and transforms them
+ * into its local space exactly as DetermineCollidingShapes does
+ * (SoftBodyMotionProperties.cpp:120-255).  Friction / restitution are the
+ * collider body's own; they are combined with the soft body's as
+ * sqrt(f1*f2) / max(r1,r2) (ContactConstraintManager.h:554-555).
+ */
+typedef struct xpbd_collider_desc {
+    int32_t type;
+    float params[5];
+    float position[3];
+    float rotation[4];
+    float friction;
+    float restitution;
+    float world_bounds[6];      /* Body::GetWorldSpaceBounds(): min xyz, max xyz */
+} xpbd_collider_desc;
+
+int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *colliders, uint32_t num_colliders);
+int xpbd_set_gravity(xpbd_context *ctx, const float gravity[3]);
+
+/*
+ * Advances every active soft body by `frames` calls of
+ * PhysicsSystem::Update(dt, collision_steps) (PhysicsSystem.cpp:176): per
+ * collision step InitializeUpdateContext, DetermineCollidingShapes,
+ * DetermineCollisionPlanes, num_iterations x (pressure, integrate, update
+ * groups, collision + velocity update), UpdateSoftBodyState and the position
+ * update of JobSoftBodyFinalize.  Asynchronous on the context stream.
+ */
+int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t frames);
+int xpbd_synchronize(xpbd_context *ctx);
+
+/*
+ * Vertex state in body-local space (SoftBodyMotionProperties::GetVertices,
+ * SoftBodyMotionProperties.h:56-57).  Any pointer may be NULL.
+ *   pos/prev/vel: V x 3 floats; plane: V x 4 (normal, constant);
+ *   shape_idx: V int32 (-1 none); contact: V int32.
+ * body_pos[3], body_rot[4]: body position and rotation.  Synchronizes.
+ */
+int xpbd_body_read(xpbd_context *ctx, int32_t body, float *pos, float *prev, float *vel, float *plane,
+                   int32_t *shape_idx, int32_t *contact, float *body_pos, float *body_rot);
+/* Overwrites positions / velocities (body-local space).  NULL skips. */
+int xpbd_body_write(xpbd_context *ctx, int32_t body, const float *pos, const float *vel);
+/* Body summary: out[0..2] linear velocity, out[3..5] angular velocity (world),
+ * out[6..11] local bounds, out[12] active flag, out[13] sleep timer. */
+int xpbd_body_state(xpbd_context *ctx, int32_t body, float out[14]);
+
+/*
+ * Kernel timing on the context stream (hipEvents around every launch of
+ * xpbd_update while enabled).  xpbd_profile_read returns the number of timed
+ * launches and their summed duration in milliseconds, then resets.
+ */
+int xpbd_profile_enable(xpbd_context *ctx, int32_t enable);
+int xpbd_profile_read(xpbd_context *ctx, int32_t *num_launches, double *total_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JOLT_XPBD_H */

@@ -1,0 +1,180 @@
+// Float math with the operation order of Jolt's SSE code, for host and device.
+//
+// Every helper states the reference it matches (paths relative to the
+// reference's Jolt/ directory).  Compiled with -ffp-contract=off and without
+// fast-math, each expression rounds exactly like the reference's SSE build
+// with FMA contraction off, so results agree bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define XHD __host__ __device__ __forceinline__
+
+namespace xpbd {
+
+struct V3 { float x, y, z; };
+
+XHD V3 v3(float x, float y, float z) { V3 r; r.x = x; r.y = y; r.z = z; return r; }
+XHD V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+XHD V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+XHD V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }        // Vec3.inl:518
+XHD V3 operator*(float s, V3 a) { return v3(s * a.x, s * a.y, s * a.z); }        // Vec3.inl:535
+XHD V3 operator/(V3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }        // Vec3.inl:552
+XHD V3 mulv(V3 a, V3 b) { return v3(a.x * b.x, a.y * b.y, a.z * b.z); }          // Vec3.inl:500
+// Unary minus is 0 - v (Vec3.inl:669-672)
+XHD V3 neg(V3 a) { return v3(0.0f - a.x, 0.0f - a.y, 0.0f - a.z); }
+// Dot = (x*x' + y*y') + z*z' (Vec3.inl:901-934, ReduceSum)
+XHD float dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+XHD float len_sq(V3 a) { return dot(a, a); }
+XHD float len(V3 a) { return sqrtf(len_sq(a)); }                                  // Vec3.inl:951
+// Cross (Vec3.inl:855-863): per component mul, mul, sub
+XHD V3 cross(V3 a, V3 b) { return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+// _mm_min_ps / _mm_max_ps lane semantics
+XHD float smin(float a, float b) { return a < b ? a : b; }
+XHD float smax(float a, float b) { return a > b ? a : b; }
+XHD V3 vmin(V3 a, V3 b) { return v3(smin(a.x, b.x), smin(a.y, b.y), smin(a.z, b.z)); }
+XHD V3 vmax(V3 a, V3 b) { return v3(smax(a.x, b.x), smax(a.y, b.y), smax(a.z, b.z)); }
+XHD float comp(V3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+// Abs = max(0 - v, v) (Vec3.inl:817-822)
+XHD V3 vabs(V3 a) { return vmax(neg(a), a); }
+// GetSign: sign bit or'ed onto 1.0 (Vec3.inl:1161-1166)
+XHD float sign_bit_one(float v) { return __builtin_signbit(v) ? -1.0f : 1.0f; }
+XHD V3 vsign(V3 a) { return v3(sign_bit_one(a.x), sign_bit_one(a.y), sign_bit_one(a.z)); }
+// GetLowestComponentIndex (Vec3.inl:809-812)
+XHD int lowest_index(V3 a) { return a.x < a.y ? (a.z < a.x ? 2 : 0) : (a.z < a.y ? 2 : 1); }
+// Sign(0) = +1 (Math/Math.h:116-119)
+XHD float sign_of(float v) { return v < 0.0f ? -1.0f : 1.0f; }
+
+constexpr float kPi = 3.14159265358979323846f;     // JPH_PI
+
+// ACosApproximate (Math/Trigonometry.h:48-65)
+XHD float acos_approx(float x)
+{
+    float ax = fabsf(x);
+    float abs_x = ax < 1.0f ? ax : 1.0f;
+    float val = sqrtf(1.0f - abs_x) * (kPi / 2 - 0.175394f * abs_x);
+    return x < 0.0f ? kPi - val : val;
+}
+
+// Column-major 4x4: c[col][row]
+struct M44 { float c[4][4]; };
+
+XHD V3 col3(const M44 &m, int i) { return v3(m.c[i][0], m.c[i][1], m.c[i][2]); }
+
+// Mat44::sRotationTranslation (Mat44.inl:85-154; the scalar branch is written to
+// equal the SSE4.1 one)
+XHD M44 rotation_translation(float qx, float qy, float qz, float qw, V3 t)
+{
+    float tx = qx + qx, ty = qy + qy, tz = qz + qz;
+    float xx = tx * qx, yy = ty * qy, zz = tz * qz;
+    float xy = tx * qy, xz = tx * qz, xw = tx * qw;
+    float yz = ty * qz, yw = ty * qw, zw = tz * qw;
+    M44 m;
+    m.c[0][0] = (1.0f - yy) - zz; m.c[0][1] = xy + zw; m.c[0][2] = xz - yw; m.c[0][3] = 0.0f;
+    m.c[1][0] = xy - zw; m.c[1][1] = (1.0f - zz) - xx; m.c[1][2] = yz + xw; m.c[1][3] = 0.0f;
+    m.c[2][0] = xz + yw; m.c[2][1] = yz - xw; m.c[2][2] = (1.0f - xx) - yy; m.c[2][3] = 0.0f;
+    m.c[3][0] = t.x; m.c[3][1] = t.y; m.c[3][2] = t.z; m.c[3][3] = 1.0f;
+    return m;
+}
+
+// Mat44 * Vec3 (Mat44.inl:295-301): ((c0*x + c1*y) + c2*z) + c3
+XHD V3 mul_point(const M44 &m, V3 v)
+{
+    return v3(((m.c[0][0] * v.x + m.c[1][0] * v.y) + m.c[2][0] * v.z) + m.c[3][0],
+              ((m.c[0][1] * v.x + m.c[1][1] * v.y) + m.c[2][1] * v.z) + m.c[3][1],
+              ((m.c[0][2] * v.x + m.c[1][2] * v.y) + m.c[2][2] * v.z) + m.c[3][2]);
+}
+
+// Mat44::Multiply3x3 (Mat44.inl:384-389)
+XHD V3 mul3x3(const M44 &m, V3 v)
+{
+    return v3((m.c[0][0] * v.x + m.c[1][0] * v.y) + m.c[2][0] * v.z,
+              (m.c[0][1] * v.x + m.c[1][1] * v.y) + m.c[2][1] * v.z,
+              (m.c[0][2] * v.x + m.c[1][2] * v.y) + m.c[2][2] * v.z);
+}
+
+// Mat44::Multiply3x3Transposed (Mat44.inl:423-426) = Transposed3x3().Multiply3x3()
+XHD V3 mul3x3_transposed(const M44 &m, V3 v)
+{
+    return v3((m.c[0][0] * v.x + m.c[0][1] * v.y) + m.c[0][2] * v.z,
+              (m.c[1][0] * v.x + m.c[1][1] * v.y) + m.c[1][2] * v.z,
+              (m.c[2][0] * v.x + m.c[2][1] * v.y) + m.c[2][2] * v.z);
+}
+
+// Mat44 * Mat44 (Mat44.inl:241-252)
+XHD M44 mul(const M44 &a, const M44 &b)
+{
+    M44 r;
+    for (int i = 0; i < 4; ++i)
+        for (int k = 0; k < 4; ++k)
+            r.c[i][k] = ((a.c[0][k] * b.c[i][0] + a.c[1][k] * b.c[i][1]) + a.c[2][k] * b.c[i][2]) + a.c[3][k] * b.c[i][3];
+    return r;
+}
+
+// InversedRotationTranslation (Mat44.inl:944-949)
+XHD M44 inversed_rotation_translation(const M44 &m)
+{
+    M44 r;
+    for (int i = 0; i < 3; ++i)
+    {
+        for (int k = 0; k < 3; ++k)
+            r.c[i][k] = m.c[k][i];
+        r.c[i][3] = 0.0f;
+    }
+    r.c[3][0] = 0.0f; r.c[3][1] = 0.0f; r.c[3][2] = 0.0f; r.c[3][3] = 1.0f;
+    V3 t = neg(mul3x3(r, col3(m, 3)));
+    r.c[3][0] = t.x; r.c[3][1] = t.y; r.c[3][2] = t.z;
+    return r;
+}
+
+// Mat44::GetQuaternion (Mat44.inl:998-1050)
+XHD void get_quaternion(const M44 &m, float q[4])
+{
+    float tr = m.c[0][0] + m.c[1][1] + m.c[2][2];
+    if (tr >= 0.0f)
+    {
+        float s = sqrtf(tr + 1.0f);
+        float is = 0.5f / s;
+        q[0] = (m.c[1][2] - m.c[2][1]) * is; q[1] = (m.c[2][0] - m.c[0][2]) * is;
+        q[2] = (m.c[0][1] - m.c[1][0]) * is; q[3] = 0.5f * s;
+        return;
+    }
+    int i = 0;
+    if (m.c[1][1] > m.c[0][0]) i = 1;
+    if (m.c[2][2] > m.c[i][i]) i = 2;
+    if (i == 0)
+    {
+        float s = sqrtf(m.c[0][0] - (m.c[1][1] + m.c[2][2]) + 1.0f);
+        float is = 0.5f / s;
+        q[0] = 0.5f * s; q[1] = (m.c[1][0] + m.c[0][1]) * is; q[2] = (m.c[0][2] + m.c[2][0]) * is; q[3] = (m.c[1][2] - m.c[2][1]) * is;
+    }
+    else if (i == 1)
+    {
+        float s = sqrtf(m.c[1][1] - (m.c[2][2] + m.c[0][0]) + 1.0f);
+        float is = 0.5f / s;
+        q[0] = (m.c[1][0] + m.c[0][1]) * is; q[1] = 0.5f * s; q[2] = (m.c[2][1] + m.c[1][2]) * is; q[3] = (m.c[2][0] - m.c[0][2]) * is;
+    }
+    else
+    {
+        float s = sqrtf(m.c[2][2] - (m.c[0][0] + m.c[1][1]) + 1.0f);
+        float is = 0.5f / s;
+        q[0] = (m.c[0][2] + m.c[2][0]) * is; q[1] = (m.c[2][1] + m.c[1][2]) * is; q[2] = 0.5f * s; q[3] = (m.c[0][1] - m.c[1][0]) * is;
+    }
+}
+
+// Plane (Geometry/Plane.h): normal + constant
+struct Pl { V3 n; float c; };
+// sFromPointAndNormal (Plane.h:21)
+XHD Pl plane_from_point_normal(V3 p, V3 n) { Pl r; r.n = n; r.c = -dot(n, p); return r; }
+// GetTransformed (Plane.h:42-46)
+XHD Pl plane_transformed(const Pl &p, const M44 &m)
+{
+    V3 tn = mul3x3(m, p.n);
+    Pl r; r.n = tn; r.c = p.c - dot(col3(m, 3), tn);
+    return r;
+}
+// SignedDistance (Plane.h:57)
+XHD float signed_distance(const Pl &p, V3 x) { return dot(x, p.n) + p.c; }
+
+} // namespace xpbd

@@ -1,0 +1,804 @@
+// Soft-body XPBD update kernels for gfx950 (MI355X).
+//
+// One workgroup owns one soft body for a whole xpbd_update call (all frames,
+// collision steps and iterations): the body's positions and inverse masses
+// live in LDS as float4 for the entire launch ("LDS-resident" bodies).
+//
+// Constraint projection follows Jolt's update groups
+// (SoftBodyMotionProperties.cpp:1030-1115): the parallel groups are
+// vertex-disjoint and each runs on one wavefront; the last (non-parallel)
+// group runs after them on the whole workgroup.  Inside a group the reference
+// applies its constraints sequentially (volume, bend, edge, LRA order); the
+// host converts that sequence into levels in which no two constraints touch
+// the same vertex, ordered so that every read-after-write / write-after-read
+// pair keeps its sequential order.  Running a level lane-parallel therefore
+// gives bit-identical results to the sequential loop.
+//
+// All arithmetic follows the reference's operation order (see jolt_math.h);
+// the file is compiled with -ffp-contract=off and IEEE division / sqrt.
+
+#include <cfloat>
+
+#include "jolt_math.h"
+#include "xpbd_types.h"
+
+namespace xpbd {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / kWave;
+
+// Collider resolved into the soft body's local space for one collision step
+// (the reference's LeafShape + CollidingShape, SoftBodyMotionProperties.h).
+struct Leaf
+{
+    M44 t;          // leaf transform (soft-body local)
+    M44 inv;        // InversedRotationTranslation of t (box, capsule)
+    float pn[3];    // plane shape: transformed plane normal
+    float pc;       //              and constant
+    float a, b, c;  // sphere {radius, center from t} | box half extent | capsule {half height, radius}
+    int32_t type;
+    float friction, restitution;
+};
+
+// LDS layout of the resident kernel (dynamic shared memory):
+//   float4 X[num_vertices]
+//   Leaf   leaves[max_leaves]
+//   int    collider_slot[num_colliders]
+//   float4 scratch[num_faces]       (pressure only)
+//   float  red[kWaves * 16]         (reductions)
+struct LdsLayout
+{
+    uint32_t leaves, slots, scratch, red, total;
+};
+
+__host__ __device__ inline LdsLayout lds_layout(uint32_t nv, uint32_t max_leaves, uint32_t nc, uint32_t nf)
+{
+    LdsLayout l;
+    uint32_t off = nv * 16u;
+    off = (off + 15u) & ~15u; l.leaves = off; off += max_leaves * (uint32_t)sizeof(Leaf);
+    off = (off + 15u) & ~15u; l.slots = off; off += nc * 4u;
+    off = (off + 15u) & ~15u; l.scratch = off; off += nf * 16u;
+    off = (off + 15u) & ~15u; l.red = off; off += kWaves * 16u * 4u;
+    l.total = (off + 15u) & ~15u;
+    return l;
+}
+
+__device__ __forceinline__ V3 xyz(float4 p) { return v3(p.x, p.y, p.z); }
+__device__ __forceinline__ float4 f4(V3 p, float w) { return make_float4(p.x, p.y, p.z, w); }
+
+// Orders the LDS accesses of one wavefront around a level boundary.  LDS
+// instructions of a wavefront execute in issue order, so a later read by any
+// lane observes an earlier write; the fence keeps the compiler from moving
+// memory operations across the boundary.
+__device__ __forceinline__ void wave_level_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ---------------------------------------------------------------- constraints
+// ApplyEdgeConstraints body (SoftBodyMotionProperties.cpp:583-600)
+__device__ __forceinline__ void project_edge(float4 *X, uint32_t i0, uint32_t i1, float rest, float compliance, float inv_dt_sq)
+{
+    float4 p0 = X[i0], p1 = X[i1];
+    V3 x0 = xyz(p0), x1 = xyz(p1);
+    float w0 = p0.w, w1 = p1.w;
+    V3 delta = x1 - x0;
+    float length = len(delta);
+    float denom = length * ((w0 + w1) + compliance * inv_dt_sq);
+    if (denom < 1.0e-12f)
+        return;
+    V3 correction = (delta * (length - rest)) / denom;
+    X[i0] = f4(x0 + w0 * correction, w0);
+    X[i1] = f4(x1 - w1 * correction, w1);
+}
+
+// ApplyDihedralBendConstraints body (SoftBodyMotionProperties.cpp:375-455)
+__device__ __forceinline__ void project_bend(float4 *X, uint32_t i0, uint32_t i1, uint32_t i2, uint32_t i3, float compliance, float initial_angle, float inv_dt_sq)
+{
+    float4 p0 = X[i0], p1 = X[i1], p2 = X[i2], p3 = X[i3];
+    V3 x0 = xyz(p0), x1 = xyz(p1), x2 = xyz(p2), x3 = xyz(p3);
+    V3 e = x1 - x0;
+    float e_len = len(e);
+    if (e_len < 1.0e-6f)
+        return;
+    V3 x1x2 = x2 - x1;
+    V3 x1x3 = x3 - x1;
+    V3 n1 = cross(x2 - x0, x1x2);
+    V3 n2 = cross(x1x3, x3 - x0);
+    float n1_len_sq = len_sq(n1);
+    float n2_len_sq = len_sq(n2);
+    float n1n2 = n1_len_sq * n2_len_sq;
+    if (n1n2 < 1.0e-24f)
+        return;
+    float sign = sign_of(dot(cross(n2, n1), e));
+    float d = dot(n1, n2) / sqrtf(n1n2);
+    float c = sign * acos_approx(d) - initial_angle;
+    if (c > kPi)
+        c -= 2.0f * kPi;
+    else if (c < -kPi)
+        c += 2.0f * kPi;
+    n1 = n1 / n1_len_sq;
+    n2 = n2 / n2_len_sq;
+    V3 d0c = (dot(x1x2, e) * n1 + dot(x1x3, e) * n2) / e_len;
+    V3 d2c = e_len * n1;
+    V3 d3c = e_len * n2;
+    V3 d1c = (neg(d0c) - d2c) - d3c;
+    float w0 = p0.w, w1 = p1.w, w2 = p2.w, w3 = p3.w;
+    float denom = (((w0 * len_sq(d0c) + w1 * len_sq(d1c)) + w2 * len_sq(d2c)) + w3 * len_sq(d3c)) + compliance * inv_dt_sq;
+    if (denom < 1.0e-12f)
+        return;
+    float minus_lambda = c / denom;
+    X[i0] = f4(x0 - (minus_lambda * w0) * d0c, w0);
+    X[i1] = f4(x1 - (minus_lambda * w1) * d1c, w1);
+    X[i2] = f4(x2 - (minus_lambda * w2) * d2c, w2);
+    X[i3] = f4(x3 - (minus_lambda * w3) * d3c, w3);
+}
+
+// ApplyVolumeConstraints body (SoftBodyMotionProperties.cpp:467-506)
+__device__ __forceinline__ void project_volume(float4 *X, uint32_t i1, uint32_t i2, uint32_t i3, uint32_t i4, float six_rest_volume, float compliance, float inv_dt_sq)
+{
+    float4 p1 = X[i1], p2 = X[i2], p3 = X[i3], p4 = X[i4];
+    V3 x1 = xyz(p1), x2 = xyz(p2), x3 = xyz(p3), x4 = xyz(p4);
+    V3 x1x2 = x2 - x1;
+    V3 x1x3 = x3 - x1;
+    V3 x1x4 = x4 - x1;
+    float c = fabsf(dot(cross(x1x2, x1x3), x1x4)) - six_rest_volume;
+    V3 d1c = cross(x4 - x2, x3 - x2);
+    V3 d2c = cross(x1x3, x1x4);
+    V3 d3c = cross(x1x4, x1x2);
+    V3 d4c = cross(x1x2, x1x3);
+    float w1 = p1.w, w2 = p2.w, w3 = p3.w, w4 = p4.w;
+    float denom = (((w1 * len_sq(d1c) + w2 * len_sq(d2c)) + w3 * len_sq(d3c)) + w4 * len_sq(d4c)) + compliance * inv_dt_sq;
+    if (denom < 1.0e-12f)
+        return;
+    float minus_lambda = c / denom;
+    X[i1] = f4(x1 - (minus_lambda * w1) * d1c, w1);
+    X[i2] = f4(x2 - (minus_lambda * w2) * d2c, w2);
+    X[i3] = f4(x3 - (minus_lambda * w3) * d3c, w3);
+    X[i4] = f4(x4 - (minus_lambda * w4) * d4c, w4);
+}
+
+// ApplyLRAConstraints body (SoftBodyMotionProperties.cpp:685-697): only vertex 1 moves
+__device__ __forceinline__ void project_lra(float4 *X, uint32_t i0, uint32_t i1, float max_distance)
+{
+    V3 x0 = xyz(X[i0]);
+    float4 p1 = X[i1];
+    V3 delta = xyz(p1) - x0;
+    float delta_len_sq = len_sq(delta);
+    if (delta_len_sq > max_distance * max_distance)
+        X[i1] = f4(x0 + (delta * max_distance) / sqrtf(delta_len_sq), p1.w);
+}
+
+__device__ __forceinline__ void project_record(float4 *X, uint4 r, float inv_dt_sq)
+{
+    uint32_t type = (r.x >> 14) & 3u;
+    uint32_t i0 = r.x & 0x3fffu, i1 = r.x >> 16, i2 = r.y & 0xffffu, i3 = r.y >> 16;
+    float p0 = __uint_as_float(r.z), p1 = __uint_as_float(r.w);
+    switch (type)
+    {
+    case kTypeEdge: project_edge(X, i0, i1, p0, p1, inv_dt_sq); break;
+    case kTypeBend: project_bend(X, i0, i1, i2, i3, p0, p1, inv_dt_sq); break;
+    case kTypeVolume: project_volume(X, i0, i1, i2, i3, p0, p1, inv_dt_sq); break;
+    default: project_lra(X, i0, i1, p0); break;
+    }
+}
+
+// One update group on one wavefront, level by level.
+__device__ __forceinline__ void run_group_wave(const SettingsDev &S, uint32_t g, float4 *X, float inv_dt_sq, int lane)
+{
+    uint2 gi = S.groups[g];
+    const uint32_t *lvl = S.lvl + gi.x;
+    uint32_t begin = lvl[0];
+    for (uint32_t L = 0; L < gi.y; ++L)
+    {
+        uint32_t end = lvl[L + 1];
+        for (uint32_t i = begin + lane; i < end; i += kWave)
+            project_record(X, S.recs[i], inv_dt_sq);
+        wave_level_sync();
+        begin = end;
+    }
+}
+
+// The non-parallel group on the whole workgroup (a barrier per level).
+__device__ __forceinline__ void run_group_block(const SettingsDev &S, uint32_t g, float4 *X, float inv_dt_sq, int tid)
+{
+    uint2 gi = S.groups[g];
+    const uint32_t *lvl = S.lvl + gi.x;
+    uint32_t begin = lvl[0];
+    for (uint32_t L = 0; L < gi.y; ++L)
+    {
+        uint32_t end = lvl[L + 1];
+        for (uint32_t i = begin + tid; i < end; i += kBlock)
+            project_record(X, S.recs[i], inv_dt_sq);
+        __syncthreads();
+        begin = end;
+    }
+}
+
+// ---------------------------------------------------------------- collision
+// Shape::CollideSoftBodyVertices for plane / sphere / box / capsule with the
+// argmax rule of CollideSoftBodyVertexIterator::UpdatePenetration
+// (CollideSoftBodyVertexIterator.h:83-97): a leaf replaces the stored plane
+// unless the stored penetration is >= the new one.
+__device__ __forceinline__ void collide_vertex(const Leaf *leaves, uint32_t num_leaves, V3 x, float &largest, int32_t &idx, Pl &plane)
+{
+    for (uint32_t l = 0; l < num_leaves; ++l)
+    {
+        const Leaf &L = leaves[l];
+        switch (L.type)
+        {
+        case 0: // PlaneShape.cpp:243-256
+            {
+                Pl p; p.n = v3(L.pn[0], L.pn[1], L.pn[2]); p.c = L.pc;
+                float pen = -signed_distance(p, x);
+                if (!(largest >= pen)) { largest = pen; plane = p; idx = (int32_t)l; }
+            }
+            break;
+        case 1: // SphereShape.cpp:279-301
+            {
+                V3 center = col3(L.t, 3);
+                float radius = L.a;
+                V3 delta = x - center;
+                float distance = len(delta);
+                float pen = radius - distance;
+                if (!(largest >= pen))
+                {
+                    largest = pen;
+                    V3 normal = distance > 0.0f ? delta / distance : v3(0.0f, 1.0f, 0.0f);
+                    V3 point = center + radius * normal;
+                    plane = plane_from_point_normal(point, normal);
+                    idx = (int32_t)l;
+                }
+            }
+            break;
+        case 2: // BoxShape.cpp:234-283
+            {
+                V3 he = v3(L.a, L.b, L.c);
+                V3 local = mul_point(L.inv, x);
+                V3 clamped = vmax(vmin(local, he), neg(he));
+                if (clamped.x == local.x && clamped.y == local.y && clamped.z == local.z)
+                {
+                    V3 delta = he - vabs(local);
+                    int index = lowest_index(delta);
+                    float pen = comp(delta, index);
+                    if (!(largest >= pen))
+                    {
+                        largest = pen;
+                        V3 axis = v3(index == 0 ? 1.0f : 0.0f, index == 1 ? 1.0f : 0.0f, index == 2 ? 1.0f : 0.0f);
+                        V3 normal = mulv(vsign(local), axis);
+                        V3 point = mulv(normal, he);
+                        plane = plane_transformed(plane_from_point_normal(point, normal), L.t);
+                        idx = (int32_t)l;
+                    }
+                }
+                else
+                {
+                    V3 normal = local - clamped;
+                    float normal_length = len(normal);
+                    float pen = -normal_length;
+                    if (!(largest >= pen))
+                    {
+                        largest = pen;
+                        normal = normal / normal_length;
+                        plane = plane_transformed(plane_from_point_normal(clamped, normal), L.t);
+                        idx = (int32_t)l;
+                    }
+                }
+            }
+            break;
+        default: // CapsuleShape.cpp:324-375
+            {
+                float hh = L.a, radius = L.b;
+                V3 local = mul_point(L.inv, x);
+                if (fabsf(local.y) <= hh)
+                {
+                    V3 normal = v3(local.x, 0.0f, local.z);
+                    float normal_length = len(normal);
+                    float pen = radius - normal_length;
+                    if (!(largest >= pen))
+                    {
+                        largest = pen;
+                        normal = normal_length > 0.0f ? normal / normal_length : v3(1.0f, 0.0f, 0.0f);
+                        V3 point = radius * normal;
+                        plane = plane_transformed(plane_from_point_normal(point, normal), L.t);
+                        idx = (int32_t)l;
+                    }
+                }
+                else
+                {
+                    V3 center = v3(0.0f, sign_of(local.y) * hh, 0.0f);
+                    V3 delta = local - center;
+                    float distance = len(delta);
+                    float pen = radius - distance;
+                    if (!(largest >= pen))
+                    {
+                        largest = pen;
+                        V3 normal = delta / distance;
+                        V3 point = center + radius * normal;
+                        plane = plane_transformed(plane_from_point_normal(point, normal), L.t);
+                        idx = (int32_t)l;
+                    }
+                }
+            }
+            break;
+        }
+    }
+}
+
+// Leaf of collider c in the soft body's local space (SoftBodyMotionProperties.cpp:174-186:
+// com = (inverse soft body transform * collider transform), then the leaf
+// transform is rebuilt from com's quaternion and translation).
+__device__ __forceinline__ void make_leaf(const ColliderDev &c, const M44 &inv_com, float body_friction, float body_restitution, Leaf &out)
+{
+    M44 ccom = rotation_translation(c.rot[0], c.rot[1], c.rot[2], c.rot[3], v3(c.pos[0], c.pos[1], c.pos[2]));
+    M44 rel = mul(inv_com, ccom);
+    float q[4];
+    get_quaternion(rel, q);
+    M44 t = rotation_translation(q[0], q[1], q[2], q[3], col3(rel, 3));
+    out.t = t;
+    out.type = c.type;
+    out.friction = sqrtf(body_friction * c.friction);                                 // ContactConstraintManager.h:554
+    out.restitution = body_restitution > c.restitution ? body_restitution : c.restitution; // ContactConstraintManager.h:555
+    switch (c.type)
+    {
+    case 0:
+        {
+            // Plane::Scaled with unit scale (Plane.h:49-54) then GetTransformed
+            V3 n = v3(c.params[0], c.params[1], c.params[2]);
+            V3 sn = v3(n.x / 1.0f, n.y / 1.0f, n.z / 1.0f);
+            float snl = len(sn);
+            Pl p; p.n = sn / snl; p.c = c.params[3] / snl;
+            Pl tp = plane_transformed(p, t);
+            out.pn[0] = tp.n.x; out.pn[1] = tp.n.y; out.pn[2] = tp.n.z; out.pc = tp.c;
+        }
+        break;
+    case 1:
+        out.a = fabsf(1.0f) * c.params[0];
+        break;
+    case 2:
+        out.inv = inversed_rotation_translation(t);
+        out.a = 1.0f * c.params[0]; out.b = 1.0f * c.params[1]; out.c = 1.0f * c.params[2];
+        break;
+    default:
+        out.inv = inversed_rotation_translation(t);
+        out.a = 1.0f * c.params[0]; out.b = 1.0f * c.params[1];
+        break;
+    }
+}
+
+// ---------------------------------------------------------------- reductions
+__device__ __forceinline__ float wave_min(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) { float u = __shfl_xor(v, o, kWave); v = u < v ? u : v; }
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) { float u = __shfl_xor(v, o, kWave); v = u > v ? u : v; }
+    return v;
+}
+__device__ __forceinline__ float wave_sum(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// ---------------------------------------------------------------- kernel
+// Advances each body of `body_list` by `frames` x `steps` collision steps.
+__global__ __launch_bounds__(kBlock) void resident_update_kernel(
+    const BodyDev *__restrict__ bodies, BodyStateDev *__restrict__ states, const uint32_t *__restrict__ body_list,
+    const ColliderDev *__restrict__ colliders, uint32_t num_colliders, uint32_t max_leaves,
+    float gx, float gy, float gz, float dt, int32_t steps, int32_t frames, int32_t *__restrict__ error_flag)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ BodyStateDev st;
+    __shared__ float sh_scalars[8];
+    __shared__ uint32_t sh_num_leaves;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = tid >> 6;
+    const uint32_t bi = body_list[blockIdx.x];
+    const BodyDev &B = bodies[bi];
+    const SettingsDev &S = B.s;
+    const uint32_t nv = S.num_vertices;
+    const bool has_pressure = B.pressure > 0.0f && S.num_faces > 0;
+    const LdsLayout lay = lds_layout(nv, max_leaves, num_colliders, has_pressure ? S.num_faces : 0u);
+    float4 *X = reinterpret_cast<float4 *>(lds_raw);
+    Leaf *leaves = reinterpret_cast<Leaf *>(lds_raw + lay.leaves);
+    int32_t *slots = reinterpret_cast<int32_t *>(lds_raw + lay.slots);
+    float4 *scratch = reinterpret_cast<float4 *>(lds_raw + lay.scratch);
+    float *red = reinterpret_cast<float *>(lds_raw + lay.red);
+
+    if (tid == 0)
+        st = states[bi];
+    for (uint32_t i = tid; i < nv; i += kBlock)
+        X[i] = B.x[i];
+    __syncthreads();
+
+    const V3 gravity_world = v3(gx, gy, gz);
+    // PhysicsSystem::Update: step delta time (PhysicsSystem.cpp:215)
+    const float step_dt = dt / (float)steps;
+    const uint32_t iterations = B.iterations;
+
+    for (int32_t frame = 0; frame < frames; ++frame)
+        for (int32_t step = 0; step < steps; ++step)
+        {
+            if (!st.active)
+                break;
+
+            // InitializeUpdateContext (SoftBodyMotionProperties.cpp:937-960)
+            const M44 com = rotation_translation(st.rot[0], st.rot[1], st.rot[2], st.rot[3], v3(st.pos[0], st.pos[1], st.pos[2]));
+            const V3 g_local = mul3x3_transposed(com, B.gravity_factor * gravity_world);
+            const float dt_sub = step_dt / (float)iterations;
+            const V3 disp_gravity = (((0.5f * (float)iterations) * (float)(iterations + 1)) * (dt_sub * dt_sub)) * g_local;
+            const float inv_dt_sq = 1.0f / (dt_sub * dt_sub);
+
+            // DetermineCollidingShapes (SoftBodyMotionProperties.cpp:236-253), static colliders
+            {
+                V3 lmin = vmin(v3(st.bmin[0], st.bmin[1], st.bmin[2]), v3(st.pmin[0], st.pmin[1], st.pmin[2]));
+                V3 lmax = vmax(v3(st.bmax[0], st.bmax[1], st.bmax[2]), v3(st.pmax[0], st.pmax[1], st.pmax[2]));
+                V3 r = v3(B.vertex_radius, B.vertex_radius, B.vertex_radius);
+                lmin = lmin - r;
+                lmax = lmax + r;
+                // AABox::Transformed (Geometry/AABox.h:193-213)
+                V3 wmin = col3(com, 3), wmax = col3(com, 3);
+                for (int c = 0; c < 3; ++c)
+                {
+                    V3 col = col3(com, c);
+                    V3 a = col * comp(lmin, c);
+                    V3 b = col * comp(lmax, c);
+                    wmin = wmin + vmin(a, b);
+                    wmax = wmax + vmax(a, b);
+                }
+                for (uint32_t ci = tid; ci < num_colliders; ci += kBlock)
+                {
+                    const ColliderDev &cd = colliders[ci];
+                    // AABox::Overlaps (Geometry/AABox.h:164-167)
+                    bool overlap = !(wmin.x > cd.bounds[3] || wmin.y > cd.bounds[4] || wmin.z > cd.bounds[5]
+                                     || wmax.x < cd.bounds[0] || wmax.y < cd.bounds[1] || wmax.z < cd.bounds[2]);
+                    slots[ci] = overlap ? 1 : 0;
+                }
+                __syncthreads();
+                if (tid == 0)
+                {
+                    uint32_t n = 0;
+                    for (uint32_t ci = 0; ci < num_colliders; ++ci)
+                    {
+                        if (slots[ci])
+                        {
+                            if (n < max_leaves)
+                                slots[ci] = (int32_t)n;
+                            else
+                            {
+                                slots[ci] = -1;
+                                atomicOr(error_flag, 1);
+                            }
+                            ++n;
+                        }
+                        else
+                            slots[ci] = -1;
+                    }
+                    sh_num_leaves = n < max_leaves ? n : max_leaves;
+                }
+                __syncthreads();
+                const M44 inv_com = inversed_rotation_translation(com);
+                for (uint32_t ci = tid; ci < num_colliders; ci += kBlock)
+                    if (slots[ci] >= 0)
+                        make_leaf(colliders[ci], inv_com, B.friction, B.restitution, leaves[slots[ci]]);
+                __syncthreads();
+            }
+            const uint32_t num_leaves = sh_num_leaves;
+
+            // DetermineCollisionPlanes (SoftBodyMotionProperties.cpp:257-265)
+            for (uint32_t i = tid; i < nv; i += kBlock)
+            {
+                float4 p = X[i];
+                if (p.w > 0.0f)
+                {
+                    float4 pl = B.plane[i];
+                    Pl plane; plane.n = v3(pl.x, pl.y, pl.z); plane.c = pl.w;
+                    float largest = -FLT_MAX;
+                    int32_t idx = -1;
+                    collide_vertex(leaves, num_leaves, xyz(p), largest, idx, plane);
+                    B.idx[i] = idx;
+                    if (idx >= 0)
+                        B.plane[i] = make_float4(plane.n.x, plane.n.y, plane.n.z, plane.c);
+                }
+            }
+
+            // ApplyPressure (SoftBodyMotionProperties.cpp:291-323)
+            auto apply_pressure = [&]() {
+                const uint32_t nf = S.num_faces;
+                for (uint32_t f = tid; f < nf; f += kBlock)
+                {
+                    V3 x1 = xyz(X[S.faces[3 * f]]), x2 = xyz(X[S.faces[3 * f + 1]]), x3 = xyz(X[S.faces[3 * f + 2]]);
+                    scratch[f] = make_float4(0.0f, 0.0f, 0.0f, dot(cross(x1, x2), x3));
+                }
+                __syncthreads();
+                if (tid == 0)
+                {
+                    // GetVolumeTimesSix: sequential sum in face order (SoftBodyMotionProperties.cpp:107-118)
+                    float six_volume = 0.0f;
+                    for (uint32_t f = 0; f < nf; ++f)
+                        six_volume += scratch[f].w;
+                    sh_scalars[0] = six_volume;
+                }
+                __syncthreads();
+                const float six_volume = sh_scalars[0];
+                if (six_volume > 0.0f)
+                {
+                    const float coefficient = B.pressure * dt_sub / six_volume;
+                    for (uint32_t f = tid; f < nf; f += kBlock)
+                    {
+                        V3 x1 = xyz(X[S.faces[3 * f]]), x2 = xyz(X[S.faces[3 * f + 1]]), x3 = xyz(X[S.faces[3 * f + 2]]);
+                        V3 impulse = coefficient * cross(x2 - x1, x3 - x1);
+                        scratch[f] = make_float4(impulse.x, impulse.y, impulse.z, 0.0f);
+                    }
+                    __syncthreads();
+                    // v_i += w_i * impulse_f for the faces of vertex i in face order
+                    for (uint32_t i = tid; i < nv; i += kBlock)
+                    {
+                        uint32_t fb = S.vf_begin[i], fe = S.vf_begin[i + 1];
+                        if (fb == fe)
+                            continue;
+                        float w = X[i].w;
+                        float4 vv = B.v[i];
+                        V3 v = xyz(vv);
+                        for (uint32_t k = fb; k < fe; ++k)
+                        {
+                            float4 imp = scratch[S.vf_list[k]];
+                            v = v + w * xyz(imp);
+                        }
+                        B.v[i] = f4(v, vv.w);
+                    }
+                }
+                __syncthreads();
+            };
+
+            // IntegratePositions (SoftBodyMotionProperties.cpp:325-350); accumulated force is zero
+            const float damping = smax(0.0f, 1.0f - B.linear_damping * dt_sub);
+            const V3 sub_step_gravity = g_local * dt_sub;
+            const V3 sub_step_impulse = (v3(0.0f, 0.0f, 0.0f) * dt_sub) / smax((float)nv, 1.0f);
+            auto integrate_vertex = [&](uint32_t i, float4 p, V3 v) {
+                if (p.w > 0.0f)
+                {
+                    v = v + (sub_step_gravity + sub_step_impulse * p.w);
+                    v = v * damping;
+                }
+                V3 position = xyz(p);
+                B.prev[i] = f4(position, 0.0f);
+                X[i] = f4(position + v * dt_sub, p.w);
+                B.v[i] = f4(v, 0.0f);
+            };
+
+            if (has_pressure)
+                apply_pressure();
+            for (uint32_t i = tid; i < nv; i += kBlock)
+                integrate_vertex(i, X[i], xyz(B.v[i]));
+            __syncthreads();
+
+            const float restitution_threshold = -2.0f * len(g_local) * dt_sub;
+            const float max_lin_sq = B.max_linear_velocity * B.max_linear_velocity;
+            float max_v_sq = 0.0f;
+            V3 bmin = v3(FLT_MAX, FLT_MAX, FLT_MAX), bmax = v3(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+            V3 pmin = bmin, pmax = bmax;
+            V3 lin = v3(0.0f, 0.0f, 0.0f), ang = v3(0.0f, 0.0f, 0.0f);
+
+            for (uint32_t it = 0; it < iterations; ++it)
+            {
+                const bool last = it + 1 == iterations;
+
+                // Parallel groups: group g on wave g mod kWaves
+                for (uint32_t g = wave; g < S.num_parallel; g += kWaves)
+                    run_group_wave(S, g, X, inv_dt_sq, lane);
+                __syncthreads();
+
+                // Non-parallel group
+                if (S.tail_one_wave)
+                {
+                    if (wave == 0)
+                        run_group_wave(S, S.num_parallel, X, inv_dt_sq, lane);
+                }
+                else
+                    run_group_block(S, S.num_parallel, X, inv_dt_sq, tid);
+                __syncthreads();
+
+                // ApplyCollisionConstraintsAndUpdateVelocities (SoftBodyMotionProperties.cpp:700-825),
+                // static-body branch; fused with the next iteration's integration when there is
+                // no pressure, or with UpdateSoftBodyState's per-vertex work after the last one.
+                for (uint32_t i = tid; i < nv; i += kBlock)
+                {
+                    float4 p = X[i];
+                    V3 x = xyz(p);
+                    V3 v = xyz(B.v[i]);
+                    if (p.w > 0.0f)
+                    {
+                        V3 prev_v = v;
+                        v = (x - xyz(B.prev[i])) / dt_sub;
+                        int32_t idx = B.idx[i];
+                        if (idx >= 0)
+                        {
+                            float4 pl = B.plane[i];
+                            Pl plane; plane.n = v3(pl.x, pl.y, pl.z); plane.c = pl.w;
+                            float projected_distance = -signed_distance(plane, x) + B.vertex_radius;
+                            if (projected_distance > 0.0f)
+                            {
+                                B.contact[i] = 1;
+                                const Leaf &cs = leaves[idx];
+                                V3 n = plane.n;
+                                x = x + n * projected_distance;
+                                V3 v_normal = n * dot(n, v);
+                                V3 v_tangential = v - v_normal;
+                                float vtl = len(v_tangential);
+                                if (vtl > 0.0f)
+                                {
+                                    float f = cs.friction * projected_distance / (vtl * dt_sub);
+                                    v = v - v_tangential * (f < 1.0f ? f : 1.0f);
+                                }
+                                v = v - v_normal;
+                                float prev_v_normal = dot(prev_v, n);
+                                if (prev_v_normal < restitution_threshold)
+                                    v = v - (cs.restitution * prev_v_normal) * n;
+                            }
+                        }
+                    }
+                    if (!last)
+                    {
+                        if (has_pressure)
+                        {
+                            X[i] = f4(x, p.w);
+                            B.v[i] = f4(v, 0.0f);
+                        }
+                        else
+                            integrate_vertex(i, f4(x, p.w), v);
+                    }
+                    else
+                    {
+                        // UpdateSoftBodyState per-vertex part (SoftBodyMotionProperties.cpp:852-880)
+                        float v_sq = len_sq(v);
+                        max_v_sq = smax(max_v_sq, v_sq);
+                        if (v_sq > max_lin_sq)
+                            v = v * sqrtf(max_lin_sq / v_sq);
+                        lin = lin + v;
+                        ang = ang + cross(x, v);
+                        bmin = vmin(bmin, x);
+                        bmax = vmax(bmax, x);
+                        V3 pp = (x + v * step_dt) + disp_gravity;
+                        pmin = vmin(pmin, pp);
+                        pmax = vmax(pmax, pp);
+                        X[i] = f4(x, p.w);
+                        B.v[i] = f4(v, 0.0f);
+                        B.idx[i] = -1;
+                        B.contact[i] = 0;
+                    }
+                }
+                __syncthreads();
+                if (!last && has_pressure)
+                {
+                    apply_pressure();
+                    for (uint32_t i = tid; i < nv; i += kBlock)
+                        integrate_vertex(i, X[i], xyz(B.v[i]));
+                    __syncthreads();
+                }
+            }
+
+            // UpdateSoftBodyState reductions (SoftBodyMotionProperties.cpp:846-921)
+            {
+                float vals[16] = { bmin.x, bmin.y, bmin.z, pmin.x, pmin.y, pmin.z,
+                                   bmax.x, bmax.y, bmax.z, pmax.x, pmax.y, pmax.z, max_v_sq, 0, 0, 0 };
+                for (int k = 0; k < 6; ++k) vals[k] = wave_min(vals[k]);
+                for (int k = 6; k < 13; ++k) vals[k] = wave_max(vals[k]);
+                if (lane == 0)
+                    for (int k = 0; k < 13; ++k) red[wave * 16 + k] = vals[k];
+                __syncthreads();
+                if (tid == 0)
+                {
+                    for (int w = 1; w < kWaves; ++w)
+                    {
+                        for (int k = 0; k < 6; ++k) vals[k] = smin(vals[k], red[w * 16 + k]);
+                        for (int k = 6; k < 13; ++k) vals[k] = smax(vals[k], red[w * 16 + k]);
+                    }
+                }
+                __syncthreads();
+                // body velocity sums (order differs from the reference's sequential sum)
+                float sums[6] = { wave_sum(lin.x), wave_sum(lin.y), wave_sum(lin.z), wave_sum(ang.x), wave_sum(ang.y), wave_sum(ang.z) };
+                if (lane == 0)
+                    for (int k = 0; k < 6; ++k) red[wave * 16 + k] = sums[k];
+                __syncthreads();
+                if (tid == 0)
+                {
+                    for (int w = 1; w < kWaves; ++w)
+                        for (int k = 0; k < 6; ++k) sums[k] += red[w * 16 + k];
+                    V3 bmn = v3(vals[0], vals[1], vals[2]), pmn = v3(vals[3], vals[4], vals[5]);
+                    V3 bmx = v3(vals[6], vals[7], vals[8]), pmx = v3(vals[9], vals[10], vals[11]);
+                    float mvsq = vals[12];
+                    float divider = (float)(nv > 1 ? nv : 1u);
+                    V3 lv = mul3x3(com, v3(sums[0], sums[1], sums[2]) / divider);
+                    V3 av = mul3x3(com, v3(sums[3], sums[4], sums[5]) / divider);
+                    st.lin[0] = lv.x; st.lin[1] = lv.y; st.lin[2] = lv.z;
+                    st.ang[0] = av.x; st.ang[1] = av.y; st.ang[2] = av.z;
+                    V3 delta = v3(0.0f, 0.0f, 0.0f);
+                    if (B.update_position)
+                    {
+                        delta = 0.5f * (bmn + bmx);                   // AABox::GetCenter
+                        V3 dw = mul3x3(com, delta);
+                        V3 nd = neg(delta);
+                        bmn = bmn + nd; bmx = bmx + nd; pmn = pmn + nd; pmx = pmx + nd;
+                        // JobSoftBodyFinalize: GetPosition() + mDeltaPosition (PhysicsSystem.cpp:2864)
+                        st.pos[0] = st.pos[0] + dw.x; st.pos[1] = st.pos[1] + dw.y; st.pos[2] = st.pos[2] + dw.z;
+                    }
+                    st.bmin[0] = bmn.x; st.bmin[1] = bmn.y; st.bmin[2] = bmn.z;
+                    st.bmax[0] = bmx.x; st.bmax[1] = bmx.y; st.bmax[2] = bmx.z;
+                    st.pmin[0] = pmn.x; st.pmin[1] = pmn.y; st.pmin[2] = pmn.z;
+                    st.pmax[0] = pmx.x; st.pmax[1] = pmx.y; st.pmax[2] = pmx.z;
+                    sh_scalars[1] = delta.x; sh_scalars[2] = delta.y; sh_scalars[3] = delta.z;
+                    if (B.allow_sleeping)
+                    {
+                        // MotionProperties sleep test (SoftBodyMotionProperties.cpp:903-914, MotionProperties.inl:169-173)
+                        if (mvsq > 0.03f)
+                            st.sleep_timer = 0.0f;
+                        else
+                        {
+                            st.sleep_timer += step_dt;
+                            if (st.sleep_timer >= 0.5f)
+                                st.active = 0;
+                        }
+                    }
+                }
+                __syncthreads();
+                if (B.update_position)
+                {
+                    V3 delta = v3(sh_scalars[1], sh_scalars[2], sh_scalars[3]);
+                    for (uint32_t i = tid; i < nv; i += kBlock)
+                    {
+                        float4 p = X[i];
+                        X[i] = f4(xyz(p) - delta, p.w);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+
+    for (uint32_t i = tid; i < nv; i += kBlock)
+        B.x[i] = X[i];
+    if (tid == 0)
+        states[bi] = st;
+}
+
+} // namespace xpbd
+
+// ---------------------------------------------------------------- launchers
+namespace xpbd {
+
+uint32_t resident_lds_bytes(uint32_t nv, uint32_t max_leaves, uint32_t nc, uint32_t nf)
+{
+    return lds_layout(nv, max_leaves, nc, nf).total;
+}
+
+uint32_t leaf_bytes() { return (uint32_t)sizeof(Leaf); }
+
+hipError_t launch_resident(const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list, uint32_t num_blocks,
+                           const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves, uint32_t lds_bytes,
+                           float gx, float gy, float gz, float dt, int32_t steps, int32_t frames, int32_t *error_flag,
+                           hipStream_t stream)
+{
+    if (num_blocks == 0)
+        return hipSuccess;
+    static bool configured = false;
+    if (!configured)
+    {
+        hipError_t e = hipFuncSetAttribute((const void *)resident_update_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+        if (e != hipSuccess)
+            return e;
+        configured = true;
+    }
+    hipLaunchKernelGGL(resident_update_kernel, dim3(num_blocks), dim3(kBlock), lds_bytes, stream,
+                       bodies, states, body_list, colliders, num_colliders, max_leaves, gx, gy, gz, dt, steps, frames, error_flag);
+    return hipGetLastError();
+}
+
+} // namespace xpbd

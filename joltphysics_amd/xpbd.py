@@ -1,0 +1,257 @@
+"""ctypes binding of the C ABI in include/jolt_xpbd.h (libjolt_xpbd.so).
+
+The binding is a thin mirror of the reference's soft-body interface:
+``Context.create_settings`` takes the arrays of a SoftBodySharedSettings after
+Optimize, ``Context.create_body`` the fields of SoftBodyCreationSettings,
+``Context.set_colliders`` the static bodies and ``Context.update`` runs
+PhysicsSystem::Update(dt, collision_steps) for every soft body on the GPU.
+
+There is no CPU fallback: if the HIP library is missing or no GPU is present
+the calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .records import (BEND, EDGE, FACE, GROUP, LRA, VERTEX, VOLUME, BodyParams, SettingsArrays)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libjolt_xpbd.so")
+
+XPBD_OK = 0
+
+
+class XpbdError(RuntimeError):
+    pass
+
+
+class _SettingsDesc(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("num_vertices", C.c_uint32),
+                ("faces", C.c_void_p), ("num_faces", C.c_uint32),
+                ("edges", C.c_void_p), ("num_edges", C.c_uint32),
+                ("dihedral_bends", C.c_void_p), ("num_dihedral_bends", C.c_uint32),
+                ("volumes", C.c_void_p), ("num_volumes", C.c_uint32),
+                ("lras", C.c_void_p), ("num_lras", C.c_uint32),
+                ("update_groups", C.c_void_p), ("num_update_groups", C.c_uint32),
+                ("num_skinned", C.c_uint32), ("num_rod_stretch_shear", C.c_uint32),
+                ("num_rod_bend_twist", C.c_uint32)]
+
+
+class _BodyDesc(C.Structure):
+    _fields_ = [("settings", C.c_void_p), ("position", C.c_float * 3), ("rotation", C.c_float * 4),
+                ("num_iterations", C.c_uint32), ("linear_damping", C.c_float),
+                ("max_linear_velocity", C.c_float), ("restitution", C.c_float), ("friction", C.c_float),
+                ("pressure", C.c_float), ("gravity_factor", C.c_float), ("vertex_radius", C.c_float),
+                ("update_position", C.c_int32), ("make_rotation_identity", C.c_int32),
+                ("allow_sleeping", C.c_int32)]
+
+
+class _ColliderDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("params", C.c_float * 5), ("position", C.c_float * 3),
+                ("rotation", C.c_float * 4), ("friction", C.c_float), ("restitution", C.c_float),
+                ("world_bounds", C.c_float * 6)]
+
+
+# Exported symbols of the C ABI, in declaration order of include/jolt_xpbd.h.
+EXPORTED_SYMBOLS = (
+    "xpbd_last_error", "xpbd_context_create", "xpbd_context_destroy", "xpbd_context_stream",
+    "xpbd_settings_create", "xpbd_settings_release", "xpbd_settings_stats", "xpbd_body_create",
+    "xpbd_set_colliders", "xpbd_set_gravity", "xpbd_update", "xpbd_synchronize", "xpbd_body_read",
+    "xpbd_body_write", "xpbd_body_state", "xpbd_profile_enable", "xpbd_profile_read",
+)
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads libjolt_xpbd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise XpbdError(f"HIP library {path} is missing; run joltphysics_amd/build.py "
+                        "(there is no CPU fallback)")
+    lib = C.CDLL(path)
+    vp, i32, u32, f = C.c_void_p, C.c_int32, C.c_uint32, C.c_float
+    lib.xpbd_last_error.restype = C.c_char_p
+    lib.xpbd_context_create.argtypes = [C.c_int, C.POINTER(vp)]
+    lib.xpbd_context_destroy.argtypes = [vp]
+    lib.xpbd_context_stream.restype = vp
+    lib.xpbd_context_stream.argtypes = [vp]
+    lib.xpbd_settings_create.argtypes = [vp, C.POINTER(_SettingsDesc), C.POINTER(vp)]
+    lib.xpbd_settings_release.argtypes = [vp]
+    lib.xpbd_settings_stats.argtypes = [vp, C.POINTER(u32)]
+    lib.xpbd_body_create.argtypes = [vp, C.POINTER(_BodyDesc), C.POINTER(i32)]
+    lib.xpbd_set_colliders.argtypes = [vp, C.POINTER(_ColliderDesc), u32]
+    lib.xpbd_set_gravity.argtypes = [vp, C.POINTER(f)]
+    lib.xpbd_update.argtypes = [vp, f, i32, i32]
+    lib.xpbd_synchronize.argtypes = [vp]
+    lib.xpbd_body_read.argtypes = [vp, i32] + [vp] * 8
+    lib.xpbd_body_write.argtypes = [vp, i32, vp, vp]
+    lib.xpbd_body_state.argtypes = [vp, i32, vp]
+    lib.xpbd_profile_enable.argtypes = [vp, i32]
+    lib.xpbd_profile_read.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_double)]
+    for name in EXPORTED_SYMBOLS:
+        if name != "xpbd_last_error" and name != "xpbd_context_stream":
+            getattr(lib, name).restype = C.c_int
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != XPBD_OK:
+        raise XpbdError(f"xpbd error {rc}: {_lib.xpbd_last_error().decode()}")
+
+
+def _ptr(a):
+    if a is None or a.size == 0:
+        return None
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Settings:
+    def __init__(self, ctx, handle, arrays):
+        self.ctx = ctx
+        self.handle = handle
+        self.arrays = arrays
+
+    def stats(self):
+        out = (C.c_uint32 * 5)()
+        _check(self.ctx.lib.xpbd_settings_stats(self.handle, out))
+        return dict(parallel_groups=out[0], max_group_levels=out[1], tail_levels=out[2],
+                    tail_constraints=out[3], constraints=out[4])
+
+
+class Context:
+    """One device context: settings, bodies, colliders, one HIP stream."""
+
+    def __init__(self, device=0):
+        self.lib = load_library()
+        h = C.c_void_p()
+        _check(self.lib.xpbd_context_create(device, C.byref(h)))
+        self.h = h
+        self.sizes = []
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            self.lib.xpbd_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stream(self):
+        return self.lib.xpbd_context_stream(self.h)
+
+    def set_gravity(self, g):
+        arr = (C.c_float * 3)(*map(float, g))
+        _check(self.lib.xpbd_set_gravity(self.h, arr))
+
+    def set_colliders(self, colliders):
+        n = len(colliders)
+        arr = (_ColliderDesc * max(n, 1))()
+        for i, c in enumerate(colliders):
+            arr[i].type = int(c.type)
+            p = c.param_array()
+            for k in range(5):
+                arr[i].params[k] = float(p[k])
+            for k in range(3):
+                arr[i].position[k] = float(c.pos[k])
+            for k in range(4):
+                arr[i].rotation[k] = float(c.rot[k])
+            arr[i].friction = float(c.friction)
+            arr[i].restitution = float(c.restitution)
+            if c.bounds is None:
+                raise XpbdError("collider world bounds are required")
+            for k in range(6):
+                arr[i].world_bounds[k] = float(c.bounds[k])
+        _check(self.lib.xpbd_set_colliders(self.h, arr, n))
+
+    def create_settings(self, s: SettingsArrays) -> Settings:
+        v = np.ascontiguousarray(s.vertices, VERTEX)
+        f = s.faces
+        if f.dtype != FACE:
+            f4 = np.zeros(len(f), FACE)
+            f4["v"] = f
+            f = f4
+        f = np.ascontiguousarray(f, FACE)
+        e = np.ascontiguousarray(s.edges, EDGE)
+        b = np.ascontiguousarray(s.bends, BEND)
+        t = np.ascontiguousarray(s.volumes, VOLUME)
+        lr = np.ascontiguousarray(s.lras, LRA)
+        g = np.ascontiguousarray(s.groups, GROUP)
+        d = _SettingsDesc(_ptr(v), len(v), _ptr(f), len(f), _ptr(e), len(e), _ptr(b), len(b),
+                          _ptr(t), len(t), _ptr(lr), len(lr), _ptr(g), len(g), 0, 0, 0)
+        h = C.c_void_p()
+        _check(self.lib.xpbd_settings_create(self.h, C.byref(d), C.byref(h)))
+        st = Settings(self, h, s)
+        self._keep.append(st)
+        return st
+
+    def create_body(self, settings: Settings, position=(0, 0, 0), rotation=(0, 0, 0, 1),
+                    params: BodyParams | None = None) -> int:
+        p = params or BodyParams()
+        d = _BodyDesc()
+        d.settings = settings.handle
+        for k in range(3):
+            d.position[k] = float(position[k])
+        for k in range(4):
+            d.rotation[k] = float(rotation[k])
+        d.num_iterations = int(p.num_iterations)
+        d.linear_damping = p.linear_damping
+        d.max_linear_velocity = p.max_linear_velocity
+        d.restitution = p.restitution
+        d.friction = p.friction
+        d.pressure = p.pressure
+        d.gravity_factor = p.gravity_factor
+        d.vertex_radius = p.vertex_radius
+        d.update_position = int(p.update_position)
+        d.make_rotation_identity = int(p.make_rotation_identity)
+        d.allow_sleeping = int(p.allow_sleeping)
+        out = C.c_int32()
+        _check(self.lib.xpbd_body_create(self.h, C.byref(d), C.byref(out)))
+        self.sizes.append(len(settings.arrays.vertices))
+        return out.value
+
+    def update(self, dt, collision_steps=1, frames=1):
+        _check(self.lib.xpbd_update(self.h, float(dt), int(collision_steps), int(frames)))
+
+    def synchronize(self):
+        _check(self.lib.xpbd_synchronize(self.h))
+
+    def read(self, body):
+        n = self.sizes[body]
+        out = dict(pos=np.zeros((n, 3), np.float32), prev=np.zeros((n, 3), np.float32),
+                   vel=np.zeros((n, 3), np.float32), plane=np.zeros((n, 4), np.float32),
+                   shape_idx=np.zeros(n, np.int32), contact=np.zeros(n, np.int32),
+                   body_pos=np.zeros(3, np.float32), body_rot=np.zeros(4, np.float32))
+        _check(self.lib.xpbd_body_read(self.h, body, *(_ptr(out[k]) for k in (
+            "pos", "prev", "vel", "plane", "shape_idx", "contact", "body_pos", "body_rot"))))
+        return out
+
+    def write(self, body, pos=None, vel=None):
+        pos = None if pos is None else np.ascontiguousarray(pos, np.float32)
+        vel = None if vel is None else np.ascontiguousarray(vel, np.float32)
+        _check(self.lib.xpbd_body_write(self.h, body, _ptr(pos), _ptr(vel)))
+
+    def body_state(self, body):
+        out = np.zeros(14, np.float32)
+        _check(self.lib.xpbd_body_state(self.h, body, _ptr(out)))
+        return dict(linear_velocity=out[0:3], angular_velocity=out[3:6], bounds_min=out[6:9],
+                    bounds_max=out[9:12], active=bool(out[12]), sleep_timer=float(out[13]))
+
+    def profile_enable(self, on=True):
+        _check(self.lib.xpbd_profile_enable(self.h, int(on)))
+
+    def profile_read(self):
+        n = C.c_int32()
+        ms = C.c_double()
+        _check(self.lib.xpbd_profile_read(self.h, C.byref(n), C.byref(ms)))
+        return n.value, ms.value

@@ -1,0 +1,71 @@
+"""Shared helpers of the test-suite: fixture loading and scene runners."""
+from __future__ import annotations
+
+import os
+
+import numpy as np
+
+from joltphysics_amd.records import (BEND, EDGE, FACE, GROUP, LRA, VERTEX, VOLUME, SettingsArrays)
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load_config1_fixture():
+    fx = np.load(os.path.join(GOLDEN, "config1_cloth32_box.npz"))
+    s = SettingsArrays(fx["vertices"].astype(VERTEX), fx["faces"].astype(FACE), fx["edges"].astype(EDGE),
+                       fx["bends"].astype(BEND), fx["volumes"].astype(VOLUME), fx["lras"].astype(LRA),
+                       fx["groups"].astype(GROUP))
+    return fx, s
+
+
+def run_gpu(scene, settings_list, bounds=None, frames_to_keep=(1,), device=0):
+    """Runs a scene on the GPU through the C ABI; returns {frame: [read(body)...]}."""
+    from joltphysics_amd.xpbd import Context
+    ctx = Context(device)
+    ctx.set_gravity(scene.gravity)
+    cols = []
+    for i, c in enumerate(scene.colliders):
+        if bounds is not None:
+            c.bounds = tuple(float(x) for x in bounds[i])
+        cols.append(c)
+    ctx.set_colliders(cols)
+    ids = []
+    for b, s in zip(scene.bodies, settings_list):
+        st = ctx.create_settings(s)
+        ids.append(ctx.create_body(st, b.position, b.rotation, b.params))
+    out = {}
+    done = 0
+    for f in sorted(frames_to_keep):
+        ctx.update(scene.dt, scene.collision_steps, f - done)
+        done = f
+        out[f] = [ctx.read(i) for i in ids]
+    ctx.close()
+    return out
+
+
+def run_restatement(scene, settings_list, bounds, frames_to_keep=(1,), precision=32):
+    from oracle import Restatement
+    r = Restatement(scene.gravity, precision)
+    for c, b in zip(scene.colliders, bounds):
+        r.add_collider(c, b)
+    for b, s in zip(scene.bodies, settings_list):
+        r.add_soft_body(s, b.position, b.rotation, b.params)
+    out = {}
+    done = 0
+    for f in sorted(frames_to_keep):
+        r.update(scene.dt, scene.collision_steps, f - done)
+        done = f
+        out[f] = [r.read(i) for i in range(len(scene.bodies))]
+    r.close()
+    return out
+
+
+def max_abs(a, b):
+    return float(np.max(np.abs(np.asarray(a, np.float64) - np.asarray(b, np.float64)))) if np.size(a) else 0.0
+
+
+def float_equal(a, b):
+    """Equality of float arrays (+0 == -0, NaN never equal)."""
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    return a.shape == b.shape and bool(np.all(a == b))
