@@ -83,8 +83,10 @@ typedef struct xpbd_settings_desc {
 int xpbd_settings_create(xpbd_context *ctx, const xpbd_settings_desc *desc, xpbd_settings **out_settings);
 int xpbd_settings_release(xpbd_settings *settings);
 /* Schedule statistics: out[0]=parallel groups, out[1]=max levels of a parallel
- * group, out[2]=tail levels, out[3]=tail constraints, out[4]=total constraints. */
-int xpbd_settings_stats(const xpbd_settings *settings, uint32_t out[5]);
+ * group, out[2]=levels of the non-parallel group, out[3]=constraints of the
+ * non-parallel group, out[4]=total constraints, out[5]=1 if the body keeps all
+ * positions in LDS, out[6]/out[7]=parallel / non-parallel windows. */
+int xpbd_settings_stats(const xpbd_settings *settings, uint32_t out[8]);
 
 /*
  * Soft body: replaces BodyInterface::CreateSoftBody + SoftBodyMotionProperties::

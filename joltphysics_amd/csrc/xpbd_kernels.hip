@@ -1,21 +1,29 @@
 // Soft-body XPBD update kernels for gfx950 (MI355X).
 //
 // One workgroup owns one soft body for a whole xpbd_update call (all frames,
-// collision steps and iterations): the body's positions and inverse masses
-// live in LDS as float4 for the entire launch ("LDS-resident" bodies).
+// collision steps and iterations).
 //
-// Constraint projection follows Jolt's update groups
-// (SoftBodyMotionProperties.cpp:1030-1115): the parallel groups are
-// vertex-disjoint and each runs on one wavefront; the last (non-parallel)
-// group runs after them on the whole workgroup.  Inside a group the reference
+// Constraint projection follows the reference's update groups
+// (SoftBodyMotionProperties.cpp:1030-1115).  Inside a group the reference
 // applies its constraints sequentially (volume, bend, edge, LRA order); the
-// host converts that sequence into levels in which no two constraints touch
-// the same vertex, ordered so that every read-after-write / write-after-read
-// pair keeps its sequential order.  Running a level lane-parallel therefore
-// gives bit-identical results to the sequential loop.
+// host turns that sequence into levels in which no two constraints touch the
+// same vertex, ordered so that every read-after-write / write-after-read pair
+// keeps its sequential order.  The parallel groups are vertex-disjoint, so
+// their levels are merged (level L of every group runs together) and the
+// non-parallel group runs after them.  A level runs lane-parallel across the
+// workgroup with a barrier after it, which gives bit-identical results to the
+// reference's sequential loops.
 //
-// All arithmetic follows the reference's operation order (see jolt_math.h);
-// the file is compiled with -ffp-contract=off and IEEE division / sqrt.
+// Two storage modes, chosen per body by the host:
+//   RESIDENT  positions + inverse masses stay in LDS (float4) for the whole
+//             launch; the level windows index body vertices directly;
+//   streamed  positions live in global memory; each window of levels gathers
+//             its vertex subset into LDS, runs, and scatters it back.
+// STAGED additionally copies the constraint records and level offsets into LDS
+// once per launch (they are reused every iteration of every step).
+//
+// All arithmetic follows the reference's operation order (jolt_math.h); the
+// file is compiled with -ffp-contract=off and IEEE division / sqrt.
 
 #include <cfloat>
 
@@ -36,47 +44,40 @@ struct Leaf
     M44 inv;        // InversedRotationTranslation of t (box, capsule)
     float pn[3];    // plane shape: transformed plane normal
     float pc;       //              and constant
-    float a, b, c;  // sphere {radius, center from t} | box half extent | capsule {half height, radius}
+    float a, b, c;  // sphere {radius} | box half extent | capsule {half height, radius}
     int32_t type;
     float friction, restitution;
 };
 
-// LDS layout of the resident kernel (dynamic shared memory):
-//   float4 X[num_vertices]
+// Dynamic LDS layout:
+//   float4 X[xcount]            positions (resident: all vertices, streamed: window)
+//   uint4  recs[nrec]           staged records
+//   uint32 lvl[nlvl]            staged level offsets
 //   Leaf   leaves[max_leaves]
 //   int    collider_slot[num_colliders]
-//   float4 scratch[num_faces]       (pressure only)
-//   float  red[kWaves * 16]         (reductions)
+//   float4 scratch[num_faces]   (pressure)
+//   float  red[kWaves * 32]     (reductions)
 struct LdsLayout
 {
-    uint32_t leaves, slots, scratch, red, total;
+    uint32_t recs, lvl, leaves, slots, scratch, red, total;
 };
 
-__host__ __device__ inline LdsLayout lds_layout(uint32_t nv, uint32_t max_leaves, uint32_t nc, uint32_t nf)
+__host__ __device__ inline LdsLayout lds_layout(uint32_t xcount, uint32_t nrec, uint32_t nlvl, uint32_t max_leaves, uint32_t nc, uint32_t nf)
 {
     LdsLayout l;
-    uint32_t off = nv * 16u;
+    uint32_t off = xcount * 16u;
+    l.recs = off; off += nrec * 16u;
+    l.lvl = off; off += nlvl * 4u;
     off = (off + 15u) & ~15u; l.leaves = off; off += max_leaves * (uint32_t)sizeof(Leaf);
     off = (off + 15u) & ~15u; l.slots = off; off += nc * 4u;
     off = (off + 15u) & ~15u; l.scratch = off; off += nf * 16u;
-    off = (off + 15u) & ~15u; l.red = off; off += kWaves * 16u * 4u;
+    off = (off + 15u) & ~15u; l.red = off; off += kWaves * 32u * 4u;
     l.total = (off + 15u) & ~15u;
     return l;
 }
 
 __device__ __forceinline__ V3 xyz(float4 p) { return v3(p.x, p.y, p.z); }
 __device__ __forceinline__ float4 f4(V3 p, float w) { return make_float4(p.x, p.y, p.z, w); }
-
-// Orders the LDS accesses of one wavefront around a level boundary.  LDS
-// instructions of a wavefront execute in issue order, so a later read by any
-// lane observes an earlier write; the fence keeps the compiler from moving
-// memory operations across the boundary.
-__device__ __forceinline__ void wave_level_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // ---------------------------------------------------------------- constraints
 // ApplyEdgeConstraints body (SoftBodyMotionProperties.cpp:583-600)
@@ -186,35 +187,48 @@ __device__ __forceinline__ void project_record(float4 *X, uint4 r, float inv_dt_
     }
 }
 
-// One update group on one wavefront, level by level.
-__device__ __forceinline__ void run_group_wave(const SettingsDev &S, uint32_t g, float4 *X, float inv_dt_sq, int lane)
+// Runs `nlev` levels on the window X.  Records of one level touch disjoint
+// vertices, so the workgroup projects a level lane-parallel and a barrier
+// separates consecutive levels.
+//
+// STAGED: records and level offsets are in LDS.  Otherwise they stream from
+// global memory and the records of level L+1 are prefetched into registers
+// while level L runs.
+template <bool STAGED>
+__device__ __forceinline__ void run_levels(const uint4 *recs, const uint32_t *lvl, uint32_t nlev, float4 *X, float inv_dt_sq, int tid)
 {
-    uint2 gi = S.groups[g];
-    const uint32_t *lvl = S.lvl + gi.x;
-    uint32_t begin = lvl[0];
-    for (uint32_t L = 0; L < gi.y; ++L)
+    if (nlev == 0)
+        return;
+    if (STAGED)
     {
-        uint32_t end = lvl[L + 1];
-        for (uint32_t i = begin + lane; i < end; i += kWave)
-            project_record(X, S.recs[i], inv_dt_sq);
-        wave_level_sync();
-        begin = end;
+        uint32_t begin = lvl[0];
+        for (uint32_t L = 0; L < nlev; ++L)
+        {
+            uint32_t end = lvl[L + 1];
+            for (uint32_t i = begin + tid; i < end; i += kBlock)
+                project_record(X, recs[i], inv_dt_sq);
+            __syncthreads();
+            begin = end;
+        }
+        return;
     }
-}
-
-// The non-parallel group on the whole workgroup (a barrier per level).
-__device__ __forceinline__ void run_group_block(const SettingsDev &S, uint32_t g, float4 *X, float inv_dt_sq, int tid)
-{
-    uint2 gi = S.groups[g];
-    const uint32_t *lvl = S.lvl + gi.x;
-    uint32_t begin = lvl[0];
-    for (uint32_t L = 0; L < gi.y; ++L)
+    const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
+    uint32_t b = lvl[0], e = lvl[1];
+    uint4 r0 = b + tid < e ? recs[b + tid] : zero;
+    uint4 r1 = b + tid + kBlock < e ? recs[b + tid + kBlock] : zero;
+    for (uint32_t L = 0; L < nlev; ++L)
     {
-        uint32_t end = lvl[L + 1];
-        for (uint32_t i = begin + tid; i < end; i += kBlock)
-            project_record(X, S.recs[i], inv_dt_sq);
+        uint32_t nb = e, ne = L + 1 < nlev ? lvl[L + 2] : e;
+        uint4 n0 = nb + tid < ne ? recs[nb + tid] : zero;
+        uint4 n1 = nb + tid + kBlock < ne ? recs[nb + tid + kBlock] : zero;
+        if (b + tid < e)
+            project_record(X, r0, inv_dt_sq);
+        if (b + tid + kBlock < e)
+            project_record(X, r1, inv_dt_sq);
+        for (uint32_t i = b + tid + 2 * kBlock; i < e; i += kBlock)
+            project_record(X, recs[i], inv_dt_sq);
         __syncthreads();
-        begin = end;
+        r0 = n0; r1 = n1; b = nb; e = ne;
     }
 }
 
@@ -388,7 +402,8 @@ __device__ __forceinline__ float wave_sum(float v)
 
 // ---------------------------------------------------------------- kernel
 // Advances each body of `body_list` by `frames` x `steps` collision steps.
-__global__ __launch_bounds__(kBlock) void resident_update_kernel(
+template <bool RESIDENT, bool STAGED>
+__global__ __launch_bounds__(kBlock) void update_kernel(
     const BodyDev *__restrict__ bodies, BodyStateDev *__restrict__ states, const uint32_t *__restrict__ body_list,
     const ColliderDev *__restrict__ colliders, uint32_t num_colliders, uint32_t max_leaves,
     float gx, float gy, float gz, float dt, int32_t steps, int32_t frames, int32_t *__restrict__ error_flag)
@@ -406,23 +421,40 @@ __global__ __launch_bounds__(kBlock) void resident_update_kernel(
     const SettingsDev &S = B.s;
     const uint32_t nv = S.num_vertices;
     const bool has_pressure = B.pressure > 0.0f && S.num_faces > 0;
-    const LdsLayout lay = lds_layout(nv, max_leaves, num_colliders, has_pressure ? S.num_faces : 0u);
+    const LdsLayout lay = lds_layout(RESIDENT ? nv : S.window_capacity, STAGED ? S.num_records : 0u,
+                                     STAGED ? S.num_levels_total : 0u, max_leaves, num_colliders,
+                                     has_pressure ? S.num_faces : 0u);
     float4 *X = reinterpret_cast<float4 *>(lds_raw);
+    const uint4 *recs = STAGED ? reinterpret_cast<const uint4 *>(lds_raw + lay.recs) : S.recs;
+    const uint32_t *lvl = STAGED ? reinterpret_cast<const uint32_t *>(lds_raw + lay.lvl) : S.lvl;
     Leaf *leaves = reinterpret_cast<Leaf *>(lds_raw + lay.leaves);
     int32_t *slots = reinterpret_cast<int32_t *>(lds_raw + lay.slots);
     float4 *scratch = reinterpret_cast<float4 *>(lds_raw + lay.scratch);
     float *red = reinterpret_cast<float *>(lds_raw + lay.red);
+    // Per-vertex phases address positions here
+    float4 *P = RESIDENT ? X : B.x;
 
     if (tid == 0)
         st = states[bi];
-    for (uint32_t i = tid; i < nv; i += kBlock)
-        X[i] = B.x[i];
+    if (STAGED)
+    {
+        uint4 *rd = reinterpret_cast<uint4 *>(lds_raw + lay.recs);
+        for (uint32_t i = tid; i < S.num_records; i += kBlock)
+            rd[i] = S.recs[i];
+        uint32_t *ld = reinterpret_cast<uint32_t *>(lds_raw + lay.lvl);
+        for (uint32_t i = tid; i < S.num_levels_total; i += kBlock)
+            ld[i] = S.lvl[i];
+    }
+    if (RESIDENT)
+        for (uint32_t i = tid; i < nv; i += kBlock)
+            X[i] = B.x[i];
     __syncthreads();
 
     const V3 gravity_world = v3(gx, gy, gz);
     // PhysicsSystem::Update: step delta time (PhysicsSystem.cpp:215)
     const float step_dt = dt / (float)steps;
     const uint32_t iterations = B.iterations;
+    const uint32_t num_windows = S.num_par_windows + S.num_tail_windows;
 
     for (int32_t frame = 0; frame < frames; ++frame)
         for (int32_t step = 0; step < steps; ++step)
@@ -494,28 +526,31 @@ __global__ __launch_bounds__(kBlock) void resident_update_kernel(
             const uint32_t num_leaves = sh_num_leaves;
 
             // DetermineCollisionPlanes (SoftBodyMotionProperties.cpp:257-265)
-            for (uint32_t i = tid; i < nv; i += kBlock)
-            {
-                float4 p = X[i];
-                if (p.w > 0.0f)
+            if (num_leaves > 0)
+                for (uint32_t i = tid; i < nv; i += kBlock)
                 {
-                    float4 pl = B.plane[i];
-                    Pl plane; plane.n = v3(pl.x, pl.y, pl.z); plane.c = pl.w;
-                    float largest = -FLT_MAX;
-                    int32_t idx = -1;
-                    collide_vertex(leaves, num_leaves, xyz(p), largest, idx, plane);
-                    B.idx[i] = idx;
-                    if (idx >= 0)
-                        B.plane[i] = make_float4(plane.n.x, plane.n.y, plane.n.z, plane.c);
+                    float4 p = P[i];
+                    if (p.w > 0.0f)
+                    {
+                        float4 pl = B.plane[i];
+                        Pl plane; plane.n = v3(pl.x, pl.y, pl.z); plane.c = pl.w;
+                        float largest = -FLT_MAX;
+                        int32_t idx = -1;
+                        collide_vertex(leaves, num_leaves, xyz(p), largest, idx, plane);
+                        if (idx >= 0)
+                        {
+                            B.idx[i] = idx;
+                            B.plane[i] = make_float4(plane.n.x, plane.n.y, plane.n.z, plane.c);
+                        }
+                    }
                 }
-            }
 
             // ApplyPressure (SoftBodyMotionProperties.cpp:291-323)
             auto apply_pressure = [&]() {
                 const uint32_t nf = S.num_faces;
                 for (uint32_t f = tid; f < nf; f += kBlock)
                 {
-                    V3 x1 = xyz(X[S.faces[3 * f]]), x2 = xyz(X[S.faces[3 * f + 1]]), x3 = xyz(X[S.faces[3 * f + 2]]);
+                    V3 x1 = xyz(P[S.faces[3 * f]]), x2 = xyz(P[S.faces[3 * f + 1]]), x3 = xyz(P[S.faces[3 * f + 2]]);
                     scratch[f] = make_float4(0.0f, 0.0f, 0.0f, dot(cross(x1, x2), x3));
                 }
                 __syncthreads();
@@ -534,7 +569,7 @@ __global__ __launch_bounds__(kBlock) void resident_update_kernel(
                     const float coefficient = B.pressure * dt_sub / six_volume;
                     for (uint32_t f = tid; f < nf; f += kBlock)
                     {
-                        V3 x1 = xyz(X[S.faces[3 * f]]), x2 = xyz(X[S.faces[3 * f + 1]]), x3 = xyz(X[S.faces[3 * f + 2]]);
+                        V3 x1 = xyz(P[S.faces[3 * f]]), x2 = xyz(P[S.faces[3 * f + 1]]), x3 = xyz(P[S.faces[3 * f + 2]]);
                         V3 impulse = coefficient * cross(x2 - x1, x3 - x1);
                         scratch[f] = make_float4(impulse.x, impulse.y, impulse.z, 0.0f);
                     }
@@ -545,15 +580,11 @@ __global__ __launch_bounds__(kBlock) void resident_update_kernel(
                         uint32_t fb = S.vf_begin[i], fe = S.vf_begin[i + 1];
                         if (fb == fe)
                             continue;
-                        float w = X[i].w;
-                        float4 vv = B.v[i];
-                        V3 v = xyz(vv);
+                        float w = P[i].w;
+                        V3 v = xyz(B.v[i]);
                         for (uint32_t k = fb; k < fe; ++k)
-                        {
-                            float4 imp = scratch[S.vf_list[k]];
-                            v = v + w * xyz(imp);
-                        }
-                        B.v[i] = f4(v, vv.w);
+                            v = v + w * xyz(scratch[S.vf_list[k]]);
+                        B.v[i] = f4(v, 0.0f);
                     }
                 }
                 __syncthreads();
@@ -571,14 +602,14 @@ __global__ __launch_bounds__(kBlock) void resident_update_kernel(
                 }
                 V3 position = xyz(p);
                 B.prev[i] = f4(position, 0.0f);
-                X[i] = f4(position + v * dt_sub, p.w);
+                P[i] = f4(position + v * dt_sub, p.w);
                 B.v[i] = f4(v, 0.0f);
             };
 
             if (has_pressure)
                 apply_pressure();
             for (uint32_t i = tid; i < nv; i += kBlock)
-                integrate_vertex(i, X[i], xyz(B.v[i]));
+                integrate_vertex(i, P[i], xyz(B.v[i]));
             __syncthreads();
 
             const float restitution_threshold = -2.0f * len(g_local) * dt_sub;
@@ -592,27 +623,31 @@ __global__ __launch_bounds__(kBlock) void resident_update_kernel(
             {
                 const bool last = it + 1 == iterations;
 
-                // Parallel groups: group g on wave g mod kWaves
-                for (uint32_t g = wave; g < S.num_parallel; g += kWaves)
-                    run_group_wave(S, g, X, inv_dt_sq, lane);
-                __syncthreads();
-
-                // Non-parallel group
-                if (S.tail_one_wave)
+                // Parallel-group windows, then the non-parallel group's windows
+                for (uint32_t w = 0; w < num_windows; ++w)
                 {
-                    if (wave == 0)
-                        run_group_wave(S, S.num_parallel, X, inv_dt_sq, lane);
+                    const WindowDev W = S.windows[w];
+                    if (!RESIDENT && W.vcount)
+                    {
+                        for (uint32_t j = tid; j < W.vcount; j += kBlock)
+                            X[j] = B.x[S.vlist[W.vbegin + j]];
+                        __syncthreads();
+                    }
+                    run_levels<STAGED>(recs, lvl + W.lbegin, W.nlev, X, inv_dt_sq, tid);
+                    if (!RESIDENT && W.vcount)
+                    {
+                        for (uint32_t j = tid; j < W.vcount; j += kBlock)
+                            B.x[S.vlist[W.vbegin + j]] = X[j];
+                        __syncthreads();
+                    }
                 }
-                else
-                    run_group_block(S, S.num_parallel, X, inv_dt_sq, tid);
-                __syncthreads();
 
                 // ApplyCollisionConstraintsAndUpdateVelocities (SoftBodyMotionProperties.cpp:700-825),
                 // static-body branch; fused with the next iteration's integration when there is
                 // no pressure, or with UpdateSoftBodyState's per-vertex work after the last one.
                 for (uint32_t i = tid; i < nv; i += kBlock)
                 {
-                    float4 p = X[i];
+                    float4 p = P[i];
                     V3 x = xyz(p);
                     V3 v = xyz(B.v[i]);
                     if (p.w > 0.0f)
@@ -650,7 +685,7 @@ __global__ __launch_bounds__(kBlock) void resident_update_kernel(
                     {
                         if (has_pressure)
                         {
-                            X[i] = f4(x, p.w);
+                            P[i] = f4(x, p.w);
                             B.v[i] = f4(v, 0.0f);
                         }
                         else
@@ -670,7 +705,7 @@ __global__ __launch_bounds__(kBlock) void resident_update_kernel(
                         V3 pp = (x + v * step_dt) + disp_gravity;
                         pmin = vmin(pmin, pp);
                         pmax = vmax(pmax, pp);
-                        X[i] = f4(x, p.w);
+                        P[i] = f4(x, p.w);
                         B.v[i] = f4(v, 0.0f);
                         B.idx[i] = -1;
                         B.contact[i] = 0;
@@ -681,42 +716,44 @@ __global__ __launch_bounds__(kBlock) void resident_update_kernel(
                 {
                     apply_pressure();
                     for (uint32_t i = tid; i < nv; i += kBlock)
-                        integrate_vertex(i, X[i], xyz(B.v[i]));
+                        integrate_vertex(i, P[i], xyz(B.v[i]));
                     __syncthreads();
                 }
             }
 
             // UpdateSoftBodyState reductions (SoftBodyMotionProperties.cpp:846-921)
             {
-                float vals[16] = { bmin.x, bmin.y, bmin.z, pmin.x, pmin.y, pmin.z,
-                                   bmax.x, bmax.y, bmax.z, pmax.x, pmax.y, pmax.z, max_v_sq, 0, 0, 0 };
+                float vals[13] = { bmin.x, bmin.y, bmin.z, pmin.x, pmin.y, pmin.z,
+                                   bmax.x, bmax.y, bmax.z, pmax.x, pmax.y, pmax.z, max_v_sq };
+#pragma unroll
                 for (int k = 0; k < 6; ++k) vals[k] = wave_min(vals[k]);
+#pragma unroll
                 for (int k = 6; k < 13; ++k) vals[k] = wave_max(vals[k]);
+                float sums[6] = { wave_sum(lin.x), wave_sum(lin.y), wave_sum(lin.z), wave_sum(ang.x), wave_sum(ang.y), wave_sum(ang.z) };
                 if (lane == 0)
-                    for (int k = 0; k < 13; ++k) red[wave * 16 + k] = vals[k];
+                {
+#pragma unroll
+                    for (int k = 0; k < 13; ++k) red[wave * 32 + k] = vals[k];
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) red[wave * 32 + 16 + k] = sums[k];
+                }
                 __syncthreads();
                 if (tid == 0)
                 {
                     for (int w = 1; w < kWaves; ++w)
                     {
-                        for (int k = 0; k < 6; ++k) vals[k] = smin(vals[k], red[w * 16 + k]);
-                        for (int k = 6; k < 13; ++k) vals[k] = smax(vals[k], red[w * 16 + k]);
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) vals[k] = smin(vals[k], red[w * 32 + k]);
+#pragma unroll
+                        for (int k = 6; k < 13; ++k) vals[k] = smax(vals[k], red[w * 32 + k]);
+#pragma unroll
+                        for (int k = 0; k < 6; ++k) sums[k] += red[w * 32 + 16 + k];
                     }
-                }
-                __syncthreads();
-                // body velocity sums (order differs from the reference's sequential sum)
-                float sums[6] = { wave_sum(lin.x), wave_sum(lin.y), wave_sum(lin.z), wave_sum(ang.x), wave_sum(ang.y), wave_sum(ang.z) };
-                if (lane == 0)
-                    for (int k = 0; k < 6; ++k) red[wave * 16 + k] = sums[k];
-                __syncthreads();
-                if (tid == 0)
-                {
-                    for (int w = 1; w < kWaves; ++w)
-                        for (int k = 0; k < 6; ++k) sums[k] += red[w * 16 + k];
                     V3 bmn = v3(vals[0], vals[1], vals[2]), pmn = v3(vals[3], vals[4], vals[5]);
                     V3 bmx = v3(vals[6], vals[7], vals[8]), pmx = v3(vals[9], vals[10], vals[11]);
                     float mvsq = vals[12];
                     float divider = (float)(nv > 1 ? nv : 1u);
+                    // body velocity (sums in tree order, the reference sums sequentially)
                     V3 lv = mul3x3(com, v3(sums[0], sums[1], sums[2]) / divider);
                     V3 av = mul3x3(com, v3(sums[3], sums[4], sums[5]) / divider);
                     st.lin[0] = lv.x; st.lin[1] = lv.y; st.lin[2] = lv.z;
@@ -738,7 +775,7 @@ __global__ __launch_bounds__(kBlock) void resident_update_kernel(
                     sh_scalars[1] = delta.x; sh_scalars[2] = delta.y; sh_scalars[3] = delta.z;
                     if (B.allow_sleeping)
                     {
-                        // MotionProperties sleep test (SoftBodyMotionProperties.cpp:903-914, MotionProperties.inl:169-173)
+                        // sleep test (SoftBodyMotionProperties.cpp:903-914, MotionProperties.inl:169-173)
                         if (mvsq > 0.03f)
                             st.sleep_timer = 0.0f;
                         else
@@ -755,16 +792,17 @@ __global__ __launch_bounds__(kBlock) void resident_update_kernel(
                     V3 delta = v3(sh_scalars[1], sh_scalars[2], sh_scalars[3]);
                     for (uint32_t i = tid; i < nv; i += kBlock)
                     {
-                        float4 p = X[i];
-                        X[i] = f4(xyz(p) - delta, p.w);
+                        float4 p = P[i];
+                        P[i] = f4(xyz(p) - delta, p.w);
                     }
                 }
                 __syncthreads();
             }
         }
 
-    for (uint32_t i = tid; i < nv; i += kBlock)
-        B.x[i] = X[i];
+    if (RESIDENT)
+        for (uint32_t i = tid; i < nv; i += kBlock)
+            B.x[i] = X[i];
     if (tid == 0)
         states[bi] = st;
 }
@@ -774,31 +812,43 @@ __global__ __launch_bounds__(kBlock) void resident_update_kernel(
 // ---------------------------------------------------------------- launchers
 namespace xpbd {
 
-uint32_t resident_lds_bytes(uint32_t nv, uint32_t max_leaves, uint32_t nc, uint32_t nf)
+uint32_t kernel_lds_bytes(bool resident, bool staged, const SettingsDev &s, uint32_t max_leaves, uint32_t nc, uint32_t nf)
 {
-    return lds_layout(nv, max_leaves, nc, nf).total;
+    return lds_layout(resident ? s.num_vertices : s.window_capacity, staged ? s.num_records : 0u,
+                      staged ? s.num_levels_total : 0u, max_leaves, nc, nf).total;
 }
 
-uint32_t leaf_bytes() { return (uint32_t)sizeof(Leaf); }
-
-hipError_t launch_resident(const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list, uint32_t num_blocks,
-                           const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves, uint32_t lds_bytes,
-                           float gx, float gy, float gz, float dt, int32_t steps, int32_t frames, int32_t *error_flag,
-                           hipStream_t stream)
+template <bool R, bool S>
+static hipError_t launch_variant(const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list, uint32_t num_blocks,
+                                 const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves, uint32_t lds_bytes,
+                                 float gx, float gy, float gz, float dt, int32_t steps, int32_t frames, int32_t *error_flag,
+                                 hipStream_t stream)
 {
-    if (num_blocks == 0)
-        return hipSuccess;
     static bool configured = false;
     if (!configured)
     {
-        hipError_t e = hipFuncSetAttribute((const void *)resident_update_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+        hipError_t e = hipFuncSetAttribute((const void *)update_kernel<R, S>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
         if (e != hipSuccess)
             return e;
         configured = true;
     }
-    hipLaunchKernelGGL(resident_update_kernel, dim3(num_blocks), dim3(kBlock), lds_bytes, stream,
+    hipLaunchKernelGGL((update_kernel<R, S>), dim3(num_blocks), dim3(kBlock), lds_bytes, stream,
                        bodies, states, body_list, colliders, num_colliders, max_leaves, gx, gy, gz, dt, steps, frames, error_flag);
     return hipGetLastError();
+}
+
+hipError_t launch_update(bool resident, bool staged, const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list,
+                         uint32_t num_blocks, const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves,
+                         uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
+                         int32_t *error_flag, hipStream_t stream)
+{
+    if (num_blocks == 0)
+        return hipSuccess;
+    if (resident && staged)
+        return launch_variant<true, true>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
+    if (resident)
+        return launch_variant<true, false>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
+    return launch_variant<false, false>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
 }
 
 } // namespace xpbd

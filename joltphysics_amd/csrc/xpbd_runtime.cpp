@@ -16,12 +16,11 @@
 #include "xpbd_types.h"
 
 namespace xpbd {
-uint32_t resident_lds_bytes(uint32_t nv, uint32_t max_leaves, uint32_t nc, uint32_t nf);
-uint32_t leaf_bytes();
-hipError_t launch_resident(const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list, uint32_t num_blocks,
-                           const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves, uint32_t lds_bytes,
-                           float gx, float gy, float gz, float dt, int32_t steps, int32_t frames, int32_t *error_flag,
-                           hipStream_t stream);
+uint32_t kernel_lds_bytes(bool resident, bool staged, const SettingsDev &s, uint32_t max_leaves, uint32_t nc, uint32_t nf);
+hipError_t launch_update(bool resident, bool staged, const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list,
+                         uint32_t num_blocks, const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves,
+                         uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
+                         int32_t *error_flag, hipStream_t stream);
 } // namespace xpbd
 
 using namespace xpbd;
@@ -90,11 +89,11 @@ struct xpbd_settings
     std::vector<VertexRec> vertices;        // host copy for body creation
     DevBuf<uint4> recs;
     DevBuf<uint32_t> lvl;
-    DevBuf<uint2> groups;
+    DevBuf<WindowDev> windows;
+    DevBuf<uint32_t> vlist;
     DevBuf<uint32_t> faces, vf_begin, vf_list;
-    uint32_t num_parallel = 0;
-    uint32_t tail_one_wave = 0;
-    uint32_t stats[5] = { 0, 0, 0, 0, 0 };
+    SettingsDev dev{};
+    uint32_t stats[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
 };
 
 struct Body
@@ -125,6 +124,8 @@ struct xpbd_context
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
     double profiled_ms = 0.0;
     int32_t profiled_launches = 0;
+    std::vector<uint32_t> host_list;   // sources of the asynchronous table uploads
+    std::vector<BodyDev> host_devs;
 };
 
 namespace {
@@ -136,25 +137,29 @@ namespace {
 //     level(c) = 1 + max( last_write[r] for r in R,
 //                         max(last_write[w], last_read[w]) for w in W )
 // so every pair that shares a vertex with at least one writer keeps its
-// order, and constraints of one level touch disjoint write sets.
+// order, and the constraints of one level have disjoint write sets that no
+// other constraint of the level reads.
 struct Item { uint32_t type; uint32_t v[4]; uint32_t nv; float p0, p1; };
 
-void LevelGroup(const std::vector<Item> &seq, std::vector<uint32_t> &last_write, std::vector<uint32_t> &last_read,
-                std::vector<uint4> &recs, std::vector<uint32_t> &lvl, uint32_t &num_levels, uint32_t &max_width)
+// Bodies up to this many vertices keep all positions in LDS for the launch.
+constexpr uint32_t kResidentMaxVertices = 8192;
+// Vertex capacity of one gathered window of a streamed body.
+constexpr uint32_t kWindowCapacity = 4096;
+
+std::vector<std::vector<uint32_t>> LevelGroup(const std::vector<Item> &seq, std::vector<uint32_t> &last_write, std::vector<uint32_t> &last_read)
 {
     std::vector<uint32_t> level(seq.size());
     uint32_t nl = 0;
     for (size_t k = 0; k < seq.size(); ++k)
     {
         const Item &it = seq[k];
+        const bool lra = it.type == kTypeLRA;
         uint32_t L = 0;
-        bool lra = it.type == kTypeLRA;
         for (uint32_t j = 0; j < it.nv; ++j)
         {
             uint32_t v = it.v[j];
             L = std::max(L, last_write[v]);
-            bool writes = !lra || j == 1;
-            if (writes)
+            if (!lra || j == 1)
                 L = std::max(L, last_read[v]);
         }
         ++L;
@@ -162,45 +167,68 @@ void LevelGroup(const std::vector<Item> &seq, std::vector<uint32_t> &last_write,
         for (uint32_t j = 0; j < it.nv; ++j)
         {
             uint32_t v = it.v[j];
-            bool writes = !lra || j == 1;
-            if (writes)
+            if (!lra || j == 1)
                 last_write[v] = L;
             last_read[v] = std::max(last_read[v], L);
         }
         nl = std::max(nl, L);
     }
-    // counting sort by level (stable)
-    std::vector<uint32_t> count(nl + 2, 0);
-    for (uint32_t L : level) ++count[L];
-    std::vector<uint32_t> start(nl + 2, 0);
-    for (uint32_t L = 1; L <= nl; ++L) start[L + 1] = start[L] + count[L];
-    uint32_t base = (uint32_t)recs.size();
-    recs.resize(base + seq.size());
-    max_width = 0;
-    for (uint32_t L = 1; L <= nl; ++L)
-    {
-        lvl.push_back(base + start[L]);
-        max_width = std::max(max_width, count[L]);
-    }
-    lvl.push_back(base + (uint32_t)seq.size());
-    std::vector<uint32_t> fill(start.begin(), start.end());
+    std::vector<std::vector<uint32_t>> out(nl);
     for (size_t k = 0; k < seq.size(); ++k)
-    {
-        const Item &it = seq[k];
-        uint4 r;
-        r.x = (it.v[0] & 0x3fffu) | (it.type << 14) | ((it.nv > 1 ? it.v[1] : 0u) << 16);
-        r.y = (it.nv > 2 ? it.v[2] : 0u) | ((it.nv > 3 ? it.v[3] : 0u) << 16);
-        float p0 = it.p0, p1 = it.p1;
-        memcpy(&r.z, &p0, 4);
-        memcpy(&r.w, &p1, 4);
-        recs[base + fill[level[k]]++] = r;
-    }
-    num_levels = nl;
-    // reset the per-vertex trackers touched by this group
+        out[level[k] - 1].push_back((uint32_t)k);
     for (const Item &it : seq)
         for (uint32_t j = 0; j < it.nv; ++j)
             last_write[it.v[j]] = last_read[it.v[j]] = 0;
+    return out;
 }
+
+uint4 Pack(const Item &it, const uint32_t *local)
+{
+    auto m = [&](uint32_t j) { return j < it.nv ? (local ? local[it.v[j]] : it.v[j]) : 0u; };
+    uint4 r;
+    r.x = (m(0) & 0x3fffu) | (it.type << 14) | (m(1) << 16);
+    r.y = m(2) | (m(3) << 16);
+    float p0 = it.p0, p1 = it.p1;
+    memcpy(&r.z, &p0, 4);
+    memcpy(&r.w, &p1, 4);
+    return r;
+}
+
+struct GroupLevels
+{
+    std::vector<Item> seq;
+    std::vector<std::vector<uint32_t>> levels;
+};
+
+struct ProgramBuilder
+{
+    std::vector<uint4> recs;
+    std::vector<uint32_t> lvl;
+    std::vector<WindowDev> windows;
+    std::vector<uint32_t> vlist;
+
+    // Appends one window: levels[L] = list of (group, item) pairs; local maps
+    // body vertex -> window slot (null: identity window).
+    void AddWindow(const std::vector<std::vector<std::pair<const GroupLevels *, uint32_t>>> &levels, const uint32_t *local,
+                   const std::vector<uint32_t> &verts)
+    {
+        WindowDev w;
+        w.vbegin = (uint32_t)vlist.size();
+        w.vcount = local ? (uint32_t)verts.size() : 0u;
+        if (local)
+            vlist.insert(vlist.end(), verts.begin(), verts.end());
+        w.lbegin = (uint32_t)lvl.size();
+        w.nlev = (uint32_t)levels.size();
+        for (const auto &lv : levels)
+        {
+            lvl.push_back((uint32_t)recs.size());
+            for (const auto &gi : lv)
+                recs.push_back(Pack(gi.first->seq[gi.second], local));
+        }
+        lvl.push_back((uint32_t)recs.size());
+        windows.push_back(w);
+    }
+};
 
 int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
 {
@@ -209,8 +237,6 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
         return Fail(XPBD_E_INVALID, "settings have no vertices");
     if (d.num_skinned || d.num_rod_stretch_shear || d.num_rod_bend_twist)
         return Fail(XPBD_E_UNSUPPORTED, "skinned constraints and Cosserat rods are not supported by this build");
-    if (nv >= kMaxWindowVertices)
-        return Fail(XPBD_E_CAPACITY, "more than 16383 vertices in one body is not supported by the LDS-resident kernel");
     if (d.num_update_groups == 0 || d.update_groups == nullptr)
         return Fail(XPBD_E_INVALID, "settings have no update groups (SoftBodySharedSettings::Optimize was not run)");
 
@@ -220,7 +246,6 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
     const LRARec *lras = (const LRARec *)d.lras;
     const GroupRec *groups = (const GroupRec *)d.update_groups;
 
-    // validate group ranges
     GroupRec prev{ 0, 0, 0, 0, 0, 0, 0 };
     for (uint32_t g = 0; g < d.num_update_groups; ++g)
     {
@@ -240,17 +265,15 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
         return true;
     };
 
-    std::vector<uint4> recs;
-    std::vector<uint32_t> lvl;
-    std::vector<uint2> gtab;
+    // Per group: sequence in ProcessGroup order, then levels
+    const uint32_t ng = d.num_update_groups;
+    std::vector<GroupLevels> gl(ng);
     std::vector<uint32_t> last_write(nv, 0), last_read(nv, 0);
-    std::vector<Item> seq;
     prev = GroupRec{ 0, 0, 0, 0, 0, 0, 0 };
-    uint32_t max_par_levels = 0, tail_levels = 0, tail_count = 0, tail_width = 0;
-    for (uint32_t g = 0; g < d.num_update_groups; ++g)
+    for (uint32_t g = 0; g < ng; ++g)
     {
         const GroupRec &c = groups[g];
-        seq.clear();
+        std::vector<Item> &seq = gl[g].seq;
         for (uint32_t k = prev.volume_end; k < c.volume_end; ++k)
         {
             if (!check(vols[k].v, 4)) return Fail(XPBD_E_INVALID, "volume constraint vertex out of range");
@@ -271,30 +294,142 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
             if (!check(lras[k].v, 2)) return Fail(XPBD_E_INVALID, "LRA constraint vertex out of range");
             seq.push_back({ kTypeLRA, { lras[k].v[0], lras[k].v[1], 0, 0 }, 2, lras[k].max_distance, 0.0f });
         }
-        uint2 gt;
-        gt.x = (uint32_t)lvl.size();
-        uint32_t nl = 0, width = 0;
-        LevelGroup(seq, last_write, last_read, recs, lvl, nl, width);
-        gt.y = nl;
-        gtab.push_back(gt);
-        if (g + 1 < d.num_update_groups)
-            max_par_levels = std::max(max_par_levels, nl);
-        else
-        {
-            tail_levels = nl;
-            tail_count = (uint32_t)seq.size();
-            tail_width = width;
-        }
+        gl[g].levels = LevelGroup(seq, last_write, last_read);
         prev = c;
     }
+    const uint32_t npar = ng - 1;
+    const GroupLevels &tail = gl[npar];
 
-    s.num_parallel = d.num_update_groups - 1;
-    s.tail_one_wave = tail_width <= 64 ? 1u : 0u;
-    s.stats[0] = s.num_parallel;
+    uint32_t max_par_levels = 0;
+    for (uint32_t g = 0; g < npar; ++g)
+        max_par_levels = std::max(max_par_levels, (uint32_t)gl[g].levels.size());
+
+    using LevelList = std::vector<std::vector<std::pair<const GroupLevels *, uint32_t>>>;
+    ProgramBuilder pb;
+    const bool resident = nv <= kResidentMaxVertices;
+    uint32_t window_capacity = 0;
+    if (resident)
+    {
+        // One identity window with the merged levels of all parallel groups, one for the tail
+        LevelList par(max_par_levels);
+        for (uint32_t g = 0; g < npar; ++g)
+            for (uint32_t L = 0; L < gl[g].levels.size(); ++L)
+                for (uint32_t k : gl[g].levels[L])
+                    par[L].push_back({ &gl[g], k });
+        std::vector<uint32_t> none;
+        if (npar > 0)
+            pb.AddWindow(par, nullptr, none);
+        s.dev.num_par_windows = npar > 0 ? 1u : 0u;
+        LevelList tl(tail.levels.size());
+        for (uint32_t L = 0; L < tail.levels.size(); ++L)
+            for (uint32_t k : tail.levels[L])
+                tl[L].push_back({ &tail, k });
+        pb.AddWindow(tl, nullptr, none);
+        s.dev.num_tail_windows = 1;
+    }
+    else
+    {
+        // Gathered windows of at most kWindowCapacity vertices
+        std::vector<uint32_t> local(nv, 0xffffffffu);
+        auto vertices_of = [](const GroupLevels &G, const std::vector<uint32_t> &items, std::vector<uint32_t> &out, std::vector<uint32_t> &mark, uint32_t stamp) {
+            for (uint32_t k : items)
+                for (uint32_t j = 0; j < G.seq[k].nv; ++j)
+                {
+                    uint32_t v = G.seq[k].v[j];
+                    if (mark[v] != stamp) { mark[v] = stamp; out.push_back(v); }
+                }
+        };
+        std::vector<uint32_t> mark(nv, 0);
+        uint32_t stamp = 0;
+        // parallel groups: whole groups per window
+        uint32_t g = 0;
+        uint32_t npw = 0;
+        while (g < npar)
+        {
+            std::vector<uint32_t> verts;
+            ++stamp;
+            uint32_t g_end = g;
+            while (g_end < npar)
+            {
+                std::vector<uint32_t> gv;
+                for (const auto &lv : gl[g_end].levels)
+                    vertices_of(gl[g_end], lv, gv, mark, stamp);
+                if (!verts.empty() && verts.size() + gv.size() > kWindowCapacity)
+                {
+                    for (uint32_t v : gv) mark[v] = 0;   // not in this window
+                    break;
+                }
+                if (gv.size() > kWindowCapacity)
+                    return Fail(XPBD_E_CAPACITY, "an update group touches more vertices than a window holds");
+                verts.insert(verts.end(), gv.begin(), gv.end());
+                ++g_end;
+            }
+            std::sort(verts.begin(), verts.end());
+            for (uint32_t i = 0; i < verts.size(); ++i) local[verts[i]] = i;
+            uint32_t nl = 0;
+            for (uint32_t h = g; h < g_end; ++h) nl = std::max(nl, (uint32_t)gl[h].levels.size());
+            LevelList par(nl);
+            for (uint32_t h = g; h < g_end; ++h)
+                for (uint32_t L = 0; L < gl[h].levels.size(); ++L)
+                    for (uint32_t k : gl[h].levels[L])
+                        par[L].push_back({ &gl[h], k });
+            pb.AddWindow(par, local.data(), verts);
+            window_capacity = std::max(window_capacity, (uint32_t)verts.size());
+            ++npw;
+            g = g_end;
+        }
+        s.dev.num_par_windows = npw;
+        // tail: consecutive levels per window
+        uint32_t ntw = 0;
+        uint32_t L = 0;
+        const uint32_t ntl = (uint32_t)tail.levels.size();
+        do
+        {
+            std::vector<uint32_t> verts;
+            ++stamp;
+            uint32_t L_end = L;
+            while (L_end < ntl)
+            {
+                std::vector<uint32_t> lv;
+                uint32_t sub = stamp;
+                // count the new vertices of this level without committing them
+                std::vector<uint32_t> fresh;
+                for (uint32_t k : tail.levels[L_end])
+                    for (uint32_t j = 0; j < tail.seq[k].nv; ++j)
+                    {
+                        uint32_t v = tail.seq[k].v[j];
+                        if (mark[v] != sub && std::find(fresh.begin(), fresh.end(), v) == fresh.end())
+                            fresh.push_back(v);
+                    }
+                if (!verts.empty() && verts.size() + fresh.size() > kWindowCapacity)
+                    break;
+                if (fresh.size() > kWindowCapacity)
+                    return Fail(XPBD_E_CAPACITY, "a level of the non-parallel group touches more vertices than a window holds");
+                for (uint32_t v : fresh) { mark[v] = sub; verts.push_back(v); }
+                ++L_end;
+            }
+            std::sort(verts.begin(), verts.end());
+            for (uint32_t i = 0; i < verts.size(); ++i) local[verts[i]] = i;
+            LevelList tl(L_end - L);
+            for (uint32_t q = L; q < L_end; ++q)
+                for (uint32_t k : tail.levels[q])
+                    tl[q - L].push_back({ &tail, k });
+            pb.AddWindow(tl, local.data(), verts);
+            window_capacity = std::max(window_capacity, (uint32_t)verts.size());
+            ++ntw;
+            L = L_end;
+        } while (L < ntl);
+        s.dev.num_tail_windows = ntw;
+    }
+
+    s.stats[0] = npar;
     s.stats[1] = max_par_levels;
-    s.stats[2] = tail_levels;
-    s.stats[3] = tail_count;
-    s.stats[4] = (uint32_t)recs.size();
+    s.stats[2] = (uint32_t)tail.levels.size();
+    s.stats[3] = (uint32_t)tail.seq.size();
+    s.stats[4] = (uint32_t)pb.recs.size();
+    s.stats[5] = resident ? 1u : 0u;
+    s.stats[6] = s.dev.num_par_windows;
+    s.stats[7] = s.dev.num_tail_windows;
 
     // faces and vertex -> face lists (pressure)
     const FaceRec *faces = (const FaceRec *)d.faces;
@@ -319,13 +454,30 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
     }
 
     hipError_t e;
-    if ((e = s.recs.upload(recs)) != hipSuccess || (e = s.lvl.upload(lvl)) != hipSuccess || (e = s.groups.upload(gtab)) != hipSuccess
-        || (e = s.faces.upload(f3)) != hipSuccess || (e = s.vf_begin.upload(vf_begin)) != hipSuccess || (e = s.vf_list.upload(vf_list)) != hipSuccess)
+    if ((e = s.recs.upload(pb.recs)) != hipSuccess || (e = s.lvl.upload(pb.lvl)) != hipSuccess
+        || (e = s.windows.upload(pb.windows)) != hipSuccess || (e = s.vlist.upload(pb.vlist)) != hipSuccess
+        || (e = s.faces.upload(f3)) != hipSuccess || (e = s.vf_begin.upload(vf_begin)) != hipSuccess
+        || (e = s.vf_list.upload(vf_list)) != hipSuccess)
         return Fail(XPBD_E_DEVICE, std::string("settings upload: ") + hipGetErrorString(e));
     s.num_vertices = nv;
     s.num_faces = d.num_faces;
     const VertexRec *vr = (const VertexRec *)d.vertices;
     s.vertices.assign(vr, vr + nv);
+
+    SettingsDev &sd = s.dev;
+    sd.recs = s.recs.p;
+    sd.lvl = s.lvl.p;
+    sd.windows = s.windows.p;
+    sd.vlist = s.vlist.p;
+    sd.faces = s.faces.p;
+    sd.vf_begin = s.vf_begin.p;
+    sd.vf_list = s.vf_list.p;
+    sd.num_vertices = nv;
+    sd.num_faces = d.num_faces;
+    sd.num_records = (uint32_t)pb.recs.size();
+    sd.num_levels_total = (uint32_t)pb.lvl.size();
+    sd.resident = resident ? 1u : 0u;
+    sd.window_capacity = window_capacity;
     return XPBD_OK;
 }
 
@@ -430,7 +582,7 @@ int xpbd_settings_release(xpbd_settings *s)
     return XPBD_OK;
 }
 
-int xpbd_settings_stats(const xpbd_settings *s, uint32_t out[5])
+int xpbd_settings_stats(const xpbd_settings *s, uint32_t out[8])
 {
     if (!s || !out)
         return Fail(XPBD_E_INVALID, "null argument");
@@ -481,16 +633,7 @@ int xpbd_body_create(xpbd_context *ctx, const xpbd_body_desc *d, int32_t *out_bo
     HIP_TRY(b->contact.upload(contact));
 
     BodyDev &dv = b->dev;
-    dv.s.recs = s->recs.p;
-    dv.s.lvl = s->lvl.p;
-    dv.s.groups = s->groups.p;
-    dv.s.faces = s->faces.p;
-    dv.s.vf_begin = s->vf_begin.p;
-    dv.s.vf_list = s->vf_list.p;
-    dv.s.num_vertices = nv;
-    dv.s.num_parallel = s->num_parallel;
-    dv.s.num_faces = s->num_faces;
-    dv.s.tail_one_wave = s->tail_one_wave;
+    dv.s = s->dev;
     dv.x = b->x.p; dv.v = b->v.p; dv.prev = b->prev.p; dv.plane = b->plane.p; dv.idx = b->idx.p; dv.contact = b->contact.p;
     dv.iterations = d->num_iterations;
     dv.linear_damping = d->linear_damping;
@@ -573,24 +716,64 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
         return rc;
 
     const uint32_t nc = (uint32_t)ctx->colliders.size();
-    // Largest bodies first so the long blocks start early.
+    // Launch classes: (resident, staged) kernel variant x LDS size bin, so that
+    // small bodies are not given the LDS reservation of the largest one.
+    struct Launch { bool resident, staged; uint32_t lds; std::vector<uint32_t> bodies; };
+    std::vector<Launch> launches;
+    const uint32_t bins[] = { 24 * 1024, 40 * 1024, 64 * 1024, 96 * 1024, kMaxLdsBytes };
     std::vector<uint32_t> order(ctx->bodies.size());
     for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
+    // Largest bodies first so the long workgroups start early.
     std::stable_sort(order.begin(), order.end(), [ctx](uint32_t a, uint32_t b) { return ctx->bodies[a]->nv > ctx->bodies[b]->nv; });
-    uint32_t lds = 0;
     for (uint32_t i : order)
     {
-        const Body &b = *ctx->bodies[i];
-        uint32_t nf = b.dev.pressure > 0.0f ? b.settings->num_faces : 0u;
-        uint32_t need = resident_lds_bytes(b.nv, nc, nc, nf);
+        Body &b = *ctx->bodies[i];
+        const SettingsDev &sd = b.dev.s;
+        const uint32_t nf = b.dev.pressure > 0.0f ? sd.num_faces : 0u;
+        const bool resident = sd.resident != 0;
+        bool staged = false;
+        uint32_t need = kernel_lds_bytes(resident, false, sd, nc, nc, nf);
+        if (resident)
+        {
+            uint32_t need_staged = kernel_lds_bytes(true, true, sd, nc, nc, nf);
+            if (need_staged <= kMaxLdsBytes)
+            {
+                staged = true;
+                need = need_staged;
+            }
+        }
         if (need > kMaxLdsBytes)
             return Fail(XPBD_E_CAPACITY, "body " + std::to_string(i) + " needs " + std::to_string(need)
-                                             + " bytes of LDS; the LDS-resident kernel allows " + std::to_string(kMaxLdsBytes));
-        lds = std::max(lds, need);
+                                             + " bytes of LDS; a workgroup has " + std::to_string(kMaxLdsBytes));
+        b.dev.stage_records = staged ? 1u : 0u;
+        uint32_t bin = 0;
+        while (bins[bin] < need) ++bin;
+        Launch *L = nullptr;
+        for (Launch &l : launches)
+            if (l.resident == resident && l.staged == staged && l.lds == bins[bin])
+                L = &l;
+        if (!L)
+        {
+            launches.push_back({ resident, staged, bins[bin], {} });
+            L = &launches.back();
+        }
+        L->bodies.push_back(i);
     }
-    if (ctx->d_list.n != order.size())
-        HIP_TRY(ctx->d_list.alloc(order.size()));
-    HIP_TRY(hipMemcpyAsync(ctx->d_list.p, order.data(), sizeof(uint32_t) * order.size(), hipMemcpyHostToDevice, ctx->stream));
+    // stage_records may have changed
+    {
+        std::vector<BodyDev> &devs = ctx->host_devs;
+        devs.clear();
+        for (auto &b : ctx->bodies) devs.push_back(b->dev);
+        if (ctx->d_bodies.n != devs.size())
+            HIP_TRY(ctx->d_bodies.alloc(devs.size()));
+        HIP_TRY(hipMemcpyAsync(ctx->d_bodies.p, devs.data(), sizeof(BodyDev) * devs.size(), hipMemcpyHostToDevice, ctx->stream));
+    }
+    std::vector<uint32_t> list;
+    for (const Launch &l : launches) list.insert(list.end(), l.bodies.begin(), l.bodies.end());
+    if (ctx->d_list.n != list.size())
+        HIP_TRY(ctx->d_list.alloc(list.size()));
+    HIP_TRY(hipMemcpyAsync(ctx->d_list.p, list.data(), sizeof(uint32_t) * list.size(), hipMemcpyHostToDevice, ctx->stream));
+    ctx->host_list = list;
 
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ctx->profiling)
@@ -599,8 +782,14 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
         HIP_TRY(hipEventCreate(&e1));
         HIP_TRY(hipEventRecord(e0, ctx->stream));
     }
-    HIP_TRY(launch_resident(ctx->d_bodies.p, ctx->d_states.p, ctx->d_list.p, (uint32_t)order.size(), ctx->d_colliders.p, nc, nc, lds,
-                            ctx->gravity[0], ctx->gravity[1], ctx->gravity[2], dt, collision_steps, frames, ctx->d_error.p, ctx->stream));
+    uint32_t offset = 0;
+    for (const Launch &l : launches)
+    {
+        HIP_TRY(launch_update(l.resident, l.staged, ctx->d_bodies.p, ctx->d_states.p, ctx->d_list.p + offset, (uint32_t)l.bodies.size(),
+                              ctx->d_colliders.p, nc, nc, l.lds, ctx->gravity[0], ctx->gravity[1], ctx->gravity[2], dt,
+                              collision_steps, frames, ctx->d_error.p, ctx->stream));
+        offset += (uint32_t)l.bodies.size();
+    }
     if (ctx->profiling)
     {
         HIP_TRY(hipEventRecord(e1, ctx->stream));

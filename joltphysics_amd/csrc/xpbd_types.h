@@ -12,22 +12,38 @@ namespace xpbd {
 //   b: bits 0-15 vertex 2, bits 16-31 vertex 3
 //   p0, p1: edge {rest, compliance}, bend {compliance, initialAngle},
 //           volume {sixRestVolume, compliance}, LRA {maxDistance, -}
-// Vertex numbers are indices into the LDS window of the body (< 16384).
+// Vertex numbers index the LDS window the record runs in (< 16384).
 enum : uint32_t { kTypeVolume = 0, kTypeBend = 1, kTypeEdge = 2, kTypeLRA = 3 };
 constexpr uint32_t kMaxWindowVertices = 1u << 14;
 
+// A window is a set of vertices held in LDS while a run of levels executes on
+// it.  Resident bodies keep all their vertices in LDS for the whole launch and
+// use identity windows (vcount == 0: record vertex numbers are body vertex
+// indices).  Other bodies gather vlist[vbegin, vbegin+vcount) into LDS, run the
+// levels, and scatter the vertices back.
+struct WindowDev
+{
+    uint32_t vbegin, vcount;    // vertex list range (vcount 0: identity window)
+    uint32_t lbegin, nlev;      // index of the first level offset in lvl, number of levels
+};
+
 struct SettingsDev
 {
-    const uint4 *recs;          // records ordered (group, level, sequence)
-    const uint32_t *lvl;        // absolute record offset of each level start, nlev+1 per group
-    const uint2 *groups;        // {index of first lvl entry, num levels}; parallel groups then the tail
+    const uint4 *recs;          // records ordered (window, level)
+    const uint32_t *lvl;        // absolute record offset of each level start, nlev+1 per window
+    const WindowDev *windows;   // parallel-group windows, then tail windows
+    const uint32_t *vlist;      // gathered windows' vertex lists (body vertex indices)
     const uint32_t *faces;      // 3 vertex indices per face (pressure)
     const uint32_t *vf_begin;   // vertex -> face CSR, faces in increasing order (pressure)
     const uint32_t *vf_list;
     uint32_t num_vertices;
-    uint32_t num_parallel;      // number of parallel groups (tail group index = num_parallel)
+    uint32_t num_par_windows;   // windows [0, num_par_windows) hold the parallel groups
+    uint32_t num_tail_windows;  // then the non-parallel group
     uint32_t num_faces;
-    uint32_t tail_one_wave;     // 1 when every tail level has <= 64 constraints
+    uint32_t num_records;
+    uint32_t num_levels_total;  // entries of lvl
+    uint32_t resident;          // 1: identity windows, positions stay in LDS
+    uint32_t window_capacity;   // largest gathered window (vertices)
 };
 
 struct BodyDev
@@ -42,6 +58,7 @@ struct BodyDev
     uint32_t iterations;
     float linear_damping, max_linear_velocity, restitution, friction, pressure, gravity_factor, vertex_radius;
     int32_t update_position, allow_sleeping;
+    uint32_t stage_records;     // 1: records + level table are copied into LDS once per launch
 };
 
 // Per-body dynamic state kept on the device between launches.

@@ -119,10 +119,11 @@ class Settings:
         self.arrays = arrays
 
     def stats(self):
-        out = (C.c_uint32 * 5)()
+        out = (C.c_uint32 * 8)()
         _check(self.ctx.lib.xpbd_settings_stats(self.handle, out))
         return dict(parallel_groups=out[0], max_group_levels=out[1], tail_levels=out[2],
-                    tail_constraints=out[3], constraints=out[4])
+                    tail_constraints=out[3], constraints=out[4], resident=bool(out[5]),
+                    parallel_windows=out[6], tail_windows=out[7])
 
 
 class Context:

@@ -69,3 +69,29 @@ def float_equal(a, b):
     a = np.asarray(a, np.float32)
     b = np.asarray(b, np.float32)
     return a.shape == b.shape and bool(np.all(a == b))
+
+
+def reference_run(scene, frames_to_keep, variant="nofma", num_threads=0):
+    """Builds settings with the reference's own code and runs PhysicsSystem::Update.
+
+    Returns (settings_list, collider_bounds[n,6], {frame: [read(body)...]})."""
+    from oracle import JoltReference, JoltWorld
+    ref = JoltReference(variant)
+    settings = [ref.build_settings(b.mesh) for b in scene.bodies]
+    for b, s in zip(scene.bodies, settings):
+        if b.post is not None:
+            b.post(s)
+    world = JoltWorld(ref, scene.gravity, num_threads=num_threads)
+    ids = [world.add_collider(c) for c in scene.colliders]
+    bounds = np.array([world.collider_bounds(i) for i in ids], np.float32).reshape(-1, 6)
+    for b, s in zip(scene.bodies, settings):
+        world.add_soft_body(s, b.position, b.rotation, b.params)
+    world.optimize_broadphase()
+    out = {}
+    done = 0
+    for f in sorted(frames_to_keep):
+        world.update(scene.dt, scene.collision_steps, f - done)
+        done = f
+        out[f] = [world.read(i) for i in range(len(scene.bodies))]
+    world.close()
+    return settings, bounds, out

@@ -1,0 +1,105 @@
+"""Small scenes covering every constraint type, collider shape and storage mode.
+
+Used by the parity tests on both sides (reference oracle and the HIP path).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from joltphysics_amd import scenes
+from joltphysics_amd.records import (BEND_DIHEDRAL, BEND_DISTANCE, LRA_GEODESIC, SHAPE_BOX, SHAPE_CAPSULE,
+                                     SHAPE_PLANE, SHAPE_SPHERE, BodyParams, Collider)
+from joltphysics_amd.scenes import (CALC_EDGE_LENGTHS, CALC_VOLUMES, CREATE_CONSTRAINTS, OPTIMIZE, BodySpec,
+                                    MeshSpec, Scene)
+
+
+def _quat(axis, angle):
+    axis = np.asarray(axis, np.float64)
+    axis = axis / np.linalg.norm(axis)
+    s = np.sin(angle / 2)
+    return tuple(float(x) for x in np.float32([axis[0] * s, axis[1] * s, axis[2] * s, np.cos(angle / 2)]))
+
+
+def _bounded(cols):
+    for c in cols:
+        c.bounds = scenes.collider_world_bounds(c)
+    return cols
+
+
+def big_cloth_on_sphere(n=100, frames=10):
+    """n x n cloth (streamed windows when n*n > 8192), top corners pinned, over a sphere."""
+    v, f = scenes.cloth(n, 0.02, 50.0, pinned=((n - 1) * n, n * n - 1), jitter=1e-4, seed=11)
+    mesh = MeshSpec(vertices=v, faces=f, attrs=scenes.attrs(1e-6, 1e-6, 1e-4), bend_type=BEND_DIHEDRAL)
+    body = BodySpec(mesh=mesh, position=(0.0, 0.0, 0.5), params=BodyParams(num_iterations=4, allow_sleeping=False))
+    cols = _bounded([Collider(SHAPE_SPHERE, (0.4,), pos=(0.1, -0.2, -0.1), friction=0.5)])
+    return Scene(f"cloth{n}_sphere", [body], cols, gravity=(0.0, 0.0, -9.81), dt=1 / 60, collision_steps=1, frames=frames)
+
+
+def tet_cube_on_plane(cells=6, frames=30):
+    """Rotated tet block with edge + volume constraints dropping onto a plane and a box."""
+    v, e, t = scenes.tet_cube(cells, 0.1, inv_mass=2.0, compliance=1e-5)
+    mesh = MeshSpec(vertices=v, edges=e, volumes=t, flags=CALC_EDGE_LENGTHS | CALC_VOLUMES | OPTIMIZE)
+    rot = _quat((1, 2, 3), 0.7)
+    body = BodySpec(mesh=mesh, position=(0.0, 0.0, 0.8), rotation=rot,
+                    params=BodyParams(num_iterations=5, allow_sleeping=False, restitution=0.3))
+    cols = _bounded([Collider(SHAPE_PLANE, (0.0, 0.0, 1.0, 0.0, 50.0)),
+                     Collider(SHAPE_BOX, (0.3, 0.2, 0.1), pos=(0.2, 0.1, 0.15), rot=_quat((0, 0, 1), 0.4), friction=0.8)])
+    return Scene("tetcube_plane_box", [body], cols, gravity=(0.0, 0.0, -9.81), dt=1 / 60, collision_steps=2, frames=frames)
+
+
+def pressure_sphere(subdiv=2, frames=30):
+    """Icosphere with pressure falling on a capsule and a plane."""
+    v, f = scenes.icosphere(subdiv, radius=0.5, inv_mass=1.0)
+    mesh = MeshSpec(vertices=v, faces=f, attrs=scenes.attrs(1e-4, 1e-4, 1e-3), bend_type=BEND_DIHEDRAL)
+    body = BodySpec(mesh=mesh, position=(0.05, 0.0, 1.2),
+                    params=BodyParams(num_iterations=5, pressure=1000.0, allow_sleeping=False))
+    cols = _bounded([Collider(SHAPE_PLANE, (0.0, 0.0, 1.0, 0.0, 50.0)),
+                     Collider(SHAPE_CAPSULE, (0.3, 0.15), pos=(0.0, 0.0, 0.4), rot=_quat((1, 0, 0), 1.2))])
+    return Scene("pressure_sphere", [body], cols, gravity=(0.0, 0.0, -9.81), dt=1 / 60, collision_steps=1, frames=frames)
+
+
+def lra_cloth(n=20, frames=30):
+    """Cloth hanging from its top row with geodesic long-range attachments and
+    distance bends, swinging into a rotated box."""
+    pinned = tuple((n - 1) * n + i for i in range(n))
+    v, f = scenes.cloth(n, 0.05, 100.0, pinned=pinned, jitter=1e-4, seed=5)
+    v["pos"][:, [1, 2]] = v["pos"][:, [2, 1]]          # hang in the x-z plane
+    mesh = MeshSpec(vertices=v, faces=f, attrs=scenes.attrs(1e-5, 1e-5, 1e-4, LRA_GEODESIC, 1.0),
+                    bend_type=BEND_DISTANCE)
+    body = BodySpec(mesh=mesh, params=BodyParams(num_iterations=6, allow_sleeping=False, friction=0.4))
+    cols = _bounded([Collider(SHAPE_BOX, (0.2, 0.2, 0.2), pos=(0.0, 0.3, -0.5), rot=_quat((0, 1, 1), 0.5))])
+    return Scene("lra_cloth", [body], cols, gravity=(0.0, 0.0, -9.81), dt=1 / 60, collision_steps=1, frames=frames)
+
+
+def sleeping_cloth(frames=60):
+    """Small cloth resting on a plane with sleeping allowed (deactivation path)."""
+    v, f = scenes.cloth(10, 0.05, 10.0, jitter=0.0)
+    mesh = MeshSpec(vertices=v, faces=f, attrs=scenes.attrs(1e-5, 1e-5, 1e-3), bend_type=BEND_DIHEDRAL)
+    body = BodySpec(mesh=mesh, position=(0.0, 0.0, 0.02), params=BodyParams(num_iterations=5, allow_sleeping=True))
+    cols = _bounded([Collider(SHAPE_PLANE, (0.0, 0.0, 1.0, 0.0, 50.0), friction=1.0)])
+    return Scene("sleeping_cloth", [body], cols, gravity=(0.0, 0.0, -9.81), dt=1 / 60, collision_steps=1, frames=frames)
+
+
+def batched(frames=20):
+    """All of the above in one world (one launch for all bodies)."""
+    parts = [big_cloth_on_sphere(100), tet_cube_on_plane(), pressure_sphere(), lra_cloth()]
+    offsets = [(0.0, 0.0, 0.0), (3.0, 0.0, 0.0), (0.0, 3.0, 0.0), (3.0, 3.0, 0.0)]
+    bodies, cols = [], []
+    for p, off in zip(parts, offsets):
+        for b in p.bodies:
+            b.position = tuple(np.float32(np.add(b.position, off)))
+            bodies.append(b)
+        for c in p.colliders:
+            c.pos = tuple(np.float32(np.add(c.pos, off)))
+            cols.append(c)
+    return Scene("batched", bodies, _bounded(cols), gravity=(0.0, 0.0, -9.81), dt=1 / 60, collision_steps=1, frames=frames)
+
+
+ALL = {
+    "cloth100_sphere": big_cloth_on_sphere,
+    "tetcube_plane_box": tet_cube_on_plane,
+    "pressure_sphere": pressure_sphere,
+    "lra_cloth": lra_cloth,
+    "sleeping_cloth": sleeping_cloth,
+    "batched": batched,
+}
