@@ -89,6 +89,60 @@ int xpbd_settings_release(xpbd_settings *settings);
 int xpbd_settings_stats(const xpbd_settings *settings, uint32_t out[8]);
 
 /*
+ * Host-side settings builder: for callers that have a mesh but not the
+ * reference library.  Produces exactly the arrays the reference's
+ * SoftBodySharedSettings would hold after the requested steps (same
+ * constraints, same order, same update groups):
+ *   CreateConstraints     (SoftBodySharedSettings.cpp:189-369)
+ *   CalculateEdgeLengths  (:371-379)
+ *   CalculateVolumeConstraintVolumes (:536-555)
+ *   CalculateLRALengths   (:483-491)
+ *   CalculateBendConstraintConstants (:493-534)
+ *   Optimize              (:613-1182)
+ * Runs on the CPU; no device needed.
+ */
+#define XPBD_BUILD_CREATE_CONSTRAINTS 1
+#define XPBD_BUILD_EDGE_LENGTHS 2
+#define XPBD_BUILD_VOLUMES 4
+#define XPBD_BUILD_LRA_LENGTHS 8
+#define XPBD_BUILD_OPTIMIZE 16
+#define XPBD_BUILD_BEND_CONSTANTS 32
+
+/* SoftBodySharedSettings::VertexAttributes (SoftBodySharedSettings.h:39-50);
+ * lra_type: 0 none, 1 Euclidean, 2 geodesic (ELRAType). */
+typedef struct xpbd_vertex_attributes {
+    float compliance;
+    float shear_compliance;
+    float bend_compliance;
+    int32_t lra_type;
+    float lra_max_distance_multiplier;
+} xpbd_vertex_attributes;
+
+typedef struct xpbd_build_desc {
+    const void *vertices;       uint32_t num_vertices;   /* Vertex records */
+    const uint32_t *faces;      uint32_t num_faces;      /* 3 indices per face (AddFace order) */
+    const void *edges;          uint32_t num_edges;      /* explicit Edge records */
+    const void *volumes;        uint32_t num_volumes;    /* explicit Volume records */
+    const xpbd_vertex_attributes *attributes; uint32_t num_attributes;
+    int32_t bend_type;          /* EBendType: 0 none, 1 distance, 2 dihedral */
+    float angle_tolerance;      /* radians (reference default DegreesToRadians(8)) */
+    float lra_max_distance_multiplier;  /* for XPBD_BUILD_LRA_LENGTHS */
+    uint32_t flags;             /* XPBD_BUILD_* steps, applied in the order listed above */
+} xpbd_build_desc;
+
+typedef struct xpbd_built_settings xpbd_built_settings;
+
+int xpbd_settings_build(const xpbd_build_desc *desc, xpbd_built_settings **out_built);
+/* counts[7] = vertices, faces, edges, dihedral bends, volumes, LRAs, update groups */
+int xpbd_built_counts(const xpbd_built_settings *built, uint32_t counts[7]);
+/* Copies the arrays out in the reference's record layouts (NULL skips). */
+int xpbd_built_read(const xpbd_built_settings *built, void *vertices, void *faces, void *edges, void *dihedral_bends,
+                    void *volumes, void *lras, void *update_groups);
+int xpbd_built_free(xpbd_built_settings *built);
+/* Last error of the settings builder on the calling thread. */
+const char *xpbd_build_last_error(void);
+
+/*
  * Soft body: replaces BodyInterface::CreateSoftBody + SoftBodyMotionProperties::
  * Initialize (Body.cpp:466-492, SoftBodyMotionProperties.cpp:59-105) for the
  * fields of SoftBodyCreationSettings that the update uses (.h:61-72).

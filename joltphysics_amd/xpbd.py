@@ -55,10 +55,27 @@ class _ColliderDesc(C.Structure):
                 ("world_bounds", C.c_float * 6)]
 
 
+class _VertexAttributes(C.Structure):
+    _fields_ = [("compliance", C.c_float), ("shear_compliance", C.c_float), ("bend_compliance", C.c_float),
+                ("lra_type", C.c_int32), ("lra_max_distance_multiplier", C.c_float)]
+
+
+class _BuildDesc(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("num_vertices", C.c_uint32),
+                ("faces", C.c_void_p), ("num_faces", C.c_uint32),
+                ("edges", C.c_void_p), ("num_edges", C.c_uint32),
+                ("volumes", C.c_void_p), ("num_volumes", C.c_uint32),
+                ("attributes", C.c_void_p), ("num_attributes", C.c_uint32),
+                ("bend_type", C.c_int32), ("angle_tolerance", C.c_float),
+                ("lra_max_distance_multiplier", C.c_float), ("flags", C.c_uint32)]
+
+
 # Exported symbols of the C ABI, in declaration order of include/jolt_xpbd.h.
 EXPORTED_SYMBOLS = (
     "xpbd_last_error", "xpbd_context_create", "xpbd_context_destroy", "xpbd_context_stream",
-    "xpbd_settings_create", "xpbd_settings_release", "xpbd_settings_stats", "xpbd_body_create",
+    "xpbd_settings_create", "xpbd_settings_release", "xpbd_settings_stats",
+    "xpbd_settings_build", "xpbd_built_counts", "xpbd_built_read", "xpbd_built_free",
+    "xpbd_build_last_error", "xpbd_body_create",
     "xpbd_set_colliders", "xpbd_set_gravity", "xpbd_update", "xpbd_synchronize", "xpbd_body_read",
     "xpbd_body_write", "xpbd_body_state", "xpbd_profile_enable", "xpbd_profile_read",
 )
@@ -84,6 +101,11 @@ def load_library(path=LIB_PATH):
     lib.xpbd_settings_create.argtypes = [vp, C.POINTER(_SettingsDesc), C.POINTER(vp)]
     lib.xpbd_settings_release.argtypes = [vp]
     lib.xpbd_settings_stats.argtypes = [vp, C.POINTER(u32)]
+    lib.xpbd_settings_build.argtypes = [C.POINTER(_BuildDesc), C.POINTER(vp)]
+    lib.xpbd_built_counts.argtypes = [vp, C.POINTER(u32)]
+    lib.xpbd_built_read.argtypes = [vp] * 8
+    lib.xpbd_built_free.argtypes = [vp]
+    lib.xpbd_build_last_error.restype = C.c_char_p
     lib.xpbd_body_create.argtypes = [vp, C.POINTER(_BodyDesc), C.POINTER(i32)]
     lib.xpbd_set_colliders.argtypes = [vp, C.POINTER(_ColliderDesc), u32]
     lib.xpbd_set_gravity.argtypes = [vp, C.POINTER(f)]
@@ -95,7 +117,7 @@ def load_library(path=LIB_PATH):
     lib.xpbd_profile_enable.argtypes = [vp, i32]
     lib.xpbd_profile_read.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_double)]
     for name in EXPORTED_SYMBOLS:
-        if name != "xpbd_last_error" and name != "xpbd_context_stream":
+        if name not in ("xpbd_last_error", "xpbd_context_stream", "xpbd_build_last_error"):
             getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib
@@ -110,6 +132,40 @@ def _ptr(a):
     if a is None or a.size == 0:
         return None
     return a.ctypes.data_as(C.c_void_p)
+
+
+def build_settings(mesh) -> SettingsArrays:
+    """Builds a SoftBodySharedSettings from a MeshSpec on the CPU (CreateConstraints,
+    Calculate*, Optimize restated by the host library; no device needed)."""
+    lib = load_library()
+    v = np.ascontiguousarray(mesh.vertices, VERTEX)
+    f = np.ascontiguousarray(mesh.faces, np.uint32).reshape(-1, 3)
+    e = np.ascontiguousarray(mesh.edges, EDGE)
+    t = np.ascontiguousarray(mesh.volumes, VOLUME)
+    a = np.asarray(mesh.attrs, np.float32).reshape(-1, 5)
+    attrs = (_VertexAttributes * max(len(a), 1))()
+    for i, row in enumerate(a):
+        attrs[i].compliance, attrs[i].shear_compliance, attrs[i].bend_compliance = float(row[0]), float(row[1]), float(row[2])
+        attrs[i].lra_type = int(row[3])
+        attrs[i].lra_max_distance_multiplier = float(row[4])
+    d = _BuildDesc(_ptr(v), len(v), _ptr(f), len(f), _ptr(e), len(e), _ptr(t), len(t),
+                   C.cast(attrs, C.c_void_p) if len(a) else None, len(a), int(mesh.bend_type),
+                   float(mesh.angle_tolerance), float(mesh.lra_multiplier), int(mesh.flags))
+    h = C.c_void_p()
+    rc = lib.xpbd_settings_build(C.byref(d), C.byref(h))
+    if rc != XPBD_OK:
+        raise XpbdError(f"settings build error {rc}: {lib.xpbd_build_last_error().decode()}")
+    try:
+        counts = (C.c_uint32 * 7)()
+        lib.xpbd_built_counts(h, counts)
+        nv, nf, ne, nb, nt, nl, ng = (int(c) for c in counts)
+        s = SettingsArrays(np.zeros(nv, VERTEX), np.zeros(nf, FACE), np.zeros(ne, EDGE), np.zeros(nb, BEND),
+                           np.zeros(nt, VOLUME), np.zeros(nl, LRA), np.zeros(ng, GROUP))
+        lib.xpbd_built_read(h, _ptr(s.vertices), _ptr(s.faces), _ptr(s.edges), _ptr(s.bends),
+                            _ptr(s.volumes), _ptr(s.lras), _ptr(s.groups))
+        return s
+    finally:
+        lib.xpbd_built_free(h)
 
 
 class Settings:
