@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X soft-body XPBD update.
+
+One "step" is one frame: PhysicsSystem::Update(dt = 1/60, collision_steps) for
+every soft body of the workload, i.e. collision_steps x iterations XPBD
+sub-steps per body.  The default workload is the batched uneven scene (2048
+bodies: cloths 8^2-128^2, 5-tet cubes 4^3-16^3 cells, pressurised icospheres;
+2 collision steps x 5 iterations) with synthetic geometry.  The K timed frames
+run as one launch per kernel variant on the library's stream, with the inputs
+resident in HBM.
+
+Prints one JSON line:
+  metric/value/unit   constraint solves per second (sum over bodies of
+                      steps x iterations x constraints, per frame)
+  roofline            algorithmic HBM bytes of the update kernels per launch /
+                      launch duration (hipEvents on the library stream)
+  cpu_baseline        the reference (compiled from its sources, shipped x86-64
+                      flags, PhysicsSystem::Update with JobSystemThreadPool on
+                      all host cores) on a bounded sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from joltphysics_amd import scenes  # noqa: E402
+from joltphysics_amd.xpbd import Context, build_settings  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per unit (fp32 SoA; each constraint gathers position +
+# inverse mass of its vertices and scatters positions, its record is read once
+# per sub-step; per vertex and sub-step the velocity update + integration read
+# x, prev, v, w, plane, shape index and write prev, x, v; per vertex and
+# collision step the plane search reads 16 B and writes 24 B).
+BYTES_EDGE, BYTES_BEND, BYTES_VOLUME, BYTES_LRA = 72, 136, 136, 48
+BYTES_VERTEX_SUBSTEP, BYTES_VERTEX_STEP = 96, 40
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_workload(name):
+    if name == "batched2048":
+        return scenes.config3_batched()
+    if name == "cloth512_cube48":
+        return scenes.config2_cloth_and_cube()
+    if name == "cloth32_on_box":
+        return scenes.config1_cloth_on_box()
+    raise SystemExit(f"unknown workload {name}")
+
+
+def algorithmic_bytes_per_frame(scene, settings):
+    total = 0
+    for b, s in zip(scene.bodies, settings):
+        sub = scene.collision_steps * b.params.num_iterations
+        total += sub * (BYTES_EDGE * len(s.edges) + BYTES_BEND * len(s.bends) + BYTES_VOLUME * len(s.volumes)
+                        + BYTES_LRA * len(s.lras) + BYTES_VERTEX_SUBSTEP * len(s.vertices))
+        total += scene.collision_steps * BYTES_VERTEX_STEP * len(s.vertices)
+    return total
+
+
+def usable_cpus():
+    """CPUs this process may run on: affinity mask, capped by a cgroup CPU quota."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()
+            if quota != "max":
+                n = min(n, max(1, int(int(quota) / int(period))))
+    except Exception:
+        pass
+    return n
+
+
+def cpu_baseline(scene, settings, target_seconds=15.0, max_bodies=256):
+    """Reference CPU path on a bounded sample of the workload (first bodies of the
+    same randomised list), all host cores."""
+    try:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle import JoltReference, JoltWorld, have_reference
+        if not have_reference("fma"):
+            log("cpu_baseline: oracle/_ref/libjolt_oracle_fma.so not built; skipped")
+            return None
+        ref = JoltReference("fma")
+        threads = max(min(ref.hardware_concurrency(), usable_cpus()) - 1, 0)
+        nb = min(max_bodies, len(scene.bodies))
+        world = JoltWorld(ref, scene.gravity, max_bodies=nb + len(scene.colliders) + 16, num_threads=threads)
+        for c in scene.colliders:
+            world.add_collider(c)
+        for b, s in zip(scene.bodies[:nb], settings[:nb]):
+            world.add_soft_body(s, b.position, b.rotation, b.params)
+        world.optimize_broadphase()
+        sub_scene_solves = scenes.solves_per_frame(scene, settings[:nb]) if nb == len(scene.bodies) else \
+            sum(scene.collision_steps * b.params.num_iterations * s.num_constraints()
+                for b, s in zip(scene.bodies[:nb], settings[:nb]))
+        world.update(scene.dt, scene.collision_steps, 1)          # warm-up frame
+        t1 = world.update(scene.dt, scene.collision_steps, 1)
+        frames = int(max(1, min(600, target_seconds / max(t1, 1e-6))))
+        t = world.update(scene.dt, scene.collision_steps, frames)
+        world.close()
+        return {"value": sub_scene_solves * frames / t, "unit": "constraint-solves/s", "cores": threads + 1,
+                "kind": "reference",
+                "sample": f"first {nb} of {len(scene.bodies)} bodies, {frames} frames, "
+                          f"PhysicsSystem::Update on JobSystemThreadPool({threads} workers + caller), "
+                          f"reference built with -O3 -mavx2 -mfma -ffp-contract=fast",
+                "ms_per_frame": 1000.0 * t / frames}
+    except Exception as e:  # the baseline is reported, never required
+        log(f"cpu_baseline failed: {e!r}")
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="batched2048")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "independent bodies would shard across GPUs by cost with no exchange", file=sys.stderr)
+        sys.exit(2)
+
+    scene = make_workload(args.workload)
+    t0 = time.time()
+    settings = scenes.resolve_meshes(scene, build_settings)
+    log(f"workload {scene.name}: {len(settings)} bodies, {sum(len(s.vertices) for s in settings)} vertices, "
+        f"{sum(s.num_constraints() for s in settings)} constraints, settings built in {time.time() - t0:.1f}s")
+
+    ctx = Context(0)
+    ctx.set_gravity(scene.gravity)
+    ctx.set_colliders(scene.colliders)
+    t0 = time.time()
+    for b, s in zip(scene.bodies, settings):
+        st = ctx.create_settings(s)
+        ctx.create_body(st, b.position, b.rotation, b.params)
+    log(f"uploaded in {time.time() - t0:.1f}s")
+
+    if args.warmup > 0:
+        ctx.update(scene.dt, scene.collision_steps, args.warmup)
+    ctx.synchronize()
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.cuda.synchronize()
+    except Exception:
+        torch = None
+
+    ctx.profile_enable(True)
+    t0 = time.perf_counter()
+    ctx.update(scene.dt, scene.collision_steps, args.steps)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    n_launch, kernel_ms = ctx.profile_read()
+    ctx.profile_enable(False)
+    wall = t1 - t0
+
+    solves = scenes.solves_per_frame(scene, settings)
+    value = solves * args.steps / wall
+    bytes_frame = algorithmic_bytes_per_frame(scene, settings)
+    achieved = bytes_frame * args.steps / (kernel_ms / 1000.0) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "update_kernel (all variants of one launch)",
+                "algorithmic_bytes_per_launch": bytes_frame * args.steps,
+                "launch_ms": round(kernel_ms / max(n_launch, 1), 3)}
+    cpu = None if args.no_cpu_baseline else cpu_baseline(scene, settings, args.cpu_seconds)
+    out = {
+        "metric": "constraint-solves/sec", "value": value, "unit": "constraint-solves/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1000.0 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": scene.name, "bodies": len(settings),
+                   "vertices": int(sum(len(s.vertices) for s in settings)),
+                   "constraints": int(sum(s.num_constraints() for s in settings)),
+                   "solves_per_frame": int(solves), "collision_steps": scene.collision_steps,
+                   "iterations": int(scene.bodies[0].params.num_iterations), "dt": scene.dt,
+                   "parity": "bit-exact with the reference (FMA off)"},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = round(value / cpu["value"], 2)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

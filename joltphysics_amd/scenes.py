@@ -256,3 +256,94 @@ def config2_cloth_and_cube(cloth_n=512, cube_cells=48, num_spheres=64, seed=2, f
         c.bounds = collider_world_bounds(c)
     return Scene("cloth512_cube48", [cloth_body, cube_body], colliders, gravity=(0.0, 0.0, -9.81),
                  dt=1.0 / 60.0, collision_steps=4, frames=frames)
+
+
+def random_quaternion(rng):
+    """Uniform random rotation (Shoemake), float32 components, normalised in float64."""
+    u1, u2, u3 = rng.uniform(size=3)
+    q = np.array([np.sqrt(1 - u1) * np.sin(2 * np.pi * u2), np.sqrt(1 - u1) * np.cos(2 * np.pi * u2),
+                  np.sqrt(u1) * np.sin(2 * np.pi * u3), np.sqrt(u1) * np.cos(2 * np.pi * u3)])
+    return tuple(float(x) for x in np.float32(q / np.linalg.norm(q)))
+
+
+def config3_batched(num_bodies=2048, seed=3, frames=600):
+    """Batched uneven bodies: 60 % cloths (side log-uniform 8-128 vertices, spacing 0.05 m,
+    edges + shear + dihedral bends), 30 % 5-tet cubes (4^3-16^3 cells, spacing 0.1 m, edges +
+    volumes), 10 % icospheres (subdivision 2-4, radius 1 m, pressure U(500, 2000)); one
+    compliance per body, log-uniform in [1e-6, 1e-3]; random position in a 100 m box above a
+    ground plane and random rotation; dt 1/60, 2 collision steps x 5 iterations."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n_cloth = int(round(0.6 * num_bodies))
+    n_cube = int(round(0.3 * num_bodies))
+    n_ico = num_bodies - n_cloth - n_cube
+    kinds = ["cloth"] * n_cloth + ["cube"] * n_cube + ["ico"] * n_ico
+    rng.shuffle(kinds)
+    bodies = []
+    for kind in kinds:
+        compliance = float(np.float32(np.exp(rng.uniform(np.log(1e-6), np.log(1e-3)))))
+        pos = (float(rng.uniform(-50, 50)), float(rng.uniform(-50, 50)), float(rng.uniform(5, 105)))
+        rot = random_quaternion(rng)
+        if kind == "cloth":
+            n = int(round(np.exp(rng.uniform(np.log(8), np.log(128)))))
+            key = ("cloth", n)
+            params = BodyParams(num_iterations=5, allow_sleeping=False)
+        elif kind == "cube":
+            n = int(rng.integers(4, 17))
+            key = ("cube", n)
+            params = BodyParams(num_iterations=5, allow_sleeping=False)
+        else:
+            n = int(rng.integers(2, 5))
+            key = ("ico", n)
+            params = BodyParams(num_iterations=5, allow_sleeping=False, pressure=float(rng.uniform(500, 2000)))
+        bodies.append(BodySpec(mesh=key, position=pos, rotation=rot, params=params,
+                               post=_uniform_compliance(compliance)))
+    plane = Collider(SHAPE_PLANE, (0.0, 0.0, 1.0, 0.0, 1000.0))
+    plane.bounds = collider_world_bounds(plane)
+    return Scene("batched2048", bodies, [plane], gravity=(0.0, 0.0, -9.81), dt=1.0 / 60.0,
+                 collision_steps=2, frames=frames)
+
+
+def _uniform_compliance(c):
+    def apply(s):
+        s.edges["compliance"] = np.float32(c)
+        s.bends["compliance"] = np.float32(c)
+        s.volumes["compliance"] = np.float32(c)
+    return apply
+
+
+def mesh_for_key(key):
+    """MeshSpec of a shape key used by config3_batched (meshes shared between bodies)."""
+    kind, n = key
+    if kind == "cloth":
+        v, f = cloth(n, 0.05, 100.0)
+        return MeshSpec(vertices=v, faces=f, attrs=attrs(1e-4, 1e-4, 1e-4), bend_type=BEND_DIHEDRAL)
+    if kind == "cube":
+        v, e, t = tet_cube(n, 0.1, inv_mass=1.0, compliance=1e-4)
+        return MeshSpec(vertices=v, edges=e, volumes=t, flags=CALC_EDGE_LENGTHS | CALC_VOLUMES | OPTIMIZE)
+    v, f = icosphere(n, radius=1.0, inv_mass=1.0)
+    return MeshSpec(vertices=v, faces=f, attrs=attrs(1e-4, 1e-4, 1e-4), bend_type=BEND_DIHEDRAL)
+
+
+def resolve_meshes(scene, build):
+    """Replaces shape keys by MeshSpecs and returns per-body settings built with `build`
+    (one build per distinct shape; the per-body compliance is applied on a copy)."""
+    cache = {}
+    settings = []
+    for b in scene.bodies:
+        key = b.mesh if isinstance(b.mesh, tuple) else None
+        if key is not None:
+            if key not in cache:
+                cache[key] = build(mesh_for_key(key))
+            s = cache[key].copy()
+        else:
+            s = build(b.mesh)
+        if b.post is not None:
+            b.post(s)
+        settings.append(s)
+    return settings
+
+
+def solves_per_frame(scene, settings):
+    """Constraint solves per frame: sum over bodies of steps x iterations x constraints."""
+    return sum(scene.collision_steps * b.params.num_iterations * s.num_constraints()
+               for b, s in zip(scene.bodies, settings))

@@ -8,6 +8,7 @@ import numpy as np
 from joltphysics_amd.records import (BEND, EDGE, FACE, GROUP, LRA, VERTEX, VOLUME, SettingsArrays)
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+ROOT_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def load_config1_fixture():
