@@ -27,27 +27,11 @@
 
 #include <cfloat>
 
-#include "jolt_math.h"
-#include "xpbd_types.h"
+#include "xpbd_device.h"
 
 namespace xpbd {
 
-constexpr int kWave = 64;
-constexpr int kBlock = 256;
-constexpr int kWaves = kBlock / kWave;
-
-// Collider resolved into the soft body's local space for one collision step
-// (the reference's LeafShape + CollidingShape, SoftBodyMotionProperties.h).
-struct Leaf
-{
-    M44 t;          // leaf transform (soft-body local)
-    M44 inv;        // InversedRotationTranslation of t (box, capsule)
-    float pn[3];    // plane shape: transformed plane normal
-    float pc;       //              and constant
-    float a, b, c;  // sphere {radius} | box half extent | capsule {half height, radius}
-    int32_t type;
-    float friction, restitution;
-};
+constexpr int kMaxWaves = 16;       // 1024-thread workgroups
 
 // Dynamic LDS layout:
 //   float4 X[xcount]            positions (resident: all vertices, streamed: window)
@@ -56,7 +40,7 @@ struct Leaf
 //   Leaf   leaves[max_leaves]
 //   int    collider_slot[num_colliders]
 //   float4 scratch[num_faces]   (pressure)
-//   float  red[kWaves * 32]     (reductions)
+//   float  red[kMaxWaves * 32]  (reductions)
 struct LdsLayout
 {
     uint32_t recs, lvl, leaves, slots, scratch, red, total;
@@ -71,120 +55,9 @@ __host__ __device__ inline LdsLayout lds_layout(uint32_t xcount, uint32_t nrec, 
     off = (off + 15u) & ~15u; l.leaves = off; off += max_leaves * (uint32_t)sizeof(Leaf);
     off = (off + 15u) & ~15u; l.slots = off; off += nc * 4u;
     off = (off + 15u) & ~15u; l.scratch = off; off += nf * 16u;
-    off = (off + 15u) & ~15u; l.red = off; off += kWaves * 32u * 4u;
+    off = (off + 15u) & ~15u; l.red = off; off += kMaxWaves * 32u * 4u;
     l.total = (off + 15u) & ~15u;
     return l;
-}
-
-__device__ __forceinline__ V3 xyz(float4 p) { return v3(p.x, p.y, p.z); }
-__device__ __forceinline__ float4 f4(V3 p, float w) { return make_float4(p.x, p.y, p.z, w); }
-
-// ---------------------------------------------------------------- constraints
-// ApplyEdgeConstraints body (SoftBodyMotionProperties.cpp:583-600)
-__device__ __forceinline__ void project_edge(float4 *X, uint32_t i0, uint32_t i1, float rest, float compliance, float inv_dt_sq)
-{
-    float4 p0 = X[i0], p1 = X[i1];
-    V3 x0 = xyz(p0), x1 = xyz(p1);
-    float w0 = p0.w, w1 = p1.w;
-    V3 delta = x1 - x0;
-    float length = len(delta);
-    float denom = length * ((w0 + w1) + compliance * inv_dt_sq);
-    if (denom < 1.0e-12f)
-        return;
-    V3 correction = (delta * (length - rest)) / denom;
-    X[i0] = f4(x0 + w0 * correction, w0);
-    X[i1] = f4(x1 - w1 * correction, w1);
-}
-
-// ApplyDihedralBendConstraints body (SoftBodyMotionProperties.cpp:375-455)
-__device__ __forceinline__ void project_bend(float4 *X, uint32_t i0, uint32_t i1, uint32_t i2, uint32_t i3, float compliance, float initial_angle, float inv_dt_sq)
-{
-    float4 p0 = X[i0], p1 = X[i1], p2 = X[i2], p3 = X[i3];
-    V3 x0 = xyz(p0), x1 = xyz(p1), x2 = xyz(p2), x3 = xyz(p3);
-    V3 e = x1 - x0;
-    float e_len = len(e);
-    if (e_len < 1.0e-6f)
-        return;
-    V3 x1x2 = x2 - x1;
-    V3 x1x3 = x3 - x1;
-    V3 n1 = cross(x2 - x0, x1x2);
-    V3 n2 = cross(x1x3, x3 - x0);
-    float n1_len_sq = len_sq(n1);
-    float n2_len_sq = len_sq(n2);
-    float n1n2 = n1_len_sq * n2_len_sq;
-    if (n1n2 < 1.0e-24f)
-        return;
-    float sign = sign_of(dot(cross(n2, n1), e));
-    float d = dot(n1, n2) / sqrtf(n1n2);
-    float c = sign * acos_approx(d) - initial_angle;
-    if (c > kPi)
-        c -= 2.0f * kPi;
-    else if (c < -kPi)
-        c += 2.0f * kPi;
-    n1 = n1 / n1_len_sq;
-    n2 = n2 / n2_len_sq;
-    V3 d0c = (dot(x1x2, e) * n1 + dot(x1x3, e) * n2) / e_len;
-    V3 d2c = e_len * n1;
-    V3 d3c = e_len * n2;
-    V3 d1c = (neg(d0c) - d2c) - d3c;
-    float w0 = p0.w, w1 = p1.w, w2 = p2.w, w3 = p3.w;
-    float denom = (((w0 * len_sq(d0c) + w1 * len_sq(d1c)) + w2 * len_sq(d2c)) + w3 * len_sq(d3c)) + compliance * inv_dt_sq;
-    if (denom < 1.0e-12f)
-        return;
-    float minus_lambda = c / denom;
-    X[i0] = f4(x0 - (minus_lambda * w0) * d0c, w0);
-    X[i1] = f4(x1 - (minus_lambda * w1) * d1c, w1);
-    X[i2] = f4(x2 - (minus_lambda * w2) * d2c, w2);
-    X[i3] = f4(x3 - (minus_lambda * w3) * d3c, w3);
-}
-
-// ApplyVolumeConstraints body (SoftBodyMotionProperties.cpp:467-506)
-__device__ __forceinline__ void project_volume(float4 *X, uint32_t i1, uint32_t i2, uint32_t i3, uint32_t i4, float six_rest_volume, float compliance, float inv_dt_sq)
-{
-    float4 p1 = X[i1], p2 = X[i2], p3 = X[i3], p4 = X[i4];
-    V3 x1 = xyz(p1), x2 = xyz(p2), x3 = xyz(p3), x4 = xyz(p4);
-    V3 x1x2 = x2 - x1;
-    V3 x1x3 = x3 - x1;
-    V3 x1x4 = x4 - x1;
-    float c = fabsf(dot(cross(x1x2, x1x3), x1x4)) - six_rest_volume;
-    V3 d1c = cross(x4 - x2, x3 - x2);
-    V3 d2c = cross(x1x3, x1x4);
-    V3 d3c = cross(x1x4, x1x2);
-    V3 d4c = cross(x1x2, x1x3);
-    float w1 = p1.w, w2 = p2.w, w3 = p3.w, w4 = p4.w;
-    float denom = (((w1 * len_sq(d1c) + w2 * len_sq(d2c)) + w3 * len_sq(d3c)) + w4 * len_sq(d4c)) + compliance * inv_dt_sq;
-    if (denom < 1.0e-12f)
-        return;
-    float minus_lambda = c / denom;
-    X[i1] = f4(x1 - (minus_lambda * w1) * d1c, w1);
-    X[i2] = f4(x2 - (minus_lambda * w2) * d2c, w2);
-    X[i3] = f4(x3 - (minus_lambda * w3) * d3c, w3);
-    X[i4] = f4(x4 - (minus_lambda * w4) * d4c, w4);
-}
-
-// ApplyLRAConstraints body (SoftBodyMotionProperties.cpp:685-697): only vertex 1 moves
-__device__ __forceinline__ void project_lra(float4 *X, uint32_t i0, uint32_t i1, float max_distance)
-{
-    V3 x0 = xyz(X[i0]);
-    float4 p1 = X[i1];
-    V3 delta = xyz(p1) - x0;
-    float delta_len_sq = len_sq(delta);
-    if (delta_len_sq > max_distance * max_distance)
-        X[i1] = f4(x0 + (delta * max_distance) / sqrtf(delta_len_sq), p1.w);
-}
-
-__device__ __forceinline__ void project_record(float4 *X, uint4 r, float inv_dt_sq)
-{
-    uint32_t type = (r.x >> 14) & 3u;
-    uint32_t i0 = r.x & 0x3fffu, i1 = r.x >> 16, i2 = r.y & 0xffffu, i3 = r.y >> 16;
-    float p0 = __uint_as_float(r.z), p1 = __uint_as_float(r.w);
-    switch (type)
-    {
-    case kTypeEdge: project_edge(X, i0, i1, p0, p1, inv_dt_sq); break;
-    case kTypeBend: project_bend(X, i0, i1, i2, i3, p0, p1, inv_dt_sq); break;
-    case kTypeVolume: project_volume(X, i0, i1, i2, i3, p0, p1, inv_dt_sq); break;
-    default: project_lra(X, i0, i1, p0); break;
-    }
 }
 
 // Runs `nlev` levels on the window X.  Records of one level touch disjoint
@@ -192,217 +65,21 @@ __device__ __forceinline__ void project_record(float4 *X, uint4 r, float inv_dt_
 // separates consecutive levels.
 //
 // STAGED: records and level offsets are in LDS.  Otherwise they stream from
-// global memory and the records of level L+1 are prefetched into registers
-// while level L runs.
-template <bool STAGED>
+// global memory: each thread holds its record of the next two levels in
+// registers (issued two levels ahead), so the load latency overlaps two levels
+// of projection.
+template <bool STAGED, int BLOCK>
 __device__ __forceinline__ void run_levels(const uint4 *recs, const uint32_t *lvl, uint32_t nlev, float4 *X, float inv_dt_sq, int tid)
 {
-    if (nlev == 0)
-        return;
     if (STAGED)
-    {
-        uint32_t begin = lvl[0];
-        for (uint32_t L = 0; L < nlev; ++L)
-        {
-            uint32_t end = lvl[L + 1];
-            for (uint32_t i = begin + tid; i < end; i += kBlock)
-                project_record(X, recs[i], inv_dt_sq);
-            __syncthreads();
-            begin = end;
-        }
-        return;
-    }
-    const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
-    uint32_t b = lvl[0], e = lvl[1];
-    uint4 r0 = b + tid < e ? recs[b + tid] : zero;
-    uint4 r1 = b + tid + kBlock < e ? recs[b + tid + kBlock] : zero;
-    for (uint32_t L = 0; L < nlev; ++L)
-    {
-        uint32_t nb = e, ne = L + 1 < nlev ? lvl[L + 2] : e;
-        uint4 n0 = nb + tid < ne ? recs[nb + tid] : zero;
-        uint4 n1 = nb + tid + kBlock < ne ? recs[nb + tid + kBlock] : zero;
-        if (b + tid < e)
-            project_record(X, r0, inv_dt_sq);
-        if (b + tid + kBlock < e)
-            project_record(X, r1, inv_dt_sq);
-        for (uint32_t i = b + tid + 2 * kBlock; i < e; i += kBlock)
-            project_record(X, recs[i], inv_dt_sq);
-        __syncthreads();
-        r0 = n0; r1 = n1; b = nb; e = ne;
-    }
-}
-
-// ---------------------------------------------------------------- collision
-// Shape::CollideSoftBodyVertices for plane / sphere / box / capsule with the
-// argmax rule of CollideSoftBodyVertexIterator::UpdatePenetration
-// (CollideSoftBodyVertexIterator.h:83-97): a leaf replaces the stored plane
-// unless the stored penetration is >= the new one.
-__device__ __forceinline__ void collide_vertex(const Leaf *leaves, uint32_t num_leaves, V3 x, float &largest, int32_t &idx, Pl &plane)
-{
-    for (uint32_t l = 0; l < num_leaves; ++l)
-    {
-        const Leaf &L = leaves[l];
-        switch (L.type)
-        {
-        case 0: // PlaneShape.cpp:243-256
-            {
-                Pl p; p.n = v3(L.pn[0], L.pn[1], L.pn[2]); p.c = L.pc;
-                float pen = -signed_distance(p, x);
-                if (!(largest >= pen)) { largest = pen; plane = p; idx = (int32_t)l; }
-            }
-            break;
-        case 1: // SphereShape.cpp:279-301
-            {
-                V3 center = col3(L.t, 3);
-                float radius = L.a;
-                V3 delta = x - center;
-                float distance = len(delta);
-                float pen = radius - distance;
-                if (!(largest >= pen))
-                {
-                    largest = pen;
-                    V3 normal = distance > 0.0f ? delta / distance : v3(0.0f, 1.0f, 0.0f);
-                    V3 point = center + radius * normal;
-                    plane = plane_from_point_normal(point, normal);
-                    idx = (int32_t)l;
-                }
-            }
-            break;
-        case 2: // BoxShape.cpp:234-283
-            {
-                V3 he = v3(L.a, L.b, L.c);
-                V3 local = mul_point(L.inv, x);
-                V3 clamped = vmax(vmin(local, he), neg(he));
-                if (clamped.x == local.x && clamped.y == local.y && clamped.z == local.z)
-                {
-                    V3 delta = he - vabs(local);
-                    int index = lowest_index(delta);
-                    float pen = comp(delta, index);
-                    if (!(largest >= pen))
-                    {
-                        largest = pen;
-                        V3 axis = v3(index == 0 ? 1.0f : 0.0f, index == 1 ? 1.0f : 0.0f, index == 2 ? 1.0f : 0.0f);
-                        V3 normal = mulv(vsign(local), axis);
-                        V3 point = mulv(normal, he);
-                        plane = plane_transformed(plane_from_point_normal(point, normal), L.t);
-                        idx = (int32_t)l;
-                    }
-                }
-                else
-                {
-                    V3 normal = local - clamped;
-                    float normal_length = len(normal);
-                    float pen = -normal_length;
-                    if (!(largest >= pen))
-                    {
-                        largest = pen;
-                        normal = normal / normal_length;
-                        plane = plane_transformed(plane_from_point_normal(clamped, normal), L.t);
-                        idx = (int32_t)l;
-                    }
-                }
-            }
-            break;
-        default: // CapsuleShape.cpp:324-375
-            {
-                float hh = L.a, radius = L.b;
-                V3 local = mul_point(L.inv, x);
-                if (fabsf(local.y) <= hh)
-                {
-                    V3 normal = v3(local.x, 0.0f, local.z);
-                    float normal_length = len(normal);
-                    float pen = radius - normal_length;
-                    if (!(largest >= pen))
-                    {
-                        largest = pen;
-                        normal = normal_length > 0.0f ? normal / normal_length : v3(1.0f, 0.0f, 0.0f);
-                        V3 point = radius * normal;
-                        plane = plane_transformed(plane_from_point_normal(point, normal), L.t);
-                        idx = (int32_t)l;
-                    }
-                }
-                else
-                {
-                    V3 center = v3(0.0f, sign_of(local.y) * hh, 0.0f);
-                    V3 delta = local - center;
-                    float distance = len(delta);
-                    float pen = radius - distance;
-                    if (!(largest >= pen))
-                    {
-                        largest = pen;
-                        V3 normal = delta / distance;
-                        V3 point = center + radius * normal;
-                        plane = plane_transformed(plane_from_point_normal(point, normal), L.t);
-                        idx = (int32_t)l;
-                    }
-                }
-            }
-            break;
-        }
-    }
-}
-
-// Leaf of collider c in the soft body's local space (SoftBodyMotionProperties.cpp:174-186:
-// com = (inverse soft body transform * collider transform), then the leaf
-// transform is rebuilt from com's quaternion and translation).
-__device__ __forceinline__ void make_leaf(const ColliderDev &c, const M44 &inv_com, float body_friction, float body_restitution, Leaf &out)
-{
-    M44 ccom = rotation_translation(c.rot[0], c.rot[1], c.rot[2], c.rot[3], v3(c.pos[0], c.pos[1], c.pos[2]));
-    M44 rel = mul(inv_com, ccom);
-    float q[4];
-    get_quaternion(rel, q);
-    M44 t = rotation_translation(q[0], q[1], q[2], q[3], col3(rel, 3));
-    out.t = t;
-    out.type = c.type;
-    out.friction = sqrtf(body_friction * c.friction);                                 // ContactConstraintManager.h:554
-    out.restitution = body_restitution > c.restitution ? body_restitution : c.restitution; // ContactConstraintManager.h:555
-    switch (c.type)
-    {
-    case 0:
-        {
-            // Plane::Scaled with unit scale (Plane.h:49-54) then GetTransformed
-            V3 n = v3(c.params[0], c.params[1], c.params[2]);
-            V3 sn = v3(n.x / 1.0f, n.y / 1.0f, n.z / 1.0f);
-            float snl = len(sn);
-            Pl p; p.n = sn / snl; p.c = c.params[3] / snl;
-            Pl tp = plane_transformed(p, t);
-            out.pn[0] = tp.n.x; out.pn[1] = tp.n.y; out.pn[2] = tp.n.z; out.pc = tp.c;
-        }
-        break;
-    case 1:
-        out.a = fabsf(1.0f) * c.params[0];
-        break;
-    case 2:
-        out.inv = inversed_rotation_translation(t);
-        out.a = 1.0f * c.params[0]; out.b = 1.0f * c.params[1]; out.c = 1.0f * c.params[2];
-        break;
-    default:
-        out.inv = inversed_rotation_translation(t);
-        out.a = 1.0f * c.params[0]; out.b = 1.0f * c.params[1];
-        break;
-    }
-}
-
-// ---------------------------------------------------------------- reductions
-__device__ __forceinline__ float wave_min(float v)
-{
-    for (int o = 32; o > 0; o >>= 1) { float u = __shfl_xor(v, o, kWave); v = u < v ? u : v; }
-    return v;
-}
-__device__ __forceinline__ float wave_max(float v)
-{
-    for (int o = 32; o > 0; o >>= 1) { float u = __shfl_xor(v, o, kWave); v = u > v ? u : v; }
-    return v;
-}
-__device__ __forceinline__ float wave_sum(float v)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+        run_levels_lds<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, (uint32_t)tid, NoSkip());
+    else
+        run_levels_global<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, (uint32_t)tid, NoSkip());
 }
 
 // ---------------------------------------------------------------- kernel
 // Advances each body of `body_list` by `frames` x `steps` collision steps.
-template <bool RESIDENT, bool STAGED>
+template <bool RESIDENT, bool STAGED, int kBlock>
 __global__ __launch_bounds__(kBlock) void update_kernel(
     const BodyDev *__restrict__ bodies, BodyStateDev *__restrict__ states, const uint32_t *__restrict__ body_list,
     const ColliderDev *__restrict__ colliders, uint32_t num_colliders, uint32_t max_leaves,
@@ -633,7 +310,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                             X[j] = B.x[S.vlist[W.vbegin + j]];
                         __syncthreads();
                     }
-                    run_levels<STAGED>(recs, lvl + W.lbegin, W.nlev, X, inv_dt_sq, tid);
+                    run_levels<STAGED, kBlock>(recs, lvl + W.lbegin, W.nlev, X, inv_dt_sq, tid);
                     if (!RESIDENT && W.vcount)
                     {
                         for (uint32_t j = tid; j < W.vcount; j += kBlock)
@@ -740,7 +417,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                 __syncthreads();
                 if (tid == 0)
                 {
-                    for (int w = 1; w < kWaves; ++w)
+                    for (int w = 1; w < kBlock / kWave; ++w)
                     {
 #pragma unroll
                         for (int k = 0; k < 6; ++k) vals[k] = smin(vals[k], red[w * 32 + k]);
@@ -818,7 +495,7 @@ uint32_t kernel_lds_bytes(bool resident, bool staged, const SettingsDev &s, uint
                       staged ? s.num_levels_total : 0u, max_leaves, nc, nf).total;
 }
 
-template <bool R, bool S>
+template <bool R, bool S, int BLOCK>
 static hipError_t launch_variant(const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list, uint32_t num_blocks,
                                  const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves, uint32_t lds_bytes,
                                  float gx, float gy, float gz, float dt, int32_t steps, int32_t frames, int32_t *error_flag,
@@ -827,16 +504,18 @@ static hipError_t launch_variant(const BodyDev *bodies, BodyStateDev *states, co
     static bool configured = false;
     if (!configured)
     {
-        hipError_t e = hipFuncSetAttribute((const void *)update_kernel<R, S>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+        hipError_t e = hipFuncSetAttribute((const void *)update_kernel<R, S, BLOCK>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
         if (e != hipSuccess)
             return e;
         configured = true;
     }
-    hipLaunchKernelGGL((update_kernel<R, S>), dim3(num_blocks), dim3(kBlock), lds_bytes, stream,
+    hipLaunchKernelGGL((update_kernel<R, S, BLOCK>), dim3(num_blocks), dim3(BLOCK), lds_bytes, stream,
                        bodies, states, body_list, colliders, num_colliders, max_leaves, gx, gy, gz, dt, steps, frames, error_flag);
     return hipGetLastError();
 }
 
+// Resident single-body launches use 256-thread workgroups; streamed bodies use
+// 1024 threads so one pass covers the wide merged levels of an 8192-vertex window.
 hipError_t launch_update(bool resident, bool staged, const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list,
                          uint32_t num_blocks, const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves,
                          uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
@@ -845,10 +524,10 @@ hipError_t launch_update(bool resident, bool staged, const BodyDev *bodies, Body
     if (num_blocks == 0)
         return hipSuccess;
     if (resident && staged)
-        return launch_variant<true, true>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
+        return launch_variant<true, true, 256>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
     if (resident)
-        return launch_variant<true, false>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
-    return launch_variant<false, false>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
+        return launch_variant<true, false, 256>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
+    return launch_variant<false, false, 1024>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
 }
 
 } // namespace xpbd

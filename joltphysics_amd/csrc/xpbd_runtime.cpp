@@ -17,6 +17,11 @@
 
 namespace xpbd {
 uint32_t kernel_lds_bytes(bool resident, bool staged, const SettingsDev &s, uint32_t max_leaves, uint32_t nc, uint32_t nf);
+uint32_t bundle_lds_bytes(uint32_t nv, uint32_t nrec, uint32_t nlvl, uint32_t nb);
+uint32_t bundle_leaf_bytes();
+hipError_t launch_bundles(bool staged, const BundleDev *bundles, uint32_t num_bundles, const BodyDev *bodies, BodyStateDev *states,
+                          const ColliderDev *colliders, uint32_t num_colliders, uint32_t lds_bytes,
+                          float gx, float gy, float gz, float dt, int32_t steps, int32_t frames, hipStream_t stream);
 hipError_t launch_update(bool resident, bool staged, const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list,
                          uint32_t num_blocks, const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves,
                          uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
@@ -43,6 +48,7 @@ int Fail(int code, const std::string &msg)
     } while (0)
 
 constexpr uint32_t kMaxLdsBytes = 160 * 1024 - 1024;
+constexpr uint32_t kNopRecordHost = 0xffffffffu;    // padding record (skipped by the kernels)
 
 // Reference record layouts (SoftBodySharedSettings.h)
 struct VertexRec { float pos[3], vel[3], inv_mass; };
@@ -94,16 +100,36 @@ struct xpbd_settings
     DevBuf<uint32_t> faces, vf_begin, vf_list;
     SettingsDev dev{};
     uint32_t stats[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+    // host copy of a resident body's level sequence (merged parallel levels,
+    // then the non-parallel group's levels): records with body vertex indices
+    std::vector<uint4> host_recs;
+    std::vector<uint32_t> host_level_begin;     // nlev + 1 offsets into host_recs
 };
 
 struct Body
 {
     xpbd_settings *settings = nullptr;
     uint32_t nv = 0;
+    bool owns_storage = true;       // false once the state lives in a bundle's arrays
     DevBuf<float4> x, v, prev, plane;
     DevBuf<int32_t> idx, contact;
     BodyDev dev{};
     BodyStateDev state{};
+};
+
+struct BundleHost
+{
+    std::vector<uint32_t> bodies;
+    uint32_t nv = 0;
+    bool staged = false;
+    uint32_t lds = 0;
+    DevBuf<uint4> recs;
+    DevBuf<uint32_t> lvl;
+    DevBuf<BundleBodyDev> table;
+    DevBuf<float4> x, v, prev, plane, face_scratch;
+    DevBuf<int32_t> idx, contact;
+    DevBuf<unsigned char> leaves_full, leaves;
+    BundleDev dev{};
 };
 
 struct xpbd_context
@@ -126,6 +152,26 @@ struct xpbd_context
     int32_t profiled_launches = 0;
     std::vector<uint32_t> host_list;   // sources of the asynchronous table uploads
     std::vector<BodyDev> host_devs;
+    // bundles of LDS-resident bodies and the streamed bodies
+    bool layout_dirty = true;
+    uint32_t layout_colliders = 0;
+    std::vector<std::unique_ptr<BundleHost>> bundles;
+    std::vector<uint32_t> streamed;
+    struct BundleLaunch
+    {
+        bool staged = false;
+        uint32_t lds = 0;
+        DevBuf<BundleDev> table;
+        uint32_t count = 0;
+        double cost = 0.0;      // estimated duration (level passes per step, see LaunchCost)
+    };
+    std::vector<std::unique_ptr<BundleLaunch>> bundle_launches;
+    // The context stream plus three auxiliary streams: one per hardware queue
+    // of a process (a fifth stream would share a queue and serialise behind it).
+    static constexpr int kAux = 3;
+    hipStream_t aux[kAux] = { nullptr, nullptr, nullptr };
+    hipEvent_t fork = nullptr, join[kAux] = { nullptr, nullptr, nullptr };
+    bool use_bundles = true;
 };
 
 namespace {
@@ -144,7 +190,7 @@ struct Item { uint32_t type; uint32_t v[4]; uint32_t nv; float p0, p1; };
 // Bodies up to this many vertices keep all positions in LDS for the launch.
 constexpr uint32_t kResidentMaxVertices = 8192;
 // Vertex capacity of one gathered window of a streamed body.
-constexpr uint32_t kWindowCapacity = 4096;
+constexpr uint32_t kWindowCapacity = 8192;
 
 std::vector<std::vector<uint32_t>> LevelGroup(const std::vector<Item> &seq, std::vector<uint32_t> &last_write, std::vector<uint32_t> &last_read)
 {
@@ -200,12 +246,39 @@ struct GroupLevels
     std::vector<std::vector<uint32_t>> levels;
 };
 
+// Appends one level: records grouped by constraint type (bend, volume, edge,
+// LRA), each group padded with no-op records to whole wavefronts when that does
+// not add a pass of the 256-thread workgroup, so a wavefront runs one
+// constraint path.  `pad` false keeps the groups unpadded.
+void AppendTypedLevel(std::vector<uint4> bucket[4], std::vector<uint4> &recs, bool allow_pad = true)
+{
+    size_t n = 0, padded = 0;
+    for (int t = 0; t < 4; ++t)
+    {
+        n += bucket[t].size();
+        padded += (bucket[t].size() + 63) / 64 * 64;
+    }
+    const bool pad = allow_pad && (padded + 255) / 256 <= (n + 255) / 256;
+    const size_t base = recs.size();
+    const uint4 nop = make_uint4(0xffffffffu, 0u, 0u, 0u);
+    const int order[4] = { (int)kTypeBend, (int)kTypeVolume, (int)kTypeEdge, (int)kTypeLRA };
+    for (int t : order)
+    {
+        recs.insert(recs.end(), bucket[t].begin(), bucket[t].end());
+        if (pad)
+            while ((recs.size() - base) % 64 != 0)
+                recs.push_back(nop);
+    }
+}
+
 struct ProgramBuilder
 {
     std::vector<uint4> recs;
     std::vector<uint32_t> lvl;
     std::vector<WindowDev> windows;
     std::vector<uint32_t> vlist;
+
+    void AppendLevel(const std::vector<std::pair<const struct GroupLevels *, uint32_t>> &lv, const uint32_t *local);
 
     // Appends one window: levels[L] = list of (group, item) pairs; local maps
     // body vertex -> window slot (null: identity window).
@@ -222,13 +295,23 @@ struct ProgramBuilder
         for (const auto &lv : levels)
         {
             lvl.push_back((uint32_t)recs.size());
-            for (const auto &gi : lv)
-                recs.push_back(Pack(gi.first->seq[gi.second], local));
+            AppendLevel(lv, local);
         }
         lvl.push_back((uint32_t)recs.size());
         windows.push_back(w);
     }
 };
+
+void ProgramBuilder::AppendLevel(const std::vector<std::pair<const GroupLevels *, uint32_t>> &lv, const uint32_t *local)
+{
+    std::vector<uint4> bucket[4];
+    for (const auto &gi : lv)
+    {
+        const Item &it = gi.first->seq[gi.second];
+        bucket[it.type].push_back(Pack(it, local));
+    }
+    AppendTypedLevel(bucket, recs);
+}
 
 int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
 {
@@ -422,6 +505,16 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
         s.dev.num_tail_windows = ntw;
     }
 
+    if (resident)
+    {
+        s.host_recs = pb.recs;
+        s.host_level_begin.clear();
+        for (const WindowDev &w : pb.windows)
+            for (uint32_t L = 0; L < w.nlev; ++L)
+                s.host_level_begin.push_back(pb.lvl[w.lbegin + L]);
+        s.host_level_begin.push_back((uint32_t)pb.recs.size());
+    }
+
     s.stats[0] = npar;
     s.stats[1] = max_par_levels;
     s.stats[2] = (uint32_t)tail.levels.size();
@@ -481,22 +574,229 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
     return XPBD_OK;
 }
 
-int SyncTables(xpbd_context *ctx)
+// ------------------------------------------------------------------ bundles
+// Bodies whose positions fit in LDS are packed into bundles that share one
+// workgroup.  Bodies are sorted by the length of their level sequence so a
+// bundle holds bodies of similar depth, then packed greedily up to a vertex
+// capacity.  The bundle's program merges the bodies' level sequences by index;
+// inside a merged level the records are grouped by constraint type and each
+// group is padded to whole wavefronts when that does not add a pass.
+constexpr uint32_t kBundleMaxBodies = 32;
+constexpr uint32_t kBundleBlock = 256;
+
+uint32_t BundleCapacity()
 {
-    if (!ctx->tables_dirty)
-        return XPBD_OK;
-    std::vector<BodyDev> devs;
-    std::vector<BodyStateDev> states;
-    for (auto &b : ctx->bodies)
+    const char *e = getenv("XPBD_BUNDLE_VERTICES");
+    uint32_t v = e ? (uint32_t)atoi(e) : 2688u;
+    return std::max(64u, std::min(v, kResidentMaxVertices));
+}
+
+uint4 OffsetRecord(uint4 r, uint32_t voff)
+{
+    uint32_t type = (r.x >> 14) & 3u;
+    uint32_t v0 = (r.x & 0x3fffu) + voff, v1 = (r.x >> 16) + voff;
+    uint32_t v2 = (r.y & 0xffffu) + voff, v3 = (r.y >> 16) + voff;
+    uint4 o;
+    o.x = (v0 & 0x3fffu) | (type << 14) | (v1 << 16);
+    o.y = (v2 & 0xffffu) | (v3 << 16);
+    o.z = r.z;
+    o.w = r.w;
+    return o;
+}
+
+void BuildBundleProgram(const xpbd_context *ctx, const BundleHost &bh, const std::vector<uint32_t> &voffs,
+                        std::vector<uint4> &recs, std::vector<uint32_t> &lvl)
+{
+    size_t K = 0;
+    for (uint32_t bi : bh.bodies)
+        K = std::max(K, ctx->bodies[bi]->settings->host_level_begin.size() - 1);
+    std::vector<uint4> bucket[4];
+    const uint4 nop = make_uint4(kNopRecordHost, 0u, 0u, 0u);
+    for (size_t k = 0; k < K; ++k)
     {
-        devs.push_back(b->dev);
-        states.push_back(b->state);
+        for (auto &b : bucket) b.clear();
+        for (size_t j = 0; j < bh.bodies.size(); ++j)
+        {
+            const xpbd_settings &st = *ctx->bodies[bh.bodies[j]]->settings;
+            if (k + 1 >= st.host_level_begin.size())
+                continue;
+            for (uint32_t r = st.host_level_begin[k]; r < st.host_level_begin[k + 1]; ++r)
+            {
+                if (st.host_recs[r].x == kNopRecordHost)
+                    continue;
+                uint4 rec = OffsetRecord(st.host_recs[r], voffs[j]);
+                bucket[(rec.x >> 14) & 3u].push_back(rec);
+            }
+        }
+        lvl.push_back((uint32_t)recs.size());
+        AppendTypedLevel(bucket, recs);
     }
-    // states already on the device are kept (they are pulled before adding bodies)
-    HIP_TRY(ctx->d_bodies.upload(devs));
-    HIP_TRY(ctx->d_states.upload(states));
-    ctx->tables_dirty = false;
-    ctx->states_on_device = true;
+    lvl.push_back((uint32_t)recs.size());
+}
+
+// Estimated duration of a launch in level passes per step: its workgroups are
+// latency bound (one level pass after another), so a launch takes as long as
+// its deepest workgroup, or longer when its workgroups do not all fit on the
+// chip at once (workgroups per CU are set by the LDS each one holds).
+double LaunchCost(const std::vector<double> &work, uint32_t lds)
+{
+    const double per_cu = std::max(1u, kMaxLdsBytes / std::max(lds, 1u));
+    const double slots = 256.0 * per_cu;
+    double total = 0.0, deepest = 0.0;
+    for (double w : work)
+    {
+        total += w;
+        deepest = std::max(deepest, w);
+    }
+    return std::max(deepest, total / slots);
+}
+
+int BuildLayout(xpbd_context *ctx)
+{
+    const uint32_t nc = (uint32_t)ctx->colliders.size();
+    const uint32_t cap = BundleCapacity();
+    std::vector<uint32_t> resident;
+    ctx->streamed.clear();
+    for (uint32_t i = 0; i < ctx->bodies.size(); ++i)
+    {
+        const Body &b = *ctx->bodies[i];
+        if (ctx->use_bundles && b.settings->dev.resident && !b.settings->host_level_begin.empty())
+            resident.push_back(i);
+        else
+            ctx->streamed.push_back(i);
+    }
+    // deepest level sequences first; bodies of a bundle share the iteration count
+    std::stable_sort(resident.begin(), resident.end(), [ctx](uint32_t a, uint32_t b) {
+        const Body &A = *ctx->bodies[a], &B = *ctx->bodies[b];
+        if (A.dev.iterations != B.dev.iterations)
+            return A.dev.iterations < B.dev.iterations;
+        return A.settings->host_level_begin.size() > B.settings->host_level_begin.size();
+    });
+    std::vector<std::unique_ptr<BundleHost>> bundles;
+    for (uint32_t bi : resident)
+    {
+        const Body &b = *ctx->bodies[bi];
+        BundleHost *cur = bundles.empty() ? nullptr : bundles.back().get();
+        if (!cur || cur->bodies.size() >= kBundleMaxBodies || cur->nv + b.nv > std::max(cap, b.nv)
+            || ctx->bodies[cur->bodies[0]]->dev.iterations != b.dev.iterations || (cur->nv > 0 && cur->nv + b.nv > cap))
+        {
+            bundles.emplace_back(new BundleHost);
+            cur = bundles.back().get();
+        }
+        cur->bodies.push_back(bi);
+        cur->nv += b.nv;
+    }
+
+    std::vector<std::unique_ptr<xpbd_context::BundleLaunch>> launches;
+    std::vector<std::vector<BundleDev>> launch_tables;
+    std::vector<std::vector<double>> launch_work;
+    // LDS bins: 4, 3, 2 and 1 workgroups per CU
+    const uint32_t bins[] = { 40u * 1024u, 54u * 1024u, 80u * 1024u, kMaxLdsBytes };
+    for (auto &bp : bundles)
+    {
+        BundleHost &bh = *bp;
+        const uint32_t nb = (uint32_t)bh.bodies.size();
+        std::vector<uint32_t> voffs(nb);
+        std::vector<BundleBodyDev> table(nb);
+        uint32_t voff = 0, foff = 0;
+        bool has_pressure = false, has_sleepers = false;
+        for (uint32_t j = 0; j < nb; ++j)
+        {
+            const Body &b = *ctx->bodies[bh.bodies[j]];
+            voffs[j] = voff;
+            table[j] = { bh.bodies[j], voff, b.nv, foff };
+            voff += b.nv;
+            if (b.dev.pressure > 0.0f && b.settings->num_faces > 0)
+            {
+                foff += b.settings->num_faces;
+                has_pressure = true;
+            }
+            has_sleepers |= b.dev.allow_sleeping != 0;
+        }
+        if (voff >= kMaxWindowVertices)
+            return Fail(XPBD_E_CAPACITY, "bundle exceeds the LDS window index range");
+        std::vector<uint4> recs;
+        std::vector<uint32_t> lvl;
+        BuildBundleProgram(ctx, bh, voffs, recs, lvl);
+        HIP_TRY(bh.recs.upload(recs));
+        HIP_TRY(bh.lvl.upload(lvl));
+        HIP_TRY(bh.table.upload(table));
+        // bundle-contiguous vertex state, gathered from wherever each body's state lives now
+        HIP_TRY(bh.x.alloc(voff)); HIP_TRY(bh.v.alloc(voff)); HIP_TRY(bh.prev.alloc(voff)); HIP_TRY(bh.plane.alloc(voff));
+        HIP_TRY(bh.idx.alloc(voff)); HIP_TRY(bh.contact.alloc(voff));
+        for (uint32_t j = 0; j < nb; ++j)
+        {
+            Body &b = *ctx->bodies[bh.bodies[j]];
+            const size_t o = voffs[j], n = b.nv;
+            HIP_TRY(hipMemcpy(bh.x.p + o, b.dev.x, n * sizeof(float4), hipMemcpyDeviceToDevice));
+            HIP_TRY(hipMemcpy(bh.v.p + o, b.dev.v, n * sizeof(float4), hipMemcpyDeviceToDevice));
+            HIP_TRY(hipMemcpy(bh.prev.p + o, b.dev.prev, n * sizeof(float4), hipMemcpyDeviceToDevice));
+            HIP_TRY(hipMemcpy(bh.plane.p + o, b.dev.plane, n * sizeof(float4), hipMemcpyDeviceToDevice));
+            HIP_TRY(hipMemcpy(bh.idx.p + o, b.dev.idx, n * sizeof(int32_t), hipMemcpyDeviceToDevice));
+            HIP_TRY(hipMemcpy(bh.contact.p + o, b.dev.contact, n * sizeof(int32_t), hipMemcpyDeviceToDevice));
+        }
+        HIP_TRY(bh.leaves_full.alloc((size_t)std::max(1u, nb * nc) * bundle_leaf_bytes()));
+        HIP_TRY(bh.leaves.alloc((size_t)std::max(1u, nb * nc) * bundle_leaf_bytes()));
+        HIP_TRY(bh.face_scratch.alloc(std::max(1u, foff)));
+        bh.nv = voff;
+        BundleDev &d = bh.dev;
+        d.recs = bh.recs.p; d.lvl = bh.lvl.p; d.nlev = (uint32_t)lvl.size() - 1; d.num_records = (uint32_t)recs.size();
+        d.bodies = bh.table.p; d.num_bodies = nb; d.nv = voff;
+        d.x = bh.x.p; d.v = bh.v.p; d.prev = bh.prev.p; d.plane = bh.plane.p; d.idx = bh.idx.p; d.contact = bh.contact.p;
+        d.leaves_full = bh.leaves_full.p; d.leaves = bh.leaves.p; d.face_scratch = bh.face_scratch.p;
+        d.has_pressure = has_pressure ? 1u : 0u;
+        d.has_sleepers = has_sleepers ? 1u : 0u;
+        d.iterations = ctx->bodies[bh.bodies[0]]->dev.iterations;
+        const uint32_t need_staged = bundle_lds_bytes(voff, d.num_records, d.nlev + 1, nb);
+        const uint32_t need_plain = bundle_lds_bytes(voff, 0, 0, nb);
+        if (need_plain > kMaxLdsBytes)
+            return Fail(XPBD_E_CAPACITY, "bundle needs more LDS than a workgroup has");
+        bh.staged = need_staged <= 40u * 1024u;
+        d.staged = bh.staged ? 1u : 0u;
+        bh.lds = bh.staged ? need_staged : need_plain;
+        uint32_t bin = 0;
+        while (bins[bin] < bh.lds) ++bin;
+        size_t li = 0;
+        for (; li < launches.size(); ++li)
+            if (launches[li]->staged == bh.staged && launches[li]->lds == bins[bin])
+                break;
+        if (li == launches.size())
+        {
+            launches.emplace_back(new xpbd_context::BundleLaunch);
+            launches.back()->staged = bh.staged;
+            launches.back()->lds = bins[bin];
+            launch_tables.emplace_back();
+            launch_work.emplace_back();
+        }
+        launch_tables[li].push_back(d);
+        launch_work[li].push_back((double)d.nlev * (double)d.iterations);
+    }
+    // bodies now live in bundle storage: repoint them and drop their own buffers
+    for (auto &bp : bundles)
+        for (uint32_t j = 0; j < bp->bodies.size(); ++j)
+        {
+            Body &b = *ctx->bodies[bp->bodies[j]];
+            uint32_t o = 0;
+            for (uint32_t k = 0; k < j; ++k) o += ctx->bodies[bp->bodies[k]]->nv;
+            b.dev.x = bp->x.p + o; b.dev.v = bp->v.p + o; b.dev.prev = bp->prev.p + o; b.dev.plane = bp->plane.p + o;
+            b.dev.idx = bp->idx.p + o; b.dev.contact = bp->contact.p + o;
+            if (b.owns_storage)
+            {
+                (void)b.x.alloc(0); (void)b.v.alloc(0); (void)b.prev.alloc(0); (void)b.plane.alloc(0); (void)b.idx.alloc(0); (void)b.contact.alloc(0);
+                b.owns_storage = false;
+            }
+        }
+    ctx->bundles.swap(bundles);      // old bundles (if any) are freed here, after the copies
+    for (size_t li = 0; li < launches.size(); ++li)
+    {
+        HIP_TRY(launches[li]->table.upload(launch_tables[li]));
+        launches[li]->count = (uint32_t)launch_tables[li].size();
+        launches[li]->cost = LaunchCost(launch_work[li], launches[li]->lds);
+    }
+    ctx->bundle_launches.swap(launches);
+    ctx->layout_dirty = false;
+    ctx->layout_colliders = nc;
+    ctx->tables_dirty = true;
     return XPBD_OK;
 }
 
@@ -509,6 +809,28 @@ int PullStates(xpbd_context *ctx)
     HIP_TRY(hipMemcpy(states.data(), ctx->d_states.p, sizeof(BodyStateDev) * states.size(), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < states.size() && i < ctx->bodies.size(); ++i)
         ctx->bodies[i]->state = states[i];
+    return XPBD_OK;
+}
+
+int SyncTables(xpbd_context *ctx)
+{
+    if (!ctx->tables_dirty)
+        return XPBD_OK;
+    // keep the device-side body states: pull them before re-uploading the tables
+    int rc = PullStates(ctx);
+    if (rc != XPBD_OK)
+        return rc;
+    std::vector<BodyDev> devs;
+    std::vector<BodyStateDev> states;
+    for (auto &b : ctx->bodies)
+    {
+        devs.push_back(b->dev);
+        states.push_back(b->state);
+    }
+    HIP_TRY(ctx->d_bodies.upload(devs));
+    HIP_TRY(ctx->d_states.upload(states));
+    ctx->tables_dirty = false;
+    ctx->states_on_device = true;
     return XPBD_OK;
 }
 
@@ -533,6 +855,16 @@ int xpbd_context_create(int device, xpbd_context **out_ctx)
     std::unique_ptr<xpbd_context> ctx(new xpbd_context);
     ctx->device = device;
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
+    for (int a = 0; a < xpbd_context::kAux; ++a)
+    {
+        HIP_TRY(hipStreamCreateWithFlags(&ctx->aux[a], hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&ctx->join[a], hipEventDisableTiming));
+    }
+    {
+        const char *e = getenv("XPBD_BUNDLES");
+        ctx->use_bundles = !(e && e[0] == '0');
+    }
     HIP_TRY(ctx->d_error.alloc(1));
     HIP_TRY(hipMemset(ctx->d_error.p, 0, sizeof(int32_t)));
     *out_ctx = ctx.release();
@@ -550,7 +882,15 @@ int xpbd_context_destroy(xpbd_context *ctx)
         (void)hipEventDestroy(ev.first);
         (void)hipEventDestroy(ev.second);
     }
+    ctx->bundles.clear();
     ctx->bodies.clear();
+    ctx->bundle_launches.clear();
+    (void)hipEventDestroy(ctx->fork);
+    for (int a = 0; a < xpbd_context::kAux; ++a)
+    {
+        (void)hipEventDestroy(ctx->join[a]);
+        (void)hipStreamDestroy(ctx->aux[a]);
+    }
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return XPBD_OK;
@@ -666,6 +1006,7 @@ int xpbd_body_create(xpbd_context *ctx, const xpbd_body_desc *d, int32_t *out_bo
     s->refcount++;
     ctx->bodies.push_back(std::move(b));
     ctx->tables_dirty = true;
+    ctx->layout_dirty = true;
     *out_body = (int32_t)(ctx->bodies.size() - 1);
     return XPBD_OK;
 }
@@ -689,6 +1030,8 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
         memcpy(v[i].bounds, c[i].world_bounds, sizeof(v[i].bounds));
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (v.size() != ctx->colliders.size())
+        ctx->layout_dirty = true;
     ctx->colliders = v;
     HIP_TRY(ctx->d_colliders.upload(v));
     return XPBD_OK;
@@ -711,19 +1054,22 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
     if (frames == 0 || ctx->bodies.empty())
         return XPBD_OK;
     HIP_TRY(hipSetDevice(ctx->device));
-    int rc = SyncTables(ctx);
-    if (rc != XPBD_OK)
-        return rc;
-
     const uint32_t nc = (uint32_t)ctx->colliders.size();
-    // Launch classes: (resident, staged) kernel variant x LDS size bin, so that
-    // small bodies are not given the LDS reservation of the largest one.
+    int rc;
+    if (ctx->layout_dirty || ctx->layout_colliders != nc)
+    {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if ((rc = PullStates(ctx)) != XPBD_OK)
+            return rc;
+        if ((rc = BuildLayout(ctx)) != XPBD_OK)
+            return rc;
+    }
+
+    // Streamed bodies: launch classes (kernel variant x LDS size bin)
     struct Launch { bool resident, staged; uint32_t lds; std::vector<uint32_t> bodies; };
     std::vector<Launch> launches;
     const uint32_t bins[] = { 24 * 1024, 40 * 1024, 64 * 1024, 96 * 1024, kMaxLdsBytes };
-    std::vector<uint32_t> order(ctx->bodies.size());
-    for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
-    // Largest bodies first so the long workgroups start early.
+    std::vector<uint32_t> order = ctx->streamed;
     std::stable_sort(order.begin(), order.end(), [ctx](uint32_t a, uint32_t b) { return ctx->bodies[a]->nv > ctx->bodies[b]->nv; });
     for (uint32_t i : order)
     {
@@ -745,7 +1091,11 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
         if (need > kMaxLdsBytes)
             return Fail(XPBD_E_CAPACITY, "body " + std::to_string(i) + " needs " + std::to_string(need)
                                              + " bytes of LDS; a workgroup has " + std::to_string(kMaxLdsBytes));
-        b.dev.stage_records = staged ? 1u : 0u;
+        if (b.dev.stage_records != (staged ? 1u : 0u))
+        {
+            b.dev.stage_records = staged ? 1u : 0u;
+            ctx->tables_dirty = true;
+        }
         uint32_t bin = 0;
         while (bins[bin] < need) ++bin;
         Launch *L = nullptr;
@@ -759,21 +1109,17 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
         }
         L->bodies.push_back(i);
     }
-    // stage_records may have changed
-    {
-        std::vector<BodyDev> &devs = ctx->host_devs;
-        devs.clear();
-        for (auto &b : ctx->bodies) devs.push_back(b->dev);
-        if (ctx->d_bodies.n != devs.size())
-            HIP_TRY(ctx->d_bodies.alloc(devs.size()));
-        HIP_TRY(hipMemcpyAsync(ctx->d_bodies.p, devs.data(), sizeof(BodyDev) * devs.size(), hipMemcpyHostToDevice, ctx->stream));
-    }
-    std::vector<uint32_t> list;
+    if ((rc = SyncTables(ctx)) != XPBD_OK)
+        return rc;
+    std::vector<uint32_t> &list = ctx->host_list;
+    list.clear();
     for (const Launch &l : launches) list.insert(list.end(), l.bodies.begin(), l.bodies.end());
-    if (ctx->d_list.n != list.size())
-        HIP_TRY(ctx->d_list.alloc(list.size()));
-    HIP_TRY(hipMemcpyAsync(ctx->d_list.p, list.data(), sizeof(uint32_t) * list.size(), hipMemcpyHostToDevice, ctx->stream));
-    ctx->host_list = list;
+    if (!list.empty())
+    {
+        if (ctx->d_list.n != list.size())
+            HIP_TRY(ctx->d_list.alloc(list.size()));
+        HIP_TRY(hipMemcpyAsync(ctx->d_list.p, list.data(), sizeof(uint32_t) * list.size(), hipMemcpyHostToDevice, ctx->stream));
+    }
 
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ctx->profiling)
@@ -782,13 +1128,54 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
         HIP_TRY(hipEventCreate(&e1));
         HIP_TRY(hipEventRecord(e0, ctx->stream));
     }
+    // Independent groups of work run concurrently on the context stream and the
+    // auxiliary streams.  The streamed bodies go first on the context stream;
+    // bundle launches follow, longest first, each on the stream with the least
+    // estimated work queued.
+    constexpr int kAux = xpbd_context::kAux;
+    HIP_TRY(hipEventRecord(ctx->fork, ctx->stream));
+    for (int a = 0; a < kAux; ++a)
+        HIP_TRY(hipStreamWaitEvent(ctx->aux[a], ctx->fork, 0));
+    const float gx = ctx->gravity[0], gy = ctx->gravity[1], gz = ctx->gravity[2];
+    hipStream_t streams[kAux + 1] = { ctx->stream };
+    for (int a = 0; a < kAux; ++a) streams[a + 1] = ctx->aux[a];
+    double queued[kAux + 1] = {};
     uint32_t offset = 0;
     for (const Launch &l : launches)
     {
         HIP_TRY(launch_update(l.resident, l.staged, ctx->d_bodies.p, ctx->d_states.p, ctx->d_list.p + offset, (uint32_t)l.bodies.size(),
-                              ctx->d_colliders.p, nc, nc, l.lds, ctx->gravity[0], ctx->gravity[1], ctx->gravity[2], dt,
-                              collision_steps, frames, ctx->d_error.p, ctx->stream));
+                              ctx->d_colliders.p, nc, nc, l.lds, gx, gy, gz, dt, collision_steps, frames, ctx->d_error.p, ctx->stream));
         offset += (uint32_t)l.bodies.size();
+        std::vector<double> work;
+        for (uint32_t bi : l.bodies)
+        {
+            const Body &b = *ctx->bodies[bi];
+            // gathered windows cost about one level pass each for the gather and the scatter
+            const SettingsDev &sd = b.settings->dev;
+            work.push_back((double)(sd.num_levels_total + 2u * (sd.num_par_windows + sd.num_tail_windows)) * (double)b.dev.iterations);
+        }
+        queued[0] += LaunchCost(work, l.lds);
+    }
+    std::vector<size_t> by_cost(ctx->bundle_launches.size());
+    for (size_t i = 0; i < by_cost.size(); ++i) by_cost[i] = i;
+    std::stable_sort(by_cost.begin(), by_cost.end(), [ctx](size_t a, size_t b) {
+        return ctx->bundle_launches[a]->cost > ctx->bundle_launches[b]->cost;
+    });
+    for (size_t li : by_cost)
+    {
+        const auto &L = *ctx->bundle_launches[li];
+        int best = 0;
+        for (int q = 1; q <= kAux; ++q)
+            if (queued[q] < queued[best])
+                best = q;
+        queued[best] += L.cost;
+        HIP_TRY(launch_bundles(L.staged, L.table.p, L.count, ctx->d_bodies.p, ctx->d_states.p, ctx->d_colliders.p, nc,
+                               L.lds, gx, gy, gz, dt, collision_steps, frames, streams[best]));
+    }
+    for (int a = 0; a < kAux; ++a)
+    {
+        HIP_TRY(hipEventRecord(ctx->join[a], ctx->aux[a]));
+        HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->join[a], 0));
     }
     if (ctx->profiling)
     {
@@ -828,10 +1215,10 @@ int xpbd_body_read(xpbd_context *ctx, int32_t body, float *pos, float *prev, flo
     Body &b = *ctx->bodies[body];
     const uint32_t nv = b.nv;
     std::vector<float4> tmp(nv);
-    auto get3 = [&](DevBuf<float4> &src, float *dst, int comps) -> int {
+    auto get3 = [&](const float4 *src, float *dst, int comps) -> int {
         if (!dst)
             return XPBD_OK;
-        HIP_TRY(hipMemcpy(tmp.data(), src.p, sizeof(float4) * nv, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(tmp.data(), src, sizeof(float4) * nv, hipMemcpyDeviceToHost));
         for (uint32_t i = 0; i < nv; ++i)
         {
             dst[comps * i] = tmp[i].x; dst[comps * i + 1] = tmp[i].y; dst[comps * i + 2] = tmp[i].z;
@@ -839,13 +1226,13 @@ int xpbd_body_read(xpbd_context *ctx, int32_t body, float *pos, float *prev, flo
         }
         return XPBD_OK;
     };
-    if ((rc = get3(b.x, pos, 3)) != XPBD_OK || (rc = get3(b.prev, prev, 3)) != XPBD_OK || (rc = get3(b.v, vel, 3)) != XPBD_OK
-        || (rc = get3(b.plane, plane, 4)) != XPBD_OK)
+    if ((rc = get3(b.dev.x, pos, 3)) != XPBD_OK || (rc = get3(b.dev.prev, prev, 3)) != XPBD_OK || (rc = get3(b.dev.v, vel, 3)) != XPBD_OK
+        || (rc = get3(b.dev.plane, plane, 4)) != XPBD_OK)
         return rc;
     if (shape_idx)
-        HIP_TRY(hipMemcpy(shape_idx, b.idx.p, sizeof(int32_t) * nv, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(shape_idx, b.dev.idx, sizeof(int32_t) * nv, hipMemcpyDeviceToHost));
     if (contact)
-        HIP_TRY(hipMemcpy(contact, b.contact.p, sizeof(int32_t) * nv, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(contact, b.dev.contact, sizeof(int32_t) * nv, hipMemcpyDeviceToHost));
     if (body_pos)
         memcpy(body_pos, b.state.pos, sizeof(b.state.pos));
     if (body_rot)
@@ -864,16 +1251,16 @@ int xpbd_body_write(xpbd_context *ctx, int32_t body, const float *pos, const flo
     std::vector<float4> tmp(nv);
     if (pos)
     {
-        HIP_TRY(hipMemcpy(tmp.data(), b.x.p, sizeof(float4) * nv, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(tmp.data(), b.dev.x, sizeof(float4) * nv, hipMemcpyDeviceToHost));
         for (uint32_t i = 0; i < nv; ++i)
             tmp[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], tmp[i].w);
-        HIP_TRY(hipMemcpy(b.x.p, tmp.data(), sizeof(float4) * nv, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b.dev.x, tmp.data(), sizeof(float4) * nv, hipMemcpyHostToDevice));
     }
     if (vel)
     {
         for (uint32_t i = 0; i < nv; ++i)
             tmp[i] = make_float4(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2], 0.0f);
-        HIP_TRY(hipMemcpy(b.v.p, tmp.data(), sizeof(float4) * nv, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b.dev.v, tmp.data(), sizeof(float4) * nv, hipMemcpyHostToDevice));
     }
     return XPBD_OK;
 }

@@ -84,3 +84,40 @@ struct ColliderDev
 };
 
 } // namespace xpbd
+
+namespace xpbd {
+
+// A bundle is a set of LDS-resident bodies that share one workgroup.  Each
+// body's level sequence (merged parallel-group levels, then the non-parallel
+// group's levels) is merged with the others' by index, so level k of every
+// body runs in one barrier-separated pass; within a level the records are
+// grouped by constraint type and padded to whole wavefronts.
+struct BundleBodyDev
+{
+    uint32_t body;              // index into the context's body table
+    uint32_t voff;              // first vertex of the body in the bundle
+    uint32_t nv;
+    uint32_t face_off;          // first face slot in the bundle's face scratch (pressure)
+};
+
+struct BundleDev
+{
+    const uint4 *recs;          // merged level program (vertex numbers are bundle-local)
+    const uint32_t *lvl;        // nlev + 1 absolute record offsets
+    uint32_t nlev;
+    uint32_t num_records;
+    const BundleBodyDev *bodies;
+    uint32_t num_bodies;
+    uint32_t nv;                // vertices of all bodies of the bundle
+    float4 *x, *v, *prev, *plane;   // bundle-contiguous vertex state (bodies point into these)
+    int32_t *idx, *contact;
+    void *leaves_full;          // num_bodies x num_colliders Leaf: per-step candidate leaves
+    void *leaves;               // num_bodies x num_colliders Leaf: compacted colliding list
+    float4 *face_scratch;       // pressure scratch, one slot per face of the bundle's pressure bodies
+    uint32_t has_pressure;
+    uint32_t has_sleepers;
+    uint32_t iterations;
+    uint32_t staged;            // records + level offsets copied into LDS once per launch
+};
+
+} // namespace xpbd
