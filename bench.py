@@ -119,6 +119,53 @@ def cpu_baseline(scene, settings, target_seconds=15.0, max_bodies=256):
         return None
 
 
+def differentiation_run(warmup=1, repeats=3):
+    """fwd+bwd of the differentiation workload (64 cloths 64 x 64, 60 frames x 5 iterations,
+    checkpoints every 10 frames): the float forward pass and the adjoint through the
+    torch.autograd front end, timed with device synchronisation around each pass."""
+    import torch
+    from joltphysics_amd.autograd import rollout
+    case = scenes.config4_rollout()
+    s = build_settings(case.mesh)
+    ctx = Context(0)
+    ctx.set_gravity(case.gravity)
+    st = ctx.create_settings(s)
+    r = ctx.create_rollout([st] * case.num_bodies, num_iterations=case.iterations, dt=case.dt, frames=case.frames,
+                           checkpoint_interval=case.checkpoint_interval)
+    dev = torch.device("cuda")
+    target_c = torch.from_numpy(case.compliance * case.target_scale).to(dev)
+    vel0 = torch.from_numpy(case.velocity.reshape(-1)).to(dev)
+    with torch.no_grad():
+        target = rollout(target_c, vel0, r).clone()
+    fwd, tot = [], []
+    for k in range(warmup + repeats):
+        comp = torch.from_numpy(case.compliance).to(dev).requires_grad_(True)
+        vel = vel0.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        x = rollout(comp, vel, r)
+        loss = ((x - target) ** 2).sum(dim=1).mean()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        loss.backward()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        if k >= warmup:
+            fwd.append(t1 - t0)
+            tot.append(t2 - t0)
+    out = {"workload": case.name, "bodies": case.num_bodies, "vertices": int(r.num_vertices),
+           "edges": int(r.num_edges), "frames": case.frames, "iterations": case.iterations,
+           "checkpoint_interval": case.checkpoint_interval,
+           "fwd_ms_per_frame": round(1000.0 * min(fwd) / case.frames, 4),
+           "fwd_bwd_ms_per_frame": round(1000.0 * min(tot) / case.frames, 4),
+           "fwd_bwd_ms_total": round(1000.0 * min(tot), 3),
+           "loss": float(loss.detach().cpu()),
+           "grad_norms": {"compliance": float(comp.grad.norm().cpu()), "velocity": float(vel.grad.norm().cpu())}}
+    r.close()
+    ctx.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +174,7 @@ def main():
     ap.add_argument("--workload", default="batched2048")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-diff", action="store_true", help="skip the fwd+bwd differentiation measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -176,6 +224,8 @@ def main():
                 "kernel": "update_kernel (all variants of one launch)",
                 "algorithmic_bytes_per_launch": bytes_frame * args.steps,
                 "launch_ms": round(kernel_ms / max(n_launch, 1), 3)}
+    ctx.close()
+    diff = None if args.no_diff else differentiation_run()
     cpu = None if args.no_cpu_baseline else cpu_baseline(scene, settings, args.cpu_seconds)
     out = {
         "metric": "constraint-solves/sec", "value": value, "unit": "constraint-solves/s",
@@ -188,12 +238,11 @@ def main():
                    "solves_per_frame": int(solves), "collision_steps": scene.collision_steps,
                    "iterations": int(scene.bodies[0].params.num_iterations), "dt": scene.dt,
                    "parity": "bit-exact with the reference (FMA off)"},
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "differentiation": diff,
     }
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 2)
     print(json.dumps(out), flush=True)
-    ctx.close()
 
 
 if __name__ == "__main__":

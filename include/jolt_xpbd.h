@@ -223,6 +223,61 @@ int xpbd_body_state(xpbd_context *ctx, int32_t body, float out[14]);
 int xpbd_profile_enable(xpbd_context *ctx, int32_t enable);
 int xpbd_profile_read(xpbd_context *ctx, int32_t *num_launches, double *total_ms);
 
+/*
+ * Differentiable rollout.  No counterpart in the reference (Jolt computes no
+ * gradients); it differentiates exactly the update above for bodies with edge
+ * constraints only, no colliders, no pressure, no sleeping and
+ * mUpdatePosition false, one collision step per frame.  Each frame runs
+ * InitializeUpdateContext (SoftBodyMotionProperties.cpp:937-960), then
+ * num_iterations x (IntegratePositions :325-350, the update groups' edge
+ * projections :573-601, the velocity update :700-710) and the velocity clamp
+ * of UpdateSoftBodyState (:852-857); the float forward is bit-identical to
+ * xpbd_update on such a body with identity rotation.
+ *
+ * Parameters are the per-edge compliance (SoftBodySharedSettings::Edge::
+ * mCompliance, in each body's mEdgeConstraints order, bodies concatenated) and
+ * the initial vertex velocities (3 floats per vertex, bodies concatenated);
+ * initial positions and inverse masses are the settings' vertices.  The output
+ * is the body-local vertex positions after `frames` frames.
+ *
+ * The backward pass is a hand-written adjoint: (x, v) checkpoints every
+ * checkpoint_interval frames, each segment recomputed forward, then every
+ * substep re-run with its pre-projection positions recorded and reversed level
+ * by level in reverse order.  It returns d(loss)/d(compliance) and
+ * d(loss)/d(initial velocity) given d(loss)/d(final positions).
+ *
+ * All array arguments are device pointers; `stream` is the hipStream_t the
+ * work is ordered on (NULL: the default stream, as torch's default stream is;
+ * xpbd_context_stream() gives the context's own); calls are asynchronous on it
+ * and allocate nothing.
+ */
+typedef struct xpbd_rollout xpbd_rollout;
+
+typedef struct xpbd_rollout_desc {
+    xpbd_settings *const *settings;     /* one per body, edge constraints only */
+    uint32_t num_bodies;
+    uint32_t num_iterations;            /* SoftBodyCreationSettings::mNumIterations */
+    float linear_damping;               /* mLinearDamping */
+    float max_linear_velocity;          /* mMaxLinearVelocity */
+    float gravity_factor;               /* mGravityFactor; gravity is the context's */
+    float dt;                           /* frame time step (one collision step per frame) */
+    uint32_t frames;
+    uint32_t checkpoint_interval;       /* frames between (x, v) checkpoints, >= 1 */
+} xpbd_rollout_desc;
+
+int xpbd_rollout_create(xpbd_context *ctx, const xpbd_rollout_desc *desc, xpbd_rollout **out_rollout);
+int xpbd_rollout_destroy(xpbd_rollout *rollout);
+/* Total edges and vertices over all bodies (parameter and output lengths). */
+int xpbd_rollout_sizes(xpbd_rollout *rollout, uint32_t *num_edges, uint32_t *num_vertices);
+/* float forward; stores the checkpoints the backward pass starts from. */
+int xpbd_rollout_forward(xpbd_rollout *rollout, const float *compliance, const float *velocity, float *positions, void *stream);
+/* double forward of the same update (finite-difference checks); no checkpoints. */
+int xpbd_rollout_forward_f64(xpbd_rollout *rollout, const double *compliance, const double *velocity, double *positions,
+                             void *stream);
+/* Adjoint of the last float forward.  grad_positions: 3 floats per vertex. */
+int xpbd_rollout_backward(xpbd_rollout *rollout, const float *grad_positions, float *grad_compliance, float *grad_velocity,
+                          void *stream);
+
 #ifdef __cplusplus
 }
 #endif

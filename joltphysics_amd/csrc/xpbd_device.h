@@ -187,13 +187,17 @@ struct NoSkip
     __device__ __forceinline__ bool operator()(const uint4 &) const { return false; }
 };
 
-// Level program with records and offsets in global memory.  Thread t's
-// record of each level is prefetched three levels ahead into a ring of three
-// registers (the loop is unrolled three times so the ring never moves);
-// records beyond the first BLOCK of a level are loaded when needed.
-template <int BLOCK, class Skip>
-__device__ __forceinline__ void run_levels_global(const uint4 *recs, const uint32_t *lvl, uint32_t nlev, float4 *X,
-                                                  float inv_dt_sq, uint32_t tid, Skip skip)
+// Level program with records and offsets in global memory, forward or in
+// reverse level order.  Thread t's record of each level is prefetched three
+// levels ahead into a ring of three registers (the loop is unrolled three
+// times so the ring never moves); records beyond the first BLOCK of a level are
+// loaded when needed.  `handle(record, index)` processes one record (no-op records
+// included) and its index; a level's records are always assigned to threads the same way,
+// whatever the direction, so per-record data written by a thread in one pass is
+// read back by the same thread in another.
+template <int BLOCK, bool REVERSE, class Handle>
+__device__ __forceinline__ void run_program_global(const uint4 *recs, const uint32_t *lvl, uint32_t nlev, uint32_t tid,
+                                                   Handle handle)
 {
     recs = uniform_ptr(recs);
     lvl = uniform_ptr(lvl);
@@ -211,8 +215,10 @@ __device__ __forceinline__ void run_levels_global(const uint4 *recs, const uint3
     // earlier phase still pending, the compiler treats the counter as out of
     // order and would wait for every load at every level.
     __builtin_amdgcn_s_waitcnt(kWaitVmcntZero);
-    auto fetch = [&](uint32_t L) -> RecordVec {
-        const uint32_t lo = load_level_offset(lvl, L < nlev ? L : nlev);
+    // level of step k (k >= nlev: an empty level past the end)
+    auto level_of = [&](uint32_t k) -> uint32_t { return k < nlev ? (REVERSE ? nlev - 1 - k : k) : nlev; };
+    auto fetch = [&](uint32_t k) -> RecordVec {
+        const uint32_t lo = load_level_offset(lvl, level_of(k));
         const uint32_t i = lo + tid;
         return load_record(recs, i < max_index ? i : max_index);    // validity is checked at use
     };
@@ -221,32 +227,34 @@ __device__ __forceinline__ void run_levels_global(const uint4 *recs, const uint3
     RecordVec r1 = fetch(1);
     asm volatile("" ::: "memory");
     RecordVec r2 = fetch(2);
-    uint32_t lo = first;
-    auto level = [&](uint32_t L, RecordVec &r) {
-        const uint32_t hi = load_level_offset(lvl, L + 1 < nlev ? L + 1 : nlev);
+    auto level = [&](uint32_t k, RecordVec &r) {
+        const uint32_t L = level_of(k);
+        const uint32_t lo = load_level_offset(lvl, L);
+        const uint32_t hi = L < nlev ? load_level_offset(lvl, L + 1) : lo;
         pin_record(r);
         if (lo + tid < hi)
-        {
-            const uint4 q = as_uint4(r);
-            if (!skip(q))
-                project_record(X, q, inv_dt_sq);
-        }
+            handle(as_uint4(r), lo + tid);
         for (uint32_t i = lo + tid + BLOCK; i < hi; i += BLOCK)
-        {
-            const uint4 q = as_uint4(load_record(recs, i));
-            if (!skip(q))
-                project_record(X, q, inv_dt_sq);
-        }
-        r = fetch(L + 3);
-        lo = hi;
+            handle(as_uint4(load_record(recs, i)), i);
+        r = fetch(k + 3);
         __syncthreads();
     };
-    for (uint32_t L = 0; L < nlev; L += 3)
+    for (uint32_t k = 0; k < nlev; k += 3)
     {
-        level(L, r0);
-        level(L + 1, r1);
-        level(L + 2, r2);
+        level(k, r0);
+        level(k + 1, r1);
+        level(k + 2, r2);
     }
+}
+
+template <int BLOCK, class Skip>
+__device__ __forceinline__ void run_levels_global(const uint4 *recs, const uint32_t *lvl, uint32_t nlev, float4 *X,
+                                                  float inv_dt_sq, uint32_t tid, Skip skip)
+{
+    run_program_global<BLOCK, false>(recs, lvl, nlev, tid, [&](const uint4 &q, uint32_t) {
+        if (!skip(q))
+            project_record(X, q, inv_dt_sq);
+    });
 }
 
 // Level program with records and offsets in LDS: the first record of the next

@@ -1,0 +1,420 @@
+// Differentiable rollout of the soft-body update for bodies with edge
+// constraints: float and double forward passes and the hand-written adjoint.
+//
+// The forward pass follows the update of the main kernels step for step
+// (SoftBodyMotionProperties.cpp: InitializeUpdateContext :937-960,
+// IntegratePositions :325-350, ApplyEdgeConstraints :573-601, the velocity
+// update :700-710 and the velocity clamp of UpdateSoftBodyState :852-857), so
+// its float instantiation rounds exactly like them.  One workgroup runs one
+// body with its positions in LDS.
+//
+// Adjoint of one sub-step (w = inverse mass, h = sub-step, d = damping):
+//   forward   v1 = w>0 ? (v0 + g h) d : v0,  y0 = x0 + v1 h,  y = C(y0),
+//             v2 = w>0 ? (y - x0) / h : v1,  x1 = y   (+ clamp of v2 on the last
+//             iteration of a frame)
+//   reverse   ybar = x1bar + [w>0] v2bar / h,  y0bar = C^T ybar,
+//             x0bar = y0bar - [w>0] v2bar / h,
+//             v1bar = [w=0] v2bar + y0bar h,  v0bar = w>0 ? v1bar d : v1bar.
+// C is the sequence of edge projections; C^T runs the levels in reverse, each
+// edge from its recorded pre-projection positions:
+//   delta = x1 - x0, L = |delta|, s = w0 + w1 + alpha / h^2, c = L - rest,
+//   corr = delta c / (L s);  x0' = x0 + w0 corr,  x1' = x1 - w1 corr
+//   cbar = w0 x0'bar - w1 x1'bar,  n = delta / L
+//   deltabar = ((c / L) cbar + (rest / L) n (n . cbar)) / s
+//   x0bar = x0'bar - deltabar,  x1bar = x1'bar + deltabar,
+//   alphabar += (cbar . corr) (-1 / (h^2 s)).
+#include "xpbd_device.h"
+
+namespace xpbd {
+
+constexpr int kRolloutBlock = 256;
+
+template <class T> struct R3 { T x, y, z; };
+template <class T> struct alignas(sizeof(T) * 4) R4 { T x, y, z, w; };
+
+template <class T> __device__ __forceinline__ R3<T> r3(T x, T y, T z) { R3<T> r; r.x = x; r.y = y; r.z = z; return r; }
+template <class T> __device__ __forceinline__ R4<T> r4(R3<T> a, T w) { R4<T> r; r.x = a.x; r.y = a.y; r.z = a.z; r.w = w; return r; }
+template <class T> __device__ __forceinline__ R3<T> xyz3(R4<T> p) { return r3(p.x, p.y, p.z); }
+template <class T> __device__ __forceinline__ R3<T> operator+(R3<T> a, R3<T> b) { return r3(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <class T> __device__ __forceinline__ R3<T> operator-(R3<T> a, R3<T> b) { return r3(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <class T> __device__ __forceinline__ R3<T> operator*(R3<T> a, T s) { return r3(a.x * s, a.y * s, a.z * s); }
+template <class T> __device__ __forceinline__ R3<T> operator*(T s, R3<T> a) { return r3(s * a.x, s * a.y, s * a.z); }
+template <class T> __device__ __forceinline__ R3<T> operator/(R3<T> a, T s) { return r3(a.x / s, a.y / s, a.z / s); }
+// Dot = (x*x' + y*y') + z*z' (Vec3.inl:901-934)
+template <class T> __device__ __forceinline__ T rdot(R3<T> a, R3<T> b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ __forceinline__ float rsqrt_exact(float v) { return sqrtf(v); }
+__device__ __forceinline__ double rsqrt_exact(double v) { return sqrt(v); }
+
+template <class T> __device__ __forceinline__ R4<T> to_r4(float4 p) { R4<T> r; r.x = (T)p.x; r.y = (T)p.y; r.z = (T)p.z; r.w = (T)p.w; return r; }
+__device__ __forceinline__ float4 to_f4(R4<float> p) { return make_float4(p.x, p.y, p.z, p.w); }
+
+// Per-launch constants of InitializeUpdateContext for one collision step per
+// frame and identity body rotation (SoftBodyMotionProperties.cpp:937-960).
+template <class T>
+struct StepConstants
+{
+    T dt_sub, inv_dt_sq, damping, max_lin_sq, impulse;
+    R3<T> sub_step_gravity;
+
+    __device__ StepConstants(const RolloutDev &R, uint32_t nv)
+    {
+        const T step_dt = (T)R.dt / (T)1;                          // PhysicsSystem.cpp:215, one collision step
+        dt_sub = step_dt / (T)R.iterations;
+        inv_dt_sq = (T)1 / (dt_sub * dt_sub);
+        const T d = (T)1 - (T)R.linear_damping * dt_sub;
+        damping = (T)0 > d ? (T)0 : d;
+        const T gf = (T)R.gravity_factor;
+        const R3<T> g_local = r3(gf * (T)R.gravity[0], gf * (T)R.gravity[1], gf * (T)R.gravity[2]);
+        sub_step_gravity = g_local * dt_sub;
+        const T nvf = (T)nv;
+        impulse = ((T)0 * dt_sub) / (nvf > (T)1 ? nvf : (T)1);    // accumulated force is zero
+        max_lin_sq = (T)R.max_linear_velocity * (T)R.max_linear_velocity;
+    }
+
+    // IntegratePositions for one vertex: returns v1 (x0, w from p)
+    __device__ __forceinline__ R3<T> integrate_velocity(R3<T> v, T w) const
+    {
+        if (w > (T)0)
+        {
+            v = v + (sub_step_gravity + r3(impulse * w, impulse * w, impulse * w));
+            v = v * damping;
+        }
+        return v;
+    }
+};
+
+// ApplyEdgeConstraints body (SoftBodyMotionProperties.cpp:583-600)
+template <class T>
+__device__ __forceinline__ void edge_project(R4<T> *X, uint32_t i0, uint32_t i1, T rest, T compliance, T inv_dt_sq)
+{
+    const R4<T> p0 = X[i0], p1 = X[i1];
+    const R3<T> x0 = xyz3(p0), x1 = xyz3(p1);
+    const T w0 = p0.w, w1 = p1.w;
+    const R3<T> delta = x1 - x0;
+    const T length = rsqrt_exact(rdot(delta, delta));
+    const T denom = length * ((w0 + w1) + compliance * inv_dt_sq);
+    if (denom < (T)1.0e-12f)
+        return;
+    const R3<T> correction = (delta * (length - rest)) / denom;
+    X[i0] = r4(x0 + w0 * correction, w0);
+    X[i1] = r4(x1 - w1 * correction, w1);
+}
+
+// Adjoint of edge_project from its pre-projection positions q0, q1.
+__device__ __forceinline__ void edge_adjoint(float4 *A, uint32_t i0, uint32_t i1, float4 q0, float4 q1, float rest,
+                                             float compliance, float inv_dt_sq, float &alpha_bar)
+{
+    const R3<float> x0 = r3(q0.x, q0.y, q0.z), x1 = r3(q1.x, q1.y, q1.z);
+    const float w0 = q0.w, w1 = q1.w;
+    const R3<float> delta = x1 - x0;
+    const float length = sqrtf(rdot(delta, delta));
+    const float s = (w0 + w1) + compliance * inv_dt_sq;
+    const float denom = length * s;
+    if (denom < 1.0e-12f)
+        return;
+    const float c = length - rest;
+    const R3<float> correction = (delta * c) / denom;
+    const float4 a0 = A[i0], a1 = A[i1];
+    const R3<float> ab0 = r3(a0.x, a0.y, a0.z), ab1 = r3(a1.x, a1.y, a1.z);
+    const R3<float> cbar = w0 * ab0 - w1 * ab1;
+    const R3<float> n = delta / length;
+    const R3<float> dbar = ((c / length) * cbar + ((rest / length) * rdot(n, cbar)) * n) / s;
+    A[i0] = make_float4(ab0.x - dbar.x, ab0.y - dbar.y, ab0.z - dbar.z, a0.w);
+    A[i1] = make_float4(ab1.x + dbar.x, ab1.y + dbar.y, ab1.z + dbar.z, a1.w);
+    alpha_bar += rdot(cbar, correction) * (-inv_dt_sq / s);
+}
+
+template <class T> __device__ __forceinline__ R4<T> *state_vel(const RolloutDev &R);
+template <> __device__ __forceinline__ R4<float> *state_vel<float>(const RolloutDev &R) { return reinterpret_cast<R4<float> *>(R.vel); }
+template <> __device__ __forceinline__ R4<double> *state_vel<double>(const RolloutDev &R) { return reinterpret_cast<R4<double> *>(R.vel64); }
+template <class T> __device__ __forceinline__ R4<T> *state_prev(const RolloutDev &R);
+template <> __device__ __forceinline__ R4<float> *state_prev<float>(const RolloutDev &R) { return reinterpret_cast<R4<float> *>(R.prev); }
+template <> __device__ __forceinline__ R4<double> *state_prev<double>(const RolloutDev &R) { return reinterpret_cast<R4<double> *>(R.prev64); }
+
+// One sub-step of the forward pass on the LDS positions X; V / P are the
+// body's velocity / previous-position arrays.  The compliance of record r is
+// comp(r).
+template <class T, class Comp>
+__device__ __forceinline__ void forward_substep(const RolloutBodyDev &B, const StepConstants<T> &K, R4<T> *X, R4<T> *V, R4<T> *P,
+                                                bool last, uint32_t tid, Comp comp)
+{
+    const uint32_t nv = B.nv;
+    for (uint32_t i = tid; i < nv; i += kRolloutBlock)
+    {
+        const R4<T> p = X[i];
+        const R3<T> v = K.integrate_velocity(xyz3(V[i]), p.w);
+        const R3<T> position = xyz3(p);
+        P[i] = r4(position, (T)0);
+        X[i] = r4(position + v * K.dt_sub, p.w);
+        V[i] = r4(v, (T)0);
+    }
+    __syncthreads();
+    run_program_global<kRolloutBlock, false>(B.recs, B.lvl, B.nlev, tid, [&](const uint4 &q, uint32_t) {
+        if (q.x == kNopRecord)
+            return;
+        edge_project<T>(X, q.x & 0x3fffu, q.x >> 16, (T)__uint_as_float(q.z), comp(q), K.inv_dt_sq);
+    });
+    for (uint32_t i = tid; i < nv; i += kRolloutBlock)
+    {
+        const R4<T> p = X[i];
+        R3<T> v = xyz3(V[i]);
+        if (p.w > (T)0)
+            v = (xyz3(p) - xyz3(P[i])) / K.dt_sub;
+        if (last)
+        {
+            const T v_sq = rdot(v, v);
+            if (v_sq > K.max_lin_sq)
+                v = v * rsqrt_exact(K.max_lin_sq / v_sq);
+        }
+        V[i] = r4(v, (T)0);
+    }
+    __syncthreads();
+}
+
+// Forward pass.  The float pass first writes each record's compliance into
+// the record (.w), where the backward pass reads it, and stores the (x, v)
+// checkpoints; the double pass reads the compliance array directly.
+template <class T>
+__global__ __launch_bounds__(kRolloutBlock) void rollout_forward_kernel(RolloutDev R, const T *__restrict__ compliance,
+                                                                      const T *__restrict__ velocity, T *__restrict__ positions)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    R4<T> *X = reinterpret_cast<R4<T> *>(lds_raw);
+    const uint32_t tid = threadIdx.x;
+    const RolloutBodyDev B = R.bodies[blockIdx.x];
+    const uint32_t nv = B.nv;
+    R4<T> *V = state_vel<T>(R) + B.voff;
+    R4<T> *P = state_prev<T>(R) + B.voff;
+    constexpr bool kFloat = sizeof(T) == 4;
+    if (kFloat)
+    {
+        uint4 *recs = const_cast<uint4 *>(B.recs);
+        for (uint32_t r = tid; r < B.num_records; r += kRolloutBlock)
+        {
+            uint4 q = recs[r];
+            if (q.x != kNopRecord)
+            {
+                q.w = __float_as_uint((float)compliance[B.eoff + q.y]);
+                recs[r] = q;
+            }
+        }
+    }
+    for (uint32_t i = tid; i < nv; i += kRolloutBlock)
+    {
+        X[i] = to_r4<T>(B.x0[i]);
+        const uint32_t g = 3u * (B.voff + i);
+        R4<T> v; v.x = velocity[g]; v.y = velocity[g + 1]; v.z = velocity[g + 2]; v.w = (T)0;
+        V[i] = v;
+    }
+    __syncthreads();
+    const StepConstants<T> K(R, nv);
+    auto comp = [&](const uint4 &q) -> T {
+        if (kFloat)
+            return (T)__uint_as_float(q.w);
+        return compliance[B.eoff + q.y];
+    };
+    for (uint32_t frame = 0; frame < R.frames; ++frame)
+    {
+        if (kFloat && frame % R.interval == 0)
+        {
+            const size_t base = (size_t)(frame / R.interval) * R.total_nv + B.voff;
+            for (uint32_t i = tid; i < nv; i += kRolloutBlock)
+            {
+                const R4<T> p = X[i], v = V[i];
+                R.ckpt_x[base + i] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
+                R.ckpt_v[base + i] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.0f);
+            }
+        }
+        for (uint32_t it = 0; it < R.iterations; ++it)
+            forward_substep<T>(B, K, X, V, P, it + 1 == R.iterations, tid, comp);
+    }
+    for (uint32_t i = tid; i < nv; i += kRolloutBlock)
+    {
+        const uint32_t g = 3u * (B.voff + i);
+        const R4<T> p = X[i];
+        positions[g] = p.x; positions[g + 1] = p.y; positions[g + 2] = p.z;
+    }
+}
+
+// Adjoint of the last float forward pass.
+__global__ __launch_bounds__(kRolloutBlock) void rollout_backward_kernel(RolloutDev R, const float *__restrict__ grad_positions,
+                                                                       float *__restrict__ grad_compliance,
+                                                                       float *__restrict__ grad_velocity)
+{
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const uint32_t tid = threadIdx.x;
+    const RolloutBodyDev B = R.bodies[blockIdx.x];
+    const uint32_t nv = B.nv, ne = B.num_edges;
+    float4 *A = reinterpret_cast<float4 *>(lds_raw);                 // positions, then their adjoints
+    R4<float> *X = reinterpret_cast<R4<float> *>(lds_raw);
+    float *alpha_bar = reinterpret_cast<float *>(lds_raw + 16u * nv);
+    R4<float> *V = state_vel<float>(R) + B.voff;
+    R4<float> *P = state_prev<float>(R) + B.voff;
+    float4 *adj_x = R.adj_x + B.voff, *adj_v = R.adj_v + B.voff;
+    float4 *tape = R.rec_tape + 2u * (size_t)B.rbase;
+
+    for (uint32_t e = tid; e < ne; e += kRolloutBlock)
+        alpha_bar[e] = 0.0f;
+    for (uint32_t i = tid; i < nv; i += kRolloutBlock)
+    {
+        const uint32_t g = 3u * (B.voff + i);
+        adj_x[i] = make_float4(grad_positions[g], grad_positions[g + 1], grad_positions[g + 2], 0.0f);
+        adj_v[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    const StepConstants<float> K(R, nv);
+    const float h = K.dt_sub;
+    auto comp = [](const uint4 &q) -> float { return __uint_as_float(q.w); };
+    const uint32_t I = R.iterations;
+    const uint32_t num_ckpt = (R.frames + R.interval - 1) / R.interval;
+    __syncthreads();
+
+    for (uint32_t seg = num_ckpt; seg-- > 0;)
+    {
+        const uint32_t f0 = seg * R.interval;
+        const uint32_t f1 = f0 + R.interval < R.frames ? f0 + R.interval : R.frames;
+        const uint32_t nsub = (f1 - f0) * I;
+        // recompute the segment from its checkpoint, keeping every sub-step's start state
+        const size_t cbase = (size_t)seg * R.total_nv + B.voff;
+        for (uint32_t i = tid; i < nv; i += kRolloutBlock)
+        {
+            const float4 p = R.ckpt_x[cbase + i], v = R.ckpt_v[cbase + i];
+            X[i] = to_r4<float>(p);
+            V[i] = to_r4<float>(v);
+        }
+        __syncthreads();
+        for (uint32_t s = 0; s < nsub; ++s)
+        {
+            const size_t sbase = (size_t)s * R.total_nv + B.voff;
+            for (uint32_t i = tid; i < nv; i += kRolloutBlock)
+            {
+                R.seg_x[sbase + i] = A[i];
+                R.seg_v[sbase + i] = to_f4(V[i]);
+            }
+            forward_substep<float>(B, K, X, V, P, s % I == I - 1, tid, comp);
+        }
+        // reverse the segment sub-step by sub-step
+        for (uint32_t s = nsub; s-- > 0;)
+        {
+            const size_t sbase = (size_t)s * R.total_nv + B.voff;
+            const bool last = s % I == I - 1;
+            // re-run the sub-step, recording every edge's pre-projection positions
+            for (uint32_t i = tid; i < nv; i += kRolloutBlock)
+            {
+                const float4 p = R.seg_x[sbase + i], v0 = R.seg_v[sbase + i];
+                const R3<float> v1 = K.integrate_velocity(r3(v0.x, v0.y, v0.z), p.w);
+                X[i] = r4(r3(p.x, p.y, p.z) + v1 * h, p.w);
+            }
+            __syncthreads();
+            run_program_global<kRolloutBlock, false>(B.recs, B.lvl, B.nlev, tid, [&](const uint4 &q, uint32_t r) {
+                if (q.x == kNopRecord)
+                    return;
+                const uint32_t i0 = q.x & 0x3fffu, i1 = q.x >> 16;
+                tape[2u * r] = A[i0];
+                tape[2u * r + 1u] = A[i1];
+                edge_project<float>(X, i0, i1, __uint_as_float(q.z), __uint_as_float(q.w), K.inv_dt_sq);
+            });
+            // adjoint of the velocity update (and of the clamp after the last iteration)
+            for (uint32_t i = tid; i < nv; i += kRolloutBlock)
+            {
+                const float4 p = R.seg_x[sbase + i], v0 = R.seg_v[sbase + i];
+                const float w = p.w;
+                const R3<float> x0 = r3(p.x, p.y, p.z);
+                const R3<float> v1 = K.integrate_velocity(r3(v0.x, v0.y, v0.z), w);
+                const R3<float> y = xyz3(X[i]);
+                const R3<float> v2 = w > 0.0f ? (y - x0) / h : v1;
+                const float4 vbf = adj_v[i], xbf = adj_x[i];
+                R3<float> vb = r3(vbf.x, vbf.y, vbf.z);
+                if (last)
+                {
+                    const float v_sq = rdot(v2, v2);
+                    if (v_sq > K.max_lin_sq)
+                    {
+                        const float l = sqrtf(v_sq);
+                        const R3<float> n = v2 / l;
+                        vb = (sqrtf(K.max_lin_sq) / l) * (vb - rdot(n, vb) * n);
+                    }
+                }
+                adj_v[i] = make_float4(vb.x, vb.y, vb.z, 0.0f);
+                R3<float> yb = r3(xbf.x, xbf.y, xbf.z);
+                if (w > 0.0f)
+                    yb = yb + vb / h;
+                A[i] = make_float4(yb.x, yb.y, yb.z, w);
+            }
+            __syncthreads();
+            run_program_global<kRolloutBlock, true>(B.recs, B.lvl, B.nlev, tid, [&](const uint4 &q, uint32_t r) {
+                if (q.x == kNopRecord)
+                    return;
+                edge_adjoint(A, q.x & 0x3fffu, q.x >> 16, tape[2u * r], tape[2u * r + 1u], __uint_as_float(q.z),
+                             __uint_as_float(q.w), K.inv_dt_sq, alpha_bar[q.y]);
+            });
+            // adjoint of prev = x0, y0 = x0 + v1 h and the integration of v
+            for (uint32_t i = tid; i < nv; i += kRolloutBlock)
+            {
+                const float4 yb4 = A[i], vbf = adj_v[i];
+                const float w = yb4.w;
+                const R3<float> y0b = r3(yb4.x, yb4.y, yb4.z), vb = r3(vbf.x, vbf.y, vbf.z);
+                R3<float> x0b = y0b, v1b = y0b * h;
+                if (w > 0.0f)
+                    x0b = x0b - vb / h;
+                else
+                    v1b = v1b + vb;
+                const R3<float> v0b = w > 0.0f ? v1b * K.damping : v1b;
+                adj_x[i] = make_float4(x0b.x, x0b.y, x0b.z, 0.0f);
+                adj_v[i] = make_float4(v0b.x, v0b.y, v0b.z, 0.0f);
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = tid; i < nv; i += kRolloutBlock)
+    {
+        const uint32_t g = 3u * (B.voff + i);
+        const float4 v = adj_v[i];
+        grad_velocity[g] = v.x; grad_velocity[g + 1] = v.y; grad_velocity[g + 2] = v.z;
+    }
+    for (uint32_t e = tid; e < ne; e += kRolloutBlock)
+        grad_compliance[B.eoff + e] = alpha_bar[e];
+}
+
+uint32_t rollout_forward_lds_bytes(uint32_t nv, bool f64) { return nv * (f64 ? 32u : 16u); }
+uint32_t rollout_backward_lds_bytes(uint32_t nv, uint32_t ne) { return 16u * nv + ((4u * ne + 15u) & ~15u); }
+
+template <class K>
+static hipError_t allow_lds(K kernel)
+{
+    return hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+}
+
+hipError_t launch_rollout_forward_f32(const RolloutDev &R, const float *compliance, const float *velocity, float *positions,
+                                      uint32_t lds, hipStream_t stream)
+{
+    static hipError_t configured = allow_lds(rollout_forward_kernel<float>);
+    if (configured != hipSuccess)
+        return configured;
+    hipLaunchKernelGGL(rollout_forward_kernel<float>, dim3(R.num_bodies), dim3(kRolloutBlock), lds, stream, R, compliance,
+                       velocity, positions);
+    return hipGetLastError();
+}
+
+hipError_t launch_rollout_forward_f64(const RolloutDev &R, const double *compliance, const double *velocity, double *positions,
+                                      uint32_t lds, hipStream_t stream)
+{
+    static hipError_t configured = allow_lds(rollout_forward_kernel<double>);
+    if (configured != hipSuccess)
+        return configured;
+    hipLaunchKernelGGL(rollout_forward_kernel<double>, dim3(R.num_bodies), dim3(kRolloutBlock), lds, stream, R, compliance,
+                       velocity, positions);
+    return hipGetLastError();
+}
+
+hipError_t launch_rollout_backward(const RolloutDev &R, const float *grad_positions, float *grad_compliance, float *grad_velocity,
+                                   uint32_t lds, hipStream_t stream)
+{
+    static hipError_t configured = allow_lds(rollout_backward_kernel);
+    if (configured != hipSuccess)
+        return configured;
+    hipLaunchKernelGGL(rollout_backward_kernel, dim3(R.num_bodies), dim3(kRolloutBlock), lds, stream, R, grad_positions,
+                       grad_compliance, grad_velocity);
+    return hipGetLastError();
+}
+
+} // namespace xpbd

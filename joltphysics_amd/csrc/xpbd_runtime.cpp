@@ -26,6 +26,14 @@ hipError_t launch_update(bool resident, bool staged, const BodyDev *bodies, Body
                          uint32_t num_blocks, const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves,
                          uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
                          int32_t *error_flag, hipStream_t stream);
+uint32_t rollout_forward_lds_bytes(uint32_t nv, bool f64);
+uint32_t rollout_backward_lds_bytes(uint32_t nv, uint32_t ne);
+hipError_t launch_rollout_forward_f32(const RolloutDev &R, const float *compliance, const float *velocity, float *positions,
+                                      uint32_t lds, hipStream_t stream);
+hipError_t launch_rollout_forward_f64(const RolloutDev &R, const double *compliance, const double *velocity, double *positions,
+                                      uint32_t lds, hipStream_t stream);
+hipError_t launch_rollout_backward(const RolloutDev &R, const float *grad_positions, float *grad_compliance, float *grad_velocity,
+                                   uint32_t lds, hipStream_t stream);
 } // namespace xpbd
 
 using namespace xpbd;
@@ -104,6 +112,8 @@ struct xpbd_settings
     // then the non-parallel group's levels): records with body vertex indices
     std::vector<uint4> host_recs;
     std::vector<uint32_t> host_level_begin;     // nlev + 1 offsets into host_recs
+    std::vector<uint32_t> host_rec_index;       // constraint index of each host record (no-op: ~0u)
+    uint32_t num_edges = 0, num_bends = 0, num_volumes = 0, num_lras = 0;
 };
 
 struct Body
@@ -185,7 +195,7 @@ namespace {
 // so every pair that shares a vertex with at least one writer keeps its
 // order, and the constraints of one level have disjoint write sets that no
 // other constraint of the level reads.
-struct Item { uint32_t type; uint32_t v[4]; uint32_t nv; float p0, p1; };
+struct Item { uint32_t type; uint32_t v[4]; uint32_t nv; float p0, p1; uint32_t index; };   // index: in its type's array
 
 // Bodies up to this many vertices keep all positions in LDS for the launch.
 constexpr uint32_t kResidentMaxVertices = 8192;
@@ -250,7 +260,8 @@ struct GroupLevels
 // LRA), each group padded with no-op records to whole wavefronts when that does
 // not add a pass of the 256-thread workgroup, so a wavefront runs one
 // constraint path.  `pad` false keeps the groups unpadded.
-void AppendTypedLevel(std::vector<uint4> bucket[4], std::vector<uint4> &recs, bool allow_pad = true)
+void AppendTypedLevel(std::vector<uint4> bucket[4], std::vector<uint4> &recs, bool allow_pad = true,
+                      const std::vector<uint32_t> *ibucket = nullptr, std::vector<uint32_t> *rec_index = nullptr)
 {
     size_t n = 0, padded = 0;
     for (int t = 0; t < 4; ++t)
@@ -265,15 +276,22 @@ void AppendTypedLevel(std::vector<uint4> bucket[4], std::vector<uint4> &recs, bo
     for (int t : order)
     {
         recs.insert(recs.end(), bucket[t].begin(), bucket[t].end());
+        if (rec_index)
+            rec_index->insert(rec_index->end(), ibucket[t].begin(), ibucket[t].end());
         if (pad)
             while ((recs.size() - base) % 64 != 0)
+            {
                 recs.push_back(nop);
+                if (rec_index)
+                    rec_index->push_back(~0u);
+            }
     }
 }
 
 struct ProgramBuilder
 {
     std::vector<uint4> recs;
+    std::vector<uint32_t> rec_index;        // constraint index of each record (no-op: ~0u)
     std::vector<uint32_t> lvl;
     std::vector<WindowDev> windows;
     std::vector<uint32_t> vlist;
@@ -305,12 +323,14 @@ struct ProgramBuilder
 void ProgramBuilder::AppendLevel(const std::vector<std::pair<const GroupLevels *, uint32_t>> &lv, const uint32_t *local)
 {
     std::vector<uint4> bucket[4];
+    std::vector<uint32_t> ibucket[4];
     for (const auto &gi : lv)
     {
         const Item &it = gi.first->seq[gi.second];
         bucket[it.type].push_back(Pack(it, local));
+        ibucket[it.type].push_back(it.index);
     }
-    AppendTypedLevel(bucket, recs);
+    AppendTypedLevel(bucket, recs, true, ibucket, &rec_index);
 }
 
 int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
@@ -360,22 +380,22 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
         for (uint32_t k = prev.volume_end; k < c.volume_end; ++k)
         {
             if (!check(vols[k].v, 4)) return Fail(XPBD_E_INVALID, "volume constraint vertex out of range");
-            seq.push_back({ kTypeVolume, { vols[k].v[0], vols[k].v[1], vols[k].v[2], vols[k].v[3] }, 4, vols[k].six_rest_volume, vols[k].compliance });
+            seq.push_back({ kTypeVolume, { vols[k].v[0], vols[k].v[1], vols[k].v[2], vols[k].v[3] }, 4, vols[k].six_rest_volume, vols[k].compliance, k });
         }
         for (uint32_t k = prev.bend_end; k < c.bend_end; ++k)
         {
             if (!check(bends[k].v, 4)) return Fail(XPBD_E_INVALID, "bend constraint vertex out of range");
-            seq.push_back({ kTypeBend, { bends[k].v[0], bends[k].v[1], bends[k].v[2], bends[k].v[3] }, 4, bends[k].compliance, bends[k].initial_angle });
+            seq.push_back({ kTypeBend, { bends[k].v[0], bends[k].v[1], bends[k].v[2], bends[k].v[3] }, 4, bends[k].compliance, bends[k].initial_angle, k });
         }
         for (uint32_t k = prev.edge_end; k < c.edge_end; ++k)
         {
             if (!check(edges[k].v, 2)) return Fail(XPBD_E_INVALID, "edge constraint vertex out of range");
-            seq.push_back({ kTypeEdge, { edges[k].v[0], edges[k].v[1], 0, 0 }, 2, edges[k].rest, edges[k].compliance });
+            seq.push_back({ kTypeEdge, { edges[k].v[0], edges[k].v[1], 0, 0 }, 2, edges[k].rest, edges[k].compliance, k });
         }
         for (uint32_t k = prev.lra_end; k < c.lra_end; ++k)
         {
             if (!check(lras[k].v, 2)) return Fail(XPBD_E_INVALID, "LRA constraint vertex out of range");
-            seq.push_back({ kTypeLRA, { lras[k].v[0], lras[k].v[1], 0, 0 }, 2, lras[k].max_distance, 0.0f });
+            seq.push_back({ kTypeLRA, { lras[k].v[0], lras[k].v[1], 0, 0 }, 2, lras[k].max_distance, 0.0f, k });
         }
         gl[g].levels = LevelGroup(seq, last_write, last_read);
         prev = c;
@@ -505,9 +525,14 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
         s.dev.num_tail_windows = ntw;
     }
 
+    s.num_edges = d.num_edges;
+    s.num_bends = d.num_dihedral_bends;
+    s.num_volumes = d.num_volumes;
+    s.num_lras = d.num_lras;
     if (resident)
     {
         s.host_recs = pb.recs;
+        s.host_rec_index = pb.rec_index;
         s.host_level_begin.clear();
         for (const WindowDev &w : pb.windows)
             for (uint32_t L = 0; L < w.nlev; ++L)
@@ -1311,6 +1336,182 @@ int xpbd_profile_read(xpbd_context *ctx, int32_t *num_launches, double *total_ms
     *num_launches = (int32_t)ctx->events.size();
     *total_ms = total;
     ctx->events.clear();
+    return XPBD_OK;
+}
+
+// ------------------------------------------------------------------ rollout
+struct xpbd_rollout
+{
+    xpbd_context *ctx = nullptr;
+    std::vector<xpbd_settings *> settings;
+    DevBuf<uint4> recs;
+    DevBuf<uint32_t> lvl;
+    DevBuf<float4> x0;
+    DevBuf<RolloutBodyDev> bodies;
+    DevBuf<float4> ckpt_x, ckpt_v, seg_x, seg_v, vel, prev, rec_tape, adj_x, adj_v;
+    DevBuf<double> vel64, prev64;
+    RolloutDev dev{};
+    uint32_t total_nv = 0, total_ne = 0;
+    uint32_t lds_f32 = 0, lds_f64 = 0, lds_bwd = 0;
+    bool has_forward = false;
+
+    ~xpbd_rollout()
+    {
+        for (xpbd_settings *s : settings)
+            xpbd_settings_release(s);
+    }
+};
+
+namespace {
+// The caller's stream as given: NULL is the default (null) stream, as in torch.
+hipStream_t StreamOf(xpbd_rollout *, void *stream) { return (hipStream_t)stream; }
+}
+
+int xpbd_rollout_create(xpbd_context *ctx, const xpbd_rollout_desc *d, xpbd_rollout **out)
+{
+    if (!ctx || !d || !out || (d->num_bodies > 0 && !d->settings))
+        return Fail(XPBD_E_INVALID, "null argument");
+    *out = nullptr;
+    if (d->num_bodies == 0 || d->num_iterations == 0 || d->frames == 0 || d->checkpoint_interval == 0 || !(d->dt > 0.0f))
+        return Fail(XPBD_E_INVALID, "num_bodies, num_iterations, frames, checkpoint_interval and dt must be positive");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<xpbd_rollout> r(new xpbd_rollout);
+    r->ctx = ctx;
+    std::vector<uint4> recs;
+    std::vector<uint32_t> lvl;
+    std::vector<float4> x0;
+    std::vector<RolloutBodyDev> bodies(d->num_bodies);
+    uint32_t max_nv = 0;
+    for (uint32_t b = 0; b < d->num_bodies; ++b)
+    {
+        xpbd_settings *s = d->settings[b];
+        if (!s || s->ctx != ctx)
+            return Fail(XPBD_E_INVALID, "settings " + std::to_string(b) + " missing or from another context");
+        if (s->num_bends || s->num_volumes || s->num_lras)
+            return Fail(XPBD_E_UNSUPPORTED, "rollout bodies may have edge constraints only (body " + std::to_string(b) + ")");
+        if (s->host_level_begin.empty())
+            return Fail(XPBD_E_CAPACITY, "rollout body " + std::to_string(b) + " does not fit in LDS");
+        const uint32_t nv = s->num_vertices, ne = s->num_edges;
+        if (rollout_backward_lds_bytes(nv, ne) > kMaxLdsBytes)
+            return Fail(XPBD_E_CAPACITY, "rollout body " + std::to_string(b) + ": positions and edge adjoints exceed LDS");
+        ++s->refcount;
+        r->settings.push_back(s);
+        RolloutBodyDev &B = bodies[b];
+        B.nv = nv;
+        B.num_edges = ne;
+        B.voff = r->total_nv;
+        B.eoff = r->total_ne;
+        B.rbase = (uint32_t)recs.size();
+        B.nlev = (uint32_t)s->host_level_begin.size() - 1;
+        B.num_records = (uint32_t)s->host_recs.size();
+        B.lvl = reinterpret_cast<const uint32_t *>((uintptr_t)lvl.size());     // index until uploaded
+        for (uint32_t L : s->host_level_begin)
+            lvl.push_back(L);                       // relative to the body's first record
+        for (size_t k = 0; k < s->host_recs.size(); ++k)
+        {
+            uint4 q = s->host_recs[k];
+            if (q.x != kNopRecordHost)
+            {
+                q.y = s->host_rec_index[k];         // edge index in the body
+                q.w = 0u;                           // compliance, written by each float forward pass
+            }
+            recs.push_back(q);
+        }
+        for (const VertexRec &v : s->vertices)
+            x0.push_back(make_float4(v.pos[0], v.pos[1], v.pos[2], v.inv_mass));
+        r->total_nv += nv;
+        r->total_ne += ne;
+        max_nv = std::max(max_nv, nv);
+        r->lds_bwd = std::max(r->lds_bwd, rollout_backward_lds_bytes(nv, ne));
+    }
+    r->lds_f32 = rollout_forward_lds_bytes(max_nv, false);
+    r->lds_f64 = rollout_forward_lds_bytes(max_nv, true);
+    HIP_TRY(r->recs.upload(recs));
+    HIP_TRY(r->lvl.upload(lvl));
+    HIP_TRY(r->x0.upload(x0));
+    for (RolloutBodyDev &B : bodies)
+    {
+        B.recs = r->recs.p + B.rbase;
+        B.lvl = r->lvl.p + (uintptr_t)B.lvl;
+        B.x0 = r->x0.p + B.voff;
+    }
+    HIP_TRY(r->bodies.upload(bodies));
+    const uint32_t num_ckpt = (d->frames + d->checkpoint_interval - 1) / d->checkpoint_interval;
+    const size_t nv = r->total_nv;
+    const size_t seg_sub = (size_t)d->checkpoint_interval * d->num_iterations;
+    HIP_TRY(r->ckpt_x.alloc(num_ckpt * nv)); HIP_TRY(r->ckpt_v.alloc(num_ckpt * nv));
+    HIP_TRY(r->seg_x.alloc(seg_sub * nv)); HIP_TRY(r->seg_v.alloc(seg_sub * nv));
+    HIP_TRY(r->vel.alloc(nv)); HIP_TRY(r->prev.alloc(nv));
+    HIP_TRY(r->vel64.alloc(4 * nv)); HIP_TRY(r->prev64.alloc(4 * nv));
+    HIP_TRY(r->rec_tape.alloc(2 * std::max<size_t>(recs.size(), 1)));
+    HIP_TRY(r->adj_x.alloc(nv)); HIP_TRY(r->adj_v.alloc(nv));
+    RolloutDev &R = r->dev;
+    R.bodies = r->bodies.p;
+    R.num_bodies = d->num_bodies;
+    R.iterations = d->num_iterations;
+    R.frames = d->frames;
+    R.interval = d->checkpoint_interval;
+    R.total_nv = r->total_nv;
+    R.dt = d->dt;
+    R.linear_damping = d->linear_damping;
+    R.max_linear_velocity = d->max_linear_velocity;
+    R.gravity[0] = ctx->gravity[0]; R.gravity[1] = ctx->gravity[1]; R.gravity[2] = ctx->gravity[2];
+    R.gravity_factor = d->gravity_factor;
+    R.ckpt_x = r->ckpt_x.p; R.ckpt_v = r->ckpt_v.p; R.seg_x = r->seg_x.p; R.seg_v = r->seg_v.p;
+    R.vel = r->vel.p; R.prev = r->prev.p; R.vel64 = r->vel64.p; R.prev64 = r->prev64.p;
+    R.rec_tape = r->rec_tape.p; R.adj_x = r->adj_x.p; R.adj_v = r->adj_v.p;
+    *out = r.release();
+    return XPBD_OK;
+}
+
+int xpbd_rollout_destroy(xpbd_rollout *r)
+{
+    if (!r)
+        return Fail(XPBD_E_INVALID, "null rollout");
+    (void)hipSetDevice(r->ctx->device);
+    (void)hipDeviceSynchronize();       // work may be queued on any stream the caller passed
+    delete r;
+    return XPBD_OK;
+}
+
+int xpbd_rollout_sizes(xpbd_rollout *r, uint32_t *num_edges, uint32_t *num_vertices)
+{
+    if (!r)
+        return Fail(XPBD_E_INVALID, "null rollout");
+    if (num_edges) *num_edges = r->total_ne;
+    if (num_vertices) *num_vertices = r->total_nv;
+    return XPBD_OK;
+}
+
+int xpbd_rollout_forward(xpbd_rollout *r, const float *compliance, const float *velocity, float *positions, void *stream)
+{
+    if (!r || !compliance || !velocity || !positions)
+        return Fail(XPBD_E_INVALID, "null argument");
+    HIP_TRY(hipSetDevice(r->ctx->device));
+    HIP_TRY(launch_rollout_forward_f32(r->dev, compliance, velocity, positions, r->lds_f32, StreamOf(r, stream)));
+    r->has_forward = true;
+    return XPBD_OK;
+}
+
+int xpbd_rollout_forward_f64(xpbd_rollout *r, const double *compliance, const double *velocity, double *positions, void *stream)
+{
+    if (!r || !compliance || !velocity || !positions)
+        return Fail(XPBD_E_INVALID, "null argument");
+    if (r->lds_f64 > kMaxLdsBytes)
+        return Fail(XPBD_E_CAPACITY, "double positions of the largest body exceed LDS");
+    HIP_TRY(hipSetDevice(r->ctx->device));
+    HIP_TRY(launch_rollout_forward_f64(r->dev, compliance, velocity, positions, r->lds_f64, StreamOf(r, stream)));
+    return XPBD_OK;
+}
+
+int xpbd_rollout_backward(xpbd_rollout *r, const float *grad_positions, float *grad_compliance, float *grad_velocity, void *stream)
+{
+    if (!r || !grad_positions || !grad_compliance || !grad_velocity)
+        return Fail(XPBD_E_INVALID, "null argument");
+    if (!r->has_forward)
+        return Fail(XPBD_E_INVALID, "backward needs a float forward pass first");
+    HIP_TRY(hipSetDevice(r->ctx->device));
+    HIP_TRY(launch_rollout_backward(r->dev, grad_positions, grad_compliance, grad_velocity, r->lds_bwd, StreamOf(r, stream)));
     return XPBD_OK;
 }
 

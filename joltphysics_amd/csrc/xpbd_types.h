@@ -121,3 +121,37 @@ struct BundleDev
 };
 
 } // namespace xpbd
+
+namespace xpbd {
+
+// Differentiable rollout: one workgroup per body.  Records of the body's
+// level program carry {a: v0 | type | v1, b: edge index in the body, z: rest
+// length}; the compliance comes from the parameter array.
+struct RolloutBodyDev
+{
+    const uint4 *recs;
+    const uint32_t *lvl;        // nlev + 1 absolute offsets
+    uint32_t nlev, num_records;
+    uint32_t nv, num_edges;
+    uint32_t voff, eoff;        // first vertex / edge in the concatenated arrays
+    uint32_t rbase;             // first record slot in the record tape
+    const float4 *x0;           // initial positions + inverse mass
+};
+
+struct RolloutDev
+{
+    const RolloutBodyDev *bodies;
+    uint32_t num_bodies, iterations, frames, interval, total_nv;
+    float dt, linear_damping, max_linear_velocity;
+    float gravity[3];           // world gravity (bodies have identity rotation, so also body-local)
+    float gravity_factor;
+    float4 *ckpt_x, *ckpt_v;    // (x + inverse mass, v) at frames 0, interval, 2 interval, ...
+    float4 *seg_x, *seg_v;      // state at every sub-step start of the segment being reversed
+    float4 *vel, *prev;         // float forward state
+    double *vel64, *prev64;     // double forward state (4 per vertex)
+    float4 *rec_tape;           // 2 per record: pre-projection positions (+ inverse mass)
+    float4 *adj_x, *adj_v;      // adjoint state
+};
+
+} // namespace xpbd
+

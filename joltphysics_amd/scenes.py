@@ -17,7 +17,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .records import (BEND_DIHEDRAL, EDGE, LRA_NONE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_PLANE,
+from .records import (BEND_DIHEDRAL, BEND_NONE, EDGE, LRA_NONE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_PLANE,
                       SHAPE_SPHERE, VERTEX, VOLUME, BodyParams, Collider)
 
 # Build flags (same bit meaning on the product and oracle builders)
@@ -309,6 +309,41 @@ def _uniform_compliance(c):
         s.bends["compliance"] = np.float32(c)
         s.volumes["compliance"] = np.float32(c)
     return apply
+
+
+@dataclass
+class RolloutCase:
+    """Differentiation workload: `num_bodies` copies of one edge-only cloth, per-edge
+    compliance and initial velocities as parameters, and a target produced by a forward
+    run with the compliance scaled per edge."""
+
+    name: str
+    mesh: MeshSpec
+    num_bodies: int
+    frames: int
+    iterations: int
+    dt: float
+    gravity: tuple
+    compliance: np.ndarray          # (num_bodies * E,) float32: the parameter values
+    velocity: np.ndarray            # (num_bodies * V, 3) float32
+    target_scale: np.ndarray        # (num_bodies * E,) float32: compliance multiplier of the target run
+    checkpoint_interval: int = 10
+
+
+def config4_rollout(num_bodies=64, n=64, frames=60, seed=4, compliance=1e-3):
+    """64 cloths of 64 x 64 vertices (spacing 0.05 m, inv-mass 100, two top corners
+    pinned), structural + shear edges only (compliance 1e-3), gravity -9.81 z, dt 1/60,
+    one collision step x 5 iterations, 60 frames, no colliders.  Initial velocities
+    U(-0.1, 0.1) m/s and target compliance multipliers U(0.5, 2.0) per edge, from `seed`."""
+    v, f = cloth(n, 0.05, 100.0, pinned=((n - 1) * n, n * n - 1))
+    mesh = MeshSpec(vertices=v, faces=f, attrs=attrs(compliance, compliance, 0.0), bend_type=BEND_NONE)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    num_edges = 2 * n * (n - 1) + 2 * (n - 1) * (n - 1)     # structural + shear
+    comp = np.full(num_bodies * num_edges, compliance, np.float32)
+    vel = rng.uniform(-0.1, 0.1, size=(num_bodies * n * n, 3)).astype(np.float32)
+    scale = rng.uniform(0.5, 2.0, size=num_bodies * num_edges).astype(np.float32)
+    return RolloutCase(f"cloth{n}x{n}x{num_bodies}_rollout", mesh, num_bodies, frames, 5, 1.0 / 60.0,
+                       (0.0, 0.0, -9.81), comp, vel, scale)
 
 
 def mesh_for_key(key):

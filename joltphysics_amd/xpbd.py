@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -55,6 +56,12 @@ class _ColliderDesc(C.Structure):
                 ("world_bounds", C.c_float * 6)]
 
 
+class _RolloutDesc(C.Structure):
+    _fields_ = [("settings", C.c_void_p), ("num_bodies", C.c_uint32), ("num_iterations", C.c_uint32),
+                ("linear_damping", C.c_float), ("max_linear_velocity", C.c_float), ("gravity_factor", C.c_float),
+                ("dt", C.c_float), ("frames", C.c_uint32), ("checkpoint_interval", C.c_uint32)]
+
+
 class _VertexAttributes(C.Structure):
     _fields_ = [("compliance", C.c_float), ("shear_compliance", C.c_float), ("bend_compliance", C.c_float),
                 ("lra_type", C.c_int32), ("lra_max_distance_multiplier", C.c_float)]
@@ -78,16 +85,26 @@ EXPORTED_SYMBOLS = (
     "xpbd_build_last_error", "xpbd_body_create",
     "xpbd_set_colliders", "xpbd_set_gravity", "xpbd_update", "xpbd_synchronize", "xpbd_body_read",
     "xpbd_body_write", "xpbd_body_state", "xpbd_profile_enable", "xpbd_profile_read",
+    "xpbd_rollout_create", "xpbd_rollout_destroy", "xpbd_rollout_sizes", "xpbd_rollout_forward",
+    "xpbd_rollout_forward_f64", "xpbd_rollout_backward",
 )
 
 _lib = None
 
 
 def load_library(path=LIB_PATH):
-    """Loads libjolt_xpbd.so; raises if it has not been built."""
+    """Loads libjolt_xpbd.so; raises if it has not been built.
+
+    PyTorch ships its own HIP runtime under the same soname; importing it first
+    makes the library bind to that one runtime, so torch device pointers and
+    stream handles are valid for the library's kernels."""
     global _lib
     if _lib is not None:
         return _lib
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not os.path.exists(path):
         raise XpbdError(f"HIP library {path} is missing; run joltphysics_amd/build.py "
                         "(there is no CPU fallback)")
@@ -116,6 +133,12 @@ def load_library(path=LIB_PATH):
     lib.xpbd_body_state.argtypes = [vp, i32, vp]
     lib.xpbd_profile_enable.argtypes = [vp, i32]
     lib.xpbd_profile_read.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_double)]
+    lib.xpbd_rollout_create.argtypes = [vp, C.POINTER(_RolloutDesc), C.POINTER(vp)]
+    lib.xpbd_rollout_destroy.argtypes = [vp]
+    lib.xpbd_rollout_sizes.argtypes = [vp, C.POINTER(u32), C.POINTER(u32)]
+    lib.xpbd_rollout_forward.argtypes = [vp] * 5
+    lib.xpbd_rollout_forward_f64.argtypes = [vp] * 5
+    lib.xpbd_rollout_backward.argtypes = [vp] * 5
     for name in EXPORTED_SYMBOLS:
         if name not in ("xpbd_last_error", "xpbd_context_stream", "xpbd_build_last_error"):
             getattr(lib, name).restype = C.c_int
@@ -199,6 +222,9 @@ class Context:
             self.h = None
 
     def __del__(self):
+        # at interpreter exit the HIP runtime may already be torn down: leave it to the process exit
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -304,6 +330,12 @@ class Context:
         return dict(linear_velocity=out[0:3], angular_velocity=out[3:6], bounds_min=out[6:9],
                     bounds_max=out[9:12], active=bool(out[12]), sleep_timer=float(out[13]))
 
+    def create_rollout(self, settings, num_iterations=5, dt=1.0 / 60.0, frames=60, checkpoint_interval=10,
+                       linear_damping=0.1, max_linear_velocity=500.0, gravity_factor=1.0) -> "Rollout":
+        """Differentiable rollout of bodies with edge constraints only (see Rollout)."""
+        return Rollout(self, settings, num_iterations, dt, frames, checkpoint_interval, linear_damping,
+                       max_linear_velocity, gravity_factor)
+
     def profile_enable(self, on=True):
         _check(self.lib.xpbd_profile_enable(self.h, int(on)))
 
@@ -312,3 +344,59 @@ class Context:
         ms = C.c_double()
         _check(self.lib.xpbd_profile_read(self.h, C.byref(n), C.byref(ms)))
         return n.value, ms.value
+
+
+class Rollout:
+    """Differentiable rollout (xpbd_rollout_* in include/jolt_xpbd.h).
+
+    Bodies start at their settings' vertex positions with identity rotation and
+    run `frames` frames of one collision step and `num_iterations` iterations.
+    Parameters: per-edge compliance (bodies' mEdgeConstraints concatenated)
+    and initial velocities (3 per vertex).  All array arguments are device
+    pointers (ints, e.g. torch.Tensor.data_ptr()); `stream` is the hipStream_t
+    handle the work is ordered on (None or 0: the default stream, where torch's
+    default stream runs).
+    """
+
+    def __init__(self, ctx, settings, num_iterations, dt, frames, checkpoint_interval, linear_damping,
+                 max_linear_velocity, gravity_factor):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.settings = list(settings)
+        handles = (C.c_void_p * len(self.settings))(*[s.handle.value for s in self.settings])
+        d = _RolloutDesc(C.cast(handles, C.c_void_p), len(self.settings), int(num_iterations), float(linear_damping),
+                         float(max_linear_velocity), float(gravity_factor), float(dt), int(frames),
+                         int(checkpoint_interval))
+        h = C.c_void_p()
+        _check(self.lib.xpbd_rollout_create(ctx.h, C.byref(d), C.byref(h)))
+        self.h = h
+        ne, nv = C.c_uint32(), C.c_uint32()
+        _check(self.lib.xpbd_rollout_sizes(h, C.byref(ne), C.byref(nv)))
+        self.num_edges, self.num_vertices = ne.value, nv.value
+        self.frames, self.num_iterations, self.dt = int(frames), int(num_iterations), float(dt)
+        self.generation = 0     # float forward passes so far (the adjoint differentiates the last one)
+
+    def close(self):
+        if self.h:
+            self.lib.xpbd_rollout_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        # at interpreter exit the HIP runtime may already be torn down: leave it to the process exit
+        if sys.is_finalizing():
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, compliance_ptr, velocity_ptr, positions_ptr, stream=None):
+        _check(self.lib.xpbd_rollout_forward(self.h, compliance_ptr, velocity_ptr, positions_ptr, stream))
+        self.generation += 1
+
+    def forward_f64(self, compliance_ptr, velocity_ptr, positions_ptr, stream=None):
+        _check(self.lib.xpbd_rollout_forward_f64(self.h, compliance_ptr, velocity_ptr, positions_ptr, stream))
+
+    def backward(self, grad_positions_ptr, grad_compliance_ptr, grad_velocity_ptr, stream=None):
+        _check(self.lib.xpbd_rollout_backward(self.h, grad_positions_ptr, grad_compliance_ptr, grad_velocity_ptr,
+                                              stream))

@@ -303,3 +303,60 @@ class Restatement:
 
     def acos_approximate(self, x):
         return self.lib.xr_acos_approximate(float(x))
+
+
+class RolloutReference:
+    """CPU restatement of the differentiable edge-only rollout (oracle/xpbd_adjoint.cpp):
+    FP32 or FP64 forward pass and the FP64 adjoint from a full tape."""
+
+    _lib = None
+
+    def __init__(self, settings: SettingsArrays, iterations=5, frames=60, dt=1.0 / 60.0, linear_damping=0.1,
+                 max_linear_velocity=500.0, gravity=(0.0, 0.0, -9.81), gravity_factor=1.0):
+        if RolloutReference._lib is None:
+            lib = C.CDLL(restatement_path())
+            common = [_u32, _vp, _vp, _u32, _vp, _vp, _u32, _u32, _f, _f, _f, _vp, _f]
+            lib.xa_rollout_forward.restype = _i
+            lib.xa_rollout_forward.argtypes = [_i] + common + [_vp, _vp, _vp]
+            lib.xa_rollout_gradient.restype = _i
+            lib.xa_rollout_gradient.argtypes = [_i] + common + [_vp] * 6
+            RolloutReference._lib = lib
+        self.lib = RolloutReference._lib
+        if len(settings.bends) or len(settings.volumes) or len(settings.lras):
+            raise ValueError("the rollout oracle covers edge constraints only")
+        v = settings.vertices
+        self.nv, self.ne = len(v), len(settings.edges)
+        self.pos = np.ascontiguousarray(v["pos"], np.float32).reshape(-1)
+        self.inv_mass = np.ascontiguousarray(v["inv_mass"], np.float32)
+        self.ev = np.ascontiguousarray(settings.edges["v"], np.uint32).reshape(-1)
+        self.rest = np.ascontiguousarray(settings.edges["rest"], np.float32)
+        self.gravity = np.asarray(gravity, np.float32)
+        self.args = (int(iterations), int(frames), float(dt), float(linear_damping), float(max_linear_velocity))
+        self.gravity_factor = float(gravity_factor)
+
+    def _common(self):
+        return [self.nv, _ptr(self.pos), _ptr(self.inv_mass), self.ne, _ptr(self.ev), _ptr(self.rest),
+                *self.args, _ptr(self.gravity), self.gravity_factor]
+
+    def forward(self, compliance, velocity, precision=64):
+        c = np.ascontiguousarray(compliance, np.float64).reshape(-1)
+        vel = np.ascontiguousarray(velocity, np.float64).reshape(-1)
+        out = np.zeros(3 * self.nv, np.float64)
+        rc = self.lib.xa_rollout_forward(int(precision), *self._common(), _ptr(c), _ptr(vel), _ptr(out))
+        if rc != 0:
+            raise RuntimeError("xa_rollout_forward failed")
+        return out.reshape(-1, 3)
+
+    def gradient(self, compliance, velocity, grad_positions, forward_precision=64):
+        """FP64 adjoint on the tape of the FP64 (64) or FP32 (32) forward pass."""
+        c = np.ascontiguousarray(compliance, np.float64).reshape(-1)
+        vel = np.ascontiguousarray(velocity, np.float64).reshape(-1)
+        g = np.ascontiguousarray(grad_positions, np.float64).reshape(-1)
+        gc = np.zeros(self.ne, np.float64)
+        gv = np.zeros(3 * self.nv, np.float64)
+        out = np.zeros(3 * self.nv, np.float64)
+        rc = self.lib.xa_rollout_gradient(int(forward_precision), *self._common(), _ptr(c), _ptr(vel), _ptr(g), _ptr(gc),
+                                          _ptr(gv), _ptr(out))
+        if rc != 0:
+            raise RuntimeError("xa_rollout_gradient failed")
+        return out.reshape(-1, 3), gc, gv.reshape(-1, 3)

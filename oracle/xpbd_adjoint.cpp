@@ -1,0 +1,300 @@
+// CPU restatement of the differentiable rollout and its adjoint.
+//
+// TEST INFRASTRUCTURE ONLY -- this is the oracle for the GPU rollout
+// (include/jolt_xpbd.h, xpbd_rollout_*).  It is loaded by tests/ as the
+// checker; the product never links, loads or calls it.
+//
+// Scope: one body with edge constraints only, no colliders, no pressure, one
+// collision step per frame, identity rotation.  The update is restated from
+// the reference (paths relative to its Jolt/ directory):
+//   InitializeUpdateContext        Physics/SoftBody/SoftBodyMotionProperties.cpp:937-960
+//   IntegratePositions             :325-350
+//   ProcessGroup / ApplyEdgeConstraints :1030-1060, :573-601 -- with edge
+//       constraints only, the update groups in order, each in array order, visit
+//       the edges in array order: a plain sequential Gauss-Seidel sweep
+//   velocity update                :700-710 (no colliding shape)
+//   velocity clamp                 UpdateSoftBodyState :852-857
+// The forward pass is templated (float: the FP32 update, double: the FP64
+// re-run).  The adjoint is FP64 and works from a full tape of every sub-step:
+// its state, every edge's pre-projection positions and the projected positions.
+// It is pinned by the tests against central finite differences of the FP64
+// forward pass.  It can also run on the tape of the FP32 forward pass (the
+// FP32 program's gradient in FP64 arithmetic), which separates the rounding of
+// a float adjoint from the FP32 trajectory's own departure from the FP64 one.
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+template <class F> struct V { F x, y, z; };
+template <class F> inline V<F> mk(F x, F y, F z) { return V<F>{ x, y, z }; }
+template <class F> inline V<F> operator+(V<F> a, V<F> b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <class F> inline V<F> operator-(V<F> a, V<F> b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <class F> inline V<F> operator*(V<F> a, F s) { return mk(a.x * s, a.y * s, a.z * s); }
+template <class F> inline V<F> operator*(F s, V<F> a) { return mk(s * a.x, s * a.y, s * a.z); }
+template <class F> inline V<F> operator/(V<F> a, F s) { return mk(a.x / s, a.y / s, a.z / s); }
+template <class F> inline F dot(V<F> a, V<F> b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }   // Vec3.inl:901-934
+
+struct Spec
+{
+    uint32_t nv, ne;
+    const float *pos;       // 3 per vertex
+    const float *inv_mass;
+    const uint32_t *ev;     // 2 per edge
+    const float *rest;
+    uint32_t iterations, frames;
+    float dt, linear_damping, max_linear_velocity, gravity[3], gravity_factor;
+};
+
+template <class F>
+struct Consts
+{
+    F dt_sub, inv_dt_sq, damping, max_lin_sq, impulse;
+    V<F> sub_step_gravity;
+    explicit Consts(const Spec &s)
+    {
+        const F step_dt = (F)s.dt / (F)1;                      // PhysicsSystem.cpp:215, one collision step
+        dt_sub = step_dt / (F)s.iterations;                    // SoftBodyMotionProperties.cpp:953
+        inv_dt_sq = (F)1 / (dt_sub * dt_sub);
+        const F d = (F)1 - (F)s.linear_damping * dt_sub;       // :331, MotionProperties max(0, 1 - c dt)
+        damping = (F)0 > d ? (F)0 : d;
+        const F gf = (F)s.gravity_factor;
+        const V<F> g = mk(gf * (F)s.gravity[0], gf * (F)s.gravity[1], gf * (F)s.gravity[2]);   // :949, identity rotation
+        sub_step_gravity = g * dt_sub;                         // :334
+        const F nvf = (F)s.nv;
+        impulse = ((F)0 * dt_sub) / (nvf > (F)1 ? nvf : (F)1); // :335, no accumulated force
+        max_lin_sq = (F)s.max_linear_velocity * (F)s.max_linear_velocity;
+    }
+    V<F> integrate(V<F> v, F w) const                          // :336-345
+    {
+        if (w > (F)0)
+        {
+            v = v + (sub_step_gravity + mk(impulse * w, impulse * w, impulse * w));
+            v = v * damping;
+        }
+        return v;
+    }
+};
+
+struct Tape
+{
+    std::vector<double> x0, v0, pre, y;     // per sub-step: 3 nv, 3 nv, 6 ne, 3 nv
+};
+
+// Forward pass; with a tape, records every sub-step (double only).
+template <class F>
+void Forward(const Spec &s, const F *compliance, const F *velocity, F *out, Tape *tape)
+{
+    const Consts<F> K(s);
+    const uint32_t nv = s.nv, ne = s.ne;
+    std::vector<V<F>> x(nv), v(nv), prev(nv);
+    std::vector<F> w(nv);
+    for (uint32_t i = 0; i < nv; ++i)
+    {
+        x[i] = mk((F)s.pos[3 * i], (F)s.pos[3 * i + 1], (F)s.pos[3 * i + 2]);
+        w[i] = (F)s.inv_mass[i];
+        v[i] = mk(velocity[3 * i], velocity[3 * i + 1], velocity[3 * i + 2]);
+    }
+    for (uint32_t frame = 0; frame < s.frames; ++frame)
+        for (uint32_t it = 0; it < s.iterations; ++it)
+        {
+            if (tape)
+                for (uint32_t i = 0; i < nv; ++i)
+                {
+                    const double a[6] = { (double)x[i].x, (double)x[i].y, (double)x[i].z, (double)v[i].x, (double)v[i].y, (double)v[i].z };
+                    tape->x0.insert(tape->x0.end(), a, a + 3);
+                    tape->v0.insert(tape->v0.end(), a + 3, a + 6);
+                }
+            for (uint32_t i = 0; i < nv; ++i)                  // IntegratePositions
+            {
+                v[i] = K.integrate(v[i], w[i]);
+                prev[i] = x[i];
+                x[i] = x[i] + v[i] * K.dt_sub;
+            }
+            for (uint32_t e = 0; e < ne; ++e)                  // ApplyEdgeConstraints :583-600
+            {
+                const uint32_t i0 = s.ev[2 * e], i1 = s.ev[2 * e + 1];
+                if (tape)
+                {
+                    const double a[6] = { (double)x[i0].x, (double)x[i0].y, (double)x[i0].z, (double)x[i1].x, (double)x[i1].y, (double)x[i1].z };
+                    tape->pre.insert(tape->pre.end(), a, a + 6);
+                }
+                const V<F> delta = x[i1] - x[i0];
+                const F length = std::sqrt(dot(delta, delta));
+                const F denom = length * ((w[i0] + w[i1]) + compliance[e] * K.inv_dt_sq);
+                if (denom < (F)1.0e-12f)
+                    continue;
+                const V<F> correction = (delta * (length - (F)s.rest[e])) / denom;
+                x[i0] = x[i0] + w[i0] * correction;
+                x[i1] = x[i1] - w[i1] * correction;
+            }
+            if (tape)
+                for (uint32_t i = 0; i < nv; ++i)
+                {
+                    const double a[3] = { (double)x[i].x, (double)x[i].y, (double)x[i].z };
+                    tape->y.insert(tape->y.end(), a, a + 3);
+                }
+            const bool last = it + 1 == s.iterations;
+            for (uint32_t i = 0; i < nv; ++i)                  // velocity update :700-710, clamp :852-857
+            {
+                if (w[i] > (F)0)
+                    v[i] = (x[i] - prev[i]) / K.dt_sub;
+                if (last)
+                {
+                    const F v_sq = dot(v[i], v[i]);
+                    if (v_sq > K.max_lin_sq)
+                        v[i] = v[i] * std::sqrt(K.max_lin_sq / v_sq);
+                }
+            }
+        }
+    for (uint32_t i = 0; i < nv; ++i)
+    {
+        out[3 * i] = x[i].x; out[3 * i + 1] = x[i].y; out[3 * i + 2] = x[i].z;
+    }
+}
+
+// Reverse pass over the tape, in FP64.  forward_precision 64: the tape of the
+// FP64 forward pass (the FP64 program's gradient); 32: the tape of the FP32
+// forward pass (the FP32 program's Jacobians, evaluated in FP64 arithmetic).
+void Adjoint(const Spec &s, const double *compliance, const double *velocity, const double *grad_pos, double *grad_comp,
+             double *grad_vel, double *out_pos, int forward_precision)
+{
+    Tape tape;
+    if (forward_precision == 32)
+    {
+        std::vector<float> c(compliance, compliance + s.ne), v(velocity, velocity + 3 * (size_t)s.nv), out(3 * (size_t)s.nv);
+        Forward<float>(s, c.data(), v.data(), out.data(), &tape);
+        for (size_t k = 0; k < out.size(); ++k)
+            out_pos[k] = out[k];
+    }
+    else
+        Forward<double>(s, compliance, velocity, out_pos, &tape);
+    const Consts<double> K(s);
+    const uint32_t nv = s.nv, ne = s.ne, I = s.iterations;
+    const double h = K.dt_sub;
+    std::vector<V<double>> xb(nv), vb(nv), A(nv);
+    for (uint32_t i = 0; i < nv; ++i)
+    {
+        xb[i] = mk(grad_pos[3 * i], grad_pos[3 * i + 1], grad_pos[3 * i + 2]);
+        vb[i] = mk(0.0, 0.0, 0.0);
+    }
+    for (uint32_t e = 0; e < ne; ++e)
+        grad_comp[e] = 0.0;
+    const uint32_t nsub = s.frames * I;
+    for (uint32_t sub = nsub; sub-- > 0;)
+    {
+        const bool last = sub % I == I - 1;
+        const double *x0 = &tape.x0[(size_t)sub * 3 * nv], *v0 = &tape.v0[(size_t)sub * 3 * nv];
+        const double *y = &tape.y[(size_t)sub * 3 * nv], *pre = &tape.pre[(size_t)sub * 6 * ne];
+        for (uint32_t i = 0; i < nv; ++i)
+        {
+            const double w = s.inv_mass[i];
+            const V<double> vv1 = K.integrate(mk(v0[3 * i], v0[3 * i + 1], v0[3 * i + 2]), w);
+            const V<double> yy = mk(y[3 * i], y[3 * i + 1], y[3 * i + 2]), xx0 = mk(x0[3 * i], x0[3 * i + 1], x0[3 * i + 2]);
+            const V<double> v2 = w > 0.0 ? (yy - xx0) / h : vv1;
+            if (last)
+            {
+                const double v_sq = dot(v2, v2);
+                if (v_sq > K.max_lin_sq)
+                {
+                    const double l = std::sqrt(v_sq);
+                    const V<double> n = v2 / l;
+                    vb[i] = (std::sqrt(K.max_lin_sq) / l) * (vb[i] - dot(n, vb[i]) * n);
+                }
+            }
+            A[i] = w > 0.0 ? xb[i] + vb[i] / h : xb[i];
+        }
+        for (uint32_t e = ne; e-- > 0;)
+        {
+            const uint32_t i0 = s.ev[2 * e], i1 = s.ev[2 * e + 1];
+            const double w0 = s.inv_mass[i0], w1 = s.inv_mass[i1];
+            const V<double> p0 = mk(pre[6 * e], pre[6 * e + 1], pre[6 * e + 2]), p1 = mk(pre[6 * e + 3], pre[6 * e + 4], pre[6 * e + 5]);
+            const V<double> delta = p1 - p0;
+            const double L = std::sqrt(dot(delta, delta));
+            const double sc = (w0 + w1) + compliance[e] * K.inv_dt_sq;
+            if (L * sc < (double)1.0e-12f)
+                continue;
+            const double c = L - (double)s.rest[e];
+            const V<double> corr = (delta * c) / (L * sc);
+            const V<double> cbar = w0 * A[i0] - w1 * A[i1];
+            const V<double> n = delta / L;
+            const V<double> dbar = ((c / L) * cbar + ((double)s.rest[e] / L * dot(n, cbar)) * n) / sc;
+            A[i0] = A[i0] - dbar;
+            A[i1] = A[i1] + dbar;
+            grad_comp[e] += dot(cbar, corr) * (-K.inv_dt_sq / sc);
+        }
+        for (uint32_t i = 0; i < nv; ++i)
+        {
+            const double w = s.inv_mass[i];
+            V<double> x0b = A[i], v1b = A[i] * h;
+            if (w > 0.0)
+                x0b = x0b - vb[i] / h;
+            else
+                v1b = v1b + vb[i];
+            xb[i] = x0b;
+            vb[i] = w > 0.0 ? v1b * K.damping : v1b;
+        }
+    }
+    for (uint32_t i = 0; i < nv; ++i)
+    {
+        grad_vel[3 * i] = vb[i].x; grad_vel[3 * i + 1] = vb[i].y; grad_vel[3 * i + 2] = vb[i].z;
+    }
+}
+
+Spec MakeSpec(uint32_t nv, const float *pos, const float *inv_mass, uint32_t ne, const uint32_t *ev, const float *rest,
+              uint32_t iterations, uint32_t frames, float dt, float linear_damping, float max_linear_velocity,
+              const float *gravity, float gravity_factor)
+{
+    Spec s;
+    s.nv = nv; s.ne = ne; s.pos = pos; s.inv_mass = inv_mass; s.ev = ev; s.rest = rest;
+    s.iterations = iterations; s.frames = frames; s.dt = dt; s.linear_damping = linear_damping;
+    s.max_linear_velocity = max_linear_velocity;
+    s.gravity[0] = gravity[0]; s.gravity[1] = gravity[1]; s.gravity[2] = gravity[2];
+    s.gravity_factor = gravity_factor;
+    return s;
+}
+
+} // namespace
+
+extern "C" {
+
+// precision 32: FP32 forward (parameters rounded to float); 64: FP64.
+int xa_rollout_forward(int precision, uint32_t nv, const float *pos, const float *inv_mass, uint32_t ne, const uint32_t *ev,
+                       const float *rest, uint32_t iterations, uint32_t frames, float dt, float linear_damping,
+                       float max_linear_velocity, const float *gravity, float gravity_factor, const double *compliance,
+                       const double *velocity, double *positions)
+{
+    const Spec s = MakeSpec(nv, pos, inv_mass, ne, ev, rest, iterations, frames, dt, linear_damping, max_linear_velocity,
+                            gravity, gravity_factor);
+    if (precision == 64)
+    {
+        Forward<double>(s, compliance, velocity, positions, nullptr);
+        return 0;
+    }
+    if (precision != 32)
+        return -1;
+    std::vector<float> c(compliance, compliance + ne), v(velocity, velocity + 3 * (size_t)nv), out(3 * (size_t)nv);
+    Forward<float>(s, c.data(), v.data(), out.data(), nullptr);
+    for (size_t k = 0; k < out.size(); ++k)
+        positions[k] = out[k];
+    return 0;
+}
+
+int xa_rollout_gradient(int forward_precision, uint32_t nv, const float *pos, const float *inv_mass, uint32_t ne,
+                        const uint32_t *ev, const float *rest, uint32_t iterations, uint32_t frames, float dt,
+                        float linear_damping, float max_linear_velocity, const float *gravity, float gravity_factor,
+                        const double *compliance, const double *velocity, const double *grad_positions,
+                        double *grad_compliance, double *grad_velocity, double *positions)
+{
+    if (forward_precision != 32 && forward_precision != 64)
+        return -1;
+    const Spec s = MakeSpec(nv, pos, inv_mass, ne, ev, rest, iterations, frames, dt, linear_damping, max_linear_velocity,
+                            gravity, gravity_factor);
+    Adjoint(s, compliance, velocity, grad_positions, grad_compliance, grad_velocity, positions, forward_precision);
+    return 0;
+}
+
+} // extern "C"

@@ -1,0 +1,85 @@
+"""The FP64 CPU oracle of the differentiable rollout (oracle/xpbd_adjoint.cpp).
+
+Its adjoint is pinned against central finite differences of its own FP64
+forward pass: the derivative is fixed by the mathematics, so a wrong Jacobian
+term (a dropped rest-length term, a sign, the compliance factor) fails here.
+Its FP32 forward pass is pinned bit for bit against the reference-parity
+restatement of the update (the FP32 update of an edge-only body without
+colliders).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from joltphysics_amd import scenes
+from joltphysics_amd.records import BodyParams
+from joltphysics_amd.xpbd import build_settings
+from oracle import RolloutReference, have_restatement
+
+pytestmark = pytest.mark.skipif(not have_restatement(), reason="oracle restatement not built (make -C oracle restatement)")
+
+
+def _case(n=6, frames=8):
+    case = scenes.config4_rollout(num_bodies=1, n=n, frames=frames)
+    s = build_settings(case.mesh)
+    ref = RolloutReference(s, iterations=case.iterations, frames=frames, dt=case.dt, gravity=case.gravity)
+    comp = case.compliance.astype(np.float64)
+    vel = case.velocity.astype(np.float64)
+    target = ref.forward(comp * case.target_scale, vel)
+    return case, s, ref, comp, vel, target
+
+
+def _loss(ref, target, c, v):
+    x = ref.forward(c, v)
+    return float(np.mean(np.sum((x - target) ** 2, axis=1)))
+
+
+def test_adjoint_matches_central_differences_fp64():
+    case, s, ref, comp, vel, target = _case()
+    x = ref.forward(comp, vel)
+    g = 2.0 * (x - target) / ref.nv
+    x2, gc, gv = ref.gradient(comp, vel, g)
+    assert np.array_equal(x, x2)
+    scale_c = np.abs(gc).max()
+    for e in range(0, ref.ne, 5):
+        h = 1e-3 * comp[e]
+        cp, cm = comp.copy(), comp.copy()
+        cp[e] += h
+        cm[e] -= h
+        fd = (_loss(ref, target, cp, vel) - _loss(ref, target, cm, vel)) / (2 * h)
+        assert abs(fd - gc[e]) <= 1e-5 * max(abs(fd), 1e-3 * scale_c), (e, fd, gc[e])
+    flat = vel.reshape(-1)
+    gvf = gv.reshape(-1)
+    scale_v = np.abs(gvf).max()
+    for k in range(0, flat.size, 7):
+        h = 1e-5
+        vp, vm = flat.copy(), flat.copy()
+        vp[k] += h
+        vm[k] -= h
+        fd = (_loss(ref, target, comp, vp.reshape(-1, 3)) - _loss(ref, target, comp, vm.reshape(-1, 3))) / (2 * h)
+        assert abs(fd - gvf[k]) <= 1e-5 * max(abs(fd), 1e-3 * scale_v), (k, fd, gvf[k])
+
+
+def test_fp32_forward_matches_update_restatement():
+    """The rollout oracle's FP32 pass equals the parity restatement of the full update
+    on the same body (no colliders, identity rotation, no recentring)."""
+    from oracle import Restatement
+    case, s, ref, comp, vel, target = _case(n=8, frames=6)
+    x32 = ref.forward(comp, vel, precision=32)
+    st = s.copy()
+    st.edges["compliance"] = comp.astype(np.float32)
+    st.vertices["vel"] = vel.astype(np.float32)
+    r = Restatement(case.gravity, 32)
+    r.add_soft_body(st, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0),
+                    BodyParams(num_iterations=case.iterations, allow_sleeping=False, update_position=False))
+    r.update(case.dt, 1, 6)
+    out = r.read(0)
+    r.close()
+    assert np.array_equal(out["pos"].astype(np.float64), x32)
+
+
+def test_fp32_forward_close_to_fp64():
+    case, s, ref, comp, vel, target = _case(n=8, frames=20)
+    d = np.abs(ref.forward(comp, vel, precision=32) - ref.forward(comp, vel, precision=64)).max()
+    assert d < 1e-4, d
