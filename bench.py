@@ -119,6 +119,18 @@ def cpu_baseline(scene, settings, target_seconds=15.0, max_bodies=256):
         return None
 
 
+def measured_traffic(workload):
+    """HBM bytes per frame of the update measured with rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    (calibrated against a plain copy), as recorded in profiles/<workload>_hbm_traffic.md."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f"{workload}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_frame:"):
+            return float(line.split(":")[1]), os.path.relpath(path, os.path.dirname(os.path.abspath(__file__)))
+    return None, None
+
+
 def differentiation_run(warmup=1, repeats=3):
     """fwd+bwd of the differentiation workload (64 cloths 64 x 64, 60 frames x 5 iterations,
     checkpoints every 10 frames): the float forward pass and the adjoint through the
@@ -175,7 +187,20 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-diff", action="store_true", help="skip the fwd+bwd differentiation measurement")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB three times (the known-size reference "
+                         "for HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        n = args.calibrate_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "independent bodies would shard across GPUs by cost with no exchange", file=sys.stderr)
@@ -219,9 +244,13 @@ def main():
     value = solves * args.steps / wall
     bytes_frame = algorithmic_bytes_per_frame(scene, settings)
     achieved = bytes_frame * args.steps / (kernel_ms / 1000.0) / 1e9
+    traffic, traffic_src = measured_traffic(scene.name)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "update_kernel (all variants of one launch)",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None if traffic is None else traffic * args.steps,
+                "traffic_source": traffic_src,
+                "kernel": "one xpbd_update: bundle_kernel launches (one per LDS bin) + update_kernel (streamed bodies), "
+                          "concurrent on four streams, timed fork to join",
                 "algorithmic_bytes_per_launch": bytes_frame * args.steps,
                 "launch_ms": round(kernel_ms / max(n_launch, 1), 3)}
     ctx.close()

@@ -123,6 +123,7 @@ typedef struct xpbd_build_desc {
     const uint32_t *faces;      uint32_t num_faces;      /* 3 indices per face (AddFace order) */
     const void *edges;          uint32_t num_edges;      /* explicit Edge records */
     const void *volumes;        uint32_t num_volumes;    /* explicit Volume records */
+    const void *lras;           uint32_t num_lras;       /* explicit LRA records (mLRAConstraints before the steps) */
     const xpbd_vertex_attributes *attributes; uint32_t num_attributes;
     int32_t bend_type;          /* EBendType: 0 none, 1 distance, 2 dihedral */
     float angle_tolerance;      /* radians (reference default DegreesToRadians(8)) */

@@ -694,6 +694,15 @@ int xpbd_settings_build(const xpbd_build_desc *d, xpbd_built_settings **out)
                 return BuildFail(XPBD_E_INVALID, "volume vertex out of range");
         b.volumes.push_back(tr[k]);
     }
+    const LRARec *lr = (const LRARec *)d->lras;
+    if (d->num_lras && !lr)
+        return BuildFail(XPBD_E_INVALID, "null LRA array");
+    for (uint32_t k = 0; k < d->num_lras; ++k)
+    {
+        if (lr[k].v[0] >= nv || lr[k].v[1] >= nv || lr[k].v[0] == lr[k].v[1])
+            return BuildFail(XPBD_E_INVALID, "LRA vertices out of range or equal");
+        b.lras.push_back(lr[k]);
+    }
     if (d->flags & XPBD_BUILD_CREATE_CONSTRAINTS)
     {
         if (d->num_attributes == 0 || !d->attributes)

@@ -19,8 +19,6 @@
 
 namespace xpbd {
 
-constexpr int kBundleBlock = 256;
-constexpr int kBundleWaves = kBundleBlock / kWave;
 
 // Per-body constants of the bundle (LDS), refreshed every collision step.
 struct BodyLocal
@@ -48,10 +46,11 @@ struct BundleLds
     uint32_t recs, lvl, vb, locals, states, total;
 };
 
-__host__ __device__ inline BundleLds bundle_layout(uint32_t nv, uint32_t nrec, uint32_t nlvl, uint32_t nb)
+// pos_bytes: 16 (float4 position + inverse mass) or 12 (float3; one inverse mass per bundle)
+__host__ __device__ inline BundleLds bundle_layout(uint32_t nv, uint32_t nrec, uint32_t nlvl, uint32_t nb, uint32_t pos_bytes)
 {
     BundleLds l;
-    uint32_t off = nv * 16u;
+    uint32_t off = (nv * pos_bytes + 15u) & ~15u;
     l.recs = off; off += nrec * 16u;
     l.lvl = off; off += nlvl * 4u;
     off = (off + 15u) & ~15u; l.vb = off; off += nv;
@@ -61,8 +60,28 @@ __host__ __device__ inline BundleLds bundle_layout(uint32_t nv, uint32_t nrec, u
     return l;
 }
 
-template <bool STAGED>
-__global__ __launch_bounds__(kBundleBlock) void bundle_kernel(
+// Position container of a bundle: float4 per vertex, or float3 per vertex plus the
+// inverse mass all vertices of the bundle share.
+template <bool UNIFORM> struct BundlePositions;
+template <> struct BundlePositions<false>
+{
+    using Type = float4 *;
+    static __device__ Type make(unsigned char *lds, float) { return reinterpret_cast<float4 *>(lds); }
+};
+template <> struct BundlePositions<true>
+{
+    using Type = PositionsF3;
+    static __device__ Type make(unsigned char *lds, float w)
+    {
+        PositionsF3 p;
+        p.x = reinterpret_cast<float *>(lds);
+        p.w = w;
+        return p;
+    }
+};
+
+template <bool STAGED, int BLOCK, bool UNIFORM>
+__global__ __launch_bounds__(BLOCK) void bundle_kernel(
     const BundleDev *__restrict__ bundles, const BodyDev *__restrict__ bodies, BodyStateDev *__restrict__ states,
     const ColliderDev *__restrict__ colliders, uint32_t num_colliders,
     float gx, float gy, float gz, float dt, int32_t steps, int32_t frames)
@@ -76,8 +95,8 @@ __global__ __launch_bounds__(kBundleBlock) void bundle_kernel(
     const uint32_t nb = G.num_bodies;
     const uint32_t nlev = G.nlev;
     const bool has_sleepers = G.has_sleepers != 0;
-    const BundleLds lay = bundle_layout(nv, STAGED ? G.num_records : 0u, STAGED ? G.nlev + 1 : 0u, nb);
-    float4 *X = reinterpret_cast<float4 *>(lds_raw);
+    const BundleLds lay = bundle_layout(nv, STAGED ? G.num_records : 0u, STAGED ? G.nlev + 1 : 0u, nb, UNIFORM ? 12u : 16u);
+    const typename BundlePositions<UNIFORM>::Type X = BundlePositions<UNIFORM>::make(lds_raw, G.uniform_inv_mass);
     const uint4 *recs = STAGED ? reinterpret_cast<const uint4 *>(lds_raw + lay.recs) : G.recs;
     const uint32_t *lvl = STAGED ? reinterpret_cast<const uint32_t *>(lds_raw + lay.lvl) : G.lvl;
     uint8_t *vb = reinterpret_cast<uint8_t *>(lds_raw + lay.vb);
@@ -90,21 +109,21 @@ __global__ __launch_bounds__(kBundleBlock) void bundle_kernel(
     if (STAGED)
     {
         uint4 *rd = reinterpret_cast<uint4 *>(lds_raw + lay.recs);
-        for (uint32_t i = tid; i < G.num_records; i += kBundleBlock)
+        for (uint32_t i = tid; i < G.num_records; i += BLOCK)
             rd[i] = G.recs[i];
         uint32_t *ld = reinterpret_cast<uint32_t *>(lds_raw + lay.lvl);
-        for (uint32_t i = tid; i <= G.nlev; i += kBundleBlock)
+        for (uint32_t i = tid; i <= G.nlev; i += BLOCK)
             ld[i] = G.lvl[i];
     }
-    for (uint32_t i = tid; i < nv; i += kBundleBlock)
-        X[i] = G.x[i];
+    for (uint32_t i = tid; i < nv; i += BLOCK)
+        pset(X, i, G.x[i]);
     for (uint32_t b = 0; b < nb; ++b)
     {
         const BundleBodyDev bb = G.bodies[b];
-        for (uint32_t i = tid; i < bb.nv; i += kBundleBlock)
+        for (uint32_t i = tid; i < bb.nv; i += BLOCK)
             vb[bb.voff + i] = (uint8_t)b;
     }
-    for (uint32_t b = tid; b < nb; b += kBundleBlock)
+    for (uint32_t b = tid; b < nb; b += BLOCK)
     {
         const BodyDev &B = bodies[G.bodies[b].body];
         BodyLocal &L = loc[b];
@@ -132,7 +151,7 @@ __global__ __launch_bounds__(kBundleBlock) void bundle_kernel(
         for (int32_t step = 0; step < steps; ++step)
         {
             // ---- InitializeUpdateContext per body (SoftBodyMotionProperties.cpp:937-960)
-            for (uint32_t b = tid; b < nb; b += kBundleBlock)
+            for (uint32_t b = tid; b < nb; b += BLOCK)
             {
                 BodyLocal &L = loc[b];
                 const BodyStateDev &s = st[b];
@@ -151,7 +170,7 @@ __global__ __launch_bounds__(kBundleBlock) void bundle_kernel(
                 L.max_lin_sq = L.max_linear_velocity * L.max_linear_velocity;
             }
             // ---- DetermineCollidingShapes: candidate leaves per (body, collider)
-            for (uint32_t p = tid; p < nb * nc; p += kBundleBlock)
+            for (uint32_t p = tid; p < nb * nc; p += BLOCK)
             {
                 const uint32_t b = p / nc, c = p - b * nc;
                 const BodyStateDev &s = st[b];
@@ -189,7 +208,7 @@ __global__ __launch_bounds__(kBundleBlock) void bundle_kernel(
             }
             __syncthreads();
             // colliding list of each body in collider (broadphase) order
-            for (uint32_t b = tid; b < nb; b += kBundleBlock)
+            for (uint32_t b = tid; b < nb; b += BLOCK)
             {
                 uint32_t n = 0;
                 for (uint32_t c = 0; c < nc; ++c)
@@ -200,11 +219,11 @@ __global__ __launch_bounds__(kBundleBlock) void bundle_kernel(
             __syncthreads();
 
             // ---- DetermineCollisionPlanes (SoftBodyMotionProperties.cpp:257-265)
-            for (uint32_t i = tid; i < nv; i += kBundleBlock)
+            for (uint32_t i = tid; i < nv; i += BLOCK)
             {
                 const uint32_t b = body_of(i);
                 const uint32_t n = loc[b].num_leaves;
-                float4 p = X[i];
+                float4 p = pget(X, i);
                 if (n > 0 && st[b].active && p.w > 0.0f)
                 {
                     Pl plane; plane.n = v3(0.0f, 0.0f, 0.0f); plane.c = 0.0f;
@@ -227,15 +246,15 @@ __global__ __launch_bounds__(kBundleBlock) void bundle_kernel(
                     if (L.nf == 0 || !st[b].active)
                         continue;
                     const BundleBodyDev bb = G.bodies[b];
-                    for (uint32_t f = tid; f < L.nf; f += kBundleBlock)
+                    for (uint32_t f = tid; f < L.nf; f += BLOCK)
                     {
-                        V3 x1 = xyz(X[bb.voff + L.faces[3 * f]]), x2 = xyz(X[bb.voff + L.faces[3 * f + 1]]), x3 = xyz(X[bb.voff + L.faces[3 * f + 2]]);
+                        V3 x1 = xyz(pget(X, bb.voff + L.faces[3 * f])), x2 = xyz(pget(X, bb.voff + L.faces[3 * f + 1])), x3 = xyz(pget(X, bb.voff + L.faces[3 * f + 2]));
                         G.face_scratch[bb.face_off + f] = make_float4(0.0f, 0.0f, 0.0f, dot(cross(x1, x2), x3));
                     }
                 }
                 __syncthreads();
                 // GetVolumeTimesSix: sequential sum in face order, one lane per body
-                for (uint32_t b = wave; b < nb; b += kBundleWaves)
+                for (uint32_t b = wave; b < nb; b += (BLOCK / kWave))
                     if (lane == 0 && loc[b].nf > 0 && st[b].active)
                     {
                         const uint32_t fo = G.bodies[b].face_off;
@@ -252,15 +271,15 @@ __global__ __launch_bounds__(kBundleBlock) void bundle_kernel(
                         continue;
                     const BundleBodyDev bb = G.bodies[b];
                     const float coefficient = L.pressure * L.dt_sub / L.six_volume;
-                    for (uint32_t f = tid; f < L.nf; f += kBundleBlock)
+                    for (uint32_t f = tid; f < L.nf; f += BLOCK)
                     {
-                        V3 x1 = xyz(X[bb.voff + L.faces[3 * f]]), x2 = xyz(X[bb.voff + L.faces[3 * f + 1]]), x3 = xyz(X[bb.voff + L.faces[3 * f + 2]]);
+                        V3 x1 = xyz(pget(X, bb.voff + L.faces[3 * f])), x2 = xyz(pget(X, bb.voff + L.faces[3 * f + 1])), x3 = xyz(pget(X, bb.voff + L.faces[3 * f + 2]));
                         V3 impulse = coefficient * cross(x2 - x1, x3 - x1);
                         G.face_scratch[bb.face_off + f] = make_float4(impulse.x, impulse.y, impulse.z, 0.0f);
                     }
                 }
                 __syncthreads();
-                for (uint32_t i = tid; i < nv; i += kBundleBlock)
+                for (uint32_t i = tid; i < nv; i += BLOCK)
                 {
                     const uint32_t b = body_of(i);
                     const BodyLocal &L = loc[b];
@@ -271,7 +290,7 @@ __global__ __launch_bounds__(kBundleBlock) void bundle_kernel(
                     uint32_t fb = L.vf_begin[li], fe = L.vf_begin[li + 1];
                     if (fb == fe)
                         continue;
-                    float w = X[i].w;
+                    float w = pget(X, i).w;
                     V3 v = xyz(G.v[i]);
                     for (uint32_t k = fb; k < fe; ++k)
                         v = v + w * xyz(G.face_scratch[bb.face_off + L.vf_list[k]]);
@@ -290,17 +309,17 @@ __global__ __launch_bounds__(kBundleBlock) void bundle_kernel(
                 }
                 V3 position = xyz(p);
                 G.prev[i] = f4(position, 0.0f);
-                X[i] = f4(position + v * L.dt_sub, p.w);
+                pset(X, i, f4(position + v * L.dt_sub, p.w));
                 G.v[i] = f4(v, 0.0f);
             };
 
             if (G.has_pressure)
                 apply_pressure();
-            for (uint32_t i = tid; i < nv; i += kBundleBlock)
+            for (uint32_t i = tid; i < nv; i += BLOCK)
             {
                 const uint32_t b = body_of(i);
                 if (st[b].active)
-                    integrate_vertex(i, loc[b], X[i], xyz(G.v[i]));
+                    integrate_vertex(i, loc[b], pget(X, i), xyz(G.v[i]));
             }
             __syncthreads();
 
@@ -312,24 +331,24 @@ __global__ __launch_bounds__(kBundleBlock) void bundle_kernel(
                 if (has_sleepers)
                 {
                     auto skip = [&](const uint4 &r) { return r.x == kNopRecord || !st[vb[r.x & 0x3fffu]].active; };
-                    if (STAGED) run_levels_lds<kBundleBlock>(recs, lvl, nlev, X, inv_dt_sq, tid, skip);
-                    else run_levels_global<kBundleBlock>(recs, lvl, nlev, X, inv_dt_sq, tid, skip);
+                    if (STAGED) run_levels_lds<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, tid, skip);
+                    else run_levels_global<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, tid, skip);
                 }
                 else
                 {
-                    if (STAGED) run_levels_lds<kBundleBlock>(recs, lvl, nlev, X, inv_dt_sq, tid, NoSkip());
-                    else run_levels_global<kBundleBlock>(recs, lvl, nlev, X, inv_dt_sq, tid, NoSkip());
+                    if (STAGED) run_levels_lds<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, tid, NoSkip());
+                    else run_levels_global<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, tid, NoSkip());
                 }
 
                 // ---- collision + velocity (SoftBodyMotionProperties.cpp:700-825), fused with the
                 // next integration (no pressure) or the per-vertex part of UpdateSoftBodyState
-                for (uint32_t i = tid; i < nv; i += kBundleBlock)
+                for (uint32_t i = tid; i < nv; i += BLOCK)
                 {
                     const uint32_t b = body_of(i);
                     if (!st[b].active)
                         continue;
                     const BodyLocal &L = loc[b];
-                    float4 p = X[i];
+                    float4 p = pget(X, i);
                     V3 x = xyz(p);
                     V3 v = xyz(G.v[i]);
                     if (p.w > 0.0f)
@@ -367,7 +386,7 @@ __global__ __launch_bounds__(kBundleBlock) void bundle_kernel(
                     {
                         if (L.nf > 0)
                         {
-                            X[i] = f4(x, p.w);
+                            pset(X, i, f4(x, p.w));
                             G.v[i] = f4(v, 0.0f);
                         }
                         else
@@ -379,7 +398,7 @@ __global__ __launch_bounds__(kBundleBlock) void bundle_kernel(
                         float v_sq = len_sq(v);
                         if (v_sq > L.max_lin_sq)
                             v = v * sqrtf(L.max_lin_sq / v_sq);
-                        X[i] = f4(x, p.w);
+                        pset(X, i, f4(x, p.w));
                         G.v[i] = f4(v, 0.0f);
                         G.prev[i].w = v_sq;        // pre-clamp |v|^2 for the sleep test
                         G.idx[i] = -1;
@@ -390,19 +409,19 @@ __global__ __launch_bounds__(kBundleBlock) void bundle_kernel(
                 if (!last && G.has_pressure)
                 {
                     apply_pressure();
-                    for (uint32_t i = tid; i < nv; i += kBundleBlock)
+                    for (uint32_t i = tid; i < nv; i += BLOCK)
                     {
                         const uint32_t b = body_of(i);
                         const BodyLocal &L = loc[b];
                         if (st[b].active && L.nf > 0)
-                            integrate_vertex(i, L, X[i], xyz(G.v[i]));
+                            integrate_vertex(i, L, pget(X, i), xyz(G.v[i]));
                     }
                     __syncthreads();
                 }
             }
 
             // ---- UpdateSoftBodyState reductions, one wavefront per body (SoftBodyMotionProperties.cpp:846-921)
-            for (uint32_t b = wave; b < nb; b += kBundleWaves)
+            for (uint32_t b = wave; b < nb; b += (BLOCK / kWave))
             {
                 BodyStateDev &s = st[b];
                 if (!s.active)
@@ -416,7 +435,7 @@ __global__ __launch_bounds__(kBundleBlock) void bundle_kernel(
                 for (uint32_t k = lane; k < bb.nv; k += kWave)
                 {
                     const uint32_t i = bb.voff + k;
-                    V3 x = xyz(X[i]);
+                    V3 x = xyz(pget(X, i));
                     V3 v = xyz(G.v[i]);
                     vals[12] = smax(vals[12], G.prev[i].w);
                     V3 pp = (x + v * step_dt) + disp;
@@ -473,7 +492,7 @@ __global__ __launch_bounds__(kBundleBlock) void bundle_kernel(
                 }
             }
             __syncthreads();
-            for (uint32_t i = tid; i < nv; i += kBundleBlock)
+            for (uint32_t i = tid; i < nv; i += BLOCK)
             {
                 const uint32_t b = body_of(i);
                 const BodyLocal &L = loc[b];
@@ -482,57 +501,71 @@ __global__ __launch_bounds__(kBundleBlock) void bundle_kernel(
                 if (L.update_position && G.bodies[b].nv > 0)
                 {
                     V3 delta = v3(L.delta[0], L.delta[1], L.delta[2]);
-                    float4 p = X[i];
-                    X[i] = f4(xyz(p) - delta, p.w);
+                    float4 p = pget(X, i);
+                    pset(X, i, f4(xyz(p) - delta, p.w));
                 }
             }
-            for (uint32_t b = tid; b < nb; b += kBundleBlock)
+            for (uint32_t b = tid; b < nb; b += BLOCK)
             {
                 loc[b].delta[0] = 0.0f; loc[b].delta[1] = 0.0f; loc[b].delta[2] = 0.0f;
             }
             __syncthreads();
         }
 
-    for (uint32_t i = tid; i < nv; i += kBundleBlock)
-        G.x[i] = X[i];
-    for (uint32_t b = tid; b < nb; b += kBundleBlock)
+    for (uint32_t i = tid; i < nv; i += BLOCK)
+        G.x[i] = pget(X, i);
+    for (uint32_t b = tid; b < nb; b += BLOCK)
         states[G.bodies[b].body] = st[b];
 }
 
-uint32_t bundle_lds_bytes(uint32_t nv, uint32_t nrec, uint32_t nlvl, uint32_t nb)
+uint32_t bundle_lds_bytes(uint32_t nv, uint32_t nrec, uint32_t nlvl, uint32_t nb, bool uniform)
 {
-    return bundle_layout(nv, nrec, nlvl, nb).total;
+    return bundle_layout(nv, nrec, nlvl, nb, uniform ? 12u : 16u).total;
 }
 
 uint32_t bundle_leaf_bytes() { return (uint32_t)sizeof(Leaf); }
 
-template <bool S>
+template <bool S, int BLOCK, bool U>
 static hipError_t launch_bundles_variant(const BundleDev *bundles, uint32_t num_bundles, const BodyDev *bodies, BodyStateDev *states,
                                          const ColliderDev *colliders, uint32_t num_colliders, uint32_t lds_bytes,
                                          float gx, float gy, float gz, float dt, int32_t steps, int32_t frames, hipStream_t stream)
 {
-    static bool configured = false;
-    if (!configured)
-    {
-        hipError_t e = hipFuncSetAttribute((const void *)bundle_kernel<S>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
-        if (e != hipSuccess)
-            return e;
-        configured = true;
-    }
-    hipLaunchKernelGGL((bundle_kernel<S>), dim3(num_bundles), dim3(kBundleBlock), lds_bytes, stream,
+    static const hipError_t configured = hipFuncSetAttribute((const void *)bundle_kernel<S, BLOCK, U>,
+                                                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+    if (configured != hipSuccess)
+        return configured;
+    hipLaunchKernelGGL((bundle_kernel<S, BLOCK, U>), dim3(num_bundles), dim3(BLOCK), lds_bytes, stream,
                        bundles, bodies, states, colliders, num_colliders, gx, gy, gz, dt, steps, frames);
     return hipGetLastError();
 }
 
-hipError_t launch_bundles(bool staged, const BundleDev *bundles, uint32_t num_bundles, const BodyDev *bodies, BodyStateDev *states,
-                          const ColliderDev *colliders, uint32_t num_colliders, uint32_t lds_bytes,
-                          float gx, float gy, float gz, float dt, int32_t steps, int32_t frames, hipStream_t stream)
+template <bool U>
+static hipError_t launch_bundles_uniform(bool staged, uint32_t block, const BundleDev *bundles, uint32_t num_bundles,
+                                         const BodyDev *bodies, BodyStateDev *states, const ColliderDev *colliders,
+                                         uint32_t num_colliders, uint32_t lds_bytes, float gx, float gy, float gz, float dt,
+                                         int32_t steps, int32_t frames, hipStream_t stream)
+{
+    if (staged)
+        return launch_bundles_variant<true, 256, U>(bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
+    if (block == 1024)
+        return launch_bundles_variant<false, 1024, U>(bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
+    if (block == 512)
+        return launch_bundles_variant<false, 512, U>(bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
+    return launch_bundles_variant<false, 256, U>(bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
+}
+
+// block: 256, 512 or 1024 threads (staged bundles always use 256); uniform: the
+// bundle's vertices share one inverse mass and keep three floats per vertex in LDS
+hipError_t launch_bundles(bool staged, uint32_t block, bool uniform, const BundleDev *bundles, uint32_t num_bundles,
+                          const BodyDev *bodies, BodyStateDev *states, const ColliderDev *colliders, uint32_t num_colliders,
+                          uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
+                          hipStream_t stream)
 {
     if (num_bundles == 0)
         return hipSuccess;
-    if (staged)
-        return launch_bundles_variant<true>(bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
-    return launch_bundles_variant<false>(bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
+    if (uniform)
+        return launch_bundles_uniform<true>(staged, block, bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
+    return launch_bundles_uniform<false>(staged, block, bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
 }
 
 } // namespace xpbd

@@ -28,11 +28,36 @@ struct Leaf
 __device__ __forceinline__ V3 xyz(float4 p) { return v3(p.x, p.y, p.z); }
 __device__ __forceinline__ float4 f4(V3 p, float w) { return make_float4(p.x, p.y, p.z, w); }
 
+// Positions in LDS.  A float4* holds position + inverse mass per vertex (16 B);
+// PositionsF3 holds three floats per vertex (12 B) for bodies whose vertices all
+// share one inverse mass.  pget / pset give both the same float4 interface.
+struct PositionsF3
+{
+    float *x;
+    float w;
+};
+
+__device__ __forceinline__ float4 pget(float4 *X, uint32_t i) { return X[i]; }
+__device__ __forceinline__ void pset(float4 *X, uint32_t i, float4 p) { X[i] = p; }
+__device__ __forceinline__ float4 pget(const PositionsF3 &X, uint32_t i)
+{
+    const float *q = X.x + 3u * i;
+    return make_float4(q[0], q[1], q[2], X.w);
+}
+__device__ __forceinline__ void pset(const PositionsF3 &X, uint32_t i, float4 p)
+{
+    float *q = X.x + 3u * i;
+    q[0] = p.x;
+    q[1] = p.y;
+    q[2] = p.z;
+}
+
 // ---------------------------------------------------------------- constraints
 // ApplyEdgeConstraints body (SoftBodyMotionProperties.cpp:583-600)
-__device__ __forceinline__ void project_edge(float4 *X, uint32_t i0, uint32_t i1, float rest, float compliance, float inv_dt_sq)
+template <class P>
+__device__ __forceinline__ void project_edge(const P &X, uint32_t i0, uint32_t i1, float rest, float compliance, float inv_dt_sq)
 {
-    float4 p0 = X[i0], p1 = X[i1];
+    float4 p0 = pget(X, i0), p1 = pget(X, i1);
     V3 x0 = xyz(p0), x1 = xyz(p1);
     float w0 = p0.w, w1 = p1.w;
     V3 delta = x1 - x0;
@@ -41,14 +66,15 @@ __device__ __forceinline__ void project_edge(float4 *X, uint32_t i0, uint32_t i1
     if (denom < 1.0e-12f)
         return;
     V3 correction = (delta * (length - rest)) / denom;
-    X[i0] = f4(x0 + w0 * correction, w0);
-    X[i1] = f4(x1 - w1 * correction, w1);
+    pset(X, i0, f4(x0 + w0 * correction, w0));
+    pset(X, i1, f4(x1 - w1 * correction, w1));
 }
 
 // ApplyDihedralBendConstraints body (SoftBodyMotionProperties.cpp:375-455)
-__device__ __forceinline__ void project_bend(float4 *X, uint32_t i0, uint32_t i1, uint32_t i2, uint32_t i3, float compliance, float initial_angle, float inv_dt_sq)
+template <class P>
+__device__ __forceinline__ void project_bend(const P &X, uint32_t i0, uint32_t i1, uint32_t i2, uint32_t i3, float compliance, float initial_angle, float inv_dt_sq)
 {
-    float4 p0 = X[i0], p1 = X[i1], p2 = X[i2], p3 = X[i3];
+    float4 p0 = pget(X, i0), p1 = pget(X, i1), p2 = pget(X, i2), p3 = pget(X, i3);
     V3 x0 = xyz(p0), x1 = xyz(p1), x2 = xyz(p2), x3 = xyz(p3);
     V3 e = x1 - x0;
     float e_len = len(e);
@@ -81,16 +107,17 @@ __device__ __forceinline__ void project_bend(float4 *X, uint32_t i0, uint32_t i1
     if (denom < 1.0e-12f)
         return;
     float minus_lambda = c / denom;
-    X[i0] = f4(x0 - (minus_lambda * w0) * d0c, w0);
-    X[i1] = f4(x1 - (minus_lambda * w1) * d1c, w1);
-    X[i2] = f4(x2 - (minus_lambda * w2) * d2c, w2);
-    X[i3] = f4(x3 - (minus_lambda * w3) * d3c, w3);
+    pset(X, i0, f4(x0 - (minus_lambda * w0) * d0c, w0));
+    pset(X, i1, f4(x1 - (minus_lambda * w1) * d1c, w1));
+    pset(X, i2, f4(x2 - (minus_lambda * w2) * d2c, w2));
+    pset(X, i3, f4(x3 - (minus_lambda * w3) * d3c, w3));
 }
 
 // ApplyVolumeConstraints body (SoftBodyMotionProperties.cpp:467-506)
-__device__ __forceinline__ void project_volume(float4 *X, uint32_t i1, uint32_t i2, uint32_t i3, uint32_t i4, float six_rest_volume, float compliance, float inv_dt_sq)
+template <class P>
+__device__ __forceinline__ void project_volume(const P &X, uint32_t i1, uint32_t i2, uint32_t i3, uint32_t i4, float six_rest_volume, float compliance, float inv_dt_sq)
 {
-    float4 p1 = X[i1], p2 = X[i2], p3 = X[i3], p4 = X[i4];
+    float4 p1 = pget(X, i1), p2 = pget(X, i2), p3 = pget(X, i3), p4 = pget(X, i4);
     V3 x1 = xyz(p1), x2 = xyz(p2), x3 = xyz(p3), x4 = xyz(p4);
     V3 x1x2 = x2 - x1;
     V3 x1x3 = x3 - x1;
@@ -105,27 +132,29 @@ __device__ __forceinline__ void project_volume(float4 *X, uint32_t i1, uint32_t 
     if (denom < 1.0e-12f)
         return;
     float minus_lambda = c / denom;
-    X[i1] = f4(x1 - (minus_lambda * w1) * d1c, w1);
-    X[i2] = f4(x2 - (minus_lambda * w2) * d2c, w2);
-    X[i3] = f4(x3 - (minus_lambda * w3) * d3c, w3);
-    X[i4] = f4(x4 - (minus_lambda * w4) * d4c, w4);
+    pset(X, i1, f4(x1 - (minus_lambda * w1) * d1c, w1));
+    pset(X, i2, f4(x2 - (minus_lambda * w2) * d2c, w2));
+    pset(X, i3, f4(x3 - (minus_lambda * w3) * d3c, w3));
+    pset(X, i4, f4(x4 - (minus_lambda * w4) * d4c, w4));
 }
 
 // ApplyLRAConstraints body (SoftBodyMotionProperties.cpp:685-697): only vertex 1 moves
-__device__ __forceinline__ void project_lra(float4 *X, uint32_t i0, uint32_t i1, float max_distance)
+template <class P>
+__device__ __forceinline__ void project_lra(const P &X, uint32_t i0, uint32_t i1, float max_distance)
 {
-    V3 x0 = xyz(X[i0]);
-    float4 p1 = X[i1];
+    V3 x0 = xyz(pget(X, i0));
+    float4 p1 = pget(X, i1);
     V3 delta = xyz(p1) - x0;
     float delta_len_sq = len_sq(delta);
     if (delta_len_sq > max_distance * max_distance)
-        X[i1] = f4(x0 + (delta * max_distance) / sqrtf(delta_len_sq), p1.w);
+        pset(X, i1, f4(x0 + (delta * max_distance) / sqrtf(delta_len_sq), p1.w));
 }
 
 // Padding record: skipped (levels pad each constraint type to whole wavefronts)
 constexpr uint32_t kNopRecord = 0xffffffffu;
 
-__device__ __forceinline__ void project_record(float4 *X, uint4 r, float inv_dt_sq)
+template <class P>
+__device__ __forceinline__ void project_record(const P &X, uint4 r, float inv_dt_sq)
 {
     if (r.x == kNopRecord)
         return;
@@ -247,8 +276,8 @@ __device__ __forceinline__ void run_program_global(const uint4 *recs, const uint
     }
 }
 
-template <int BLOCK, class Skip>
-__device__ __forceinline__ void run_levels_global(const uint4 *recs, const uint32_t *lvl, uint32_t nlev, float4 *X,
+template <int BLOCK, class P, class Skip>
+__device__ __forceinline__ void run_levels_global(const uint4 *recs, const uint32_t *lvl, uint32_t nlev, const P &X,
                                                   float inv_dt_sq, uint32_t tid, Skip skip)
 {
     run_program_global<BLOCK, false>(recs, lvl, nlev, tid, [&](const uint4 &q, uint32_t) {
@@ -259,8 +288,8 @@ __device__ __forceinline__ void run_levels_global(const uint4 *recs, const uint3
 
 // Level program with records and offsets in LDS: the first record of the next
 // level is read before the barrier (the barrier's LDS wait covers it).
-template <int BLOCK, class Skip>
-__device__ __forceinline__ void run_levels_lds(const uint4 *recs, const uint32_t *lvl, uint32_t nlev, float4 *X,
+template <int BLOCK, class P, class Skip>
+__device__ __forceinline__ void run_levels_lds(const uint4 *recs, const uint32_t *lvl, uint32_t nlev, const P &X,
                                                float inv_dt_sq, uint32_t tid, Skip skip)
 {
     if (nlev == 0)

@@ -17,11 +17,12 @@
 
 namespace xpbd {
 uint32_t kernel_lds_bytes(bool resident, bool staged, const SettingsDev &s, uint32_t max_leaves, uint32_t nc, uint32_t nf);
-uint32_t bundle_lds_bytes(uint32_t nv, uint32_t nrec, uint32_t nlvl, uint32_t nb);
+uint32_t bundle_lds_bytes(uint32_t nv, uint32_t nrec, uint32_t nlvl, uint32_t nb, bool uniform);
 uint32_t bundle_leaf_bytes();
-hipError_t launch_bundles(bool staged, const BundleDev *bundles, uint32_t num_bundles, const BodyDev *bodies, BodyStateDev *states,
-                          const ColliderDev *colliders, uint32_t num_colliders, uint32_t lds_bytes,
-                          float gx, float gy, float gz, float dt, int32_t steps, int32_t frames, hipStream_t stream);
+hipError_t launch_bundles(bool staged, uint32_t block, bool uniform, const BundleDev *bundles, uint32_t num_bundles,
+                          const BodyDev *bodies, BodyStateDev *states, const ColliderDev *colliders, uint32_t num_colliders,
+                          uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
+                          hipStream_t stream);
 hipError_t launch_update(bool resident, bool staged, const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list,
                          uint32_t num_blocks, const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves,
                          uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
@@ -113,6 +114,8 @@ struct xpbd_settings
     std::vector<uint4> host_recs;
     std::vector<uint32_t> host_level_begin;     // nlev + 1 offsets into host_recs
     std::vector<uint32_t> host_rec_index;       // constraint index of each host record (no-op: ~0u)
+    bool uniform_mass = false;                  // every vertex has the same inverse mass
+    float inv_mass = 0.0f;                      // that inverse mass
     uint32_t num_edges = 0, num_bends = 0, num_volumes = 0, num_lras = 0;
 };
 
@@ -132,6 +135,8 @@ struct BundleHost
     std::vector<uint32_t> bodies;
     uint32_t nv = 0;
     bool staged = false;
+    bool uniform = false;
+    float inv_mass = 0.0f;
     uint32_t lds = 0;
     DevBuf<uint4> recs;
     DevBuf<uint32_t> lvl;
@@ -170,12 +175,31 @@ struct xpbd_context
     struct BundleLaunch
     {
         bool staged = false;
+        bool uniform = false;   // bundles whose vertices share one inverse mass (12 B LDS positions)
+        uint32_t block = 256;   // threads per workgroup
         uint32_t lds = 0;
         DevBuf<BundleDev> table;
         uint32_t count = 0;
         double cost = 0.0;      // estimated duration (level passes per step, see LaunchCost)
     };
     std::vector<std::unique_ptr<BundleLaunch>> bundle_launches;
+    // Measured duration of each launch group (per bundle launch, plus one for the
+    // streamed bodies): events around the launch on its stream, harvested at the
+    // next update, so later updates assign launches to streams by measured time.
+    struct LaunchTiming
+    {
+        hipEvent_t start = nullptr, end = nullptr;
+        int32_t frames = 0;
+        bool pending = false;
+        double ms_per_frame = -1.0;
+        ~LaunchTiming()
+        {
+            if (start) (void)hipEventDestroy(start);
+            if (end) (void)hipEventDestroy(end);
+        }
+    };
+    std::vector<std::unique_ptr<LaunchTiming>> bundle_timing;
+    LaunchTiming streamed_timing;
     // The context stream plus three auxiliary streams: one per hardware queue
     // of a process (a fifth stream would share a queue and serialise behind it).
     static constexpr int kAux = 3;
@@ -258,10 +282,11 @@ struct GroupLevels
 
 // Appends one level: records grouped by constraint type (bend, volume, edge,
 // LRA), each group padded with no-op records to whole wavefronts when that does
-// not add a pass of the 256-thread workgroup, so a wavefront runs one
-// constraint path.  `pad` false keeps the groups unpadded.
+// not add a pass of the `block`-thread workgroup, so a wavefront runs one
+// constraint path.  `allow_pad` false keeps the groups unpadded.
 void AppendTypedLevel(std::vector<uint4> bucket[4], std::vector<uint4> &recs, bool allow_pad = true,
-                      const std::vector<uint32_t> *ibucket = nullptr, std::vector<uint32_t> *rec_index = nullptr)
+                      const std::vector<uint32_t> *ibucket = nullptr, std::vector<uint32_t> *rec_index = nullptr,
+                      uint32_t block = 256)
 {
     size_t n = 0, padded = 0;
     for (int t = 0; t < 4; ++t)
@@ -269,7 +294,7 @@ void AppendTypedLevel(std::vector<uint4> bucket[4], std::vector<uint4> &recs, bo
         n += bucket[t].size();
         padded += (bucket[t].size() + 63) / 64 * 64;
     }
-    const bool pad = allow_pad && (padded + 255) / 256 <= (n + 255) / 256;
+    const bool pad = allow_pad && (padded + block - 1) / block <= (n + block - 1) / block;
     const size_t base = recs.size();
     const uint4 nop = make_uint4(0xffffffffu, 0u, 0u, 0u);
     const int order[4] = { (int)kTypeBend, (int)kTypeVolume, (int)kTypeEdge, (int)kTypeLRA };
@@ -411,9 +436,8 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
     ProgramBuilder pb;
     const bool resident = nv <= kResidentMaxVertices;
     uint32_t window_capacity = 0;
-    if (resident)
-    {
-        // One identity window with the merged levels of all parallel groups, one for the tail
+    // One identity window with the merged levels of all parallel groups, one for the tail
+    auto build_identity = [&](ProgramBuilder &out) {
         LevelList par(max_par_levels);
         for (uint32_t g = 0; g < npar; ++g)
             for (uint32_t L = 0; L < gl[g].levels.size(); ++L)
@@ -421,13 +445,17 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
                     par[L].push_back({ &gl[g], k });
         std::vector<uint32_t> none;
         if (npar > 0)
-            pb.AddWindow(par, nullptr, none);
-        s.dev.num_par_windows = npar > 0 ? 1u : 0u;
+            out.AddWindow(par, nullptr, none);
         LevelList tl(tail.levels.size());
         for (uint32_t L = 0; L < tail.levels.size(); ++L)
             for (uint32_t k : tail.levels[L])
                 tl[L].push_back({ &tail, k });
-        pb.AddWindow(tl, nullptr, none);
+        out.AddWindow(tl, nullptr, none);
+    };
+    if (resident)
+    {
+        build_identity(pb);
+        s.dev.num_par_windows = npar > 0 ? 1u : 0u;
         s.dev.num_tail_windows = 1;
     }
     else
@@ -529,15 +557,25 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
     s.num_bends = d.num_dihedral_bends;
     s.num_volumes = d.num_volumes;
     s.num_lras = d.num_lras;
-    if (resident)
+    // host copy of the identity program (bundles): every body whose vertex numbers
+    // fit the records' 14 bits, streamed ones included (they join a bundle when
+    // their positions fit in LDS at 12 bytes per vertex)
+    if (nv < kMaxWindowVertices)
     {
-        s.host_recs = pb.recs;
-        s.host_rec_index = pb.rec_index;
+        ProgramBuilder idp;
+        const ProgramBuilder *src = &pb;
+        if (!resident)
+        {
+            build_identity(idp);
+            src = &idp;
+        }
+        s.host_recs = src->recs;
+        s.host_rec_index = src->rec_index;
         s.host_level_begin.clear();
-        for (const WindowDev &w : pb.windows)
+        for (const WindowDev &w : src->windows)
             for (uint32_t L = 0; L < w.nlev; ++L)
-                s.host_level_begin.push_back(pb.lvl[w.lbegin + L]);
-        s.host_level_begin.push_back((uint32_t)pb.recs.size());
+                s.host_level_begin.push_back(src->lvl[w.lbegin + L]);
+        s.host_level_begin.push_back((uint32_t)src->recs.size());
     }
 
     s.stats[0] = npar;
@@ -581,6 +619,10 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
     s.num_faces = d.num_faces;
     const VertexRec *vr = (const VertexRec *)d.vertices;
     s.vertices.assign(vr, vr + nv);
+    s.uniform_mass = true;
+    s.inv_mass = vr[0].inv_mass;
+    for (uint32_t i = 1; i < nv && s.uniform_mass; ++i)
+        s.uniform_mass = memcmp(&vr[i].inv_mass, &s.inv_mass, sizeof(float)) == 0;
 
     SettingsDev &sd = s.dev;
     sd.recs = s.recs.p;
@@ -629,7 +671,7 @@ uint4 OffsetRecord(uint4 r, uint32_t voff)
     return o;
 }
 
-void BuildBundleProgram(const xpbd_context *ctx, const BundleHost &bh, const std::vector<uint32_t> &voffs,
+void BuildBundleProgram(const xpbd_context *ctx, const BundleHost &bh, const std::vector<uint32_t> &voffs, uint32_t block,
                         std::vector<uint4> &recs, std::vector<uint32_t> &lvl)
 {
     size_t K = 0;
@@ -654,9 +696,21 @@ void BuildBundleProgram(const xpbd_context *ctx, const BundleHost &bh, const std
             }
         }
         lvl.push_back((uint32_t)recs.size());
-        AppendTypedLevel(bucket, recs);
+        AppendTypedLevel(bucket, recs, true, nullptr, nullptr, block);
     }
     lvl.push_back((uint32_t)recs.size());
+}
+
+// Threads per bundle workgroup: bundles of one large body get wider workgroups
+// (their parallel levels hold several hundred records and their per-vertex
+// loops thousands of vertices); they hold a CU's LDS alone either way.
+uint32_t BundleBlock(uint32_t nv)
+{
+    if (nv > 4096u)
+        return 1024u;
+    if (nv > 2688u)
+        return 512u;
+    return 256u;
 }
 
 // Estimated duration of a launch in level passes per step: its workgroups are
@@ -680,21 +734,35 @@ int BuildLayout(xpbd_context *ctx)
 {
     const uint32_t nc = (uint32_t)ctx->colliders.size();
     const uint32_t cap = BundleCapacity();
+    // A body joins the bundles when its identity program exists and its positions
+    // fit in one workgroup's LDS: 12 B per vertex when all its vertices share one
+    // inverse mass, else 16 B.  Bundles never mix inverse-mass classes.
+    auto uniform_of = [ctx](uint32_t i) { return ctx->bodies[i]->settings->uniform_mass; };
+    auto mass_bits = [ctx](uint32_t i) {
+        uint32_t u;
+        memcpy(&u, &ctx->bodies[i]->settings->inv_mass, 4);
+        return u;
+    };
     std::vector<uint32_t> resident;
     ctx->streamed.clear();
     for (uint32_t i = 0; i < ctx->bodies.size(); ++i)
     {
         const Body &b = *ctx->bodies[i];
-        if (ctx->use_bundles && b.settings->dev.resident && !b.settings->host_level_begin.empty())
+        const bool fits = bundle_lds_bytes(b.nv, 0, 0, 1, b.settings->uniform_mass) <= kMaxLdsBytes;
+        if (ctx->use_bundles && fits && !b.settings->host_level_begin.empty())
             resident.push_back(i);
         else
             ctx->streamed.push_back(i);
     }
-    // deepest level sequences first; bodies of a bundle share the iteration count
-    std::stable_sort(resident.begin(), resident.end(), [ctx](uint32_t a, uint32_t b) {
+    // same iteration count and inverse-mass class together, deepest level sequences first
+    std::stable_sort(resident.begin(), resident.end(), [&](uint32_t a, uint32_t b) {
         const Body &A = *ctx->bodies[a], &B = *ctx->bodies[b];
         if (A.dev.iterations != B.dev.iterations)
             return A.dev.iterations < B.dev.iterations;
+        if (uniform_of(a) != uniform_of(b))
+            return uniform_of(a);
+        if (uniform_of(a) && mass_bits(a) != mass_bits(b))
+            return mass_bits(a) < mass_bits(b);
         return A.settings->host_level_begin.size() > B.settings->host_level_begin.size();
     });
     std::vector<std::unique_ptr<BundleHost>> bundles;
@@ -702,11 +770,15 @@ int BuildLayout(xpbd_context *ctx)
     {
         const Body &b = *ctx->bodies[bi];
         BundleHost *cur = bundles.empty() ? nullptr : bundles.back().get();
-        if (!cur || cur->bodies.size() >= kBundleMaxBodies || cur->nv + b.nv > std::max(cap, b.nv)
+        const bool same_class = cur && uniform_of(cur->bodies[0]) == uniform_of(bi)
+                                && (!uniform_of(bi) || mass_bits(cur->bodies[0]) == mass_bits(bi));
+        if (!cur || !same_class || cur->bodies.size() >= kBundleMaxBodies || cur->nv + b.nv > std::max(cap, b.nv)
             || ctx->bodies[cur->bodies[0]]->dev.iterations != b.dev.iterations || (cur->nv > 0 && cur->nv + b.nv > cap))
         {
             bundles.emplace_back(new BundleHost);
             cur = bundles.back().get();
+            cur->uniform = uniform_of(bi);
+            cur->inv_mass = b.settings->inv_mass;
         }
         cur->bodies.push_back(bi);
         cur->nv += b.nv;
@@ -742,7 +814,8 @@ int BuildLayout(xpbd_context *ctx)
             return Fail(XPBD_E_CAPACITY, "bundle exceeds the LDS window index range");
         std::vector<uint4> recs;
         std::vector<uint32_t> lvl;
-        BuildBundleProgram(ctx, bh, voffs, recs, lvl);
+        const uint32_t block = BundleBlock(voff);
+        BuildBundleProgram(ctx, bh, voffs, block, recs, lvl);
         HIP_TRY(bh.recs.upload(recs));
         HIP_TRY(bh.lvl.upload(lvl));
         HIP_TRY(bh.table.upload(table));
@@ -772,23 +845,27 @@ int BuildLayout(xpbd_context *ctx)
         d.has_pressure = has_pressure ? 1u : 0u;
         d.has_sleepers = has_sleepers ? 1u : 0u;
         d.iterations = ctx->bodies[bh.bodies[0]]->dev.iterations;
-        const uint32_t need_staged = bundle_lds_bytes(voff, d.num_records, d.nlev + 1, nb);
-        const uint32_t need_plain = bundle_lds_bytes(voff, 0, 0, nb);
+        d.uniform_inv_mass = bh.uniform ? bh.inv_mass : 0.0f;
+        const uint32_t need_staged = bundle_lds_bytes(voff, d.num_records, d.nlev + 1, nb, bh.uniform);
+        const uint32_t need_plain = bundle_lds_bytes(voff, 0, 0, nb, bh.uniform);
         if (need_plain > kMaxLdsBytes)
             return Fail(XPBD_E_CAPACITY, "bundle needs more LDS than a workgroup has");
-        bh.staged = need_staged <= 40u * 1024u;
+        bh.staged = block == 256u && need_staged <= 40u * 1024u;
         d.staged = bh.staged ? 1u : 0u;
         bh.lds = bh.staged ? need_staged : need_plain;
         uint32_t bin = 0;
         while (bins[bin] < bh.lds) ++bin;
         size_t li = 0;
         for (; li < launches.size(); ++li)
-            if (launches[li]->staged == bh.staged && launches[li]->lds == bins[bin])
+            if (launches[li]->staged == bh.staged && launches[li]->lds == bins[bin] && launches[li]->block == block
+                && launches[li]->uniform == bh.uniform)
                 break;
         if (li == launches.size())
         {
             launches.emplace_back(new xpbd_context::BundleLaunch);
             launches.back()->staged = bh.staged;
+            launches.back()->uniform = bh.uniform;
+            launches.back()->block = block;
             launches.back()->lds = bins[bin];
             launch_tables.emplace_back();
             launch_work.emplace_back();
@@ -819,6 +896,9 @@ int BuildLayout(xpbd_context *ctx)
         launches[li]->cost = LaunchCost(launch_work[li], launches[li]->lds);
     }
     ctx->bundle_launches.swap(launches);
+    ctx->bundle_timing.clear();
+    for (size_t li = 0; li < ctx->bundle_launches.size(); ++li)
+        ctx->bundle_timing.emplace_back(new xpbd_context::LaunchTiming);
     ctx->layout_dirty = false;
     ctx->layout_colliders = nc;
     ctx->tables_dirty = true;
@@ -1091,7 +1171,7 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
     }
 
     // Streamed bodies: launch classes (kernel variant x LDS size bin)
-    struct Launch { bool resident, staged; uint32_t lds; std::vector<uint32_t> bodies; };
+    struct Launch { bool resident, staged; uint32_t lds, max_leaves; std::vector<uint32_t> bodies; };
     std::vector<Launch> launches;
     const uint32_t bins[] = { 24 * 1024, 40 * 1024, 64 * 1024, 96 * 1024, kMaxLdsBytes };
     std::vector<uint32_t> order = ctx->streamed;
@@ -1103,10 +1183,16 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
         const uint32_t nf = b.dev.pressure > 0.0f ? sd.num_faces : 0u;
         const bool resident = sd.resident != 0;
         bool staged = false;
-        uint32_t need = kernel_lds_bytes(resident, false, sd, nc, nc, nf);
+        // collider leaves: as many as the LDS left over holds, at most one per collider (a
+        // body that overlaps more colliders in one step raises the capacity error)
+        uint32_t max_leaves = nc;
+        const uint32_t no_leaves = kernel_lds_bytes(resident, false, sd, 0, nc, nf);
+        if (no_leaves <= kMaxLdsBytes)
+            max_leaves = std::min(nc, (kMaxLdsBytes - no_leaves) / bundle_leaf_bytes());
+        uint32_t need = kernel_lds_bytes(resident, false, sd, max_leaves, nc, nf);
         if (resident)
         {
-            uint32_t need_staged = kernel_lds_bytes(true, true, sd, nc, nc, nf);
+            uint32_t need_staged = kernel_lds_bytes(true, true, sd, max_leaves, nc, nf);
             if (need_staged <= kMaxLdsBytes)
             {
                 staged = true;
@@ -1125,11 +1211,11 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
         while (bins[bin] < need) ++bin;
         Launch *L = nullptr;
         for (Launch &l : launches)
-            if (l.resident == resident && l.staged == staged && l.lds == bins[bin])
+            if (l.resident == resident && l.staged == staged && l.lds == bins[bin] && l.max_leaves == max_leaves)
                 L = &l;
         if (!L)
         {
-            launches.push_back({ resident, staged, bins[bin], {} });
+            launches.push_back({ resident, staged, bins[bin], max_leaves, {} });
             L = &launches.back();
         }
         L->bodies.push_back(i);
@@ -1156,8 +1242,39 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
     // Independent groups of work run concurrently on the context stream and the
     // auxiliary streams.  The streamed bodies go first on the context stream;
     // bundle launches follow, longest first, each on the stream with the least
-    // estimated work queued.
+    // work queued (longest-processing-time assignment).  Durations are the ones
+    // measured on earlier updates when every launch has one, else estimates.
     constexpr int kAux = xpbd_context::kAux;
+    using Timing = xpbd_context::LaunchTiming;
+    auto harvest = [](Timing &t) {
+        if (!t.pending || hipEventQuery(t.end) != hipSuccess)
+            return;
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, t.start, t.end) == hipSuccess && t.frames > 0)
+            t.ms_per_frame = (double)ms / t.frames;
+        t.pending = false;
+    };
+    auto record = [](Timing &t, hipStream_t stream, bool begin, int32_t frames) -> hipError_t {
+        hipError_t e = hipSuccess;
+        if (!t.start && (e = hipEventCreate(&t.start)) != hipSuccess)
+            return e;
+        if (!t.end && (e = hipEventCreate(&t.end)) != hipSuccess)
+            return e;
+        if (begin)
+        {
+            t.frames = frames;
+            t.pending = true;
+        }
+        return hipEventRecord(begin ? t.start : t.end, stream);
+    };
+    harvest(ctx->streamed_timing);
+    bool measured = launches.empty() || ctx->streamed_timing.ms_per_frame >= 0.0;
+    for (auto &t : ctx->bundle_timing)
+    {
+        harvest(*t);
+        measured = measured && t->ms_per_frame >= 0.0;
+    }
+
     HIP_TRY(hipEventRecord(ctx->fork, ctx->stream));
     for (int a = 0; a < kAux; ++a)
         HIP_TRY(hipStreamWaitEvent(ctx->aux[a], ctx->fork, 0));
@@ -1166,10 +1283,13 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
     for (int a = 0; a < kAux; ++a) streams[a + 1] = ctx->aux[a];
     double queued[kAux + 1] = {};
     uint32_t offset = 0;
+    if (!launches.empty())
+        HIP_TRY(record(ctx->streamed_timing, ctx->stream, true, frames));
+    double streamed_estimate = 0.0;
     for (const Launch &l : launches)
     {
         HIP_TRY(launch_update(l.resident, l.staged, ctx->d_bodies.p, ctx->d_states.p, ctx->d_list.p + offset, (uint32_t)l.bodies.size(),
-                              ctx->d_colliders.p, nc, nc, l.lds, gx, gy, gz, dt, collision_steps, frames, ctx->d_error.p, ctx->stream));
+                              ctx->d_colliders.p, nc, l.max_leaves, l.lds, gx, gy, gz, dt, collision_steps, frames, ctx->d_error.p, ctx->stream));
         offset += (uint32_t)l.bodies.size();
         std::vector<double> work;
         for (uint32_t bi : l.bodies)
@@ -1179,13 +1299,15 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
             const SettingsDev &sd = b.settings->dev;
             work.push_back((double)(sd.num_levels_total + 2u * (sd.num_par_windows + sd.num_tail_windows)) * (double)b.dev.iterations);
         }
-        queued[0] += LaunchCost(work, l.lds);
+        streamed_estimate += LaunchCost(work, l.lds);
     }
+    if (!launches.empty())
+        HIP_TRY(record(ctx->streamed_timing, ctx->stream, false, frames));
+    queued[0] = measured ? ctx->streamed_timing.ms_per_frame * (launches.empty() ? 0.0 : 1.0) : streamed_estimate;
+    auto cost_of = [&](size_t li) { return measured ? ctx->bundle_timing[li]->ms_per_frame : ctx->bundle_launches[li]->cost; };
     std::vector<size_t> by_cost(ctx->bundle_launches.size());
     for (size_t i = 0; i < by_cost.size(); ++i) by_cost[i] = i;
-    std::stable_sort(by_cost.begin(), by_cost.end(), [ctx](size_t a, size_t b) {
-        return ctx->bundle_launches[a]->cost > ctx->bundle_launches[b]->cost;
-    });
+    std::stable_sort(by_cost.begin(), by_cost.end(), [&](size_t a, size_t b) { return cost_of(a) > cost_of(b); });
     for (size_t li : by_cost)
     {
         const auto &L = *ctx->bundle_launches[li];
@@ -1193,9 +1315,11 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
         for (int q = 1; q <= kAux; ++q)
             if (queued[q] < queued[best])
                 best = q;
-        queued[best] += L.cost;
-        HIP_TRY(launch_bundles(L.staged, L.table.p, L.count, ctx->d_bodies.p, ctx->d_states.p, ctx->d_colliders.p, nc,
+        queued[best] += cost_of(li);
+        HIP_TRY(record(*ctx->bundle_timing[li], streams[best], true, frames));
+        HIP_TRY(launch_bundles(L.staged, L.block, L.uniform, L.table.p, L.count, ctx->d_bodies.p, ctx->d_states.p, ctx->d_colliders.p, nc,
                                L.lds, gx, gy, gz, dt, collision_steps, frames, streams[best]));
+        HIP_TRY(record(*ctx->bundle_timing[li], streams[best], false, frames));
     }
     for (int a = 0; a < kAux; ++a)
     {

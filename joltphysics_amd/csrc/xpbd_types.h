@@ -118,6 +118,7 @@ struct BundleDev
     uint32_t has_sleepers;
     uint32_t iterations;
     uint32_t staged;            // records + level offsets copied into LDS once per launch
+    float uniform_inv_mass;     // the inverse mass every vertex shares (bundles with 12 B LDS positions)
 };
 
 } // namespace xpbd

@@ -17,7 +17,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .records import (BEND_DIHEDRAL, BEND_NONE, EDGE, LRA_NONE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_PLANE,
+from .records import (BEND_DIHEDRAL, BEND_NONE, EDGE, LRA, LRA_NONE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_PLANE,
                       SHAPE_SPHERE, VERTEX, VOLUME, BodyParams, Collider)
 
 # Build flags (same bit meaning on the product and oracle builders)
@@ -35,6 +35,7 @@ class MeshSpec:
     faces: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.uint32))
     edges: np.ndarray = field(default_factory=lambda: np.zeros(0, EDGE))
     volumes: np.ndarray = field(default_factory=lambda: np.zeros(0, VOLUME))
+    lras: np.ndarray = field(default_factory=lambda: np.zeros(0, LRA))
     attrs: np.ndarray = field(default_factory=lambda: np.zeros((0, 5), np.float32))
     bend_type: int = BEND_DIHEDRAL
     angle_tolerance: float = float(np.float32(np.deg2rad(8.0)))
@@ -256,6 +257,48 @@ def config2_cloth_and_cube(cloth_n=512, cube_cells=48, num_spheres=64, seed=2, f
         c.bounds = collider_world_bounds(c)
     return Scene("cloth512_cube48", [cloth_body, cube_body], colliders, gravity=(0.0, 0.0, -9.81),
                  dt=1.0 / 60.0, collision_steps=4, frames=frames)
+
+
+def config5_block(cells=64, spacing=0.05, num_boxes=512, seed=5, frames=1, region=10.0):
+    """Tetrahedralised cells^3 block (5 tets per cell, spacing 0.05 m, inv-mass 1) with edge,
+    volume and long-range-attachment constraints: the top face's first cells x cells vertices
+    are pinned (4096 for 64 cells) and every free vertex gets a Euclidean LRA to its nearest
+    pinned vertex (max distance = rest distance).  Compliances log-uniform [1e-7, 1e-4] per
+    constraint; 512 static boxes (half extents U(0.2, 2) m, random rotation) in a 20 x 20 x 10 m
+    region around the block (`region` = 10: its half width); 8 collision steps x 10 iterations."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    v, e, t = tet_cube(cells, spacing, inv_mass=1.0, compliance=1e-6)
+    g = cells + 1
+    top = cells * g * g
+    pinned = np.array([top + ix + iy * g for iy in range(cells) for ix in range(cells)], np.int64)
+    v["inv_mass"][pinned] = 0.0
+    idx = np.arange(g ** 3, dtype=np.int64)
+    free = idx[v["inv_mass"] > 0.0]
+    ix, iy = free % g, (free // g) % g
+    anchor = top + np.minimum(ix, cells - 1) + np.minimum(iy, cells - 1) * g     # nearest pinned vertex
+    lra = np.zeros(len(free), LRA)
+    lra["v"][:, 0] = anchor
+    lra["v"][:, 1] = free
+    mesh = MeshSpec(vertices=v, edges=e, volumes=t, lras=lra, lra_multiplier=1.0,
+                    flags=CALC_EDGE_LENGTHS | CALC_VOLUMES | CALC_LRA_LENGTHS | OPTIMIZE)
+
+    def compliance(s):
+        r = np.random.Generator(np.random.PCG64(seed + 100))
+        s.edges["compliance"] = _log_uniform(r, 1e-7, 1e-4, len(s.edges)).astype(np.float32)
+        s.volumes["compliance"] = _log_uniform(r, 1e-7, 1e-4, len(s.volumes)).astype(np.float32)
+
+    body = BodySpec(mesh=mesh, position=(0.0, 0.0, 2.0),
+                    params=BodyParams(num_iterations=10, allow_sleeping=False), post=compliance)
+    colliders = []
+    for _ in range(num_boxes):
+        he = tuple(float(x) for x in rng.uniform(0.2, 2.0, 3))
+        p = (float(rng.uniform(-region, region)), float(rng.uniform(-region, region)),
+             float(rng.uniform(-0.5 * region, 0.5 * region)))
+        colliders.append(Collider(SHAPE_BOX, he, pos=p, rot=random_quaternion(rng)))
+    for c in colliders:
+        c.bounds = collider_world_bounds(c)
+    return Scene(f"block{cells}_lra_boxes{num_boxes}", [body], colliders, gravity=(0.0, 0.0, -9.81),
+                 dt=1.0 / 60.0, collision_steps=8, frames=frames)
 
 
 def random_quaternion(rng):

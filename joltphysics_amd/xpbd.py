@@ -72,6 +72,7 @@ class _BuildDesc(C.Structure):
                 ("faces", C.c_void_p), ("num_faces", C.c_uint32),
                 ("edges", C.c_void_p), ("num_edges", C.c_uint32),
                 ("volumes", C.c_void_p), ("num_volumes", C.c_uint32),
+                ("lras", C.c_void_p), ("num_lras", C.c_uint32),
                 ("attributes", C.c_void_p), ("num_attributes", C.c_uint32),
                 ("bend_type", C.c_int32), ("angle_tolerance", C.c_float),
                 ("lra_max_distance_multiplier", C.c_float), ("flags", C.c_uint32)]
@@ -165,13 +166,14 @@ def build_settings(mesh) -> SettingsArrays:
     f = np.ascontiguousarray(mesh.faces, np.uint32).reshape(-1, 3)
     e = np.ascontiguousarray(mesh.edges, EDGE)
     t = np.ascontiguousarray(mesh.volumes, VOLUME)
+    lr = np.ascontiguousarray(mesh.lras, LRA)
     a = np.asarray(mesh.attrs, np.float32).reshape(-1, 5)
     attrs = (_VertexAttributes * max(len(a), 1))()
     for i, row in enumerate(a):
         attrs[i].compliance, attrs[i].shear_compliance, attrs[i].bend_compliance = float(row[0]), float(row[1]), float(row[2])
         attrs[i].lra_type = int(row[3])
         attrs[i].lra_max_distance_multiplier = float(row[4])
-    d = _BuildDesc(_ptr(v), len(v), _ptr(f), len(f), _ptr(e), len(e), _ptr(t), len(t),
+    d = _BuildDesc(_ptr(v), len(v), _ptr(f), len(f), _ptr(e), len(e), _ptr(t), len(t), _ptr(lr), len(lr),
                    C.cast(attrs, C.c_void_p) if len(a) else None, len(a), int(mesh.bend_type),
                    float(mesh.angle_tolerance), float(mesh.lra_multiplier), int(mesh.flags))
     h = C.c_void_p()

@@ -108,9 +108,10 @@ class JoltReference:
         f = np.ascontiguousarray(mesh.faces, np.uint32).reshape(-1, 3)
         e = np.ascontiguousarray(mesh.edges, EDGE)
         t = np.ascontiguousarray(mesh.volumes, VOLUME)
+        lr = np.ascontiguousarray(mesh.lras, LRA)
         a = np.ascontiguousarray(mesh.attrs, np.float32).reshape(-1, 5)
         h = self.lib.jo_settings_build(_ptr(v), len(v), _ptr(f), len(f), _ptr(e), len(e), None, 0,
-                                       _ptr(t), len(t), None, 0, _ptr(a), len(a), int(mesh.bend_type),
+                                       _ptr(t), len(t), _ptr(lr), len(lr), _ptr(a), len(a), int(mesh.bend_type),
                                        float(mesh.angle_tolerance), float(mesh.lra_multiplier), int(mesh.flags))
         try:
             return self._read_settings(h)

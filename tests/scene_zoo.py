@@ -95,6 +95,23 @@ def batched(frames=20):
     return Scene("batched", bodies, _bounded(cols), gravity=(0.0, 0.0, -9.81), dt=1 / 60, collision_steps=1, frames=frames)
 
 
+def cloth_cube_spheres(frames=4):
+    """Config 2 at reduced size: 48^2 cloth with per-constraint compliance log-U[1e-6, 1e-3]
+    and an 8^3-cell tet cube, ground plane + 64 spheres, 4 collision steps x 8 iterations."""
+    return scenes.config2_cloth_and_cube(cloth_n=48, cube_cells=8, num_spheres=64, frames=frames)
+
+
+def block_lra_boxes(cells=10, num_boxes=64, frames=2):
+    """Config 5 at reduced size: tet block with edge, volume and LRA constraints among dense
+    static boxes (many leaves per body), 8 collision steps x 10 iterations."""
+    return scenes.config5_block(cells=cells, num_boxes=num_boxes, frames=frames, region=1.5)
+
+
+def block_lra_boxes_streamed(frames=1):
+    """Config 5 with a 21^3-cell block: 10,648 vertices, streamed windows."""
+    return block_lra_boxes(cells=21, num_boxes=128, frames=frames)
+
+
 ALL = {
     "cloth100_sphere": big_cloth_on_sphere,
     "tetcube_plane_box": tet_cube_on_plane,
@@ -102,4 +119,7 @@ ALL = {
     "lra_cloth": lra_cloth,
     "sleeping_cloth": sleeping_cloth,
     "batched": batched,
+    "cloth_cube_spheres": cloth_cube_spheres,
+    "block_lra_boxes": block_lra_boxes,
+    "block_lra_boxes_streamed": block_lra_boxes_streamed,
 }

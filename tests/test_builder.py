@@ -39,6 +39,7 @@ def _meshes():
     out["cloth40_euclid_lra"] = scenes.MeshSpec(vertices=v, faces=f, attrs=scenes.attrs(1e-5, 1e-5, 1e-3, 1, 1.1))
     v, f = scenes.cloth(24, 0.02, 1.0)
     out["cloth24_no_bend"] = scenes.MeshSpec(vertices=v, faces=f, attrs=scenes.attrs(1e-5, 2e-5, 1e-3), bend_type=0)
+    out["block6_explicit_lra"] = scenes.config5_block(cells=6, num_boxes=1).bodies[0].mesh   # explicit LRAs + lengths
     return out
 
 

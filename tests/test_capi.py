@@ -46,6 +46,12 @@ def test_null_arguments_are_rejected():
     assert lib.xpbd_synchronize(None) == -1
     assert lib.xpbd_settings_release(None) == -1
     assert lib.xpbd_context_destroy(None) == -1
+    h = ctypes.c_void_p()
+    assert lib.xpbd_rollout_create(None, None, ctypes.byref(h)) == -1 and not h.value
+    assert lib.xpbd_rollout_forward(None, None, None, None, None) == -1
+    assert lib.xpbd_rollout_forward_f64(None, None, None, None, None) == -1
+    assert lib.xpbd_rollout_backward(None, None, None, None, None) == -1
+    assert lib.xpbd_rollout_destroy(None) == -1
 
 
 def test_bench_refuses_more_than_one_gpu():
