@@ -81,16 +81,16 @@ template <> struct BundlePositions<true>
 };
 
 template <bool STAGED, int BLOCK, bool UNIFORM>
-__global__ __launch_bounds__(BLOCK) void bundle_kernel(
-    const BundleDev *__restrict__ bundles, const BodyDev *__restrict__ bodies, BodyStateDev *__restrict__ states,
+__device__ __forceinline__ void bundle_update(
+    const BundleDev &G, const BodyDev *__restrict__ bodies, BodyStateDev *__restrict__ states,
     const ColliderDev *__restrict__ colliders, uint32_t num_colliders,
     float gx, float gy, float gz, float dt, int32_t steps, int32_t frames)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
+    const unsigned long long t_begin = __builtin_amdgcn_s_memrealtime();
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wave = tid >> 6;
-    const BundleDev &G = bundles[blockIdx.x];
     const uint32_t nv = G.nv;
     const uint32_t nb = G.num_bodies;
     const uint32_t nlev = G.nlev;
@@ -240,29 +240,32 @@ __global__ __launch_bounds__(BLOCK) void bundle_kernel(
 
             // ---- ApplyPressure (SoftBodyMotionProperties.cpp:291-323)
             auto apply_pressure = [&]() {
-                for (uint32_t b = 0; b < nb; ++b)
+                // GetVolumeTimesSix: a sequential sum in face order, one wavefront per
+                // body.  The lanes compute the terms of 64 faces at a time and the sum
+                // takes them in lane order, so the additions happen in face order.
+                for (uint32_t b = wave; b < nb; b += (BLOCK / kWave))
                 {
                     const BodyLocal &L = loc[b];
                     if (L.nf == 0 || !st[b].active)
                         continue;
-                    const BundleBodyDev bb = G.bodies[b];
-                    for (uint32_t f = tid; f < L.nf; f += BLOCK)
+                    const uint32_t voff = G.bodies[b].voff;
+                    float six = 0.0f;
+                    for (uint32_t f0 = 0; f0 < L.nf; f0 += kWave)
                     {
-                        V3 x1 = xyz(pget(X, bb.voff + L.faces[3 * f])), x2 = xyz(pget(X, bb.voff + L.faces[3 * f + 1])), x3 = xyz(pget(X, bb.voff + L.faces[3 * f + 2]));
-                        G.face_scratch[bb.face_off + f] = make_float4(0.0f, 0.0f, 0.0f, dot(cross(x1, x2), x3));
+                        const uint32_t f = f0 + (uint32_t)lane;
+                        float term = 0.0f;
+                        if (f < L.nf)
+                        {
+                            V3 x1 = xyz(pget(X, voff + L.faces[3 * f])), x2 = xyz(pget(X, voff + L.faces[3 * f + 1])), x3 = xyz(pget(X, voff + L.faces[3 * f + 2]));
+                            term = dot(cross(x1, x2), x3);
+                        }
+                        const uint32_t count = L.nf - f0 < (uint32_t)kWave ? L.nf - f0 : (uint32_t)kWave;
+                        for (uint32_t k = 0; k < count; ++k)
+                            six += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(term), k));
                     }
-                }
-                __syncthreads();
-                // GetVolumeTimesSix: sequential sum in face order, one lane per body
-                for (uint32_t b = wave; b < nb; b += (BLOCK / kWave))
-                    if (lane == 0 && loc[b].nf > 0 && st[b].active)
-                    {
-                        const uint32_t fo = G.bodies[b].face_off;
-                        float six = 0.0f;
-                        for (uint32_t f = 0; f < loc[b].nf; ++f)
-                            six += G.face_scratch[fo + f].w;
+                    if (lane == 0)
                         loc[b].six_volume = six;
-                    }
+                }
                 __syncthreads();
                 for (uint32_t b = 0; b < nb; ++b)
                 {
@@ -516,6 +519,22 @@ __global__ __launch_bounds__(BLOCK) void bundle_kernel(
         G.x[i] = pget(X, i);
     for (uint32_t b = tid; b < nb; b += BLOCK)
         states[G.bodies[b].body] = st[b];
+    // duration of the whole launch for this bundle: the host orders the bundles of
+    // later updates by it, longest first
+    if (tid == 0 && G.elapsed)
+        *G.elapsed = __builtin_amdgcn_s_memrealtime() - t_begin;
+}
+
+// One workgroup per bundle.  STAGED: the records and level offsets are copied
+// into LDS at launch (small programs only, 256-thread launches); UNIFORM: three
+// floats per vertex in LDS and the one inverse mass of the bundle.
+template <bool STAGED, int BLOCK, bool UNIFORM>
+__global__ __launch_bounds__(BLOCK) void bundle_kernel(
+    const BundleDev *__restrict__ bundles, const BodyDev *__restrict__ bodies, BodyStateDev *__restrict__ states,
+    const ColliderDev *__restrict__ colliders, uint32_t num_colliders,
+    float gx, float gy, float gz, float dt, int32_t steps, int32_t frames)
+{
+    bundle_update<STAGED, BLOCK, UNIFORM>(bundles[blockIdx.x], bodies, states, colliders, num_colliders, gx, gy, gz, dt, steps, frames);
 }
 
 uint32_t bundle_lds_bytes(uint32_t nv, uint32_t nrec, uint32_t nlvl, uint32_t nb, bool uniform)
@@ -545,17 +564,21 @@ static hipError_t launch_bundles_uniform(bool staged, uint32_t block, const Bund
                                          uint32_t num_colliders, uint32_t lds_bytes, float gx, float gy, float gz, float dt,
                                          int32_t steps, int32_t frames, hipStream_t stream)
 {
-    if (staged)
+    if (staged && block == 256)
         return launch_bundles_variant<true, 256, U>(bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
+    if (staged)
+        return hipErrorInvalidValue;
     if (block == 1024)
         return launch_bundles_variant<false, 1024, U>(bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
     if (block == 512)
         return launch_bundles_variant<false, 512, U>(bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
-    return launch_bundles_variant<false, 256, U>(bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
+    if (block == 256)
+        return launch_bundles_variant<false, 256, U>(bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
+    return hipErrorInvalidValue;
 }
 
-// block: 256, 512 or 1024 threads (staged bundles always use 256); uniform: the
-// bundle's vertices share one inverse mass and keep three floats per vertex in LDS
+// block: 256, 512 or 1024 threads (staged bundles only in 256-thread launches); uniform:
+// the bundles' vertices share one inverse mass and keep three floats per vertex in LDS
 hipError_t launch_bundles(bool staged, uint32_t block, bool uniform, const BundleDev *bundles, uint32_t num_bundles,
                           const BodyDev *bodies, BodyStateDev *states, const ColliderDev *colliders, uint32_t num_colliders,
                           uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,

@@ -174,32 +174,26 @@ struct xpbd_context
     std::vector<uint32_t> streamed;
     struct BundleLaunch
     {
-        bool staged = false;
-        bool uniform = false;   // bundles whose vertices share one inverse mass (12 B LDS positions)
+        bool staged = false;    // records staged in LDS (256-thread launches)
+        bool uniform = false;   // 12 B LDS positions, one inverse mass per bundle
         uint32_t block = 256;   // threads per workgroup
-        uint32_t lds = 0;
+        uint32_t lds = 0;       // LDS bytes per workgroup
         DevBuf<BundleDev> table;
         uint32_t count = 0;
-        double cost = 0.0;      // estimated duration (level passes per step, see LaunchCost)
+        double cost = 0.0;      // estimated duration of its longest workgroup (µs per collision step)
+        double work = 0.0;      // estimated CU-microseconds per collision step
+        std::vector<BundleDev> host;    // the table in launch order
     };
     std::vector<std::unique_ptr<BundleLaunch>> bundle_launches;
-    // Measured duration of each launch group (per bundle launch, plus one for the
-    // streamed bodies): events around the launch on its stream, harvested at the
-    // next update, so later updates assign launches to streams by measured time.
-    struct LaunchTiming
-    {
-        hipEvent_t start = nullptr, end = nullptr;
-        int32_t frames = 0;
-        bool pending = false;
-        double ms_per_frame = -1.0;
-        ~LaunchTiming()
-        {
-            if (start) (void)hipEventDestroy(start);
-            if (end) (void)hipEventDestroy(end);
-        }
-    };
-    std::vector<std::unique_ptr<LaunchTiming>> bundle_timing;
-    LaunchTiming streamed_timing;
+    // Measured duration of every bundle's workgroup (realtime-clock ticks, one slot
+    // per bundle).  The first update of a layout reads them back; the next one
+    // orders each launch's bundles longest first by them (workgroups are
+    // dispatched in index order, so the longest ones start first).
+    DevBuf<unsigned long long> d_elapsed;
+    unsigned long long *h_elapsed = nullptr;    // pinned host copy
+    size_t h_elapsed_n = 0;
+    hipEvent_t elapsed_ready = nullptr;
+    enum { kOrderMeasure, kOrderPending, kOrderMeasured } order_state = kOrderMeasure;
     // The context stream plus three auxiliary streams: one per hardware queue
     // of a process (a fifth stream would share a queue and serialise behind it).
     static constexpr int kAux = 3;
@@ -644,18 +638,41 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
 // ------------------------------------------------------------------ bundles
 // Bodies whose positions fit in LDS are packed into bundles that share one
 // workgroup.  Bodies are sorted by the length of their level sequence so a
-// bundle holds bodies of similar depth, then packed greedily up to a vertex
-// capacity.  The bundle's program merges the bodies' level sequences by index;
-// inside a merged level the records are grouped by constraint type and each
-// group is padded to whole wavefronts when that does not add a pass.
-constexpr uint32_t kBundleMaxBodies = 32;
-constexpr uint32_t kBundleBlock = 256;
+// bundle holds bodies of similar depth, then packed greedily while the bundle
+// fits the middle launch class.  The bundle's program merges the bodies' level
+// sequences by index; inside a merged level the records are grouped by
+// constraint type and each group is padded to whole wavefronts when that does
+// not add a pass.
+constexpr uint32_t kBundleMaxBodies = 64;
 
-uint32_t BundleCapacity()
+// Launch classes of the bundles.  A CU has 160 KB of LDS, so the classes hold
+// 4, 2 and 1 workgroups per CU; their threads per workgroup keep the wavefronts
+// of co-resident workgroups within what the kernel's registers allow (5 per
+// SIMD at its ~86 VGPRs).  Bundles are packed to the middle class, so most of
+// a large update runs in two launches (middle and large) plus the streamed
+// bodies' launch, each on a hardware queue of its own; the small class takes
+// the bundles that end up small (the last of a group, small scenes).
+struct BundleClass { uint32_t lds; uint32_t block; double per_cu; };
+constexpr int kNumBundleClasses = 3;
+constexpr BundleClass kBundleClasses[kNumBundleClasses] = {
+    { 40u * 1024u, 256u, 4.0 }, { 80u * 1024u, 512u, 2.0 }, { kMaxLdsBytes, 1024u, 1.0 } };
+
+// Duration of one level pass (µs) of a workgroup of `block` threads, measured
+// with per-workgroup timestamps on MI355X: the pass is a dependent chain of LDS
+// reads, one projection and LDS writes, then a workgroup barrier.
+double LevelPassMicroseconds(uint32_t block)
 {
-    const char *e = getenv("XPBD_BUNDLE_VERTICES");
-    uint32_t v = e ? (uint32_t)atoi(e) : 2688u;
-    return std::max(64u, std::min(v, kResidentMaxVertices));
+    return block >= 1024u ? 1.06 : block >= 512u ? 0.97 : 0.90;
+}
+
+// Level passes of a program for a workgroup of `block` threads: a level of n
+// records takes ceil(n / block) passes.
+double ProgramPasses(const std::vector<uint32_t> &lvl, uint32_t block)
+{
+    double p = 0.0;
+    for (size_t k = 0; k + 1 < lvl.size(); ++k)
+        p += (double)((lvl[k + 1] - lvl[k] + block - 1) / block);
+    return p;
 }
 
 uint4 OffsetRecord(uint4 r, uint32_t voff)
@@ -674,11 +691,12 @@ uint4 OffsetRecord(uint4 r, uint32_t voff)
 void BuildBundleProgram(const xpbd_context *ctx, const BundleHost &bh, const std::vector<uint32_t> &voffs, uint32_t block,
                         std::vector<uint4> &recs, std::vector<uint32_t> &lvl)
 {
+    recs.clear();
+    lvl.clear();
     size_t K = 0;
     for (uint32_t bi : bh.bodies)
         K = std::max(K, ctx->bodies[bi]->settings->host_level_begin.size() - 1);
     std::vector<uint4> bucket[4];
-    const uint4 nop = make_uint4(kNopRecordHost, 0u, 0u, 0u);
     for (size_t k = 0; k < K; ++k)
     {
         for (auto &b : bucket) b.clear();
@@ -701,39 +719,18 @@ void BuildBundleProgram(const xpbd_context *ctx, const BundleHost &bh, const std
     lvl.push_back((uint32_t)recs.size());
 }
 
-// Threads per bundle workgroup: bundles of one large body get wider workgroups
-// (their parallel levels hold several hundred records and their per-vertex
-// loops thousands of vertices); they hold a CU's LDS alone either way.
-uint32_t BundleBlock(uint32_t nv)
+// Estimated duration (µs per collision step) of a streamed body: its gathered
+// windows cost about one level pass each for the gather and the scatter.
+double StreamedMicroseconds(const Body &b)
 {
-    if (nv > 4096u)
-        return 1024u;
-    if (nv > 2688u)
-        return 512u;
-    return 256u;
-}
-
-// Estimated duration of a launch in level passes per step: its workgroups are
-// latency bound (one level pass after another), so a launch takes as long as
-// its deepest workgroup, or longer when its workgroups do not all fit on the
-// chip at once (workgroups per CU are set by the LDS each one holds).
-double LaunchCost(const std::vector<double> &work, uint32_t lds)
-{
-    const double per_cu = std::max(1u, kMaxLdsBytes / std::max(lds, 1u));
-    const double slots = 256.0 * per_cu;
-    double total = 0.0, deepest = 0.0;
-    for (double w : work)
-    {
-        total += w;
-        deepest = std::max(deepest, w);
-    }
-    return std::max(deepest, total / slots);
+    const SettingsDev &sd = b.settings->dev;
+    return (double)(sd.num_levels_total + 2u * (sd.num_par_windows + sd.num_tail_windows)) * (double)b.dev.iterations
+           * LevelPassMicroseconds(1024u);
 }
 
 int BuildLayout(xpbd_context *ctx)
 {
     const uint32_t nc = (uint32_t)ctx->colliders.size();
-    const uint32_t cap = BundleCapacity();
     // A body joins the bundles when its identity program exists and its positions
     // fit in one workgroup's LDS: 12 B per vertex when all its vertices share one
     // inverse mass, else 16 B.  Bundles never mix inverse-mass classes.
@@ -745,6 +742,7 @@ int BuildLayout(xpbd_context *ctx)
     };
     std::vector<uint32_t> resident;
     ctx->streamed.clear();
+    double critical = 0.0;      // longest estimated duration of a single workgroup (µs per step)
     for (uint32_t i = 0; i < ctx->bodies.size(); ++i)
     {
         const Body &b = *ctx->bodies[i];
@@ -752,7 +750,10 @@ int BuildLayout(xpbd_context *ctx)
         if (ctx->use_bundles && fits && !b.settings->host_level_begin.empty())
             resident.push_back(i);
         else
+        {
             ctx->streamed.push_back(i);
+            critical = std::max(critical, StreamedMicroseconds(b));
+        }
     }
     // same iteration count and inverse-mass class together, deepest level sequences first
     std::stable_sort(resident.begin(), resident.end(), [&](uint32_t a, uint32_t b) {
@@ -766,14 +767,17 @@ int BuildLayout(xpbd_context *ctx)
         return A.settings->host_level_begin.size() > B.settings->host_level_begin.size();
     });
     std::vector<std::unique_ptr<BundleHost>> bundles;
+    const uint32_t pack_lds = kBundleClasses[1].lds;
     for (uint32_t bi : resident)
     {
         const Body &b = *ctx->bodies[bi];
         BundleHost *cur = bundles.empty() ? nullptr : bundles.back().get();
         const bool same_class = cur && uniform_of(cur->bodies[0]) == uniform_of(bi)
-                                && (!uniform_of(bi) || mass_bits(cur->bodies[0]) == mass_bits(bi));
-        if (!cur || !same_class || cur->bodies.size() >= kBundleMaxBodies || cur->nv + b.nv > std::max(cap, b.nv)
-            || ctx->bodies[cur->bodies[0]]->dev.iterations != b.dev.iterations || (cur->nv > 0 && cur->nv + b.nv > cap))
+                                && (!uniform_of(bi) || mass_bits(cur->bodies[0]) == mass_bits(bi))
+                                && ctx->bodies[cur->bodies[0]]->dev.iterations == b.dev.iterations;
+        const bool fits = same_class && cur->bodies.size() < kBundleMaxBodies
+                          && bundle_lds_bytes(cur->nv + b.nv, 0, 0, (uint32_t)cur->bodies.size() + 1u, uniform_of(bi)) <= pack_lds;
+        if (!fits)
         {
             bundles.emplace_back(new BundleHost);
             cur = bundles.back().get();
@@ -784,40 +788,129 @@ int BuildLayout(xpbd_context *ctx)
         cur->nv += b.nv;
     }
 
-    std::vector<std::unique_ptr<xpbd_context::BundleLaunch>> launches;
-    std::vector<std::vector<BundleDev>> launch_tables;
-    std::vector<std::vector<double>> launch_work;
-    // LDS bins: 4, 3, 2 and 1 workgroups per CU
-    const uint32_t bins[] = { 40u * 1024u, 54u * 1024u, 80u * 1024u, kMaxLdsBytes };
-    for (auto &bp : bundles)
+    // Programs and launch classes.  Each bundle goes to the class that spends the
+    // fewest CU-microseconds on it (duration / workgroups per CU) without making
+    // it longer than the longest workgroup the update has anyway.
+    struct Plan
     {
-        BundleHost &bh = *bp;
+        std::vector<uint32_t> voffs;
+        std::vector<uint4> recs;
+        std::vector<uint32_t> lvl;
+        double passes[3];       // level passes per iteration at 256, 512, 1024 threads
+        uint32_t plain_lds, staged_lds, iterations, foff;
+        bool has_pressure, has_sleepers;
+        int cls;
+        double us;              // estimated µs per collision step in its class
+    };
+    std::vector<Plan> plans(bundles.size());
+    const uint32_t blocks[3] = { 256u, 512u, 1024u };
+    for (size_t k = 0; k < bundles.size(); ++k)
+    {
+        BundleHost &bh = *bundles[k];
+        Plan &P = plans[k];
         const uint32_t nb = (uint32_t)bh.bodies.size();
-        std::vector<uint32_t> voffs(nb);
-        std::vector<BundleBodyDev> table(nb);
+        P.voffs.resize(nb);
         uint32_t voff = 0, foff = 0;
-        bool has_pressure = false, has_sleepers = false;
+        P.has_pressure = P.has_sleepers = false;
         for (uint32_t j = 0; j < nb; ++j)
         {
             const Body &b = *ctx->bodies[bh.bodies[j]];
-            voffs[j] = voff;
-            table[j] = { bh.bodies[j], voff, b.nv, foff };
+            P.voffs[j] = voff;
             voff += b.nv;
             if (b.dev.pressure > 0.0f && b.settings->num_faces > 0)
             {
                 foff += b.settings->num_faces;
-                has_pressure = true;
+                P.has_pressure = true;
             }
-            has_sleepers |= b.dev.allow_sleeping != 0;
+            P.has_sleepers |= b.dev.allow_sleeping != 0;
         }
         if (voff >= kMaxWindowVertices)
             return Fail(XPBD_E_CAPACITY, "bundle exceeds the LDS window index range");
-        std::vector<uint4> recs;
-        std::vector<uint32_t> lvl;
-        const uint32_t block = BundleBlock(voff);
-        BuildBundleProgram(ctx, bh, voffs, block, recs, lvl);
-        HIP_TRY(bh.recs.upload(recs));
-        HIP_TRY(bh.lvl.upload(lvl));
+        bh.nv = voff;
+        P.foff = foff;
+        P.iterations = ctx->bodies[bh.bodies[0]]->dev.iterations;
+        BuildBundleProgram(ctx, bh, P.voffs, 256u, P.recs, P.lvl);
+        for (int q = 0; q < 3; ++q)
+            P.passes[q] = ProgramPasses(P.lvl, blocks[q]);
+        const uint32_t nlev = (uint32_t)P.lvl.size() - 1;
+        P.plain_lds = bundle_lds_bytes(voff, 0, 0, nb, bh.uniform);
+        P.staged_lds = bundle_lds_bytes(voff, (uint32_t)P.recs.size(), nlev + 1, nb, bh.uniform);
+        if (P.plain_lds > kMaxLdsBytes)
+            return Fail(XPBD_E_CAPACITY, "bundle needs more LDS than a workgroup has");
+        // shortest duration any class can give this bundle
+        double best = 1e300;
+        for (int q = 0; q < 3; ++q)
+            best = std::min(best, P.iterations * P.passes[q] * LevelPassMicroseconds(blocks[q]));
+        critical = std::max(critical, best);
+    }
+    auto class_block = [&](int c) { return kBundleClasses[c].block; };
+    auto class_us = [&](const Plan &P, int c) {
+        const uint32_t blk = class_block(c);
+        return P.iterations * P.passes[blk == 1024u ? 2 : blk == 512u ? 1 : 0] * LevelPassMicroseconds(blk);
+    };
+    for (Plan &P : plans)
+    {
+        // fewest CU-microseconds among the classes that keep the bundle within the
+        // critical duration; the shortest class when none does
+        P.cls = -1;
+        int fastest = -1;
+        double best_cost = 1e300, fastest_us = 1e300;
+        for (int c = 0; c < kNumBundleClasses; ++c)
+        {
+            if (P.plain_lds > kBundleClasses[c].lds)
+                continue;
+            const double us = class_us(P, c);
+            if (us < fastest_us)
+            {
+                fastest = c;
+                fastest_us = us;
+            }
+            const double cost = us / kBundleClasses[c].per_cu;
+            if (us <= critical * 1.0001 && cost < best_cost)
+            {
+                P.cls = c;
+                best_cost = cost;
+            }
+        }
+        if (P.cls < 0)
+            P.cls = fastest;
+        P.us = class_us(P, P.cls);
+    }
+
+    HIP_TRY(ctx->d_elapsed.alloc(std::max<size_t>(1, bundles.size())));
+    if (ctx->h_elapsed_n < bundles.size())
+    {
+        if (ctx->h_elapsed)
+            HIP_TRY(hipHostFree(ctx->h_elapsed));
+        ctx->h_elapsed = nullptr;
+        ctx->h_elapsed_n = 0;
+        HIP_TRY(hipHostMalloc((void **)&ctx->h_elapsed, sizeof(unsigned long long) * bundles.size(), hipHostMallocDefault));
+        ctx->h_elapsed_n = bundles.size();
+    }
+    ctx->order_state = xpbd_context::kOrderMeasure;
+    // one launch per class and LDS layout (staged records, 12 B positions): slot 4 class + 2 staged + uniform
+    std::vector<std::unique_ptr<xpbd_context::BundleLaunch>> launches(4 * kNumBundleClasses);
+    std::vector<std::vector<std::pair<double, BundleDev>>> launch_tables(4 * kNumBundleClasses);
+    for (size_t k = 0; k < bundles.size(); ++k)
+    {
+        BundleHost &bh = *bundles[k];
+        Plan &P = plans[k];
+        const uint32_t nb = (uint32_t)bh.bodies.size();
+        const uint32_t block = class_block(P.cls);
+        if (block != 256u)
+            BuildBundleProgram(ctx, bh, P.voffs, block, P.recs, P.lvl);
+        std::vector<BundleBodyDev> table(nb);
+        uint32_t foff = 0;
+        for (uint32_t j = 0; j < nb; ++j)
+        {
+            const Body &b = *ctx->bodies[bh.bodies[j]];
+            table[j] = { bh.bodies[j], P.voffs[j], b.nv, foff };
+            if (b.dev.pressure > 0.0f && b.settings->num_faces > 0)
+                foff += b.settings->num_faces;
+        }
+        const uint32_t voff = bh.nv;
+        HIP_TRY(bh.recs.upload(P.recs));
+        HIP_TRY(bh.lvl.upload(P.lvl));
         HIP_TRY(bh.table.upload(table));
         // bundle-contiguous vertex state, gathered from wherever each body's state lives now
         HIP_TRY(bh.x.alloc(voff)); HIP_TRY(bh.v.alloc(voff)); HIP_TRY(bh.prev.alloc(voff)); HIP_TRY(bh.plane.alloc(voff));
@@ -825,7 +918,7 @@ int BuildLayout(xpbd_context *ctx)
         for (uint32_t j = 0; j < nb; ++j)
         {
             Body &b = *ctx->bodies[bh.bodies[j]];
-            const size_t o = voffs[j], n = b.nv;
+            const size_t o = P.voffs[j], n = b.nv;
             HIP_TRY(hipMemcpy(bh.x.p + o, b.dev.x, n * sizeof(float4), hipMemcpyDeviceToDevice));
             HIP_TRY(hipMemcpy(bh.v.p + o, b.dev.v, n * sizeof(float4), hipMemcpyDeviceToDevice));
             HIP_TRY(hipMemcpy(bh.prev.p + o, b.dev.prev, n * sizeof(float4), hipMemcpyDeviceToDevice));
@@ -835,43 +928,34 @@ int BuildLayout(xpbd_context *ctx)
         }
         HIP_TRY(bh.leaves_full.alloc((size_t)std::max(1u, nb * nc) * bundle_leaf_bytes()));
         HIP_TRY(bh.leaves.alloc((size_t)std::max(1u, nb * nc) * bundle_leaf_bytes()));
-        HIP_TRY(bh.face_scratch.alloc(std::max(1u, foff)));
-        bh.nv = voff;
+        HIP_TRY(bh.face_scratch.alloc(std::max(1u, P.foff)));
         BundleDev &d = bh.dev;
-        d.recs = bh.recs.p; d.lvl = bh.lvl.p; d.nlev = (uint32_t)lvl.size() - 1; d.num_records = (uint32_t)recs.size();
+        d.recs = bh.recs.p; d.lvl = bh.lvl.p; d.nlev = (uint32_t)P.lvl.size() - 1; d.num_records = (uint32_t)P.recs.size();
         d.bodies = bh.table.p; d.num_bodies = nb; d.nv = voff;
         d.x = bh.x.p; d.v = bh.v.p; d.prev = bh.prev.p; d.plane = bh.plane.p; d.idx = bh.idx.p; d.contact = bh.contact.p;
         d.leaves_full = bh.leaves_full.p; d.leaves = bh.leaves.p; d.face_scratch = bh.face_scratch.p;
-        d.has_pressure = has_pressure ? 1u : 0u;
-        d.has_sleepers = has_sleepers ? 1u : 0u;
-        d.iterations = ctx->bodies[bh.bodies[0]]->dev.iterations;
+        d.has_pressure = P.has_pressure ? 1u : 0u;
+        d.has_sleepers = P.has_sleepers ? 1u : 0u;
+        d.iterations = P.iterations;
+        d.uniform = bh.uniform ? 1u : 0u;
         d.uniform_inv_mass = bh.uniform ? bh.inv_mass : 0.0f;
-        const uint32_t need_staged = bundle_lds_bytes(voff, d.num_records, d.nlev + 1, nb, bh.uniform);
-        const uint32_t need_plain = bundle_lds_bytes(voff, 0, 0, nb, bh.uniform);
-        if (need_plain > kMaxLdsBytes)
-            return Fail(XPBD_E_CAPACITY, "bundle needs more LDS than a workgroup has");
-        bh.staged = block == 256u && need_staged <= 40u * 1024u;
+        d.elapsed = ctx->d_elapsed.p + k;
+        // records staged in LDS when the bundle still fits the smallest class with them
+        bh.staged = block == 256u && bundle_lds_bytes(voff, d.num_records, d.nlev + 1, nb, bh.uniform) <= kBundleClasses[0].lds;
         d.staged = bh.staged ? 1u : 0u;
-        bh.lds = bh.staged ? need_staged : need_plain;
-        uint32_t bin = 0;
-        while (bins[bin] < bh.lds) ++bin;
-        size_t li = 0;
-        for (; li < launches.size(); ++li)
-            if (launches[li]->staged == bh.staged && launches[li]->lds == bins[bin] && launches[li]->block == block
-                && launches[li]->uniform == bh.uniform)
-                break;
-        if (li == launches.size())
+        bh.lds = kBundleClasses[P.cls].lds;
+        const int slot = 4 * P.cls + (bh.staged ? 2 : 0) + (bh.uniform ? 1 : 0);
+        if (!launches[slot])
         {
-            launches.emplace_back(new xpbd_context::BundleLaunch);
-            launches.back()->staged = bh.staged;
-            launches.back()->uniform = bh.uniform;
-            launches.back()->block = block;
-            launches.back()->lds = bins[bin];
-            launch_tables.emplace_back();
-            launch_work.emplace_back();
+            launches[slot].reset(new xpbd_context::BundleLaunch);
+            launches[slot]->staged = bh.staged;
+            launches[slot]->uniform = bh.uniform;
+            launches[slot]->block = block;
+            launches[slot]->lds = kBundleClasses[P.cls].lds;
         }
-        launch_tables[li].push_back(d);
-        launch_work[li].push_back((double)d.nlev * (double)d.iterations);
+        launch_tables[slot].emplace_back(P.us, d);
+        launches[slot]->cost = std::max(launches[slot]->cost, P.us);
+        launches[slot]->work += P.us / kBundleClasses[P.cls].per_cu;
     }
     // bodies now live in bundle storage: repoint them and drop their own buffers
     for (auto &bp : bundles)
@@ -889,16 +973,24 @@ int BuildLayout(xpbd_context *ctx)
             }
         }
     ctx->bundles.swap(bundles);      // old bundles (if any) are freed here, after the copies
-    for (size_t li = 0; li < launches.size(); ++li)
+    // Launch order, largest LDS first: a workgroup that needs a whole CU's LDS
+    // starts before smaller workgroups take parts of every CU.  Inside a launch the
+    // longest bundles come first (workgroups are dispatched in index order).
+    ctx->bundle_launches.clear();
+    for (int c = 4 * kNumBundleClasses - 1; c >= 0; --c)
     {
-        HIP_TRY(launches[li]->table.upload(launch_tables[li]));
-        launches[li]->count = (uint32_t)launch_tables[li].size();
-        launches[li]->cost = LaunchCost(launch_work[li], launches[li]->lds);
+        if (!launches[c])
+            continue;
+        auto &tab = launch_tables[c];
+        std::stable_sort(tab.begin(), tab.end(), [](const std::pair<double, BundleDev> &a, const std::pair<double, BundleDev> &b) {
+            return a.first > b.first;
+        });
+        std::vector<BundleDev> &devs = launches[c]->host;
+        for (auto &t : tab) devs.push_back(t.second);
+        HIP_TRY(launches[c]->table.upload(devs));
+        launches[c]->count = (uint32_t)devs.size();
+        ctx->bundle_launches.push_back(std::move(launches[c]));
     }
-    ctx->bundle_launches.swap(launches);
-    ctx->bundle_timing.clear();
-    for (size_t li = 0; li < ctx->bundle_launches.size(); ++li)
-        ctx->bundle_timing.emplace_back(new xpbd_context::LaunchTiming);
     ctx->layout_dirty = false;
     ctx->layout_colliders = nc;
     ctx->tables_dirty = true;
@@ -961,6 +1053,7 @@ int xpbd_context_create(int device, xpbd_context **out_ctx)
     ctx->device = device;
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&ctx->elapsed_ready, hipEventDisableTiming));
     for (int a = 0; a < xpbd_context::kAux; ++a)
     {
         HIP_TRY(hipStreamCreateWithFlags(&ctx->aux[a], hipStreamNonBlocking));
@@ -991,6 +1084,9 @@ int xpbd_context_destroy(xpbd_context *ctx)
     ctx->bodies.clear();
     ctx->bundle_launches.clear();
     (void)hipEventDestroy(ctx->fork);
+    (void)hipEventDestroy(ctx->elapsed_ready);
+    if (ctx->h_elapsed)
+        (void)hipHostFree(ctx->h_elapsed);
     for (int a = 0; a < xpbd_context::kAux; ++a)
     {
         (void)hipEventDestroy(ctx->join[a]);
@@ -1239,42 +1335,28 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
         HIP_TRY(hipEventCreate(&e1));
         HIP_TRY(hipEventRecord(e0, ctx->stream));
     }
-    // Independent groups of work run concurrently on the context stream and the
-    // auxiliary streams.  The streamed bodies go first on the context stream;
-    // bundle launches follow, longest first, each on the stream with the least
-    // work queued (longest-processing-time assignment).  Durations are the ones
-    // measured on earlier updates when every launch has one, else estimates.
-    constexpr int kAux = xpbd_context::kAux;
-    using Timing = xpbd_context::LaunchTiming;
-    auto harvest = [](Timing &t) {
-        if (!t.pending || hipEventQuery(t.end) != hipSuccess)
-            return;
-        float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, t.start, t.end) == hipSuccess && t.frames > 0)
-            t.ms_per_frame = (double)ms / t.frames;
-        t.pending = false;
-    };
-    auto record = [](Timing &t, hipStream_t stream, bool begin, int32_t frames) -> hipError_t {
-        hipError_t e = hipSuccess;
-        if (!t.start && (e = hipEventCreate(&t.start)) != hipSuccess)
-            return e;
-        if (!t.end && (e = hipEventCreate(&t.end)) != hipSuccess)
-            return e;
-        if (begin)
-        {
-            t.frames = frames;
-            t.pending = true;
-        }
-        return hipEventRecord(begin ? t.start : t.end, stream);
-    };
-    harvest(ctx->streamed_timing);
-    bool measured = launches.empty() || ctx->streamed_timing.ms_per_frame >= 0.0;
-    for (auto &t : ctx->bundle_timing)
+    // Order each bundle launch by the durations the first update of this layout
+    // measured (ordered on the context stream after the previous update's join,
+    // so no running workgroup reads a table while it is rewritten).
+    if (ctx->order_state == xpbd_context::kOrderPending && hipEventQuery(ctx->elapsed_ready) == hipSuccess)
     {
-        harvest(*t);
-        measured = measured && t->ms_per_frame >= 0.0;
+        for (auto &Lp : ctx->bundle_launches)
+        {
+            auto &L = *Lp;
+            auto ticks = [&](const BundleDev &d) { return ctx->h_elapsed[d.elapsed - ctx->d_elapsed.p]; };
+            std::stable_sort(L.host.begin(), L.host.end(), [&](const BundleDev &a, const BundleDev &b) { return ticks(a) > ticks(b); });
+            HIP_TRY(hipMemcpyAsync(L.table.p, L.host.data(), sizeof(BundleDev) * L.host.size(), hipMemcpyHostToDevice, ctx->stream));
+        }
+        ctx->order_state = xpbd_context::kOrderMeasured;
     }
 
+    // Independent groups of work run concurrently on the context stream and the
+    // auxiliary streams, one per hardware queue.  Kernels of one stream run one
+    // after the other, so each launch group gets a stream of its own while there
+    // are enough: the streamed bodies on the context stream, then the bundle
+    // launches, largest LDS first.  With more groups than streams a group goes to
+    // the stream with the least estimated work queued.
+    constexpr int kAux = xpbd_context::kAux;
     HIP_TRY(hipEventRecord(ctx->fork, ctx->stream));
     for (int a = 0; a < kAux; ++a)
         HIP_TRY(hipStreamWaitEvent(ctx->aux[a], ctx->fork, 0));
@@ -1283,43 +1365,26 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
     for (int a = 0; a < kAux; ++a) streams[a + 1] = ctx->aux[a];
     double queued[kAux + 1] = {};
     uint32_t offset = 0;
-    if (!launches.empty())
-        HIP_TRY(record(ctx->streamed_timing, ctx->stream, true, frames));
-    double streamed_estimate = 0.0;
     for (const Launch &l : launches)
     {
         HIP_TRY(launch_update(l.resident, l.staged, ctx->d_bodies.p, ctx->d_states.p, ctx->d_list.p + offset, (uint32_t)l.bodies.size(),
                               ctx->d_colliders.p, nc, l.max_leaves, l.lds, gx, gy, gz, dt, collision_steps, frames, ctx->d_error.p, ctx->stream));
         offset += (uint32_t)l.bodies.size();
-        std::vector<double> work;
+        double longest = 0.0;
         for (uint32_t bi : l.bodies)
-        {
-            const Body &b = *ctx->bodies[bi];
-            // gathered windows cost about one level pass each for the gather and the scatter
-            const SettingsDev &sd = b.settings->dev;
-            work.push_back((double)(sd.num_levels_total + 2u * (sd.num_par_windows + sd.num_tail_windows)) * (double)b.dev.iterations);
-        }
-        streamed_estimate += LaunchCost(work, l.lds);
+            longest = std::max(longest, StreamedMicroseconds(*ctx->bodies[bi]));
+        queued[0] += longest;
     }
-    if (!launches.empty())
-        HIP_TRY(record(ctx->streamed_timing, ctx->stream, false, frames));
-    queued[0] = measured ? ctx->streamed_timing.ms_per_frame * (launches.empty() ? 0.0 : 1.0) : streamed_estimate;
-    auto cost_of = [&](size_t li) { return measured ? ctx->bundle_timing[li]->ms_per_frame : ctx->bundle_launches[li]->cost; };
-    std::vector<size_t> by_cost(ctx->bundle_launches.size());
-    for (size_t i = 0; i < by_cost.size(); ++i) by_cost[i] = i;
-    std::stable_sort(by_cost.begin(), by_cost.end(), [&](size_t a, size_t b) { return cost_of(a) > cost_of(b); });
-    for (size_t li : by_cost)
+    for (const auto &Lp : ctx->bundle_launches)
     {
-        const auto &L = *ctx->bundle_launches[li];
+        const auto &L = *Lp;
         int best = 0;
         for (int q = 1; q <= kAux; ++q)
             if (queued[q] < queued[best])
                 best = q;
-        queued[best] += cost_of(li);
-        HIP_TRY(record(*ctx->bundle_timing[li], streams[best], true, frames));
+        queued[best] += std::max(L.cost, L.work / 256.0);
         HIP_TRY(launch_bundles(L.staged, L.block, L.uniform, L.table.p, L.count, ctx->d_bodies.p, ctx->d_states.p, ctx->d_colliders.p, nc,
                                L.lds, gx, gy, gz, dt, collision_steps, frames, streams[best]));
-        HIP_TRY(record(*ctx->bundle_timing[li], streams[best], false, frames));
     }
     for (int a = 0; a < kAux; ++a)
     {
@@ -1330,6 +1395,13 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
     {
         HIP_TRY(hipEventRecord(e1, ctx->stream));
         ctx->events.emplace_back(e0, e1);
+    }
+    if (ctx->order_state == xpbd_context::kOrderMeasure && !ctx->bundle_launches.empty())
+    {
+        HIP_TRY(hipMemcpyAsync(ctx->h_elapsed, ctx->d_elapsed.p, sizeof(unsigned long long) * ctx->bundles.size(),
+                               hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipEventRecord(ctx->elapsed_ready, ctx->stream));
+        ctx->order_state = xpbd_context::kOrderPending;
     }
     return XPBD_OK;
 }

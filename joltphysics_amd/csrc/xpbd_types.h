@@ -117,8 +117,10 @@ struct BundleDev
     uint32_t has_pressure;
     uint32_t has_sleepers;
     uint32_t iterations;
-    uint32_t staged;            // records + level offsets copied into LDS once per launch
-    float uniform_inv_mass;     // the inverse mass every vertex shares (bundles with 12 B LDS positions)
+    uint32_t staged;            // records + level offsets copied into LDS once per launch (256-thread launches)
+    uint32_t uniform;           // 12 B LDS positions: every vertex has the inverse mass uniform_inv_mass
+    float uniform_inv_mass;
+    unsigned long long *elapsed;    // where the workgroup stores its duration (realtime-clock ticks), or null
 };
 
 } // namespace xpbd
