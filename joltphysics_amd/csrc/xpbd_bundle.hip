@@ -206,7 +206,7 @@ __device__ __forceinline__ void bundle_update(
                 }
                 make_leaf(cd, inversed_rotation_translation(com), L.friction, L.restitution, out);
             }
-            __syncthreads();
+            sync_global();
             // colliding list of each body in collider (broadphase) order
             for (uint32_t b = tid; b < nb; b += BLOCK)
             {
@@ -216,7 +216,7 @@ __device__ __forceinline__ void bundle_update(
                         leaves[b * nc + n++] = leaves_full[b * nc + c];
                 loc[b].num_leaves = n;
             }
-            __syncthreads();
+            sync_global();
 
             // ---- DetermineCollisionPlanes (SoftBodyMotionProperties.cpp:257-265)
             for (uint32_t i = tid; i < nv; i += BLOCK)
@@ -281,7 +281,7 @@ __device__ __forceinline__ void bundle_update(
                         G.face_scratch[bb.face_off + f] = make_float4(impulse.x, impulse.y, impulse.z, 0.0f);
                     }
                 }
-                __syncthreads();
+                sync_global();
                 for (uint32_t i = tid; i < nv; i += BLOCK)
                 {
                     const uint32_t b = body_of(i);
@@ -423,6 +423,8 @@ __device__ __forceinline__ void bundle_update(
                 }
             }
 
+            // the reductions read velocities other wavefronts stored
+            sync_global();
             // ---- UpdateSoftBodyState reductions, one wavefront per body (SoftBodyMotionProperties.cpp:846-921)
             for (uint32_t b = wave; b < nb; b += (BLOCK / kWave))
             {

@@ -179,6 +179,19 @@ typedef uint32_t RecordVec __attribute__((ext_vector_type(4)));
 // s_waitcnt immediate (gfx9 encoding): vmcnt 0, expcnt 7, lgkmcnt 15.
 constexpr int kWaitVmcntZero = 0x0f70;
 
+// Workgroup barrier for data that the wavefronts of a workgroup hand to each
+// other through global memory (the level programs exchange data through LDS
+// only).  Every wavefront first waits until its own global stores have
+// completed; after the barrier the CU's vector L1 is invalidated, so the loads
+// that follow read the other wavefronts' stores from L2.  __syncthreads() alone
+// emits neither on gfx950: it orders LDS, and leaves global stores in flight.
+__device__ __forceinline__ void sync_global()
+{
+    __builtin_amdgcn_s_waitcnt(kWaitVmcntZero);
+    __syncthreads();
+    asm volatile("buffer_inv sc0" ::: "memory");
+}
+
 // Record load from global memory through an address-space-1 pointer, so the
 // load is a global (not flat) load: only vmcnt counts it and it can stay in
 // flight across the LDS-only barriers between levels.

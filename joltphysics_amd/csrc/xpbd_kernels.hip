@@ -110,6 +110,14 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
     float *red = reinterpret_cast<float *>(lds_raw + lay.red);
     // Per-vertex phases address positions here
     float4 *P = RESIDENT ? X : B.x;
+    // barrier after a per-vertex phase: streamed positions are handed between
+    // wavefronts through global memory
+    auto sync_positions = [&]() {
+        if (RESIDENT)
+            __syncthreads();
+        else
+            sync_global();
+    };
 
     if (tid == 0)
         st = states[bi];
@@ -287,7 +295,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                 apply_pressure();
             for (uint32_t i = tid; i < nv; i += kBlock)
                 integrate_vertex(i, P[i], xyz(B.v[i]));
-            __syncthreads();
+            sync_positions();
 
             const float restitution_threshold = -2.0f * len(g_local) * dt_sub;
             const float max_lin_sq = B.max_linear_velocity * B.max_linear_velocity;
@@ -315,7 +323,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                     {
                         for (uint32_t j = tid; j < W.vcount; j += kBlock)
                             B.x[S.vlist[W.vbegin + j]] = X[j];
-                        __syncthreads();
+                        sync_global();
                     }
                 }
 
@@ -388,13 +396,13 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                         B.contact[i] = 0;
                     }
                 }
-                __syncthreads();
+                sync_positions();
                 if (!last && has_pressure)
                 {
                     apply_pressure();
                     for (uint32_t i = tid; i < nv; i += kBlock)
                         integrate_vertex(i, P[i], xyz(B.v[i]));
-                    __syncthreads();
+                    sync_positions();
                 }
             }
 

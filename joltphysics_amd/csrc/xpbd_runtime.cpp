@@ -648,10 +648,11 @@ constexpr uint32_t kBundleMaxBodies = 64;
 // Launch classes of the bundles.  A CU has 160 KB of LDS, so the classes hold
 // 4, 2 and 1 workgroups per CU; their threads per workgroup keep the wavefronts
 // of co-resident workgroups within what the kernel's registers allow (5 per
-// SIMD at its ~86 VGPRs).  Bundles are packed to the middle class, so most of
-// a large update runs in two launches (middle and large) plus the streamed
-// bodies' launch, each on a hardware queue of its own; the small class takes
-// the bundles that end up small (the last of a group, small scenes).
+// SIMD at its ~86 VGPRs).  Bundles are packed to the middle class; single
+// bodies too large for it take the largest.  A large update runs in two bundle
+// launches plus the streamed bodies' launch, each on a hardware queue of its
+// own; the smallest class only takes bundles whose programs fit in LDS with
+// them (small scenes).
 struct BundleClass { uint32_t lds; uint32_t block; double per_cu; };
 constexpr int kNumBundleClasses = 3;
 constexpr BundleClass kBundleClasses[kNumBundleClasses] = {
@@ -857,7 +858,9 @@ int BuildLayout(xpbd_context *ctx)
         double best_cost = 1e300, fastest_us = 1e300;
         for (int c = 0; c < kNumBundleClasses; ++c)
         {
-            if (P.plain_lds > kBundleClasses[c].lds)
+            // the smallest class is for bundles whose records fit in LDS with them
+            // (small programs): a fourth bundle launch would share a hardware queue
+            if (P.plain_lds > kBundleClasses[c].lds || (c == 0 && P.staged_lds > kBundleClasses[0].lds))
                 continue;
             const double us = class_us(P, c);
             if (us < fastest_us)
