@@ -194,11 +194,13 @@ struct xpbd_context
     size_t h_elapsed_n = 0;
     hipEvent_t elapsed_ready = nullptr;
     enum { kOrderMeasure, kOrderPending, kOrderMeasured } order_state = kOrderMeasure;
-    // The context stream plus three auxiliary streams: one per hardware queue
-    // of a process (a fifth stream would share a queue and serialise behind it).
-    static constexpr int kAux = 3;
-    hipStream_t aux[kAux] = { nullptr, nullptr, nullptr };
-    hipEvent_t fork = nullptr, join[kAux] = { nullptr, nullptr, nullptr };
+    // The context stream plus auxiliary streams: one per hardware queue the HIP
+    // runtime gives the process (GPU_MAX_HW_QUEUES, 4 by default; at most 8 used).
+    // A stream beyond that would share a queue and serialise behind it.
+    static constexpr int kMaxAux = 7;
+    int num_aux = 3;
+    hipStream_t aux[kMaxAux] = {};
+    hipEvent_t fork = nullptr, join[kMaxAux] = {};
     bool use_bundles = true;
 };
 
@@ -1057,7 +1059,12 @@ int xpbd_context_create(int device, xpbd_context **out_ctx)
     HIP_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     HIP_TRY(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
     HIP_TRY(hipEventCreateWithFlags(&ctx->elapsed_ready, hipEventDisableTiming));
-    for (int a = 0; a < xpbd_context::kAux; ++a)
+    {
+        const char *q = getenv("GPU_MAX_HW_QUEUES");
+        const int queues = q && atoi(q) > 0 ? atoi(q) : 4;
+        ctx->num_aux = std::max(0, std::min(xpbd_context::kMaxAux, queues - 1));
+    }
+    for (int a = 0; a < ctx->num_aux; ++a)
     {
         HIP_TRY(hipStreamCreateWithFlags(&ctx->aux[a], hipStreamNonBlocking));
         HIP_TRY(hipEventCreateWithFlags(&ctx->join[a], hipEventDisableTiming));
@@ -1090,7 +1097,7 @@ int xpbd_context_destroy(xpbd_context *ctx)
     (void)hipEventDestroy(ctx->elapsed_ready);
     if (ctx->h_elapsed)
         (void)hipHostFree(ctx->h_elapsed);
-    for (int a = 0; a < xpbd_context::kAux; ++a)
+    for (int a = 0; a < ctx->num_aux; ++a)
     {
         (void)hipEventDestroy(ctx->join[a]);
         (void)hipStreamDestroy(ctx->aux[a]);
@@ -1359,14 +1366,14 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
     // are enough: the streamed bodies on the context stream, then the bundle
     // launches, largest LDS first.  With more groups than streams a group goes to
     // the stream with the least estimated work queued.
-    constexpr int kAux = xpbd_context::kAux;
+    const int num_aux = ctx->num_aux;
     HIP_TRY(hipEventRecord(ctx->fork, ctx->stream));
-    for (int a = 0; a < kAux; ++a)
+    for (int a = 0; a < num_aux; ++a)
         HIP_TRY(hipStreamWaitEvent(ctx->aux[a], ctx->fork, 0));
     const float gx = ctx->gravity[0], gy = ctx->gravity[1], gz = ctx->gravity[2];
-    hipStream_t streams[kAux + 1] = { ctx->stream };
-    for (int a = 0; a < kAux; ++a) streams[a + 1] = ctx->aux[a];
-    double queued[kAux + 1] = {};
+    hipStream_t streams[xpbd_context::kMaxAux + 1] = { ctx->stream };
+    for (int a = 0; a < num_aux; ++a) streams[a + 1] = ctx->aux[a];
+    double queued[xpbd_context::kMaxAux + 1] = {};
     uint32_t offset = 0;
     for (const Launch &l : launches)
     {
@@ -1382,14 +1389,14 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
     {
         const auto &L = *Lp;
         int best = 0;
-        for (int q = 1; q <= kAux; ++q)
+        for (int q = 1; q <= num_aux; ++q)
             if (queued[q] < queued[best])
                 best = q;
         queued[best] += std::max(L.cost, L.work / 256.0);
         HIP_TRY(launch_bundles(L.staged, L.block, L.uniform, L.table.p, L.count, ctx->d_bodies.p, ctx->d_states.p, ctx->d_colliders.p, nc,
                                L.lds, gx, gy, gz, dt, collision_steps, frames, streams[best]));
     }
-    for (int a = 0; a < kAux; ++a)
+    for (int a = 0; a < num_aux; ++a)
     {
         HIP_TRY(hipEventRecord(ctx->join[a], ctx->aux[a]));
         HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->join[a], 0));
