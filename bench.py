@@ -249,8 +249,8 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": None if traffic is None else traffic * args.steps,
                 "traffic_source": traffic_src,
-                "kernel": "one xpbd_update: bundle_kernel launches (one per LDS bin) + update_kernel (streamed bodies), "
-                          "concurrent on four streams, timed fork to join",
+                "kernel": "one xpbd_update: bundle_kernel launches (one per launch class) + update_kernel (streamed "
+                          "bodies), concurrent on one stream per hardware queue, timed fork to join",
                 "algorithmic_bytes_per_launch": bytes_frame * args.steps,
                 "launch_ms": round(kernel_ms / max(n_launch, 1), 3)}
     ctx.close()
