@@ -46,6 +46,11 @@ extern "C" {
 #define XPBD_SHAPE_SPHERE 1     /* params {radius}                     SphereShape */
 #define XPBD_SHAPE_BOX 2        /* params {hx, hy, hz}                 BoxShape */
 #define XPBD_SHAPE_CAPSULE 3    /* params {halfHeightOfCylinder, radius} CapsuleShape */
+#define XPBD_SHAPE_CYLINDER 4   /* params {halfHeight, radius}         CylinderShape */
+/* TaperedCapsuleShape: params {topCenter, bottomCenter, topRadius, bottomRadius, sinAlpha,
+ * tanAlpha} as the shape holds them (TaperedCapsuleShape.cpp:86-131: the centres are relative
+ * to the shape's centre of mass; tanAlpha = Tan(ASin(sinAlpha)) with Jolt's approximations) */
+#define XPBD_SHAPE_TAPERED_CAPSULE 5
 
 typedef struct xpbd_context xpbd_context;
 typedef struct xpbd_settings xpbd_settings;
@@ -179,8 +184,9 @@ int xpbd_body_create(xpbd_context *ctx, const xpbd_body_desc *desc, int32_t *out
  */
 typedef struct xpbd_collider_desc {
     int32_t type;
-    float params[5];
-    float position[3];
+    float params[8];
+    float position[3];          /* centre of mass (Body::GetCenterOfMassPosition()); the body
+                                   position for every shape but the tapered capsule */
     float rotation[4];
     float friction;
     float restitution;

@@ -18,9 +18,10 @@ struct Leaf
 {
     M44 t;          // leaf transform (soft-body local)
     M44 inv;        // InversedRotationTranslation of t (box, capsule)
-    float pn[3];    // plane shape: transformed plane normal
+    float pn[3];    // plane shape: transformed plane normal | tapered capsule {bottom radius, sin, tan alpha}
     float pc;       //              and constant
-    float a, b, c;  // sphere {radius} | box half extent | capsule {half height, radius}
+    float a, b, c;  // sphere {radius} | box half extent | capsule, cylinder {half height, radius} |
+                    // tapered capsule {top centre, bottom centre, top radius}
     int32_t type;
     float friction, restitution;
 };
@@ -334,6 +335,13 @@ __device__ __forceinline__ void run_levels_lds(const uint4 *recs, const uint32_t
 // argmax rule of CollideSoftBodyVertexIterator::UpdatePenetration
 // (CollideSoftBodyVertexIterator.h:83-97): a leaf replaces the stored plane
 // unless the stored penetration is >= the new one.
+// Vec3::NormalizedOr (Vec3.inl:978-1000): zero_value when |v|^2 <= FLT_MIN
+__device__ __forceinline__ V3 normalized_or(V3 v, V3 zero_value)
+{
+    const float l2 = len_sq(v);
+    return l2 <= FLT_MIN ? zero_value : v / sqrtf(l2);
+}
+
 __device__ __forceinline__ void collide_vertex(const Leaf *leaves, uint32_t num_leaves, V3 x, float &largest, int32_t &idx, Pl &plane)
 {
     for (uint32_t l = 0; l < num_leaves; ++l)
@@ -400,7 +408,7 @@ __device__ __forceinline__ void collide_vertex(const Leaf *leaves, uint32_t num_
                 }
             }
             break;
-        default: // CapsuleShape.cpp:324-375
+        case 3: // CapsuleShape.cpp:324-375
             {
                 float hh = L.a, radius = L.b;
                 V3 local = mul_point(L.inv, x);
@@ -432,6 +440,77 @@ __device__ __forceinline__ void collide_vertex(const Leaf *leaves, uint32_t num_
                         plane = plane_transformed(plane_from_point_normal(point, normal), L.t);
                         idx = (int32_t)l;
                     }
+                }
+            }
+            break;
+        case 4: // CylinderShape.cpp:317-366 (scale 1)
+            {
+                const float half_height = L.a, radius = L.b;
+                const V3 local = mul_point(L.inv, x);
+                const V3 side_normal = v3(local.x, 0.0f, local.z);
+                const float side_normal_length = len(side_normal);
+                const float side_penetration = radius - side_normal_length;
+                const float top_penetration = half_height - fabsf(local.y);
+                V3 point, normal;
+                if (side_penetration < 0.0f && top_penetration < 0.0f)
+                {
+                    point = side_normal * (radius / side_normal_length) + v3(0.0f, half_height * sign_of(local.y), 0.0f);
+                    normal = normalized_or(local - point, v3(0.0f, 1.0f, 0.0f));
+                }
+                else if (side_penetration < top_penetration)
+                {
+                    normal = side_normal_length > 0.0f ? side_normal / side_normal_length : v3(1.0f, 0.0f, 0.0f);
+                    point = radius * normal;
+                }
+                else
+                {
+                    normal = v3(0.0f, sign_of(local.y), 0.0f);
+                    point = half_height * normal;
+                }
+                const Pl p = plane_from_point_normal(point, normal);
+                const float pen = -signed_distance(p, local);
+                if (!(largest >= pen))
+                {
+                    largest = pen;
+                    plane = plane_transformed(p, L.t);
+                    idx = (int32_t)l;
+                }
+            }
+            break;
+        case 5: // TaperedCapsuleShape.cpp:304-371 (scale 1: the y flip and scale factors are 1)
+            {
+                const V3 top_center = v3(0.0f, L.a, 0.0f), bottom_center = v3(0.0f, L.b, 0.0f);
+                const float top_radius = L.c, bottom_radius = L.pn[0], sin_alpha = L.pn[1], tan_alpha = L.pn[2];
+                const V3 local = mul_point(L.inv, x);
+                V3 position, normal;
+                const V3 to_top = local - top_center;
+                const float to_top_len = len(to_top);
+                if (to_top.y >= sin_alpha * to_top_len)
+                {
+                    normal = to_top_len != 0.0f ? to_top / to_top_len : v3(0.0f, 1.0f, 0.0f);
+                    position = top_center + top_radius * normal;
+                }
+                else
+                {
+                    const V3 to_bottom = local - bottom_center;
+                    const float to_bottom_len = len(to_bottom);
+                    if (to_bottom.y <= sin_alpha * to_bottom_len)
+                        normal = to_bottom_len != 0.0f ? to_bottom / to_bottom_len : neg(v3(0.0f, 1.0f, 0.0f));
+                    else
+                    {
+                        normal = normalized_or(v3(local.x, 0.0f, local.z), v3(1.0f, 0.0f, 0.0f));
+                        normal.y = tan_alpha;
+                        normal = normalized_or(normal, v3(1.0f, 0.0f, 0.0f));
+                    }
+                    position = bottom_center + bottom_radius * normal;
+                }
+                const Pl p = plane_from_point_normal(position, normal);
+                const float pen = -signed_distance(p, local);
+                if (!(largest >= pen))
+                {
+                    largest = pen;
+                    plane = plane_transformed(p, L.t);
+                    idx = (int32_t)l;
                 }
             }
             break;
@@ -473,9 +552,15 @@ __device__ __forceinline__ void make_leaf(const ColliderDev &c, const M44 &inv_c
         out.inv = inversed_rotation_translation(t);
         out.a = 1.0f * c.params[0]; out.b = 1.0f * c.params[1]; out.c = 1.0f * c.params[2];
         break;
-    default:
+    case 3:
+    case 4:
         out.inv = inversed_rotation_translation(t);
         out.a = 1.0f * c.params[0]; out.b = 1.0f * c.params[1];
+        break;
+    default:
+        out.inv = inversed_rotation_translation(t);
+        out.a = c.params[0]; out.b = c.params[1]; out.c = c.params[2];
+        out.pn[0] = c.params[3]; out.pn[1] = c.params[4]; out.pn[2] = c.params[5];
         break;
     }
 }

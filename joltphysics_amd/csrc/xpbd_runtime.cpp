@@ -1230,8 +1230,9 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
     std::vector<ColliderDev> v(n);
     for (uint32_t i = 0; i < n; ++i)
     {
-        if (c[i].type < XPBD_SHAPE_PLANE || c[i].type > XPBD_SHAPE_CAPSULE)
-            return Fail(XPBD_E_UNSUPPORTED, "collider shape type not supported (plane, sphere, box, capsule only)");
+        if (c[i].type < XPBD_SHAPE_PLANE || c[i].type > XPBD_SHAPE_TAPERED_CAPSULE)
+            return Fail(XPBD_E_UNSUPPORTED, "collider shape type not supported (plane, sphere, box, capsule, cylinder, "
+                                            "tapered capsule only)");
         v[i].type = c[i].type;
         memcpy(v[i].params, c[i].params, sizeof(v[i].params));
         memcpy(v[i].pos, c[i].position, sizeof(v[i].pos));

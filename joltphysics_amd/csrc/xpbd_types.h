@@ -76,7 +76,7 @@ struct BodyStateDev
 struct ColliderDev
 {
     int32_t type;
-    float params[5];
+    float params[8];
     float pos[3];
     float rot[4];
     float friction, restitution;

@@ -41,7 +41,7 @@ BEND_NONE, BEND_DISTANCE, BEND_DIHEDRAL = 0, 1, 2
 LRA_NONE, LRA_EUCLIDEAN, LRA_GEODESIC = 0, 1, 2
 
 # Collider shape types of the C ABI
-SHAPE_PLANE, SHAPE_SPHERE, SHAPE_BOX, SHAPE_CAPSULE = 0, 1, 2, 3
+SHAPE_PLANE, SHAPE_SPHERE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_CYLINDER, SHAPE_TAPERED_CAPSULE = 0, 1, 2, 3, 4, 5
 
 
 def _empty(dtype):
@@ -98,7 +98,16 @@ class BodyParams:
 
 @dataclass
 class Collider:
-    """A static collider body: shape, world transform, material, world bounds."""
+    """A static collider body: shape, world transform (body position and rotation),
+    material, world bounds.
+
+    `params` are the shape's construction parameters (plane {nx, ny, nz, c, half extent},
+    sphere {r}, box {hx, hy, hz}, capsule / cylinder {half height, r}, tapered capsule
+    {half height of the tapered cylinder, top radius, bottom radius}).  `device_params`
+    and `com` are what the soft-body update consumes (shape parameters as the shape holds
+    them, centre-of-mass position); when unset they are derived here, for the tapered
+    capsule approximately (the reference derives them with its own sin / tan; the tests
+    take them from the reference)."""
 
     type: int
     params: tuple
@@ -107,8 +116,50 @@ class Collider:
     friction: float = 0.2
     restitution: float = 0.0
     bounds: tuple | None = None  # world-space AABB (min xyz, max xyz)
+    device_params: tuple | None = None
+    com: tuple | None = None
 
     def param_array(self):
-        p = np.zeros(5, dtype=np.float32)
+        p = np.zeros(8, dtype=np.float32)
         p[:len(self.params)] = self.params
         return p
+
+    def device_param_array(self):
+        if self.device_params is not None:
+            p = np.zeros(8, dtype=np.float32)
+            p[:len(self.device_params)] = self.device_params
+            return p
+        if self.type == SHAPE_TAPERED_CAPSULE:
+            return tapered_capsule_device_params(*self.params[:3])
+        return self.param_array()
+
+    def com_position(self):
+        if self.com is not None:
+            return tuple(self.com)
+        if self.type == SHAPE_TAPERED_CAPSULE:
+            p = tapered_capsule_device_params(*self.params[:3])
+            off = np.array([0.0, self.params[0] - p[0], 0.0])      # centre of mass in the shape's frame
+            return tuple(np.array(self.pos, np.float64) + _rotate(self.rot, off))
+        return tuple(self.pos)
+
+
+def tapered_capsule_device_params(half_height, top_radius, bottom_radius):
+    """TaperedCapsuleShape's derived parameters {top centre, bottom centre, top radius,
+    bottom radius, sin alpha, tan alpha} (TaperedCapsuleShape.cpp:117-131), in float32
+    with numpy's tan / arcsin (the reference uses its own approximations)."""
+    f = np.float32
+    h, t, b = f(half_height), f(top_radius), f(bottom_radius)
+    tc = f(h + f(f(0.5) * f(b - t)))
+    bc = f(f(-h) + f(f(0.5) * f(b - t)))
+    sa = f(f(b - t) / f(tc - bc))
+    ta = f(np.tan(np.arcsin(np.float64(sa))))
+    p = np.zeros(8, dtype=np.float32)
+    p[:6] = (tc, bc, t, b, sa, ta)
+    return p
+
+
+def _rotate(q, v):
+    x, y, z, w = (float(c) for c in q)
+    u = np.array([x, y, z])
+    v = np.asarray(v, np.float64)
+    return 2.0 * np.dot(u, v) * u + (w * w - np.dot(u, u)) * v + 2.0 * w * np.cross(u, v)

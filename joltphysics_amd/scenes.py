@@ -17,8 +17,9 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .records import (BEND_DIHEDRAL, BEND_NONE, EDGE, LRA, LRA_NONE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_PLANE,
-                      SHAPE_SPHERE, VERTEX, VOLUME, BodyParams, Collider)
+from .records import (BEND_DIHEDRAL, BEND_NONE, EDGE, LRA, LRA_NONE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_CYLINDER,
+                      SHAPE_PLANE, SHAPE_SPHERE, SHAPE_TAPERED_CAPSULE, VERTEX, VOLUME, BodyParams, Collider,
+                      tapered_capsule_device_params)
 
 # Build flags (same bit meaning on the product and oracle builders)
 CREATE_CONSTRAINTS = 1
@@ -199,6 +200,16 @@ def collider_world_bounds(c: Collider):
     elif c.type == SHAPE_CAPSULE:
         hh, r = c.params[0], c.params[1]
         he = np.abs(R) @ np.array([r, hh + r, r]) + 0.0
+    elif c.type == SHAPE_CYLINDER:
+        hh, r = c.params[0], c.params[1]
+        he = np.abs(R) @ np.array([r, hh, r])
+    elif c.type == SHAPE_TAPERED_CAPSULE:
+        # conservative: a box around both end spheres, centred on the centre of mass
+        d = tapered_capsule_device_params(*c.params[:3])
+        r = max(float(d[2]), float(d[3]))
+        ext = max(abs(float(d[0])) + float(d[2]), abs(float(d[1])) + float(d[3]))
+        he = np.abs(R) @ np.array([r, ext, r])
+        p = np.array(c.com_position(), np.float64)
     else:  # plane: square of half extent around the plane point, extruded half extent behind
         n = np.array(c.params[:3], np.float64)
         he = np.full(3, 2.0 * c.params[4])

@@ -51,7 +51,7 @@ class _BodyDesc(C.Structure):
 
 
 class _ColliderDesc(C.Structure):
-    _fields_ = [("type", C.c_int32), ("params", C.c_float * 5), ("position", C.c_float * 3),
+    _fields_ = [("type", C.c_int32), ("params", C.c_float * 8), ("position", C.c_float * 3),
                 ("rotation", C.c_float * 4), ("friction", C.c_float), ("restitution", C.c_float),
                 ("world_bounds", C.c_float * 6)]
 
@@ -244,11 +244,12 @@ class Context:
         arr = (_ColliderDesc * max(n, 1))()
         for i, c in enumerate(colliders):
             arr[i].type = int(c.type)
-            p = c.param_array()
-            for k in range(5):
+            p = c.device_param_array()
+            for k in range(8):
                 arr[i].params[k] = float(p[k])
+            com = c.com_position()
             for k in range(3):
-                arr[i].position[k] = float(c.pos[k])
+                arr[i].position[k] = float(com[k])
             for k in range(4):
                 arr[i].rotation[k] = float(c.rot[k])
             arr[i].friction = float(c.friction)

@@ -39,6 +39,8 @@
 #include <Jolt/Physics/Collision/Shape/SphereShape.h>
 #include <Jolt/Physics/Collision/Shape/CapsuleShape.h>
 #include <Jolt/Physics/Collision/Shape/PlaneShape.h>
+#include <Jolt/Physics/Collision/Shape/CylinderShape.h>
+#include <Jolt/Physics/Collision/Shape/TaperedCapsuleShape.h>
 #include <Jolt/Physics/SoftBody/SoftBodySharedSettings.h>
 #include <Jolt/Physics/SoftBody/SoftBodyCreationSettings.h>
 #include <Jolt/Physics/SoftBody/SoftBodyMotionProperties.h>
@@ -50,6 +52,7 @@
 #include <sstream>
 #include <thread>
 #include <vector>
+#include <array>
 
 using namespace JPH;
 
@@ -119,6 +122,8 @@ struct World
 	std::unique_ptr<JobSystemThreadPool> mJobs;
 	std::vector<BodyID> mSoftBodies;
 	std::vector<BodyID> mColliders;
+	std::vector<std::array<float, 8>> mColliderParams;	// shape parameters as given to jo_world_add_collider
+	std::vector<int> mColliderTypes;
 };
 
 template <class T>
@@ -311,7 +316,9 @@ void jo_world_set_gravity(void *wp, float gx, float gy, float gz)
 }
 
 // Static collider. type 0 plane {nx,ny,nz,c, halfExtent}, 1 sphere {r},
-// 2 box {hx,hy,hz}, 3 capsule {halfHeight, r}.  pos[3], rot[4] (x,y,z,w).
+// 2 box {hx,hy,hz}, 3 capsule {halfHeight, r}, 4 cylinder {halfHeight, r},
+// 5 tapered capsule {halfHeightOfTaperedCylinder, topRadius, bottomRadius}.
+// pos[3], rot[4] (x,y,z,w): the body's position and rotation.
 int jo_world_add_collider(void *wp, int type, const float *params, const float *pos, const float *rot, float friction, float restitution)
 {
 	World *w = (World *)wp;
@@ -322,6 +329,16 @@ int jo_world_add_collider(void *wp, int type, const float *params, const float *
 	case 1: shape = new SphereShape(params[0]); break;
 	case 2: shape = new BoxShape(Vec3(params[0], params[1], params[2]), 0.0f); break;
 	case 3: shape = new CapsuleShape(params[0], params[1]); break;
+	case 4: shape = new CylinderShape(params[0], params[1], 0.0f); break;
+	case 5:
+		{
+			TaperedCapsuleShapeSettings ts(params[0], params[1], params[2]);
+			Shape::ShapeResult r = ts.Create();
+			if (r.HasError() || r.Get()->GetSubType() != EShapeSubType::TaperedCapsule)
+				return -1;
+			shape = r.Get();
+		}
+		break;
 	default: return -1;
 	}
 	BodyCreationSettings bcs(shape, RVec3(pos[0], pos[1], pos[2]), Quat(rot[0], rot[1], rot[2], rot[3]), EMotionType::Static, Layers::STATIC);
@@ -331,7 +348,39 @@ int jo_world_add_collider(void *wp, int type, const float *params, const float *
 	if (id.IsInvalid())
 		return -1;
 	w->mColliders.push_back(id);
+	std::array<float, 8> kept{};
+	const int np = type == 0 ? 5 : type == 1 ? 1 : type == 2 ? 3 : type == 5 ? 3 : 2;
+	for (int k = 0; k < np; ++k) kept[k] = params[k];
+	w->mColliderParams.push_back(kept);
+	w->mColliderTypes.push_back(type);
 	return int(w->mColliders.size() - 1);
+}
+
+// The collider as the soft-body update consumes it: the body's centre-of-mass
+// position and the shape's parameters as the shape holds them.  For the tapered
+// capsule these are the constructor's (TaperedCapsuleShape.cpp:117-131): top and
+// bottom centre relative to the centre of mass, the radii, sin and tan of the
+// cone angle (Jolt's Tan and ASin).
+int jo_world_collider_device(void *wp, int collider, float *params8, float *com3)
+{
+	World *w = (World *)wp;
+	BodyLockRead lock(w->mSystem.GetBodyLockInterface(), w->mColliders[collider]);
+	if (!lock.Succeeded())
+		return -1;
+	RVec3 com = lock.GetBody().GetCenterOfMassPosition();
+	com3[0] = float(com.GetX()); com3[1] = float(com.GetY()); com3[2] = float(com.GetZ());
+	const std::array<float, 8> &p = w->mColliderParams[collider];
+	for (int k = 0; k < 8; ++k) params8[k] = p[k];
+	if (w->mColliderTypes[collider] == 5)
+	{
+		const float half_height = p[0], top_radius = p[1], bottom_radius = p[2];
+		const float top_center = half_height + 0.5f * (bottom_radius - top_radius);
+		const float bottom_center = -half_height + 0.5f * (bottom_radius - top_radius);
+		const float sin_alpha = (bottom_radius - top_radius) / (top_center - bottom_center);
+		params8[0] = top_center; params8[1] = bottom_center; params8[2] = top_radius; params8[3] = bottom_radius;
+		params8[4] = sin_alpha; params8[5] = Tan(ASin(sin_alpha)); params8[6] = 0.0f; params8[7] = 0.0f;
+	}
+	return 0;
 }
 
 // Soft body.  p: {numIterations, linearDamping, maxLinearVelocity, restitution,

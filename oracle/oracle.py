@@ -85,6 +85,8 @@ class JoltReference:
             lib.jo_world_add_collider.argtypes = [_vp, _i, _vp, _vp, _vp, _f, _f]
             lib.jo_world_collider_bounds.restype = _i
             lib.jo_world_collider_bounds.argtypes = [_vp, _i, _vp]
+            lib.jo_world_collider_device.restype = _i
+            lib.jo_world_collider_device.argtypes = [_vp, _i, _vp, _vp]
             lib.jo_world_add_soft_body.restype = _i
             lib.jo_world_add_soft_body.argtypes = [_vp, _vp, _vp, _vp, _vp]
             lib.jo_world_optimize_broadphase.argtypes = [_vp]
@@ -181,6 +183,13 @@ class JoltWorld:
         self.lib.jo_world_collider_bounds(self.h, i, _ptr(out))
         return tuple(float(x) for x in out)
 
+    def collider_device(self, i):
+        """(shape parameters as the shape holds them [8], centre-of-mass position [3])."""
+        params = np.zeros(8, np.float32)
+        com = np.zeros(3, np.float32)
+        self.lib.jo_world_collider_device(self.h, i, _ptr(params), _ptr(com))
+        return tuple(float(x) for x in params), tuple(float(x) for x in com)
+
     def add_soft_body(self, settings: SettingsArrays, position, rotation, params):
         sh = self.ref.settings_handle(settings)
         self._settings.append(sh)
@@ -264,8 +273,8 @@ class Restatement:
         self.close()
 
     def add_collider(self, c, bounds=None):
-        p = c.param_array()
-        pos = np.array(c.pos, np.float32)
+        p = c.device_param_array()
+        pos = np.array(c.com_position(), np.float32)
         rot = np.array(c.rot, np.float32)
         b = np.array(bounds if bounds is not None else c.bounds, np.float32)
         return self.lib.xr_world_add_collider(self.h, int(c.type), _ptr(p), _ptr(pos), _ptr(rot),

@@ -73,6 +73,12 @@ template <> struct Constants<double> { static constexpr double kPi = 3.141592653
 
 // Sign(0) = +1 (Math/Math.h:116-119)
 template <class F> inline F SignOf(F v) { return v < F(0) ? F(-1) : F(1); }
+// Vec3::NormalizedOr (Vec3.inl:978-1000): zero_value when |v|^2 <= FLT_MIN
+template <class F> inline V3<F> NormalizedOr(V3<F> v, V3<F> zero_value)
+{
+	const F l2 = LenSq(v);
+	return l2 <= F(FLT_MIN) ? zero_value : v / std::sqrt(l2);
+}
 
 // ACosApproximate (Math/Trigonometry.h:48-65)
 template <class F> inline F ACosApprox(F x)
@@ -236,8 +242,8 @@ struct GroupRec { u32 edge_end, lra_end, rod_ss_end, rod_bt_end, bend_end, volum
 // ------------------------------------------------------------------ world
 struct Collider
 {
-	int type;				// 0 plane, 1 sphere, 2 box, 3 capsule
-	float params[5];
+	int type;				// 0 plane, 1 sphere, 2 box, 3 capsule, 4 cylinder, 5 tapered capsule
+	float params[8];		// as the shape holds them (tapered capsule: top / bottom centre, radii, sin, tan)
 	float pos[3], rot[4];
 	float friction, restitution;
 	float bounds[6];		// world space bounds of the static body (min xyz, max xyz)
@@ -422,6 +428,87 @@ template <class F> void CollideVertices(Body<F> &b, const Leaf<F> &leaf, int sha
 							b.plane[i] = PlaneTransformed(PlaneFromPointNormal(point, normal), T);
 							b.shape_idx[i] = shape_index;
 						}
+					}
+				}
+		}
+		break;
+	case 4: // CylinderShape.cpp:317-366 (scale 1)
+		{
+			M44<F> inv = InvRT(T);
+			F half_height = std::abs(F(1)) * F(c.params[0]);
+			F radius = std::abs(F(1)) * F(c.params[1]);
+			for (size_t i = 0; i < nv; ++i)
+				if (b.w[i] > F(0))
+				{
+					V3<F> local = MulPoint(inv, b.x[i]);
+					V3<F> side_normal{ local.x, 0, local.z };
+					F side_normal_length = Len(side_normal);
+					F side_penetration = radius - side_normal_length;
+					F top_penetration = half_height - std::abs(local.y);
+					V3<F> point, normal;
+					if (side_penetration < F(0) && top_penetration < F(0))
+					{
+						point = side_normal * (radius / side_normal_length) + V3<F>{ 0, half_height * SignOf(local.y), 0 };
+						normal = NormalizedOr(local - point, V3<F>{ 0, 1, 0 });
+					}
+					else if (side_penetration < top_penetration)
+					{
+						normal = side_normal_length > F(0) ? side_normal / side_normal_length : V3<F>{ 1, 0, 0 };
+						point = radius * normal;
+					}
+					else
+					{
+						normal = V3<F>{ 0, SignOf(local.y), 0 };
+						point = half_height * normal;
+					}
+					PlaneT<F> plane = PlaneFromPointNormal(point, normal);
+					F pen = -SignedDistance(plane, local);
+					if (update(i, pen))
+					{
+						b.plane[i] = PlaneTransformed(plane, T);
+						b.shape_idx[i] = shape_index;
+					}
+				}
+		}
+		break;
+	case 5: // TaperedCapsuleShape.cpp:304-371 (scale 1)
+		{
+			M44<F> inv = InvRT(T);
+			V3<F> top_center{ 0, F(c.params[0]), 0 }, bottom_center{ 0, F(c.params[1]), 0 };
+			F top_radius = F(c.params[2]), bottom_radius = F(c.params[3]);
+			F sin_alpha = F(c.params[4]), tan_alpha = F(c.params[5]);
+			for (size_t i = 0; i < nv; ++i)
+				if (b.w[i] > F(0))
+				{
+					V3<F> local = MulPoint(inv, b.x[i]);
+					V3<F> position, normal;
+					V3<F> to_top = local - top_center;
+					F to_top_len = Len(to_top);
+					if (to_top.y >= sin_alpha * to_top_len)
+					{
+						normal = to_top_len != F(0) ? to_top / to_top_len : V3<F>{ 0, 1, 0 };
+						position = top_center + top_radius * normal;
+					}
+					else
+					{
+						V3<F> to_bottom = local - bottom_center;
+						F to_bottom_len = Len(to_bottom);
+						if (to_bottom.y <= sin_alpha * to_bottom_len)
+							normal = to_bottom_len != F(0) ? to_bottom / to_bottom_len : V3<F>{ F(0) - F(0), F(0) - F(1), F(0) - F(0) };
+						else
+						{
+							normal = NormalizedOr(V3<F>{ local.x, 0, local.z }, V3<F>{ 1, 0, 0 });
+							normal.y = tan_alpha;
+							normal = NormalizedOr(normal, V3<F>{ 1, 0, 0 });
+						}
+						position = bottom_center + bottom_radius * normal;
+					}
+					PlaneT<F> plane = PlaneFromPointNormal(position, normal);
+					F pen = -SignedDistance(plane, local);
+					if (update(i, pen))
+					{
+						b.plane[i] = PlaneTransformed(plane, T);
+						b.shape_idx[i] = shape_index;
 					}
 				}
 		}
@@ -881,16 +968,18 @@ void xr_world_set_gravity(void *wp, float gx, float gy, float gz)
 	w->d.gravity = { gx, gy, gz };
 }
 
-// type 0 plane {nx,ny,nz,c,halfExtent}, 1 sphere {r}, 2 box {hx,hy,hz}, 3 capsule {halfHeight, r};
+// type 0 plane {nx,ny,nz,c,halfExtent}, 1 sphere {r}, 2 box {hx,hy,hz}, 3 capsule {halfHeight, r},
+// 4 cylinder {halfHeight, r}, 5 tapered capsule {topCenter, bottomCenter, topRadius, bottomRadius,
+// sinAlpha, tanAlpha} (as the shape holds them); pos: the centre of mass;
 // bounds: the static body's world-space bounds (min xyz, max xyz).
 int xr_world_add_collider(void *wp, int type, const float *params, const float *pos, const float *rot, float friction, float restitution, const float *bounds)
 {
 	XrWorld *w = (XrWorld *)wp;
-	if (type < 0 || type > 3)
+	if (type < 0 || type > 5)
 		return -1;
 	Collider c{};
 	c.type = type;
-	int np = type == 0 ? 5 : (type == 1 ? 1 : (type == 2 ? 3 : 2));
+	int np = type == 0 ? 5 : type == 1 ? 1 : type == 2 ? 3 : type == 5 ? 6 : 2;
 	for (int i = 0; i < np; ++i) c.params[i] = params[i];
 	for (int i = 0; i < 3; ++i) c.pos[i] = pos[i];
 	for (int i = 0; i < 4; ++i) c.rot[i] = rot[i];

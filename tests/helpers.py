@@ -85,6 +85,9 @@ def reference_run(scene, frames_to_keep, variant="nofma", num_threads=0):
     world = JoltWorld(ref, scene.gravity, num_threads=num_threads)
     ids = [world.add_collider(c) for c in scene.colliders]
     bounds = np.array([world.collider_bounds(i) for i in ids], np.float32).reshape(-1, 6)
+    # the colliders as the update consumes them, as the reference derived them
+    for c, i in zip(scene.colliders, ids):
+        c.device_params, c.com = world.collider_device(i)
     for b, s in zip(scene.bodies, settings):
         world.add_soft_body(s, b.position, b.rotation, b.params)
     world.optimize_broadphase()

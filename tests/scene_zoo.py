@@ -8,7 +8,8 @@ import numpy as np
 
 from joltphysics_amd import scenes
 from joltphysics_amd.records import (BEND_DIHEDRAL, BEND_DISTANCE, LRA_GEODESIC, SHAPE_BOX, SHAPE_CAPSULE,
-                                     SHAPE_PLANE, SHAPE_SPHERE, BodyParams, Collider)
+                                     SHAPE_CYLINDER, SHAPE_PLANE, SHAPE_SPHERE, SHAPE_TAPERED_CAPSULE, BodyParams,
+                                     Collider)
 from joltphysics_amd.scenes import (CALC_EDGE_LENGTHS, CALC_VOLUMES, CREATE_CONSTRAINTS, OPTIMIZE, BodySpec,
                                     MeshSpec, Scene)
 
@@ -112,6 +113,21 @@ def block_lra_boxes_streamed(frames=1):
     return block_lra_boxes(cells=21, num_boxes=128, frames=frames)
 
 
+def cloth_cylinder_tapered(n=40, frames=40):
+    """n x n cloth draping over an upright cylinder (top cap, rim, side) and a tapered
+    capsule lying on its side (both end spheres and the cone); no ground (the cloth
+    squeezed between these shapes and a ground plane diverges in the reference too)."""
+    v, f = scenes.cloth(n, 0.05, 60.0, jitter=1e-4, seed=12)
+    mesh = MeshSpec(vertices=v, faces=f, attrs=scenes.attrs(1e-6, 1e-6, 1e-4), bend_type=BEND_DIHEDRAL)
+    body = BodySpec(mesh=mesh, position=(0.0, 0.0, 0.6), params=BodyParams(num_iterations=5, allow_sleeping=False))
+    cols = _bounded([Collider(SHAPE_CYLINDER, (0.25, 0.3), pos=(-0.45, 0.1, 0.25), rot=_quat((1, 0.1, 0), 1.5708),
+                              friction=0.4),
+                     Collider(SHAPE_TAPERED_CAPSULE, (0.35, 0.22, 0.1), pos=(0.45, -0.1, 0.3),
+                              rot=_quat((0.1, 0, 1), 1.5708), friction=0.6, restitution=0.2)])
+    return Scene("cloth_cylinder_tapered", [body], cols, gravity=(0.0, 0.0, -9.81), dt=1 / 60, collision_steps=2,
+                 frames=frames)
+
+
 ALL = {
     "cloth100_sphere": big_cloth_on_sphere,
     "tetcube_plane_box": tet_cube_on_plane,
@@ -120,6 +136,7 @@ ALL = {
     "sleeping_cloth": sleeping_cloth,
     "batched": batched,
     "cloth_cube_spheres": cloth_cube_spheres,
+    "cloth_cylinder_tapered": cloth_cylinder_tapered,
     "block_lra_boxes": block_lra_boxes,
     "block_lra_boxes_streamed": block_lra_boxes_streamed,
 }

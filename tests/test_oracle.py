@@ -20,8 +20,9 @@ import pytest
 import scene_zoo
 from helpers import ROOT_DIR, float_equal, load_config1_fixture, max_abs, reference_run, run_restatement
 from joltphysics_amd import scenes
-from joltphysics_amd.records import (EDGE, GROUP, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_PLANE, SHAPE_SPHERE, VERTEX,
-                                     VOLUME, BodyParams, Collider, SettingsArrays)
+from joltphysics_amd.records import (EDGE, GROUP, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_CYLINDER, SHAPE_PLANE, SHAPE_SPHERE,
+                                     SHAPE_TAPERED_CAPSULE, VERTEX, VOLUME, BodyParams, Collider, SettingsArrays,
+                                     tapered_capsule_device_params)
 
 
 @pytest.fixture(scope="session", autouse=True)
@@ -265,3 +266,57 @@ def test_box_shape_corner_region():
 def test_capsule_shape_closest_feature(point, normal, const):
     p = _probe_plane(Collider(SHAPE_CAPSULE, (0.5, 0.3)), point)
     np.testing.assert_allclose(p, list(normal) + [const], atol=1e-7)
+
+
+@pytest.mark.parametrize("point,normal,const", [
+    ((0.05, 0.45, 0.0), (0, 1, 0), -0.5),               # inside, top cap nearest
+    ((0.05, -0.45, 0.0), (0, -1, 0), -0.5),             # inside, bottom cap nearest
+    ((0.2, 0.1, 0.0), (1, 0, 0), -0.3),                 # inside, side nearer than the cap
+    ((0.0, 0.1, 0.28), (0, 0, 1), -0.3),                # inside, side nearest
+    ((0.9, 0.2, 0.0), (1, 0, 0), -0.3),                 # outside the side, within the height
+    ((0.1, 1.2, 0.0), (0, 1, 0), -0.5),                 # above the top cap, inside the radius
+])
+def test_cylinder_shape_closest_feature(point, normal, const):
+    p = _probe_plane(Collider(SHAPE_CYLINDER, (0.5, 0.3)), point)
+    np.testing.assert_allclose(p, list(normal) + [const], atol=1e-7)
+
+
+def test_cylinder_shape_rim_region():
+    # outside both the radius and the height: closest point on the rim (0.3, 0.5, 0)
+    p = _probe_plane(Collider(SHAPE_CYLINDER, (0.5, 0.3)), (1.3, 1.5, 0.0))
+    n = np.array([1.0, 1.0, 0.0]) / math.sqrt(2)
+    np.testing.assert_allclose(p[:3], n, atol=1e-7)
+    assert p[3] == pytest.approx(-np.dot(n, [0.3, 0.5, 0.0]), abs=1e-7)
+
+
+def _tapered(hh=0.4, top=0.3, bottom=0.1):
+    c = Collider(SHAPE_TAPERED_CAPSULE, (hh, top, bottom))
+    c.com = (0.0, 0.0, 0.0)             # probe in the shape's centre-of-mass frame
+    return c
+
+
+def test_tapered_capsule_top_and_bottom_spheres():
+    d = tapered_capsule_device_params(0.4, 0.3, 0.1)
+    top_c, bottom_c, top_r, bottom_r = (float(x) for x in d[:4])
+    # straight above the top sphere centre: normal +y, plane through the sphere top
+    p = _probe_plane(_tapered(), (0.0, top_c + 1.0, 0.0))
+    np.testing.assert_allclose(p, [0, 1, 0, -(top_c + top_r)], atol=1e-6)
+    # straight below the bottom sphere centre: normal -y
+    p = _probe_plane(_tapered(), (0.0, bottom_c - 1.0, 0.0))
+    np.testing.assert_allclose(p, [0, -1, 0, bottom_c - bottom_r], atol=1e-6)
+
+
+def test_tapered_capsule_cone_normal():
+    # beside the cone at mid height: the normal is the cone's outward normal
+    # (x, tan(alpha), 0) normalised, with sin(alpha) = (r_bottom - r_top) / (c_top - c_bottom)
+    d = tapered_capsule_device_params(0.4, 0.3, 0.1)
+    top_c, bottom_c, top_r, bottom_r = (float(x) for x in d[:4])
+    sin_a = (bottom_r - top_r) / (top_c - bottom_c)
+    n = np.array([1.0, math.tan(math.asin(sin_a)), 0.0])
+    n /= np.linalg.norm(n)
+    mid = 0.5 * (top_c + bottom_c)
+    p = _probe_plane(_tapered(), (1.5, mid, 0.0))
+    np.testing.assert_allclose(p[:3], n, atol=1e-6)
+    # the plane passes through the bottom sphere's tangent point bottom_c + r_bottom * n
+    point = np.array([0.0, bottom_c, 0.0]) + bottom_r * n
+    assert p[3] == pytest.approx(-np.dot(n, point), abs=1e-6)
