@@ -51,6 +51,23 @@ extern "C" {
  * tanAlpha} as the shape holds them (TaperedCapsuleShape.cpp:86-131: the centres are relative
  * to the shape's centre of mass; tanAlpha = Tan(ASin(sinAlpha)) with Jolt's approximations) */
 #define XPBD_SHAPE_TAPERED_CAPSULE 5
+/* ConvexHullShape: no params; the hull's own data in xpbd_collider_desc.hull
+ * (ConvexHullShape.cpp:1060-1164, unscaled) */
+#define XPBD_SHAPE_CONVEX_HULL 6
+
+/* A convex hull as ConvexHullShape holds it: points relative to its centre of mass
+ * (GetPoint), one plane per face (GetPlanes: normal xyz, constant) and each face's
+ * vertex indices in counter-clockwise order (GetFaceVertices), concatenated. */
+typedef struct xpbd_convex_hull {
+    uint32_t num_points;
+    const float *points;            /* num_points x 3 */
+    uint32_t num_faces;
+    const float *planes;            /* num_faces x 4 */
+    const uint32_t *face_first;     /* num_faces: first entry of the face in vertex_index */
+    const uint32_t *face_count;     /* num_faces: vertices of the face */
+    uint32_t num_vertex_index;
+    const uint32_t *vertex_index;   /* indices into points */
+} xpbd_convex_hull;
 
 typedef struct xpbd_context xpbd_context;
 typedef struct xpbd_settings xpbd_settings;
@@ -191,6 +208,7 @@ typedef struct xpbd_collider_desc {
     float friction;
     float restitution;
     float world_bounds[6];      /* Body::GetWorldSpaceBounds(): min xyz, max xyz */
+    const xpbd_convex_hull *hull;   /* XPBD_SHAPE_CONVEX_HULL only (copied by xpbd_set_colliders) */
 } xpbd_collider_desc;
 
 int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *colliders, uint32_t num_colliders);

@@ -24,6 +24,7 @@ struct Leaf
                     // tapered capsule {top centre, bottom centre, top radius}
     int32_t type;
     float friction, restitution;
+    const HullDev *hull;    // convex hull tables (device memory)
 };
 
 __device__ __forceinline__ V3 xyz(float4 p) { return v3(p.x, p.y, p.z); }
@@ -331,15 +332,95 @@ __device__ __forceinline__ void run_levels_lds(const uint4 *recs, const uint32_t
 }
 
 // ---------------------------------------------------------------- collision
-// Shape::CollideSoftBodyVertices for plane / sphere / box / capsule with the
-// argmax rule of CollideSoftBodyVertexIterator::UpdatePenetration
-// (CollideSoftBodyVertexIterator.h:83-97): a leaf replaces the stored plane
-// unless the stored penetration is >= the new one.
+// Shape::CollideSoftBodyVertices for plane / sphere / box / capsule / cylinder /
+// tapered capsule / convex hull with the argmax rule of
+// CollideSoftBodyVertexIterator::UpdatePenetration (CollideSoftBodyVertexIterator.h:83-97):
+// a leaf replaces the stored plane unless the stored penetration is >= the new one.
 // Vec3::NormalizedOr (Vec3.inl:978-1000): zero_value when |v|^2 <= FLT_MIN
 __device__ __forceinline__ V3 normalized_or(V3 v, V3 zero_value)
 {
     const float l2 = len_sq(v);
     return l2 <= FLT_MIN ? zero_value : v / sqrtf(l2);
+}
+
+// ClosestPoint::GetClosestPointOnLine (ClosestPoint.h:129-151) with
+// GetBaryCentricCoordinates for a line (ClosestPoint.h:15-41): closest point to
+// the origin on segment (a, b); a degenerate segment picks the nearer end.
+__device__ __forceinline__ V3 closest_point_on_line(V3 a, V3 b)
+{
+    const V3 ab = b - a;
+    const float denominator = len_sq(ab);
+    if (denominator < FLT_EPSILON * FLT_EPSILON)
+        return len_sq(a) < len_sq(b) ? a : b;
+    const float v = -dot(a, ab) / denominator;
+    const float u = 1.0f - v;
+    if (v <= 0.0f)
+        return a;
+    if (u <= 0.0f)
+        return b;
+    return u * a + v * b;
+}
+
+// ConvexHullShape::CollideSoftBodyVertices for one vertex, unscaled
+// (ConvexHullShape.cpp:1060-1164): the most facing plane, then, outside the
+// hull, the closest point on that face's edges that the vertex lies beyond.
+__device__ __forceinline__ void collide_hull(const Leaf &L, uint32_t l, V3 x, float &largest, int32_t &idx, Pl &plane)
+{
+    const HullDev &H = *L.hull;
+    const V3 local = mul_point(L.inv, x);
+    float max_distance = -FLT_MAX;
+    V3 max_plane_normal = v3(0.0f, 0.0f, 0.0f);
+    uint32_t max_plane_idx = 0;
+    for (uint32_t f = 0; f < H.num_faces; ++f)
+    {
+        const float4 pf = H.planes[f];
+        Pl p; p.n = xyz(pf); p.c = pf.w;
+        const float distance = signed_distance(p, local);
+        if (distance > max_distance)
+        {
+            max_distance = distance;
+            max_plane_normal = p.n;
+            max_plane_idx = f;
+        }
+    }
+    V3 closest_point = -max_distance * max_plane_normal;
+    const bool is_outside = max_distance > 0.0f;
+    if (is_outside)
+    {
+        float closest_point_dist_sq = FLT_MAX;
+        const uint32_t *vi = H.index + H.face_first[max_plane_idx];
+        const uint32_t count = H.face_count[max_plane_idx];
+        for (uint32_t k = 0; k < count; ++k)
+        {
+            const V3 p1 = xyz(H.points[vi[k]]);
+            const V3 p2 = xyz(H.points[vi[k + 1 == count ? 0 : k + 1]]);
+            const V3 edge_normal = cross(p2 - p1, max_plane_normal);
+            if (dot(edge_normal, local - p1) > 0.0f)
+            {
+                const V3 closest = closest_point_on_line(p1 - local, p2 - local);
+                const float distance_sq = len_sq(closest);
+                if (distance_sq < closest_point_dist_sq)
+                {
+                    closest_point_dist_sq = distance_sq;
+                    closest_point = closest;
+                }
+            }
+        }
+    }
+    V3 normal = neg(closest_point);
+    const float normal_length = len(normal);
+    float penetration = normal_length;
+    if (is_outside)
+        penetration = -penetration;
+    else
+        normal = neg(normal);
+    if (!(largest >= penetration))
+    {
+        largest = penetration;
+        normal = normal_length > 1.0e-12f ? normal / normal_length : max_plane_normal;
+        plane = plane_transformed(plane_from_point_normal(local + closest_point, normal), L.t);
+        idx = (int32_t)l;
+    }
 }
 
 __device__ __forceinline__ void collide_vertex(const Leaf *leaves, uint32_t num_leaves, V3 x, float &largest, int32_t &idx, Pl &plane)
@@ -514,6 +595,9 @@ __device__ __forceinline__ void collide_vertex(const Leaf *leaves, uint32_t num_
                 }
             }
             break;
+        case 6:
+            collide_hull(L, l, x, largest, idx, plane);
+            break;
         }
     }
 }
@@ -557,10 +641,14 @@ __device__ __forceinline__ void make_leaf(const ColliderDev &c, const M44 &inv_c
         out.inv = inversed_rotation_translation(t);
         out.a = 1.0f * c.params[0]; out.b = 1.0f * c.params[1];
         break;
-    default:
+    case 5:
         out.inv = inversed_rotation_translation(t);
         out.a = c.params[0]; out.b = c.params[1]; out.c = c.params[2];
         out.pn[0] = c.params[3]; out.pn[1] = c.params[4]; out.pn[2] = c.params[5];
+        break;
+    default:
+        out.inv = inversed_rotation_translation(t);
+        out.hull = c.hull;
         break;
     }
 }

@@ -159,6 +159,9 @@ struct xpbd_context
     bool states_on_device = false;
     std::vector<ColliderDev> colliders;
     DevBuf<ColliderDev> d_colliders;
+    DevBuf<float4> d_hull_vec;      // convex hulls: planes, then points, per hull
+    DevBuf<uint32_t> d_hull_idx;    // convex hulls: face first, face count, vertex index, per hull
+    DevBuf<HullDev> d_hulls;
     DevBuf<int32_t> d_error;
     float gravity[3] = { 0.0f, -9.81f, 0.0f };       // PhysicsSystem.h:392 default
     bool profiling = false;
@@ -1228,12 +1231,51 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
         return Fail(XPBD_E_INVALID, "null argument");
     HIP_TRY(hipSetDevice(ctx->device));
     std::vector<ColliderDev> v(n);
+    // convex hull tables: validated here, flattened into two pools, pointers set after upload
+    std::vector<float4> hull_vec;
+    std::vector<uint32_t> hull_idx;
+    struct HullOffsets { uint32_t planes, points, face_first, face_count, index, num_faces, collider; };
+    std::vector<HullOffsets> hull_offsets;
     for (uint32_t i = 0; i < n; ++i)
     {
-        if (c[i].type < XPBD_SHAPE_PLANE || c[i].type > XPBD_SHAPE_TAPERED_CAPSULE)
+        if (c[i].type < XPBD_SHAPE_PLANE || c[i].type > XPBD_SHAPE_CONVEX_HULL)
             return Fail(XPBD_E_UNSUPPORTED, "collider shape type not supported (plane, sphere, box, capsule, cylinder, "
-                                            "tapered capsule only)");
+                                            "tapered capsule, convex hull only)");
+        if (c[i].type == XPBD_SHAPE_CONVEX_HULL)
+        {
+            const xpbd_convex_hull *h = c[i].hull;
+            if (!h || !h->points || !h->planes || !h->face_first || !h->face_count || !h->vertex_index)
+                return Fail(XPBD_E_INVALID, "convex hull collider without hull data");
+            if (h->num_faces == 0 || h->num_points == 0)
+                return Fail(XPBD_E_INVALID, "convex hull without faces or points");
+            for (uint32_t f = 0; f < h->num_faces; ++f)
+            {
+                const uint64_t end = (uint64_t)h->face_first[f] + h->face_count[f];
+                if (h->face_count[f] < 2 || end > h->num_vertex_index)
+                    return Fail(XPBD_E_INVALID, "convex hull face outside the vertex index table");
+            }
+            for (uint32_t k = 0; k < h->num_vertex_index; ++k)
+                if (h->vertex_index[k] >= h->num_points)
+                    return Fail(XPBD_E_INVALID, "convex hull vertex index out of range");
+            HullOffsets o;
+            o.planes = (uint32_t)hull_vec.size();
+            for (uint32_t f = 0; f < h->num_faces; ++f)
+                hull_vec.push_back(make_float4(h->planes[4 * f], h->planes[4 * f + 1], h->planes[4 * f + 2], h->planes[4 * f + 3]));
+            o.points = (uint32_t)hull_vec.size();
+            for (uint32_t k = 0; k < h->num_points; ++k)
+                hull_vec.push_back(make_float4(h->points[3 * k], h->points[3 * k + 1], h->points[3 * k + 2], 0.0f));
+            o.face_first = (uint32_t)hull_idx.size();
+            hull_idx.insert(hull_idx.end(), h->face_first, h->face_first + h->num_faces);
+            o.face_count = (uint32_t)hull_idx.size();
+            hull_idx.insert(hull_idx.end(), h->face_count, h->face_count + h->num_faces);
+            o.index = (uint32_t)hull_idx.size();
+            hull_idx.insert(hull_idx.end(), h->vertex_index, h->vertex_index + h->num_vertex_index);
+            o.num_faces = h->num_faces;
+            o.collider = i;
+            hull_offsets.push_back(o);
+        }
         v[i].type = c[i].type;
+        v[i].hull = nullptr;
         memcpy(v[i].params, c[i].params, sizeof(v[i].params));
         memcpy(v[i].pos, c[i].position, sizeof(v[i].pos));
         memcpy(v[i].rot, c[i].rotation, sizeof(v[i].rot));
@@ -1242,6 +1284,22 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
         memcpy(v[i].bounds, c[i].world_bounds, sizeof(v[i].bounds));
     }
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx->d_hull_vec.upload(hull_vec));
+    HIP_TRY(ctx->d_hull_idx.upload(hull_idx));
+    std::vector<HullDev> hulls(hull_offsets.size());
+    for (size_t k = 0; k < hull_offsets.size(); ++k)
+    {
+        const HullOffsets &o = hull_offsets[k];
+        hulls[k].planes = ctx->d_hull_vec.p + o.planes;
+        hulls[k].points = ctx->d_hull_vec.p + o.points;
+        hulls[k].face_first = ctx->d_hull_idx.p + o.face_first;
+        hulls[k].face_count = ctx->d_hull_idx.p + o.face_count;
+        hulls[k].index = ctx->d_hull_idx.p + o.index;
+        hulls[k].num_faces = o.num_faces;
+    }
+    HIP_TRY(ctx->d_hulls.upload(hulls));
+    for (size_t k = 0; k < hull_offsets.size(); ++k)
+        v[hull_offsets[k].collider].hull = ctx->d_hulls.p + k;
     if (v.size() != ctx->colliders.size())
         ctx->layout_dirty = true;
     ctx->colliders = v;

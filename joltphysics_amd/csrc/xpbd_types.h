@@ -73,10 +73,22 @@ struct BodyStateDev
     int32_t active;
 };
 
+// Convex hull data in device memory (a context-wide table, one entry per hull collider)
+struct HullDev
+{
+    const float4 *planes;       // per face: normal, constant
+    const float4 *points;       // xyz relative to the centre of mass
+    const uint32_t *face_first; // per face: first entry in index
+    const uint32_t *face_count;
+    const uint32_t *index;      // face vertex indices, counter-clockwise
+    uint32_t num_faces;
+};
+
 struct ColliderDev
 {
     int32_t type;
     float params[8];
+    const HullDev *hull;        // XPBD_SHAPE_CONVEX_HULL
     float pos[3];
     float rot[4];
     float friction, restitution;

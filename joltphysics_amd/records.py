@@ -42,6 +42,7 @@ LRA_NONE, LRA_EUCLIDEAN, LRA_GEODESIC = 0, 1, 2
 
 # Collider shape types of the C ABI
 SHAPE_PLANE, SHAPE_SPHERE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_CYLINDER, SHAPE_TAPERED_CAPSULE = 0, 1, 2, 3, 4, 5
+SHAPE_CONVEX_HULL = 6
 
 
 def _empty(dtype):
@@ -97,17 +98,70 @@ class BodyParams:
 
 
 @dataclass
+class ConvexHull:
+    """A convex hull as ConvexHullShape holds it (ConvexHullShape.h:114-141): points
+    relative to the centre of mass, one plane per face (normal xyz, constant), each
+    face's first entry and vertex count in `vertex_index` (counter-clockwise), and the
+    centre of mass in the shape's frame."""
+
+    points: np.ndarray          # (P, 3) float32
+    planes: np.ndarray          # (F, 4) float32
+    face_first: np.ndarray      # (F,) uint32
+    face_count: np.ndarray      # (F,) uint32
+    vertex_index: np.ndarray    # (sum of face_count,) uint32
+    center_of_mass: tuple = (0.0, 0.0, 0.0)
+
+    @staticmethod
+    def from_polytope(points, faces):
+        """Hull data for a convex polytope given by its vertices and counter-clockwise
+        faces (lists of vertex indices), in float64 then rounded: centre of mass of the
+        solid by a tetrahedron fan, plane of each face through its centroid with the
+        normalised Newell normal.  The reference derives these with its own hull
+        builder (ConvexHullShape.cpp:50-180), so values agree to rounding, not bits."""
+        pts = np.asarray(points, np.float64).reshape(-1, 3)
+        faces = [list(map(int, f)) for f in faces]
+        vol = 0.0
+        acc = np.zeros(3)
+        o = pts[faces[0][0]]
+        for f in faces:
+            for k in range(1, len(f) - 1):
+                a, b, c = pts[f[0]] - o, pts[f[k]] - o, pts[f[k + 1]] - o
+                v6 = float(np.dot(a, np.cross(b, c)))
+                vol += v6
+                acc += v6 * (a + b + c) / 4.0
+        com = o + acc / vol
+        rel = pts - com
+        planes, first, count, index = [], [], [], []
+        for f in faces:
+            q = rel[f]
+            n = np.zeros(3)
+            for k in range(len(f)):
+                a, b = q[k], q[(k + 1) % len(f)]
+                n += np.array([(a[1] - b[1]) * (a[2] + b[2]), (a[2] - b[2]) * (a[0] + b[0]),
+                               (a[0] - b[0]) * (a[1] + b[1])])
+            n /= np.linalg.norm(n)
+            planes.append([n[0], n[1], n[2], -float(np.dot(n, q.mean(axis=0)))])
+            first.append(len(index))
+            count.append(len(f))
+            index.extend(f)
+        return ConvexHull(rel.astype(np.float32), np.array(planes, np.float32), np.array(first, np.uint32),
+                          np.array(count, np.uint32), np.array(index, np.uint32), tuple(float(x) for x in com))
+
+
+@dataclass
 class Collider:
     """A static collider body: shape, world transform (body position and rotation),
     material, world bounds.
 
     `params` are the shape's construction parameters (plane {nx, ny, nz, c, half extent},
     sphere {r}, box {hx, hy, hz}, capsule / cylinder {half height, r}, tapered capsule
-    {half height of the tapered cylinder, top radius, bottom radius}).  `device_params`
-    and `com` are what the soft-body update consumes (shape parameters as the shape holds
-    them, centre-of-mass position); when unset they are derived here, for the tapered
-    capsule approximately (the reference derives them with its own sin / tan; the tests
-    take them from the reference)."""
+    {half height of the tapered cylinder, top radius, bottom radius}, convex hull: its
+    construction points, flattened xyz).  `device_params`, `hull` and `com` are what the
+    soft-body update consumes (shape parameters and hull data as the shape holds them,
+    centre-of-mass position); when unset they are derived here, for the tapered capsule
+    approximately (the reference derives them with its own sin / tan; the tests take
+    them from the reference).  A convex hull needs `hull` (ConvexHull.from_polytope, or
+    the reference's own)."""
 
     type: int
     params: tuple
@@ -118,8 +172,11 @@ class Collider:
     bounds: tuple | None = None  # world-space AABB (min xyz, max xyz)
     device_params: tuple | None = None
     com: tuple | None = None
+    hull: ConvexHull | None = None
 
     def param_array(self):
+        if self.type == SHAPE_CONVEX_HULL:
+            return np.zeros(8, dtype=np.float32)
         p = np.zeros(8, dtype=np.float32)
         p[:len(self.params)] = self.params
         return p
@@ -140,7 +197,15 @@ class Collider:
             p = tapered_capsule_device_params(*self.params[:3])
             off = np.array([0.0, self.params[0] - p[0], 0.0])      # centre of mass in the shape's frame
             return tuple(np.array(self.pos, np.float64) + _rotate(self.rot, off))
+        if self.type == SHAPE_CONVEX_HULL:
+            if self.hull is None:
+                raise ValueError("convex hull collider without hull data")
+            return tuple(np.array(self.pos, np.float64) + _rotate(self.rot, self.hull.center_of_mass))
         return tuple(self.pos)
+
+    def hull_points(self):
+        """Construction points of a convex hull collider, (P, 3) float32."""
+        return np.asarray(self.params, np.float32).reshape(-1, 3)
 
 
 def tapered_capsule_device_params(half_height, top_radius, bottom_radius):

@@ -18,7 +18,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .records import (BEND_DIHEDRAL, BEND_NONE, EDGE, LRA, LRA_NONE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_CYLINDER,
-                      SHAPE_PLANE, SHAPE_SPHERE, SHAPE_TAPERED_CAPSULE, VERTEX, VOLUME, BodyParams, Collider,
+                      SHAPE_CONVEX_HULL, SHAPE_PLANE, SHAPE_SPHERE, SHAPE_TAPERED_CAPSULE, VERTEX, VOLUME, BodyParams, Collider,
                       tapered_capsule_device_params)
 
 # Build flags (same bit meaning on the product and oracle builders)
@@ -210,6 +210,9 @@ def collider_world_bounds(c: Collider):
         ext = max(abs(float(d[0])) + float(d[2]), abs(float(d[1])) + float(d[3]))
         he = np.abs(R) @ np.array([r, ext, r])
         p = np.array(c.com_position(), np.float64)
+    elif c.type == SHAPE_CONVEX_HULL:
+        w = c.hull_points().astype(np.float64) @ R.T + p
+        return tuple(np.concatenate([w.min(axis=0), w.max(axis=0)]).astype(np.float32))
     else:  # plane: square of half extent around the plane point, extruded half extent behind
         n = np.array(c.params[:3], np.float64)
         he = np.full(3, 2.0 * c.params[4])

@@ -17,7 +17,7 @@ import sys
 
 import numpy as np
 
-from .records import (BEND, EDGE, FACE, GROUP, LRA, VERTEX, VOLUME, BodyParams, SettingsArrays)
+from .records import (BEND, EDGE, FACE, GROUP, LRA, SHAPE_CONVEX_HULL, VERTEX, VOLUME, BodyParams, SettingsArrays)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libjolt_xpbd.so")
@@ -50,10 +50,16 @@ class _BodyDesc(C.Structure):
                 ("allow_sleeping", C.c_int32)]
 
 
+class _ConvexHull(C.Structure):
+    _fields_ = [("num_points", C.c_uint32), ("points", C.c_void_p), ("num_faces", C.c_uint32),
+                ("planes", C.c_void_p), ("face_first", C.c_void_p), ("face_count", C.c_void_p),
+                ("num_vertex_index", C.c_uint32), ("vertex_index", C.c_void_p)]
+
+
 class _ColliderDesc(C.Structure):
     _fields_ = [("type", C.c_int32), ("params", C.c_float * 8), ("position", C.c_float * 3),
                 ("rotation", C.c_float * 4), ("friction", C.c_float), ("restitution", C.c_float),
-                ("world_bounds", C.c_float * 6)]
+                ("world_bounds", C.c_float * 6), ("hull", C.POINTER(_ConvexHull))]
 
 
 class _RolloutDesc(C.Structure):
@@ -242,8 +248,20 @@ class Context:
     def set_colliders(self, colliders):
         n = len(colliders)
         arr = (_ColliderDesc * max(n, 1))()
+        keep = []   # hull arrays stay alive until the call has copied them
         for i, c in enumerate(colliders):
             arr[i].type = int(c.type)
+            if c.type == SHAPE_CONVEX_HULL:
+                if c.hull is None:
+                    raise XpbdError("convex hull collider without hull data")
+                h = c.hull
+                a = (np.ascontiguousarray(h.points, np.float32), np.ascontiguousarray(h.planes, np.float32),
+                     np.ascontiguousarray(h.face_first, np.uint32), np.ascontiguousarray(h.face_count, np.uint32),
+                     np.ascontiguousarray(h.vertex_index, np.uint32))
+                d = _ConvexHull(len(a[0]), a[0].ctypes.data, len(a[1]), a[1].ctypes.data, a[2].ctypes.data,
+                                a[3].ctypes.data, len(a[4]), a[4].ctypes.data)
+                keep.append((a, d))
+                arr[i].hull = C.pointer(d)
             p = c.device_param_array()
             for k in range(8):
                 arr[i].params[k] = float(p[k])

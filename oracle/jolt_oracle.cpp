@@ -41,6 +41,7 @@
 #include <Jolt/Physics/Collision/Shape/PlaneShape.h>
 #include <Jolt/Physics/Collision/Shape/CylinderShape.h>
 #include <Jolt/Physics/Collision/Shape/TaperedCapsuleShape.h>
+#include <Jolt/Physics/Collision/Shape/ConvexHullShape.h>
 #include <Jolt/Physics/SoftBody/SoftBodySharedSettings.h>
 #include <Jolt/Physics/SoftBody/SoftBodyCreationSettings.h>
 #include <Jolt/Physics/SoftBody/SoftBodyMotionProperties.h>
@@ -354,6 +355,72 @@ int jo_world_add_collider(void *wp, int type, const float *params, const float *
 	w->mColliderParams.push_back(kept);
 	w->mColliderTypes.push_back(type);
 	return int(w->mColliders.size() - 1);
+}
+
+// Static convex hull collider built by ConvexHullShapeSettings from `num_points`
+// points (xyz), convex radius 0 (ConvexHullShape.h:27).  pos[3], rot[4] as above.
+int jo_world_add_hull(void *wp, const float *points, int num_points, const float *pos, const float *rot, float friction, float restitution)
+{
+	World *w = (World *)wp;
+	std::vector<Vec3> pts;
+	for (int i = 0; i < num_points; ++i)
+		pts.push_back(Vec3(points[3 * i], points[3 * i + 1], points[3 * i + 2]));
+	ConvexHullShapeSettings hs(pts.data(), num_points, 0.0f);
+	Shape::ShapeResult r = hs.Create();
+	if (r.HasError() || r.Get()->GetSubType() != EShapeSubType::ConvexHull)
+		return -1;
+	BodyCreationSettings bcs(r.Get(), RVec3(pos[0], pos[1], pos[2]), Quat(rot[0], rot[1], rot[2], rot[3]), EMotionType::Static, Layers::STATIC);
+	bcs.mFriction = friction;
+	bcs.mRestitution = restitution;
+	BodyID id = w->mSystem.GetBodyInterface().CreateAndAddBody(bcs, EActivation::DontActivate);
+	if (id.IsInvalid())
+		return -1;
+	w->mColliders.push_back(id);
+	w->mColliderParams.push_back(std::array<float, 8>{});
+	w->mColliderTypes.push_back(6);
+	return int(w->mColliders.size() - 1);
+}
+
+// The hull as the shape holds it (ConvexHullShape.h:114-141): sizes[3] =
+// {points, faces, face vertex indices}; with non-null arrays also the points
+// (relative to the centre of mass, xyz), planes (normal xyz, constant), each
+// face's first entry and count, and the concatenated face vertex indices.
+int jo_world_hull_data(void *wp, int collider, int *sizes, float *points, float *planes, uint32 *face_first, uint32 *face_count, uint32 *index)
+{
+	World *w = (World *)wp;
+	if (w->mColliderTypes[collider] != 6)
+		return -1;
+	BodyLockRead lock(w->mSystem.GetBodyLockInterface(), w->mColliders[collider]);
+	if (!lock.Succeeded())
+		return -1;
+	const ConvexHullShape *h = static_cast<const ConvexHullShape *>(lock.GetBody().GetShape());
+	uint total = 0;
+	for (uint f = 0; f < h->GetNumFaces(); ++f)
+		total += h->GetNumVerticesInFace(f);
+	sizes[0] = int(h->GetNumPoints()); sizes[1] = int(h->GetNumFaces()); sizes[2] = int(total);
+	if (!points)
+		return 0;
+	for (uint i = 0; i < h->GetNumPoints(); ++i)
+	{
+		Vec3 p = h->GetPoint(i);
+		points[3 * i] = p.GetX(); points[3 * i + 1] = p.GetY(); points[3 * i + 2] = p.GetZ();
+	}
+	uint at = 0;
+	for (uint f = 0; f < h->GetNumFaces(); ++f)
+	{
+		const Plane &pl = h->GetPlanes()[f];
+		planes[4 * f] = pl.GetNormal().GetX(); planes[4 * f + 1] = pl.GetNormal().GetY();
+		planes[4 * f + 2] = pl.GetNormal().GetZ(); planes[4 * f + 3] = pl.GetConstant();
+		const uint n = h->GetNumVerticesInFace(f);
+		std::vector<uint> vi(n);
+		h->GetFaceVertices(f, n, vi.data());
+		face_first[f] = at;
+		face_count[f] = n;
+		for (uint k = 0; k < n; ++k)
+			index[at + k] = vi[k];
+		at += n;
+	}
+	return 0;
 }
 
 // The collider as the soft-body update consumes it: the body's centre-of-mass

@@ -80,6 +80,31 @@ template <class F> inline V3<F> NormalizedOr(V3<F> v, V3<F> zero_value)
 	return l2 <= F(FLT_MIN) ? zero_value : v / std::sqrt(l2);
 }
 
+// ClosestPoint::GetClosestPointOnLine with the line's GetBaryCentricCoordinates
+// (ClosestPoint.h:15-41, 129-151): closest point to the origin on segment (a, b);
+// a degenerate segment (|b - a|^2 < FLT_EPSILON^2) falls back to the nearer end.
+template <class F> inline V3<F> ClosestPointOnLine(V3<F> a, V3<F> b)
+{
+	V3<F> ab = b - a;
+	F denominator = LenSq(ab);
+	F u, v;
+	if (denominator < F(FLT_EPSILON) * F(FLT_EPSILON))
+	{
+		if (LenSq(a) < LenSq(b)) { u = 1; v = 0; }
+		else { u = 0; v = 1; }
+	}
+	else
+	{
+		v = -Dot(a, ab) / denominator;
+		u = F(1) - v;
+	}
+	if (v <= F(0))
+		return a;
+	if (u <= F(0))
+		return b;
+	return u * a + v * b;
+}
+
 // ACosApproximate (Math/Trigonometry.h:48-65)
 template <class F> inline F ACosApprox(F x)
 {
@@ -242,8 +267,12 @@ struct GroupRec { u32 edge_end, lra_end, rod_ss_end, rod_bt_end, bend_end, volum
 // ------------------------------------------------------------------ world
 struct Collider
 {
-	int type;				// 0 plane, 1 sphere, 2 box, 3 capsule, 4 cylinder, 5 tapered capsule
+	int type;				// 0 plane, 1 sphere, 2 box, 3 capsule, 4 cylinder, 5 tapered capsule, 6 convex hull
 	float params[8];		// as the shape holds them (tapered capsule: top / bottom centre, radii, sin, tan)
+	// convex hull as the shape holds it: points relative to the centre of mass (xyz),
+	// planes (normal xyz, constant), per face first entry / count into index
+	std::vector<float> hull_points, hull_planes;
+	std::vector<u32> face_first, face_count, index;
 	float pos[3], rot[4];
 	float friction, restitution;
 	float bounds[6];		// world space bounds of the static body (min xyz, max xyz)
@@ -508,6 +537,70 @@ template <class F> void CollideVertices(Body<F> &b, const Leaf<F> &leaf, int sha
 					if (update(i, pen))
 					{
 						b.plane[i] = PlaneTransformed(plane, T);
+						b.shape_idx[i] = shape_index;
+					}
+				}
+		}
+		break;
+	case 6: // ConvexHullShape.cpp:1060-1164 (scale 1: the scaled-plane branch and scale_flip drop out)
+		{
+			M44<F> inv = InvRT(T);
+			const size_t nf = c.face_count.size();
+			auto point = [&](u32 k) { return V3<F>{ F(c.hull_points[3 * k]), F(c.hull_points[3 * k + 1]), F(c.hull_points[3 * k + 2]) }; };
+			for (size_t i = 0; i < nv; ++i)
+				if (b.w[i] > F(0))
+				{
+					V3<F> local = MulPoint(inv, b.x[i]);
+					// most facing plane, first of equals (strict >)
+					F max_distance = -F(FLT_MAX);
+					V3<F> max_plane_normal{ 0, 0, 0 };
+					size_t max_plane_idx = 0;
+					for (size_t f = 0; f < nf; ++f)
+					{
+						PlaneT<F> p{ { F(c.hull_planes[4 * f]), F(c.hull_planes[4 * f + 1]), F(c.hull_planes[4 * f + 2]) }, F(c.hull_planes[4 * f + 3]) };
+						F distance = SignedDistance(p, local);
+						if (distance > max_distance)
+						{
+							max_distance = distance;
+							max_plane_normal = p.n;
+							max_plane_idx = f;
+						}
+					}
+					V3<F> closest_point = (-max_distance) * max_plane_normal;
+					const bool is_outside = max_distance > F(0);
+					if (is_outside)
+					{
+						// edges of the face the vertex lies beyond; nearest wins, first of equals
+						F closest_point_dist_sq = F(FLT_MAX);
+						const u32 first = c.face_first[max_plane_idx], count = c.face_count[max_plane_idx];
+						for (u32 k = 0; k < count; ++k)
+						{
+							V3<F> p1 = point(c.index[first + k]);
+							V3<F> p2 = point(c.index[first + (k + 1 == count ? 0 : k + 1)]);
+							V3<F> edge_normal = Cross(p2 - p1, max_plane_normal);
+							if (Dot(edge_normal, local - p1) > F(0))
+							{
+								V3<F> closest = ClosestPointOnLine(p1 - local, p2 - local);
+								F distance_sq = LenSq(closest);
+								if (distance_sq < closest_point_dist_sq)
+								{
+									closest_point_dist_sq = distance_sq;
+									closest_point = closest;
+								}
+							}
+						}
+					}
+					V3<F> normal = Neg(closest_point);
+					F normal_length = Len(normal);
+					F penetration = normal_length;
+					if (is_outside)
+						penetration = -penetration;
+					else
+						normal = Neg(normal);
+					if (update(i, penetration))
+					{
+						normal = normal_length > F(1.0e-12f) ? normal / normal_length : max_plane_normal;
+						b.plane[i] = PlaneTransformed(PlaneFromPointNormal(local + closest_point, normal), T);
 						b.shape_idx[i] = shape_index;
 					}
 				}
@@ -975,11 +1068,11 @@ void xr_world_set_gravity(void *wp, float gx, float gy, float gz)
 int xr_world_add_collider(void *wp, int type, const float *params, const float *pos, const float *rot, float friction, float restitution, const float *bounds)
 {
 	XrWorld *w = (XrWorld *)wp;
-	if (type < 0 || type > 5)
+	if (type < 0 || type > 6)
 		return -1;
 	Collider c{};
 	c.type = type;
-	int np = type == 0 ? 5 : type == 1 ? 1 : type == 2 ? 3 : type == 5 ? 6 : 2;
+	int np = type == 0 ? 5 : type == 1 ? 1 : type == 2 ? 3 : type == 5 ? 6 : type == 6 ? 0 : 2;
 	for (int i = 0; i < np; ++i) c.params[i] = params[i];
 	for (int i = 0; i < 3; ++i) c.pos[i] = pos[i];
 	for (int i = 0; i < 4; ++i) c.rot[i] = rot[i];
@@ -989,6 +1082,33 @@ int xr_world_add_collider(void *wp, int type, const float *params, const float *
 	w->f.colliders.push_back(c);
 	w->d.colliders.push_back(c);
 	return int(w->f.colliders.size() - 1);
+}
+
+// Hull data of convex hull collider `collider` (type 6): np points relative to the
+// centre of mass (xyz), nf planes (normal xyz, constant), per face first entry and
+// vertex count into the ni concatenated face vertex indices.
+int xr_world_set_hull(void *wp, int collider, const float *points, u32 np, const float *planes, u32 nf,
+					  const u32 *face_first, const u32 *face_count, const u32 *index, u32 ni)
+{
+	XrWorld *w = (XrWorld *)wp;
+	if (collider < 0 || size_t(collider) >= w->f.colliders.size() || w->f.colliders[collider].type != 6)
+		return -1;
+	for (u32 f = 0; f < nf; ++f)
+		if (face_count[f] < 2 || size_t(face_first[f]) + face_count[f] > ni)
+			return -1;
+	for (u32 k = 0; k < ni; ++k)
+		if (index[k] >= np)
+			return -1;
+	Collider *targets[2] = { &w->f.colliders[collider], &w->d.colliders[collider] };
+	for (Collider *c : targets)
+	{
+		c->hull_points.assign(points, points + 3 * np);
+		c->hull_planes.assign(planes, planes + 4 * nf);
+		c->face_first.assign(face_first, face_first + nf);
+		c->face_count.assign(face_count, face_count + nf);
+		c->index.assign(index, index + ni);
+	}
+	return 0;
 }
 
 // p: {numIterations, linearDamping, maxLinearVelocity, restitution, friction,

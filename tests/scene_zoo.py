@@ -8,7 +8,7 @@ import numpy as np
 
 from joltphysics_amd import scenes
 from joltphysics_amd.records import (BEND_DIHEDRAL, BEND_DISTANCE, LRA_GEODESIC, SHAPE_BOX, SHAPE_CAPSULE,
-                                     SHAPE_CYLINDER, SHAPE_PLANE, SHAPE_SPHERE, SHAPE_TAPERED_CAPSULE, BodyParams,
+                                     SHAPE_CONVEX_HULL, SHAPE_CYLINDER, SHAPE_PLANE, SHAPE_SPHERE, SHAPE_TAPERED_CAPSULE, BodyParams,
                                      Collider)
 from joltphysics_amd.scenes import (CALC_EDGE_LENGTHS, CALC_VOLUMES, CREATE_CONSTRAINTS, OPTIMIZE, BodySpec,
                                     MeshSpec, Scene)
@@ -128,6 +128,43 @@ def cloth_cylinder_tapered(n=40, frames=40):
                  frames=frames)
 
 
+def _hull_points():
+    """Two hulls' construction points: an irregular cloud (triangle faces, some
+    coplanar-merged) with interior points the hull builder drops, and a square
+    pyramid (quad base, apex and slanted edges) plus one interior point."""
+    rng = np.random.default_rng(21)
+    d = rng.normal(size=(28, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    cloud = d * np.array([0.3, 0.22, 0.25]) * rng.uniform(0.6, 1.0, size=(28, 1))
+    pyramid = np.array([[-0.25, -0.25, -0.15], [0.25, -0.25, -0.15], [0.25, 0.25, -0.15], [-0.25, 0.25, -0.15],
+                        [0.02, -0.01, 0.25], [0.0, 0.0, 0.0]])
+    return cloud.astype(np.float32).ravel(), pyramid.astype(np.float32).ravel()
+
+
+def cloth_hulls(n=40, frames=40, spacing=0.05, height=0.6, iterations=5, collision_steps=2):
+    """n x n cloth draping over two convex hull colliders (faces, edges and vertices of
+    both, the edge search of a vertex outside a face), no ground."""
+    v, f = scenes.cloth(n, spacing, 60.0, jitter=1e-4, seed=13)
+    mesh = MeshSpec(vertices=v, faces=f, attrs=scenes.attrs(1e-6, 1e-6, 1e-4), bend_type=BEND_DIHEDRAL)
+    body = BodySpec(mesh=mesh, position=(0.0, 0.0, height),
+                    params=BodyParams(num_iterations=iterations, allow_sleeping=False))
+    cloud, pyramid = _hull_points()
+    cols = _bounded([Collider(SHAPE_CONVEX_HULL, tuple(cloud), pos=(-0.4, 0.05, 0.2), rot=_quat((0.3, 1, 0.2), 0.7),
+                              friction=0.5),
+                     Collider(SHAPE_CONVEX_HULL, tuple(pyramid), pos=(0.42, -0.05, 0.2), rot=_quat((1, 0, 0.4), 0.25),
+                              friction=0.3, restitution=0.1)])
+    return Scene("cloth_hulls", [body], cols, gravity=(0.0, 0.0, -9.81), dt=1 / 60, collision_steps=collision_steps,
+                 frames=frames)
+
+
+def cloth_hulls_streamed():
+    """The hull scene with a 120 x 120 cloth (14,400 vertices, past the LDS budget) so
+    the streamed kernel runs the hull search; dropped just above the hulls and kept to
+    a few frames (a free 120^2 cloth at these compliances diverges after about ten
+    frames in the reference itself)."""
+    return cloth_hulls(n=120, frames=8, spacing=0.02, height=0.46, iterations=4, collision_steps=1)
+
+
 ALL = {
     "cloth100_sphere": big_cloth_on_sphere,
     "tetcube_plane_box": tet_cube_on_plane,
@@ -137,6 +174,8 @@ ALL = {
     "batched": batched,
     "cloth_cube_spheres": cloth_cube_spheres,
     "cloth_cylinder_tapered": cloth_cylinder_tapered,
+    "cloth_hulls": cloth_hulls,
+    "cloth_hulls_streamed": cloth_hulls_streamed,
     "block_lra_boxes": block_lra_boxes,
     "block_lra_boxes_streamed": block_lra_boxes_streamed,
 }

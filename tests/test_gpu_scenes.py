@@ -1,5 +1,6 @@
-"""GPU parity on scenes covering every constraint type, collider shape, the
-LDS-resident and streamed storage modes, pressure, LRA, sleeping and batching.
+"""GPU parity on scenes covering every constraint type, collider shape (plane, sphere,
+box, capsule, cylinder, tapered capsule, convex hull), the LDS-resident and
+streamed storage modes, pressure, LRA, sleeping and batching.
 
 The reference (compiled from its sources with FMA contraction off, loaded
 through oracle/oracle.py) builds the settings with its own CreateConstraints
@@ -32,3 +33,26 @@ def test_scene_bit_exact(name):
             assert float_equal(g["pos"], r["pos"]), f"{name}: positions differ at frame {f}, body {i}"
             assert float_equal(g["vel"], r["vel"]), f"{name}: velocities differ at frame {f}, body {i}"
             assert float_equal(g["body_pos"], r["body_pos"]), f"{name}: body position differs at frame {f}, body {i}"
+
+
+def test_convex_hull_tables_are_validated():
+    """xpbd_set_colliders rejects hull tables that index outside themselves (a vertex
+    index past the points, a face running past the index table) before any upload."""
+    import numpy as np
+    from joltphysics_amd.records import SHAPE_CONVEX_HULL, Collider, ConvexHull
+    from joltphysics_amd.xpbd import Context, XpbdError
+    cube = np.array([[x, y, z] for x in (-0.5, 0.5) for y in (-0.5, 0.5) for z in (-0.5, 0.5)])
+    faces = [[0, 1, 3, 2], [4, 6, 7, 5], [0, 4, 5, 1], [2, 3, 7, 6], [0, 2, 6, 4], [1, 5, 7, 3]]
+    bad = ConvexHull.from_polytope(cube, faces)
+    bad.vertex_index = bad.vertex_index.copy()
+    bad.vertex_index[3] = 8
+    c = Collider(SHAPE_CONVEX_HULL, tuple(cube.ravel()), hull=bad, bounds=(-0.5, -0.5, -0.5, 0.5, 0.5, 0.5))
+    ctx = Context(0)
+    with pytest.raises(XpbdError, match="vertex index"):
+        ctx.set_colliders([c])
+    bad.vertex_index[3] = 3
+    bad.face_count = bad.face_count.copy()
+    bad.face_count[5] = 30
+    with pytest.raises(XpbdError, match="face"):
+        ctx.set_colliders([c])
+    ctx.close()
