@@ -209,6 +209,16 @@ typedef struct xpbd_collider_desc {
     float restitution;
     float world_bounds[6];      /* Body::GetWorldSpaceBounds(): min xyz, max xyz */
     const xpbd_convex_hull *hull;   /* XPBD_SHAPE_CONVEX_HULL only (copied by xpbd_set_colliders) */
+    /* Motion (Body::GetMotionType(), EMotionType: 0 static, 1 kinematic, 2 dynamic).  Static and
+     * kinematic bodies take the static-body contact branch; a dynamic body shares each contact
+     * impulse with the vertex (SoftBodyMotionProperties.cpp:752-797), as without a contact
+     * listener (every SoftBodyContactSettings scale 1, all angular DOFs allowed).  Dynamic only: */
+    int32_t motion_type;
+    float inv_mass;                 /* MotionProperties::GetInverseMass() */
+    float inv_inertia_diagonal[3];  /* MotionProperties::GetInverseInertiaDiagonal() */
+    float inertia_rotation[4];      /* MotionProperties::GetInertiaRotation(), x y z w */
+    float linear_velocity[3];       /* world space; held fixed for the whole xpbd_update call */
+    float angular_velocity[3];
 } xpbd_collider_desc;
 
 int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *colliders, uint32_t num_colliders);
@@ -239,6 +249,18 @@ int xpbd_body_write(xpbd_context *ctx, int32_t body, const float *pos, const flo
 /* Body summary: out[0..2] linear velocity, out[3..5] angular velocity (world),
  * out[6..11] local bounds, out[12] active flag, out[13] sleep timer. */
 int xpbd_body_state(xpbd_context *ctx, int32_t body, float out[14]);
+
+/*
+ * What soft body `body` did to the dynamic colliders during the last xpbd_update
+ * (SoftBodyMotionProperties::UpdateRigidBodyVelocities, SoftBodyMotionProperties.cpp:923-935):
+ * per collider of the last xpbd_set_colliders, the world-space linear and angular
+ * velocity changes (3 floats each) and whether the body touched it (updated: the
+ * reference's mUpdateVelocities; the caller then applies
+ * BodyInterface::AddLinearAndAngularVelocity).  Summed over the call's collision steps;
+ * zero for static and kinematic colliders.  Any output may be NULL.
+ */
+int xpbd_body_velocity_deltas(xpbd_context *ctx, int32_t body, uint32_t num_colliders, float *linear, float *angular,
+                              int32_t *updated);
 
 /*
  * Kernel timing on the context stream (hipEvents around every launch of

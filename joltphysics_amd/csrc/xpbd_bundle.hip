@@ -35,6 +35,9 @@ struct BodyLocal
     float delta[3];             // recentring of this step
     float six_volume;           // pressure
     uint32_t num_leaves;
+    uint32_t num_dyn_leaves;    // leaves of dynamic colliders this step
+    uint32_t num_dynamic;       // dynamic colliders of the context
+    DynStateDev *dyn;           // the body's row of dynamic-collider states
     uint32_t nf;
     const uint32_t *faces;
     const uint32_t *vf_begin;
@@ -133,9 +136,17 @@ __device__ __forceinline__ void bundle_update(
         L.update_position = B.update_position; L.allow_sleeping = B.allow_sleeping;
         L.nf = B.pressure > 0.0f ? B.s.num_faces : 0u;
         L.faces = B.s.faces; L.vf_begin = B.s.vf_begin; L.vf_list = B.s.vf_list;
+        L.dyn = B.dyn;
+        L.num_dynamic = B.dyn ? B.num_dynamic : 0u;
+        for (uint32_t k = 0; k < L.num_dynamic; ++k)
+            clear_dynamic(L.dyn[k]);
         st[b] = states[G.bodies[b].body];
     }
     __syncthreads();
+    // dynamic colliders anywhere in the bundle (workgroup-uniform)
+    bool any_dynamic = false;
+    for (uint32_t b = 0; b < nb; ++b)
+        any_dynamic |= loc[b].num_dynamic > 0;
 
     const V3 gravity_world = v3(gx, gy, gz);
     const float step_dt = dt / (float)steps;                        // PhysicsSystem.cpp:215
@@ -204,17 +215,21 @@ __device__ __forceinline__ void bundle_update(
                     out.type = -1;
                     continue;
                 }
-                make_leaf(cd, inversed_rotation_translation(com), L.friction, L.restitution, out);
+                make_leaf(cd, inversed_rotation_translation(com), L.friction, L.restitution, out, L.dyn);
             }
             sync_global();
             // colliding list of each body in collider (broadphase) order
             for (uint32_t b = tid; b < nb; b += BLOCK)
             {
-                uint32_t n = 0;
+                uint32_t n = 0, nd = 0;
                 for (uint32_t c = 0; c < nc; ++c)
                     if (leaves_full[b * nc + c].type >= 0)
+                    {
+                        nd += leaves_full[b * nc + c].dyn >= 0 ? 1u : 0u;
                         leaves[b * nc + n++] = leaves_full[b * nc + c];
+                    }
                 loc[b].num_leaves = n;
+                loc[b].num_dyn_leaves = nd;
             }
             sync_global();
 
@@ -344,12 +359,12 @@ __device__ __forceinline__ void bundle_update(
                 }
 
                 // ---- collision + velocity (SoftBodyMotionProperties.cpp:700-825), fused with the
-                // next integration (no pressure) or the per-vertex part of UpdateSoftBodyState
-                for (uint32_t i = tid; i < nv; i += BLOCK)
-                {
+                // next integration (no pressure) or the per-vertex part of UpdateSoftBodyState.
+                // A vertex in contact with a dynamic collider changes that body's velocities,
+                // which the next such vertex reads: the parallel pass marks these vertices
+                // (contact 2) and leaves them; one lane per body then runs them in vertex order.
+                auto collide_update = [&](uint32_t i, bool in_order) {
                     const uint32_t b = body_of(i);
-                    if (!st[b].active)
-                        continue;
                     const BodyLocal &L = loc[b];
                     float4 p = pget(X, i);
                     V3 x = xyz(p);
@@ -366,22 +381,33 @@ __device__ __forceinline__ void bundle_update(
                             float projected_distance = -signed_distance(plane, x) + L.vertex_radius;
                             if (projected_distance > 0.0f)
                             {
-                                G.contact[i] = 1;
                                 const Leaf &cs = leaves[b * nc + idx];
+                                if (cs.dyn >= 0 && !in_order)
+                                {
+                                    G.contact[i] = 2;
+                                    return;
+                                }
+                                G.contact[i] = 1;
                                 V3 n = plane.n;
                                 x = x + n * projected_distance;
-                                V3 v_normal = n * dot(n, v);
-                                V3 v_tangential = v - v_normal;
-                                float vtl = len(v_tangential);
-                                if (vtl > 0.0f)
+                                if (cs.dyn >= 0)
+                                    dynamic_contact(L.dyn[cs.dyn], col3(cs.t, 3), x, v, prev_v, n, projected_distance, p.w,
+                                                    cs.friction, cs.restitution, L.dt_sub, L.restitution_threshold);
+                                else
                                 {
-                                    float f = cs.friction * projected_distance / (vtl * L.dt_sub);
-                                    v = v - v_tangential * (f < 1.0f ? f : 1.0f);
+                                    V3 v_normal = n * dot(n, v);
+                                    V3 v_tangential = v - v_normal;
+                                    float vtl = len(v_tangential);
+                                    if (vtl > 0.0f)
+                                    {
+                                        float f = cs.friction * projected_distance / (vtl * L.dt_sub);
+                                        v = v - v_tangential * (f < 1.0f ? f : 1.0f);
+                                    }
+                                    v = v - v_normal;
+                                    float prev_v_normal = dot(prev_v, n);
+                                    if (prev_v_normal < L.restitution_threshold)
+                                        v = v - (cs.restitution * prev_v_normal) * n;
                                 }
-                                v = v - v_normal;
-                                float prev_v_normal = dot(prev_v, n);
-                                if (prev_v_normal < L.restitution_threshold)
-                                    v = v - (cs.restitution * prev_v_normal) * n;
                             }
                         }
                     }
@@ -407,6 +433,33 @@ __device__ __forceinline__ void bundle_update(
                         G.idx[i] = -1;
                         G.contact[i] = 0;
                     }
+                };
+                for (uint32_t i = tid; i < nv; i += BLOCK)
+                    if (st[body_of(i)].active)
+                        collide_update(i, false);
+                if (any_dynamic)
+                {
+                    // the marks and the dynamic states were written by other wavefronts
+                    sync_global();
+                    for (uint32_t b = wave; b < nb; b += (BLOCK / kWave))
+                    {
+                        if (loc[b].num_dyn_leaves == 0 || !st[b].active)
+                            continue;
+                        const BundleBodyDev bb = G.bodies[b];
+                        for (uint32_t k0 = 0; k0 < bb.nv; k0 += kWave)
+                        {
+                            const uint32_t k = k0 + (uint32_t)lane;
+                            unsigned long long marked = __ballot(k < bb.nv && G.contact[bb.voff + k] == 2);
+                            while (marked)
+                            {
+                                const uint32_t j = (uint32_t)__builtin_ctzll(marked);
+                                marked &= marked - 1ull;
+                                if (lane == 0)
+                                    collide_update(bb.voff + k0 + j, true);
+                            }
+                        }
+                    }
+                    sync_global();
                 }
                 __syncthreads();
                 if (!last && G.has_pressure)
@@ -461,6 +514,8 @@ __device__ __forceinline__ void bundle_update(
                 if (lane == 0)
                 {
                     M44 com = rotation_translation(s.rot[0], s.rot[1], s.rot[2], s.rot[3], v3(s.pos[0], s.pos[1], s.pos[2]));
+                    for (uint32_t k = 0; k < L.num_dynamic; ++k)
+                        finish_dynamic(L.dyn[k], com);
                     V3 bmn = v3(vals[0], vals[1], vals[2]), pmn = v3(vals[3], vals[4], vals[5]);
                     V3 bmx = v3(vals[6], vals[7], vals[8]), pmx = v3(vals[9], vals[10], vals[11]);
                     float divider = (float)(bb.nv > 1 ? bb.nv : 1u);

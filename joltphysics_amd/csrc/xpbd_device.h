@@ -25,6 +25,7 @@ struct Leaf
     int32_t type;
     float friction, restitution;
     const HullDev *hull;    // convex hull tables (device memory)
+    int32_t dyn;            // dynamic collider: rank into the body's DynStateDev row; -1 otherwise
 };
 
 __device__ __forceinline__ V3 xyz(float4 p) { return v3(p.x, p.y, p.z); }
@@ -602,10 +603,120 @@ __device__ __forceinline__ void collide_vertex(const Leaf *leaves, uint32_t num_
     }
 }
 
+// ---------------------------------------------------------------- dynamic colliders
+__device__ __forceinline__ V3 ld3(const float *p) { return v3(p[0], p[1], p[2]); }
+__device__ __forceinline__ void st3(float *p, V3 v) { p[0] = v.x; p[1] = v.y; p[2] = v.z; }
+
+// The inverse inertia as a Mat44 with zero translation, so that M * v adds the
+// zero column as Mat44 * Vec3 does (Mat44.inl:295-301).
+__device__ __forceinline__ M44 inv_inertia_matrix(const DynStateDev &D)
+{
+    M44 m;
+    for (int i = 0; i < 3; ++i)
+    {
+        m.c[i][0] = D.inv_inertia[3 * i]; m.c[i][1] = D.inv_inertia[3 * i + 1]; m.c[i][2] = D.inv_inertia[3 * i + 2];
+        m.c[i][3] = 0.0f;
+    }
+    m.c[3][0] = 0.0f; m.c[3][1] = 0.0f; m.c[3][2] = 0.0f; m.c[3][3] = 1.0f;
+    return m;
+}
+
+// The CollidingShape fields of a dynamic body (SoftBodyMotionProperties.cpp:224-233, no
+// contact listener: every scale 1): inverse mass, inverse inertia for the relative
+// rotation (MotionProperties::GetInverseInertiaForRotation, MotionProperties.inl:66-82,
+// all angular DOFs allowed) and the velocities in soft-body space.
+__device__ __forceinline__ void init_dynamic(const ColliderDev &c, const M44 &inv_com, const M44 &rel, DynStateDev &D)
+{
+    // rotation = rel.Multiply3x3(sRotation(mInertiaRotation)) (Mat44.inl:428-447)
+    const M44 ir = rotation_translation(c.inertia_rot[0], c.inertia_rot[1], c.inertia_rot[2], c.inertia_rot[3], v3(0.0f, 0.0f, 0.0f));
+    V3 rc[3];
+    for (int i = 0; i < 3; ++i)
+        rc[i] = (col3(rel, 0) * ir.c[i][0] + col3(rel, 1) * ir.c[i][1]) + col3(rel, 2) * ir.c[i][2];
+    // columns of rotation scaled by the inverse inertia diagonal
+    const V3 sc[3] = { c.inv_inertia_diag[0] * rc[0], c.inv_inertia_diag[1] * rc[1], c.inv_inertia_diag[2] * rc[2] };
+    // rotation.Multiply3x3RightTransposed(scaled) (Mat44.inl:495-507)
+    for (int j = 0; j < 3; ++j)
+    {
+        const V3 col = (rc[0] * comp(sc[0], j) + rc[1] * comp(sc[1], j)) + rc[2] * comp(sc[2], j);
+        D.inv_inertia[3 * j] = col.x; D.inv_inertia[3 * j + 1] = col.y; D.inv_inertia[3 * j + 2] = col.z;
+    }
+    D.inv_mass = c.inv_mass;
+    const V3 lin = mul3x3(inv_com, ld3(c.lin_vel)), ang = mul3x3(inv_com, ld3(c.ang_vel));
+    st3(D.lin, lin); st3(D.orig_lin, lin);
+    st3(D.ang, ang); st3(D.orig_ang, ang);
+    D.updated = 0;
+}
+
+// The dynamic-body branch of ApplyCollisionConstraintsAndUpdateVelocities
+// (SoftBodyMotionProperties.cpp:752-797) for one vertex in contact: friction and
+// restitution impulse shared between the vertex and the body, whose velocities the
+// next vertex in contact sees.  x: the vertex after the push out, v: its XPBD velocity.
+__device__ __forceinline__ void dynamic_contact(DynStateDev &D, V3 com_t, V3 x, V3 &v, V3 prev_v, V3 n, float projected_distance,
+                                                float w, float friction, float restitution, float dt, float restitution_threshold)
+{
+    const V3 r2 = x - com_t;
+    V3 lin = ld3(D.lin), ang = ld3(D.ang);
+    const V3 v2 = lin + cross(ang, r2);
+    const V3 relative_velocity = v - v2;
+    const V3 v_normal = n * dot(n, relative_velocity);
+    const V3 v_tangential = relative_velocity - v_normal;
+    const float v_tangential_length = len(v_tangential);
+    const float vertex_inv_mass = 1.0f * w;
+    const M44 I = inv_inertia_matrix(D);
+    const V3 r2_cross_n = cross(r2, n);
+    const float w2 = D.inv_mass + dot(r2_cross_n, mul_point(I, r2_cross_n));
+    const float w1_plus_w2 = vertex_inv_mass + w2;
+    if (w1_plus_w2 > 0.0f)
+    {
+        V3 dv;
+        if (v_tangential_length > 0.0f)
+        {
+            const float f = friction * projected_distance / (v_tangential_length * dt);
+            dv = v_tangential * (f < 1.0f ? f : 1.0f);
+        }
+        else
+            dv = v3(0.0f, 0.0f, 0.0f);
+        dv = dv + v_normal;
+        const float prev_v_normal = dot(prev_v - v2, n);
+        if (prev_v_normal < restitution_threshold)
+            dv = dv + (restitution * prev_v_normal) * n;
+        const V3 p = dv / w1_plus_w2;
+        v = v - p * vertex_inv_mass;
+        lin = lin + p * D.inv_mass;
+        ang = ang + mul_point(I, cross(r2, p));
+        st3(D.lin, lin);
+        st3(D.ang, ang);
+        D.updated = 1;
+    }
+}
+
+// UpdateRigidBodyVelocities for one soft body (SoftBodyMotionProperties.cpp:923-935):
+// the velocity changes of the step in world space, added to the call's sums.
+__device__ __forceinline__ void finish_dynamic(DynStateDev &D, const M44 &com)
+{
+    if (D.updated)
+    {
+        st3(D.acc_lin, ld3(D.acc_lin) + mul3x3(com, ld3(D.lin) - ld3(D.orig_lin)));
+        st3(D.acc_ang, ld3(D.acc_ang) + mul3x3(com, ld3(D.ang) - ld3(D.orig_ang)));
+        D.any_updated = 1;
+        D.updated = 0;
+    }
+}
+
+__device__ __forceinline__ void clear_dynamic(DynStateDev &D)
+{
+    st3(D.acc_lin, v3(0.0f, 0.0f, 0.0f));
+    st3(D.acc_ang, v3(0.0f, 0.0f, 0.0f));
+    D.any_updated = 0;
+    D.updated = 0;
+}
+
 // Leaf of collider c in the soft body's local space (SoftBodyMotionProperties.cpp:174-186:
 // com = (inverse soft body transform * collider transform), then the leaf
-// transform is rebuilt from com's quaternion and translation).
-__device__ __forceinline__ void make_leaf(const ColliderDev &c, const M44 &inv_com, float body_friction, float body_restitution, Leaf &out)
+// transform is rebuilt from com's quaternion and translation).  A dynamic collider
+// also initialises its entry of the body's DynStateDev row.
+__device__ __forceinline__ void make_leaf(const ColliderDev &c, const M44 &inv_com, float body_friction, float body_restitution, Leaf &out,
+                                          DynStateDev *dyn_row)
 {
     M44 ccom = rotation_translation(c.rot[0], c.rot[1], c.rot[2], c.rot[3], v3(c.pos[0], c.pos[1], c.pos[2]));
     M44 rel = mul(inv_com, ccom);
@@ -614,6 +725,12 @@ __device__ __forceinline__ void make_leaf(const ColliderDev &c, const M44 &inv_c
     M44 t = rotation_translation(q[0], q[1], q[2], q[3], col3(rel, 3));
     out.t = t;
     out.type = c.type;
+    out.dyn = -1;
+    if (c.motion == 2 && dyn_row)
+    {
+        out.dyn = c.dyn;
+        init_dynamic(c, inv_com, rel, dyn_row[c.dyn]);
+    }
     out.friction = sqrtf(body_friction * c.friction);                                 // ContactConstraintManager.h:554
     out.restitution = body_restitution > c.restitution ? body_restitution : c.restitution; // ContactConstraintManager.h:555
     switch (c.type)

@@ -89,6 +89,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
     __shared__ BodyStateDev st;
     __shared__ float sh_scalars[8];
     __shared__ uint32_t sh_num_leaves;
+    __shared__ uint32_t sh_num_dyn_leaves;
 
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
@@ -133,6 +134,12 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
     if (RESIDENT)
         for (uint32_t i = tid; i < nv; i += kBlock)
             X[i] = B.x[i];
+    // dynamic colliders: the call's velocity-change sums start at zero
+    DynStateDev *const dyn = B.dyn;
+    const bool has_dynamic = dyn != nullptr && B.num_dynamic > 0;
+    if (has_dynamic)
+        for (uint32_t k = tid; k < B.num_dynamic; k += kBlock)
+            clear_dynamic(dyn[k]);
     __syncthreads();
 
     const V3 gravity_world = v3(gx, gy, gz);
@@ -182,13 +189,16 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                 __syncthreads();
                 if (tid == 0)
                 {
-                    uint32_t n = 0;
+                    uint32_t n = 0, nd = 0;
                     for (uint32_t ci = 0; ci < num_colliders; ++ci)
                     {
                         if (slots[ci])
                         {
                             if (n < max_leaves)
+                            {
                                 slots[ci] = (int32_t)n;
+                                nd += colliders[ci].motion == 2 ? 1u : 0u;
+                            }
                             else
                             {
                                 slots[ci] = -1;
@@ -200,15 +210,21 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                             slots[ci] = -1;
                     }
                     sh_num_leaves = n < max_leaves ? n : max_leaves;
+                    sh_num_dyn_leaves = has_dynamic ? nd : 0u;
                 }
                 __syncthreads();
                 const M44 inv_com = inversed_rotation_translation(com);
                 for (uint32_t ci = tid; ci < num_colliders; ci += kBlock)
                     if (slots[ci] >= 0)
-                        make_leaf(colliders[ci], inv_com, B.friction, B.restitution, leaves[slots[ci]]);
-                __syncthreads();
+                        make_leaf(colliders[ci], inv_com, B.friction, B.restitution, leaves[slots[ci]], dyn);
+                // dynamic states are read by the in-order contact pass of another wavefront
+                if (has_dynamic)
+                    sync_global();
+                else
+                    __syncthreads();
             }
             const uint32_t num_leaves = sh_num_leaves;
+            const uint32_t num_dyn_leaves = sh_num_dyn_leaves;
 
             // DetermineCollisionPlanes (SoftBodyMotionProperties.cpp:257-265)
             if (num_leaves > 0)
@@ -327,11 +343,13 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                     }
                 }
 
-                // ApplyCollisionConstraintsAndUpdateVelocities (SoftBodyMotionProperties.cpp:700-825),
-                // static-body branch; fused with the next iteration's integration when there is
-                // no pressure, or with UpdateSoftBodyState's per-vertex work after the last one.
-                for (uint32_t i = tid; i < nv; i += kBlock)
-                {
+                // ApplyCollisionConstraintsAndUpdateVelocities (SoftBodyMotionProperties.cpp:700-825);
+                // fused with the next iteration's integration when there is no pressure, or with
+                // UpdateSoftBodyState's per-vertex work after the last one.  A vertex in contact
+                // with a dynamic collider changes that body's velocities, which the next such
+                // vertex reads: the parallel pass marks these vertices (contact 2) and leaves
+                // them; thread 0 then runs them in vertex order.
+                auto collide_update = [&](uint32_t i, bool in_order) {
                     float4 p = P[i];
                     V3 x = xyz(p);
                     V3 v = xyz(B.v[i]);
@@ -347,22 +365,33 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                             float projected_distance = -signed_distance(plane, x) + B.vertex_radius;
                             if (projected_distance > 0.0f)
                             {
-                                B.contact[i] = 1;
                                 const Leaf &cs = leaves[idx];
+                                if (cs.dyn >= 0 && !in_order)
+                                {
+                                    B.contact[i] = 2;
+                                    return;
+                                }
+                                B.contact[i] = 1;
                                 V3 n = plane.n;
                                 x = x + n * projected_distance;
-                                V3 v_normal = n * dot(n, v);
-                                V3 v_tangential = v - v_normal;
-                                float vtl = len(v_tangential);
-                                if (vtl > 0.0f)
+                                if (cs.dyn >= 0)
+                                    dynamic_contact(dyn[cs.dyn], col3(cs.t, 3), x, v, prev_v, n, projected_distance, p.w,
+                                                    cs.friction, cs.restitution, dt_sub, restitution_threshold);
+                                else
                                 {
-                                    float f = cs.friction * projected_distance / (vtl * dt_sub);
-                                    v = v - v_tangential * (f < 1.0f ? f : 1.0f);
+                                    V3 v_normal = n * dot(n, v);
+                                    V3 v_tangential = v - v_normal;
+                                    float vtl = len(v_tangential);
+                                    if (vtl > 0.0f)
+                                    {
+                                        float f = cs.friction * projected_distance / (vtl * dt_sub);
+                                        v = v - v_tangential * (f < 1.0f ? f : 1.0f);
+                                    }
+                                    v = v - v_normal;
+                                    float prev_v_normal = dot(prev_v, n);
+                                    if (prev_v_normal < restitution_threshold)
+                                        v = v - (cs.restitution * prev_v_normal) * n;
                                 }
-                                v = v - v_normal;
-                                float prev_v_normal = dot(prev_v, n);
-                                if (prev_v_normal < restitution_threshold)
-                                    v = v - (cs.restitution * prev_v_normal) * n;
                             }
                         }
                     }
@@ -395,6 +424,27 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                         B.idx[i] = -1;
                         B.contact[i] = 0;
                     }
+                };
+                for (uint32_t i = tid; i < nv; i += kBlock)
+                    collide_update(i, false);
+                if (num_dyn_leaves > 0)
+                {
+                    // the marks were written by other wavefronts
+                    sync_global();
+                    if (wave == 0)
+                        for (uint32_t k0 = 0; k0 < nv; k0 += kWave)
+                        {
+                            const uint32_t k = k0 + (uint32_t)lane;
+                            unsigned long long marked = __ballot(k < nv && B.contact[k] == 2);
+                            while (marked)
+                            {
+                                const uint32_t j = (uint32_t)__builtin_ctzll(marked);
+                                marked &= marked - 1ull;
+                                if (lane == 0)
+                                    collide_update(k0 + j, true);
+                            }
+                        }
+                    sync_global();
                 }
                 sync_positions();
                 if (!last && has_pressure)
@@ -443,6 +493,9 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                     V3 av = mul3x3(com, v3(sums[3], sums[4], sums[5]) / divider);
                     st.lin[0] = lv.x; st.lin[1] = lv.y; st.lin[2] = lv.z;
                     st.ang[0] = av.x; st.ang[1] = av.y; st.ang[2] = av.z;
+                    if (has_dynamic)
+                        for (uint32_t k = 0; k < B.num_dynamic; ++k)
+                            finish_dynamic(dyn[k], com);
                     V3 delta = v3(0.0f, 0.0f, 0.0f);
                     if (B.update_position)
                     {

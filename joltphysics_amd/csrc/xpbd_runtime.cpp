@@ -162,6 +162,9 @@ struct xpbd_context
     DevBuf<float4> d_hull_vec;      // convex hulls: planes, then points, per hull
     DevBuf<uint32_t> d_hull_idx;    // convex hulls: face first, face count, vertex index, per hull
     DevBuf<HullDev> d_hulls;
+    uint32_t num_dynamic = 0;       // dynamic colliders of the last xpbd_set_colliders
+    DevBuf<DynStateDev> d_dyn;      // bodies x num_dynamic
+    uint32_t dyn_stride = 0;        // num_dynamic the rows of d_dyn are laid out for
     DevBuf<int32_t> d_error;
     float gravity[3] = { 0.0f, -9.81f, 0.0f };       // PhysicsSystem.h:392 default
     bool profiling = false;
@@ -1231,6 +1234,7 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
         return Fail(XPBD_E_INVALID, "null argument");
     HIP_TRY(hipSetDevice(ctx->device));
     std::vector<ColliderDev> v(n);
+    uint32_t num_dynamic = 0;
     // convex hull tables: validated here, flattened into two pools, pointers set after upload
     std::vector<float4> hull_vec;
     std::vector<uint32_t> hull_idx;
@@ -1274,8 +1278,21 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
             o.collider = i;
             hull_offsets.push_back(o);
         }
+        if (c[i].motion_type < 0 || c[i].motion_type > 2)
+            return Fail(XPBD_E_INVALID, "collider motion type must be 0 (static), 1 (kinematic) or 2 (dynamic)");
         v[i].type = c[i].type;
         v[i].hull = nullptr;
+        v[i].motion = c[i].motion_type;
+        v[i].dyn = -1;
+        if (c[i].motion_type == 2)
+        {
+            v[i].dyn = (int32_t)num_dynamic++;
+            v[i].inv_mass = c[i].inv_mass;
+            memcpy(v[i].inv_inertia_diag, c[i].inv_inertia_diagonal, sizeof(v[i].inv_inertia_diag));
+            memcpy(v[i].inertia_rot, c[i].inertia_rotation, sizeof(v[i].inertia_rot));
+            memcpy(v[i].lin_vel, c[i].linear_velocity, sizeof(v[i].lin_vel));
+            memcpy(v[i].ang_vel, c[i].angular_velocity, sizeof(v[i].ang_vel));
+        }
         memcpy(v[i].params, c[i].params, sizeof(v[i].params));
         memcpy(v[i].pos, c[i].position, sizeof(v[i].pos));
         memcpy(v[i].rot, c[i].rotation, sizeof(v[i].rot));
@@ -1303,6 +1320,7 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
     if (v.size() != ctx->colliders.size())
         ctx->layout_dirty = true;
     ctx->colliders = v;
+    ctx->num_dynamic = num_dynamic;
     HIP_TRY(ctx->d_colliders.upload(v));
     return XPBD_OK;
 }
@@ -1384,6 +1402,26 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
             L = &launches.back();
         }
         L->bodies.push_back(i);
+    }
+    // dynamic colliders: one row of states per body (the kernels clear a body's row at
+    // the start of every call; zeroed here too, for bodies no launch runs)
+    {
+        const uint32_t nd = ctx->num_dynamic;
+        const size_t need = nd ? ctx->bodies.size() * nd : 0;
+        if (ctx->d_dyn.n != need || ctx->dyn_stride != nd)
+        {
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            HIP_TRY(ctx->d_dyn.alloc(need));
+            if (need)
+                HIP_TRY(hipMemset(ctx->d_dyn.p, 0, sizeof(DynStateDev) * need));
+            for (size_t i = 0; i < ctx->bodies.size(); ++i)
+            {
+                ctx->bodies[i]->dev.dyn = nd ? ctx->d_dyn.p + i * nd : nullptr;
+                ctx->bodies[i]->dev.num_dynamic = nd;
+            }
+            ctx->dyn_stride = nd;
+            ctx->tables_dirty = true;
+        }
     }
     if ((rc = SyncTables(ctx)) != XPBD_OK)
         return rc;
@@ -1572,6 +1610,34 @@ int xpbd_body_state(xpbd_context *ctx, int32_t body, float out[14])
     memcpy(out + 9, s.bmax, 12);
     out[12] = (float)s.active;
     out[13] = s.sleep_timer;
+    return XPBD_OK;
+}
+
+int xpbd_body_velocity_deltas(xpbd_context *ctx, int32_t body, uint32_t num_colliders, float *linear, float *angular,
+                              int32_t *updated)
+{
+    if (!ctx || body < 0 || body >= (int32_t)ctx->bodies.size())
+        return Fail(XPBD_E_INVALID, "invalid body");
+    if (num_colliders != ctx->colliders.size())
+        return Fail(XPBD_E_INVALID, "num_colliders differs from the last xpbd_set_colliders");
+    int rc = xpbd_synchronize(ctx);
+    if (rc != XPBD_OK)
+        return rc;
+    std::vector<DynStateDev> row(ctx->dyn_stride);
+    if (ctx->dyn_stride && ctx->d_dyn.n)
+        HIP_TRY(hipMemcpy(row.data(), ctx->d_dyn.p + (size_t)body * ctx->dyn_stride, sizeof(DynStateDev) * row.size(),
+                          hipMemcpyDeviceToHost));
+    for (uint32_t c = 0; c < num_colliders; ++c)
+    {
+        const int32_t k = ctx->colliders[c].dyn;
+        const bool have = k >= 0 && (uint32_t)k < row.size();
+        for (int j = 0; j < 3; ++j)
+        {
+            if (linear) linear[3 * c + j] = have ? row[k].acc_lin[j] : 0.0f;
+            if (angular) angular[3 * c + j] = have ? row[k].acc_ang[j] : 0.0f;
+        }
+        if (updated) updated[c] = have ? row[k].any_updated : 0;
+    }
     return XPBD_OK;
 }
 

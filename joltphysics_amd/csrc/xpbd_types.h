@@ -46,6 +46,8 @@ struct SettingsDev
     uint32_t window_capacity;   // largest gathered window (vertices)
 };
 
+struct DynStateDev;
+
 struct BodyDev
 {
     SettingsDev s;
@@ -59,6 +61,8 @@ struct BodyDev
     float linear_damping, max_linear_velocity, restitution, friction, pressure, gravity_factor, vertex_radius;
     int32_t update_position, allow_sleeping;
     uint32_t stage_records;     // 1: records + level table are copied into LDS once per launch
+    DynStateDev *dyn;           // this body's row of dynamic-collider states (num_dynamic entries), or null
+    uint32_t num_dynamic;
 };
 
 // Per-body dynamic state kept on the device between launches.
@@ -93,6 +97,27 @@ struct ColliderDev
     float rot[4];
     float friction, restitution;
     float bounds[6];
+    // motion (EMotionType: 0 static, 1 kinematic, 2 dynamic); dynamic bodies only:
+    int32_t motion;
+    int32_t dyn;                // rank among the dynamic colliders
+    float inv_mass;
+    float inv_inertia_diag[3];  // MotionProperties::GetInverseInertiaDiagonal
+    float inertia_rot[4];       // MotionProperties::GetInertiaRotation
+    float lin_vel[3], ang_vel[3];   // world space
+};
+
+// State of one dynamic collider as one soft body sees it during a collision step
+// (the reference's CollidingShape velocity fields, SoftBodyMotionProperties.h:190-212),
+// plus what the step changed, summed over the steps of an update call.
+struct DynStateDev
+{
+    float inv_inertia[9];       // mInvInertia, soft-body local, column major 3x3
+    float inv_mass;
+    float lin[3], ang[3];       // mLinearVelocity / mAngularVelocity (soft-body local)
+    float orig_lin[3], orig_ang[3];
+    float acc_lin[3], acc_ang[3];   // world-space velocity changes, summed over the call's steps
+    int32_t updated;            // mUpdateVelocities of the current step
+    int32_t any_updated;        // some step of the call set mUpdateVelocities
 };
 
 } // namespace xpbd

@@ -43,6 +43,8 @@ LRA_NONE, LRA_EUCLIDEAN, LRA_GEODESIC = 0, 1, 2
 # Collider shape types of the C ABI
 SHAPE_PLANE, SHAPE_SPHERE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_CYLINDER, SHAPE_TAPERED_CAPSULE = 0, 1, 2, 3, 4, 5
 SHAPE_CONVEX_HULL = 6
+# EMotionType (MotionType.h:10-15)
+MOTION_STATIC, MOTION_KINEMATIC, MOTION_DYNAMIC = 0, 1, 2
 
 
 def _empty(dtype):
@@ -173,6 +175,14 @@ class Collider:
     device_params: tuple | None = None
     com: tuple | None = None
     hull: ConvexHull | None = None
+    # motion: static / kinematic colliders take the static contact branch; a dynamic
+    # one needs its mass properties (MotionProperties) and world velocities
+    motion_type: int = MOTION_STATIC
+    inv_mass: float = 0.0
+    inv_inertia_diagonal: tuple = (0.0, 0.0, 0.0)
+    inertia_rotation: tuple = (0.0, 0.0, 0.0, 1.0)
+    linear_velocity: tuple = (0.0, 0.0, 0.0)
+    angular_velocity: tuple = (0.0, 0.0, 0.0)
 
     def param_array(self):
         if self.type == SHAPE_CONVEX_HULL:
@@ -228,3 +238,22 @@ def _rotate(q, v):
     u = np.array([x, y, z])
     v = np.asarray(v, np.float64)
     return 2.0 * np.dot(u, v) * u + (w * w - np.dot(u, u)) * v + 2.0 * w * np.cross(u, v)
+
+
+# BodyCreationSettings defaults (BodyCreationSettings.h:111-112)
+MAX_LINEAR_VELOCITY = np.float32(500.0)
+MAX_ANGULAR_VELOCITY = np.float32(np.float32(np.float32(0.25) * np.float32(3.14159265358979323846)) * np.float32(60.0))
+
+
+def add_velocity_clamped(velocity, delta, max_velocity):
+    """BodyInterface::AddLinearAndAngularVelocity for one of the two velocities
+    (BodyInterface.cpp:640-655): v + delta, clamped to max_velocity in length as
+    MotionProperties::ClampLinearVelocity / ClampAngularVelocity do
+    (MotionProperties.inl:23-41), in float32."""
+    f = np.float32
+    v = np.asarray(velocity, f) + np.asarray(delta, f)
+    len_sq = f(f(v[0] * v[0]) + f(v[1] * v[1])) + f(v[2] * v[2])
+    m = f(max_velocity)
+    if len_sq > f(m * m):
+        v = v * f(m / np.sqrt(len_sq))
+    return tuple(float(x) for x in v)

@@ -17,7 +17,8 @@ import sys
 
 import numpy as np
 
-from .records import (BEND, EDGE, FACE, GROUP, LRA, SHAPE_CONVEX_HULL, VERTEX, VOLUME, BodyParams, SettingsArrays)
+from .records import (BEND, EDGE, FACE, GROUP, LRA, MOTION_DYNAMIC, SHAPE_CONVEX_HULL, VERTEX, VOLUME, BodyParams,
+                      SettingsArrays)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libjolt_xpbd.so")
@@ -59,7 +60,10 @@ class _ConvexHull(C.Structure):
 class _ColliderDesc(C.Structure):
     _fields_ = [("type", C.c_int32), ("params", C.c_float * 8), ("position", C.c_float * 3),
                 ("rotation", C.c_float * 4), ("friction", C.c_float), ("restitution", C.c_float),
-                ("world_bounds", C.c_float * 6), ("hull", C.POINTER(_ConvexHull))]
+                ("world_bounds", C.c_float * 6), ("hull", C.POINTER(_ConvexHull)),
+                ("motion_type", C.c_int32), ("inv_mass", C.c_float), ("inv_inertia_diagonal", C.c_float * 3),
+                ("inertia_rotation", C.c_float * 4), ("linear_velocity", C.c_float * 3),
+                ("angular_velocity", C.c_float * 3)]
 
 
 class _RolloutDesc(C.Structure):
@@ -91,7 +95,7 @@ EXPORTED_SYMBOLS = (
     "xpbd_settings_build", "xpbd_built_counts", "xpbd_built_read", "xpbd_built_free",
     "xpbd_build_last_error", "xpbd_body_create",
     "xpbd_set_colliders", "xpbd_set_gravity", "xpbd_update", "xpbd_synchronize", "xpbd_body_read",
-    "xpbd_body_write", "xpbd_body_state", "xpbd_profile_enable", "xpbd_profile_read",
+    "xpbd_body_write", "xpbd_body_state", "xpbd_body_velocity_deltas", "xpbd_profile_enable", "xpbd_profile_read",
     "xpbd_rollout_create", "xpbd_rollout_destroy", "xpbd_rollout_sizes", "xpbd_rollout_forward",
     "xpbd_rollout_forward_f64", "xpbd_rollout_backward",
 )
@@ -138,6 +142,7 @@ def load_library(path=LIB_PATH):
     lib.xpbd_body_read.argtypes = [vp, i32] + [vp] * 8
     lib.xpbd_body_write.argtypes = [vp, i32, vp, vp]
     lib.xpbd_body_state.argtypes = [vp, i32, vp]
+    lib.xpbd_body_velocity_deltas.argtypes = [vp, i32, u32, vp, vp, vp]
     lib.xpbd_profile_enable.argtypes = [vp, i32]
     lib.xpbd_profile_read.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_double)]
     lib.xpbd_rollout_create.argtypes = [vp, C.POINTER(_RolloutDesc), C.POINTER(vp)]
@@ -276,7 +281,29 @@ class Context:
                 raise XpbdError("collider world bounds are required")
             for k in range(6):
                 arr[i].world_bounds[k] = float(c.bounds[k])
+            arr[i].motion_type = int(c.motion_type)
+            if c.motion_type == MOTION_DYNAMIC:
+                arr[i].inv_mass = float(c.inv_mass)
+                for k in range(3):
+                    arr[i].inv_inertia_diagonal[k] = float(c.inv_inertia_diagonal[k])
+                    arr[i].linear_velocity[k] = float(c.linear_velocity[k])
+                    arr[i].angular_velocity[k] = float(c.angular_velocity[k])
+                for k in range(4):
+                    arr[i].inertia_rotation[k] = float(c.inertia_rotation[k])
         _check(self.lib.xpbd_set_colliders(self.h, arr, n))
+        self._num_colliders = n
+
+    def velocity_deltas(self, body):
+        """(linear (n, 3), angular (n, 3), updated (n,)): the world-space velocity changes soft
+        body `body` applied to each collider during the last update
+        (SoftBodyMotionProperties::UpdateRigidBodyVelocities)."""
+        n = getattr(self, "_num_colliders", 0)
+        lin = np.zeros((max(n, 1), 3), np.float32)
+        ang = np.zeros((max(n, 1), 3), np.float32)
+        upd = np.zeros(max(n, 1), np.int32)
+        _check(self.lib.xpbd_body_velocity_deltas(self.h, int(body), n, lin.ctypes.data, ang.ctypes.data,
+                                                  upd.ctypes.data))
+        return lin[:n], ang[:n], upd[:n]
 
     def create_settings(self, s: SettingsArrays) -> Settings:
         v = np.ascontiguousarray(s.vertices, VERTEX)

@@ -320,9 +320,8 @@ void jo_world_set_gravity(void *wp, float gx, float gy, float gz)
 // 2 box {hx,hy,hz}, 3 capsule {halfHeight, r}, 4 cylinder {halfHeight, r},
 // 5 tapered capsule {halfHeightOfTaperedCylinder, topRadius, bottomRadius}.
 // pos[3], rot[4] (x,y,z,w): the body's position and rotation.
-int jo_world_add_collider(void *wp, int type, const float *params, const float *pos, const float *rot, float friction, float restitution)
+static Ref<Shape> MakeShape(int type, const float *params)
 {
-	World *w = (World *)wp;
 	Ref<Shape> shape;
 	switch (type)
 	{
@@ -336,12 +335,21 @@ int jo_world_add_collider(void *wp, int type, const float *params, const float *
 			TaperedCapsuleShapeSettings ts(params[0], params[1], params[2]);
 			Shape::ShapeResult r = ts.Create();
 			if (r.HasError() || r.Get()->GetSubType() != EShapeSubType::TaperedCapsule)
-				return -1;
+				return nullptr;
 			shape = r.Get();
 		}
 		break;
-	default: return -1;
+	default: return nullptr;
 	}
+	return shape;
+}
+
+int jo_world_add_collider(void *wp, int type, const float *params, const float *pos, const float *rot, float friction, float restitution)
+{
+	World *w = (World *)wp;
+	Ref<Shape> shape = MakeShape(type, params);
+	if (shape == nullptr)
+		return -1;
 	BodyCreationSettings bcs(shape, RVec3(pos[0], pos[1], pos[2]), Quat(rot[0], rot[1], rot[2], rot[3]), EMotionType::Static, Layers::STATIC);
 	bcs.mFriction = friction;
 	bcs.mRestitution = restitution;
@@ -379,6 +387,58 @@ int jo_world_add_hull(void *wp, const float *points, int num_points, const float
 	w->mColliderParams.push_back(std::array<float, 8>{});
 	w->mColliderTypes.push_back(6);
 	return int(w->mColliders.size() - 1);
+}
+
+// Dynamic collider (EMotionType::Dynamic, moving layer) of shape `type` / `params` as in
+// jo_world_add_collider (type 0, the plane, excluded), mass properties from the shape at
+// density 1000 (BodyCreationSettings defaults), initial world velocities lin[3], ang[3].
+int jo_world_add_dynamic(void *wp, int type, const float *params, const float *pos, const float *rot, float friction, float restitution,
+						 const float *lin, const float *ang)
+{
+	World *w = (World *)wp;
+	Ref<Shape> shape = type == 0 ? nullptr : MakeShape(type, params);
+	if (shape == nullptr)
+		return -1;
+	BodyCreationSettings bcs(shape, RVec3(pos[0], pos[1], pos[2]), Quat(rot[0], rot[1], rot[2], rot[3]), EMotionType::Dynamic, Layers::MOVING);
+	bcs.mFriction = friction;
+	bcs.mRestitution = restitution;
+	bcs.mLinearVelocity = Vec3(lin[0], lin[1], lin[2]);
+	bcs.mAngularVelocity = Vec3(ang[0], ang[1], ang[2]);
+	BodyID id = w->mSystem.GetBodyInterface().CreateAndAddBody(bcs, EActivation::DontActivate);
+	if (id.IsInvalid())
+		return -1;
+	w->mColliders.push_back(id);
+	std::array<float, 8> kept{};
+	const int np = type == 1 ? 1 : type == 2 ? 3 : type == 5 ? 3 : 2;
+	for (int k = 0; k < np; ++k) kept[k] = params[k];
+	w->mColliderParams.push_back(kept);
+	w->mColliderTypes.push_back(type);
+	return int(w->mColliders.size() - 1);
+}
+
+// Motion of collider `collider`: returns EMotionType; for a dynamic body out16 =
+// {inverse mass, inverse inertia diagonal xyz, inertia rotation xyzw, linear velocity xyz,
+// angular velocity xyz} (MotionProperties getters; velocities in world space, current).
+int jo_world_collider_motion(void *wp, int collider, float *out16)
+{
+	World *w = (World *)wp;
+	BodyLockRead lock(w->mSystem.GetBodyLockInterface(), w->mColliders[collider]);
+	if (!lock.Succeeded())
+		return -1;
+	const Body &b = lock.GetBody();
+	for (int k = 0; k < 16; ++k) out16[k] = 0.0f;
+	if (b.GetMotionType() != EMotionType::Dynamic)
+		return int(b.GetMotionType());
+	const MotionProperties *mp = b.GetMotionProperties();
+	out16[0] = mp->GetInverseMass();
+	Vec3 d = mp->GetInverseInertiaDiagonal();
+	out16[1] = d.GetX(); out16[2] = d.GetY(); out16[3] = d.GetZ();
+	Quat q = mp->GetInertiaRotation();
+	out16[4] = q.GetX(); out16[5] = q.GetY(); out16[6] = q.GetZ(); out16[7] = q.GetW();
+	Vec3 l = b.GetLinearVelocity(), a = b.GetAngularVelocity();
+	out16[8] = l.GetX(); out16[9] = l.GetY(); out16[10] = l.GetZ();
+	out16[11] = a.GetX(); out16[12] = a.GetY(); out16[13] = a.GetZ();
+	return int(EMotionType::Dynamic);
 }
 
 // The hull as the shape holds it (ConvexHullShape.h:114-141): sizes[3] =

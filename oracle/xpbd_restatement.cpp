@@ -276,13 +276,25 @@ struct Collider
 	float pos[3], rot[4];
 	float friction, restitution;
 	float bounds[6];		// world space bounds of the static body (min xyz, max xyz)
+	// motion (EMotionType 0 static, 1 kinematic, 2 dynamic); dynamic: mass properties and
+	// world velocities, fixed for an update call
+	int motion = 0;
+	float inv_mass = 0, inv_inertia_diag[3] = { 0, 0, 0 }, inertia_rot[4] = { 0, 0, 0, 1 };
+	float lin_vel[3] = { 0, 0, 0 }, ang_vel[3] = { 0, 0, 0 };
 };
 
 template <class F> struct Leaf
 {
 	M44<F> transform;		// leaf shape transform in soft-body local space
 	const Collider *col;
+	size_t col_index;
 	F friction, restitution;
+	// CollidingShape fields of a dynamic body (SoftBodyMotionProperties.h:190-212)
+	bool dynamic = false;
+	F inv_mass = 0;
+	M44<F> inv_inertia;		// translation column zero
+	V3<F> com_translation, lin, ang, orig_lin, orig_ang;
+	bool update_velocities = false;
 };
 
 template <class F> struct Body
@@ -321,6 +333,10 @@ template <class F> struct Body
 	F dt_sub;
 	V3<F> disp_gravity;
 	std::vector<Leaf<F>> leaves;
+
+	// UpdateRigidBodyVelocities output per collider, summed over the steps of an update call
+	std::vector<V3<F>> dyn_lin, dyn_ang;
+	std::vector<int> dyn_updated;
 };
 
 template <class F> struct World
@@ -779,8 +795,41 @@ template <class F> void ProcessGroup(Body<F> &b, size_t g)
 	ApplyLRAs(b, p.lra_end, c.lra_end);
 }
 
-// ApplyCollisionConstraintsAndUpdateVelocities, static / kinematic branch
-// (SoftBodyMotionProperties.cpp:700-825).
+// Dynamic-body branch of ApplyCollisionConstraintsAndUpdateVelocities
+// (SoftBodyMotionProperties.cpp:752-797), every contact-settings scale 1.
+template <class F> void DynamicContact(Leaf<F> &cs, V3<F> x, V3<F> &v, V3<F> prev_v, V3<F> n, F pd, F w, F dt, F restitution_threshold)
+{
+	V3<F> r2 = x - cs.com_translation;
+	V3<F> v2 = cs.lin + Cross(cs.ang, r2); // CollidingShape::GetPointVelocity
+	V3<F> relative_velocity = v - v2;
+	V3<F> v_normal = n * Dot(n, relative_velocity);
+	V3<F> v_tangential = relative_velocity - v_normal;
+	F v_tangential_length = Len(v_tangential);
+	F vertex_inv_mass = F(1) * w;
+	V3<F> r2_cross_n = Cross(r2, n);
+	F w2 = cs.inv_mass + Dot(r2_cross_n, MulPoint(cs.inv_inertia, r2_cross_n));
+	F w1_plus_w2 = vertex_inv_mass + w2;
+	if (w1_plus_w2 > F(0))
+	{
+		V3<F> dv;
+		if (v_tangential_length > F(0))
+			dv = v_tangential * std::min(cs.friction * pd / (v_tangential_length * dt), F(1));
+		else
+			dv = V3<F>{ 0, 0, 0 };
+		dv = dv + v_normal;
+		F prev_v_normal = Dot(prev_v - v2, n);
+		if (prev_v_normal < restitution_threshold)
+			dv = dv + (cs.restitution * prev_v_normal) * n;
+		V3<F> p = dv / w1_plus_w2;
+		v = v - p * vertex_inv_mass;
+		cs.lin = cs.lin + p * cs.inv_mass;
+		cs.ang = cs.ang + MulPoint(cs.inv_inertia, Cross(r2, p));
+		cs.update_velocities = true;
+	}
+}
+
+// ApplyCollisionConstraintsAndUpdateVelocities (SoftBodyMotionProperties.cpp:700-825): the
+// static / kinematic branch, and the dynamic branch in vertex order.
 template <class F> void CollisionAndVelocity(Body<F> &b)
 {
 	F dt = b.dt_sub;
@@ -797,10 +846,15 @@ template <class F> void CollisionAndVelocity(Body<F> &b)
 				if (pd > F(0))
 				{
 					b.contact[i] = 1;
-					const Leaf<F> &cs = b.leaves[b.shape_idx[i]];
+					Leaf<F> &cs = b.leaves[b.shape_idx[i]];
 					V3<F> n = b.plane[i].n;
 					b.x[i] = b.x[i] + n * pd;
-					// static body; mSoftBodyInvMassScale = 1 > 0
+					if (cs.dynamic)
+					{
+						DynamicContact(cs, b.x[i], b.v[i], prev_v, n, pd, b.w[i], dt, restitution_threshold);
+						continue;
+					}
+					// static or kinematic body; mSoftBodyInvMassScale = 1 > 0
 					V3<F> vn = n * Dot(n, b.v[i]);
 					V3<F> vt = b.v[i] - vn;
 					F vtl = Len(vt);
@@ -902,8 +956,34 @@ template <class F> void StepBody(World<F> &w, Body<F> &b, F step_dt)
 		Leaf<F> leaf;
 		leaf.transform = RotationTranslation(lq, Col3(rel, 3));
 		leaf.col = &c;
+		leaf.col_index = size_t(&c - w.colliders.data());
 		leaf.friction = std::sqrt(F(b.friction) * F(c.friction));		// ContactConstraintManager.h:554
 		leaf.restitution = std::max(F(b.restitution), F(c.restitution)); // ContactConstraintManager.h:555
+		if (c.motion == 2)
+		{
+			// SoftBodyMotionProperties.cpp:224-233 with MotionProperties::GetInverseInertiaForRotation
+			// (MotionProperties.inl:66-82): rotation = rel.Multiply3x3(sRotation(mInertiaRotation)),
+			// inverse inertia = rotation.Multiply3x3RightTransposed(diag-scaled columns of rotation)
+			leaf.dynamic = true;
+			leaf.inv_mass = F(c.inv_mass);
+			F iq[4] = { F(c.inertia_rot[0]), F(c.inertia_rot[1]), F(c.inertia_rot[2]), F(c.inertia_rot[3]) };
+			M44<F> ir = RotationTranslation(iq, V3<F>{ 0, 0, 0 });
+			V3<F> rc[3], sc[3];
+			for (int k = 0; k < 3; ++k)
+				rc[k] = (Col3(rel, 0) * ir.c[k][0] + Col3(rel, 1) * ir.c[k][1]) + Col3(rel, 2) * ir.c[k][2];
+			for (int k = 0; k < 3; ++k)
+				sc[k] = F(c.inv_inertia_diag[k]) * rc[k];
+			for (int j = 0; j < 3; ++j)
+			{
+				V3<F> col = (rc[0] * Comp(sc[0], j) + rc[1] * Comp(sc[1], j)) + rc[2] * Comp(sc[2], j);
+				leaf.inv_inertia.c[j][0] = col.x; leaf.inv_inertia.c[j][1] = col.y; leaf.inv_inertia.c[j][2] = col.z;
+				leaf.inv_inertia.c[j][3] = 0;
+			}
+			leaf.inv_inertia.c[3][0] = 0; leaf.inv_inertia.c[3][1] = 0; leaf.inv_inertia.c[3][2] = 0; leaf.inv_inertia.c[3][3] = 1;
+			leaf.com_translation = Col3(rel, 3);
+			leaf.lin = leaf.orig_lin = Mul3x3(inv, V3<F>{ F(c.lin_vel[0]), F(c.lin_vel[1]), F(c.lin_vel[2]) });
+			leaf.ang = leaf.orig_ang = Mul3x3(inv, V3<F>{ F(c.ang_vel[0]), F(c.ang_vel[1]), F(c.ang_vel[2]) });
+		}
 		b.leaves.push_back(leaf);
 	}
 
@@ -928,6 +1008,14 @@ template <class F> void StepBody(World<F> &w, Body<F> &b, F step_dt)
 		}
 	}
 	V3<F> delta_world = UpdateState(b, step_dt, com);
+	// UpdateRigidBodyVelocities (SoftBodyMotionProperties.cpp:923-935): world-space changes
+	for (const Leaf<F> &cs : b.leaves)
+		if (cs.dynamic && cs.update_velocities)
+		{
+			b.dyn_lin[cs.col_index] = b.dyn_lin[cs.col_index] + Mul3x3(com, cs.lin - cs.orig_lin);
+			b.dyn_ang[cs.col_index] = b.dyn_ang[cs.col_index] + Mul3x3(com, cs.ang - cs.orig_ang);
+			b.dyn_updated[cs.col_index] = 1;
+		}
 	// JobSoftBodyFinalize: SetPositionAndRotationInternal(GetPosition() + delta)
 	b.position = b.position + delta_world;
 }
@@ -1001,6 +1089,12 @@ template <class F> static void UpdateWorld(XrWorld *xw, float dt, int steps)
 	World<F> &w = Get<F>(xw);
 	// PhysicsSystem::Update: step delta time = dt / collision steps (PhysicsSystem.cpp:215)
 	F step_dt = F(dt) / F(steps);
+	for (Body<F> &b : w.bodies)
+	{
+		b.dyn_lin.assign(w.colliders.size(), V3<F>{ 0, 0, 0 });
+		b.dyn_ang.assign(w.colliders.size(), V3<F>{ 0, 0, 0 });
+		b.dyn_updated.assign(w.colliders.size(), 0);
+	}
 	for (int s = 0; s < steps; ++s)
 		for (Body<F> &b : w.bodies)
 			if (b.active)
@@ -1037,6 +1131,25 @@ template <class F> static int WriteBody(XrWorld *xw, int bi, const double *pos, 
 	{
 		if (pos) b.x[i] = { F(pos[3 * i]), F(pos[3 * i + 1]), F(pos[3 * i + 2]) };
 		if (vel) b.v[i] = { F(vel[3 * i]), F(vel[3 * i + 1]), F(vel[3 * i + 2]) };
+	}
+	return 0;
+}
+
+// World-space velocity changes body `body` applied to each of the n colliders in the last
+// update call (3 doubles each), and whether it touched it.
+template <class F> static int VelocityDeltas(XrWorld *xw, int bi, int n, double *lin, double *ang, int *updated)
+{
+	World<F> &w = Get<F>(xw);
+	if (bi < 0 || bi >= int(w.bodies.size()) || n != int(w.colliders.size()))
+		return -1;
+	const Body<F> &b = w.bodies[bi];
+	for (int c = 0; c < n; ++c)
+	{
+		const bool have = size_t(c) < b.dyn_updated.size();
+		V3<F> l = have ? b.dyn_lin[c] : V3<F>{ 0, 0, 0 }, a = have ? b.dyn_ang[c] : V3<F>{ 0, 0, 0 };
+		lin[3 * c] = l.x; lin[3 * c + 1] = l.y; lin[3 * c + 2] = l.z;
+		ang[3 * c] = a.x; ang[3 * c + 1] = a.y; ang[3 * c + 2] = a.z;
+		updated[c] = have ? b.dyn_updated[c] : 0;
 	}
 	return 0;
 }
@@ -1150,6 +1263,37 @@ int xr_body_write(void *wp, int body, const double *pos, const double *vel)
 {
 	XrWorld *w = (XrWorld *)wp;
 	return w->precision == 32 ? WriteBody<float>(w, body, pos, vel) : WriteBody<double>(w, body, pos, vel);
+}
+
+// Motion of collider `collider`: EMotionType (0 static, 1 kinematic, 2 dynamic) and, for a
+// dynamic body, inverse mass, inverse inertia diagonal, inertia rotation, world velocities.
+int xr_world_set_motion(void *wp, int collider, int motion, float inv_mass, const float *inv_inertia_diag,
+						const float *inertia_rot, const float *lin_vel, const float *ang_vel)
+{
+	XrWorld *w = (XrWorld *)wp;
+	if (collider < 0 || size_t(collider) >= w->f.colliders.size() || motion < 0 || motion > 2)
+		return -1;
+	Collider *targets[2] = { &w->f.colliders[collider], &w->d.colliders[collider] };
+	for (Collider *c : targets)
+	{
+		c->motion = motion;
+		c->inv_mass = inv_mass;
+		for (int k = 0; k < 3; ++k)
+		{
+			c->inv_inertia_diag[k] = inv_inertia_diag[k];
+			c->lin_vel[k] = lin_vel[k];
+			c->ang_vel[k] = ang_vel[k];
+		}
+		for (int k = 0; k < 4; ++k)
+			c->inertia_rot[k] = inertia_rot[k];
+	}
+	return 0;
+}
+
+int xr_body_velocity_deltas(void *wp, int body, int n, double *lin, double *ang, int *updated)
+{
+	XrWorld *w = (XrWorld *)wp;
+	return w->precision == 32 ? VelocityDeltas<float>(w, body, n, lin, ang, updated) : VelocityDeltas<double>(w, body, n, lin, ang, updated);
 }
 
 // ACosApproximate restated (for the closed-form pins)

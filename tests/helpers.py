@@ -5,7 +5,8 @@ import os
 
 import numpy as np
 
-from joltphysics_amd.records import (BEND, EDGE, FACE, GROUP, LRA, SHAPE_CONVEX_HULL, VERTEX, VOLUME, SettingsArrays)
+from joltphysics_amd.records import (BEND, EDGE, FACE, GROUP, LRA, MOTION_DYNAMIC, SHAPE_CONVEX_HULL, VERTEX, VOLUME,
+                                     SettingsArrays)
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 ROOT_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -101,3 +102,121 @@ def reference_run(scene, frames_to_keep, variant="nofma", num_threads=0):
         out[f] = [world.read(i) for i in range(len(scene.bodies))]
     world.close()
     return settings, bounds, out
+
+
+def _take_motion(c, m):
+    c.inv_mass = m["inv_mass"]
+    c.inv_inertia_diagonal = m["inv_inertia_diagonal"]
+    c.inertia_rotation = m["inertia_rotation"]
+    c.linear_velocity = m["linear_velocity"]
+    c.angular_velocity = m["angular_velocity"]
+
+
+def reference_custom_run(scene, frames_to_keep, variant="nofma"):
+    """The reference with SoftBodyMotionProperties::CustomUpdate(dt) once per frame for the
+    (single) soft body: one collision step per frame, rigid bodies not simulated, dynamic
+    colliders' velocities changed by UpdateRigidBodyVelocities.  Sets each collider's device
+    parameters and initial motion from the reference.  Returns (settings, bounds,
+    {frame: [read(body)]}, {frame: [(linear, angular) velocity of each collider]})."""
+    from oracle import JoltReference, JoltWorld
+    assert len(scene.bodies) == 1 and scene.collision_steps == 1
+    ref = JoltReference(variant)
+    settings = [ref.build_settings(b.mesh) for b in scene.bodies]
+    world = JoltWorld(ref, scene.gravity)
+    ids = [world.add_collider(c) for c in scene.colliders]
+    bounds = np.array([world.collider_bounds(i) for i in ids], np.float32).reshape(-1, 6)
+    for c, i in zip(scene.colliders, ids):
+        c.device_params, c.com = world.collider_device(i)
+        if c.type == SHAPE_CONVEX_HULL:
+            c.hull = world.collider_hull(i)
+        motion, m = world.collider_motion(i)
+        assert motion == c.motion_type
+        if motion == MOTION_DYNAMIC:
+            _take_motion(c, m)
+    for b, s in zip(scene.bodies, settings):
+        world.add_soft_body(s, b.position, b.rotation, b.params)
+    world.optimize_broadphase()
+    out, vel = {}, {}
+    for f in range(1, max(frames_to_keep) + 1):
+        world.custom_update(0, scene.dt)
+        if f in frames_to_keep:
+            out[f] = [world.read(0)]
+            vel[f] = [(world.collider_motion(i)[1]["linear_velocity"], world.collider_motion(i)[1]["angular_velocity"])
+                      for i in ids]
+    world.close()
+    return settings, bounds, out, vel
+
+
+def custom_run(scene, settings, bounds, frames_to_keep, runner):
+    """Drives `runner` (restatement or GPU context) as reference_custom_run drives the
+    reference: one update of one collision step per frame, then the dynamic colliders'
+    velocities += the soft body's changes (clamped as BodyInterface does)."""
+    from joltphysics_amd.records import MAX_ANGULAR_VELOCITY, MAX_LINEAR_VELOCITY, add_velocity_clamped
+    cols = scene.colliders
+    out, vel = {}, {}
+    for f in range(1, max(frames_to_keep) + 1):
+        runner.set_motions(cols)
+        runner.update(scene.dt, 1, 1)
+        for b in range(len(scene.bodies)):        # each soft body's finalize, in body order
+            lin, ang, upd = runner.velocity_deltas(b)
+            for k, c in enumerate(cols):
+                if c.motion_type == MOTION_DYNAMIC and upd[k]:
+                    c.linear_velocity = add_velocity_clamped(c.linear_velocity, lin[k], MAX_LINEAR_VELOCITY)
+                    c.angular_velocity = add_velocity_clamped(c.angular_velocity, ang[k], MAX_ANGULAR_VELOCITY)
+        if f in frames_to_keep:
+            out[f] = [runner.read(b) for b in range(len(scene.bodies))]
+            vel[f] = [(tuple(c.linear_velocity), tuple(c.angular_velocity)) for c in cols]
+    return out, vel
+
+
+class RestatementRunner:
+    def __init__(self, scene, settings, bounds, precision=32):
+        from oracle import Restatement
+        self.r = Restatement(scene.gravity, precision)
+        for c, b in zip(scene.colliders, bounds):
+            self.r.add_collider(c, b)
+        for b, s in zip(scene.bodies, settings):
+            self.r.add_soft_body(s, b.position, b.rotation, b.params)
+
+    def set_motions(self, cols):
+        for i, c in enumerate(cols):
+            if c.motion_type == MOTION_DYNAMIC:
+                self.r.set_motion(i, c)
+
+    def update(self, dt, steps, frames):
+        self.r.update(dt, steps, frames)
+
+    def velocity_deltas(self, body):
+        return self.r.velocity_deltas(body)
+
+    def read(self, body):
+        return self.r.read(body)
+
+    def close(self):
+        self.r.close()
+
+
+class GpuRunner:
+    def __init__(self, scene, settings, bounds, device=0):
+        from joltphysics_amd.xpbd import Context
+        self.ctx = Context(device)
+        self.ctx.set_gravity(scene.gravity)
+        for c, b in zip(scene.colliders, bounds):
+            c.bounds = tuple(float(x) for x in b)
+        self.ids = [self.ctx.create_body(self.ctx.create_settings(s), b.position, b.rotation, b.params)
+                    for b, s in zip(scene.bodies, settings)]
+
+    def set_motions(self, cols):
+        self.ctx.set_colliders(cols)
+
+    def update(self, dt, steps, frames):
+        self.ctx.update(dt, steps, frames)
+
+    def velocity_deltas(self, body):
+        return self.ctx.velocity_deltas(self.ids[body])
+
+    def read(self, body):
+        return self.ctx.read(self.ids[body])
+
+    def close(self):
+        self.ctx.close()

@@ -8,7 +8,8 @@ import numpy as np
 
 from joltphysics_amd import scenes
 from joltphysics_amd.records import (BEND_DIHEDRAL, BEND_DISTANCE, LRA_GEODESIC, SHAPE_BOX, SHAPE_CAPSULE,
-                                     SHAPE_CONVEX_HULL, SHAPE_CYLINDER, SHAPE_PLANE, SHAPE_SPHERE, SHAPE_TAPERED_CAPSULE, BodyParams,
+                                     MOTION_DYNAMIC, SHAPE_CONVEX_HULL, SHAPE_CYLINDER, SHAPE_PLANE, SHAPE_SPHERE,
+                                     SHAPE_TAPERED_CAPSULE, BodyParams,
                                      Collider)
 from joltphysics_amd.scenes import (CALC_EDGE_LENGTHS, CALC_VOLUMES, CREATE_CONSTRAINTS, OPTIMIZE, BodySpec,
                                     MeshSpec, Scene)
@@ -178,4 +179,44 @@ ALL = {
     "cloth_hulls_streamed": cloth_hulls_streamed,
     "block_lra_boxes": block_lra_boxes,
     "block_lra_boxes_streamed": block_lra_boxes_streamed,
+}
+
+
+def cloth_on_dynamic_bodies(n=24, frames=12, spacing=0.04):
+    """n x n cloth falling onto two dynamic bodies (a rotated box, a sphere) and a static
+    box, advanced with SoftBodyMotionProperties::CustomUpdate frame by frame: the rigid
+    bodies do not move, their velocities take the soft body's impulses."""
+    v, f = scenes.cloth(n, spacing, 60.0, jitter=1e-4, seed=17)
+    mesh = MeshSpec(vertices=v, faces=f, attrs=scenes.attrs(1e-6, 1e-6, 1e-4), bend_type=BEND_DIHEDRAL)
+    body = BodySpec(mesh=mesh, position=(0.0, 0.0, 0.42), params=BodyParams(num_iterations=5, allow_sleeping=False,
+                                                                            friction=0.5, restitution=0.2))
+    cols = _bounded([Collider(SHAPE_BOX, (0.2, 0.15, 0.1), pos=(-0.15, 0.05, 0.2), rot=_quat((0.2, 1, 0.3), 0.5),
+                              friction=0.6, restitution=0.3, motion_type=MOTION_DYNAMIC, linear_velocity=(0.0, 0.0, 0.5),
+                              angular_velocity=(0.2, -0.1, 0.3)),
+                     Collider(SHAPE_SPHERE, (0.12,), pos=(0.25, -0.1, 0.22), friction=0.4, motion_type=MOTION_DYNAMIC,
+                              linear_velocity=(0.3, 0.0, 0.0)),
+                     Collider(SHAPE_BOX, (0.15, 0.15, 0.05), pos=(0.2, 0.25, 0.25), friction=0.3)])
+    return Scene("cloth_on_dynamic_bodies", [body], cols, gravity=(0.0, 0.0, -9.81), dt=1 / 60, collision_steps=1,
+                 frames=frames)
+
+
+def cloth_on_dynamic_bodies_streamed():
+    """The dynamic-body scene with a 120 x 120 cloth (14,400 vertices, streamed)."""
+    return cloth_on_dynamic_bodies(n=120, frames=10, spacing=0.008)
+
+
+def two_cloths_on_dynamic_bodies():
+    """Two cloths (one bundle) over the same dynamic bodies: both read the velocities of the
+    call's start and report their own changes."""
+    a = cloth_on_dynamic_bodies(n=16, frames=10, spacing=0.05)
+    b = cloth_on_dynamic_bodies(n=20, frames=10, spacing=0.03)
+    b.bodies[0].position = (0.05, -0.05, 0.47)
+    return Scene("two_cloths_on_dynamic_bodies", a.bodies + b.bodies, a.colliders, gravity=a.gravity, dt=a.dt,
+                 collision_steps=1, frames=a.frames)
+
+
+# advanced with CustomUpdate (one soft body) or, for several, with the restatement only
+CUSTOM = {
+    "cloth_on_dynamic_bodies": cloth_on_dynamic_bodies,
+    "cloth_on_dynamic_bodies_streamed": cloth_on_dynamic_bodies_streamed,
 }

@@ -1,0 +1,64 @@
+"""GPU parity with dynamic colliders: the soft body's contact impulses shared with
+dynamic rigid bodies and the velocity changes it reports
+(SoftBodyMotionProperties.cpp:752-797, 923-935).
+
+One soft body: against the reference's CustomUpdate sequence, bit-exact vertex state
+and rigid-body velocities every frame (LDS-resident and streamed).  Two soft bodies on
+the same dynamic bodies in one bundle: against the restatement (both read the
+velocities of the call's start, as the C ABI documents), bit-exact.
+"""
+from __future__ import annotations
+
+import pytest
+
+import scene_zoo
+from helpers import GpuRunner, RestatementRunner, custom_run, float_equal, reference_custom_run
+from oracle import have_reference
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(out, vel, ref, ref_vel, frames, ncol, nbodies=1):
+    for f in frames:
+        for b in range(nbodies):
+            assert float_equal(out[f][b]["pos"], ref[f][b]["pos"]), f"positions differ at frame {f}, body {b}"
+            assert float_equal(out[f][b]["vel"], ref[f][b]["vel"]), f"velocities differ at frame {f}, body {b}"
+        for k in range(ncol):
+            for j in range(2):
+                assert float_equal(vel[f][k][j], ref_vel[f][k][j]), f"collider {k} velocity {j} differs at frame {f}"
+
+
+@pytest.mark.skipif(not have_reference("nofma"), reason="reference library not built")
+@pytest.mark.parametrize("name", list(scene_zoo.CUSTOM))
+def test_dynamic_bodies_bit_exact_with_reference(name):
+    sc = scene_zoo.CUSTOM[name]()
+    frames = tuple(range(1, sc.frames + 1))
+    print(f"scene {name}: frames {frames[-1]}", flush=True)
+    settings, bounds, ref, ref_vel = reference_custom_run(sc, frames)
+    run = GpuRunner(sc, settings, bounds)
+    out, vel = custom_run(sc, settings, bounds, frames, run)
+    run.close()
+    _check(out, vel, ref, ref_vel, frames, len(sc.colliders))
+    assert ref_vel[frames[-1]][0] != ref_vel[1][0], "the cloth never reached the dynamic box"
+
+
+@pytest.mark.skipif(not have_reference("nofma"), reason="reference library not built")
+def test_two_bodies_on_dynamic_bodies_match_restatement():
+    import copy
+    sc = scene_zoo.two_cloths_on_dynamic_bodies()
+    frames = tuple(range(1, sc.frames + 1))
+    # settings and colliders' device data from the reference (its settings builder)
+    from oracle import JoltReference
+    ref = JoltReference("nofma")
+    settings = [ref.build_settings(b.mesh) for b in sc.bodies]
+    single = scene_zoo.cloth_on_dynamic_bodies()
+    _, bounds, _, _ = reference_custom_run(single, (1,))
+    sc.colliders = single.colliders      # device parameters and initial motion from the reference
+    sc_cpu = copy.deepcopy(sc)
+    run = RestatementRunner(sc_cpu, settings, bounds)
+    want, want_vel = custom_run(sc_cpu, settings, bounds, frames, run)
+    run.close()
+    run = GpuRunner(sc, settings, bounds)
+    out, vel = custom_run(sc, settings, bounds, frames, run)
+    run.close()
+    _check(out, vel, want, want_vel, frames, len(sc.colliders), nbodies=2)
