@@ -263,6 +263,17 @@ int xpbd_body_velocity_deltas(xpbd_context *ctx, int32_t body, uint32_t num_coll
                               int32_t *updated);
 
 /*
+ * The contacts of soft body `body` in the last collision step of the last xpbd_update, as
+ * the contact listener's SoftBodyManifold presents them to OnSoftBodyContactAdded
+ * (SoftBodyMotionProperties.cpp:832-845, SoftBodyManifold.h:24-40): per vertex the collider
+ * index (into the last xpbd_set_colliders list, the manifold's GetContactBodyID) or -1 when
+ * the vertex had no contact (HasContact), and the local contact point (GetLocalContactPoint,
+ * in the body space of that step, before the step re-centred the vertices; 3 floats).  The
+ * contact normal is minus the collision plane's normal (xpbd_body_read).  Any output may be NULL.
+ */
+int xpbd_body_contacts(xpbd_context *ctx, int32_t body, float *points, int32_t *collider);
+
+/*
  * Kernel timing on the context stream (hipEvents around every launch of
  * xpbd_update while enabled).  xpbd_profile_read returns the number of timed
  * launches and their summed duration in milliseconds, then resets.

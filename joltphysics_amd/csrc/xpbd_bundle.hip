@@ -37,6 +37,7 @@ struct BodyLocal
     uint32_t num_leaves;
     uint32_t num_dyn_leaves;    // leaves of dynamic colliders this step
     uint32_t num_dynamic;       // dynamic colliders of the context
+    uint32_t report_cleared;    // asleep: its contact report was cleared
     DynStateDev *dyn;           // the body's row of dynamic-collider states
     uint32_t nf;
     const uint32_t *faces;
@@ -137,6 +138,7 @@ __device__ __forceinline__ void bundle_update(
         L.nf = B.pressure > 0.0f ? B.s.num_faces : 0u;
         L.faces = B.s.faces; L.vf_begin = B.s.vf_begin; L.vf_list = B.s.vf_list;
         L.dyn = B.dyn;
+        L.report_cleared = 0;
         L.num_dynamic = B.dyn ? B.num_dynamic : 0u;
         for (uint32_t k = 0; k < L.num_dynamic; ++k)
             clear_dynamic(L.dyn[k]);
@@ -161,6 +163,17 @@ __device__ __forceinline__ void bundle_update(
     for (int32_t frame = 0; frame < frames; ++frame)
         for (int32_t step = 0; step < steps; ++step)
         {
+            // ---- a body that does not run in a step reports no contacts for it
+            if (has_sleepers)
+                for (uint32_t b = wave; b < nb; b += (BLOCK / kWave))
+                    if (!st[b].active && !loc[b].report_cleared)
+                    {
+                        const BundleBodyDev bb = G.bodies[b];
+                        for (uint32_t k = lane; k < bb.nv; k += kWave)
+                            G.report[bb.voff + k] = contact_report(v3(0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f), false, -1);
+                        if (lane == 0)
+                            loc[b].report_cleared = 1;
+                    }
             // ---- InitializeUpdateContext per body (SoftBodyMotionProperties.cpp:937-960)
             for (uint32_t b = tid; b < nb; b += BLOCK)
             {
@@ -216,6 +229,7 @@ __device__ __forceinline__ void bundle_update(
                     continue;
                 }
                 make_leaf(cd, inversed_rotation_translation(com), L.friction, L.restitution, out, L.dyn);
+                out.collider = (int32_t)c;
             }
             sync_global();
             // colliding list of each body in collider (broadphase) order
@@ -430,6 +444,9 @@ __device__ __forceinline__ void bundle_update(
                         pset(X, i, f4(x, p.w));
                         G.v[i] = f4(v, 0.0f);
                         G.prev[i].w = v_sq;        // pre-clamp |v|^2 for the sleep test
+                        const int32_t hit = G.idx[i];
+                        G.report[i] = contact_report(x, G.plane[i], G.contact[i] != 0 && hit >= 0,
+                                                     hit >= 0 ? leaves[b * nc + hit].collider : -1);
                         G.idx[i] = -1;
                         G.contact[i] = 0;
                     }

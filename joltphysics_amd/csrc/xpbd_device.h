@@ -26,6 +26,7 @@ struct Leaf
     float friction, restitution;
     const HullDev *hull;    // convex hull tables (device memory)
     int32_t dyn;            // dynamic collider: rank into the body's DynStateDev row; -1 otherwise
+    int32_t collider;       // index of the collider in the context's list
 };
 
 __device__ __forceinline__ V3 xyz(float4 p) { return v3(p.x, p.y, p.z); }
@@ -703,6 +704,17 @@ __device__ __forceinline__ void finish_dynamic(DynStateDev &D, const M44 &com)
     }
 }
 
+// What the contact listener's SoftBodyManifold reports for a vertex at the end of a
+// collision step (SoftBodyManifold.h:24-40, read before UpdateSoftBodyState resets the
+// collision state): the local contact point x - SignedDistance(x) * n and the collider.
+__device__ __forceinline__ float4 contact_report(V3 x, float4 pl, bool has_contact, int32_t collider)
+{
+    if (!has_contact)
+        return make_float4(0.0f, 0.0f, 0.0f, __int_as_float(-1));
+    Pl plane; plane.n = v3(pl.x, pl.y, pl.z); plane.c = pl.w;
+    return f4(x - signed_distance(plane, x) * plane.n, __int_as_float(collider));
+}
+
 __device__ __forceinline__ void clear_dynamic(DynStateDev &D)
 {
     st3(D.acc_lin, v3(0.0f, 0.0f, 0.0f));
@@ -726,6 +738,7 @@ __device__ __forceinline__ void make_leaf(const ColliderDev &c, const M44 &inv_c
     out.t = t;
     out.type = c.type;
     out.dyn = -1;
+    out.collider = -1;
     if (c.motion == 2 && dyn_row)
     {
         out.dyn = c.dyn;

@@ -148,11 +148,19 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
     const uint32_t iterations = B.iterations;
     const uint32_t num_windows = S.num_par_windows + S.num_tail_windows;
 
+    bool report_cleared = false;
     for (int32_t frame = 0; frame < frames; ++frame)
         for (int32_t step = 0; step < steps; ++step)
         {
             if (!st.active)
+            {
+                // a body that does not run in a step reports no contacts for it
+                if (!report_cleared)
+                    for (uint32_t i = tid; i < nv; i += kBlock)
+                        B.report[i] = contact_report(v3(0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f), false, -1);
+                report_cleared = true;
                 break;
+            }
 
             // InitializeUpdateContext (SoftBodyMotionProperties.cpp:937-960)
             const M44 com = rotation_translation(st.rot[0], st.rot[1], st.rot[2], st.rot[3], v3(st.pos[0], st.pos[1], st.pos[2]));
@@ -216,7 +224,10 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                 const M44 inv_com = inversed_rotation_translation(com);
                 for (uint32_t ci = tid; ci < num_colliders; ci += kBlock)
                     if (slots[ci] >= 0)
+                    {
                         make_leaf(colliders[ci], inv_com, B.friction, B.restitution, leaves[slots[ci]], dyn);
+                        leaves[slots[ci]].collider = (int32_t)ci;
+                    }
                 // dynamic states are read by the in-order contact pass of another wavefront
                 if (has_dynamic)
                     sync_global();
@@ -421,6 +432,9 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                         pmax = vmax(pmax, pp);
                         P[i] = f4(x, p.w);
                         B.v[i] = f4(v, 0.0f);
+                        const int32_t hit = B.idx[i];
+                        B.report[i] = contact_report(x, B.plane[i], B.contact[i] != 0 && hit >= 0,
+                                                     hit >= 0 ? leaves[hit].collider : -1);
                         B.idx[i] = -1;
                         B.contact[i] = 0;
                     }

@@ -124,7 +124,7 @@ struct Body
     xpbd_settings *settings = nullptr;
     uint32_t nv = 0;
     bool owns_storage = true;       // false once the state lives in a bundle's arrays
-    DevBuf<float4> x, v, prev, plane;
+    DevBuf<float4> x, v, prev, plane, report;
     DevBuf<int32_t> idx, contact;
     BodyDev dev{};
     BodyStateDev state{};
@@ -141,7 +141,7 @@ struct BundleHost
     DevBuf<uint4> recs;
     DevBuf<uint32_t> lvl;
     DevBuf<BundleBodyDev> table;
-    DevBuf<float4> x, v, prev, plane, face_scratch;
+    DevBuf<float4> x, v, prev, plane, report, face_scratch;
     DevBuf<int32_t> idx, contact;
     DevBuf<unsigned char> leaves_full, leaves;
     BundleDev dev{};
@@ -925,6 +925,7 @@ int BuildLayout(xpbd_context *ctx)
         HIP_TRY(bh.table.upload(table));
         // bundle-contiguous vertex state, gathered from wherever each body's state lives now
         HIP_TRY(bh.x.alloc(voff)); HIP_TRY(bh.v.alloc(voff)); HIP_TRY(bh.prev.alloc(voff)); HIP_TRY(bh.plane.alloc(voff));
+        HIP_TRY(bh.report.alloc(voff));
         HIP_TRY(bh.idx.alloc(voff)); HIP_TRY(bh.contact.alloc(voff));
         for (uint32_t j = 0; j < nb; ++j)
         {
@@ -934,6 +935,7 @@ int BuildLayout(xpbd_context *ctx)
             HIP_TRY(hipMemcpy(bh.v.p + o, b.dev.v, n * sizeof(float4), hipMemcpyDeviceToDevice));
             HIP_TRY(hipMemcpy(bh.prev.p + o, b.dev.prev, n * sizeof(float4), hipMemcpyDeviceToDevice));
             HIP_TRY(hipMemcpy(bh.plane.p + o, b.dev.plane, n * sizeof(float4), hipMemcpyDeviceToDevice));
+            HIP_TRY(hipMemcpy(bh.report.p + o, b.dev.report, n * sizeof(float4), hipMemcpyDeviceToDevice));
             HIP_TRY(hipMemcpy(bh.idx.p + o, b.dev.idx, n * sizeof(int32_t), hipMemcpyDeviceToDevice));
             HIP_TRY(hipMemcpy(bh.contact.p + o, b.dev.contact, n * sizeof(int32_t), hipMemcpyDeviceToDevice));
         }
@@ -944,6 +946,7 @@ int BuildLayout(xpbd_context *ctx)
         d.recs = bh.recs.p; d.lvl = bh.lvl.p; d.nlev = (uint32_t)P.lvl.size() - 1; d.num_records = (uint32_t)P.recs.size();
         d.bodies = bh.table.p; d.num_bodies = nb; d.nv = voff;
         d.x = bh.x.p; d.v = bh.v.p; d.prev = bh.prev.p; d.plane = bh.plane.p; d.idx = bh.idx.p; d.contact = bh.contact.p;
+        d.report = bh.report.p;
         d.leaves_full = bh.leaves_full.p; d.leaves = bh.leaves.p; d.face_scratch = bh.face_scratch.p;
         d.has_pressure = P.has_pressure ? 1u : 0u;
         d.has_sleepers = P.has_sleepers ? 1u : 0u;
@@ -976,10 +979,11 @@ int BuildLayout(xpbd_context *ctx)
             uint32_t o = 0;
             for (uint32_t k = 0; k < j; ++k) o += ctx->bodies[bp->bodies[k]]->nv;
             b.dev.x = bp->x.p + o; b.dev.v = bp->v.p + o; b.dev.prev = bp->prev.p + o; b.dev.plane = bp->plane.p + o;
-            b.dev.idx = bp->idx.p + o; b.dev.contact = bp->contact.p + o;
+            b.dev.idx = bp->idx.p + o; b.dev.contact = bp->contact.p + o; b.dev.report = bp->report.p + o;
             if (b.owns_storage)
             {
                 (void)b.x.alloc(0); (void)b.v.alloc(0); (void)b.prev.alloc(0); (void)b.plane.alloc(0); (void)b.idx.alloc(0); (void)b.contact.alloc(0);
+                (void)b.report.alloc(0);
                 b.owns_storage = false;
             }
         }
@@ -1182,16 +1186,24 @@ int xpbd_body_create(xpbd_context *ctx, const xpbd_body_desc *d, int32_t *out_bo
         bmax = vmax(bmax, p);
     }
     std::vector<int32_t> idx(nv, -1), contact(nv, 0);
+    float no_contact = 0.0f;
+    {
+        const int32_t minus_one = -1;
+        memcpy(&no_contact, &minus_one, sizeof(no_contact));
+    }
+    std::vector<float4> report(nv, make_float4(0.0f, 0.0f, 0.0f, no_contact));
     HIP_TRY(b->x.upload(x));
     HIP_TRY(b->v.upload(v));
     HIP_TRY(b->prev.upload(x));
     HIP_TRY(b->plane.upload(plane));
     HIP_TRY(b->idx.upload(idx));
     HIP_TRY(b->contact.upload(contact));
+    HIP_TRY(b->report.upload(report));
 
     BodyDev &dv = b->dev;
     dv.s = s->dev;
     dv.x = b->x.p; dv.v = b->v.p; dv.prev = b->prev.p; dv.plane = b->plane.p; dv.idx = b->idx.p; dv.contact = b->contact.p;
+    dv.report = b->report.p;
     dv.iterations = d->num_iterations;
     dv.linear_damping = d->linear_damping;
     dv.max_linear_velocity = d->max_linear_velocity;
@@ -1610,6 +1622,27 @@ int xpbd_body_state(xpbd_context *ctx, int32_t body, float out[14])
     memcpy(out + 9, s.bmax, 12);
     out[12] = (float)s.active;
     out[13] = s.sleep_timer;
+    return XPBD_OK;
+}
+
+int xpbd_body_contacts(xpbd_context *ctx, int32_t body, float *points, int32_t *collider)
+{
+    if (!ctx || body < 0 || body >= (int32_t)ctx->bodies.size())
+        return Fail(XPBD_E_INVALID, "invalid body");
+    int rc = xpbd_synchronize(ctx);
+    if (rc != XPBD_OK)
+        return rc;
+    const Body &b = *ctx->bodies[body];
+    std::vector<float4> rep(b.nv);
+    if (b.nv)
+        HIP_TRY(hipMemcpy(rep.data(), b.dev.report, sizeof(float4) * b.nv, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < b.nv; ++i)
+    {
+        int32_t c;
+        memcpy(&c, &rep[i].w, sizeof(c));
+        if (points) { points[3 * i] = rep[i].x; points[3 * i + 1] = rep[i].y; points[3 * i + 2] = rep[i].z; }
+        if (collider) collider[i] = c;
+    }
     return XPBD_OK;
 }
 

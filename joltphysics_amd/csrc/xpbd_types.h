@@ -57,6 +57,8 @@ struct BodyDev
     float4 *plane;              // collision plane normal + constant
     int32_t *idx;               // colliding shape index (-1 none)
     int32_t *contact;           // has-contact flag
+    float4 *report;             // last collision step's contact: point xyz (SoftBodyManifold::GetLocalContactPoint),
+                                // w = collider index as int bits, -1 without contact
     uint32_t iterations;
     float linear_damping, max_linear_velocity, restitution, friction, pressure, gravity_factor, vertex_radius;
     int32_t update_position, allow_sleeping;
@@ -146,7 +148,7 @@ struct BundleDev
     const BundleBodyDev *bodies;
     uint32_t num_bodies;
     uint32_t nv;                // vertices of all bodies of the bundle
-    float4 *x, *v, *prev, *plane;   // bundle-contiguous vertex state (bodies point into these)
+    float4 *x, *v, *prev, *plane, *report;  // bundle-contiguous vertex state (bodies point into these)
     int32_t *idx, *contact;
     void *leaves_full;          // num_bodies x num_colliders Leaf: per-step candidate leaves
     void *leaves;               // num_bodies x num_colliders Leaf: compacted colliding list

@@ -95,7 +95,8 @@ EXPORTED_SYMBOLS = (
     "xpbd_settings_build", "xpbd_built_counts", "xpbd_built_read", "xpbd_built_free",
     "xpbd_build_last_error", "xpbd_body_create",
     "xpbd_set_colliders", "xpbd_set_gravity", "xpbd_update", "xpbd_synchronize", "xpbd_body_read",
-    "xpbd_body_write", "xpbd_body_state", "xpbd_body_velocity_deltas", "xpbd_profile_enable", "xpbd_profile_read",
+    "xpbd_body_write", "xpbd_body_state", "xpbd_body_velocity_deltas", "xpbd_body_contacts", "xpbd_profile_enable",
+    "xpbd_profile_read",
     "xpbd_rollout_create", "xpbd_rollout_destroy", "xpbd_rollout_sizes", "xpbd_rollout_forward",
     "xpbd_rollout_forward_f64", "xpbd_rollout_backward",
 )
@@ -143,6 +144,7 @@ def load_library(path=LIB_PATH):
     lib.xpbd_body_write.argtypes = [vp, i32, vp, vp]
     lib.xpbd_body_state.argtypes = [vp, i32, vp]
     lib.xpbd_body_velocity_deltas.argtypes = [vp, i32, u32, vp, vp, vp]
+    lib.xpbd_body_contacts.argtypes = [vp, i32, vp, vp]
     lib.xpbd_profile_enable.argtypes = [vp, i32]
     lib.xpbd_profile_read.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_double)]
     lib.xpbd_rollout_create.argtypes = [vp, C.POINTER(_RolloutDesc), C.POINTER(vp)]
@@ -292,6 +294,15 @@ class Context:
                     arr[i].inertia_rotation[k] = float(c.inertia_rotation[k])
         _check(self.lib.xpbd_set_colliders(self.h, arr, n))
         self._num_colliders = n
+
+    def contacts(self, body):
+        """(collider index per vertex, -1 without contact; local contact points (V, 3)) of the
+        last collision step, as the contact listener's SoftBodyManifold reports them."""
+        nv = self.sizes[body]
+        col = np.zeros(max(nv, 1), np.int32)
+        pts = np.zeros((max(nv, 1), 3), np.float32)
+        _check(self.lib.xpbd_body_contacts(self.h, int(body), pts.ctypes.data, col.ctypes.data))
+        return col[:nv], pts[:nv]
 
     def velocity_deltas(self, body):
         """(linear (n, 3), angular (n, 3), updated (n,)): the world-space velocity changes soft

@@ -45,6 +45,10 @@
 #include <Jolt/Physics/SoftBody/SoftBodySharedSettings.h>
 #include <Jolt/Physics/SoftBody/SoftBodyCreationSettings.h>
 #include <Jolt/Physics/SoftBody/SoftBodyMotionProperties.h>
+#include <Jolt/Physics/SoftBody/SoftBodyContactListener.h>
+#include <Jolt/Physics/SoftBody/SoftBodyManifold.h>
+#include <Jolt/Physics/PhysicsStepListener.h>
+#include <map>
 
 #include <chrono>
 #include <cstring>
@@ -113,6 +117,45 @@ struct SettingsHandle
 	Ref<SoftBodySharedSettings> mSettings;
 };
 
+// Records what OnSoftBodyContactAdded reports (SoftBodyManifold) for each soft body
+// in the last collision step: per vertex the collider index (or -1) and the local
+// contact point.  The step listener clears the records at the start of every step.
+struct ContactRecorder final : public SoftBodyContactListener, public PhysicsStepListener
+{
+	struct Record { std::vector<int> collider; std::vector<float> point; };
+	std::mutex mMutex;
+	std::map<uint32, Record> mRecords;          // key: soft body BodyID index and sequence
+	std::map<uint32, int> mColliderIndex;       // collider BodyID -> collider index
+
+	void OnStep(const PhysicsStepListenerContext &) override
+	{
+		std::lock_guard<std::mutex> lock(mMutex);
+		mRecords.clear();
+	}
+
+	void OnSoftBodyContactAdded(const Body &inSoftBody, const SoftBodyManifold &inManifold) override
+	{
+		Record r;
+		for (const SoftBodyVertex &v : inManifold.GetVertices())
+		{
+			if (inManifold.HasContact(v))
+			{
+				auto it = mColliderIndex.find(inManifold.GetContactBodyID(v).GetIndexAndSequenceNumber());
+				r.collider.push_back(it == mColliderIndex.end() ? -2 : it->second);
+				Vec3 p = inManifold.GetLocalContactPoint(v);
+				r.point.insert(r.point.end(), { p.GetX(), p.GetY(), p.GetZ() });
+			}
+			else
+			{
+				r.collider.push_back(-1);
+				r.point.insert(r.point.end(), { 0.0f, 0.0f, 0.0f });
+			}
+		}
+		std::lock_guard<std::mutex> lock(mMutex);
+		mRecords[inSoftBody.GetID().GetIndexAndSequenceNumber()] = std::move(r);
+	}
+};
+
 struct World
 {
 	BPLayerMap mBP;
@@ -125,6 +168,7 @@ struct World
 	std::vector<BodyID> mColliders;
 	std::vector<std::array<float, 8>> mColliderParams;	// shape parameters as given to jo_world_add_collider
 	std::vector<int> mColliderTypes;
+	std::unique_ptr<ContactRecorder> mRecorder;
 };
 
 template <class T>
@@ -310,6 +354,41 @@ void *jo_world_create(uint32 max_bodies, int num_threads)
 }
 
 void jo_world_free(void *wp) { delete (World *)wp; }
+
+// Installs a contact listener that records each soft body's SoftBodyManifold of the last
+// collision step (jo_world_contacts).  Call after the colliders are added.
+void jo_world_record_contacts(void *wp)
+{
+	World *w = (World *)wp;
+	w->mRecorder = std::make_unique<ContactRecorder>();
+	for (size_t i = 0; i < w->mColliders.size(); ++i)
+		w->mRecorder->mColliderIndex[w->mColliders[i].GetIndexAndSequenceNumber()] = int(i);
+	w->mSystem.SetSoftBodyContactListener(w->mRecorder.get());
+	w->mSystem.AddStepListener(w->mRecorder.get());
+}
+
+// The recorded manifold of soft body `body` in the last collision step: per vertex the
+// collider index (-1: no contact) and the local contact point (3 floats); all -1 when the
+// step reported no contact.
+int jo_world_contacts(void *wp, int body, int *collider, float *points)
+{
+	World *w = (World *)wp;
+	if (!w->mRecorder)
+		return -1;
+	BodyLockRead lock(w->mSystem.GetBodyLockInterface(), w->mSoftBodies[body]);
+	if (!lock.Succeeded())
+		return -1;
+	const uint nv = uint(static_cast<const SoftBodyMotionProperties *>(lock.GetBody().GetMotionProperties())->GetVertices().size());
+	auto it = w->mRecorder->mRecords.find(w->mSoftBodies[body].GetIndexAndSequenceNumber());
+	for (uint i = 0; i < nv; ++i)
+	{
+		const bool have = it != w->mRecorder->mRecords.end() && i < it->second.collider.size();
+		collider[i] = have ? it->second.collider[i] : -1;
+		for (int k = 0; k < 3; ++k)
+			points[3 * i + k] = have ? it->second.point[3 * i + k] : 0.0f;
+	}
+	return 0;
+}
 
 void jo_world_set_gravity(void *wp, float gx, float gy, float gz)
 {

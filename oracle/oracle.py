@@ -94,6 +94,9 @@ class JoltReference:
             lib.jo_world_add_dynamic.restype = _i
             lib.jo_world_add_dynamic.argtypes = [_vp, _i, _vp, _vp, _vp, _f, _f, _vp, _vp]
             lib.jo_world_collider_motion.restype = _i
+            lib.jo_world_record_contacts.argtypes = [_vp]
+            lib.jo_world_contacts.restype = _i
+            lib.jo_world_contacts.argtypes = [_vp, _i, _vp, _vp]
             lib.jo_world_collider_motion.argtypes = [_vp, _i, _vp]
             lib.jo_world_hull_data.argtypes = [_vp, _i] + [_vp] * 6
             lib.jo_world_add_soft_body.restype = _i
@@ -209,6 +212,20 @@ class JoltWorld:
         com = np.zeros(3, np.float32)
         self.lib.jo_world_collider_device(self.h, i, _ptr(params), _ptr(com))
         return tuple(float(x) for x in params), tuple(float(x) for x in com)
+
+    def record_contacts(self):
+        """Installs the contact listener that records each soft body's SoftBodyManifold of
+        the last collision step (call after adding the colliders)."""
+        self.lib.jo_world_record_contacts(self.h)
+
+    def contacts(self, body):
+        """(collider index per vertex, -1 without contact; local contact points (V, 3))."""
+        nv = self.sizes[body]
+        col = np.zeros(nv, np.int32)
+        pts = np.zeros((nv, 3), np.float32)
+        if self.lib.jo_world_contacts(self.h, body, _ptr(col), _ptr(pts)) != 0:
+            raise RuntimeError("contact recording is not enabled")
+        return col, pts
 
     def collider_motion(self, i):
         """(EMotionType, {inv_mass, inv_inertia_diagonal, inertia_rotation, linear_velocity,

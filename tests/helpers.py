@@ -41,6 +41,8 @@ def run_gpu(scene, settings_list, bounds=None, frames_to_keep=(1,), device=0):
         ctx.update(scene.dt, scene.collision_steps, f - done)
         done = f
         out[f] = [ctx.read(i) for i in ids]
+        for i, r in zip(ids, out[f]):
+            r["contact_collider"], r["contact_point"] = ctx.contacts(i)
     ctx.close()
     return out
 
@@ -73,7 +75,7 @@ def float_equal(a, b):
     return a.shape == b.shape and bool(np.all(a == b))
 
 
-def reference_run(scene, frames_to_keep, variant="nofma", num_threads=0):
+def reference_run(scene, frames_to_keep, variant="nofma", num_threads=0, record_contacts=False):
     """Builds settings with the reference's own code and runs PhysicsSystem::Update.
 
     Returns (settings_list, collider_bounds[n,6], {frame: [read(body)...]})."""
@@ -86,6 +88,8 @@ def reference_run(scene, frames_to_keep, variant="nofma", num_threads=0):
     world = JoltWorld(ref, scene.gravity, num_threads=num_threads)
     ids = [world.add_collider(c) for c in scene.colliders]
     bounds = np.array([world.collider_bounds(i) for i in ids], np.float32).reshape(-1, 6)
+    if record_contacts:
+        world.record_contacts()
     # the colliders as the update consumes them, as the reference derived them
     for c, i in zip(scene.colliders, ids):
         c.device_params, c.com = world.collider_device(i)
@@ -100,6 +104,9 @@ def reference_run(scene, frames_to_keep, variant="nofma", num_threads=0):
         world.update(scene.dt, scene.collision_steps, f - done)
         done = f
         out[f] = [world.read(i) for i in range(len(scene.bodies))]
+        if record_contacts:
+            for i, r in enumerate(out[f]):
+                r["contact_collider"], r["contact_point"] = world.contacts(i)
     world.close()
     return settings, bounds, out
 
