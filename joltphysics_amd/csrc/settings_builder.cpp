@@ -470,20 +470,101 @@ struct Builder
             add_connection(c.v[1], c.v[2]); add_connection(c.v[1], c.v[3]); add_connection(c.v[2], c.v[3]);
         }
 
+        // Greedy grouping, the reference's loop (SoftBodySharedSettings.cpp:672-774) with
+        // the same choices made without its quadratic rescans:
+        // * the bounds and the start vertex of the ungrouped vertices come from sorted
+        //   orders (per axis, and per axis priority with index as the last key) read
+        //   from lazily advanced cursors that skip grouped vertices;
+        // * the vertex most connected to the current group is kept incrementally: each
+        //   ungrouped neighbour of the group carries its connection count to the group,
+        //   its distance to the group's first vertex and its first position in the
+        //   reference's scan (group member index, connection index); the reference
+        //   keeps the first of equal (count, distance) pairs in that scan, so the
+        //   choice is the maximum of (count, -distance, -first position).
         std::vector<int> group_idx(nv, -1);
         int current_group_idx = 0;
         std::vector<uint32_t> current_group;
+
+        std::vector<uint32_t> by_axis[3];
+        for (int k = 0; k < 3; ++k)
+        {
+            by_axis[k].resize(nv);
+            for (uint32_t i = 0; i < nv; ++i) by_axis[k][i] = i;
+            std::sort(by_axis[k].begin(), by_axis[k].end(), [&](uint32_t a, uint32_t b) { return vertices[a].pos[k] < vertices[b].pos[k]; });
+        }
+        size_t lo_cursor[3] = { 0, 0, 0 }, hi_cursor[3] = { nv, nv, nv };
+        std::vector<uint32_t> by_priority[9];       // index 3 * max_axis + min_axis
+        size_t priority_cursor[9] = {};
+        auto priority_order = [&](uint32_t a0, uint32_t a1, uint32_t a2) -> std::vector<uint32_t> & {
+            std::vector<uint32_t> &o = by_priority[3 * a0 + a2];
+            if (o.empty() && nv)
+            {
+                o.resize(nv);
+                for (uint32_t i = 0; i < nv; ++i) o[i] = i;
+                std::sort(o.begin(), o.end(), [&](uint32_t a, uint32_t b) {
+                    const float *p = vertices[a].pos, *q = vertices[b].pos;
+                    if (p[a0] < q[a0]) return true;
+                    if (!(p[a0] == q[a0])) return false;
+                    if (p[a1] < q[a1]) return true;
+                    if (!(p[a1] == q[a1])) return false;
+                    if (p[a2] < q[a2]) return true;
+                    if (!(p[a2] == q[a2])) return false;
+                    return a < b;
+                });
+            }
+            return o;
+        };
+
+        // candidates of the current group
+        std::vector<uint32_t> cand_num(nv, 0);
+        std::vector<uint64_t> cand_first(nv, 0);
+        std::vector<float> cand_dist(nv, 0.0f);
+        std::vector<char> is_cand(nv, 0);
+        std::vector<uint32_t> cand_list;
+        auto join = [&](uint32_t u) {
+            const uint32_t member = (uint32_t)current_group.size();
+            current_group.push_back(u);
+            group_idx[u] = current_group_idx;
+            is_cand[u] = 0;
+            const V3 front = X(current_group.front());
+            const std::vector<Connection> &cu = conn[u];
+            for (uint32_t k = 0; k < (uint32_t)cu.size(); ++k)
+            {
+                const uint32_t v = cu[k].vertex;
+                if (group_idx[v] != -1)
+                    continue;
+                if (!is_cand[v])
+                {
+                    is_cand[v] = 1;
+                    cand_num[v] = 0;
+                    cand_first[v] = ((uint64_t)member << 32) | k;
+                    cand_dist[v] = len_sq(X(v) - front);
+                    cand_list.push_back(v);
+                }
+                cand_num[v] += cu[k].count;
+            }
+        };
+        auto clear_candidates = [&]() {
+            for (uint32_t v : cand_list) is_cand[v] = 0;
+            cand_list.clear();
+        };
+
         for (;;)
         {
             // bounds of the ungrouped vertices (AABox::Encapsulate)
-            V3 bmin = v3(FLT_MAX, FLT_MAX, FLT_MAX), bmax = v3(-FLT_MAX, -FLT_MAX, -FLT_MAX);
-            for (uint32_t i = 0; i < nv; ++i)
-                if (group_idx[i] == -1)
-                {
-                    bmin = vmin(bmin, X(i));
-                    bmax = vmax(bmax, X(i));
-                }
-            if (!(bmin.x <= bmax.x && bmin.y <= bmax.y && bmin.z <= bmax.z))
+            bool any = false;
+            V3 bmin, bmax;
+            for (int k = 0; k < 3; ++k)
+            {
+                while (lo_cursor[k] < nv && group_idx[by_axis[k][lo_cursor[k]]] != -1) ++lo_cursor[k];
+                while (hi_cursor[k] > lo_cursor[k] && group_idx[by_axis[k][hi_cursor[k] - 1]] != -1) --hi_cursor[k];
+                any = lo_cursor[k] < hi_cursor[k];
+                if (!any)
+                    break;
+                (&bmin.x)[k] = vertices[by_axis[k][lo_cursor[k]]].pos[k];
+                (&bmax.x)[k] = vertices[by_axis[k][hi_cursor[k] - 1]].pos[k];
+            }
+            if (!any)
                 break;
             V3 size = bmax - bmin;
             uint32_t max_axis = size.x > size.y ? (size.z > size.x ? 2 : 0) : (size.z > size.y ? 2 : 1);
@@ -492,57 +573,36 @@ struct Builder
                 min_axis = (min_axis + 1) % 3;
             uint32_t mid_axis = 3 - min_axis - max_axis;
 
-            uint32_t current_vertex = UINT_MAX;
-            float cur[3] = { FLT_MAX, FLT_MAX, FLT_MAX };
-            for (uint32_t i = 0; i < nv; ++i)
-                if (group_idx[i] == -1)
-                {
-                    const float *p = vertices[i].pos;
-                    float a = p[max_axis], m = p[mid_axis], n = p[min_axis];
-                    if (a < cur[max_axis] || (a == cur[max_axis] && (m < cur[mid_axis] || (m == cur[mid_axis] && n < cur[min_axis]))))
-                    {
-                        cur[0] = p[0]; cur[1] = p[1]; cur[2] = p[2];
-                        current_vertex = i;
-                    }
-                }
-            if (current_vertex == UINT_MAX)
+            // the ungrouped vertex lowest on (max, mid, min axis), the lowest index among equals
+            std::vector<uint32_t> &order = priority_order(max_axis, mid_axis, min_axis);
+            size_t &cursor = priority_cursor[3 * max_axis + min_axis];
+            while (cursor < order.size() && group_idx[order[cursor]] != -1) ++cursor;
+            if (cursor == order.size())
                 break;
-            current_group.push_back(current_vertex);
-            group_idx[current_vertex] = current_group_idx;
+            join(order[cursor]);
 
             for (;;)
             {
                 uint32_t best_vertex = UINT_MAX;
-                uint32_t best_num = 0;
-                float best_dist_sq = FLT_MAX;
-                const V3 front = X(current_group.front());
-                for (uint32_t gi = 0; gi < (uint32_t)current_group.size(); ++gi)
-                    for (const Connection &c : conn[current_group[gi]])
-                    {
-                        uint32_t v = c.vertex;
-                        if (group_idx[v] == -1)
-                        {
-                            uint32_t num = 0;
-                            for (const Connection &c2 : conn[v])
-                                if (group_idx[c2.vertex] == current_group_idx)
-                                    num += c2.count;
-                            float dist_sq = len_sq(X(v) - front);
-                            if (best_vertex == UINT_MAX || num > best_num || (num == best_num && dist_sq < best_dist_sq))
-                            {
-                                best_vertex = v;
-                                best_num = num;
-                                best_dist_sq = dist_sq;
-                            }
-                        }
-                    }
-                if (best_vertex != UINT_MAX)
+                size_t live = 0;
+                for (uint32_t v : cand_list)
                 {
-                    current_group.push_back(best_vertex);
-                    group_idx[best_vertex] = current_group_idx;
+                    if (!is_cand[v])
+                        continue;
+                    cand_list[live++] = v;
+                    if (best_vertex == UINT_MAX || cand_num[v] > cand_num[best_vertex]
+                        || (cand_num[v] == cand_num[best_vertex]
+                            && (cand_dist[v] < cand_dist[best_vertex]
+                                || (cand_dist[v] == cand_dist[best_vertex] && cand_first[v] < cand_first[best_vertex]))))
+                        best_vertex = v;
                 }
+                cand_list.resize(live);
+                if (best_vertex != UINT_MAX)
+                    join(best_vertex);
                 if (current_group.size() >= kBatch || (current_group.size() > kBatch / 2 && best_vertex == UINT_MAX))
                 {
                     current_group.clear();
+                    clear_candidates();
                     current_group_idx++;
                     break;
                 }

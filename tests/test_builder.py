@@ -40,6 +40,21 @@ def _meshes():
     v, f = scenes.cloth(24, 0.02, 1.0)
     out["cloth24_no_bend"] = scenes.MeshSpec(vertices=v, faces=f, attrs=scenes.attrs(1e-5, 2e-5, 1e-3), bend_type=0)
     out["block6_explicit_lra"] = scenes.config5_block(cells=6, num_boxes=1).bodies[0].mesh   # explicit LRAs + lengths
+    # grouping stress: exact distance ties on regular grids, and islands that end a
+    # group's growth below half a group (the greedy search restarts from a new vertex)
+    v, f = scenes.cloth(48, 0.02, 1.0)
+    out["cloth48_grid_ties"] = scenes.MeshSpec(vertices=v, faces=f, attrs=scenes.attrs(1e-5, 1e-5, 1e-3))
+    v, e, t = scenes.tet_cube(12, 0.05)
+    out["tet12_grid_ties"] = scenes.MeshSpec(vertices=v, edges=e, volumes=t,
+                                             flags=scenes.CALC_EDGE_LENGTHS | scenes.CALC_VOLUMES | scenes.OPTIMIZE)
+    parts_v, parts_f = [], []
+    for k, n in enumerate((7, 9, 5, 11, 6, 8)):
+        v, f = scenes.cloth(n, 0.02, 1.0, jitter=1e-4, seed=20 + k)
+        v["pos"][:, 0] += 0.5 * k
+        parts_f.append(f + sum(len(p) for p in parts_v))
+        parts_v.append(v)
+    out["cloth_islands"] = scenes.MeshSpec(vertices=np.concatenate(parts_v), faces=np.concatenate(parts_f),
+                                           attrs=scenes.attrs(1e-5, 1e-5, 1e-3))
     return out
 
 
