@@ -274,6 +274,28 @@ int xpbd_body_velocity_deltas(xpbd_context *ctx, int32_t body, uint32_t num_coll
 int xpbd_body_contacts(xpbd_context *ctx, int32_t body, float *points, int32_t *collider);
 
 /*
+ * The byte stream SoftBodyMotionProperties::SaveState writes for a soft body without
+ * rods or skinning (SoftBodyMotionProperties.cpp:1446-1473, MotionProperties.cpp:63-76):
+ * linear and angular velocity (world), force and torque (zero between updates), the three
+ * sleep test spheres (not used by soft bodies; written as zeros), sleep timer, allow
+ * sleeping (1 byte), then position and velocity of every vertex (body space), then the
+ * local bounds and local predicted bounds (min, max each); Vec3s as 3 floats.  With `out`
+ * NULL only *size is set.  The body's position and rotation are Body state, not part
+ * of the stream (xpbd_body_read).
+ */
+int xpbd_body_save_state(xpbd_context *ctx, int32_t body, void *out, size_t capacity, size_t *size);
+
+/*
+ * Restores a stream of that layout (SoftBodyMotionProperties::RestoreState,
+ * SoftBodyMotionProperties.cpp:1475-1502): velocities, sleep timer and allow-sleeping,
+ * vertex positions and velocities, bounds; force, torque and the sleep spheres are read
+ * and ignored.  position[3] / rotation[4] (may be NULL) set the body's position and
+ * rotation, as Body::RestoreState would.  Fails on a size mismatch.
+ */
+int xpbd_body_restore_state(xpbd_context *ctx, int32_t body, const void *in, size_t size, const float *position,
+                            const float *rotation);
+
+/*
  * Kernel timing on the context stream (hipEvents around every launch of
  * xpbd_update while enabled).  xpbd_profile_read returns the number of timed
  * launches and their summed duration in milliseconds, then resets.

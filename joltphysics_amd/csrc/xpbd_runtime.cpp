@@ -1625,6 +1625,100 @@ int xpbd_body_state(xpbd_context *ctx, int32_t body, float out[14])
     return XPBD_OK;
 }
 
+namespace {
+constexpr size_t kStateHeaderBytes = 4 * 12 + 48 + 4 + 1;   // velocities, force, torque, spheres, timer, allow sleeping
+size_t StateBytes(uint32_t nv) { return kStateHeaderBytes + (size_t)nv * 24 + 4 * 12; }
+} // namespace
+
+int xpbd_body_save_state(xpbd_context *ctx, int32_t body, void *out, size_t capacity, size_t *size)
+{
+    if (!ctx || body < 0 || body >= (int32_t)ctx->bodies.size() || !size)
+        return Fail(XPBD_E_INVALID, "invalid body or size pointer");
+    const Body &b = *ctx->bodies[body];
+    *size = StateBytes(b.nv);
+    if (!out)
+        return XPBD_OK;
+    if (capacity < *size)
+        return Fail(XPBD_E_INVALID, "state buffer too small");
+    int rc = xpbd_synchronize(ctx);
+    if (rc != XPBD_OK)
+        return rc;
+    if ((rc = PullStates(ctx)) != XPBD_OK)
+        return rc;
+    const BodyStateDev &s = b.state;
+    std::vector<float4> x(b.nv), v(b.nv);
+    if (b.nv)
+    {
+        HIP_TRY(hipMemcpy(x.data(), b.dev.x, sizeof(float4) * b.nv, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(v.data(), b.dev.v, sizeof(float4) * b.nv, hipMemcpyDeviceToHost));
+    }
+    unsigned char *p = (unsigned char *)out;
+    auto put = [&p](const void *src, size_t n) { memcpy(p, src, n); p += n; };
+    const float zero[12] = {};
+    put(s.lin, 12); put(s.ang, 12);
+    put(zero, 12); put(zero, 12);               // force, torque
+    put(zero, 48);                              // sleep test spheres
+    put(&s.sleep_timer, 4);
+    const unsigned char allow = b.dev.allow_sleeping ? 1 : 0;
+    put(&allow, 1);
+    for (uint32_t i = 0; i < b.nv; ++i)
+    {
+        put(&x[i], 12);
+        put(&v[i], 12);
+    }
+    put(s.bmin, 12); put(s.bmax, 12); put(s.pmin, 12); put(s.pmax, 12);
+    return XPBD_OK;
+}
+
+int xpbd_body_restore_state(xpbd_context *ctx, int32_t body, const void *in, size_t size, const float *position,
+                            const float *rotation)
+{
+    if (!ctx || body < 0 || body >= (int32_t)ctx->bodies.size() || !in)
+        return Fail(XPBD_E_INVALID, "invalid body or stream");
+    Body &b = *ctx->bodies[body];
+    if (size != StateBytes(b.nv))
+        return Fail(XPBD_E_INVALID, "state stream size does not match the body");
+    int rc = xpbd_synchronize(ctx);
+    if (rc != XPBD_OK)
+        return rc;
+    if ((rc = PullStates(ctx)) != XPBD_OK)
+        return rc;
+    const unsigned char *p = (const unsigned char *)in;
+    auto get = [&p](void *dst, size_t n) { memcpy(dst, p, n); p += n; };
+    BodyStateDev &s = b.state;
+    get(s.lin, 12); get(s.ang, 12);
+    p += 12 + 12 + 48;                          // force, torque, sleep test spheres
+    get(&s.sleep_timer, 4);
+    unsigned char allow = 0;
+    get(&allow, 1);
+    if ((b.dev.allow_sleeping != 0) != (allow != 0))
+        ctx->layout_dirty = true;   // bundles record whether a body may sleep
+    b.dev.allow_sleeping = allow ? 1 : 0;
+    std::vector<float4> x(b.nv), v(b.nv);
+    if (b.nv)
+        HIP_TRY(hipMemcpy(x.data(), b.dev.x, sizeof(float4) * b.nv, hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < b.nv; ++i)
+    {
+        float q[6];
+        get(q, 24);
+        x[i] = make_float4(q[0], q[1], q[2], x[i].w);
+        v[i] = make_float4(q[3], q[4], q[5], 0.0f);
+    }
+    get(s.bmin, 12); get(s.bmax, 12); get(s.pmin, 12); get(s.pmax, 12);
+    if (position) memcpy(s.pos, position, 12);
+    if (rotation) memcpy(s.rot, rotation, 16);
+    if (b.nv)
+    {
+        HIP_TRY(hipMemcpy(b.dev.x, x.data(), sizeof(float4) * b.nv, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b.dev.v, v.data(), sizeof(float4) * b.nv, hipMemcpyHostToDevice));
+    }
+    // the host copies of all body states are current (pulled above) and now hold the
+    // restored one: they go up with the body table at the next update
+    ctx->states_on_device = false;
+    ctx->tables_dirty = true;
+    return XPBD_OK;
+}
+
 int xpbd_body_contacts(xpbd_context *ctx, int32_t body, float *points, int32_t *collider)
 {
     if (!ctx || body < 0 || body >= (int32_t)ctx->bodies.size())

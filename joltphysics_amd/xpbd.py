@@ -95,8 +95,8 @@ EXPORTED_SYMBOLS = (
     "xpbd_settings_build", "xpbd_built_counts", "xpbd_built_read", "xpbd_built_free",
     "xpbd_build_last_error", "xpbd_body_create",
     "xpbd_set_colliders", "xpbd_set_gravity", "xpbd_update", "xpbd_synchronize", "xpbd_body_read",
-    "xpbd_body_write", "xpbd_body_state", "xpbd_body_velocity_deltas", "xpbd_body_contacts", "xpbd_profile_enable",
-    "xpbd_profile_read",
+    "xpbd_body_write", "xpbd_body_state", "xpbd_body_velocity_deltas", "xpbd_body_contacts", "xpbd_body_save_state",
+    "xpbd_body_restore_state", "xpbd_profile_enable", "xpbd_profile_read",
     "xpbd_rollout_create", "xpbd_rollout_destroy", "xpbd_rollout_sizes", "xpbd_rollout_forward",
     "xpbd_rollout_forward_f64", "xpbd_rollout_backward",
 )
@@ -145,6 +145,8 @@ def load_library(path=LIB_PATH):
     lib.xpbd_body_state.argtypes = [vp, i32, vp]
     lib.xpbd_body_velocity_deltas.argtypes = [vp, i32, u32, vp, vp, vp]
     lib.xpbd_body_contacts.argtypes = [vp, i32, vp, vp]
+    lib.xpbd_body_save_state.argtypes = [vp, i32, vp, C.c_size_t, C.POINTER(C.c_size_t)]
+    lib.xpbd_body_restore_state.argtypes = [vp, i32, vp, C.c_size_t, vp, vp]
     lib.xpbd_profile_enable.argtypes = [vp, i32]
     lib.xpbd_profile_read.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_double)]
     lib.xpbd_rollout_create.argtypes = [vp, C.POINTER(_RolloutDesc), C.POINTER(vp)]
@@ -294,6 +296,21 @@ class Context:
                     arr[i].inertia_rotation[k] = float(c.inertia_rotation[k])
         _check(self.lib.xpbd_set_colliders(self.h, arr, n))
         self._num_colliders = n
+
+    def save_state(self, body) -> bytes:
+        """SoftBodyMotionProperties::SaveState's byte stream of `body` (see the C ABI)."""
+        n = C.c_size_t()
+        _check(self.lib.xpbd_body_save_state(self.h, int(body), None, 0, C.byref(n)))
+        buf = C.create_string_buffer(n.value)
+        _check(self.lib.xpbd_body_save_state(self.h, int(body), buf, n.value, C.byref(n)))
+        return buf.raw[:n.value]
+
+    def restore_state(self, body, data: bytes, position=None, rotation=None):
+        """SoftBodyMotionProperties::RestoreState from `data`; optionally the body's
+        position and rotation too."""
+        pos = None if position is None else np.ascontiguousarray(position, np.float32)
+        rot = None if rotation is None else np.ascontiguousarray(rotation, np.float32)
+        _check(self.lib.xpbd_body_restore_state(self.h, int(body), data, len(data), _ptr(pos), _ptr(rot)))
 
     def contacts(self, body):
         """(collider index per vertex, -1 without contact; local contact points (V, 3)) of the

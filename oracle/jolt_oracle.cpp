@@ -48,6 +48,7 @@
 #include <Jolt/Physics/SoftBody/SoftBodyContactListener.h>
 #include <Jolt/Physics/SoftBody/SoftBodyManifold.h>
 #include <Jolt/Physics/PhysicsStepListener.h>
+#include <Jolt/Physics/StateRecorderImpl.h>
 #include <map>
 
 #include <chrono>
@@ -661,6 +662,36 @@ int jo_body_custom_update(void *wp, int body, float dt)
 	lock.ReleaseLock();
 	mp->CustomUpdate(dt, b, w->mSystem);
 	return 0;
+}
+
+// SoftBodyMotionProperties::SaveState of soft body `body` into out (capacity bytes);
+// returns the stream size, or -1 (out may be null to query the size).
+long jo_body_save_state(void *wp, int body, void *out, long capacity)
+{
+	World *w = (World *)wp;
+	BodyLockRead lock(w->mSystem.GetBodyLockInterface(), w->mSoftBodies[body]);
+	if (!lock.Succeeded())
+		return -1;
+	StateRecorderImpl rec;
+	static_cast<const SoftBodyMotionProperties *>(lock.GetBody().GetMotionProperties())->SaveState(rec);
+	std::string data = rec.GetData();
+	if (out && long(data.size()) <= capacity)
+		memcpy(out, data.data(), data.size());
+	return long(data.size());
+}
+
+// SoftBodyMotionProperties::RestoreState of soft body `body` from a stream of `size` bytes.
+int jo_body_restore_state(void *wp, int body, const void *in, long size)
+{
+	World *w = (World *)wp;
+	BodyLockWrite lock(w->mSystem.GetBodyLockInterface(), w->mSoftBodies[body]);
+	if (!lock.Succeeded())
+		return -1;
+	StateRecorderImpl rec;
+	rec.WriteBytes(in, size_t(size));
+	rec.Rewind();
+	static_cast<SoftBodyMotionProperties *>(lock.GetBody().GetMotionProperties())->RestoreState(rec);
+	return rec.IsFailed() ? -1 : 0;
 }
 
 // Vertex state in body-local space.  Any output may be null.

@@ -96,6 +96,10 @@ class JoltReference:
             lib.jo_world_collider_motion.restype = _i
             lib.jo_world_record_contacts.argtypes = [_vp]
             lib.jo_world_contacts.restype = _i
+            lib.jo_body_save_state.restype = C.c_long
+            lib.jo_body_save_state.argtypes = [_vp, _i, _vp, C.c_long]
+            lib.jo_body_restore_state.restype = _i
+            lib.jo_body_restore_state.argtypes = [_vp, _i, _vp, C.c_long]
             lib.jo_world_contacts.argtypes = [_vp, _i, _vp, _vp]
             lib.jo_world_collider_motion.argtypes = [_vp, _i, _vp]
             lib.jo_world_hull_data.argtypes = [_vp, _i] + [_vp] * 6
@@ -212,6 +216,17 @@ class JoltWorld:
         com = np.zeros(3, np.float32)
         self.lib.jo_world_collider_device(self.h, i, _ptr(params), _ptr(com))
         return tuple(float(x) for x in params), tuple(float(x) for x in com)
+
+    def save_state(self, body):
+        """SoftBodyMotionProperties::SaveState bytes of soft body `body`."""
+        n = self.lib.jo_body_save_state(self.h, body, None, 0)
+        buf = C.create_string_buffer(n)
+        self.lib.jo_body_save_state(self.h, body, buf, n)
+        return buf.raw[:n]
+
+    def restore_state(self, body, data):
+        if self.lib.jo_body_restore_state(self.h, body, data, len(data)) != 0:
+            raise RuntimeError("RestoreState failed")
 
     def record_contacts(self):
         """Installs the contact listener that records each soft body's SoftBodyManifold of
