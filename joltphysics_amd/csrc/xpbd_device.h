@@ -117,6 +117,82 @@ __device__ __forceinline__ void project_bend(const P &X, uint32_t i0, uint32_t i
     pset(X, i3, f4(x3 - (minus_lambda * w3) * d3c, w3));
 }
 
+// The same projection shared by two adjacent lanes (lanes 2k, 2k + 1 hold the two
+// halves of one record; `second` tells them apart).  Every value is computed with the
+// reference's operations in the reference's order, the two normals, the two
+// normalisations, two of the three divisions of d0 and the squared lengths of the
+// gradients on one lane each, exchanged with the partner lane by a DPP quad
+// permutation; each lane then moves two of the four vertices.  Both lanes take every
+// early return together, since they decide on the same values.
+__device__ __forceinline__ float swap_pair(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0xb1, 0xf, 0xf, true));   // quad_perm [1,0,3,2]
+}
+__device__ __forceinline__ V3 swap_pair(V3 v) { return v3(swap_pair(v.x), swap_pair(v.y), swap_pair(v.z)); }
+__device__ __forceinline__ V3 select(bool c, V3 a, V3 b) { return v3(c ? a.x : b.x, c ? a.y : b.y, c ? a.z : b.z); }
+
+template <class P>
+__device__ __forceinline__ void project_bend_pair(const P &X, uint32_t i0, uint32_t i1, uint32_t i2, uint32_t i3, float compliance,
+                                                  float initial_angle, float inv_dt_sq, bool second)
+{
+    const float4 p0 = pget(X, i0), p1 = pget(X, i1), p2 = pget(X, i2), p3 = pget(X, i3);
+    const V3 x0 = xyz(p0), x1 = xyz(p1), x2 = xyz(p2), x3 = xyz(p3);
+    const V3 e = x1 - x0;
+    const float e_len = len(e);
+    if (e_len < 1.0e-6f)
+        return;
+    const V3 x1x2 = x2 - x1;
+    const V3 x1x3 = x3 - x1;
+    // this lane's normal: n1 = (x2 - x0) x x1x2 on the first, n2 = x1x3 x (x3 - x0) on the second
+    const V3 nm = cross(select(second, x1x3, x2 - x0), select(second, x3 - x0, x1x2));
+    const float nm_len_sq = len_sq(nm);
+    const V3 no = swap_pair(nm);
+    const float no_len_sq = swap_pair(nm_len_sq);
+    V3 n1 = select(second, no, nm), n2 = select(second, nm, no);
+    const float n1_len_sq = second ? no_len_sq : nm_len_sq, n2_len_sq = second ? nm_len_sq : no_len_sq;
+    const float n1n2 = n1_len_sq * n2_len_sq;
+    if (n1n2 < 1.0e-24f)
+        return;
+    const float sign = sign_of(dot(cross(n2, n1), e));
+    const float d = dot(n1, n2) / sqrtf(n1n2);
+    float c = sign * acos_approx(d) - initial_angle;
+    if (c > kPi)
+        c -= 2.0f * kPi;
+    else if (c < -kPi)
+        c += 2.0f * kPi;
+    // n1 / |n1|^2 on the first lane, n2 / |n2|^2 on the second
+    const V3 um = nm / nm_len_sq;
+    const V3 uo = swap_pair(um);
+    n1 = select(second, uo, um);
+    n2 = select(second, um, uo);
+    // dot(x1x2, e) on the first lane, dot(x1x3, e) on the second
+    const float dm = dot(select(second, x1x3, x1x2), e);
+    const float dother = swap_pair(dm);
+    const V3 t = (second ? dother : dm) * n1 + (second ? dm : dother) * n2;
+    // d0 = t / |e|: x on the first lane, z on the second, y on both
+    const float qm = (second ? t.z : t.x) / e_len;
+    const float qy = t.y / e_len;
+    const float qo = swap_pair(qm);
+    const V3 d0c = v3(second ? qo : qm, qy, second ? qm : qo);
+    const V3 d2c = e_len * n1;
+    const V3 d3c = e_len * n2;
+    const V3 d1c = (neg(d0c) - d2c) - d3c;
+    const float w0 = p0.w, w1 = p1.w, w2 = p2.w, w3 = p3.w;
+    // |d0|^2, |d2|^2 on the first lane, |d1|^2, |d3|^2 on the second
+    const V3 da = select(second, d1c, d0c), db = select(second, d3c, d2c);
+    const float la = len_sq(da), lb = len_sq(db);
+    const float lao = swap_pair(la), lbo = swap_pair(lb);
+    const float l0 = second ? lao : la, l1 = second ? la : lao, l2 = second ? lbo : lb, l3 = second ? lb : lbo;
+    const float denom = (((w0 * l0 + w1 * l1) + w2 * l2) + w3 * l3) + compliance * inv_dt_sq;
+    if (denom < 1.0e-12f)
+        return;
+    const float minus_lambda = c / denom;
+    // the first lane moves x0 and x2, the second x1 and x3
+    const float wa = second ? w1 : w0, wb = second ? w3 : w2;
+    pset(X, second ? i1 : i0, f4(select(second, x1, x0) - (minus_lambda * wa) * da, wa));
+    pset(X, second ? i3 : i2, f4(select(second, x3, x2) - (minus_lambda * wb) * db, wb));
+}
+
 // ApplyVolumeConstraints body (SoftBodyMotionProperties.cpp:467-506)
 template <class P>
 __device__ __forceinline__ void project_volume(const P &X, uint32_t i1, uint32_t i2, uint32_t i3, uint32_t i4, float six_rest_volume, float compliance, float inv_dt_sq)
@@ -163,12 +239,12 @@ __device__ __forceinline__ void project_record(const P &X, uint4 r, float inv_dt
     if (r.x == kNopRecord)
         return;
     uint32_t type = (r.x >> 14) & 3u;
-    uint32_t i0 = r.x & 0x3fffu, i1 = r.x >> 16, i2 = r.y & 0xffffu, i3 = r.y >> 16;
+    uint32_t i0 = r.x & 0x3fffu, i1 = r.x >> 16, i2 = r.y & 0x3fffu, i3 = r.y >> 16;
     float p0 = __uint_as_float(r.z), p1 = __uint_as_float(r.w);
     switch (type)
     {
     case kTypeEdge: project_edge(X, i0, i1, p0, p1, inv_dt_sq); break;
-    case kTypeBend: project_bend(X, i0, i1, i2, i3, p0, p1, inv_dt_sq); break;
+    case kTypeBend: project_bend_pair(X, i0, i1, i2, i3, p0, p1, inv_dt_sq, ((r.y >> 15) & 1u) != 0); break;
     case kTypeVolume: project_volume(X, i0, i1, i2, i3, p0, p1, inv_dt_sq); break;
     default: project_lra(X, i0, i1, p0); break;
     }

@@ -58,6 +58,7 @@ int Fail(int code, const std::string &msg)
 
 constexpr uint32_t kMaxLdsBytes = 160 * 1024 - 1024;
 constexpr uint32_t kNopRecordHost = 0xffffffffu;    // padding record (skipped by the kernels)
+constexpr uint32_t kSecondHalf = 1u << 15;          // bend record: the second lane's half (in the vertex-2 field)
 
 // Reference record layouts (SoftBodySharedSettings.h)
 struct VertexRec { float pos[3], vel[3], inv_mass; };
@@ -293,8 +294,9 @@ void AppendTypedLevel(std::vector<uint4> bucket[4], std::vector<uint4> &recs, bo
     size_t n = 0, padded = 0;
     for (int t = 0; t < 4; ++t)
     {
-        n += bucket[t].size();
-        padded += (bucket[t].size() + 63) / 64 * 64;
+        const size_t slots = bucket[t].size() * (t == (int)kTypeBend ? 2 : 1);
+        n += slots;
+        padded += (slots + 63) / 64 * 64;
     }
     const bool pad = allow_pad && (padded + block - 1) / block <= (n + block - 1) / block;
     const size_t base = recs.size();
@@ -302,9 +304,30 @@ void AppendTypedLevel(std::vector<uint4> bucket[4], std::vector<uint4> &recs, bo
     const int order[4] = { (int)kTypeBend, (int)kTypeVolume, (int)kTypeEdge, (int)kTypeLRA };
     for (int t : order)
     {
-        recs.insert(recs.end(), bucket[t].begin(), bucket[t].end());
-        if (rec_index)
-            rec_index->insert(rec_index->end(), ibucket[t].begin(), ibucket[t].end());
+        if (t == (int)kTypeBend)
+        {
+            // a bend occupies two adjacent lanes: the record, then its second half
+            // (bit 15 of the vertex-2 field set); the group starts the level, so the
+            // halves sit at an even and the following odd lane
+            for (size_t k = 0; k < bucket[t].size(); ++k)
+            {
+                uint4 second = bucket[t][k];
+                second.y |= kSecondHalf;
+                recs.push_back(bucket[t][k]);
+                recs.push_back(second);
+                if (rec_index)
+                {
+                    rec_index->push_back(ibucket[t][k]);
+                    rec_index->push_back(ibucket[t][k]);
+                }
+            }
+        }
+        else
+        {
+            recs.insert(recs.end(), bucket[t].begin(), bucket[t].end());
+            if (rec_index)
+                rec_index->insert(rec_index->end(), ibucket[t].begin(), ibucket[t].end());
+        }
         if (pad)
             while ((recs.size() - base) % 64 != 0)
             {
@@ -688,7 +711,7 @@ uint4 OffsetRecord(uint4 r, uint32_t voff)
 {
     uint32_t type = (r.x >> 14) & 3u;
     uint32_t v0 = (r.x & 0x3fffu) + voff, v1 = (r.x >> 16) + voff;
-    uint32_t v2 = (r.y & 0xffffu) + voff, v3 = (r.y >> 16) + voff;
+    uint32_t v2 = (r.y & 0x3fffu) + voff, v3 = (r.y >> 16) + voff;
     uint4 o;
     o.x = (v0 & 0x3fffu) | (type << 14) | (v1 << 16);
     o.y = (v2 & 0xffffu) | (v3 << 16);
@@ -718,6 +741,8 @@ void BuildBundleProgram(const xpbd_context *ctx, const BundleHost &bh, const std
             {
                 if (st.host_recs[r].x == kNopRecordHost)
                     continue;
+                if (((st.host_recs[r].x >> 14) & 3u) == kTypeBend && (st.host_recs[r].y & kSecondHalf))
+                    continue;       // re-emitted with its first half
                 uint4 rec = OffsetRecord(st.host_recs[r], voffs[j]);
                 bucket[(rec.x >> 14) & 3u].push_back(rec);
             }
