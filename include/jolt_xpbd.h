@@ -96,10 +96,17 @@ typedef struct xpbd_settings_desc {
     const void *volumes;        uint32_t num_volumes;
     const void *lras;           uint32_t num_lras;
     const void *update_groups;  uint32_t num_update_groups;
-    /* must be 0: skinned constraints and Cosserat rods are not part of this build */
+    /* skinned constraints: Skinned records (SoftBodySharedSettings.h:252-285, 52 B:
+     * vertex, 4 x {invBindIndex, weight}, maxDistance, backStopDistance,
+     * backStopRadius, normalInfo), at most one per vertex */
     uint32_t num_skinned;
+    /* must be 0: Cosserat rods are not part of this build */
     uint32_t num_rod_stretch_shear;
     uint32_t num_rod_bend_twist;
+    const void *skinned;
+    const uint32_t *skinned_normals;    uint32_t num_skinned_normals;   /* mSkinnedConstraintNormals */
+    const void *inv_bind_matrices;      uint32_t num_inv_bind_matrices; /* InvBind records (80 B: jointIndex,
+                                                                           12 B padding, Mat44 column major) */
 } xpbd_settings_desc;
 
 int xpbd_settings_create(xpbd_context *ctx, const xpbd_settings_desc *desc, xpbd_settings **out_settings);
@@ -283,6 +290,19 @@ int xpbd_body_contacts(xpbd_context *ctx, int32_t body, float *points, int32_t *
  * NULL only *size is set.  The body's position and rotation are Body state, not part
  * of the stream (xpbd_body_read).
  */
+/*
+ * SoftBodyMotionProperties::SkinVertices (SoftBodyMotionProperties.cpp:1139-1226): skins
+ * the skinned vertices with `num_joints` joint matrices (16 floats each, column major,
+ * in the body's centre-of-mass space), updates their skin states and normals, and with
+ * hard_skin_all sets every skinned vertex to its skinned position with zero velocity
+ * (otherwise, with skin constraints disabled, only the kinematic ones, max distance 0).
+ */
+int xpbd_body_skin_vertices(xpbd_context *ctx, int32_t body, const float *joint_matrices, uint32_t num_joints,
+                            int32_t hard_skin_all);
+/* SetEnableSkinConstraints / SetSkinnedMaxDistanceMultiplier (SoftBodyMotionProperties.h:93-98) */
+int xpbd_body_set_skin_options(xpbd_context *ctx, int32_t body, int32_t enable_skin_constraints,
+                               float max_distance_multiplier);
+
 int xpbd_body_save_state(xpbd_context *ctx, int32_t body, void *out, size_t capacity, size_t *size);
 
 /*

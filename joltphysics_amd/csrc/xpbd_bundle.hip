@@ -38,6 +38,8 @@ struct BodyLocal
     uint32_t num_dyn_leaves;    // leaves of dynamic colliders this step
     uint32_t num_dynamic;       // dynamic colliders of the context
     uint32_t report_cleared;    // asleep: its contact report was cleared
+    const float4 *skin_entries; // the body's skin entries (4 float4 per vertex), or null
+    uint32_t ran;               // the body was active at the start of this step
     DynStateDev *dyn;           // the body's row of dynamic-collider states
     uint32_t nf;
     const uint32_t *faces;
@@ -139,6 +141,7 @@ __device__ __forceinline__ void bundle_update(
         L.faces = B.s.faces; L.vf_begin = B.s.vf_begin; L.vf_list = B.s.vf_list;
         L.dyn = B.dyn;
         L.report_cleared = 0;
+        L.skin_entries = B.skin ? B.skin + 4u * (size_t)B.skin_base : nullptr;
         L.num_dynamic = B.dyn ? B.num_dynamic : 0u;
         for (uint32_t k = 0; k < L.num_dynamic; ++k)
             clear_dynamic(L.dyn[k]);
@@ -179,6 +182,7 @@ __device__ __forceinline__ void bundle_update(
             {
                 BodyLocal &L = loc[b];
                 const BodyStateDev &s = st[b];
+                L.ran = s.active ? 1u : 0u;
                 M44 com = rotation_translation(s.rot[0], s.rot[1], s.rot[2], s.rot[3], v3(s.pos[0], s.pos[1], s.pos[2]));
                 V3 g_local = mul3x3_transposed(com, L.gravity_factor * gravity_world);
                 float dt_sub = step_dt / (float)iterations;
@@ -359,17 +363,18 @@ __device__ __forceinline__ void bundle_update(
             {
                 const bool last = it + 1 == iterations;
 
-                // ---- merged level program of all bodies
+                // ---- merged level program of all bodies (skin records carry their entries)
+                const SkinCtx sk{ G.skin, 0u, (float)(it + 1) / (float)iterations };
                 if (has_sleepers)
                 {
                     auto skip = [&](const uint4 &r) { return r.x == kNopRecord || !st[vb[r.x & 0x3fffu]].active; };
-                    if (STAGED) run_levels_lds<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, tid, skip);
-                    else run_levels_global<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, tid, skip);
+                    if (STAGED) run_levels_lds<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, tid, skip, sk);
+                    else run_levels_global<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, tid, skip, sk);
                 }
                 else
                 {
-                    if (STAGED) run_levels_lds<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, tid, NoSkip());
-                    else run_levels_global<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, tid, NoSkip());
+                    if (STAGED) run_levels_lds<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, tid, NoSkip(), sk);
+                    else run_levels_global<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, tid, NoSkip(), sk);
                 }
 
                 // ---- collision + velocity (SoftBodyMotionProperties.cpp:700-825), fused with the
@@ -580,6 +585,19 @@ __device__ __forceinline__ void bundle_update(
                     V3 delta = v3(L.delta[0], L.delta[1], L.delta[2]);
                     float4 p = pget(X, i);
                     pset(X, i, f4(xyz(p) - delta, p.w));
+                }
+                // skin states (SoftBodyMotionProperties.cpp:890-917): the skinned positions
+                // follow the recentring; the previous positions stay valid only until the
+                // next SkinVertices
+                if (L.skin_entries && L.ran)
+                {
+                    float4 *e = const_cast<float4 *>(L.skin_entries) + 4u * (size_t)(i - G.bodies[b].voff);
+                    if (L.update_position)
+                    {
+                        const float4 sp = e[1];
+                        e[1] = f4(xyz(sp) - v3(L.delta[0], L.delta[1], L.delta[2]), sp.w);
+                    }
+                    e[3].w = 0.0f;
                 }
             }
             for (uint32_t b = tid; b < nb; b += BLOCK)

@@ -230,11 +230,68 @@ __device__ __forceinline__ void project_lra(const P &X, uint32_t i0, uint32_t i1
         pset(X, i1, f4(x0 + (delta * max_distance) / sqrtf(delta_len_sq), p1.w));
 }
 
+// ApplySkinConstraints body for one skinned vertex (SoftBodyMotionProperties.cpp:509-571).
+// The vertex's skin state is entry g of `skin`, four float4: previous and current
+// skinned position, normal, and {max distance multiplier, back stop radius, skin
+// constraints enabled, previous position valid}.  `factor` = (iteration + 1) /
+// iterations, the reference's mNextIteration / mNumIterations.
+struct SkinCtx
+{
+    const float4 *skin;
+    uint32_t base;              // added to the record's entry index
+    float factor;
+};
+
+template <class P>
+__device__ __forceinline__ void project_skin(const P &X, uint32_t i, uint32_t g, float max_distance_setting,
+                                             float back_stop_distance, const SkinCtx &sk)
+{
+    const float4 *e = sk.skin + 4u * (size_t)g;
+    const float4 q = e[3];
+    if (q.z == 0.0f)
+        return;                 // mEnableSkinConstraints false
+    const float factor = q.w != 0.0f ? sk.factor : 1.0f;
+    const float prev_factor = 1.0f - factor;
+    const float max_distance = max_distance_setting * q.x;
+    const V3 skin_pos = prev_factor * xyz(e[0]) + factor * xyz(e[1]);
+    const float4 p = pget(X, i);
+    V3 x = xyz(p);
+    if (max_distance > 0.0f)
+    {
+        const float radius = q.y;
+        if (back_stop_distance < max_distance)
+        {
+            const V3 normal = xyz(e[2]);
+            const V3 center = skin_pos - normal * (back_stop_distance + radius);
+            const V3 delta = x - center;
+            const float delta_len_sq = len_sq(delta);
+            if (delta_len_sq < radius * radius)
+            {
+                const float delta_len = sqrtf(delta_len_sq);
+                x = delta_len > 0.0f ? center + delta * (radius / delta_len) : center + normal * radius;
+            }
+        }
+        if (max_distance < FLT_MAX)
+        {
+            const V3 delta = x - skin_pos;
+            const float delta_len_sq = len_sq(delta);
+            const float max_distance_sq = max_distance * max_distance;
+            if (delta_len_sq > max_distance_sq)
+                x = skin_pos + delta * sqrtf(max_distance_sq / delta_len_sq);
+        }
+    }
+    else
+        x = skin_pos;
+    pset(X, i, f4(x, p.w));
+}
+
 // Padding record: skipped (levels pad each constraint type to whole wavefronts)
 constexpr uint32_t kNopRecord = 0xffffffffu;
 
+// A record of LRA type whose two vertices are equal is a skinned constraint: vertex 0,
+// y = its skin entry (before `base`), z = max distance, w = back stop distance.
 template <class P>
-__device__ __forceinline__ void project_record(const P &X, uint4 r, float inv_dt_sq)
+__device__ __forceinline__ void project_record(const P &X, uint4 r, float inv_dt_sq, const SkinCtx &sk)
 {
     if (r.x == kNopRecord)
         return;
@@ -246,7 +303,12 @@ __device__ __forceinline__ void project_record(const P &X, uint4 r, float inv_dt
     case kTypeEdge: project_edge(X, i0, i1, p0, p1, inv_dt_sq); break;
     case kTypeBend: project_bend_pair(X, i0, i1, i2, i3, p0, p1, inv_dt_sq, ((r.y >> 15) & 1u) != 0); break;
     case kTypeVolume: project_volume(X, i0, i1, i2, i3, p0, p1, inv_dt_sq); break;
-    default: project_lra(X, i0, i1, p0); break;
+    default:
+        if (i0 == i1)
+            project_skin(X, i0, r.y + sk.base, p0, p1, sk);
+        else
+            project_lra(X, i0, i1, p0);
+        break;
     }
 }
 
@@ -371,11 +433,11 @@ __device__ __forceinline__ void run_program_global(const uint4 *recs, const uint
 
 template <int BLOCK, class P, class Skip>
 __device__ __forceinline__ void run_levels_global(const uint4 *recs, const uint32_t *lvl, uint32_t nlev, const P &X,
-                                                  float inv_dt_sq, uint32_t tid, Skip skip)
+                                                  float inv_dt_sq, uint32_t tid, Skip skip, const SkinCtx &sk)
 {
     run_program_global<BLOCK, false>(recs, lvl, nlev, tid, [&](const uint4 &q, uint32_t) {
         if (!skip(q))
-            project_record(X, q, inv_dt_sq);
+            project_record(X, q, inv_dt_sq, sk);
     });
 }
 
@@ -383,7 +445,7 @@ __device__ __forceinline__ void run_levels_global(const uint4 *recs, const uint3
 // level is read before the barrier (the barrier's LDS wait covers it).
 template <int BLOCK, class P, class Skip>
 __device__ __forceinline__ void run_levels_lds(const uint4 *recs, const uint32_t *lvl, uint32_t nlev, const P &X,
-                                               float inv_dt_sq, uint32_t tid, Skip skip)
+                                               float inv_dt_sq, uint32_t tid, Skip skip, const SkinCtx &sk)
 {
     if (nlev == 0)
         return;
@@ -394,12 +456,12 @@ __device__ __forceinline__ void run_levels_lds(const uint4 *recs, const uint32_t
     {
         const uint32_t next_hi = lvl[L + 2 <= nlev ? L + 2 : nlev];
         if (lo + tid < hi && !skip(r))
-            project_record(X, r, inv_dt_sq);
+            project_record(X, r, inv_dt_sq, sk);
         for (uint32_t i = lo + tid + BLOCK; i < hi; i += BLOCK)
         {
             const uint4 q = recs[i];
             if (!skip(q))
-                project_record(X, q, inv_dt_sq);
+                project_record(X, q, inv_dt_sq, sk);
         }
         const uint4 rn = hi + tid < next_hi ? recs[hi + tid] : nop;
         __syncthreads();

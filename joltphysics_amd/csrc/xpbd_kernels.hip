@@ -69,12 +69,13 @@ __host__ __device__ inline LdsLayout lds_layout(uint32_t xcount, uint32_t nrec, 
 // registers (issued two levels ahead), so the load latency overlaps two levels
 // of projection.
 template <bool STAGED, int BLOCK>
-__device__ __forceinline__ void run_levels(const uint4 *recs, const uint32_t *lvl, uint32_t nlev, float4 *X, float inv_dt_sq, int tid)
+__device__ __forceinline__ void run_levels(const uint4 *recs, const uint32_t *lvl, uint32_t nlev, float4 *X, float inv_dt_sq, int tid,
+                                           const SkinCtx &sk)
 {
     if (STAGED)
-        run_levels_lds<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, (uint32_t)tid, NoSkip());
+        run_levels_lds<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, (uint32_t)tid, NoSkip(), sk);
     else
-        run_levels_global<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, (uint32_t)tid, NoSkip());
+        run_levels_global<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, (uint32_t)tid, NoSkip(), sk);
 }
 
 // ---------------------------------------------------------------- kernel
@@ -345,7 +346,8 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                             X[j] = B.x[S.vlist[W.vbegin + j]];
                         __syncthreads();
                     }
-                    run_levels<STAGED, kBlock>(recs, lvl + W.lbegin, W.nlev, X, inv_dt_sq, tid);
+                    run_levels<STAGED, kBlock>(recs, lvl + W.lbegin, W.nlev, X, inv_dt_sq, tid,
+                                               SkinCtx{ B.skin, B.skin_base, (float)(it + 1) / (float)iterations });
                     if (!RESIDENT && W.vcount)
                     {
                         for (uint32_t j = tid; j < W.vcount; j += kBlock)
@@ -546,6 +548,22 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                     {
                         float4 p = P[i];
                         P[i] = f4(xyz(p) - delta, p.w);
+                    }
+                }
+                if (B.skin)
+                {
+                    // skin states (SoftBodyMotionProperties.cpp:890-917): the skinned positions
+                    // follow the recentring; previous positions are invalid until the next SkinVertices
+                    float4 *e = const_cast<float4 *>(B.skin) + 4u * (size_t)B.skin_base;
+                    const V3 delta = v3(sh_scalars[1], sh_scalars[2], sh_scalars[3]);
+                    for (uint32_t i = tid; i < nv; i += kBlock)
+                    {
+                        if (B.update_position)
+                        {
+                            const float4 sp = e[4u * i + 1u];
+                            e[4u * i + 1u] = f4(xyz(sp) - delta, sp.w);
+                        }
+                        e[4u * i + 3u].w = 0.0f;
                     }
                 }
                 __syncthreads();

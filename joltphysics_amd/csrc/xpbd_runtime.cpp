@@ -10,6 +10,7 @@
 #include <memory>
 #include <string>
 #include <vector>
+#include <limits>
 
 #include "../../include/jolt_xpbd.h"
 #include "jolt_math.h"
@@ -68,6 +69,10 @@ struct BendRec { uint32_t v[4]; float compliance, initial_angle; };
 struct VolumeRec { uint32_t v[4]; float six_rest_volume, compliance; };
 struct LRARec { uint32_t v[2]; float max_distance; };
 struct GroupRec { uint32_t edge_end, lra_end, rod_ss_end, rod_bt_end, bend_end, volume_end, skinned_end; };
+struct SkinWeightRec { uint32_t inv_bind; float weight; };
+struct SkinnedRec { uint32_t vertex; SkinWeightRec w[4]; float max_distance, back_stop_distance, back_stop_radius; uint32_t normal_info; };
+struct InvBindRec { uint32_t joint; uint32_t pad[3]; float m[16]; };
+static_assert(sizeof(SkinnedRec) == 52 && sizeof(InvBindRec) == 80, "skin record size");
 static_assert(sizeof(VertexRec) == 28 && sizeof(FaceRec) == 16 && sizeof(EdgeRec) == 16, "record size");
 static_assert(sizeof(BendRec) == 24 && sizeof(VolumeRec) == 24 && sizeof(LRARec) == 12 && sizeof(GroupRec) == 28, "record size");
 
@@ -115,6 +120,11 @@ struct xpbd_settings
     std::vector<uint4> host_recs;
     std::vector<uint32_t> host_level_begin;     // nlev + 1 offsets into host_recs
     std::vector<uint32_t> host_rec_index;       // constraint index of each host record (no-op: ~0u)
+    // skinning (host copies, for SkinVertices)
+    std::vector<SkinnedRec> skinned;
+    std::vector<uint32_t> skinned_normals;
+    std::vector<InvBindRec> inv_binds;
+    std::vector<FaceRec> face_recs;
     bool uniform_mass = false;                  // every vertex has the same inverse mass
     float inv_mass = 0.0f;                      // that inverse mass
     uint32_t num_edges = 0, num_bends = 0, num_volumes = 0, num_lras = 0;
@@ -129,6 +139,7 @@ struct Body
     DevBuf<int32_t> idx, contact;
     BodyDev dev{};
     BodyStateDev state{};
+    bool skinned = false;           // has skin entries at dev.skin_base (one per vertex)
 };
 
 struct BundleHost
@@ -163,6 +174,13 @@ struct xpbd_context
     DevBuf<float4> d_hull_vec;      // convex hulls: planes, then points, per hull
     DevBuf<uint32_t> d_hull_idx;    // convex hulls: face first, face count, vertex index, per hull
     DevBuf<HullDev> d_hulls;
+    // skinned vertex states of all skinned bodies, 4 float4 per vertex: previous and current
+    // skinned position, normal, {max distance multiplier, back stop radius, enabled, previous
+    // position valid}; kept on the host, uploaded before an update when changed
+    std::vector<float4> skin_host;
+    DevBuf<float4> d_skin;
+    bool skin_dirty = false;        // host copy newer: upload before the next update
+    bool skin_on_device = false;    // device copy newer (updates move and invalidate skin states)
     uint32_t num_dynamic = 0;       // dynamic colliders of the last xpbd_set_colliders
     DevBuf<DynStateDev> d_dyn;      // bodies x num_dynamic
     uint32_t dyn_stride = 0;        // num_dynamic the rows of d_dyn are laid out for
@@ -223,6 +241,8 @@ namespace {
 // order, and the constraints of one level have disjoint write sets that no
 // other constraint of the level reads.
 struct Item { uint32_t type; uint32_t v[4]; uint32_t nv; float p0, p1; uint32_t index; };   // index: in its type's array
+// Skinned constraint item (host only): packed as an LRA record of vertex 0 to itself
+constexpr uint32_t kItemSkin = 4;
 
 // Bodies up to this many vertices keep all positions in LDS for the launch.
 constexpr uint32_t kResidentMaxVertices = 8192;
@@ -269,6 +289,16 @@ uint4 Pack(const Item &it, const uint32_t *local)
 {
     auto m = [&](uint32_t j) { return j < it.nv ? (local ? local[it.v[j]] : it.v[j]) : 0u; };
     uint4 r;
+    if (it.type == kItemSkin)
+    {
+        // LRA type, vertex 0 twice; y = the body vertex (the skin entry, before the body's base)
+        r.x = (m(0) & 0x3fffu) | (kTypeLRA << 14) | (m(0) << 16);
+        r.y = it.v[0];
+        float p0 = it.p0, p1 = it.p1;
+        memcpy(&r.z, &p0, 4);
+        memcpy(&r.w, &p1, 4);
+        return r;
+    }
     r.x = (m(0) & 0x3fffu) | (it.type << 14) | (m(1) << 16);
     r.y = m(2) | (m(3) << 16);
     float p0 = it.p0, p1 = it.p1;
@@ -377,8 +407,9 @@ void ProgramBuilder::AppendLevel(const std::vector<std::pair<const GroupLevels *
     for (const auto &gi : lv)
     {
         const Item &it = gi.first->seq[gi.second];
-        bucket[it.type].push_back(Pack(it, local));
-        ibucket[it.type].push_back(it.index);
+        const uint32_t t = it.type == kItemSkin ? kTypeLRA : it.type;
+        bucket[t].push_back(Pack(it, local));
+        ibucket[t].push_back(it.index);
     }
     AppendTypedLevel(bucket, recs, true, ibucket, &rec_index);
 }
@@ -388,8 +419,11 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
     const uint32_t nv = d.num_vertices;
     if (nv == 0 || d.vertices == nullptr)
         return Fail(XPBD_E_INVALID, "settings have no vertices");
-    if (d.num_skinned || d.num_rod_stretch_shear || d.num_rod_bend_twist)
-        return Fail(XPBD_E_UNSUPPORTED, "skinned constraints and Cosserat rods are not supported by this build");
+    if (d.num_rod_stretch_shear || d.num_rod_bend_twist)
+        return Fail(XPBD_E_UNSUPPORTED, "Cosserat rods are not supported by this build");
+    if (d.num_skinned && (!d.skinned || (d.num_inv_bind_matrices && !d.inv_bind_matrices)
+                          || (d.num_skinned_normals && !d.skinned_normals)))
+        return Fail(XPBD_E_INVALID, "skinned constraints without their arrays");
     if (d.num_update_groups == 0 || d.update_groups == nullptr)
         return Fail(XPBD_E_INVALID, "settings have no update groups (SoftBodySharedSettings::Optimize was not run)");
 
@@ -398,17 +432,43 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
     const VolumeRec *vols = (const VolumeRec *)d.volumes;
     const LRARec *lras = (const LRARec *)d.lras;
     const GroupRec *groups = (const GroupRec *)d.update_groups;
+    const SkinnedRec *skinned = (const SkinnedRec *)d.skinned;
+    if (d.num_skinned)
+    {
+        std::vector<char> seen(nv, 0);
+        for (uint32_t k = 0; k < d.num_skinned; ++k)
+        {
+            const SkinnedRec &sr = skinned[k];
+            if (sr.vertex >= nv || seen[sr.vertex])
+                return Fail(XPBD_E_INVALID, "skinned constraint vertex out of range or skinned twice");
+            seen[sr.vertex] = 1;
+            for (int j = 0; j < 4 && sr.w[j].weight != 0.0f; ++j)
+                if (sr.w[j].inv_bind >= d.num_inv_bind_matrices)
+                    return Fail(XPBD_E_INVALID, "skin weight refers to a missing inverse bind matrix");
+            const uint32_t nf = sr.normal_info >> 24, first = sr.normal_info & 0xffffffu;
+            if (nf && ((uint64_t)first + nf > d.num_skinned_normals))
+                return Fail(XPBD_E_INVALID, "skinned constraint normal faces out of range");
+        }
+        for (uint32_t k = 0; k < d.num_skinned_normals; ++k)
+            if (d.skinned_normals[k] >= d.num_faces)
+                return Fail(XPBD_E_INVALID, "skinned constraint normal face out of range");
+        s.skinned.assign(skinned, skinned + d.num_skinned);
+        s.skinned_normals.assign(d.skinned_normals, d.skinned_normals + d.num_skinned_normals);
+        s.inv_binds.assign((const InvBindRec *)d.inv_bind_matrices, (const InvBindRec *)d.inv_bind_matrices + d.num_inv_bind_matrices);
+        s.face_recs.assign((const FaceRec *)d.faces, (const FaceRec *)d.faces + d.num_faces);
+    }
 
     GroupRec prev{ 0, 0, 0, 0, 0, 0, 0 };
     for (uint32_t g = 0; g < d.num_update_groups; ++g)
     {
         const GroupRec &c = groups[g];
         if (c.edge_end < prev.edge_end || c.lra_end < prev.lra_end || c.bend_end < prev.bend_end || c.volume_end < prev.volume_end
-            || c.rod_ss_end != 0 || c.rod_bt_end != 0 || c.skinned_end != 0)
+            || c.skinned_end < prev.skinned_end || c.rod_ss_end != 0 || c.rod_bt_end != 0)
             return Fail(XPBD_E_INVALID, "update group end indices are not monotone or reference unsupported constraint types");
         prev = c;
     }
-    if (prev.edge_end != d.num_edges || prev.lra_end != d.num_lras || prev.bend_end != d.num_dihedral_bends || prev.volume_end != d.num_volumes)
+    if (prev.edge_end != d.num_edges || prev.lra_end != d.num_lras || prev.bend_end != d.num_dihedral_bends || prev.volume_end != d.num_volumes
+        || prev.skinned_end != d.num_skinned)
         return Fail(XPBD_E_INVALID, "the last update group does not end at the constraint counts");
 
     auto check = [nv](const uint32_t *v, int n) {
@@ -437,6 +497,12 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
             if (!check(bends[k].v, 4)) return Fail(XPBD_E_INVALID, "bend constraint vertex out of range");
             seq.push_back({ kTypeBend, { bends[k].v[0], bends[k].v[1], bends[k].v[2], bends[k].v[3] }, 4, bends[k].compliance, bends[k].initial_angle, k });
         }
+        // ProcessGroup order: volumes, bends, skinned, edges, LRAs (SoftBodyMotionProperties.cpp:1037-1054)
+        for (uint32_t k = prev.skinned_end; k < c.skinned_end; ++k)
+        {
+            const SkinnedRec &sr = skinned[k];
+            seq.push_back({ kItemSkin, { sr.vertex, 0, 0, 0 }, 1, sr.max_distance, sr.back_stop_distance, k });
+        }
         for (uint32_t k = prev.edge_end; k < c.edge_end; ++k)
         {
             if (!check(edges[k].v, 2)) return Fail(XPBD_E_INVALID, "edge constraint vertex out of range");
@@ -445,6 +511,8 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
         for (uint32_t k = prev.lra_end; k < c.lra_end; ++k)
         {
             if (!check(lras[k].v, 2)) return Fail(XPBD_E_INVALID, "LRA constraint vertex out of range");
+            if (lras[k].v[0] == lras[k].v[1])
+                continue;       // an LRA of a vertex to itself never moves it (that encoding marks skin records)
             seq.push_back({ kTypeLRA, { lras[k].v[0], lras[k].v[1], 0, 0 }, 2, lras[k].max_distance, 0.0f, k });
         }
         gl[g].levels = LevelGroup(seq, last_write, last_read);
@@ -707,9 +775,15 @@ double ProgramPasses(const std::vector<uint32_t> &lvl, uint32_t block)
     return p;
 }
 
-uint4 OffsetRecord(uint4 r, uint32_t voff)
+uint4 OffsetRecord(uint4 r, uint32_t voff, uint32_t skin_base)
 {
     uint32_t type = (r.x >> 14) & 3u;
+    if (type == kTypeLRA && (r.x & 0x3fffu) == (r.x >> 16))
+    {
+        // skin record: vertex 0 twice, y = the body vertex's skin entry
+        const uint32_t v0 = (r.x & 0x3fffu) + voff;
+        return make_uint4((v0 & 0x3fffu) | (type << 14) | (v0 << 16), r.y + skin_base, r.z, r.w);
+    }
     uint32_t v0 = (r.x & 0x3fffu) + voff, v1 = (r.x >> 16) + voff;
     uint32_t v2 = (r.y & 0x3fffu) + voff, v3 = (r.y >> 16) + voff;
     uint4 o;
@@ -743,7 +817,7 @@ void BuildBundleProgram(const xpbd_context *ctx, const BundleHost &bh, const std
                     continue;
                 if (((st.host_recs[r].x >> 14) & 3u) == kTypeBend && (st.host_recs[r].y & kSecondHalf))
                     continue;       // re-emitted with its first half
-                uint4 rec = OffsetRecord(st.host_recs[r], voffs[j]);
+                uint4 rec = OffsetRecord(st.host_recs[r], voffs[j], ctx->bodies[bh.bodies[j]]->dev.skin_base);
                 bucket[(rec.x >> 14) & 3u].push_back(rec);
             }
         }
@@ -979,6 +1053,7 @@ int BuildLayout(xpbd_context *ctx)
         d.uniform = bh.uniform ? 1u : 0u;
         d.uniform_inv_mass = bh.uniform ? bh.inv_mass : 0.0f;
         d.elapsed = ctx->d_elapsed.p + k;
+        d.skin = ctx->d_skin.p;
         // records staged in LDS when the bundle still fits the smallest class with them
         bh.staged = block == 256u && bundle_lds_bytes(voff, d.num_records, d.nlev + 1, nb, bh.uniform) <= kBundleClasses[0].lds;
         d.staged = bh.staged ? 1u : 0u;
@@ -1229,6 +1304,23 @@ int xpbd_body_create(xpbd_context *ctx, const xpbd_body_desc *d, int32_t *out_bo
     dv.s = s->dev;
     dv.x = b->x.p; dv.v = b->v.p; dv.prev = b->prev.p; dv.plane = b->plane.p; dv.idx = b->idx.p; dv.contact = b->contact.p;
     dv.report = b->report.p;
+    if (!s->skinned.empty())
+    {
+        // SoftBodyMotionProperties::Initialize: skin states zero / NaN until SkinVertices
+        const float nan = std::numeric_limits<float>::quiet_NaN();
+        std::vector<float> radius(nv, 0.0f);
+        for (const SkinnedRec &sr : s->skinned) radius[sr.vertex] = sr.back_stop_radius;
+        b->skinned = true;
+        dv.skin_base = (uint32_t)(ctx->skin_host.size() / 4);
+        for (uint32_t i = 0; i < nv; ++i)
+        {
+            ctx->skin_host.push_back(make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+            ctx->skin_host.push_back(make_float4(nan, nan, nan, 0.0f));
+            ctx->skin_host.push_back(make_float4(nan, nan, nan, 0.0f));
+            ctx->skin_host.push_back(make_float4(1.0f, radius[i], 1.0f, 0.0f));
+        }
+        ctx->skin_dirty = true;
+    }
     dv.iterations = d->num_iterations;
     dv.linear_damping = d->linear_damping;
     dv.max_linear_velocity = d->max_linear_velocity;
@@ -1381,6 +1473,24 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
     HIP_TRY(hipSetDevice(ctx->device));
     const uint32_t nc = (uint32_t)ctx->colliders.size();
     int rc;
+    if (ctx->skin_dirty)
+    {
+        // skin states go up before the layout, whose bundles point at them
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (ctx->d_skin.n != ctx->skin_host.size())
+        {
+            HIP_TRY(ctx->d_skin.alloc(ctx->skin_host.size()));
+            for (auto &b : ctx->bodies)
+                if (b->skinned)
+                    b->dev.skin = ctx->d_skin.p;
+            ctx->tables_dirty = true;
+            ctx->layout_dirty = true;
+        }
+        HIP_TRY(hipMemcpy(ctx->d_skin.p, ctx->skin_host.data(), sizeof(float4) * ctx->skin_host.size(), hipMemcpyHostToDevice));
+        ctx->skin_dirty = false;
+    }
+    if (!ctx->skin_host.empty())
+        ctx->skin_on_device = true;
     if (ctx->layout_dirty || ctx->layout_colliders != nc)
     {
         HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1655,11 +1765,138 @@ constexpr size_t kStateHeaderBytes = 4 * 12 + 48 + 4 + 1;   // velocities, force
 size_t StateBytes(uint32_t nv) { return kStateHeaderBytes + (size_t)nv * 24 + 4 * 12; }
 } // namespace
 
+namespace {
+// Brings the host copy of the skin states up to date after updates moved them.
+int PullSkin(xpbd_context *ctx)
+{
+    if (!ctx->skin_on_device || ctx->d_skin.n != ctx->skin_host.size())
+        return XPBD_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(ctx->skin_host.data(), ctx->d_skin.p, sizeof(float4) * ctx->skin_host.size(), hipMemcpyDeviceToHost));
+    ctx->skin_on_device = false;
+    return XPBD_OK;
+}
+} // namespace
+
+int xpbd_body_skin_vertices(xpbd_context *ctx, int32_t body, const float *joint_matrices, uint32_t num_joints,
+                            int32_t hard_skin_all)
+{
+    if (!ctx || body < 0 || body >= (int32_t)ctx->bodies.size() || (num_joints && !joint_matrices))
+        return Fail(XPBD_E_INVALID, "invalid body or joint matrices");
+    Body &b = *ctx->bodies[body];
+    const xpbd_settings &st = *b.settings;
+    if (!b.skinned)
+        return Fail(XPBD_E_INVALID, "body has no skinned constraints");
+    int rc = PullSkin(ctx);
+    if (rc != XPBD_OK)
+        return rc;
+    // skin matrices: joint matrix * inverse bind matrix (Mat44 * Mat44, Mat44.inl:241-252)
+    std::vector<M44> skin_m(st.inv_binds.size());
+    for (size_t k = 0; k < st.inv_binds.size(); ++k)
+    {
+        const InvBindRec &ib = st.inv_binds[k];
+        if (ib.joint >= num_joints)
+            return Fail(XPBD_E_INVALID, "inverse bind matrix refers to a missing joint");
+        M44 j, m;
+        memcpy(j.c, joint_matrices + 16 * (size_t)ib.joint, sizeof(j.c));
+        memcpy(m.c, ib.m, sizeof(m.c));
+        skin_m[k] = mul(j, m);
+    }
+    float4 *E = ctx->skin_host.data() + 4 * (size_t)b.dev.skin_base;
+    for (const SkinnedRec &sr : st.skinned)
+    {
+        const float *bp = st.vertices[sr.vertex].pos;
+        const V3 bind = v3(bp[0], bp[1], bp[2]);
+        V3 pos = v3(0.0f, 0.0f, 0.0f);
+        for (int k = 0; k < 4; ++k)
+        {
+            if (sr.w[k].weight == 0.0f)
+                break;
+            pos = pos + sr.w[k].weight * mul_point(skin_m[sr.w[k].inv_bind], bind);
+        }
+        float4 *e = E + 4 * (size_t)sr.vertex;
+        e[0] = make_float4(e[1].x, e[1].y, e[1].z, 0.0f);
+        e[1] = make_float4(pos.x, pos.y, pos.z, 0.0f);
+    }
+    for (const SkinnedRec &sr : st.skinned)
+    {
+        V3 normal = v3(0.0f, 0.0f, 0.0f);
+        const uint32_t nf = sr.normal_info >> 24;
+        if (nf > 0)
+        {
+            const uint32_t *f = st.skinned_normals.data() + (sr.normal_info & 0xffffffu);
+            for (uint32_t k = 0; k < nf; ++k)
+            {
+                const FaceRec &face = st.face_recs[f[k]];
+                const V3 v0 = v3(E[4 * (size_t)face.v[0] + 1].x, E[4 * (size_t)face.v[0] + 1].y, E[4 * (size_t)face.v[0] + 1].z);
+                const V3 v1 = v3(E[4 * (size_t)face.v[1] + 1].x, E[4 * (size_t)face.v[1] + 1].y, E[4 * (size_t)face.v[1] + 1].z);
+                const V3 v2 = v3(E[4 * (size_t)face.v[2] + 1].x, E[4 * (size_t)face.v[2] + 1].y, E[4 * (size_t)face.v[2] + 1].z);
+                const V3 c = cross(v1 - v0, v2 - v0);
+                const float l2 = len_sq(c);
+                normal = normal + (l2 <= FLT_MIN ? v3(0.0f, 0.0f, 0.0f) : c / sqrtf(l2));   // NormalizedOr(zero)
+            }
+            const float l2 = len_sq(normal);
+            normal = l2 <= FLT_MIN ? v3(0.0f, 0.0f, 0.0f) : normal / sqrtf(l2);
+        }
+        E[4 * (size_t)sr.vertex + 2] = make_float4(normal.x, normal.y, normal.z, 0.0f);
+    }
+    const bool enabled = st.skinned.empty() || E[4 * (size_t)st.skinned[0].vertex + 3].z != 0.0f;
+    if (hard_skin_all || !enabled)
+    {
+        HIP_TRY(hipSetDevice(ctx->device));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        std::vector<float4> x(b.nv), v(b.nv);
+        HIP_TRY(hipMemcpy(x.data(), b.dev.x, sizeof(float4) * b.nv, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(v.data(), b.dev.v, sizeof(float4) * b.nv, hipMemcpyDeviceToHost));
+        for (const SkinnedRec &sr : st.skinned)
+        {
+            float4 *e = E + 4 * (size_t)sr.vertex;
+            if (hard_skin_all)
+            {
+                e[0] = make_float4(e[1].x, e[1].y, e[1].z, 0.0f);
+                x[sr.vertex] = make_float4(e[1].x, e[1].y, e[1].z, x[sr.vertex].w);
+                v[sr.vertex] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            }
+            else if (sr.max_distance == 0.0f)
+                x[sr.vertex] = make_float4(e[1].x, e[1].y, e[1].z, x[sr.vertex].w);
+        }
+        HIP_TRY(hipMemcpy(b.dev.x, x.data(), sizeof(float4) * b.nv, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(b.dev.v, v.data(), sizeof(float4) * b.nv, hipMemcpyHostToDevice));
+    }
+    for (uint32_t i = 0; i < b.nv; ++i)
+        E[4 * (size_t)i + 3].w = 1.0f;      // mSkinStatePreviousPositionValid
+    ctx->skin_dirty = true;
+    return XPBD_OK;
+}
+
+int xpbd_body_set_skin_options(xpbd_context *ctx, int32_t body, int32_t enable_skin_constraints, float max_distance_multiplier)
+{
+    if (!ctx || body < 0 || body >= (int32_t)ctx->bodies.size())
+        return Fail(XPBD_E_INVALID, "invalid body");
+    Body &b = *ctx->bodies[body];
+    if (!b.skinned)
+        return Fail(XPBD_E_INVALID, "body has no skinned constraints");
+    int rc = PullSkin(ctx);
+    if (rc != XPBD_OK)
+        return rc;
+    float4 *E = ctx->skin_host.data() + 4 * (size_t)b.dev.skin_base;
+    for (uint32_t i = 0; i < b.nv; ++i)
+    {
+        E[4 * (size_t)i + 3].x = max_distance_multiplier;
+        E[4 * (size_t)i + 3].z = enable_skin_constraints ? 1.0f : 0.0f;
+    }
+    ctx->skin_dirty = true;
+    return XPBD_OK;
+}
+
 int xpbd_body_save_state(xpbd_context *ctx, int32_t body, void *out, size_t capacity, size_t *size)
 {
     if (!ctx || body < 0 || body >= (int32_t)ctx->bodies.size() || !size)
         return Fail(XPBD_E_INVALID, "invalid body or size pointer");
     const Body &b = *ctx->bodies[body];
+    if (b.skinned)
+        return Fail(XPBD_E_UNSUPPORTED, "state streams of skinned bodies (with skin states) are not supported");
     *size = StateBytes(b.nv);
     if (!out)
         return XPBD_OK;
@@ -1701,6 +1938,8 @@ int xpbd_body_restore_state(xpbd_context *ctx, int32_t body, const void *in, siz
     if (!ctx || body < 0 || body >= (int32_t)ctx->bodies.size() || !in)
         return Fail(XPBD_E_INVALID, "invalid body or stream");
     Body &b = *ctx->bodies[body];
+    if (b.skinned)
+        return Fail(XPBD_E_UNSUPPORTED, "state streams of skinned bodies (with skin states) are not supported");
     if (size != StateBytes(b.nv))
         return Fail(XPBD_E_INVALID, "state stream size does not match the body");
     int rc = xpbd_synchronize(ctx);

@@ -65,6 +65,8 @@ struct BodyDev
     uint32_t stage_records;     // 1: records + level table are copied into LDS once per launch
     DynStateDev *dyn;           // this body's row of dynamic-collider states (num_dynamic entries), or null
     uint32_t num_dynamic;
+    const float4 *skin;         // skinned vertex states of the context (4 per entry), or null
+    uint32_t skin_base;         // first entry of this body (entry = skin_base + body vertex index)
 };
 
 // Per-body dynamic state kept on the device between launches.
@@ -160,6 +162,7 @@ struct BundleDev
     uint32_t uniform;           // 12 B LDS positions: every vertex has the inverse mass uniform_inv_mass
     float uniform_inv_mass;
     unsigned long long *elapsed;    // where the workgroup stores its duration (realtime-clock ticks), or null
+    const float4 *skin;         // skinned vertex states of the context (the program's skin records index it)
 };
 
 } // namespace xpbd

@@ -32,6 +32,13 @@ GROUP = np.dtype([("edge_end", "<u4"), ("lra_end", "<u4"), ("rod_ss_end", "<u4")
                   ("rod_bt_end", "<u4"), ("bend_end", "<u4"), ("volume_end", "<u4"),
                   ("skinned_end", "<u4")])
 
+# Skinned (SoftBodySharedSettings.h:252-285) and InvBind (:225-236, Mat44 16-byte aligned)
+SKINNED = np.dtype([("vertex", "<u4"), ("weights", [("inv_bind", "<u4"), ("weight", "<f4")], 4),
+                    ("max_distance", "<f4"), ("back_stop_distance", "<f4"), ("back_stop_radius", "<f4"),
+                    ("normal_info", "<u4")])
+INV_BIND = np.dtype([("joint", "<u4"), ("pad", "<u4", 3), ("m", "<f4", 16)])     # m: column major
+assert SKINNED.itemsize == 52 and INV_BIND.itemsize == 80
+
 assert VERTEX.itemsize == 28 and FACE.itemsize == 16 and EDGE.itemsize == 16
 assert BEND.itemsize == 24 and VOLUME.itemsize == 24 and LRA.itemsize == 12
 assert GROUP.itemsize == 28
@@ -62,6 +69,9 @@ class SettingsArrays:
     volumes: np.ndarray = field(default_factory=lambda: _empty(VOLUME))
     lras: np.ndarray = field(default_factory=lambda: _empty(LRA))
     groups: np.ndarray = field(default_factory=lambda: _empty(GROUP))
+    skinned: np.ndarray = field(default_factory=lambda: _empty(SKINNED))
+    skinned_normals: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
+    inv_binds: np.ndarray = field(default_factory=lambda: _empty(INV_BIND))
 
     def counts(self):
         return dict(V=len(self.vertices), F=len(self.faces), E=len(self.edges), B=len(self.bends),
@@ -72,7 +82,8 @@ class SettingsArrays:
 
     def copy(self):
         return SettingsArrays(*(np.array(a, copy=True) for a in (
-            self.vertices, self.faces, self.edges, self.bends, self.volumes, self.lras, self.groups)))
+            self.vertices, self.faces, self.edges, self.bends, self.volumes, self.lras, self.groups,
+            self.skinned, self.skinned_normals, self.inv_binds)))
 
 
 @dataclass

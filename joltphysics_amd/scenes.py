@@ -42,6 +42,8 @@ class MeshSpec:
     angle_tolerance: float = float(np.float32(np.deg2rad(8.0)))
     lra_multiplier: float = 1.0
     flags: int = CREATE_CONSTRAINTS | OPTIMIZE
+    skinned: np.ndarray | None = None           # SKINNED records (normals are calculated)
+    inv_binds: np.ndarray | None = None         # INV_BIND records
 
 
 @dataclass

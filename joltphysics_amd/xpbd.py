@@ -17,8 +17,8 @@ import sys
 
 import numpy as np
 
-from .records import (BEND, EDGE, FACE, GROUP, LRA, MOTION_DYNAMIC, SHAPE_CONVEX_HULL, VERTEX, VOLUME, BodyParams,
-                      SettingsArrays)
+from .records import (BEND, EDGE, FACE, GROUP, INV_BIND, LRA, MOTION_DYNAMIC, SHAPE_CONVEX_HULL, SKINNED, VERTEX,
+                      VOLUME, BodyParams, SettingsArrays)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libjolt_xpbd.so")
@@ -39,7 +39,9 @@ class _SettingsDesc(C.Structure):
                 ("lras", C.c_void_p), ("num_lras", C.c_uint32),
                 ("update_groups", C.c_void_p), ("num_update_groups", C.c_uint32),
                 ("num_skinned", C.c_uint32), ("num_rod_stretch_shear", C.c_uint32),
-                ("num_rod_bend_twist", C.c_uint32)]
+                ("num_rod_bend_twist", C.c_uint32), ("skinned", C.c_void_p),
+                ("skinned_normals", C.c_void_p), ("num_skinned_normals", C.c_uint32),
+                ("inv_bind_matrices", C.c_void_p), ("num_inv_bind_matrices", C.c_uint32)]
 
 
 class _BodyDesc(C.Structure):
@@ -95,8 +97,9 @@ EXPORTED_SYMBOLS = (
     "xpbd_settings_build", "xpbd_built_counts", "xpbd_built_read", "xpbd_built_free",
     "xpbd_build_last_error", "xpbd_body_create",
     "xpbd_set_colliders", "xpbd_set_gravity", "xpbd_update", "xpbd_synchronize", "xpbd_body_read",
-    "xpbd_body_write", "xpbd_body_state", "xpbd_body_velocity_deltas", "xpbd_body_contacts", "xpbd_body_save_state",
-    "xpbd_body_restore_state", "xpbd_profile_enable", "xpbd_profile_read",
+    "xpbd_body_write", "xpbd_body_state", "xpbd_body_velocity_deltas", "xpbd_body_contacts", "xpbd_body_skin_vertices",
+    "xpbd_body_set_skin_options", "xpbd_body_save_state", "xpbd_body_restore_state", "xpbd_profile_enable",
+    "xpbd_profile_read",
     "xpbd_rollout_create", "xpbd_rollout_destroy", "xpbd_rollout_sizes", "xpbd_rollout_forward",
     "xpbd_rollout_forward_f64", "xpbd_rollout_backward",
 )
@@ -145,6 +148,8 @@ def load_library(path=LIB_PATH):
     lib.xpbd_body_state.argtypes = [vp, i32, vp]
     lib.xpbd_body_velocity_deltas.argtypes = [vp, i32, u32, vp, vp, vp]
     lib.xpbd_body_contacts.argtypes = [vp, i32, vp, vp]
+    lib.xpbd_body_skin_vertices.argtypes = [vp, i32, vp, u32, i32]
+    lib.xpbd_body_set_skin_options.argtypes = [vp, i32, i32, f]
     lib.xpbd_body_save_state.argtypes = [vp, i32, vp, C.c_size_t, C.POINTER(C.c_size_t)]
     lib.xpbd_body_restore_state.argtypes = [vp, i32, vp, C.c_size_t, vp, vp]
     lib.xpbd_profile_enable.argtypes = [vp, i32]
@@ -297,6 +302,14 @@ class Context:
         _check(self.lib.xpbd_set_colliders(self.h, arr, n))
         self._num_colliders = n
 
+    def skin_vertices(self, body, joints, hard_skin_all=False):
+        """SoftBodyMotionProperties::SkinVertices with joint matrices (n, 16), column major."""
+        j = np.ascontiguousarray(joints, np.float32).reshape(-1, 16)
+        _check(self.lib.xpbd_body_skin_vertices(self.h, int(body), j.ctypes.data, len(j), int(bool(hard_skin_all))))
+
+    def set_skin_options(self, body, enable=True, max_distance_multiplier=1.0):
+        _check(self.lib.xpbd_body_set_skin_options(self.h, int(body), int(bool(enable)), float(max_distance_multiplier)))
+
     def save_state(self, body) -> bytes:
         """SoftBodyMotionProperties::SaveState's byte stream of `body` (see the C ABI)."""
         n = C.c_size_t()
@@ -346,8 +359,12 @@ class Context:
         t = np.ascontiguousarray(s.volumes, VOLUME)
         lr = np.ascontiguousarray(s.lras, LRA)
         g = np.ascontiguousarray(s.groups, GROUP)
+        sk = np.ascontiguousarray(s.skinned, SKINNED)
+        skn = np.ascontiguousarray(s.skinned_normals, np.uint32)
+        ib = np.ascontiguousarray(s.inv_binds, INV_BIND)
         d = _SettingsDesc(_ptr(v), len(v), _ptr(f), len(f), _ptr(e), len(e), _ptr(b), len(b),
-                          _ptr(t), len(t), _ptr(lr), len(lr), _ptr(g), len(g), 0, 0, 0)
+                          _ptr(t), len(t), _ptr(lr), len(lr), _ptr(g), len(g), len(sk), 0, 0,
+                          _ptr(sk), _ptr(skn), len(skn), _ptr(ib), len(ib))
         h = C.c_void_p()
         _check(self.lib.xpbd_settings_create(self.h, C.byref(d), C.byref(h)))
         st = Settings(self, h, s)

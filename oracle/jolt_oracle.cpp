@@ -203,6 +203,8 @@ struct Arrays
 	std::vector<SoftBodySharedSettings::Volume> t;
 	std::vector<SoftBodySharedSettings::LRA> l;
 	std::vector<UpdateGroupRec> g;
+	std::vector<SoftBodySharedSettings::Skinned> sk;
+	std::vector<uint32> skn;
 };
 
 Arrays ReadBack(const SoftBodySharedSettings &s)
@@ -217,8 +219,8 @@ Arrays ReadBack(const SoftBodySharedSettings &s)
 	a.e = ReadArray<SoftBodySharedSettings::Edge>(is);
 	a.d = ReadArray<SoftBodySharedSettings::DihedralBend>(is);
 	a.t = ReadArray<SoftBodySharedSettings::Volume>(is);
-	uint32 n_skinned = 0; is.read((char *)&n_skinned, 4);
-	uint32 n_skinned_normals = 0; is.read((char *)&n_skinned_normals, 4);
+	a.sk = ReadArray<SoftBodySharedSettings::Skinned>(is);
+	a.skn = ReadArray<uint32>(is);
 	a.l = ReadArray<SoftBodySharedSettings::LRA>(is);
 	a.g = ReadArray<UpdateGroupRec>(is);
 	return a;
@@ -279,12 +281,72 @@ void *jo_settings_build(const void *vertices, uint32 nv, const uint32 *faces, ui
 	return new SettingsHandle{ s };
 }
 
+// Skinning: appends skinned constraints (52 B records) and inverse bind matrices (80 B:
+// joint index, 12 B padding, 16 floats column major), then CalculateSkinnedConstraintNormals.
+void jo_settings_add_skin(void *h, const void *skinned, uint32 ns, const void *inv_binds, uint32 nib)
+{
+	SoftBodySharedSettings &st = *((SettingsHandle *)h)->mSettings;
+	const size_t first = st.mSkinnedConstraints.size();
+	st.mSkinnedConstraints.resize(first + ns);
+	if (ns) memcpy(&st.mSkinnedConstraints[first], skinned, 52ull * ns);
+	for (uint32 k = 0; k < nib; ++k)
+	{
+		const char *r = (const char *)inv_binds + 80ull * k;
+		uint32 joint;
+		memcpy(&joint, r, 4);
+		Mat44 m;
+		for (int c = 0; c < 4; ++c)
+		{
+			float col[4];
+			memcpy(col, r + 16 + 16 * c, 16);
+			m.SetColumn4(c, Vec4(col[0], col[1], col[2], col[3]));
+		}
+		st.mInvBindMatrices.push_back(SoftBodySharedSettings::InvBind(joint, m));
+	}
+	st.CalculateSkinnedConstraintNormals();
+}
+
+void jo_settings_optimize(void *h) { ((SettingsHandle *)h)->mSettings->Optimize(); }
+
+// counts: skinned constraints, skinned constraint normals, inverse bind matrices
+void jo_settings_skin_counts(void *h, uint32 *counts)
+{
+	const SoftBodySharedSettings &st = *((SettingsHandle *)h)->mSettings;
+	Arrays a = ReadBack(st);
+	counts[0] = (uint32)a.sk.size();
+	counts[1] = (uint32)a.skn.size();
+	counts[2] = (uint32)st.mInvBindMatrices.size();
+}
+
+void jo_settings_skin_read(void *h, void *skinned, uint32 *normals, void *inv_binds)
+{
+	const SoftBodySharedSettings &st = *((SettingsHandle *)h)->mSettings;
+	Arrays a = ReadBack(st);
+	if (skinned && !a.sk.empty())
+		memcpy(skinned, a.sk.data(), 52ull * a.sk.size());
+	if (normals && !a.skn.empty())
+		memcpy(normals, a.skn.data(), 4ull * a.skn.size());
+	for (size_t k = 0; inv_binds && k < st.mInvBindMatrices.size(); ++k)
+	{
+		char *r = (char *)inv_binds + 80 * k;
+		memset(r, 0, 80);
+		memcpy(r, &st.mInvBindMatrices[k].mJointIndex, 4);
+		for (int c = 0; c < 4; ++c)
+		{
+			Vec4 col = st.mInvBindMatrices[k].mInvBind.GetColumn4(c);
+			float f[4] = { col.GetX(), col.GetY(), col.GetZ(), col.GetW() };
+			memcpy(r + 16 + 16 * c, f, 16);
+		}
+	}
+}
+
 // Builds settings from explicit arrays (everything already computed, including
 // the update groups) through the public RestoreBinaryState.
 void *jo_settings_from_arrays(const void *vertices, uint32 nv, const uint32 *faces4, uint32 nf,
 							  const void *edges, uint32 ne, const void *bends, uint32 nb,
 							  const void *volumes, uint32 nt, const void *lras, uint32 nl,
-							  const uint32 *groups7, uint32 ng)
+							  const uint32 *groups7, uint32 ng, const void *skinned, uint32 ns,
+							  const uint32 *skinned_normals, uint32 nsn, const void *inv_binds, uint32 nib)
 {
 	InitJolt();
 	std::stringstream ss;
@@ -294,13 +356,19 @@ void *jo_settings_from_arrays(const void *vertices, uint32 nv, const uint32 *fac
 	WriteArray(ss, (const char(*)[24])bends, nb);
 	WriteArray(ss, (const char(*)[24])volumes, nt);
 	uint32 zero = 0;
-	ss.write((const char *)&zero, 4);				// skinned
-	ss.write((const char *)&zero, 4);				// skinned normals
+	WriteArray(ss, (const char(*)[52])skinned, ns);
+	WriteArray(ss, skinned_normals, nsn);
 	WriteArray(ss, (const char(*)[12])lras, nl);
 	WriteArray(ss, (const char(*)[28])groups7, ng);
 	ss.write((const char *)&zero, 4);				// rod stretch shear
 	ss.write((const char *)&zero, 4);				// rod bend twist
-	ss.write((const char *)&zero, 4);				// inv bind matrices
+	ss.write((const char *)&nib, 4);				// inv bind matrices: joint index, Mat44 (no padding)
+	for (uint32 k = 0; k < nib; ++k)
+	{
+		const char *r = (const char *)inv_binds + 80ull * k;
+		ss.write(r, 4);
+		ss.write(r + 16, 64);
+	}
 	ss.seekg(0);
 	Ref<SoftBodySharedSettings> s = new SoftBodySharedSettings;
 	StreamInWrapper in(ss);
@@ -661,6 +729,30 @@ int jo_body_custom_update(void *wp, int body, float dt)
 	SoftBodyMotionProperties *mp = static_cast<SoftBodyMotionProperties *>(b.GetMotionProperties());
 	lock.ReleaseLock();
 	mp->CustomUpdate(dt, b, w->mSystem);
+	return 0;
+}
+
+// SoftBodyMotionProperties::SkinVertices of soft body `body` with n joint matrices
+// (16 floats each, column major), then optionally SetEnableSkinConstraints /
+// SetSkinnedMaxDistanceMultiplier first (enable < 0 leaves them).
+int jo_body_skin_vertices(void *wp, int body, const float *joints, uint32 n, int hard_skin_all, int enable, float multiplier)
+{
+	World *w = (World *)wp;
+	BodyLockWrite lock(w->mSystem.GetBodyLockInterface(), w->mSoftBodies[body]);
+	if (!lock.Succeeded())
+		return -1;
+	Body &b = lock.GetBody();
+	SoftBodyMotionProperties *mp = static_cast<SoftBodyMotionProperties *>(b.GetMotionProperties());
+	if (enable >= 0)
+	{
+		mp->SetEnableSkinConstraints(enable != 0);
+		mp->SetSkinnedMaxDistanceMultiplier(multiplier);
+	}
+	std::vector<Mat44> m(n);
+	for (uint32 j = 0; j < n; ++j)
+		for (int c = 0; c < 4; ++c)
+			m[j].SetColumn4(c, Vec4(joints[16 * j + 4 * c], joints[16 * j + 4 * c + 1], joints[16 * j + 4 * c + 2], joints[16 * j + 4 * c + 3]));
+	mp->SkinVertices(b.GetCenterOfMassTransform(), m.data(), n, hard_skin_all != 0, *w->mTemp);
 	return 0;
 }
 

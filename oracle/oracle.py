@@ -16,8 +16,8 @@ import os
 
 import numpy as np
 
-from joltphysics_amd.records import (BEND, EDGE, FACE, GROUP, LRA, MOTION_DYNAMIC, SHAPE_CONVEX_HULL, VERTEX, VOLUME,
-                                     ConvexHull, SettingsArrays)
+from joltphysics_amd.records import (BEND, EDGE, FACE, GROUP, INV_BIND, LRA, MOTION_DYNAMIC, SHAPE_CONVEX_HULL, SKINNED,
+                                     VERTEX, VOLUME, ConvexHull, SettingsArrays)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(HERE, "_ref")
@@ -72,7 +72,11 @@ class JoltReference:
                                               _vp, _u32, _i, _f, _f, _u32]
             lib.jo_settings_from_arrays.restype = _vp
             lib.jo_settings_from_arrays.argtypes = [_vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32,
-                                                    _vp, _u32, _vp, _u32]
+                                                    _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32]
+            lib.jo_settings_add_skin.argtypes = [_vp, _vp, _u32, _vp, _u32]
+            lib.jo_settings_optimize.argtypes = [_vp]
+            lib.jo_settings_skin_counts.argtypes = [_vp, _vp]
+            lib.jo_settings_skin_read.argtypes = [_vp, _vp, _vp, _vp]
             lib.jo_settings_counts.argtypes = [_vp, _vp]
             lib.jo_settings_read.argtypes = [_vp] * 8
             lib.jo_settings_free.argtypes = [_vp]
@@ -96,6 +100,8 @@ class JoltReference:
             lib.jo_world_collider_motion.restype = _i
             lib.jo_world_record_contacts.argtypes = [_vp]
             lib.jo_world_contacts.restype = _i
+            lib.jo_body_skin_vertices.restype = _i
+            lib.jo_body_skin_vertices.argtypes = [_vp, _i, _vp, _u32, _i, _i, _f]
             lib.jo_body_save_state.restype = C.c_long
             lib.jo_body_save_state.argtypes = [_vp, _i, _vp, C.c_long]
             lib.jo_body_restore_state.restype = _i
@@ -128,10 +134,21 @@ class JoltReference:
         t = np.ascontiguousarray(mesh.volumes, VOLUME)
         lr = np.ascontiguousarray(mesh.lras, LRA)
         a = np.ascontiguousarray(mesh.attrs, np.float32).reshape(-1, 5)
+        skinned = getattr(mesh, "skinned", None)
+        flags = int(mesh.flags)
+        optimize = 16
+        if skinned is not None:
+            flags &= ~optimize          # skinned constraints join before Optimize
         h = self.lib.jo_settings_build(_ptr(v), len(v), _ptr(f), len(f), _ptr(e), len(e), None, 0,
                                        _ptr(t), len(t), _ptr(lr), len(lr), _ptr(a), len(a), int(mesh.bend_type),
-                                       float(mesh.angle_tolerance), float(mesh.lra_multiplier), int(mesh.flags))
+                                       float(mesh.angle_tolerance), float(mesh.lra_multiplier), flags)
         try:
+            if skinned is not None:
+                sk = np.ascontiguousarray(skinned, SKINNED)
+                ib = np.ascontiguousarray(mesh.inv_binds, INV_BIND)
+                self.lib.jo_settings_add_skin(h, _ptr(sk), len(sk), _ptr(ib), len(ib))
+                if int(mesh.flags) & optimize:
+                    self.lib.jo_settings_optimize(h)
             return self._read_settings(h)
         finally:
             self.lib.jo_settings_free(h)
@@ -144,6 +161,13 @@ class JoltReference:
                            np.zeros(nt, VOLUME), np.zeros(nl, LRA), np.zeros(ng, GROUP))
         self.lib.jo_settings_read(h, _ptr(s.vertices), _ptr(s.faces), _ptr(s.edges), _ptr(s.bends),
                                   _ptr(s.volumes), _ptr(s.lras), _ptr(s.groups))
+        sc = np.zeros(3, np.uint32)
+        self.lib.jo_settings_skin_counts(h, _ptr(sc))
+        if sc[0] or sc[2]:
+            s.skinned = np.zeros(int(sc[0]), SKINNED)
+            s.skinned_normals = np.zeros(int(sc[1]), np.uint32)
+            s.inv_binds = np.zeros(int(sc[2]), INV_BIND)
+            self.lib.jo_settings_skin_read(h, _ptr(s.skinned), _ptr(s.skinned_normals), _ptr(s.inv_binds))
         return s
 
     def settings_handle(self, s: SettingsArrays):
@@ -151,7 +175,9 @@ class JoltReference:
         h = self.lib.jo_settings_from_arrays(_ptr(s.vertices), len(s.vertices), _ptr(f4), len(f4),
                                              _ptr(s.edges), len(s.edges), _ptr(s.bends), len(s.bends),
                                              _ptr(s.volumes), len(s.volumes), _ptr(s.lras), len(s.lras),
-                                             _ptr(s.groups), len(s.groups))
+                                             _ptr(s.groups), len(s.groups), _ptr(s.skinned), len(s.skinned),
+                                             _ptr(s.skinned_normals), len(s.skinned_normals), _ptr(s.inv_binds),
+                                             len(s.inv_binds))
         if not h:
             raise RuntimeError("reference rejected the settings arrays")
         return h
@@ -216,6 +242,14 @@ class JoltWorld:
         com = np.zeros(3, np.float32)
         self.lib.jo_world_collider_device(self.h, i, _ptr(params), _ptr(com))
         return tuple(float(x) for x in params), tuple(float(x) for x in com)
+
+    def skin_vertices(self, body, joints, hard_skin_all=False, enable=None, multiplier=1.0):
+        """SoftBodyMotionProperties::SkinVertices with joint matrices (n, 16) column major;
+        enable / multiplier set SetEnableSkinConstraints / SetSkinnedMaxDistanceMultiplier."""
+        j = np.ascontiguousarray(joints, np.float32).reshape(-1, 16)
+        e = -1 if enable is None else int(bool(enable))
+        if self.lib.jo_body_skin_vertices(self.h, body, _ptr(j), len(j), int(bool(hard_skin_all)), e, float(multiplier)) != 0:
+            raise RuntimeError("SkinVertices failed")
 
     def save_state(self, body):
         """SoftBodyMotionProperties::SaveState bytes of soft body `body`."""

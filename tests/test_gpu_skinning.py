@@ -1,0 +1,71 @@
+"""Skinned constraints (SoftBodyMotionProperties.cpp:509-571) and SkinVertices
+(:1139-1226) against the reference, bit-exact every frame: a cloth skinned to two moving
+joints (kinematic top row, max distance clamps, back stop spheres), two collision steps
+per frame (the skinned positions are interpolated over each step's iterations); then
+with skin constraints disabled (only kinematic vertices follow) and a max distance
+multiplier; and on a body too large for one workgroup's LDS (streamed)."""
+from __future__ import annotations
+
+import pytest
+
+import scene_zoo
+from helpers import float_equal
+from oracle import have_reference
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_reference(sc, settings, joints, frames, options):
+    from oracle import JoltReference, JoltWorld
+    w = JoltWorld(JoltReference("nofma"), sc.gravity)
+    b = sc.bodies[0]
+    w.add_soft_body(settings, b.position, b.rotation, b.params)
+    out = {}
+    for f in range(frames + 1):
+        enable, mult = options(f)
+        w.skin_vertices(0, joints(f), hard_skin_all=(f == 0), enable=enable, multiplier=mult)
+        if f > 0:
+            w.update(sc.dt, sc.collision_steps, 1)
+            out[f] = w.read(0)
+    w.close()
+    return out
+
+
+def _run_gpu(sc, settings, joints, frames, options):
+    from joltphysics_amd.xpbd import Context
+    ctx = Context(0)
+    ctx.set_gravity(sc.gravity)
+    b = sc.bodies[0]
+    body = ctx.create_body(ctx.create_settings(settings), b.position, b.rotation, b.params)
+    out = {}
+    for f in range(frames + 1):
+        enable, mult = options(f)
+        ctx.set_skin_options(body, enable, mult)
+        ctx.skin_vertices(body, joints(f), hard_skin_all=(f == 0))
+        if f > 0:
+            ctx.update(sc.dt, sc.collision_steps, 1)
+            out[f] = ctx.read(body)
+    ctx.close()
+    return out
+
+
+@pytest.mark.skipif(not have_reference("nofma"), reason="reference library not built")
+@pytest.mark.parametrize("variant", ["enabled", "disabled_then_multiplier", "streamed"])
+def test_skinned_cloth_bit_exact(variant):
+    from oracle import JoltReference
+    # streamed: 120 x 120 vertices, past the LDS budget of one workgroup
+    sc, joints = scene_zoo.skinned_cloth(n=120, frames=4, spacing=0.01) if variant == "streamed" else scene_zoo.skinned_cloth()
+    settings = JoltReference("nofma").build_settings(sc.bodies[0].mesh)
+    assert len(settings.skinned) == len(settings.vertices) and len(settings.skinned_normals) > 0
+    if variant == "enabled":
+        options = lambda f: (True, 1.0)
+    else:
+        options = lambda f: (f < 4 or f > 7, 1.0 if f < 8 else 0.5)
+    frames = sc.frames
+    print(f"skinned cloth {variant}: {len(settings.vertices)} vertices, {frames} frames", flush=True)
+    ref = _run_reference(sc, settings, joints, frames, options)
+    gpu = _run_gpu(sc, settings, joints, frames, options)
+    for f in range(1, frames + 1):
+        assert float_equal(gpu[f]["pos"], ref[f]["pos"]), f"positions differ at frame {f}"
+        assert float_equal(gpu[f]["vel"], ref[f]["vel"]), f"velocities differ at frame {f}"
+        assert float_equal(gpu[f]["body_pos"], ref[f]["body_pos"]), f"body position differs at frame {f}"
