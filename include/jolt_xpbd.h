@@ -226,6 +226,10 @@ typedef struct xpbd_collider_desc {
     float inertia_rotation[4];      /* MotionProperties::GetInertiaRotation(), x y z w */
     float linear_velocity[3];       /* world space; held fixed for the whole xpbd_update call */
     float angular_velocity[3];
+    /* Body::IsSensor(): the soft body does not collide with it; each step it records whether a
+     * vertex penetrates it (DetermineSensorCollisions, SoftBodyMotionProperties.cpp:267-289;
+     * xpbd_body_sensor_contacts).  The reference considers sensors only with a contact listener. */
+    int32_t is_sensor;
 } xpbd_collider_desc;
 
 int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *colliders, uint32_t num_colliders);
@@ -279,6 +283,11 @@ int xpbd_body_velocity_deltas(xpbd_context *ctx, int32_t body, uint32_t num_coll
  * contact normal is minus the collision plane's normal (xpbd_body_read).  Any output may be NULL.
  */
 int xpbd_body_contacts(xpbd_context *ctx, int32_t body, float *points, int32_t *collider);
+
+/* Sensor contacts of soft body `body` in the last collision step of the last xpbd_update:
+ * per collider of the last xpbd_set_colliders 1 when it is a sensor that detected a vertex
+ * (the manifold's GetNumSensorContacts / GetSensorContactBodyID list), else 0. */
+int xpbd_body_sensor_contacts(xpbd_context *ctx, int32_t body, uint32_t num_colliders, int32_t *hits);
 
 /*
  * The byte stream SoftBodyMotionProperties::SaveState writes for a soft body without

@@ -39,6 +39,8 @@ struct BodyLocal
     uint32_t num_dynamic;       // dynamic colliders of the context
     uint32_t report_cleared;    // asleep: its contact report was cleared
     const float4 *skin_entries; // the body's skin entries (4 float4 per vertex), or null
+    int32_t *sensor_hits;       // the body's sensor contact row, or null
+    uint32_t num_sensors;
     uint32_t ran;               // the body was active at the start of this step
     DynStateDev *dyn;           // the body's row of dynamic-collider states
     uint32_t nf;
@@ -142,6 +144,8 @@ __device__ __forceinline__ void bundle_update(
         L.dyn = B.dyn;
         L.report_cleared = 0;
         L.skin_entries = B.skin ? B.skin + 4u * (size_t)B.skin_base : nullptr;
+        L.sensor_hits = B.sensor_hits;
+        L.num_sensors = B.sensor_hits ? B.num_sensors : 0u;
         L.num_dynamic = B.dyn ? B.num_dynamic : 0u;
         for (uint32_t k = 0; k < L.num_dynamic; ++k)
             clear_dynamic(L.dyn[k]);
@@ -241,7 +245,7 @@ __device__ __forceinline__ void bundle_update(
             {
                 uint32_t n = 0, nd = 0;
                 for (uint32_t c = 0; c < nc; ++c)
-                    if (leaves_full[b * nc + c].type >= 0)
+                    if (leaves_full[b * nc + c].type >= 0 && colliders[c].sensor < 0)
                     {
                         nd += leaves_full[b * nc + c].dyn >= 0 ? 1u : 0u;
                         leaves[b * nc + n++] = leaves_full[b * nc + c];
@@ -250,6 +254,30 @@ __device__ __forceinline__ void bundle_update(
                 loc[b].num_dyn_leaves = nd;
             }
             sync_global();
+
+            // ---- DetermineSensorCollisions (SoftBodyMotionProperties.cpp:267-289), one wavefront
+            // per body: a sensor reports contact when a vertex penetrates it; sensors never collide
+            for (uint32_t b = wave; b < nb; b += (BLOCK / kWave))
+            {
+                const BodyLocal &L = loc[b];
+                if (L.num_sensors == 0)
+                    continue;
+                const BundleBodyDev bb = G.bodies[b];
+                for (uint32_t c = 0; c < nc; ++c)
+                {
+                    const int32_t rank = colliders[c].sensor;
+                    if (rank < 0)
+                        continue;
+                    const Leaf &leaf = leaves_full[b * nc + c];
+                    float largest = -FLT_MAX;
+                    if (st[b].active && leaf.type >= 0)
+                        for (uint32_t k = lane; k < bb.nv; k += kWave)
+                            largest = smax(largest, sensor_penetration(leaf, pget(X, bb.voff + k)));
+                    largest = wave_max(largest);
+                    if (lane == 0)
+                        L.sensor_hits[rank] = largest > 0.0f ? 1 : 0;
+                }
+            }
 
             // ---- DetermineCollisionPlanes (SoftBodyMotionProperties.cpp:257-265)
             for (uint32_t i = tid; i < nv; i += BLOCK)

@@ -853,6 +853,21 @@ __device__ __forceinline__ float4 contact_report(V3 x, float4 pl, bool has_conta
     return f4(x - signed_distance(plane, x) * plane.n, __int_as_float(collider));
 }
 
+// DetermineSensorCollisions (SoftBodyMotionProperties.cpp:267-289) for one vertex: the
+// largest penetration of the sensor's leaf over the vertices decides its contact.
+__device__ __forceinline__ float sensor_penetration(const Leaf &leaf, float4 p)
+{
+    if (!(p.w > 0.0f))
+        return -FLT_MAX;
+    float largest = -FLT_MAX;
+    int32_t idx = -1;
+    Pl plane;
+    plane.n = v3(0.0f, 0.0f, 0.0f);
+    plane.c = 0.0f;
+    collide_vertex(&leaf, 1, xyz(p), largest, idx, plane);
+    return largest;
+}
+
 __device__ __forceinline__ void clear_dynamic(DynStateDev &D)
 {
     st3(D.acc_lin, v3(0.0f, 0.0f, 0.0f));

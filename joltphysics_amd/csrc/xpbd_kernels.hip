@@ -91,6 +91,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
     __shared__ float sh_scalars[8];
     __shared__ uint32_t sh_num_leaves;
     __shared__ uint32_t sh_num_dyn_leaves;
+    __shared__ Leaf sh_sensor_leaf;
 
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
@@ -157,8 +158,13 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
             {
                 // a body that does not run in a step reports no contacts for it
                 if (!report_cleared)
+                {
                     for (uint32_t i = tid; i < nv; i += kBlock)
                         B.report[i] = contact_report(v3(0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f), false, -1);
+                    if (B.sensor_hits)
+                        for (uint32_t k = tid; k < B.num_sensors; k += kBlock)
+                            B.sensor_hits[k] = 0;
+                }
                 report_cleared = true;
                 break;
             }
@@ -201,6 +207,11 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                     uint32_t n = 0, nd = 0;
                     for (uint32_t ci = 0; ci < num_colliders; ++ci)
                     {
+                        if (colliders[ci].sensor >= 0)
+                        {
+                            slots[ci] = slots[ci] ? -2 : -1;    // sensors never collide; -2: overlapping
+                            continue;
+                        }
                         if (slots[ci])
                         {
                             if (n < max_leaves)
@@ -237,6 +248,39 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
             }
             const uint32_t num_leaves = sh_num_leaves;
             const uint32_t num_dyn_leaves = sh_num_dyn_leaves;
+
+            // DetermineSensorCollisions (SoftBodyMotionProperties.cpp:267-289): a sensor
+            // reports contact when a vertex penetrates it
+            if (B.sensor_hits)
+            {
+                const M44 inv_com_s = inversed_rotation_translation(com);
+                for (uint32_t ci = 0; ci < num_colliders; ++ci)
+                {
+                    const int32_t rank = colliders[ci].sensor;
+                    if (rank < 0)
+                        continue;
+                    float largest = -FLT_MAX;
+                    if (slots[ci] == -2)
+                    {
+                        if (tid == 0)
+                            make_leaf(colliders[ci], inv_com_s, B.friction, B.restitution, sh_sensor_leaf, nullptr);
+                        __syncthreads();
+                        for (uint32_t i = tid; i < nv; i += kBlock)
+                            largest = smax(largest, sensor_penetration(sh_sensor_leaf, P[i]));
+                    }
+                    largest = wave_max(largest);
+                    if (lane == 0)
+                        red[wave * 32] = largest;
+                    __syncthreads();
+                    if (tid == 0)
+                    {
+                        for (int w = 1; w < kBlock / kWave; ++w)
+                            largest = smax(largest, red[w * 32]);
+                        B.sensor_hits[rank] = largest > 0.0f ? 1 : 0;
+                    }
+                    __syncthreads();
+                }
+            }
 
             // DetermineCollisionPlanes (SoftBodyMotionProperties.cpp:257-265)
             if (num_leaves > 0)

@@ -181,6 +181,9 @@ struct xpbd_context
     DevBuf<float4> d_skin;
     bool skin_dirty = false;        // host copy newer: upload before the next update
     bool skin_on_device = false;    // device copy newer (updates move and invalidate skin states)
+    uint32_t num_sensors = 0;       // sensor colliders of the last xpbd_set_colliders
+    DevBuf<int32_t> d_sensor_hits;  // bodies x num_sensors
+    uint32_t sensor_stride = 0;
     uint32_t num_dynamic = 0;       // dynamic colliders of the last xpbd_set_colliders
     DevBuf<DynStateDev> d_dyn;      // bodies x num_dynamic
     uint32_t dyn_stride = 0;        // num_dynamic the rows of d_dyn are laid out for
@@ -1363,7 +1366,7 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
         return Fail(XPBD_E_INVALID, "null argument");
     HIP_TRY(hipSetDevice(ctx->device));
     std::vector<ColliderDev> v(n);
-    uint32_t num_dynamic = 0;
+    uint32_t num_dynamic = 0, num_sensors = 0;
     // convex hull tables: validated here, flattened into two pools, pointers set after upload
     std::vector<float4> hull_vec;
     std::vector<uint32_t> hull_idx;
@@ -1413,6 +1416,7 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
         v[i].hull = nullptr;
         v[i].motion = c[i].motion_type;
         v[i].dyn = -1;
+        v[i].sensor = c[i].is_sensor ? (int32_t)num_sensors++ : -1;
         if (c[i].motion_type == 2)
         {
             v[i].dyn = (int32_t)num_dynamic++;
@@ -1450,6 +1454,7 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
         ctx->layout_dirty = true;
     ctx->colliders = v;
     ctx->num_dynamic = num_dynamic;
+    ctx->num_sensors = num_sensors;
     HIP_TRY(ctx->d_colliders.upload(v));
     return XPBD_OK;
 }
@@ -1567,6 +1572,25 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
                 ctx->bodies[i]->dev.num_dynamic = nd;
             }
             ctx->dyn_stride = nd;
+            ctx->tables_dirty = true;
+        }
+    }
+    // sensor colliders: one row of contact flags per body
+    {
+        const uint32_t ns = ctx->num_sensors;
+        const size_t need = ns ? ctx->bodies.size() * ns : 0;
+        if (ctx->d_sensor_hits.n != need || ctx->sensor_stride != ns)
+        {
+            HIP_TRY(hipStreamSynchronize(ctx->stream));
+            HIP_TRY(ctx->d_sensor_hits.alloc(need));
+            if (need)
+                HIP_TRY(hipMemset(ctx->d_sensor_hits.p, 0, sizeof(int32_t) * need));
+            for (size_t i = 0; i < ctx->bodies.size(); ++i)
+            {
+                ctx->bodies[i]->dev.sensor_hits = ns ? ctx->d_sensor_hits.p + i * ns : nullptr;
+                ctx->bodies[i]->dev.num_sensors = ns;
+            }
+            ctx->sensor_stride = ns;
             ctx->tables_dirty = true;
         }
     }
@@ -2000,6 +2024,27 @@ int xpbd_body_contacts(xpbd_context *ctx, int32_t body, float *points, int32_t *
         memcpy(&c, &rep[i].w, sizeof(c));
         if (points) { points[3 * i] = rep[i].x; points[3 * i + 1] = rep[i].y; points[3 * i + 2] = rep[i].z; }
         if (collider) collider[i] = c;
+    }
+    return XPBD_OK;
+}
+
+int xpbd_body_sensor_contacts(xpbd_context *ctx, int32_t body, uint32_t num_colliders, int32_t *hits)
+{
+    if (!ctx || body < 0 || body >= (int32_t)ctx->bodies.size() || !hits)
+        return Fail(XPBD_E_INVALID, "invalid body or output");
+    if (num_colliders != ctx->colliders.size())
+        return Fail(XPBD_E_INVALID, "num_colliders differs from the last xpbd_set_colliders");
+    int rc = xpbd_synchronize(ctx);
+    if (rc != XPBD_OK)
+        return rc;
+    std::vector<int32_t> row(ctx->sensor_stride);
+    if (ctx->sensor_stride && ctx->d_sensor_hits.n)
+        HIP_TRY(hipMemcpy(row.data(), ctx->d_sensor_hits.p + (size_t)body * ctx->sensor_stride, sizeof(int32_t) * row.size(),
+                          hipMemcpyDeviceToHost));
+    for (uint32_t c = 0; c < num_colliders; ++c)
+    {
+        const int32_t k = ctx->colliders[c].sensor;
+        hits[c] = k >= 0 && (uint32_t)k < row.size() ? row[k] : 0;
     }
     return XPBD_OK;
 }

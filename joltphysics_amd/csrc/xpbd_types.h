@@ -67,6 +67,8 @@ struct BodyDev
     uint32_t num_dynamic;
     const float4 *skin;         // skinned vertex states of the context (4 per entry), or null
     uint32_t skin_base;         // first entry of this body (entry = skin_base + body vertex index)
+    int32_t *sensor_hits;       // this body's row: 1 where a sensor detected a vertex in the last step, or null
+    uint32_t num_sensors;
 };
 
 // Per-body dynamic state kept on the device between launches.
@@ -108,6 +110,7 @@ struct ColliderDev
     float inv_inertia_diag[3];  // MotionProperties::GetInverseInertiaDiagonal
     float inertia_rot[4];       // MotionProperties::GetInertiaRotation
     float lin_vel[3], ang_vel[3];   // world space
+    int32_t sensor;             // rank among the sensor colliders, -1 for a colliding body
 };
 
 // State of one dynamic collider as one soft body sees it during a collision step

@@ -194,6 +194,7 @@ class Collider:
     inertia_rotation: tuple = (0.0, 0.0, 0.0, 1.0)
     linear_velocity: tuple = (0.0, 0.0, 0.0)
     angular_velocity: tuple = (0.0, 0.0, 0.0)
+    is_sensor: bool = False     # Body::IsSensor(): detects vertices, never collides
 
     def param_array(self):
         if self.type == SHAPE_CONVEX_HULL:

@@ -65,7 +65,7 @@ class _ColliderDesc(C.Structure):
                 ("world_bounds", C.c_float * 6), ("hull", C.POINTER(_ConvexHull)),
                 ("motion_type", C.c_int32), ("inv_mass", C.c_float), ("inv_inertia_diagonal", C.c_float * 3),
                 ("inertia_rotation", C.c_float * 4), ("linear_velocity", C.c_float * 3),
-                ("angular_velocity", C.c_float * 3)]
+                ("angular_velocity", C.c_float * 3), ("is_sensor", C.c_int32)]
 
 
 class _RolloutDesc(C.Structure):
@@ -97,7 +97,8 @@ EXPORTED_SYMBOLS = (
     "xpbd_settings_build", "xpbd_built_counts", "xpbd_built_read", "xpbd_built_free",
     "xpbd_build_last_error", "xpbd_body_create",
     "xpbd_set_colliders", "xpbd_set_gravity", "xpbd_update", "xpbd_synchronize", "xpbd_body_read",
-    "xpbd_body_write", "xpbd_body_state", "xpbd_body_velocity_deltas", "xpbd_body_contacts", "xpbd_body_skin_vertices",
+    "xpbd_body_write", "xpbd_body_state", "xpbd_body_velocity_deltas", "xpbd_body_contacts", "xpbd_body_sensor_contacts",
+    "xpbd_body_skin_vertices",
     "xpbd_body_set_skin_options", "xpbd_body_save_state", "xpbd_body_restore_state", "xpbd_profile_enable",
     "xpbd_profile_read",
     "xpbd_rollout_create", "xpbd_rollout_destroy", "xpbd_rollout_sizes", "xpbd_rollout_forward",
@@ -148,6 +149,7 @@ def load_library(path=LIB_PATH):
     lib.xpbd_body_state.argtypes = [vp, i32, vp]
     lib.xpbd_body_velocity_deltas.argtypes = [vp, i32, u32, vp, vp, vp]
     lib.xpbd_body_contacts.argtypes = [vp, i32, vp, vp]
+    lib.xpbd_body_sensor_contacts.argtypes = [vp, i32, u32, vp]
     lib.xpbd_body_skin_vertices.argtypes = [vp, i32, vp, u32, i32]
     lib.xpbd_body_set_skin_options.argtypes = [vp, i32, i32, f]
     lib.xpbd_body_save_state.argtypes = [vp, i32, vp, C.c_size_t, C.POINTER(C.c_size_t)]
@@ -291,6 +293,7 @@ class Context:
             for k in range(6):
                 arr[i].world_bounds[k] = float(c.bounds[k])
             arr[i].motion_type = int(c.motion_type)
+            arr[i].is_sensor = int(bool(c.is_sensor))
             if c.motion_type == MOTION_DYNAMIC:
                 arr[i].inv_mass = float(c.inv_mass)
                 for k in range(3):
@@ -333,6 +336,13 @@ class Context:
         pts = np.zeros((max(nv, 1), 3), np.float32)
         _check(self.lib.xpbd_body_contacts(self.h, int(body), pts.ctypes.data, col.ctypes.data))
         return col[:nv], pts[:nv]
+
+    def sensor_contacts(self, body):
+        """Per collider: 1 where a sensor detected a vertex of `body` in the last collision step."""
+        n = getattr(self, "_num_colliders", 0)
+        hits = np.zeros(max(n, 1), np.int32)
+        _check(self.lib.xpbd_body_sensor_contacts(self.h, int(body), n, hits.ctypes.data))
+        return hits[:n]
 
     def velocity_deltas(self, body):
         """(linear (n, 3), angular (n, 3), updated (n,)): the world-space velocity changes soft

@@ -123,7 +123,7 @@ struct SettingsHandle
 // contact point.  The step listener clears the records at the start of every step.
 struct ContactRecorder final : public SoftBodyContactListener, public PhysicsStepListener
 {
-	struct Record { std::vector<int> collider; std::vector<float> point; };
+	struct Record { std::vector<int> collider; std::vector<float> point; std::vector<int> sensors; };
 	std::mutex mMutex;
 	std::map<uint32, Record> mRecords;          // key: soft body BodyID index and sequence
 	std::map<uint32, int> mColliderIndex;       // collider BodyID -> collider index
@@ -151,6 +151,11 @@ struct ContactRecorder final : public SoftBodyContactListener, public PhysicsSte
 				r.collider.push_back(-1);
 				r.point.insert(r.point.end(), { 0.0f, 0.0f, 0.0f });
 			}
+		}
+		for (uint k = 0; k < inManifold.GetNumSensorContacts(); ++k)
+		{
+			auto it = mColliderIndex.find(inManifold.GetSensorContactBodyID(k).GetIndexAndSequenceNumber());
+			r.sensors.push_back(it == mColliderIndex.end() ? -2 : it->second);
 		}
 		std::lock_guard<std::mutex> lock(mMutex);
 		mRecords[inSoftBody.GetID().GetIndexAndSequenceNumber()] = std::move(r);
@@ -423,6 +428,29 @@ void *jo_world_create(uint32 max_bodies, int num_threads)
 }
 
 void jo_world_free(void *wp) { delete (World *)wp; }
+
+// Marks collider `collider` as a sensor (Body::SetIsSensor).
+void jo_world_set_sensor(void *wp, int collider)
+{
+	World *w = (World *)wp;
+	w->mSystem.GetBodyInterface().SetIsSensor(w->mColliders[collider], true);
+}
+
+// Sensor contacts recorded for soft body `body` in the last collision step: hits[c] = 1 for
+// every collider c in the manifold's sensor list (n = number of colliders).
+int jo_world_sensor_contacts(void *wp, int body, int *hits, int n)
+{
+	World *w = (World *)wp;
+	if (!w->mRecorder)
+		return -1;
+	for (int c = 0; c < n; ++c) hits[c] = 0;
+	auto it = w->mRecorder->mRecords.find(w->mSoftBodies[body].GetIndexAndSequenceNumber());
+	if (it != w->mRecorder->mRecords.end())
+		for (int c : it->second.sensors)
+			if (c >= 0 && c < n)
+				hits[c] = 1;
+	return 0;
+}
 
 // Installs a contact listener that records each soft body's SoftBodyManifold of the last
 // collision step (jo_world_contacts).  Call after the colliders are added.

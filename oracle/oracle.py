@@ -99,6 +99,9 @@ class JoltReference:
             lib.jo_world_add_dynamic.argtypes = [_vp, _i, _vp, _vp, _vp, _f, _f, _vp, _vp]
             lib.jo_world_collider_motion.restype = _i
             lib.jo_world_record_contacts.argtypes = [_vp]
+            lib.jo_world_set_sensor.argtypes = [_vp, _i]
+            lib.jo_world_sensor_contacts.restype = _i
+            lib.jo_world_sensor_contacts.argtypes = [_vp, _i, _vp, _i]
             lib.jo_world_contacts.restype = _i
             lib.jo_body_skin_vertices.restype = _i
             lib.jo_body_skin_vertices.argtypes = [_vp, _i, _vp, _u32, _i, _i, _f]
@@ -229,6 +232,9 @@ class JoltWorld:
                                                float(c.friction), float(c.restitution))
         if i < 0:
             raise RuntimeError("add_collider failed")
+        if c.is_sensor:
+            self.lib.jo_world_set_sensor(self.h, i)
+        self._num_colliders = i + 1
         return i
 
     def collider_bounds(self, i):
@@ -266,6 +272,14 @@ class JoltWorld:
         """Installs the contact listener that records each soft body's SoftBodyManifold of
         the last collision step (call after adding the colliders)."""
         self.lib.jo_world_record_contacts(self.h)
+
+    def sensor_contacts(self, body):
+        """Per collider: 1 where the manifold of the last step listed it as a sensor contact."""
+        n = getattr(self, "_num_colliders", 0)
+        hits = np.zeros(max(n, 1), np.int32)
+        if self.lib.jo_world_sensor_contacts(self.h, body, _ptr(hits), n) != 0:
+            raise RuntimeError("contact recording is not enabled")
+        return hits[:n]
 
     def contacts(self, body):
         """(collider index per vertex, -1 without contact; local contact points (V, 3))."""
@@ -361,6 +375,8 @@ class Restatement:
             lib.xr_world_add_collider.restype = _i
             lib.xr_world_add_collider.argtypes = [_vp, _i, _vp, _vp, _vp, _f, _f, _vp]
             lib.xr_world_set_hull.restype = _i
+            lib.xr_world_set_sensor.restype = _i
+            lib.xr_world_set_sensor.argtypes = [_vp, _i]
             lib.xr_world_set_motion.restype = _i
             lib.xr_world_set_motion.argtypes = [_vp, _i, _i, _f, _vp, _vp, _vp, _vp]
             lib.xr_body_velocity_deltas.restype = _i
@@ -406,6 +422,8 @@ class Restatement:
             if self.lib.xr_world_set_hull(self.h, i, _ptr(a[0]), len(a[0]), _ptr(a[1]), len(a[1]), _ptr(a[2]),
                                           _ptr(a[3]), _ptr(a[4]), len(a[4])) != 0:
                 raise RuntimeError("invalid convex hull data")
+        if i >= 0 and c.is_sensor:
+            self.lib.xr_world_set_sensor(self.h, i)
         if i >= 0 and c.motion_type != 0:
             f = lambda t: np.array(t, np.float32)
             if self.lib.xr_world_set_motion(self.h, i, int(c.motion_type), float(c.inv_mass),

@@ -281,6 +281,7 @@ struct Collider
 	int motion = 0;
 	float inv_mass = 0, inv_inertia_diag[3] = { 0, 0, 0 }, inertia_rot[4] = { 0, 0, 0, 1 };
 	float lin_vel[3] = { 0, 0, 0 }, ang_vel[3] = { 0, 0, 0 };
+	int is_sensor = 0;     // sensors never collide (DetermineCollidingShapes keeps them apart)
 };
 
 template <class F> struct Leaf
@@ -946,7 +947,7 @@ template <class F> void StepBody(World<F> &w, Body<F> &b, F step_dt)
 	for (const Collider &c : w.colliders)
 	{
 		Box3<F> cb{ { F(c.bounds[0]), F(c.bounds[1]), F(c.bounds[2]) }, { F(c.bounds[3]), F(c.bounds[4]), F(c.bounds[5]) } };
-		if (!Overlaps(wb, cb))
+		if (c.is_sensor || !Overlaps(wb, cb))
 			continue;
 		F q[4] = { F(c.rot[0]), F(c.rot[1]), F(c.rot[2]), F(c.rot[3]) };
 		M44<F> ccom = RotationTranslation(q, V3<F>{ F(c.pos[0]), F(c.pos[1]), F(c.pos[2]) });
@@ -1263,6 +1264,17 @@ int xr_body_write(void *wp, int body, const double *pos, const double *vel)
 {
 	XrWorld *w = (XrWorld *)wp;
 	return w->precision == 32 ? WriteBody<float>(w, body, pos, vel) : WriteBody<double>(w, body, pos, vel);
+}
+
+// Marks collider `collider` as a sensor: it takes no part in the collision.
+int xr_world_set_sensor(void *wp, int collider)
+{
+	XrWorld *w = (XrWorld *)wp;
+	if (collider < 0 || size_t(collider) >= w->f.colliders.size())
+		return -1;
+	w->f.colliders[collider].is_sensor = 1;
+	w->d.colliders[collider].is_sensor = 1;
+	return 0;
 }
 
 // Motion of collider `collider`: EMotionType (0 static, 1 kinematic, 2 dynamic) and, for a

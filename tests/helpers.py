@@ -43,6 +43,7 @@ def run_gpu(scene, settings_list, bounds=None, frames_to_keep=(1,), device=0):
         out[f] = [ctx.read(i) for i in ids]
         for i, r in zip(ids, out[f]):
             r["contact_collider"], r["contact_point"] = ctx.contacts(i)
+            r["sensor_hits"] = ctx.sensor_contacts(i)
     ctx.close()
     return out
 
@@ -107,6 +108,7 @@ def reference_run(scene, frames_to_keep, variant="nofma", num_threads=0, record_
         if record_contacts:
             for i, r in enumerate(out[f]):
                 r["contact_collider"], r["contact_point"] = world.contacts(i)
+                r["sensor_hits"] = world.sensor_contacts(i)
     world.close()
     return settings, bounds, out
 

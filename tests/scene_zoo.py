@@ -177,6 +177,8 @@ ALL = {
     "cloth_cylinder_tapered": cloth_cylinder_tapered,
     "cloth_hulls": cloth_hulls,
     "cloth_hulls_streamed": cloth_hulls_streamed,
+    "cloth_sensors": lambda: cloth_sensors(),
+    "cloth_sensors_streamed": lambda: cloth_sensors(n=120, frames=6, spacing=0.01),   # 14,400 vertices
     "block_lra_boxes": block_lra_boxes,
     "block_lra_boxes_streamed": block_lra_boxes_streamed,
 }
@@ -268,3 +270,15 @@ def skinned_cloth(n=16, frames=12, spacing=0.05):
         return np.asarray(out, np.float32)
 
     return sc, joint_matrices
+
+
+def cloth_sensors(n=30, frames=24, spacing=0.04):
+    """n x n cloth falling through two sensors (a sphere it passes, a box it never reaches)
+    onto a static box: the sensors detect the vertices and never collide."""
+    v, f = scenes.cloth(n, spacing, 60.0, jitter=1e-4, seed=41)
+    mesh = MeshSpec(vertices=v, faces=f, attrs=scenes.attrs(1e-6, 1e-6, 1e-4), bend_type=BEND_DIHEDRAL)
+    body = BodySpec(mesh=mesh, position=(0.0, 0.0, 0.5), params=BodyParams(num_iterations=5, allow_sleeping=False))
+    cols = _bounded([Collider(SHAPE_SPHERE, (0.2,), pos=(0.1, 0.0, 0.3), is_sensor=True),
+                     Collider(SHAPE_BOX, (0.2, 0.2, 0.05), pos=(0.0, 0.0, 0.0), friction=0.5),
+                     Collider(SHAPE_BOX, (0.1, 0.1, 0.1), pos=(2.0, 2.0, 0.5), is_sensor=True)])
+    return Scene("cloth_sensors", [body], cols, gravity=(0.0, 0.0, -9.81), dt=1 / 60, collision_steps=1, frames=frames)

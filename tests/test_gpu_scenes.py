@@ -56,6 +56,7 @@ def test_scene_bit_exact(name):
             assert float_equal(g["body_pos"], r["body_pos"]), f"{name}: body position differs at frame {f}, body {i}"
             # the contact listener's manifold of the frame's last collision step
             _check_contact_bodies(sc, g["contact_collider"], r["contact_collider"], f"{name} frame {f} body {i}")
+            assert (g["sensor_hits"] == r["sensor_hits"]).all(), f"{name}: sensor contacts differ at frame {f}, body {i}"
             hit = r["contact_collider"] >= 0
             assert float_equal(g["contact_point"][hit], r["contact_point"][hit]), \
                 f"{name}: contact points differ at frame {f}, body {i}"
