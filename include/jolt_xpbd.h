@@ -100,13 +100,19 @@ typedef struct xpbd_settings_desc {
      * vertex, 4 x {invBindIndex, weight}, maxDistance, backStopDistance,
      * backStopRadius, normalInfo), at most one per vertex */
     uint32_t num_skinned;
-    /* must be 0: Cosserat rods are not part of this build */
+    /* Cosserat rods (SoftBodySharedSettings.h:305-342), arrays in rod_stretch_shear /
+     * rod_bend_twist below, after CalculateRodProperties and Optimize */
     uint32_t num_rod_stretch_shear;
     uint32_t num_rod_bend_twist;
     const void *skinned;
     const uint32_t *skinned_normals;    uint32_t num_skinned_normals;   /* mSkinnedConstraintNormals */
     const void *inv_bind_matrices;      uint32_t num_inv_bind_matrices; /* InvBind records (80 B: jointIndex,
                                                                            12 B padding, Mat44 column major) */
+    /* RodStretchShear records (48 B: vertex[2], length, invMass, compliance, 12 B padding,
+     * bishop quaternion xyzw) and RodBendTwist records (32 B: rod[2], compliance, 4 B
+     * padding, omega0 quaternion xyzw), in the reference's memory layout */
+    const void *rod_stretch_shear;
+    const void *rod_bend_twist;
 } xpbd_settings_desc;
 
 int xpbd_settings_create(xpbd_context *ctx, const xpbd_settings_desc *desc, xpbd_settings **out_settings);
@@ -260,6 +266,15 @@ int xpbd_body_write(xpbd_context *ctx, int32_t body, const float *pos, const flo
 /* Body summary: out[0..2] linear velocity, out[3..5] angular velocity (world),
  * out[6..11] local bounds, out[12] active flag, out[13] sleep timer. */
 int xpbd_body_state(xpbd_context *ctx, int32_t body, float out[14]);
+
+/*
+ * Cosserat rod states of a body: SoftBodyMotionProperties::GetRodRotation /
+ * GetRodAngularVelocity (SoftBodyMotionProperties.h:64-65) for all rods at once.
+ * rotations: num_rods x 4 (xyzw, relative to the centre of mass transform);
+ * angular_velocities: num_rods x 3; either may be NULL.  num_rods must equal the
+ * settings' rod stretch shear constraints.
+ */
+int xpbd_body_rod_states(xpbd_context *ctx, int32_t body, uint32_t num_rods, float *rotations, float *angular_velocities);
 
 /*
  * What soft body `body` did to the dynamic colliders during the last xpbd_update

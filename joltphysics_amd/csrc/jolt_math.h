@@ -163,6 +163,45 @@ XHD void get_quaternion(const M44 &m, float q[4])
     }
 }
 
+// Quaternion {x, y, z, w} with the SSE build's operation order (Quat.inl, Vec4.inl).
+struct Q4 { float x, y, z, w; };
+
+XHD Q4 q4(float x, float y, float z, float w) { Q4 r; r.x = x; r.y = y; r.z = z; r.w = w; return r; }
+XHD Q4 operator+(Q4 a, Q4 b) { return q4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+XHD Q4 operator-(Q4 a, Q4 b) { return q4(a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w); }
+XHD Q4 operator*(Q4 a, float s) { return q4(a.x * s, a.y * s, a.z * s, a.w * s); }        // Quat.h:157-158
+XHD Q4 operator/(Q4 a, float s) { return q4(a.x / s, a.y / s, a.z / s, a.w / s); }        // Vec4.inl:602-606
+// Conjugated: sign flips of x, y, z (Quat.h:185)
+XHD Q4 conjugated(Q4 a) { return q4(-a.x, -a.y, -a.z, a.w); }
+// Vec4::Dot = ReduceSum of the products: (x + y) + (z + w) (Vec4.inl:988-1010)
+XHD float dot4(Q4 a, Q4 b) { return (a.x * b.x + a.y * b.y) + (a.z * b.z + a.w * b.w); }
+// Normalized = v / Length, a division per component (Vec4.inl:1022-1030)
+XHD Q4 normalized(Q4 a) { return a / sqrtf(dot4(a, a)); }
+// Quat * Quat without FMA (Quat.inl:7-48):
+// [(aw+bz)+(dx-cy), (bw+cx)+(dy-az), (cw+ay)+(dz-bx), -(ax+by)+(dw-cz)] for lhs {a,b,c,d}, rhs {x,y,z,w}
+XHD Q4 quat_mul(Q4 l, Q4 r)
+{
+    return q4((l.x * r.w + l.y * r.z) + (l.w * r.x - l.z * r.y),
+              (l.y * r.w + l.z * r.x) + (l.w * r.y - l.x * r.z),
+              (l.z * r.w + l.x * r.y) + (l.w * r.z - l.y * r.x),
+              -(l.x * r.x + l.y * r.y) + (l.w * r.w - l.z * r.z));
+}
+// sMultiplyImaginary: the quaternion {v, 0} times q (Quat.inl:90-118):
+// [(aw+bz)-cy, (bw+cx)-az, (cw+ay)-bx, -(ax+by)-cz]
+XHD Q4 mul_imaginary(V3 v, Q4 q)
+{
+    return q4((v.x * q.w + v.y * q.z) - v.z * q.y,
+              (v.y * q.w + v.z * q.x) - v.x * q.z,
+              (v.z * q.w + v.x * q.y) - v.y * q.x,
+              -(v.x * q.x + v.y * q.y) - v.z * q.z);
+}
+// RotateAxisZ: q * (0, 0, 1) written out (Quat.inl:434-440)
+XHD V3 rotate_axis_z(Q4 q)
+{
+    const float tz = q.z + q.z, tw = q.w + q.w;
+    return v3(tz * q.x + tw * q.y, tz * q.y + -(tw * q.x), (tz * q.z + tw * q.w) - 1.0f);
+}
+
 // Plane (Geometry/Plane.h): normal + constant
 struct Pl { V3 n; float c; };
 // sFromPointAndNormal (Plane.h:21)

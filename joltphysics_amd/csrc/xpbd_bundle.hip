@@ -392,7 +392,7 @@ __device__ __forceinline__ void bundle_update(
                 const bool last = it + 1 == iterations;
 
                 // ---- merged level program of all bodies (skin records carry their entries)
-                const SkinCtx sk{ G.skin, 0u, (float)(it + 1) / (float)iterations };
+                const RecordCtx sk{ G.skin, 0u, (float)(it + 1) / (float)iterations };
                 if (has_sleepers)
                 {
                     auto skip = [&](const uint4 &r) { return r.x == kNopRecord || !st[vb[r.x & 0x3fffu]].active; };

@@ -235,16 +235,21 @@ __device__ __forceinline__ void project_lra(const P &X, uint32_t i0, uint32_t i1
 // skinned position, normal, and {max distance multiplier, back stop radius, skin
 // constraints enabled, previous position valid}.  `factor` = (iteration + 1) /
 // iterations, the reference's mNextIteration / mNumIterations.
-struct SkinCtx
+struct RecordCtx
 {
     const float4 *skin;
     uint32_t base;              // added to the record's entry index
     float factor;
+    // Cosserat rods of the body (null without rods): states (2 float4 per rod) and the
+    // settings' rod and bend-twist constants
+    float4 *rods;
+    const float4 *rod_const;
+    const float4 *bt_const;
 };
 
 template <class P>
 __device__ __forceinline__ void project_skin(const P &X, uint32_t i, uint32_t g, float max_distance_setting,
-                                             float back_stop_distance, const SkinCtx &sk)
+                                             float back_stop_distance, const RecordCtx &sk)
 {
     const float4 *e = sk.skin + 4u * (size_t)g;
     const float4 q = e[3];
@@ -285,13 +290,65 @@ __device__ __forceinline__ void project_skin(const P &X, uint32_t i, uint32_t g,
     pset(X, i, f4(x, p.w));
 }
 
+// ApplyRodStretchShearConstraints body for rod r between vertices i0, i1
+// (SoftBodyMotionProperties.cpp:603-635; Kugelstadt and Schoemer 2016, eq. 37).
+template <class P>
+__device__ __forceinline__ void project_rod_stretch_shear(const P &X, uint32_t i0, uint32_t i1, uint32_t r, float inv_dt_sq,
+                                                          const RecordCtx &c)
+{
+    const float4 k = c.rod_const[r];                // length, inverse mass, compliance
+    const float4 p0 = pget(X, i0), p1 = pget(X, i1);
+    const float denom = ((p0.w + p1.w) + (4.0f * k.y) * (k.x * k.x)) + k.z * inv_dt_sq;
+    if (denom < 1.0e-12f)
+        return;
+    const float4 s = c.rods[2u * r];
+    Q4 rotation = q4(s.x, s.y, s.z, s.w);
+    const V3 x0 = xyz(p0), x1 = xyz(p1);
+    const V3 delta = ((x1 - x0) - rotate_axis_z(rotation) * k.x) / denom;
+    pset(X, i0, f4(x0 + p0.w * delta, p0.w));
+    pset(X, i1, f4(x1 - p1.w * delta, p1.w));
+    // q * e3_bar = [-qy, qx, -qw, qz]
+    const Q4 q_e3_bar = q4(-rotation.y, rotation.x, -rotation.w, rotation.z);
+    rotation = rotation + mul_imaginary(delta, q_e3_bar) * ((2.0f * k.y) * k.x);
+    const Q4 n = normalized(rotation);
+    c.rods[2u * r] = make_float4(n.x, n.y, n.z, n.w);
+}
+
+// ApplyRodBendTwistConstraints body for bend-twist constraint b
+// (SoftBodyMotionProperties.cpp:637-677; Kugelstadt and Schoemer 2016, eq. 40).
+__device__ __forceinline__ void project_rod_bend_twist(uint32_t b, float inv_dt_sq, const RecordCtx &c)
+{
+    const float4 o = c.bt_const[2u * b], kb = c.bt_const[2u * b + 1u];
+    const uint32_t r1 = __float_as_uint(kb.x), r2 = __float_as_uint(kb.y);
+    const float denom = (c.rod_const[r1].y + c.rod_const[r2].y) + kb.z * inv_dt_sq;
+    if (denom < 1.0e-12f)
+        return;
+    const float4 s1 = c.rods[2u * r1], s2 = c.rods[2u * r2];
+    Q4 rotation1 = q4(s1.x, s1.y, s1.z, s1.w), rotation2 = q4(s2.x, s2.y, s2.z, s2.w);
+    const Q4 omega = quat_mul(conjugated(rotation1), rotation2);
+    const Q4 omega0 = q4(o.x, o.y, o.z, o.w);
+    const Q4 minus = omega - omega0, plus = omega + omega0;
+    // the shorter of the two rotations
+    Q4 delta_omega = dot4(plus, plus) < dot4(minus, minus) ? plus : minus;
+    delta_omega = delta_omega / denom;
+    delta_omega.w = 0.0f;       // the Darboux vector's real part does not vanish
+    const Q4 delta_rod2 = quat_mul(rotation1 * c.rod_const[r2].y, delta_omega);
+    rotation1 = rotation1 + quat_mul(rotation2 * c.rod_const[r1].y, delta_omega);
+    rotation2 = rotation2 - delta_rod2;
+    const Q4 n1 = normalized(rotation1), n2 = normalized(rotation2);
+    c.rods[2u * r1] = make_float4(n1.x, n1.y, n1.z, n1.w);
+    c.rods[2u * r2] = make_float4(n2.x, n2.y, n2.z, n2.w);
+}
+
 // Padding record: skipped (levels pad each constraint type to whole wavefronts)
 constexpr uint32_t kNopRecord = 0xffffffffu;
 
 // A record of LRA type whose two vertices are equal is a skinned constraint: vertex 0,
-// y = its skin entry (before `base`), z = max distance, w = back stop distance.
-template <class P>
-__device__ __forceinline__ void project_record(const P &X, uint4 r, float inv_dt_sq, const SkinCtx &sk)
+// y = its skin entry (before `base`), z = max distance, w = back stop distance.  Bits 31
+// and 30 of y mark rod stretch-shear and bend-twist records (their states live in
+// global memory, so their levels end in sync_global()).
+template <bool RODS, class P>
+__device__ __forceinline__ void project_record(const P &X, uint4 r, float inv_dt_sq, const RecordCtx &sk)
 {
     if (r.x == kNopRecord)
         return;
@@ -304,7 +361,11 @@ __device__ __forceinline__ void project_record(const P &X, uint4 r, float inv_dt
     case kTypeBend: project_bend_pair(X, i0, i1, i2, i3, p0, p1, inv_dt_sq, ((r.y >> 15) & 1u) != 0); break;
     case kTypeVolume: project_volume(X, i0, i1, i2, i3, p0, p1, inv_dt_sq); break;
     default:
-        if (i0 == i1)
+        if (RODS && (r.y & kRodStretchShear))
+            project_rod_stretch_shear(X, i0, i1, r.y & kRodIndexMask, inv_dt_sq, sk);
+        else if (RODS && (r.y & kRodBendTwist))
+            project_rod_bend_twist(r.y & kRodIndexMask, inv_dt_sq, sk);
+        else if (i0 == i1)
             project_skin(X, i0, r.y + sk.base, p0, p1, sk);
         else
             project_lra(X, i0, i1, p0);
@@ -381,7 +442,7 @@ struct NoSkip
 // read back by the same thread in another.
 template <int BLOCK, bool REVERSE, class Handle>
 __device__ __forceinline__ void run_program_global(const uint4 *recs, const uint32_t *lvl, uint32_t nlev, uint32_t tid,
-                                                   Handle handle)
+                                                   Handle handle, bool gsync = false)
 {
     recs = uniform_ptr(recs);
     lvl = uniform_ptr(lvl);
@@ -421,7 +482,10 @@ __device__ __forceinline__ void run_program_global(const uint4 *recs, const uint
         for (uint32_t i = lo + tid + BLOCK; i < hi; i += BLOCK)
             handle(as_uint4(load_record(recs, i)), i);
         r = fetch(k + 3);
-        __syncthreads();
+        if (gsync)
+            sync_global();
+        else
+            __syncthreads();
     };
     for (uint32_t k = 0; k < nlev; k += 3)
     {
@@ -431,21 +495,24 @@ __device__ __forceinline__ void run_program_global(const uint4 *recs, const uint
     }
 }
 
-template <int BLOCK, class P, class Skip>
+// RODS: the program may hold rod records (compiled in only where bodies with rods run)
+template <int BLOCK, bool RODS = false, class P, class Skip>
 __device__ __forceinline__ void run_levels_global(const uint4 *recs, const uint32_t *lvl, uint32_t nlev, const P &X,
-                                                  float inv_dt_sq, uint32_t tid, Skip skip, const SkinCtx &sk)
+                                                  float inv_dt_sq, uint32_t tid, Skip skip, const RecordCtx &sk,
+                                                  bool gsync = false)
 {
     run_program_global<BLOCK, false>(recs, lvl, nlev, tid, [&](const uint4 &q, uint32_t) {
         if (!skip(q))
-            project_record(X, q, inv_dt_sq, sk);
-    });
+            project_record<RODS>(X, q, inv_dt_sq, sk);
+    }, gsync);
 }
 
 // Level program with records and offsets in LDS: the first record of the next
 // level is read before the barrier (the barrier's LDS wait covers it).
-template <int BLOCK, class P, class Skip>
+template <int BLOCK, bool RODS = false, class P, class Skip>
 __device__ __forceinline__ void run_levels_lds(const uint4 *recs, const uint32_t *lvl, uint32_t nlev, const P &X,
-                                               float inv_dt_sq, uint32_t tid, Skip skip, const SkinCtx &sk)
+                                               float inv_dt_sq, uint32_t tid, Skip skip, const RecordCtx &sk,
+                                               bool gsync = false)
 {
     if (nlev == 0)
         return;
@@ -456,15 +523,18 @@ __device__ __forceinline__ void run_levels_lds(const uint4 *recs, const uint32_t
     {
         const uint32_t next_hi = lvl[L + 2 <= nlev ? L + 2 : nlev];
         if (lo + tid < hi && !skip(r))
-            project_record(X, r, inv_dt_sq, sk);
+            project_record<RODS>(X, r, inv_dt_sq, sk);
         for (uint32_t i = lo + tid + BLOCK; i < hi; i += BLOCK)
         {
             const uint4 q = recs[i];
             if (!skip(q))
-                project_record(X, q, inv_dt_sq, sk);
+                project_record<RODS>(X, q, inv_dt_sq, sk);
         }
         const uint4 rn = hi + tid < next_hi ? recs[hi + tid] : nop;
-        __syncthreads();
+        if (gsync)
+            sync_global();
+        else
+            __syncthreads();
         r = rn;
         lo = hi;
         hi = next_hi;

@@ -68,19 +68,20 @@ __host__ __device__ inline LdsLayout lds_layout(uint32_t xcount, uint32_t nrec, 
 // global memory: each thread holds its record of the next two levels in
 // registers (issued two levels ahead), so the load latency overlaps two levels
 // of projection.
-template <bool STAGED, int BLOCK>
+template <bool STAGED, int BLOCK, bool RODS>
 __device__ __forceinline__ void run_levels(const uint4 *recs, const uint32_t *lvl, uint32_t nlev, float4 *X, float inv_dt_sq, int tid,
-                                           const SkinCtx &sk)
+                                           const RecordCtx &sk, bool gsync)
 {
     if (STAGED)
-        run_levels_lds<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, (uint32_t)tid, NoSkip(), sk);
+        run_levels_lds<BLOCK, RODS>(recs, lvl, nlev, X, inv_dt_sq, (uint32_t)tid, NoSkip(), sk, gsync);
     else
-        run_levels_global<BLOCK>(recs, lvl, nlev, X, inv_dt_sq, (uint32_t)tid, NoSkip(), sk);
+        run_levels_global<BLOCK, RODS>(recs, lvl, nlev, X, inv_dt_sq, (uint32_t)tid, NoSkip(), sk, gsync);
 }
 
 // ---------------------------------------------------------------- kernel
-// Advances each body of `body_list` by `frames` x `steps` collision steps.
-template <bool RESIDENT, bool STAGED, int kBlock>
+// Advances each body of `body_list` by `frames` x `steps` collision steps.  RODS: the
+// variant for bodies with Cosserat rods.
+template <bool RESIDENT, bool STAGED, int kBlock, bool RODS>
 __global__ __launch_bounds__(kBlock) void update_kernel(
     const BodyDev *__restrict__ bodies, BodyStateDev *__restrict__ states, const uint32_t *__restrict__ body_list,
     const ColliderDev *__restrict__ colliders, uint32_t num_colliders, uint32_t max_leaves,
@@ -113,10 +114,13 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
     float *red = reinterpret_cast<float *>(lds_raw + lay.red);
     // Per-vertex phases address positions here
     float4 *P = RESIDENT ? X : B.x;
-    // barrier after a per-vertex phase: streamed positions are handed between
-    // wavefronts through global memory
+    // Cosserat rods: their states stay in global memory (2 float4 per rod)
+    const uint32_t num_rods = RODS && B.rods != nullptr ? S.num_rods : 0u;
+    const bool has_rods = num_rods > 0;
+    // barrier after a per-vertex phase: streamed positions (and rod states) are handed
+    // between wavefronts through global memory
     auto sync_positions = [&]() {
-        if (RESIDENT)
+        if (RESIDENT && !has_rods)
             __syncthreads();
         else
             sync_global();
@@ -363,10 +367,42 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                 B.v[i] = f4(v, 0.0f);
             };
 
+            // IntegratePositions, rod orientations (SoftBodyMotionProperties.cpp:352-364): the
+            // angular velocity is damped and integrated; the slot keeps the previous rotation
+            const float half_dt = 0.5f * dt_sub;
+            auto integrate_rod = [&](uint32_t r, Q4 rotation, V3 angular_velocity) {
+                angular_velocity = angular_velocity * damping;
+                const Q4 delta_rotation = mul_imaginary(angular_velocity, rotation) * half_dt;
+                B.rods[2u * r + 1u] = make_float4(rotation.x, rotation.y, rotation.z, rotation.w);
+                const Q4 n = normalized(rotation + delta_rotation);
+                B.rods[2u * r] = make_float4(n.x, n.y, n.z, n.w);
+            };
+            // ApplyCollisionConstraintsAndUpdateVelocities, rods (SoftBodyMotionProperties.cpp:821-824):
+            // the angular velocity from the rotation change, then the next sub-step's integration
+            const float two_div_dt = 2.0f / dt_sub;
+            auto update_rods = [&](bool integrate_next) {
+                for (uint32_t r = tid; r < num_rods; r += kBlock)
+                {
+                    const float4 s = B.rods[2u * r], p = B.rods[2u * r + 1u];
+                    const Q4 rotation = q4(s.x, s.y, s.z, s.w);
+                    const Q4 d = quat_mul(rotation, conjugated(q4(p.x, p.y, p.z, p.w)));
+                    const V3 angular_velocity = two_div_dt * v3(d.x, d.y, d.z);
+                    if (integrate_next)
+                        integrate_rod(r, rotation, angular_velocity);
+                    else
+                        B.rods[2u * r + 1u] = make_float4(angular_velocity.x, angular_velocity.y, angular_velocity.z, 0.0f);
+                }
+            };
+
             if (has_pressure)
                 apply_pressure();
             for (uint32_t i = tid; i < nv; i += kBlock)
                 integrate_vertex(i, P[i], xyz(B.v[i]));
+            for (uint32_t r = tid; r < num_rods; r += kBlock)
+            {
+                const float4 s = B.rods[2u * r], w = B.rods[2u * r + 1u];
+                integrate_rod(r, q4(s.x, s.y, s.z, s.w), v3(w.x, w.y, w.z));
+            }
             sync_positions();
 
             const float restitution_threshold = -2.0f * len(g_local) * dt_sub;
@@ -390,8 +426,10 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                             X[j] = B.x[S.vlist[W.vbegin + j]];
                         __syncthreads();
                     }
-                    run_levels<STAGED, kBlock>(recs, lvl + W.lbegin, W.nlev, X, inv_dt_sq, tid,
-                                               SkinCtx{ B.skin, B.skin_base, (float)(it + 1) / (float)iterations });
+                    run_levels<STAGED, kBlock, RODS>(recs, lvl + W.lbegin, W.nlev, X, inv_dt_sq, tid,
+                                               RecordCtx{ B.skin, B.skin_base, (float)(it + 1) / (float)iterations,
+                                                          B.rods, S.rod_const, S.bt_const },
+                                               has_rods);
                     if (!RESIDENT && W.vcount)
                     {
                         for (uint32_t j = tid; j < W.vcount; j += kBlock)
@@ -506,6 +544,8 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                         }
                     sync_global();
                 }
+                if (has_rods)
+                    update_rods(!last);
                 sync_positions();
                 if (!last && has_pressure)
                 {
@@ -632,7 +672,7 @@ uint32_t kernel_lds_bytes(bool resident, bool staged, const SettingsDev &s, uint
                       staged ? s.num_levels_total : 0u, max_leaves, nc, nf).total;
 }
 
-template <bool R, bool S, int BLOCK>
+template <bool R, bool S, int BLOCK, bool RODS>
 static hipError_t launch_variant(const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list, uint32_t num_blocks,
                                  const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves, uint32_t lds_bytes,
                                  float gx, float gy, float gz, float dt, int32_t steps, int32_t frames, int32_t *error_flag,
@@ -641,30 +681,42 @@ static hipError_t launch_variant(const BodyDev *bodies, BodyStateDev *states, co
     static bool configured = false;
     if (!configured)
     {
-        hipError_t e = hipFuncSetAttribute((const void *)update_kernel<R, S, BLOCK>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+        hipError_t e = hipFuncSetAttribute((const void *)update_kernel<R, S, BLOCK, RODS>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
         if (e != hipSuccess)
             return e;
         configured = true;
     }
-    hipLaunchKernelGGL((update_kernel<R, S, BLOCK>), dim3(num_blocks), dim3(BLOCK), lds_bytes, stream,
+    hipLaunchKernelGGL((update_kernel<R, S, BLOCK, RODS>), dim3(num_blocks), dim3(BLOCK), lds_bytes, stream,
                        bodies, states, body_list, colliders, num_colliders, max_leaves, gx, gy, gz, dt, steps, frames, error_flag);
     return hipGetLastError();
 }
 
+template <bool RODS>
+static hipError_t launch_update_rods(bool resident, bool staged, const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list,
+                                     uint32_t num_blocks, const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves,
+                                     uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
+                                     int32_t *error_flag, hipStream_t stream)
+{
+    if (resident && staged)
+        return launch_variant<true, true, 256, RODS>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
+    if (resident)
+        return launch_variant<true, false, 256, RODS>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
+    return launch_variant<false, false, 1024, RODS>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
+}
+
 // Resident single-body launches use 256-thread workgroups; streamed bodies use
 // 1024 threads so one pass covers the wide merged levels of an 8192-vertex window.
-hipError_t launch_update(bool resident, bool staged, const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list,
+// Bodies with Cosserat rods run in the variants that hold the rod code.
+hipError_t launch_update(bool resident, bool staged, bool rods, const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list,
                          uint32_t num_blocks, const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves,
                          uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
                          int32_t *error_flag, hipStream_t stream)
 {
     if (num_blocks == 0)
         return hipSuccess;
-    if (resident && staged)
-        return launch_variant<true, true, 256>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
-    if (resident)
-        return launch_variant<true, false, 256>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
-    return launch_variant<false, false, 1024>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
+    if (rods)
+        return launch_update_rods<true>(resident, staged, bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
+    return launch_update_rods<false>(resident, staged, bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
 }
 
 } // namespace xpbd

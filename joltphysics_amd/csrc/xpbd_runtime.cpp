@@ -24,7 +24,7 @@ hipError_t launch_bundles(bool staged, uint32_t block, bool uniform, const Bundl
                           const BodyDev *bodies, BodyStateDev *states, const ColliderDev *colliders, uint32_t num_colliders,
                           uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
                           hipStream_t stream);
-hipError_t launch_update(bool resident, bool staged, const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list,
+hipError_t launch_update(bool resident, bool staged, bool rods, const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list,
                          uint32_t num_blocks, const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves,
                          uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
                          int32_t *error_flag, hipStream_t stream);
@@ -73,6 +73,10 @@ struct SkinWeightRec { uint32_t inv_bind; float weight; };
 struct SkinnedRec { uint32_t vertex; SkinWeightRec w[4]; float max_distance, back_stop_distance, back_stop_radius; uint32_t normal_info; };
 struct InvBindRec { uint32_t joint; uint32_t pad[3]; float m[16]; };
 static_assert(sizeof(SkinnedRec) == 52 && sizeof(InvBindRec) == 80, "skin record size");
+// Cosserat rods as the reference holds them (SoftBodySharedSettings.h:312-342; Quat is 16-byte aligned)
+struct RodSSRec { uint32_t v[2]; float length, inv_mass, compliance; uint32_t pad[3]; float bishop[4]; };
+struct RodBTRec { uint32_t rod[2]; float compliance; uint32_t pad; float omega0[4]; };
+static_assert(sizeof(RodSSRec) == 48 && sizeof(RodBTRec) == 32, "rod record size");
 static_assert(sizeof(VertexRec) == 28 && sizeof(FaceRec) == 16 && sizeof(EdgeRec) == 16, "record size");
 static_assert(sizeof(BendRec) == 24 && sizeof(VolumeRec) == 24 && sizeof(LRARec) == 12 && sizeof(GroupRec) == 28, "record size");
 
@@ -125,6 +129,9 @@ struct xpbd_settings
     std::vector<uint32_t> skinned_normals;
     std::vector<InvBindRec> inv_binds;
     std::vector<FaceRec> face_recs;
+    // Cosserat rods: host copy (Bishop frames for body creation) and device constants
+    std::vector<RodSSRec> rods;
+    DevBuf<float4> rod_const, bt_const;
     bool uniform_mass = false;                  // every vertex has the same inverse mass
     float inv_mass = 0.0f;                      // that inverse mass
     uint32_t num_edges = 0, num_bends = 0, num_volumes = 0, num_lras = 0;
@@ -137,6 +144,7 @@ struct Body
     bool owns_storage = true;       // false once the state lives in a bundle's arrays
     DevBuf<float4> x, v, prev, plane, report;
     DevBuf<int32_t> idx, contact;
+    DevBuf<float4> rods;            // 2 per rod (the body keeps them; rod bodies never join a bundle)
     BodyDev dev{};
     BodyStateDev state{};
     bool skinned = false;           // has skin entries at dev.skin_base (one per vertex)
@@ -243,16 +251,22 @@ namespace {
 // so every pair that shares a vertex with at least one writer keeps its
 // order, and the constraints of one level have disjoint write sets that no
 // other constraint of the level reads.
-struct Item { uint32_t type; uint32_t v[4]; uint32_t nv; float p0, p1; uint32_t index; };   // index: in its type's array
+struct Item { uint32_t type; uint32_t v[4]; uint32_t nv; float p0, p1; uint32_t index; uint32_t rod[2]; uint32_t nrod; };   // index: in its type's array
 // Skinned constraint item (host only): packed as an LRA record of vertex 0 to itself
 constexpr uint32_t kItemSkin = 4;
+// Cosserat rod items (host only): packed as LRA-type records flagged in y.  Their rod
+// states are resources of the schedule besides the vertices (Item::rod).
+constexpr uint32_t kItemRodSS = 5, kItemRodBT = 6;
 
 // Bodies up to this many vertices keep all positions in LDS for the launch.
 constexpr uint32_t kResidentMaxVertices = 8192;
 // Vertex capacity of one gathered window of a streamed body.
 constexpr uint32_t kWindowCapacity = 8192;
 
-std::vector<std::vector<uint32_t>> LevelGroup(const std::vector<Item> &seq, std::vector<uint32_t> &last_write, std::vector<uint32_t> &last_read)
+// Rod states are read and written by every rod item that names them: rod_last keeps
+// the level of their last use.
+std::vector<std::vector<uint32_t>> LevelGroup(const std::vector<Item> &seq, std::vector<uint32_t> &last_write, std::vector<uint32_t> &last_read,
+                                              std::vector<uint32_t> &rod_last)
 {
     std::vector<uint32_t> level(seq.size());
     uint32_t nl = 0;
@@ -261,6 +275,8 @@ std::vector<std::vector<uint32_t>> LevelGroup(const std::vector<Item> &seq, std:
         const Item &it = seq[k];
         const bool lra = it.type == kTypeLRA;
         uint32_t L = 0;
+        for (uint32_t j = 0; j < it.nrod; ++j)
+            L = std::max(L, rod_last[it.rod[j]]);
         for (uint32_t j = 0; j < it.nv; ++j)
         {
             uint32_t v = it.v[j];
@@ -270,6 +286,8 @@ std::vector<std::vector<uint32_t>> LevelGroup(const std::vector<Item> &seq, std:
         }
         ++L;
         level[k] = L;
+        for (uint32_t j = 0; j < it.nrod; ++j)
+            rod_last[it.rod[j]] = L;
         for (uint32_t j = 0; j < it.nv; ++j)
         {
             uint32_t v = it.v[j];
@@ -283,8 +301,12 @@ std::vector<std::vector<uint32_t>> LevelGroup(const std::vector<Item> &seq, std:
     for (size_t k = 0; k < seq.size(); ++k)
         out[level[k] - 1].push_back((uint32_t)k);
     for (const Item &it : seq)
+    {
         for (uint32_t j = 0; j < it.nv; ++j)
             last_write[it.v[j]] = last_read[it.v[j]] = 0;
+        for (uint32_t j = 0; j < it.nrod; ++j)
+            rod_last[it.rod[j]] = 0;
+    }
     return out;
 }
 
@@ -292,6 +314,14 @@ uint4 Pack(const Item &it, const uint32_t *local)
 {
     auto m = [&](uint32_t j) { return j < it.nv ? (local ? local[it.v[j]] : it.v[j]) : 0u; };
     uint4 r;
+    if (it.type == kItemRodSS || it.type == kItemRodBT)
+    {
+        // stretch-shear: vertices 0, 1 and the rod; bend-twist: no vertices, the constraint
+        r.x = it.type == kItemRodSS ? ((m(0) & 0x3fffu) | (kTypeLRA << 14) | (m(1) << 16)) : (kTypeLRA << 14);
+        r.y = (it.type == kItemRodSS ? kRodStretchShear : kRodBendTwist) | it.index;
+        r.z = r.w = 0u;
+        return r;
+    }
     if (it.type == kItemSkin)
     {
         // LRA type, vertex 0 twice; y = the body vertex (the skin entry, before the body's base)
@@ -410,7 +440,7 @@ void ProgramBuilder::AppendLevel(const std::vector<std::pair<const GroupLevels *
     for (const auto &gi : lv)
     {
         const Item &it = gi.first->seq[gi.second];
-        const uint32_t t = it.type == kItemSkin ? kTypeLRA : it.type;
+        const uint32_t t = it.type >= kItemSkin ? kTypeLRA : it.type;
         bucket[t].push_back(Pack(it, local));
         ibucket[t].push_back(it.index);
     }
@@ -422,8 +452,18 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
     const uint32_t nv = d.num_vertices;
     if (nv == 0 || d.vertices == nullptr)
         return Fail(XPBD_E_INVALID, "settings have no vertices");
-    if (d.num_rod_stretch_shear || d.num_rod_bend_twist)
-        return Fail(XPBD_E_UNSUPPORTED, "Cosserat rods are not supported by this build");
+    if ((d.num_rod_stretch_shear && !d.rod_stretch_shear) || (d.num_rod_bend_twist && !d.rod_bend_twist))
+        return Fail(XPBD_E_INVALID, "rod constraints without their arrays");
+    if (d.num_rod_stretch_shear > kRodIndexMask || d.num_rod_bend_twist > kRodIndexMask)
+        return Fail(XPBD_E_CAPACITY, "too many rod constraints");
+    const RodSSRec *rods = (const RodSSRec *)d.rod_stretch_shear;
+    const RodBTRec *bts = (const RodBTRec *)d.rod_bend_twist;
+    for (uint32_t k = 0; k < d.num_rod_stretch_shear; ++k)
+        if (rods[k].v[0] >= nv || rods[k].v[1] >= nv || rods[k].v[0] == rods[k].v[1])
+            return Fail(XPBD_E_INVALID, "rod stretch shear constraint vertex out of range or repeated");
+    for (uint32_t k = 0; k < d.num_rod_bend_twist; ++k)
+        if (bts[k].rod[0] >= d.num_rod_stretch_shear || bts[k].rod[1] >= d.num_rod_stretch_shear || bts[k].rod[0] == bts[k].rod[1])
+            return Fail(XPBD_E_INVALID, "rod bend twist constraint rod out of range or repeated");
     if (d.num_skinned && (!d.skinned || (d.num_inv_bind_matrices && !d.inv_bind_matrices)
                           || (d.num_skinned_normals && !d.skinned_normals)))
         return Fail(XPBD_E_INVALID, "skinned constraints without their arrays");
@@ -466,12 +506,12 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
     {
         const GroupRec &c = groups[g];
         if (c.edge_end < prev.edge_end || c.lra_end < prev.lra_end || c.bend_end < prev.bend_end || c.volume_end < prev.volume_end
-            || c.skinned_end < prev.skinned_end || c.rod_ss_end != 0 || c.rod_bt_end != 0)
-            return Fail(XPBD_E_INVALID, "update group end indices are not monotone or reference unsupported constraint types");
+            || c.skinned_end < prev.skinned_end || c.rod_ss_end < prev.rod_ss_end || c.rod_bt_end < prev.rod_bt_end)
+            return Fail(XPBD_E_INVALID, "update group end indices are not monotone");
         prev = c;
     }
     if (prev.edge_end != d.num_edges || prev.lra_end != d.num_lras || prev.bend_end != d.num_dihedral_bends || prev.volume_end != d.num_volumes
-        || prev.skinned_end != d.num_skinned)
+        || prev.skinned_end != d.num_skinned || prev.rod_ss_end != d.num_rod_stretch_shear || prev.rod_bt_end != d.num_rod_bend_twist)
         return Fail(XPBD_E_INVALID, "the last update group does not end at the constraint counts");
 
     auto check = [nv](const uint32_t *v, int n) {
@@ -484,7 +524,7 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
     // Per group: sequence in ProcessGroup order, then levels
     const uint32_t ng = d.num_update_groups;
     std::vector<GroupLevels> gl(ng);
-    std::vector<uint32_t> last_write(nv, 0), last_read(nv, 0);
+    std::vector<uint32_t> last_write(nv, 0), last_read(nv, 0), rod_last(d.num_rod_stretch_shear, 0);
     prev = GroupRec{ 0, 0, 0, 0, 0, 0, 0 };
     for (uint32_t g = 0; g < ng; ++g)
     {
@@ -500,7 +540,8 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
             if (!check(bends[k].v, 4)) return Fail(XPBD_E_INVALID, "bend constraint vertex out of range");
             seq.push_back({ kTypeBend, { bends[k].v[0], bends[k].v[1], bends[k].v[2], bends[k].v[3] }, 4, bends[k].compliance, bends[k].initial_angle, k });
         }
-        // ProcessGroup order: volumes, bends, skinned, edges, LRAs (SoftBodyMotionProperties.cpp:1037-1054)
+        // ProcessGroup order: volumes, bends, skinned, edges, rod stretch-shear, rod bend-twist,
+        // LRAs (SoftBodyMotionProperties.cpp:1037-1054)
         for (uint32_t k = prev.skinned_end; k < c.skinned_end; ++k)
         {
             const SkinnedRec &sr = skinned[k];
@@ -511,6 +552,10 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
             if (!check(edges[k].v, 2)) return Fail(XPBD_E_INVALID, "edge constraint vertex out of range");
             seq.push_back({ kTypeEdge, { edges[k].v[0], edges[k].v[1], 0, 0 }, 2, edges[k].rest, edges[k].compliance, k });
         }
+        for (uint32_t k = prev.rod_ss_end; k < c.rod_ss_end; ++k)
+            seq.push_back({ kItemRodSS, { rods[k].v[0], rods[k].v[1], 0, 0 }, 2, 0.0f, 0.0f, k, { k, 0 }, 1 });
+        for (uint32_t k = prev.rod_bt_end; k < c.rod_bt_end; ++k)
+            seq.push_back({ kItemRodBT, { 0, 0, 0, 0 }, 0, 0.0f, 0.0f, k, { bts[k].rod[0], bts[k].rod[1] }, 2 });
         for (uint32_t k = prev.lra_end; k < c.lra_end; ++k)
         {
             if (!check(lras[k].v, 2)) return Fail(XPBD_E_INVALID, "LRA constraint vertex out of range");
@@ -518,7 +563,7 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
                 continue;       // an LRA of a vertex to itself never moves it (that encoding marks skin records)
             seq.push_back({ kTypeLRA, { lras[k].v[0], lras[k].v[1], 0, 0 }, 2, lras[k].max_distance, 0.0f, k });
         }
-        gl[g].levels = LevelGroup(seq, last_write, last_read);
+        gl[g].levels = LevelGroup(seq, last_write, last_read, rod_last);
         prev = c;
     }
     const uint32_t npar = ng - 1;
@@ -720,7 +765,28 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
     for (uint32_t i = 1; i < nv && s.uniform_mass; ++i)
         s.uniform_mass = memcmp(&vr[i].inv_mass, &s.inv_mass, sizeof(float)) == 0;
 
+    if (d.num_rod_stretch_shear)
+    {
+        std::vector<float4> rc(d.num_rod_stretch_shear), bc(2ull * d.num_rod_bend_twist);
+        for (uint32_t k = 0; k < d.num_rod_stretch_shear; ++k)
+            rc[k] = make_float4(rods[k].length, rods[k].inv_mass, rods[k].compliance, 0.0f);
+        for (uint32_t k = 0; k < d.num_rod_bend_twist; ++k)
+        {
+            float r1, r2;
+            memcpy(&r1, &bts[k].rod[0], 4);
+            memcpy(&r2, &bts[k].rod[1], 4);
+            bc[2 * k] = make_float4(bts[k].omega0[0], bts[k].omega0[1], bts[k].omega0[2], bts[k].omega0[3]);
+            bc[2 * k + 1] = make_float4(r1, r2, bts[k].compliance, 0.0f);
+        }
+        if ((e = s.rod_const.upload(rc)) != hipSuccess || (bc.size() && (e = s.bt_const.upload(bc)) != hipSuccess))
+            return Fail(XPBD_E_DEVICE, std::string("settings upload: ") + hipGetErrorString(e));
+        s.rods.assign(rods, rods + d.num_rod_stretch_shear);
+    }
+
     SettingsDev &sd = s.dev;
+    sd.rod_const = s.rod_const.p;
+    sd.bt_const = s.bt_const.p;
+    sd.num_rods = d.num_rod_stretch_shear;
     sd.recs = s.recs.p;
     sd.lvl = s.lvl.p;
     sd.windows = s.windows.p;
@@ -858,7 +924,8 @@ int BuildLayout(xpbd_context *ctx)
     {
         const Body &b = *ctx->bodies[i];
         const bool fits = bundle_lds_bytes(b.nv, 0, 0, 1, b.settings->uniform_mass) <= kMaxLdsBytes;
-        if (ctx->use_bundles && fits && !b.settings->host_level_begin.empty())
+        // rod states stay in global memory: bodies with rods run in the per-body kernel
+        if (ctx->use_bundles && fits && !b.settings->host_level_begin.empty() && b.settings->rods.empty())
             resident.push_back(i);
         else
         {
@@ -1307,6 +1374,24 @@ int xpbd_body_create(xpbd_context *ctx, const xpbd_body_desc *d, int32_t *out_bo
     dv.s = s->dev;
     dv.x = b->x.p; dv.v = b->v.p; dv.prev = b->prev.p; dv.plane = b->plane.p; dv.idx = b->idx.p; dv.contact = b->contact.p;
     dv.report = b->report.p;
+    if (!s->rods.empty())
+    {
+        // SoftBodyMotionProperties::Initialize (SoftBodyMotionProperties.cpp:83-95): rotation =
+        // creation rotation (when made identity) * Bishop frame, angular velocity zero
+        float rq[4];
+        get_quaternion(rot, rq);
+        const Q4 rotation_q = q4(rq[0], rq[1], rq[2], rq[3]);
+        std::vector<float4> rs(2 * s->rods.size());
+        for (size_t r = 0; r < s->rods.size(); ++r)
+        {
+            const float *bq = s->rods[r].bishop;
+            const Q4 q = quat_mul(rotation_q, q4(bq[0], bq[1], bq[2], bq[3]));
+            rs[2 * r] = make_float4(q.x, q.y, q.z, q.w);
+            rs[2 * r + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        }
+        HIP_TRY(b->rods.upload(rs));
+        dv.rods = b->rods.p;
+    }
     if (!s->skinned.empty())
     {
         // SoftBodyMotionProperties::Initialize: skin states zero / NaN until SkinVertices
@@ -1506,7 +1591,7 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
     }
 
     // Streamed bodies: launch classes (kernel variant x LDS size bin)
-    struct Launch { bool resident, staged; uint32_t lds, max_leaves; std::vector<uint32_t> bodies; };
+    struct Launch { bool resident, staged, rods; uint32_t lds, max_leaves; std::vector<uint32_t> bodies; };
     std::vector<Launch> launches;
     const uint32_t bins[] = { 24 * 1024, 40 * 1024, 64 * 1024, 96 * 1024, kMaxLdsBytes };
     std::vector<uint32_t> order = ctx->streamed;
@@ -1517,6 +1602,7 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
         const SettingsDev &sd = b.dev.s;
         const uint32_t nf = b.dev.pressure > 0.0f ? sd.num_faces : 0u;
         const bool resident = sd.resident != 0;
+        const bool rods = b.dev.rods != nullptr;
         bool staged = false;
         // collider leaves: as many as the LDS left over holds, at most one per collider (a
         // body that overlaps more colliders in one step raises the capacity error)
@@ -1546,11 +1632,11 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
         while (bins[bin] < need) ++bin;
         Launch *L = nullptr;
         for (Launch &l : launches)
-            if (l.resident == resident && l.staged == staged && l.lds == bins[bin] && l.max_leaves == max_leaves)
+            if (l.resident == resident && l.staged == staged && l.rods == rods && l.lds == bins[bin] && l.max_leaves == max_leaves)
                 L = &l;
         if (!L)
         {
-            launches.push_back({ resident, staged, bins[bin], max_leaves, {} });
+            launches.push_back({ resident, staged, rods, bins[bin], max_leaves, {} });
             L = &launches.back();
         }
         L->bodies.push_back(i);
@@ -1645,7 +1731,7 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
     uint32_t offset = 0;
     for (const Launch &l : launches)
     {
-        HIP_TRY(launch_update(l.resident, l.staged, ctx->d_bodies.p, ctx->d_states.p, ctx->d_list.p + offset, (uint32_t)l.bodies.size(),
+        HIP_TRY(launch_update(l.resident, l.staged, l.rods, ctx->d_bodies.p, ctx->d_states.p, ctx->d_list.p + offset, (uint32_t)l.bodies.size(),
                               ctx->d_colliders.p, nc, l.max_leaves, l.lds, gx, gy, gz, dt, collision_steps, frames, ctx->d_error.p, ctx->stream));
         offset += (uint32_t)l.bodies.size();
         double longest = 0.0;
@@ -1786,7 +1872,8 @@ int xpbd_body_state(xpbd_context *ctx, int32_t body, float out[14])
 
 namespace {
 constexpr size_t kStateHeaderBytes = 4 * 12 + 48 + 4 + 1;   // velocities, force, torque, spheres, timer, allow sleeping
-size_t StateBytes(uint32_t nv) { return kStateHeaderBytes + (size_t)nv * 24 + 4 * 12; }
+// header, per vertex position + velocity, per rod rotation (16 B) + angular velocity (12 B), bounds
+size_t StateBytes(uint32_t nv, size_t nrods) { return kStateHeaderBytes + (size_t)nv * 24 + nrods * 28 + 4 * 12; }
 } // namespace
 
 namespace {
@@ -1921,7 +2008,8 @@ int xpbd_body_save_state(xpbd_context *ctx, int32_t body, void *out, size_t capa
     const Body &b = *ctx->bodies[body];
     if (b.skinned)
         return Fail(XPBD_E_UNSUPPORTED, "state streams of skinned bodies (with skin states) are not supported");
-    *size = StateBytes(b.nv);
+    const size_t nrods = b.settings->rods.size();
+    *size = StateBytes(b.nv, nrods);
     if (!out)
         return XPBD_OK;
     if (capacity < *size)
@@ -1952,6 +2040,16 @@ int xpbd_body_save_state(xpbd_context *ctx, int32_t body, void *out, size_t capa
         put(&x[i], 12);
         put(&v[i], 12);
     }
+    if (nrods)
+    {
+        std::vector<float4> rs(2 * nrods);
+        HIP_TRY(hipMemcpy(rs.data(), b.dev.rods, sizeof(float4) * rs.size(), hipMemcpyDeviceToHost));
+        for (size_t r = 0; r < nrods; ++r)
+        {
+            put(&rs[2 * r], 16);
+            put(&rs[2 * r + 1], 12);
+        }
+    }
     put(s.bmin, 12); put(s.bmax, 12); put(s.pmin, 12); put(s.pmax, 12);
     return XPBD_OK;
 }
@@ -1964,7 +2062,8 @@ int xpbd_body_restore_state(xpbd_context *ctx, int32_t body, const void *in, siz
     Body &b = *ctx->bodies[body];
     if (b.skinned)
         return Fail(XPBD_E_UNSUPPORTED, "state streams of skinned bodies (with skin states) are not supported");
-    if (size != StateBytes(b.nv))
+    const size_t nrods = b.settings->rods.size();
+    if (size != StateBytes(b.nv, nrods))
         return Fail(XPBD_E_INVALID, "state stream size does not match the body");
     int rc = xpbd_synchronize(ctx);
     if (rc != XPBD_OK)
@@ -1992,6 +2091,16 @@ int xpbd_body_restore_state(xpbd_context *ctx, int32_t body, const void *in, siz
         x[i] = make_float4(q[0], q[1], q[2], x[i].w);
         v[i] = make_float4(q[3], q[4], q[5], 0.0f);
     }
+    std::vector<float4> rs(2 * nrods);
+    for (size_t r = 0; r < nrods; ++r)
+    {
+        float q[7];
+        get(q, 28);
+        rs[2 * r] = make_float4(q[0], q[1], q[2], q[3]);
+        rs[2 * r + 1] = make_float4(q[4], q[5], q[6], 0.0f);
+    }
+    if (nrods)
+        HIP_TRY(hipMemcpy(b.dev.rods, rs.data(), sizeof(float4) * rs.size(), hipMemcpyHostToDevice));
     get(s.bmin, 12); get(s.bmax, 12); get(s.pmin, 12); get(s.pmax, 12);
     if (position) memcpy(s.pos, position, 12);
     if (rotation) memcpy(s.rot, rotation, 16);
@@ -2045,6 +2154,30 @@ int xpbd_body_sensor_contacts(xpbd_context *ctx, int32_t body, uint32_t num_coll
     {
         const int32_t k = ctx->colliders[c].sensor;
         hits[c] = k >= 0 && (uint32_t)k < row.size() ? row[k] : 0;
+    }
+    return XPBD_OK;
+}
+
+int xpbd_body_rod_states(xpbd_context *ctx, int32_t body, uint32_t num_rods, float *rotations, float *angular_velocities)
+{
+    if (!ctx || body < 0 || body >= (int32_t)ctx->bodies.size())
+        return Fail(XPBD_E_INVALID, "invalid body");
+    const Body &b = *ctx->bodies[body];
+    const size_t n = b.settings->rods.size();
+    if (num_rods != n)
+        return Fail(XPBD_E_INVALID, "num_rods differs from the body's rod stretch shear constraints");
+    int rc = xpbd_synchronize(ctx);
+    if (rc != XPBD_OK)
+        return rc;
+    std::vector<float4> rs(2 * n);
+    if (n)
+        HIP_TRY(hipMemcpy(rs.data(), b.dev.rods, sizeof(float4) * rs.size(), hipMemcpyDeviceToHost));
+    for (size_t r = 0; r < n; ++r)
+    {
+        if (rotations)
+            memcpy(rotations + 4 * r, &rs[2 * r], 16);
+        if (angular_velocities)
+            memcpy(angular_velocities + 3 * r, &rs[2 * r + 1], 12);
     }
     return XPBD_OK;
 }

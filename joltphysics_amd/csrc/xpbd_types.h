@@ -13,7 +13,12 @@ namespace xpbd {
 //   p0, p1: edge {rest, compliance}, bend {compliance, initialAngle},
 //           volume {sixRestVolume, compliance}, LRA {maxDistance, -}
 // Vertex numbers index the LDS window the record runs in (< 16384).
+// Records of LRA type also carry the constraints with state beyond the vertices:
+//   b bit 31 set: Cosserat rod stretch-shear, vertices 0 and 1, b & 0x3fffffff = rod
+//   b bit 30 set: Cosserat rod bend-twist, b & 0x3fffffff = bend-twist constraint
+//   vertex 0 == vertex 1: skinned constraint, b = skin entry
 enum : uint32_t { kTypeVolume = 0, kTypeBend = 1, kTypeEdge = 2, kTypeLRA = 3 };
+constexpr uint32_t kRodStretchShear = 0x80000000u, kRodBendTwist = 0x40000000u, kRodIndexMask = 0x3fffffffu;
 constexpr uint32_t kMaxWindowVertices = 1u << 14;
 
 // A window is a set of vertices held in LDS while a run of levels executes on
@@ -44,6 +49,11 @@ struct SettingsDev
     uint32_t num_levels_total;  // entries of lvl
     uint32_t resident;          // 1: identity windows, positions stay in LDS
     uint32_t window_capacity;   // largest gathered window (vertices)
+    // Cosserat rods (SoftBodySharedSettings.h:305-342): per rod {length, inverse mass,
+    // compliance, 0}; per bend-twist constraint omega0, then {rod 1, rod 2 as bits, compliance, 0}
+    const float4 *rod_const;
+    const float4 *bt_const;
+    uint32_t num_rods;
 };
 
 struct DynStateDev;
@@ -69,6 +79,8 @@ struct BodyDev
     uint32_t skin_base;         // first entry of this body (entry = skin_base + body vertex index)
     int32_t *sensor_hits;       // this body's row: 1 where a sensor detected a vertex in the last step, or null
     uint32_t num_sensors;
+    float4 *rods;               // rod states, 2 per rod: rotation, then the angular velocity (between
+                                // steps) or the previous rotation (during a step); null without rods
 };
 
 // Per-body dynamic state kept on the device between launches.

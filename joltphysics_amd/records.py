@@ -38,6 +38,11 @@ SKINNED = np.dtype([("vertex", "<u4"), ("weights", [("inv_bind", "<u4"), ("weigh
                     ("normal_info", "<u4")])
 INV_BIND = np.dtype([("joint", "<u4"), ("pad", "<u4", 3), ("m", "<f4", 16)])     # m: column major
 assert SKINNED.itemsize == 52 and INV_BIND.itemsize == 80
+# Cosserat rods in the reference's memory layout (SoftBodySharedSettings.h:312-342)
+ROD_SS = np.dtype([("vertex", "<u4", 2), ("length", "<f4"), ("inv_mass", "<f4"), ("compliance", "<f4"),
+                   ("pad", "<u4", 3), ("bishop", "<f4", 4)])
+ROD_BT = np.dtype([("rod", "<u4", 2), ("compliance", "<f4"), ("pad", "<u4"), ("omega0", "<f4", 4)])
+assert ROD_SS.itemsize == 48 and ROD_BT.itemsize == 32
 
 assert VERTEX.itemsize == 28 and FACE.itemsize == 16 and EDGE.itemsize == 16
 assert BEND.itemsize == 24 and VOLUME.itemsize == 24 and LRA.itemsize == 12
@@ -72,13 +77,16 @@ class SettingsArrays:
     skinned: np.ndarray = field(default_factory=lambda: _empty(SKINNED))
     skinned_normals: np.ndarray = field(default_factory=lambda: np.zeros(0, np.uint32))
     inv_binds: np.ndarray = field(default_factory=lambda: _empty(INV_BIND))
+    rod_stretch_shear: np.ndarray = field(default_factory=lambda: _empty(ROD_SS))
+    rod_bend_twist: np.ndarray = field(default_factory=lambda: _empty(ROD_BT))
 
     def counts(self):
         return dict(V=len(self.vertices), F=len(self.faces), E=len(self.edges), B=len(self.bends),
                     T=len(self.volumes), L=len(self.lras), G=len(self.groups))
 
     def num_constraints(self):
-        return len(self.edges) + len(self.bends) + len(self.volumes) + len(self.lras)
+        return (len(self.edges) + len(self.bends) + len(self.volumes) + len(self.lras)
+                + len(self.rod_stretch_shear) + len(self.rod_bend_twist))
 
     def copy(self):
         return SettingsArrays(*(np.array(a, copy=True) for a in (

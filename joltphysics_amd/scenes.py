@@ -44,6 +44,9 @@ class MeshSpec:
     flags: int = CREATE_CONSTRAINTS | OPTIMIZE
     skinned: np.ndarray | None = None           # SKINNED records (normals are calculated)
     inv_binds: np.ndarray | None = None         # INV_BIND records
+    # Cosserat rods: (stretch-shear vertex pairs (n, 2), their compliances (n,), bend-twist
+    # rod pairs (m, 2), their compliances (m,)); CalculateRodProperties fills in the rest
+    rods: tuple | None = None
 
 
 @dataclass

@@ -17,8 +17,8 @@ import sys
 
 import numpy as np
 
-from .records import (BEND, EDGE, FACE, GROUP, INV_BIND, LRA, MOTION_DYNAMIC, SHAPE_CONVEX_HULL, SKINNED, VERTEX,
-                      VOLUME, BodyParams, SettingsArrays)
+from .records import (BEND, EDGE, FACE, GROUP, INV_BIND, LRA, MOTION_DYNAMIC, ROD_BT, ROD_SS, SHAPE_CONVEX_HULL,
+                      SKINNED, VERTEX, VOLUME, BodyParams, SettingsArrays)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libjolt_xpbd.so")
@@ -41,7 +41,8 @@ class _SettingsDesc(C.Structure):
                 ("num_skinned", C.c_uint32), ("num_rod_stretch_shear", C.c_uint32),
                 ("num_rod_bend_twist", C.c_uint32), ("skinned", C.c_void_p),
                 ("skinned_normals", C.c_void_p), ("num_skinned_normals", C.c_uint32),
-                ("inv_bind_matrices", C.c_void_p), ("num_inv_bind_matrices", C.c_uint32)]
+                ("inv_bind_matrices", C.c_void_p), ("num_inv_bind_matrices", C.c_uint32),
+                ("rod_stretch_shear", C.c_void_p), ("rod_bend_twist", C.c_void_p)]
 
 
 class _BodyDesc(C.Structure):
@@ -98,7 +99,7 @@ EXPORTED_SYMBOLS = (
     "xpbd_build_last_error", "xpbd_body_create",
     "xpbd_set_colliders", "xpbd_set_gravity", "xpbd_update", "xpbd_synchronize", "xpbd_body_read",
     "xpbd_body_write", "xpbd_body_state", "xpbd_body_velocity_deltas", "xpbd_body_contacts", "xpbd_body_sensor_contacts",
-    "xpbd_body_skin_vertices",
+    "xpbd_body_skin_vertices", "xpbd_body_rod_states",
     "xpbd_body_set_skin_options", "xpbd_body_save_state", "xpbd_body_restore_state", "xpbd_profile_enable",
     "xpbd_profile_read",
     "xpbd_rollout_create", "xpbd_rollout_destroy", "xpbd_rollout_sizes", "xpbd_rollout_forward",
@@ -150,6 +151,7 @@ def load_library(path=LIB_PATH):
     lib.xpbd_body_velocity_deltas.argtypes = [vp, i32, u32, vp, vp, vp]
     lib.xpbd_body_contacts.argtypes = [vp, i32, vp, vp]
     lib.xpbd_body_sensor_contacts.argtypes = [vp, i32, u32, vp]
+    lib.xpbd_body_rod_states.argtypes = [vp, i32, u32, vp, vp]
     lib.xpbd_body_skin_vertices.argtypes = [vp, i32, vp, u32, i32]
     lib.xpbd_body_set_skin_options.argtypes = [vp, i32, i32, f]
     lib.xpbd_body_save_state.argtypes = [vp, i32, vp, C.c_size_t, C.POINTER(C.c_size_t)]
@@ -239,6 +241,7 @@ class Context:
         self.h = h
         self.sizes = []
         self._keep = []
+        self._body_rods = []
 
     def close(self):
         if self.h:
@@ -344,6 +347,15 @@ class Context:
         _check(self.lib.xpbd_body_sensor_contacts(self.h, int(body), n, hits.ctypes.data))
         return hits[:n]
 
+    def rod_states(self, body):
+        """(rotations (n, 4) xyzw, angular velocities (n, 3)) of the Cosserat rods of `body`
+        (SoftBodyMotionProperties::GetRodRotation / GetRodAngularVelocity)."""
+        n = self._body_rods[int(body)]
+        rot = np.zeros((max(n, 1), 4), np.float32)
+        ang = np.zeros((max(n, 1), 3), np.float32)
+        _check(self.lib.xpbd_body_rod_states(self.h, int(body), n, rot.ctypes.data, ang.ctypes.data))
+        return rot[:n], ang[:n]
+
     def velocity_deltas(self, body):
         """(linear (n, 3), angular (n, 3), updated (n,)): the world-space velocity changes soft
         body `body` applied to each collider during the last update
@@ -372,9 +384,11 @@ class Context:
         sk = np.ascontiguousarray(s.skinned, SKINNED)
         skn = np.ascontiguousarray(s.skinned_normals, np.uint32)
         ib = np.ascontiguousarray(s.inv_binds, INV_BIND)
+        rs = np.ascontiguousarray(s.rod_stretch_shear, ROD_SS)
+        rb = np.ascontiguousarray(s.rod_bend_twist, ROD_BT)
         d = _SettingsDesc(_ptr(v), len(v), _ptr(f), len(f), _ptr(e), len(e), _ptr(b), len(b),
-                          _ptr(t), len(t), _ptr(lr), len(lr), _ptr(g), len(g), len(sk), 0, 0,
-                          _ptr(sk), _ptr(skn), len(skn), _ptr(ib), len(ib))
+                          _ptr(t), len(t), _ptr(lr), len(lr), _ptr(g), len(g), len(sk), len(rs), len(rb),
+                          _ptr(sk), _ptr(skn), len(skn), _ptr(ib), len(ib), _ptr(rs), _ptr(rb))
         h = C.c_void_p()
         _check(self.lib.xpbd_settings_create(self.h, C.byref(d), C.byref(h)))
         st = Settings(self, h, s)
@@ -404,6 +418,7 @@ class Context:
         out = C.c_int32()
         _check(self.lib.xpbd_body_create(self.h, C.byref(d), C.byref(out)))
         self.sizes.append(len(settings.arrays.vertices))
+        self._body_rods.append(len(settings.arrays.rod_stretch_shear))
         return out.value
 
     def update(self, dt, collision_steps=1, frames=1):

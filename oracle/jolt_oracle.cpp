@@ -313,6 +313,56 @@ void jo_settings_add_skin(void *h, const void *skinned, uint32 ns, const void *i
 
 void jo_settings_optimize(void *h) { ((SettingsHandle *)h)->mSettings->Optimize(); }
 
+// Cosserat rods: appends stretch-shear rods (vertex pairs + compliance) and bend-twist
+// constraints (rod pairs + compliance), then CalculateRodProperties.
+void jo_settings_add_rods(void *h, const uint32 *ss_vertices, const float *ss_compliance, uint32 nss,
+						  const uint32 *bt_rods, const float *bt_compliance, uint32 nbt)
+{
+	SoftBodySharedSettings &st = *((SettingsHandle *)h)->mSettings;
+	for (uint32 k = 0; k < nss; ++k)
+		st.mRodStretchShearConstraints.push_back(SoftBodySharedSettings::RodStretchShear(ss_vertices[2 * k], ss_vertices[2 * k + 1], ss_compliance[k]));
+	for (uint32 k = 0; k < nbt; ++k)
+		st.mRodBendTwistConstraints.push_back(SoftBodySharedSettings::RodBendTwist(bt_rods[2 * k], bt_rods[2 * k + 1], bt_compliance[k]));
+	st.CalculateRodProperties();
+}
+
+// counts: rod stretch shear, rod bend twist constraints
+void jo_settings_rod_counts(void *h, uint32 *counts)
+{
+	const SoftBodySharedSettings &st = *((SettingsHandle *)h)->mSettings;
+	counts[0] = (uint32)st.mRodStretchShearConstraints.size();
+	counts[1] = (uint32)st.mRodBendTwistConstraints.size();
+}
+
+// Rods as 48 B (vertex[2], length, invMass, compliance, 12 B zero, bishop xyzw) and 32 B
+// (rod[2], compliance, 4 B zero, omega0 xyzw) records.
+void jo_settings_rod_read(void *h, void *ss, void *bt)
+{
+	const SoftBodySharedSettings &st = *((SettingsHandle *)h)->mSettings;
+	for (size_t k = 0; ss && k < st.mRodStretchShearConstraints.size(); ++k)
+	{
+		const SoftBodySharedSettings::RodStretchShear &r = st.mRodStretchShearConstraints[k];
+		char *o = (char *)ss + 48 * k;
+		memset(o, 0, 48);
+		memcpy(o, r.mVertex, 8);
+		memcpy(o + 8, &r.mLength, 4);
+		memcpy(o + 12, &r.mInvMass, 4);
+		memcpy(o + 16, &r.mCompliance, 4);
+		float q[4] = { r.mBishop.GetX(), r.mBishop.GetY(), r.mBishop.GetZ(), r.mBishop.GetW() };
+		memcpy(o + 32, q, 16);
+	}
+	for (size_t k = 0; bt && k < st.mRodBendTwistConstraints.size(); ++k)
+	{
+		const SoftBodySharedSettings::RodBendTwist &r = st.mRodBendTwistConstraints[k];
+		char *o = (char *)bt + 32 * k;
+		memset(o, 0, 32);
+		memcpy(o, r.mRod, 8);
+		memcpy(o + 8, &r.mCompliance, 4);
+		float q[4] = { r.mOmega0.GetX(), r.mOmega0.GetY(), r.mOmega0.GetZ(), r.mOmega0.GetW() };
+		memcpy(o + 16, q, 16);
+	}
+}
+
 // counts: skinned constraints, skinned constraint normals, inverse bind matrices
 void jo_settings_skin_counts(void *h, uint32 *counts)
 {
@@ -351,7 +401,8 @@ void *jo_settings_from_arrays(const void *vertices, uint32 nv, const uint32 *fac
 							  const void *edges, uint32 ne, const void *bends, uint32 nb,
 							  const void *volumes, uint32 nt, const void *lras, uint32 nl,
 							  const uint32 *groups7, uint32 ng, const void *skinned, uint32 ns,
-							  const uint32 *skinned_normals, uint32 nsn, const void *inv_binds, uint32 nib)
+							  const uint32 *skinned_normals, uint32 nsn, const void *inv_binds, uint32 nib,
+							  const void *rods_ss, uint32 nss, const void *rods_bt, uint32 nbt)
 {
 	InitJolt();
 	std::stringstream ss;
@@ -360,13 +411,24 @@ void *jo_settings_from_arrays(const void *vertices, uint32 nv, const uint32 *fac
 	WriteArray(ss, (const char(*)[16])edges, ne);
 	WriteArray(ss, (const char(*)[24])bends, nb);
 	WriteArray(ss, (const char(*)[24])volumes, nt);
-	uint32 zero = 0;
 	WriteArray(ss, (const char(*)[52])skinned, ns);
 	WriteArray(ss, skinned_normals, nsn);
 	WriteArray(ss, (const char(*)[12])lras, nl);
 	WriteArray(ss, (const char(*)[28])groups7, ng);
-	ss.write((const char *)&zero, 4);				// rod stretch shear
-	ss.write((const char *)&zero, 4);				// rod bend twist
+	ss.write((const char *)&nss, 4);				// rod stretch shear: vertex[2], length, invMass, compliance, bishop
+	for (uint32 k = 0; k < nss; ++k)
+	{
+		const char *r = (const char *)rods_ss + 48ull * k;
+		ss.write(r, 20);
+		ss.write(r + 32, 16);
+	}
+	ss.write((const char *)&nbt, 4);				// rod bend twist: rod[2], compliance, omega0
+	for (uint32 k = 0; k < nbt; ++k)
+	{
+		const char *r = (const char *)rods_bt + 32ull * k;
+		ss.write(r, 12);
+		ss.write(r + 16, 16);
+	}
 	ss.write((const char *)&nib, 4);				// inv bind matrices: joint index, Mat44 (no padding)
 	for (uint32 k = 0; k < nib; ++k)
 	{
@@ -781,6 +843,24 @@ int jo_body_skin_vertices(void *wp, int body, const float *joints, uint32 n, int
 		for (int c = 0; c < 4; ++c)
 			m[j].SetColumn4(c, Vec4(joints[16 * j + 4 * c], joints[16 * j + 4 * c + 1], joints[16 * j + 4 * c + 2], joints[16 * j + 4 * c + 3]));
 	mp->SkinVertices(b.GetCenterOfMassTransform(), m.data(), n, hard_skin_all != 0, *w->mTemp);
+	return 0;
+}
+
+// Rod rotations (n x 4) and angular velocities (n x 3) of soft body `body`
+int jo_body_rod_states(void *wp, int body, float *rot, float *ang, uint32 n)
+{
+	World *w = (World *)wp;
+	BodyLockRead lock(w->mSystem.GetBodyLockInterface(), w->mSoftBodies[body]);
+	if (!lock.Succeeded())
+		return -1;
+	const SoftBodyMotionProperties *mp = static_cast<const SoftBodyMotionProperties *>(lock.GetBody().GetMotionProperties());
+	for (uint32 r = 0; r < n; ++r)
+	{
+		Quat q = mp->GetRodRotation(r);
+		Vec3 a = mp->GetRodAngularVelocity(r);
+		rot[4 * r] = q.GetX(); rot[4 * r + 1] = q.GetY(); rot[4 * r + 2] = q.GetZ(); rot[4 * r + 3] = q.GetW();
+		ang[3 * r] = a.GetX(); ang[3 * r + 1] = a.GetY(); ang[3 * r + 2] = a.GetZ();
+	}
 	return 0;
 }
 

@@ -16,7 +16,8 @@ import os
 
 import numpy as np
 
-from joltphysics_amd.records import (BEND, EDGE, FACE, GROUP, INV_BIND, LRA, MOTION_DYNAMIC, SHAPE_CONVEX_HULL, SKINNED,
+from joltphysics_amd.records import (BEND, EDGE, FACE, GROUP, INV_BIND, LRA, MOTION_DYNAMIC, ROD_BT, ROD_SS,
+                                     SHAPE_CONVEX_HULL, SKINNED,
                                      VERTEX, VOLUME, ConvexHull, SettingsArrays)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -72,8 +73,12 @@ class JoltReference:
                                               _vp, _u32, _i, _f, _f, _u32]
             lib.jo_settings_from_arrays.restype = _vp
             lib.jo_settings_from_arrays.argtypes = [_vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32,
-                                                    _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32]
+                                                    _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32,
+                                                    _vp, _u32, _vp, _u32]
             lib.jo_settings_add_skin.argtypes = [_vp, _vp, _u32, _vp, _u32]
+            lib.jo_settings_add_rods.argtypes = [_vp, _vp, _vp, _u32, _vp, _vp, _u32]
+            lib.jo_settings_rod_counts.argtypes = [_vp, _vp]
+            lib.jo_settings_rod_read.argtypes = [_vp, _vp, _vp]
             lib.jo_settings_optimize.argtypes = [_vp]
             lib.jo_settings_skin_counts.argtypes = [_vp, _vp]
             lib.jo_settings_skin_read.argtypes = [_vp, _vp, _vp, _vp]
@@ -105,6 +110,8 @@ class JoltReference:
             lib.jo_world_contacts.restype = _i
             lib.jo_body_skin_vertices.restype = _i
             lib.jo_body_skin_vertices.argtypes = [_vp, _i, _vp, _u32, _i, _i, _f]
+            lib.jo_body_rod_states.restype = _i
+            lib.jo_body_rod_states.argtypes = [_vp, _i, _vp, _vp, _u32]
             lib.jo_body_save_state.restype = C.c_long
             lib.jo_body_save_state.argtypes = [_vp, _i, _vp, C.c_long]
             lib.jo_body_restore_state.restype = _i
@@ -138,10 +145,11 @@ class JoltReference:
         lr = np.ascontiguousarray(mesh.lras, LRA)
         a = np.ascontiguousarray(mesh.attrs, np.float32).reshape(-1, 5)
         skinned = getattr(mesh, "skinned", None)
+        rods = getattr(mesh, "rods", None)
         flags = int(mesh.flags)
         optimize = 16
-        if skinned is not None:
-            flags &= ~optimize          # skinned constraints join before Optimize
+        if skinned is not None or rods is not None:
+            flags &= ~optimize          # skinned constraints and rods join before Optimize
         h = self.lib.jo_settings_build(_ptr(v), len(v), _ptr(f), len(f), _ptr(e), len(e), None, 0,
                                        _ptr(t), len(t), _ptr(lr), len(lr), _ptr(a), len(a), int(mesh.bend_type),
                                        float(mesh.angle_tolerance), float(mesh.lra_multiplier), flags)
@@ -150,8 +158,14 @@ class JoltReference:
                 sk = np.ascontiguousarray(skinned, SKINNED)
                 ib = np.ascontiguousarray(mesh.inv_binds, INV_BIND)
                 self.lib.jo_settings_add_skin(h, _ptr(sk), len(sk), _ptr(ib), len(ib))
-                if int(mesh.flags) & optimize:
-                    self.lib.jo_settings_optimize(h)
+            if rods is not None:
+                sv = np.ascontiguousarray(rods[0], np.uint32).reshape(-1, 2)
+                sc = np.ascontiguousarray(rods[1], np.float32).reshape(-1)
+                bv = np.ascontiguousarray(rods[2], np.uint32).reshape(-1, 2)
+                bc = np.ascontiguousarray(rods[3], np.float32).reshape(-1)
+                self.lib.jo_settings_add_rods(h, _ptr(sv), _ptr(sc), len(sv), _ptr(bv), _ptr(bc), len(bv))
+            if (skinned is not None or rods is not None) and int(mesh.flags) & optimize:
+                self.lib.jo_settings_optimize(h)
             return self._read_settings(h)
         finally:
             self.lib.jo_settings_free(h)
@@ -171,6 +185,12 @@ class JoltReference:
             s.skinned_normals = np.zeros(int(sc[1]), np.uint32)
             s.inv_binds = np.zeros(int(sc[2]), INV_BIND)
             self.lib.jo_settings_skin_read(h, _ptr(s.skinned), _ptr(s.skinned_normals), _ptr(s.inv_binds))
+        rc = np.zeros(2, np.uint32)
+        self.lib.jo_settings_rod_counts(h, _ptr(rc))
+        if rc[0] or rc[1]:
+            s.rod_stretch_shear = np.zeros(int(rc[0]), ROD_SS)
+            s.rod_bend_twist = np.zeros(int(rc[1]), ROD_BT)
+            self.lib.jo_settings_rod_read(h, _ptr(s.rod_stretch_shear), _ptr(s.rod_bend_twist))
         return s
 
     def settings_handle(self, s: SettingsArrays):
@@ -180,7 +200,8 @@ class JoltReference:
                                              _ptr(s.volumes), len(s.volumes), _ptr(s.lras), len(s.lras),
                                              _ptr(s.groups), len(s.groups), _ptr(s.skinned), len(s.skinned),
                                              _ptr(s.skinned_normals), len(s.skinned_normals), _ptr(s.inv_binds),
-                                             len(s.inv_binds))
+                                             len(s.inv_binds), _ptr(s.rod_stretch_shear), len(s.rod_stretch_shear),
+                                             _ptr(s.rod_bend_twist), len(s.rod_bend_twist))
         if not h:
             raise RuntimeError("reference rejected the settings arrays")
         return h
@@ -202,6 +223,7 @@ class JoltWorld:
         self.lib.jo_world_set_gravity(self.h, *map(float, gravity))
         self._settings = []
         self.sizes = []
+        self.rods = []
 
     def close(self):
         if self.h:
@@ -256,6 +278,15 @@ class JoltWorld:
         e = -1 if enable is None else int(bool(enable))
         if self.lib.jo_body_skin_vertices(self.h, body, _ptr(j), len(j), int(bool(hard_skin_all)), e, float(multiplier)) != 0:
             raise RuntimeError("SkinVertices failed")
+
+    def rod_states(self, body):
+        """(rotations (n, 4), angular velocities (n, 3)) of the rods of soft body `body`."""
+        n = self.rods[body]
+        rot = np.zeros((max(n, 1), 4), np.float32)
+        ang = np.zeros((max(n, 1), 3), np.float32)
+        if self.lib.jo_body_rod_states(self.h, body, _ptr(rot), _ptr(ang), n) != 0:
+            raise RuntimeError("rod states failed")
+        return rot[:n], ang[:n]
 
     def save_state(self, body):
         """SoftBodyMotionProperties::SaveState bytes of soft body `body`."""
@@ -327,6 +358,7 @@ class JoltWorld:
         if i < 0:
             raise RuntimeError("add_soft_body failed")
         self.sizes.append(len(settings.vertices))
+        self.rods.append(len(settings.rod_stretch_shear))
         return i
 
     def optimize_broadphase(self):

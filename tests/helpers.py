@@ -44,6 +44,7 @@ def run_gpu(scene, settings_list, bounds=None, frames_to_keep=(1,), device=0):
         for i, r in zip(ids, out[f]):
             r["contact_collider"], r["contact_point"] = ctx.contacts(i)
             r["sensor_hits"] = ctx.sensor_contacts(i)
+            r["rod_rot"], r["rod_ang"] = ctx.rod_states(i)
     ctx.close()
     return out
 
@@ -105,6 +106,8 @@ def reference_run(scene, frames_to_keep, variant="nofma", num_threads=0, record_
         world.update(scene.dt, scene.collision_steps, f - done)
         done = f
         out[f] = [world.read(i) for i in range(len(scene.bodies))]
+        for i, r in enumerate(out[f]):
+            r["rod_rot"], r["rod_ang"] = world.rod_states(i)
         if record_contacts:
             for i, r in enumerate(out[f]):
                 r["contact_collider"], r["contact_point"] = world.contacts(i)

@@ -282,3 +282,67 @@ def cloth_sensors(n=30, frames=24, spacing=0.04):
                      Collider(SHAPE_BOX, (0.2, 0.2, 0.05), pos=(0.0, 0.0, 0.0), friction=0.5),
                      Collider(SHAPE_BOX, (0.1, 0.1, 0.1), pos=(2.0, 2.0, 0.5), is_sensor=True)])
     return Scene("cloth_sensors", [body], cols, gravity=(0.0, 0.0, -9.81), dt=1 / 60, collision_steps=1, frames=frames)
+
+
+def _rod_strands(strands, segments, length, seed, reversed_strand=1):
+    """Vertices and rods of `strands` chains: each starts at a kinematic anchor on a small
+    circle and runs outward, turning in and out of the horizontal plane; rods join
+    consecutive vertices (strand `reversed_strand` lists every other rod backwards, which
+    CalculateRodProperties turns round), bend-twist constraints join consecutive rods of a
+    strand and the first rods of neighbouring strand pairs."""
+    from joltphysics_amd.records import VERTEX
+    rng = np.random.default_rng(seed)
+    nv = strands * (segments + 1)
+    v = np.zeros(nv, VERTEX)
+    ss_v, ss_c, bt_r, bt_c = [], [], [], []
+    for s in range(strands):
+        a0 = 2.0 * np.pi * s / strands
+        base = s * (segments + 1)
+        p = np.array([0.05 * np.cos(a0), 0.05 * np.sin(a0), 0.0])
+        first_rod = len(ss_v)
+        for k in range(segments + 1):
+            v[base + k]["pos"] = (p + rng.normal(0.0, 1e-4, 3)).astype(np.float32)
+            v[base + k]["inv_mass"] = 0.0 if k == 0 else 20.0
+            theta = a0 + 0.12 * k
+            phi = 0.25 * np.sin(0.7 * k + s)
+            p = p + length * np.array([np.cos(phi) * np.cos(theta), np.cos(phi) * np.sin(theta), np.sin(phi)])
+        for k in range(segments):
+            pair = (base + k + 1, base + k) if s == reversed_strand and k % 2 else (base + k, base + k + 1)
+            ss_v.append(pair)
+            ss_c.append(0.0 if s % 2 == 0 else 1e-6)
+        for k in range(segments - 1):
+            bt_r.append((first_rod + k, first_rod + k + 1))
+            bt_c.append(1e-4 if s % 3 else 1e-2)
+    for s in range(0, strands - 1, 2):
+        bt_r.append((s * segments, (s + 1) * segments))
+        bt_c.append(1e-3)
+    rods = (np.array(ss_v, np.uint32), np.array(ss_c, np.float32), np.array(bt_r, np.uint32), np.array(bt_c, np.float32))
+    return v, rods
+
+
+def hanging_rods(strands=8, segments=16, frames=40, length=0.04):
+    """Cosserat rods: two bodies of curved strands hanging from kinematic anchors fall,
+    swing and drape over a sphere and a box.  One body is rotated with its rotation made
+    identity (the rods start at the creation rotation times their Bishop frames), the
+    other keeps its rotation in the body transform."""
+    v, rods = _rod_strands(strands, segments, length, seed=53)
+    mesh = MeshSpec(vertices=v, flags=OPTIMIZE, rods=rods)
+    a = BodySpec(mesh=mesh, position=(0.0, 0.0, 1.0), rotation=_quat((1, 0, 0), 0.4),
+                 params=BodyParams(num_iterations=5, allow_sleeping=False, vertex_radius=0.005))
+    b = BodySpec(mesh=mesh, position=(0.8, 0.0, 1.0), rotation=_quat((0, 1, 1), 0.3),
+                 params=BodyParams(num_iterations=4, allow_sleeping=False, make_rotation_identity=False,
+                                   linear_damping=0.3))
+    cols = _bounded([Collider(SHAPE_SPHERE, (0.2,), pos=(0.1, 0.05, 0.55), friction=0.4),
+                     Collider(SHAPE_BOX, (0.3, 0.3, 0.05), pos=(0.8, 0.0, 0.45), friction=0.2)])
+    return Scene("hanging_rods", [a, b], cols, gravity=(0.0, 0.0, -9.81), dt=1 / 60, collision_steps=2, frames=frames)
+
+
+def hanging_rods_streamed(frames=6):
+    """48 strands of 200 rods (9,648 vertices): the rods run from gathered windows."""
+    v, rods = _rod_strands(48, 200, 0.004, seed=59)
+    mesh = MeshSpec(vertices=v, flags=OPTIMIZE, rods=rods)
+    body = BodySpec(mesh=mesh, position=(0.0, 0.0, 1.0),
+                    params=BodyParams(num_iterations=3, allow_sleeping=False, vertex_radius=0.002))
+    cols = _bounded([Collider(SHAPE_SPHERE, (0.3,), pos=(0.0, 0.0, 0.7), friction=0.3)])
+    return Scene("hanging_rods_streamed", [body], cols, gravity=(0.0, 0.0, -9.81), dt=1 / 60, collision_steps=1,
+                 frames=frames)

@@ -27,6 +27,11 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in names if not hasattr(lib, n)]
     assert missing == []
     assert set(xpbd.EXPORTED_SYMBOLS) <= set(names)
+    # every declared entry point with arguments has its ctypes signature (an undeclared one
+    # would pass 64-bit pointers as C ints)
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT_DIR, "include", "jolt_xpbd.h")).read(), flags=re.S)
+    no_sig = [n for n in names if not re.search(n + r"\s*\(\s*void\s*\)", text) and getattr(lib, n).argtypes is None]
+    assert no_sig == []
 
 
 def test_context_create_without_device_reports_error():

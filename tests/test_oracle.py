@@ -402,3 +402,55 @@ def test_tapered_capsule_cone_normal():
     # the plane passes through the bottom sphere's tangent point bottom_c + r_bottom * n
     point = np.array([0.0, bottom_c, 0.0]) + bottom_r * n
     assert p[3] == pytest.approx(-np.dot(n, point), abs=1e-6)
+
+
+@pytest.mark.skipif(not _have_reference(), reason="reference library not built (needs the reference sources)")
+def test_rod_properties_closed_forms():
+    """CalculateRodProperties (SoftBodySharedSettings.cpp:381-481) as the harness returns it:
+    rod lengths are the vertex distances, rod inverse masses the smaller vertex inverse mass,
+    Bishop frames unit quaternions whose z axis is the rod's tangent, consecutive rods of a
+    strand point the same way (the reversed rods were turned round) and their frames differ
+    by a rotation without twist about the tangent (parallel transport), omega0 =
+    normalize(conj(b1) b2); the arrays survive the binary round trip of the settings."""
+    import scene_zoo
+    from oracle import JoltReference
+    sc = scene_zoo.hanging_rods()
+    ref = JoltReference("nofma")
+    s = ref.build_settings(sc.bodies[0].mesh)
+    rs, bt = s.rod_stretch_shear, s.rod_bend_twist
+    assert len(rs) == 8 * 16 and len(bt) == 8 * 15 + 4
+    x = s.vertices["pos"].astype(np.float64)
+    v0, v1 = rs["vertex"][:, 0], rs["vertex"][:, 1]
+    t = x[v1] - x[v0]
+    np.testing.assert_allclose(rs["length"], np.linalg.norm(t, axis=1), rtol=1e-6)
+    np.testing.assert_array_equal(rs["inv_mass"], np.minimum(s.vertices["inv_mass"][v0], s.vertices["inv_mass"][v1]))
+
+    def qmul(a, b):
+        ax, ay, az, aw = a.T
+        bx, by, bz, bw = b.T
+        return np.stack([aw * bx + ax * bw + ay * bz - az * by, aw * by - ax * bz + ay * bw + az * bx,
+                         aw * bz + ax * by - ay * bx + az * bw, aw * bw - ax * bx - ay * by - az * bz], axis=1)
+
+    b = rs["bishop"].astype(np.float64)
+    np.testing.assert_allclose(np.linalg.norm(b, axis=1), 1.0, atol=1e-6)
+    qx, qy, qz, qw = b.T
+    zaxis = np.stack([2 * (qx * qz + qw * qy), 2 * (qy * qz - qw * qx), 1 - 2 * (qx * qx + qy * qy)], axis=1)
+    np.testing.assert_allclose(zaxis, t / np.linalg.norm(t, axis=1)[:, None], atol=1e-5)
+    conj = b * np.array([-1, -1, -1, 1])
+    r1, r2 = bt["rod"][:, 0], bt["rod"][:, 1]
+    om = qmul(conj[r1], b[r2])
+    om /= np.linalg.norm(om, axis=1)[:, None]
+    np.testing.assert_allclose(bt["omega0"], om, atol=1e-6)
+    # consecutive rods of one strand share a vertex (Optimize reorders the rods)
+    chain = (v0[r1] == v0[r2]) | (v0[r1] == v1[r2]) | (v1[r1] == v0[r2]) | (v1[r1] == v1[r2])
+    assert chain.sum() == 8 * 15
+    assert (np.einsum("ij,ij->i", t[r1[chain]], t[r2[chain]]) > 0).all()
+    np.testing.assert_allclose(om[chain, 2], 0.0, atol=2e-6)
+    # the reversed rods of strand 1 were turned round
+    assert (v1 > v0).all()
+    h = ref.settings_handle(s)
+    try:
+        back = ref._read_settings(h)
+    finally:
+        ref.lib.jo_settings_free(h)
+    assert back.rod_stretch_shear.tobytes() == rs.tobytes() and back.rod_bend_twist.tobytes() == bt.tobytes()
