@@ -305,12 +305,14 @@ int xpbd_body_contacts(xpbd_context *ctx, int32_t body, float *points, int32_t *
 int xpbd_body_sensor_contacts(xpbd_context *ctx, int32_t body, uint32_t num_colliders, int32_t *hits);
 
 /*
- * The byte stream SoftBodyMotionProperties::SaveState writes for a soft body without
- * rods or skinning (SoftBodyMotionProperties.cpp:1446-1473, MotionProperties.cpp:63-76):
- * linear and angular velocity (world), force and torque (zero between updates), the three
- * sleep test spheres (not used by soft bodies; written as zeros), sleep timer, allow
- * sleeping (1 byte), then position and velocity of every vertex (body space), then the
- * local bounds and local predicted bounds (min, max each); Vec3s as 3 floats.  With `out`
+ * The byte stream SoftBodyMotionProperties::SaveState writes for a soft body
+ * (SoftBodyMotionProperties.cpp:1446-1473, MotionProperties.cpp:63-76): linear and
+ * angular velocity (world), force and torque (zero between updates), the three sleep
+ * test spheres (not used by soft bodies; written as zeros), sleep timer, allow sleeping
+ * (1 byte), then position and velocity of every vertex (body space), rotation (4 floats)
+ * and angular velocity of every rod, previous skinned position, skinned position and
+ * normal of every vertex of a skinned body, then the local bounds and local predicted
+ * bounds (min, max each); Vec3s as 3 floats.  With `out`
  * NULL only *size is set.  The body's position and rotation are Body state, not part
  * of the stream (xpbd_body_read).
  */

@@ -1872,8 +1872,12 @@ int xpbd_body_state(xpbd_context *ctx, int32_t body, float out[14])
 
 namespace {
 constexpr size_t kStateHeaderBytes = 4 * 12 + 48 + 4 + 1;   // velocities, force, torque, spheres, timer, allow sleeping
-// header, per vertex position + velocity, per rod rotation (16 B) + angular velocity (12 B), bounds
-size_t StateBytes(uint32_t nv, size_t nrods) { return kStateHeaderBytes + (size_t)nv * 24 + nrods * 28 + 4 * 12; }
+// header, per vertex position + velocity, per rod rotation (16 B) + angular velocity (12 B),
+// per vertex of a skinned body the skin state (previous position, position, normal: 36 B), bounds
+size_t StateBytes(uint32_t nv, size_t nrods, bool skinned)
+{
+    return kStateHeaderBytes + (size_t)nv * 24 + nrods * 28 + (skinned ? (size_t)nv * 36 : 0) + 4 * 12;
+}
 } // namespace
 
 namespace {
@@ -2006,10 +2010,8 @@ int xpbd_body_save_state(xpbd_context *ctx, int32_t body, void *out, size_t capa
     if (!ctx || body < 0 || body >= (int32_t)ctx->bodies.size() || !size)
         return Fail(XPBD_E_INVALID, "invalid body or size pointer");
     const Body &b = *ctx->bodies[body];
-    if (b.skinned)
-        return Fail(XPBD_E_UNSUPPORTED, "state streams of skinned bodies (with skin states) are not supported");
     const size_t nrods = b.settings->rods.size();
-    *size = StateBytes(b.nv, nrods);
+    *size = StateBytes(b.nv, nrods, b.skinned);
     if (!out)
         return XPBD_OK;
     if (capacity < *size)
@@ -2018,6 +2020,8 @@ int xpbd_body_save_state(xpbd_context *ctx, int32_t body, void *out, size_t capa
     if (rc != XPBD_OK)
         return rc;
     if ((rc = PullStates(ctx)) != XPBD_OK)
+        return rc;
+    if (b.skinned && (rc = PullSkin(ctx)) != XPBD_OK)
         return rc;
     const BodyStateDev &s = b.state;
     std::vector<float4> x(b.nv), v(b.nv);
@@ -2050,6 +2054,14 @@ int xpbd_body_save_state(xpbd_context *ctx, int32_t body, void *out, size_t capa
             put(&rs[2 * r + 1], 12);
         }
     }
+    if (b.skinned)
+    {
+        // SkinState per vertex: previous position, position, normal
+        const float4 *E = ctx->skin_host.data() + 4 * (size_t)b.dev.skin_base;
+        for (uint32_t i = 0; i < b.nv; ++i)
+            for (int k = 0; k < 3; ++k)
+                put(&E[4 * i + k], 12);
+    }
     put(s.bmin, 12); put(s.bmax, 12); put(s.pmin, 12); put(s.pmax, 12);
     return XPBD_OK;
 }
@@ -2060,15 +2072,15 @@ int xpbd_body_restore_state(xpbd_context *ctx, int32_t body, const void *in, siz
     if (!ctx || body < 0 || body >= (int32_t)ctx->bodies.size() || !in)
         return Fail(XPBD_E_INVALID, "invalid body or stream");
     Body &b = *ctx->bodies[body];
-    if (b.skinned)
-        return Fail(XPBD_E_UNSUPPORTED, "state streams of skinned bodies (with skin states) are not supported");
     const size_t nrods = b.settings->rods.size();
-    if (size != StateBytes(b.nv, nrods))
+    if (size != StateBytes(b.nv, nrods, b.skinned))
         return Fail(XPBD_E_INVALID, "state stream size does not match the body");
     int rc = xpbd_synchronize(ctx);
     if (rc != XPBD_OK)
         return rc;
     if ((rc = PullStates(ctx)) != XPBD_OK)
+        return rc;
+    if (b.skinned && (rc = PullSkin(ctx)) != XPBD_OK)
         return rc;
     const unsigned char *p = (const unsigned char *)in;
     auto get = [&p](void *dst, size_t n) { memcpy(dst, p, n); p += n; };
@@ -2101,6 +2113,19 @@ int xpbd_body_restore_state(xpbd_context *ctx, int32_t body, const void *in, siz
     }
     if (nrods)
         HIP_TRY(hipMemcpy(b.dev.rods, rs.data(), sizeof(float4) * rs.size(), hipMemcpyHostToDevice));
+    if (b.skinned)
+    {
+        // skin states (the previous-position-valid flag is not part of the stream: it stays)
+        float4 *E = ctx->skin_host.data() + 4 * (size_t)b.dev.skin_base;
+        for (uint32_t i = 0; i < b.nv; ++i)
+            for (int k = 0; k < 3; ++k)
+            {
+                float q[3];
+                get(q, 12);
+                E[4 * i + k] = make_float4(q[0], q[1], q[2], E[4 * i + k].w);
+            }
+        ctx->skin_dirty = true;
+    }
     get(s.bmin, 12); get(s.bmax, 12); get(s.pmin, 12); get(s.pmax, 12);
     if (position) memcpy(s.pos, position, 12);
     if (rotation) memcpy(s.rot, rotation, 16);

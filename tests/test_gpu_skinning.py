@@ -69,3 +69,47 @@ def test_skinned_cloth_bit_exact(variant):
         assert float_equal(gpu[f]["pos"], ref[f]["pos"]), f"positions differ at frame {f}"
         assert float_equal(gpu[f]["vel"], ref[f]["vel"]), f"velocities differ at frame {f}"
         assert float_equal(gpu[f]["body_pos"], ref[f]["body_pos"]), f"body position differs at frame {f}"
+
+
+@pytest.mark.skipif(not have_reference("nofma"), reason="reference library not built")
+def test_skinned_state_stream_round_trip():
+    """SaveState of a skinned body (skin states after the vertices, :1462-1467): the
+    reference's stream after 6 frames restored into a fresh GPU body continues bit-exact
+    with SkinVertices every frame; the GPU's own stream holds the same skin states."""
+    from oracle import JoltReference, JoltWorld
+    from joltphysics_amd.xpbd import Context
+    sc, joints = scene_zoo.skinned_cloth(frames=10)
+    settings = JoltReference("nofma").build_settings(sc.bodies[0].mesh)
+    b = sc.bodies[0]
+    nv = len(settings.vertices)
+    w = JoltWorld(JoltReference("nofma"), sc.gravity)
+    w.add_soft_body(settings, b.position, b.rotation, b.params)
+    ctx = Context(0)
+    ctx.set_gravity(sc.gravity)
+    body = ctx.create_body(ctx.create_settings(settings), b.position, b.rotation, b.params)
+    for f in range(7):
+        w.skin_vertices(0, joints(f), hard_skin_all=(f == 0))
+        ctx.skin_vertices(body, joints(f), hard_skin_all=(f == 0))
+        if f > 0:
+            w.update(sc.dt, sc.collision_steps, 1)
+            ctx.update(sc.dt, sc.collision_steps, 1)
+    blob, at6 = w.save_state(0), w.read(0)
+    assert len(blob) == 101 + 24 * nv + 36 * nv + 48
+    skin = slice(101 + 24 * nv, 101 + 60 * nv)
+    assert ctx.save_state(body)[skin] == blob[skin], "skin states differ from the reference's"
+    ctx.close()
+
+    ctx = Context(0)
+    ctx.set_gravity(sc.gravity)
+    body = ctx.create_body(ctx.create_settings(settings), b.position, b.rotation, b.params)
+    ctx.restore_state(body, blob, position=at6["body_pos"], rotation=at6["body_rot"])
+    assert ctx.save_state(body)[101:101 + 60 * nv] == blob[101:101 + 60 * nv]
+    for f in range(7, 11):
+        w.skin_vertices(0, joints(f))
+        ctx.skin_vertices(body, joints(f))
+        w.update(sc.dt, sc.collision_steps, 1)
+        ctx.update(sc.dt, sc.collision_steps, 1)
+        r, g = w.read(0), ctx.read(body)
+        assert float_equal(g["pos"], r["pos"]) and float_equal(g["vel"], r["vel"]), f"frame {f} differs"
+    w.close()
+    ctx.close()
