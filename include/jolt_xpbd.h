@@ -268,6 +268,21 @@ int xpbd_body_write(xpbd_context *ctx, int32_t body, const float *pos, const flo
 int xpbd_body_state(xpbd_context *ctx, int32_t body, float out[14]);
 
 /*
+ * Per-pair contact settings of a body: what SoftBodyContactListener::
+ * OnSoftBodyContactValidate (SoftBodyContactListener.h:19-60) decides for the pair
+ * (body, collider), applied by DetermineCollidingShapes (SoftBodyMotionProperties.cpp:
+ * 151-216).  settings: num_colliders x {mInvMassScale1 (soft body), mInvMassScale2,
+ * mInvInertiaScale2 (the collider), accept (1 AcceptContact, 0 RejectContact)}, in the
+ * order of xpbd_set_colliders; a colliding pair without interaction (scale1 0 and, for a
+ * dynamic collider, scale2 0) is skipped as the reference skips it.  The table holds for
+ * every step of the following updates (the adapter asks the listener once per pair);
+ * changing a pair's mIsSensor is not supported.  settings NULL removes the table (no
+ * listener: every pair with unit scales).  A later xpbd_set_colliders with another
+ * collider count makes xpbd_update fail until the table is set again.
+ */
+int xpbd_body_set_contact_settings(xpbd_context *ctx, int32_t body, uint32_t num_colliders, const float *settings);
+
+/*
  * Cosserat rod states of a body: SoftBodyMotionProperties::GetRodRotation /
  * GetRodAngularVelocity (SoftBodyMotionProperties.h:64-65) for all rods at once.
  * rotations: num_rods x 4 (xyzw, relative to the centre of mass transform);

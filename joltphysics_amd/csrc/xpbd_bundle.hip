@@ -43,6 +43,7 @@ struct BodyLocal
     uint32_t num_sensors;
     uint32_t ran;               // the body was active at the start of this step
     DynStateDev *dyn;           // the body's row of dynamic-collider states
+    const float4 *pair;         // the body's per-collider contact settings, or null
     uint32_t nf;
     const uint32_t *faces;
     const uint32_t *vf_begin;
@@ -142,6 +143,7 @@ __device__ __forceinline__ void bundle_update(
         L.nf = B.pressure > 0.0f ? B.s.num_faces : 0u;
         L.faces = B.s.faces; L.vf_begin = B.s.vf_begin; L.vf_list = B.s.vf_list;
         L.dyn = B.dyn;
+        L.pair = B.pair;
         L.report_cleared = 0;
         L.skin_entries = B.skin ? B.skin + 4u * (size_t)B.skin_base : nullptr;
         L.sensor_hits = B.sensor_hits;
@@ -229,14 +231,15 @@ __device__ __forceinline__ void bundle_update(
                     wmax = wmax + vmax(a, bq);
                 }
                 const ColliderDev &cd = colliders[c];
+                const float4 pr = pair_settings(L.pair, c);
                 bool overlap = !(wmin.x > cd.bounds[3] || wmin.y > cd.bounds[4] || wmin.z > cd.bounds[5]
                                  || wmax.x < cd.bounds[0] || wmax.y < cd.bounds[1] || wmax.z < cd.bounds[2]);
-                if (!overlap)
+                if (!overlap || pair_skipped(pr, cd.sensor >= 0, cd.motion == 2))
                 {
                     out.type = -1;
                     continue;
                 }
-                make_leaf(cd, inversed_rotation_translation(com), L.friction, L.restitution, out, L.dyn);
+                make_leaf(cd, inversed_rotation_translation(com), L.friction, L.restitution, out, L.dyn, pr);
                 out.collider = (int32_t)c;
             }
             sync_global();
@@ -439,8 +442,9 @@ __device__ __forceinline__ void bundle_update(
                                 x = x + n * projected_distance;
                                 if (cs.dyn >= 0)
                                     dynamic_contact(L.dyn[cs.dyn], col3(cs.t, 3), x, v, prev_v, n, projected_distance, p.w,
-                                                    cs.friction, cs.restitution, L.dt_sub, L.restitution_threshold);
-                                else
+                                                    cs.friction, cs.restitution, L.dt_sub, L.restitution_threshold,
+                                                    cs.inv_mass_scale1);
+                                else if (cs.inv_mass_scale1 > 0.0f)
                                 {
                                     V3 v_normal = n * dot(n, v);
                                     V3 v_tangential = v - v_normal;

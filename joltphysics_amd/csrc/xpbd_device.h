@@ -27,7 +27,21 @@ struct Leaf
     const HullDev *hull;    // convex hull tables (device memory)
     int32_t dyn;            // dynamic collider: rank into the body's DynStateDev row; -1 otherwise
     int32_t collider;       // index of the collider in the context's list
+    float inv_mass_scale1;  // SoftBodyContactSettings::mInvMassScale1 of the pair (CollidingShape::mSoftBodyInvMassScale)
 };
+
+// Per-pair contact settings {mInvMassScale1, mInvMassScale2, mInvInertiaScale2, accept}
+// (SoftBodyContactListener.h:19-28); without a table every pair is accepted with unit scales.
+__device__ __forceinline__ float4 pair_settings(const float4 *row, uint32_t collider)
+{
+    return row ? row[collider] : make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+}
+// DetermineCollidingShapes with a contact listener (SoftBodyMotionProperties.cpp:160-171): a
+// rejected pair is skipped, and so is a colliding pair without any interaction
+__device__ __forceinline__ bool pair_skipped(float4 pair, bool sensor, bool dynamic)
+{
+    return pair.w == 0.0f || (!sensor && pair.x == 0.0f && (!dynamic || pair.y == 0.0f));
+}
 
 __device__ __forceinline__ V3 xyz(float4 p) { return v3(p.x, p.y, p.z); }
 __device__ __forceinline__ float4 f4(V3 p, float w) { return make_float4(p.x, p.y, p.z, w); }
@@ -834,7 +848,8 @@ __device__ __forceinline__ M44 inv_inertia_matrix(const DynStateDev &D)
 // contact listener: every scale 1): inverse mass, inverse inertia for the relative
 // rotation (MotionProperties::GetInverseInertiaForRotation, MotionProperties.inl:66-82,
 // all angular DOFs allowed) and the velocities in soft-body space.
-__device__ __forceinline__ void init_dynamic(const ColliderDev &c, const M44 &inv_com, const M44 &rel, DynStateDev &D)
+__device__ __forceinline__ void init_dynamic(const ColliderDev &c, const M44 &inv_com, const M44 &rel, DynStateDev &D,
+                                             float inv_mass_scale, float inv_inertia_scale)
 {
     // rotation = rel.Multiply3x3(sRotation(mInertiaRotation)) (Mat44.inl:428-447)
     const M44 ir = rotation_translation(c.inertia_rot[0], c.inertia_rot[1], c.inertia_rot[2], c.inertia_rot[3], v3(0.0f, 0.0f, 0.0f));
@@ -847,9 +862,11 @@ __device__ __forceinline__ void init_dynamic(const ColliderDev &c, const M44 &in
     for (int j = 0; j < 3; ++j)
     {
         const V3 col = (rc[0] * comp(sc[0], j) + rc[1] * comp(sc[1], j)) + rc[2] * comp(sc[2], j);
-        D.inv_inertia[3 * j] = col.x; D.inv_inertia[3 * j + 1] = col.y; D.inv_inertia[3 * j + 2] = col.z;
+        // times mInvInertiaScale2 (SoftBodyMotionProperties.cpp:215)
+        D.inv_inertia[3 * j] = col.x * inv_inertia_scale; D.inv_inertia[3 * j + 1] = col.y * inv_inertia_scale;
+        D.inv_inertia[3 * j + 2] = col.z * inv_inertia_scale;
     }
-    D.inv_mass = c.inv_mass;
+    D.inv_mass = inv_mass_scale * c.inv_mass;
     const V3 lin = mul3x3(inv_com, ld3(c.lin_vel)), ang = mul3x3(inv_com, ld3(c.ang_vel));
     st3(D.lin, lin); st3(D.orig_lin, lin);
     st3(D.ang, ang); st3(D.orig_ang, ang);
@@ -861,7 +878,8 @@ __device__ __forceinline__ void init_dynamic(const ColliderDev &c, const M44 &in
 // restitution impulse shared between the vertex and the body, whose velocities the
 // next vertex in contact sees.  x: the vertex after the push out, v: its XPBD velocity.
 __device__ __forceinline__ void dynamic_contact(DynStateDev &D, V3 com_t, V3 x, V3 &v, V3 prev_v, V3 n, float projected_distance,
-                                                float w, float friction, float restitution, float dt, float restitution_threshold)
+                                                float w, float friction, float restitution, float dt, float restitution_threshold,
+                                                float inv_mass_scale1)
 {
     const V3 r2 = x - com_t;
     V3 lin = ld3(D.lin), ang = ld3(D.ang);
@@ -870,7 +888,7 @@ __device__ __forceinline__ void dynamic_contact(DynStateDev &D, V3 com_t, V3 x, 
     const V3 v_normal = n * dot(n, relative_velocity);
     const V3 v_tangential = relative_velocity - v_normal;
     const float v_tangential_length = len(v_tangential);
-    const float vertex_inv_mass = 1.0f * w;
+    const float vertex_inv_mass = inv_mass_scale1 * w;
     const M44 I = inv_inertia_matrix(D);
     const V3 r2_cross_n = cross(r2, n);
     const float w2 = D.inv_mass + dot(r2_cross_n, mul_point(I, r2_cross_n));
@@ -951,7 +969,7 @@ __device__ __forceinline__ void clear_dynamic(DynStateDev &D)
 // transform is rebuilt from com's quaternion and translation).  A dynamic collider
 // also initialises its entry of the body's DynStateDev row.
 __device__ __forceinline__ void make_leaf(const ColliderDev &c, const M44 &inv_com, float body_friction, float body_restitution, Leaf &out,
-                                          DynStateDev *dyn_row)
+                                          DynStateDev *dyn_row, float4 pair = make_float4(1.0f, 1.0f, 1.0f, 1.0f))
 {
     M44 ccom = rotation_translation(c.rot[0], c.rot[1], c.rot[2], c.rot[3], v3(c.pos[0], c.pos[1], c.pos[2]));
     M44 rel = mul(inv_com, ccom);
@@ -962,10 +980,11 @@ __device__ __forceinline__ void make_leaf(const ColliderDev &c, const M44 &inv_c
     out.type = c.type;
     out.dyn = -1;
     out.collider = -1;
+    out.inv_mass_scale1 = pair.x;
     if (c.motion == 2 && dyn_row)
     {
         out.dyn = c.dyn;
-        init_dynamic(c, inv_com, rel, dyn_row[c.dyn]);
+        init_dynamic(c, inv_com, rel, dyn_row[c.dyn], pair.y, pair.z);
     }
     out.friction = sqrtf(body_friction * c.friction);                                 // ContactConstraintManager.h:554
     out.restitution = body_restitution > c.restitution ? body_restitution : c.restitution; // ContactConstraintManager.h:555

@@ -203,7 +203,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                     // AABox::Overlaps (Geometry/AABox.h:164-167)
                     bool overlap = !(wmin.x > cd.bounds[3] || wmin.y > cd.bounds[4] || wmin.z > cd.bounds[5]
                                      || wmax.x < cd.bounds[0] || wmax.y < cd.bounds[1] || wmax.z < cd.bounds[2]);
-                    slots[ci] = overlap ? 1 : 0;
+                    slots[ci] = overlap && !pair_skipped(pair_settings(B.pair, ci), cd.sensor >= 0, cd.motion == 2) ? 1 : 0;
                 }
                 __syncthreads();
                 if (tid == 0)
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                 for (uint32_t ci = tid; ci < num_colliders; ci += kBlock)
                     if (slots[ci] >= 0)
                     {
-                        make_leaf(colliders[ci], inv_com, B.friction, B.restitution, leaves[slots[ci]], dyn);
+                        make_leaf(colliders[ci], inv_com, B.friction, B.restitution, leaves[slots[ci]], dyn, pair_settings(B.pair, ci));
                         leaves[slots[ci]].collider = (int32_t)ci;
                     }
                 // dynamic states are read by the in-order contact pass of another wavefront
@@ -471,8 +471,9 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                                 x = x + n * projected_distance;
                                 if (cs.dyn >= 0)
                                     dynamic_contact(dyn[cs.dyn], col3(cs.t, 3), x, v, prev_v, n, projected_distance, p.w,
-                                                    cs.friction, cs.restitution, dt_sub, restitution_threshold);
-                                else
+                                                    cs.friction, cs.restitution, dt_sub, restitution_threshold,
+                                                    cs.inv_mass_scale1);
+                                else if (cs.inv_mass_scale1 > 0.0f)
                                 {
                                     V3 v_normal = n * dot(n, v);
                                     V3 v_tangential = v - v_normal;

@@ -145,6 +145,8 @@ struct Body
     DevBuf<float4> x, v, prev, plane, report;
     DevBuf<int32_t> idx, contact;
     DevBuf<float4> rods;            // 2 per rod (the body keeps them; rod bodies never join a bundle)
+    DevBuf<float4> pair;            // per collider contact settings (xpbd_body_set_contact_settings)
+    uint32_t pair_n = 0;            // colliders the table was given for (0: no table)
     BodyDev dev{};
     BodyStateDev state{};
     bool skinned = false;           // has skin entries at dev.skin_base (one per vertex)
@@ -1563,6 +1565,9 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
     HIP_TRY(hipSetDevice(ctx->device));
     const uint32_t nc = (uint32_t)ctx->colliders.size();
     int rc;
+    for (size_t i = 0; i < ctx->bodies.size(); ++i)
+        if (ctx->bodies[i]->pair_n && ctx->bodies[i]->pair_n != nc)
+            return Fail(XPBD_E_INVALID, "body " + std::to_string(i) + " has contact settings for another collider list");
     if (ctx->skin_dirty)
     {
         // skin states go up before the layout, whose bundles point at them
@@ -2180,6 +2185,39 @@ int xpbd_body_sensor_contacts(xpbd_context *ctx, int32_t body, uint32_t num_coll
         const int32_t k = ctx->colliders[c].sensor;
         hits[c] = k >= 0 && (uint32_t)k < row.size() ? row[k] : 0;
     }
+    return XPBD_OK;
+}
+
+int xpbd_body_set_contact_settings(xpbd_context *ctx, int32_t body, uint32_t num_colliders, const float *settings)
+{
+    if (!ctx || body < 0 || body >= (int32_t)ctx->bodies.size())
+        return Fail(XPBD_E_INVALID, "invalid body");
+    Body &b = *ctx->bodies[body];
+    if (!settings)
+    {
+        b.pair_n = 0;
+        b.dev.pair = nullptr;
+        ctx->tables_dirty = true;
+        return XPBD_OK;
+    }
+    if (num_colliders != ctx->colliders.size())
+        return Fail(XPBD_E_INVALID, "num_colliders differs from the last xpbd_set_colliders");
+    for (uint32_t c = 0; c < num_colliders; ++c)
+    {
+        const float *q = settings + 4 * c;
+        if (!(q[0] >= 0.0f && q[1] >= 0.0f && q[2] >= 0.0f) || (q[3] != 0.0f && q[3] != 1.0f))
+            return Fail(XPBD_E_INVALID, "contact settings: scales must be >= 0 and accept 0 or 1");
+    }
+    int rc = xpbd_synchronize(ctx);
+    if (rc != XPBD_OK)
+        return rc;
+    std::vector<float4> row(std::max(1u, num_colliders), make_float4(1.0f, 1.0f, 1.0f, 1.0f));
+    for (uint32_t c = 0; c < num_colliders; ++c)
+        row[c] = make_float4(settings[4 * c], settings[4 * c + 1], settings[4 * c + 2], settings[4 * c + 3]);
+    HIP_TRY(b.pair.upload(row));
+    b.pair_n = num_colliders;
+    b.dev.pair = num_colliders ? b.pair.p : nullptr;
+    ctx->tables_dirty = true;
     return XPBD_OK;
 }
 

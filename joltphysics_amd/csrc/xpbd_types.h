@@ -81,6 +81,7 @@ struct BodyDev
     uint32_t num_sensors;
     float4 *rods;               // rod states, 2 per rod: rotation, then the angular velocity (between
                                 // steps) or the previous rotation (during a step); null without rods
+    const float4 *pair;         // per collider: contact settings of the pair (see pair_settings), or null
 };
 
 // Per-body dynamic state kept on the device between launches.

@@ -99,7 +99,7 @@ EXPORTED_SYMBOLS = (
     "xpbd_build_last_error", "xpbd_body_create",
     "xpbd_set_colliders", "xpbd_set_gravity", "xpbd_update", "xpbd_synchronize", "xpbd_body_read",
     "xpbd_body_write", "xpbd_body_state", "xpbd_body_velocity_deltas", "xpbd_body_contacts", "xpbd_body_sensor_contacts",
-    "xpbd_body_skin_vertices", "xpbd_body_rod_states",
+    "xpbd_body_skin_vertices", "xpbd_body_rod_states", "xpbd_body_set_contact_settings",
     "xpbd_body_set_skin_options", "xpbd_body_save_state", "xpbd_body_restore_state", "xpbd_profile_enable",
     "xpbd_profile_read",
     "xpbd_rollout_create", "xpbd_rollout_destroy", "xpbd_rollout_sizes", "xpbd_rollout_forward",
@@ -152,6 +152,7 @@ def load_library(path=LIB_PATH):
     lib.xpbd_body_contacts.argtypes = [vp, i32, vp, vp]
     lib.xpbd_body_sensor_contacts.argtypes = [vp, i32, u32, vp]
     lib.xpbd_body_rod_states.argtypes = [vp, i32, u32, vp, vp]
+    lib.xpbd_body_set_contact_settings.argtypes = [vp, i32, u32, vp]
     lib.xpbd_body_skin_vertices.argtypes = [vp, i32, vp, u32, i32]
     lib.xpbd_body_set_skin_options.argtypes = [vp, i32, i32, f]
     lib.xpbd_body_save_state.argtypes = [vp, i32, vp, C.c_size_t, C.POINTER(C.c_size_t)]
@@ -346,6 +347,16 @@ class Context:
         hits = np.zeros(max(n, 1), np.int32)
         _check(self.lib.xpbd_body_sensor_contacts(self.h, int(body), n, hits.ctypes.data))
         return hits[:n]
+
+    def set_contact_settings(self, body, settings=None):
+        """Per-collider contact settings of `body` (n, 4): mInvMassScale1, mInvMassScale2,
+        mInvInertiaScale2, accept (1 / 0), as a contact listener's OnSoftBodyContactValidate
+        decides them; None removes the table."""
+        if settings is None:
+            _check(self.lib.xpbd_body_set_contact_settings(self.h, int(body), 0, None))
+            return
+        t = np.ascontiguousarray(settings, np.float32).reshape(-1, 4)
+        _check(self.lib.xpbd_body_set_contact_settings(self.h, int(body), len(t), t.ctypes.data))
 
     def rod_states(self, body):
         """(rotations (n, 4) xyzw, angular velocities (n, 3)) of the Cosserat rods of `body`

@@ -127,6 +127,21 @@ struct ContactRecorder final : public SoftBodyContactListener, public PhysicsSte
 	std::mutex mMutex;
 	std::map<uint32, Record> mRecords;          // key: soft body BodyID index and sequence
 	std::map<uint32, int> mColliderIndex;       // collider BodyID -> collider index
+	// per soft body (BodyID) and collider index: {scale1, scale2, inertia scale2, accept}
+	std::map<uint32, std::vector<float>> mPairSettings;
+
+	SoftBodyValidateResult OnSoftBodyContactValidate(const Body &inSoftBody, const Body &inOtherBody, SoftBodyContactSettings &ioSettings) override
+	{
+		auto t = mPairSettings.find(inSoftBody.GetID().GetIndexAndSequenceNumber());
+		auto c = mColliderIndex.find(inOtherBody.GetID().GetIndexAndSequenceNumber());
+		if (t == mPairSettings.end() || c == mColliderIndex.end())
+			return SoftBodyValidateResult::AcceptContact;
+		const float *q = t->second.data() + 4 * c->second;
+		ioSettings.mInvMassScale1 = q[0];
+		ioSettings.mInvMassScale2 = q[1];
+		ioSettings.mInvInertiaScale2 = q[2];
+		return q[3] != 0.0f ? SoftBodyValidateResult::AcceptContact : SoftBodyValidateResult::RejectContact;
+	}
 
 	void OnStep(const PhysicsStepListenerContext &) override
 	{
@@ -844,6 +859,14 @@ int jo_body_skin_vertices(void *wp, int body, const float *joints, uint32 n, int
 			m[j].SetColumn4(c, Vec4(joints[16 * j + 4 * c], joints[16 * j + 4 * c + 1], joints[16 * j + 4 * c + 2], joints[16 * j + 4 * c + 3]));
 	mp->SkinVertices(b.GetCenterOfMassTransform(), m.data(), n, hard_skin_all != 0, *w->mTemp);
 	return 0;
+}
+
+// Contact settings of soft body `body` against each collider (n x {scale1, scale2, inertia
+// scale2, accept}), answered by the recording listener (install it first)
+void jo_world_set_contact_settings(void *wp, int body, const float *settings, int n)
+{
+	World *w = (World *)wp;
+	w->mRecorder->mPairSettings[w->mSoftBodies[body].GetIndexAndSequenceNumber()].assign(settings, settings + 4 * n);
 }
 
 // Rod rotations (n x 4) and angular velocities (n x 3) of soft body `body`

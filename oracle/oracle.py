@@ -110,6 +110,7 @@ class JoltReference:
             lib.jo_world_contacts.restype = _i
             lib.jo_body_skin_vertices.restype = _i
             lib.jo_body_skin_vertices.argtypes = [_vp, _i, _vp, _u32, _i, _i, _f]
+            lib.jo_world_set_contact_settings.argtypes = [_vp, _i, _vp, _i]
             lib.jo_body_rod_states.restype = _i
             lib.jo_body_rod_states.argtypes = [_vp, _i, _vp, _vp, _u32]
             lib.jo_body_save_state.restype = C.c_long
@@ -278,6 +279,12 @@ class JoltWorld:
         e = -1 if enable is None else int(bool(enable))
         if self.lib.jo_body_skin_vertices(self.h, body, _ptr(j), len(j), int(bool(hard_skin_all)), e, float(multiplier)) != 0:
             raise RuntimeError("SkinVertices failed")
+
+    def set_contact_settings(self, body, settings):
+        """OnSoftBodyContactValidate answers for soft body `body`: per collider {scale1, scale2,
+        inertia scale2, accept}; needs the recording listener (record_contacts)."""
+        t = np.ascontiguousarray(settings, np.float32).reshape(-1, 4)
+        self.lib.jo_world_set_contact_settings(self.h, body, _ptr(t), len(t))
 
     def rod_states(self, body):
         """(rotations (n, 4), angular velocities (n, 3)) of the rods of soft body `body`."""

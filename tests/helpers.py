@@ -124,7 +124,7 @@ def _take_motion(c, m):
     c.angular_velocity = m["angular_velocity"]
 
 
-def reference_custom_run(scene, frames_to_keep, variant="nofma"):
+def reference_custom_run(scene, frames_to_keep, variant="nofma", contact_settings=None):
     """The reference with SoftBodyMotionProperties::CustomUpdate(dt) once per frame for the
     (single) soft body: one collision step per frame, rigid bodies not simulated, dynamic
     colliders' velocities changed by UpdateRigidBodyVelocities.  Sets each collider's device
@@ -148,6 +148,9 @@ def reference_custom_run(scene, frames_to_keep, variant="nofma"):
     for b, s in zip(scene.bodies, settings):
         world.add_soft_body(s, b.position, b.rotation, b.params)
     world.optimize_broadphase()
+    if contact_settings is not None:
+        world.record_contacts()
+        world.set_contact_settings(0, contact_settings)
     out, vel = {}, {}
     for f in range(1, max(frames_to_keep) + 1):
         world.custom_update(0, scene.dt)
@@ -209,7 +212,7 @@ class RestatementRunner:
 
 
 class GpuRunner:
-    def __init__(self, scene, settings, bounds, device=0):
+    def __init__(self, scene, settings, bounds, device=0, contact_settings=None):
         from joltphysics_amd.xpbd import Context
         self.ctx = Context(device)
         self.ctx.set_gravity(scene.gravity)
@@ -217,6 +220,9 @@ class GpuRunner:
             c.bounds = tuple(float(x) for x in b)
         self.ids = [self.ctx.create_body(self.ctx.create_settings(s), b.position, b.rotation, b.params)
                     for b, s in zip(scene.bodies, settings)]
+        if contact_settings is not None:
+            self.ctx.set_colliders(scene.colliders)
+            self.ctx.set_contact_settings(self.ids[0], contact_settings)
 
     def set_motions(self, cols):
         self.ctx.set_colliders(cols)

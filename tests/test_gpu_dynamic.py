@@ -9,9 +9,11 @@ velocities of the call's start, as the C ABI documents), bit-exact.
 """
 from __future__ import annotations
 
+import numpy as np
 import pytest
 
 import scene_zoo
+from joltphysics_amd.records import MOTION_DYNAMIC
 from helpers import GpuRunner, RestatementRunner, custom_run, float_equal, reference_custom_run
 from oracle import have_reference
 
@@ -62,3 +64,32 @@ def test_two_bodies_on_dynamic_bodies_match_restatement():
     out, vel = custom_run(sc, settings, bounds, frames, run)
     run.close()
     _check(out, vel, want, want_vel, frames, len(sc.colliders), nbodies=2)
+
+
+@pytest.mark.skipif(not have_reference("nofma"), reason="reference library not built")
+def test_contact_validation_settings_bit_exact():
+    """Per-pair contact settings as a contact listener's OnSoftBodyContactValidate gives them
+    (SoftBodyMotionProperties.cpp:151-216, 763): a dynamic box with all three scales changed
+    (soft body mInvMassScale1 0.5, mInvMassScale2 3, mInvInertiaScale2 2), a rejected
+    dynamic sphere, and a static box with mInvMassScale1 0 (no interaction: skipped);
+    bit-exact against the reference's listener every frame, collider velocities included."""
+    sc = scene_zoo.cloth_on_dynamic_bodies()
+    nc = len(sc.colliders)
+    table = np.tile(np.float32([1.0, 1.0, 1.0, 1.0]), (nc, 1))
+    dyn = [k for k, c in enumerate(sc.colliders) if c.motion_type == MOTION_DYNAMIC]
+    static = [k for k, c in enumerate(sc.colliders) if c.motion_type != MOTION_DYNAMIC]
+    assert len(dyn) >= 2 and len(static) >= 1
+    table[dyn[0]] = (0.5, 3.0, 2.0, 1.0)
+    table[dyn[1]] = (1.0, 1.0, 1.0, 0.0)
+    table[static[0]] = (0.0, 1.0, 1.0, 1.0)
+    frames = (1, 4, sc.frames)
+    settings, bounds, ref, ref_vel = reference_custom_run(sc, frames, contact_settings=table)
+    run = GpuRunner(sc, settings, bounds, contact_settings=table)
+    out, vel = custom_run(sc, settings, bounds, frames, run)
+    run.close()
+    for f in frames:
+        assert float_equal(out[f][0]["pos"], ref[f][0]["pos"]), f"positions differ at frame {f}"
+        assert float_equal(out[f][0]["vel"], ref[f][0]["vel"]), f"velocities differ at frame {f}"
+        for k in range(nc):
+            assert float_equal(np.float32(vel[f][k][0]), np.float32(ref_vel[f][k][0])), f"collider {k} linear velocity, frame {f}"
+            assert float_equal(np.float32(vel[f][k][1]), np.float32(ref_vel[f][k][1])), f"collider {k} angular velocity, frame {f}"
