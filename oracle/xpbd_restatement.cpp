@@ -11,6 +11,11 @@
 //   F = double : the same algorithm in FP64, the "FP64 CPU re-run" used to state
 //                the FP32 error of both the reference and the GPU path.
 //
+// Not restated here (their parity is anchored on the reference library built under
+// oracle/_ref and on closed forms in the tests): skinned constraints, Cosserat rods,
+// per-pair contact settings of a contact listener, and sensors (skipped: they never
+// change the simulation).
+//
 // Every function cites the reference lines it restates.  Paths are relative to
 // the reference's Jolt/ directory.  The SSE semantics that fix the float
 // results (Math/Vec3.inl, Math/Mat44.inl) are restated in the small vector
