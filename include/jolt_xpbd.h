@@ -412,6 +412,20 @@ int xpbd_rollout_destroy(xpbd_rollout *rollout);
 int xpbd_rollout_sizes(xpbd_rollout *rollout, uint32_t *num_edges, uint32_t *num_vertices);
 /* float forward; stores the checkpoints the backward pass starts from. */
 int xpbd_rollout_forward(xpbd_rollout *rollout, const float *compliance, const float *velocity, float *positions, void *stream);
+/*
+ * xpbd_rollout_forward for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Same results and checkpoints as xpbd_rollout_forward (a backward pass may
+ * follow it); the forward pass's per-vertex velocity and previous-position state lives in
+ * a fixed device array of the library instead of the rollout's buffers, sized for
+ * XPBD_NOALLOC_MAX_VERTICES vertices (8 MiB, held on every GPU the library runs on until
+ * the process ends).  A rollout with more vertices fails with XPBD_E_CAPACITY.  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (other streams, other
+ * threads) race.  Allocates nothing.
+ */
+#define XPBD_NOALLOC_MAX_VERTICES 262144
+int xpbd_rollout_forward_noalloc(xpbd_rollout *rollout, const float *compliance, const float *velocity, float *positions,
+                                 void *stream);
 /* double forward of the same update (finite-difference checks); no checkpoints. */
 int xpbd_rollout_forward_f64(xpbd_rollout *rollout, const double *compliance, const double *velocity, double *positions,
                              void *stream);

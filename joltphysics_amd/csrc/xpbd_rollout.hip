@@ -23,6 +23,8 @@
 //   deltabar = ((c / L) cbar + (rest / L) n (n . cbar)) / s
 //   x0bar = x0'bar - deltabar,  x1bar = x1'bar + deltabar,
 //   alphabar += (cbar . corr) (-1 / (h^2 s)).
+#include <mutex>
+
 #include "xpbd_device.h"
 
 namespace xpbd {
@@ -393,6 +395,60 @@ hipError_t launch_rollout_forward_f32(const RolloutDev &R, const float *complian
     hipLaunchKernelGGL(rollout_forward_kernel<float>, dim3(R.num_bodies), dim3(kRolloutBlock), lds, stream, R, compliance,
                        velocity, positions);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- no-allocation forward
+// The float forward keeps each vertex's velocity and previous position in global memory
+// while it runs (R.vel, R.prev; read back only by the same pass).  The allocation-free
+// entry point takes them from this fixed array instead: velocities in the first
+// kNoAllocMaxVertices entries, previous positions in the second.  Every call on a GPU
+// shares that GPU's copy; the pass writes every entry it reads first, so nothing needs
+// clearing between calls.
+constexpr uint32_t kNoAllocMaxVertices = 262144;
+static __device__ __align__(256) float4 scratch[2u * kNoAllocMaxVertices];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+float4 *g_scratch[kMaxScratchDevices] = {};
+}
+
+uint32_t rollout_noalloc_max_vertices() { return kNoAllocMaxVertices; }
+
+// Looks up this device's copy of `scratch` (the host-side name is not a device address);
+// called once per device by context creation, with the device current.
+hipError_t rollout_scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxScratchDevices)
+        return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch[device])
+        return hipSuccess;
+    float4 *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess)
+        return e;
+    g_scratch[device] = p;
+    return hipSuccess;
+}
+
+// The float forward with R.vel / R.prev in this device's scratch array.  The caller has
+// checked R.total_nv against kNoAllocMaxVertices; allocates nothing.
+hipError_t launch_rollout_forward_f32_noalloc(int device, RolloutDev R, const float *compliance, const float *velocity,
+                                              float *positions, uint32_t lds, hipStream_t stream)
+{
+    if (device < 0 || device >= kMaxScratchDevices || R.total_nv > kNoAllocMaxVertices)
+        return hipErrorInvalidValue;
+    float4 *base;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mutex);
+        base = g_scratch[device];
+    }
+    if (!base)
+        return hipErrorNotInitialized;
+    R.vel = base;
+    R.prev = base + kNoAllocMaxVertices;
+    return launch_rollout_forward_f32(R, compliance, velocity, positions, lds, stream);
 }
 
 hipError_t launch_rollout_forward_f64(const RolloutDev &R, const double *compliance, const double *velocity, double *positions,

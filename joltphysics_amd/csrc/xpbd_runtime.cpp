@@ -36,6 +36,10 @@ hipError_t launch_rollout_forward_f64(const RolloutDev &R, const double *complia
                                       uint32_t lds, hipStream_t stream);
 hipError_t launch_rollout_backward(const RolloutDev &R, const float *grad_positions, float *grad_compliance, float *grad_velocity,
                                    uint32_t lds, hipStream_t stream);
+hipError_t rollout_scratch_setup(int device);
+uint32_t rollout_noalloc_max_vertices();
+hipError_t launch_rollout_forward_f32_noalloc(int device, RolloutDev R, const float *compliance, const float *velocity,
+                                              float *positions, uint32_t lds, hipStream_t stream);
 } // namespace xpbd
 
 using namespace xpbd;
@@ -1257,6 +1261,8 @@ int xpbd_context_create(int device, xpbd_context **out_ctx)
     }
     HIP_TRY(ctx->d_error.alloc(1));
     HIP_TRY(hipMemset(ctx->d_error.p, 0, sizeof(int32_t)));
+    // the device's copy of the allocation-free rollout scratch (xpbd_rollout_forward_noalloc)
+    HIP_TRY(rollout_scratch_setup(device));
     *out_ctx = ctx.release();
     return XPBD_OK;
 }
@@ -2452,6 +2458,19 @@ int xpbd_rollout_forward(xpbd_rollout *r, const float *compliance, const float *
         return Fail(XPBD_E_INVALID, "null argument");
     HIP_TRY(hipSetDevice(r->ctx->device));
     HIP_TRY(launch_rollout_forward_f32(r->dev, compliance, velocity, positions, r->lds_f32, StreamOf(r, stream)));
+    r->has_forward = true;
+    return XPBD_OK;
+}
+
+int xpbd_rollout_forward_noalloc(xpbd_rollout *r, const float *compliance, const float *velocity, float *positions, void *stream)
+{
+    if (!r || !compliance || !velocity || !positions)
+        return Fail(XPBD_E_INVALID, "null argument");
+    if (r->total_nv > rollout_noalloc_max_vertices())
+        return Fail(XPBD_E_CAPACITY, "rollout has " + std::to_string(r->total_nv) + " vertices; the allocation-free forward pass holds at most "
+                                         + std::to_string(rollout_noalloc_max_vertices()));
+    HIP_TRY(hipSetDevice(r->ctx->device));
+    HIP_TRY(launch_rollout_forward_f32_noalloc(r->ctx->device, r->dev, compliance, velocity, positions, r->lds_f32, StreamOf(r, stream)));
     r->has_forward = true;
     return XPBD_OK;
 }

@@ -103,7 +103,7 @@ EXPORTED_SYMBOLS = (
     "xpbd_body_set_skin_options", "xpbd_body_save_state", "xpbd_body_restore_state", "xpbd_profile_enable",
     "xpbd_profile_read",
     "xpbd_rollout_create", "xpbd_rollout_destroy", "xpbd_rollout_sizes", "xpbd_rollout_forward",
-    "xpbd_rollout_forward_f64", "xpbd_rollout_backward",
+    "xpbd_rollout_forward_f64", "xpbd_rollout_backward", "xpbd_rollout_forward_noalloc",
 )
 
 _lib = None
@@ -164,6 +164,7 @@ def load_library(path=LIB_PATH):
     lib.xpbd_rollout_sizes.argtypes = [vp, C.POINTER(u32), C.POINTER(u32)]
     lib.xpbd_rollout_forward.argtypes = [vp] * 5
     lib.xpbd_rollout_forward_f64.argtypes = [vp] * 5
+    lib.xpbd_rollout_forward_noalloc.argtypes = [vp] * 5
     lib.xpbd_rollout_backward.argtypes = [vp] * 5
     for name in EXPORTED_SYMBOLS:
         if name not in ("xpbd_last_error", "xpbd_context_stream", "xpbd_build_last_error"):
@@ -521,6 +522,11 @@ class Rollout:
 
     def forward(self, compliance_ptr, velocity_ptr, positions_ptr, stream=None):
         _check(self.lib.xpbd_rollout_forward(self.h, compliance_ptr, velocity_ptr, positions_ptr, stream))
+        self.generation += 1
+
+    def forward_noalloc(self, compliance_ptr, velocity_ptr, positions_ptr, stream=None):
+        """forward() through the allocation-free entry point (xpbd_rollout_forward_noalloc)."""
+        _check(self.lib.xpbd_rollout_forward_noalloc(self.h, compliance_ptr, velocity_ptr, positions_ptr, stream))
         self.generation += 1
 
     def forward_f64(self, compliance_ptr, velocity_ptr, positions_ptr, stream=None):
