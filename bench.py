@@ -131,6 +131,32 @@ def measured_traffic(workload):
     return None, None
 
 
+def parity_check(ctx, ids, scene, settings, frame):
+    """Compares the GPU state of the batched scene's 64 check bodies after `frame` frames
+    from the initial state with digests of the reference's state (tests/golden/
+    batched2048_sample_digests.json, written from the reference built with FMA
+    contraction off)."""
+    path = os.path.join(ROOT, "tests", "golden", "batched2048_sample_digests.json")
+    if not os.path.exists(path):
+        return {"checked": False, "reason": "no reference digests for this workload"}
+    with open(path) as f:
+        fx = json.load(f)
+    if fx["scene"] != scene.name or fx["bodies"] != len(scene.bodies):
+        return {"checked": False, "reason": "no reference digests for this workload"}
+    if str(frame) not in fx["frames"]:
+        return {"checked": False, "reason": f"the reference digests cover frames {min(map(int, fx['frames']))}-"
+                                            f"{max(map(int, fx['frames']))}, the warm-up ends at frame {frame}"}
+    sample = fx["sample"]
+    if sample != scenes.batched_check_sample(scene, settings):
+        return {"checked": False, "reason": "the workload's check sample differs from the digests'"}
+    want = fx["frames"][str(frame)]
+    got = [scenes.state_digest(ctx.read(ids[k])) for k in sample]
+    mismatched = [k for k, g, w in zip(sample, got, want) if g != w]
+    return {"checked": True, "frame": frame, "bodies": len(sample), "bit_exact": not mismatched,
+            "mismatched_bodies": mismatched,
+            "against": "reference PhysicsSystem::Update (FMA off): tests/golden/batched2048_sample_digests.json"}
+
+
 def differentiation_run(warmup=1, repeats=3):
     """fwd+bwd of the differentiation workload (64 cloths 64 x 64, 60 frames x 5 iterations,
     checkpoints every 10 frames): the float forward pass and the adjoint through the
@@ -178,6 +204,68 @@ def differentiation_run(warmup=1, repeats=3):
     return out
 
 
+def large_body_run(steps=3, warmup=1):
+    """The large single body (64^3-cell tet block, 274,625 vertices, edge + volume + 270,529 LRA
+    constraints among 512 static boxes, 8 collision steps x 10 iterations) on a team of
+    workgroups: exact (the non-parallel group in the reference's order, bit-exact) and with
+    the re-coloured non-parallel group.  Times `steps` frames after `warmup` with hipEvents on
+    the library stream; the coloured run's positions are compared with the exact run's after
+    the same frames."""
+    scene = scenes.config5_block(frames=warmup + steps)
+    t0 = time.time()
+    settings = [build_settings(b.mesh) for b in scene.bodies]
+    for b, s in zip(scene.bodies, settings):
+        if b.post is not None:
+            b.post(s)
+    log(f"large body {scene.name}: {len(settings[0].vertices)} vertices, {settings[0].num_constraints()} constraints, "
+        f"settings built in {time.time() - t0:.1f}s")
+    solves = scenes.solves_per_frame(scene, settings)
+    bytes_frame = algorithmic_bytes_per_frame(scene, settings)
+    out = {"workload": scene.name, "vertices": int(len(settings[0].vertices)),
+           "constraints": int(settings[0].num_constraints()), "solves_per_frame": int(solves),
+           "collision_steps": scene.collision_steps, "iterations": scene.bodies[0].params.num_iterations,
+           "colliders": len(scene.colliders), "frames_timed": steps}
+    final = {}
+    for mode in ("exact", "coloured"):
+        ctx = Context(0)
+        ctx.set_gravity(scene.gravity)
+        ctx.set_colliders(scene.colliders)
+        st = ctx.create_settings(settings[0])
+        body = ctx.create_body(st, scene.bodies[0].position, scene.bodies[0].rotation, scene.bodies[0].params)
+        if mode == "coloured":
+            ctx.set_tail_mode(body, True)
+        ctx.update(scene.dt, scene.collision_steps, warmup)
+        ctx.synchronize()
+        ctx.profile_enable(True)
+        t0 = time.perf_counter()
+        ctx.update(scene.dt, scene.collision_steps, steps)
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        n_launch, kernel_ms = ctx.profile_read()
+        ctx.profile_enable(False)
+        kind, _, _, team = ctx.placement(body)
+        stats = st.stats()
+        final[mode] = ctx.read(body)["pos"]
+        ctx.close()
+        achieved = bytes_frame * steps / (kernel_ms / 1000.0) / 1e9
+        out[mode] = {"ms_per_frame": round(1000.0 * wall / steps, 3), "solves_per_s": solves * steps / wall,
+                     "placement": kind, "team_workgroups": team,
+                     "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(achieved / HBM_PEAK_GBS, 4),
+                                  "algorithmic_bytes_per_launch": bytes_frame * steps,
+                                  "launch_ms": round(kernel_ms / max(n_launch, 1), 3)}}
+        if mode == "coloured":
+            out[mode]["colours"] = stats["tail_colours"]
+        log(f"large body, {mode}: {out[mode]['ms_per_frame']} ms per frame ({kind}, {team} workgroups)")
+    out["parallel_windows"], out["tail_levels"], out["tail_windows"] = (stats["parallel_windows"], stats["tail_levels"],
+                                                                        stats["tail_windows"])
+    d = np.abs(final["coloured"].astype(np.float64) - final["exact"])
+    out["coloured_vs_exact_after_frames"] = {"frames": warmup + steps, "max_abs_m": float(d.max()),
+                                             "mean_abs_m": float(d.mean())}
+    out["speedup_coloured"] = round(out["exact"]["ms_per_frame"] / out["coloured"]["ms_per_frame"], 2)
+    return scene, settings, out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +275,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-diff", action="store_true", help="skip the fwd+bwd differentiation measurement")
+    ap.add_argument("--no-large", action="store_true", help="skip the large single body measurement")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (the known-size reference "
                          "for HBM byte counters) and exit")
@@ -216,14 +305,17 @@ def main():
     ctx.set_gravity(scene.gravity)
     ctx.set_colliders(scene.colliders)
     t0 = time.time()
+    ids = []
     for b, s in zip(scene.bodies, settings):
         st = ctx.create_settings(s)
-        ctx.create_body(st, b.position, b.rotation, b.params)
+        ids.append(ctx.create_body(st, b.position, b.rotation, b.params))
     log(f"uploaded in {time.time() - t0:.1f}s")
 
     if args.warmup > 0:
         ctx.update(scene.dt, scene.collision_steps, args.warmup)
     ctx.synchronize()
+    parity = parity_check(ctx, ids, scene, settings, args.warmup)
+    log(f"parity after the warm-up: {parity}")
     try:
         import torch
         if torch.cuda.is_available():
@@ -255,7 +347,15 @@ def main():
                 "launch_ms": round(kernel_ms / max(n_launch, 1), 3)}
     ctx.close()
     diff = None if args.no_diff else differentiation_run()
+    large = None
+    if not args.no_large:
+        large_scene, large_settings, large = large_body_run()
     cpu = None if args.no_cpu_baseline else cpu_baseline(scene, settings, args.cpu_seconds)
+    if large is not None and not args.no_cpu_baseline:
+        large_cpu = cpu_baseline(large_scene, large_settings, target_seconds=8.0)
+        large["cpu_baseline"] = large_cpu
+        if large_cpu:
+            large["gpu_over_cpu"] = {m: round(large[m]["solves_per_s"] / large_cpu["value"], 2) for m in ("exact", "coloured")}
     out = {
         "metric": "constraint-solves/sec", "value": value, "unit": "constraint-solves/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -266,8 +366,8 @@ def main():
                    "constraints": int(sum(s.num_constraints() for s in settings)),
                    "solves_per_frame": int(solves), "collision_steps": scene.collision_steps,
                    "iterations": int(scene.bodies[0].params.num_iterations), "dt": scene.dt,
-                   "parity": "bit-exact with the reference (FMA off)"},
-        "roofline": roofline, "cpu_baseline": cpu, "differentiation": diff,
+                   "parity": parity},
+        "roofline": roofline, "cpu_baseline": cpu, "differentiation": diff, "large_body": large,
     }
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 2)

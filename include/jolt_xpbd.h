@@ -122,6 +122,11 @@ int xpbd_settings_release(xpbd_settings *settings);
  * non-parallel group, out[4]=total constraints, out[5]=1 if the body keeps all
  * positions in LDS, out[6]/out[7]=parallel / non-parallel windows. */
 int xpbd_settings_stats(const xpbd_settings *settings, uint32_t out[8]);
+/* Team statistics of a streamed body's settings (0 for LDS-resident ones): out[0]=parallel-group
+ * windows, out[1]=windows of the non-parallel group, out[2]=1 if that group can be coloured
+ * (it holds only volumes, bends, edges and LRAs), out[3]=its colours.  No reference
+ * counterpart (a property of this build's schedule). */
+int xpbd_settings_team_stats(const xpbd_settings *settings, uint32_t out[4]);
 
 /*
  * Host-side settings builder: for callers that have a mesh but not the
@@ -236,6 +241,10 @@ typedef struct xpbd_collider_desc {
      * vertex penetrates it (DetermineSensorCollisions, SoftBodyMotionProperties.cpp:267-289;
      * xpbd_body_sensor_contacts).  The reference considers sensors only with a contact listener. */
     int32_t is_sensor;
+    /* The caller's identifier of the collider body (e.g. BodyID::GetIndexAndSequenceNumber()).
+     * A table of xpbd_body_set_contact_settings holds only while every index of the collider
+     * list keeps its identifier, shape type, motion type and sensor flag. */
+    uint32_t user_id;
 } xpbd_collider_desc;
 
 int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *colliders, uint32_t num_colliders);
@@ -250,6 +259,35 @@ int xpbd_set_gravity(xpbd_context *ctx, const float gravity[3]);
  * update of JobSoftBodyFinalize.  Asynchronous on the context stream.
  */
 int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t frames);
+
+/*
+ * Large bodies.  A streamed body (over 8192 vertices) whose parallel update groups fill at
+ * least `windows` LDS windows runs on a team of workgroups, one per window: the parallel
+ * groups' windows run concurrently (SoftBodyMotionProperties.cpp:1057-1115 spreads the same
+ * groups over its job threads), the non-parallel group follows on one of them, and the
+ * per-vertex phases are split.  Results are bit-identical to the one-workgroup path.
+ * Bodies with rods or pressure, and contexts with dynamic or sensor colliders, keep the
+ * one-workgroup path.  Default 3; 0 never forms teams.  No reference counterpart (a
+ * scheduling choice).
+ */
+int xpbd_context_set_team_min_windows(xpbd_context *ctx, uint32_t windows);
+
+/*
+ * How a body runs its non-parallel update group (the reference's last group, applied in
+ * order by one thread, SoftBodyMotionProperties.cpp:1105-1115):
+ *   XPBD_TAIL_EXACT     in the reference's order (default; bit-exact);
+ *   XPBD_TAIL_COLOURED  re-coloured: within each constraint type, in the reference's order,
+ *                       each constraint takes the lowest colour no earlier constraint holds on
+ *                       a shared vertex either of them writes; the body runs on a team and
+ *                       projects one colour at a time across all members.  A different
+ *                       Gauss-Seidel order: NOT bit-exact with the reference (a throughput mode
+ *                       for bodies whose non-parallel group is long).  Streamed bodies whose
+ *                       non-parallel group holds only volumes, bends, edges and LRAs.
+ * No reference counterpart.
+ */
+#define XPBD_TAIL_EXACT 0
+#define XPBD_TAIL_COLOURED 1
+int xpbd_body_set_tail_mode(xpbd_context *ctx, int32_t body, int32_t mode);
 int xpbd_synchronize(xpbd_context *ctx);
 
 /*
@@ -266,6 +304,14 @@ int xpbd_body_write(xpbd_context *ctx, int32_t body, const float *pos, const flo
 /* Body summary: out[0..2] linear velocity, out[3..5] angular velocity (world),
  * out[6..11] local bounds, out[12] active flag, out[13] sleep timer. */
 int xpbd_body_state(xpbd_context *ctx, int32_t body, float out[14]);
+/* Where the last xpbd_update ran a body: out[0] = XPBD_PLACED_BUNDLE (LDS-resident, sharing a
+ * workgroup), XPBD_PLACED_STREAMED (one workgroup, windows from HBM) or XPBD_PLACED_TEAM;
+ * out[1] = threads per workgroup; out[2] = bundle or team index (-1 streamed); out[3] =
+ * bodies in the bundle, or workgroups in the team.  No reference counterpart. */
+#define XPBD_PLACED_BUNDLE 0
+#define XPBD_PLACED_STREAMED 1
+#define XPBD_PLACED_TEAM 2
+int xpbd_body_placement(xpbd_context *ctx, int32_t body, int32_t out[4]);
 
 /*
  * Per-pair contact settings of a body: what SoftBodyContactListener::
@@ -277,8 +323,11 @@ int xpbd_body_state(xpbd_context *ctx, int32_t body, float out[14]);
  * dynamic collider, scale2 0) is skipped as the reference skips it.  The table holds for
  * every step of the following updates (the adapter asks the listener once per pair);
  * changing a pair's mIsSensor is not supported.  settings NULL removes the table (no
- * listener: every pair with unit scales).  A later xpbd_set_colliders with another
- * collider count makes xpbd_update fail until the table is set again.
+ * listener: every pair with unit scales; sensors are still detected and reported, so pass
+ * sensor colliders only when a contact listener exists, as the reference considers sensors
+ * only with one, SoftBodyMotionProperties.cpp:154-158).  A later xpbd_set_colliders whose
+ * list differs in count, or at any index in user_id, shape type, motion type or sensor flag,
+ * makes xpbd_update fail until the table is set again.
  */
 int xpbd_body_set_contact_settings(xpbd_context *ctx, int32_t body, uint32_t num_colliders, const float *settings);
 

@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIBNAME = "libjolt_xpbd.so"
 
-HIP_SOURCES = ["xpbd_kernels.hip", "xpbd_bundle.hip", "xpbd_rollout.hip", "xpbd_runtime.cpp", "settings_builder.cpp"]
+HIP_SOURCES = ["xpbd_kernels.hip", "xpbd_bundle.hip", "xpbd_team.hip", "xpbd_rollout.hip", "xpbd_runtime.cpp", "settings_builder.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",

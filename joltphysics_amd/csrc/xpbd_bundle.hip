@@ -632,6 +632,8 @@ __device__ __forceinline__ void bundle_update(
                     e[3].w = 0.0f;
                 }
             }
+            // every wavefront has read the deltas before they are cleared for the next step
+            __syncthreads();
             for (uint32_t b = tid; b < nb; b += BLOCK)
             {
                 loc[b].delta[0] = 0.0f; loc[b].delta[1] = 0.0f; loc[b].delta[2] = 0.0f;

@@ -28,6 +28,11 @@ hipError_t launch_update(bool resident, bool staged, bool rods, const BodyDev *b
                          uint32_t num_blocks, const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves,
                          uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
                          int32_t *error_flag, hipStream_t stream);
+uint32_t team_lds_bytes(uint32_t window_capacity, uint32_t max_leaves, uint32_t nc);
+hipError_t launch_team(const BodyDev *bodies, BodyStateDev *states, const TeamDev *teams, const uint2 *block_map,
+                       uint32_t num_blocks, const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves,
+                       uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
+                       int32_t *error_flag, hipStream_t stream);
 uint32_t rollout_forward_lds_bytes(uint32_t nv, bool f64);
 uint32_t rollout_backward_lds_bytes(uint32_t nv, uint32_t ne);
 hipError_t launch_rollout_forward_f32(const RolloutDev &R, const float *compliance, const float *velocity, float *positions,
@@ -139,6 +144,14 @@ struct xpbd_settings
     bool uniform_mass = false;                  // every vertex has the same inverse mass
     float inv_mass = 0.0f;                      // that inverse mass
     uint32_t num_edges = 0, num_bends = 0, num_volumes = 0, num_lras = 0;
+    // streamed bodies: the parallel-group windows (host copy, for splitting them over a team)
+    // and the coloured non-parallel group (empty when it holds skinned or rod constraints)
+    std::vector<WindowDev> par_windows;
+    DevBuf<uint4> colour_recs;
+    DevBuf<uint32_t> colour_begin;
+    uint32_t num_colours = 0;
+    uint32_t colour_records = 0;
+    bool colourable = false;
 };
 
 struct Body
@@ -151,6 +164,8 @@ struct Body
     DevBuf<float4> rods;            // 2 per rod (the body keeps them; rod bodies never join a bundle)
     DevBuf<float4> pair;            // per collider contact settings (xpbd_body_set_contact_settings)
     uint32_t pair_n = 0;            // colliders the table was given for (0: no table)
+    std::vector<uint32_t> pair_sig; // the collider list's signature when the table was given
+    int32_t tail_mode = 0;          // XPBD_TAIL_EXACT or XPBD_TAIL_COLOURED
     BodyDev dev{};
     BodyStateDev state{};
     bool skinned = false;           // has skin entries at dev.skin_base (one per vertex)
@@ -184,6 +199,7 @@ struct xpbd_context
     bool tables_dirty = true;
     bool states_on_device = false;
     std::vector<ColliderDev> colliders;
+    std::vector<uint32_t> collider_sig;    // per collider: type | motion << 8 | sensor << 16, user id
     DevBuf<ColliderDev> d_colliders;
     DevBuf<float4> d_hull_vec;      // convex hulls: planes, then points, per hull
     DevBuf<uint32_t> d_hull_idx;    // convex hulls: face first, face count, vertex index, per hull
@@ -214,6 +230,17 @@ struct xpbd_context
     uint32_t layout_colliders = 0;
     std::vector<std::unique_ptr<BundleHost>> bundles;
     std::vector<uint32_t> streamed;
+    // large streamed bodies run by teams of workgroups (xpbd_team.hip)
+    uint32_t team_min_windows = 3;  // a streamed body with this many parallel-group windows gets a team (0: never)
+    std::vector<TeamDev> teams;
+    std::vector<uint2> team_blocks; // per workgroup of the team launch: team, member
+    DevBuf<TeamDev> d_teams;
+    DevBuf<uint2> d_team_blocks;
+    DevBuf<uint32_t> d_team_split;
+    DevBuf<uint32_t> d_team_sync;   // one 256-byte line per team
+    DevBuf<float> d_team_red;
+    DevBuf<unsigned char> d_team_leaves;   // per team member, one leaf per collider
+    uint32_t team_lds = 0, team_max_leaves = 0;
     struct BundleLaunch
     {
         bool staged = false;    // records staged in LDS (256-thread launches)
@@ -451,6 +478,89 @@ void ProgramBuilder::AppendLevel(const std::vector<std::pair<const GroupLevels *
         ibucket[t].push_back(it.index);
     }
     AppendTypedLevel(bucket, recs, true, ibucket, &rec_index);
+}
+
+// The coloured non-parallel group (team bodies in the coloured mode).  Within each run of
+// one constraint type, in the reference's order (ProcessGroup applies the types one after
+// the other), a constraint takes the lowest colour that no earlier constraint of the run
+// holds on a vertex they share where either of them writes (an LRA reads its vertex 0);
+// the constraints of one colour can then be projected in any order.  Colours run in order.
+// Returns false when the group holds constraints other than volumes, bends, edges and LRAs.
+bool ColourGroup(const std::vector<Item> &seq, uint32_t nv, std::vector<uint4> &recs, std::vector<uint32_t> &begin)
+{
+    constexpr uint32_t kBanks = 4;      // up to 256 colours per run
+    std::vector<uint64_t> wmask((size_t)nv * kBanks, 0), rmask((size_t)nv * kBanks, 0);
+    std::vector<uint32_t> touched;
+    recs.clear();
+    begin.clear();
+    size_t k = 0;
+    while (k < seq.size())
+    {
+        const uint32_t type = seq[k].type;
+        if (type >= kItemSkin)
+            return false;
+        size_t e = k;
+        while (e < seq.size() && seq[e].type == type)
+            ++e;
+        const bool lra = type == kTypeLRA;
+        std::vector<std::vector<uint32_t>> colour;
+        for (size_t i = k; i < e; ++i)
+        {
+            const Item &it = seq[i];
+            uint32_t c = ~0u;
+            for (uint32_t bank = 0; bank < kBanks && c == ~0u; ++bank)
+            {
+                uint64_t forbidden = 0;
+                for (uint32_t j = 0; j < it.nv; ++j)
+                {
+                    const size_t m = (size_t)it.v[j] * kBanks + bank;
+                    forbidden |= wmask[m];
+                    if (!lra || j == 1)
+                        forbidden |= rmask[m];
+                }
+                if (~forbidden)
+                    c = bank * 64u + (uint32_t)__builtin_ctzll(~forbidden);
+            }
+            if (c == ~0u)
+                return false;
+            const uint64_t bit = 1ull << (c & 63u);
+            for (uint32_t j = 0; j < it.nv; ++j)
+            {
+                const size_t m = (size_t)it.v[j] * kBanks + c / 64u;
+                if (!lra || j == 1)
+                    wmask[m] |= bit;
+                else
+                    rmask[m] |= bit;
+                touched.push_back(it.v[j]);
+            }
+            if (colour.size() <= c)
+                colour.resize(c + 1);
+            colour[c].push_back((uint32_t)i);
+        }
+        for (const auto &members : colour)
+        {
+            begin.push_back((uint32_t)(recs.size() / 2));
+            for (uint32_t i : members)
+            {
+                const Item &it = seq[i];
+                float p0 = it.p0, p1 = it.p1;
+                uint4 b;
+                b.x = it.type;
+                memcpy(&b.y, &p0, 4);
+                memcpy(&b.z, &p1, 4);
+                b.w = 0u;
+                recs.push_back(make_uint4(it.v[0], it.nv > 1 ? it.v[1] : 0u, it.nv > 2 ? it.v[2] : 0u, it.nv > 3 ? it.v[3] : 0u));
+                recs.push_back(b);
+            }
+        }
+        for (uint32_t v : touched)
+            for (uint32_t bank = 0; bank < kBanks; ++bank)
+                wmask[(size_t)v * kBanks + bank] = rmask[(size_t)v * kBanks + bank] = 0;
+        touched.clear();
+        k = e;
+    }
+    begin.push_back((uint32_t)(recs.size() / 2));
+    return true;
 }
 
 int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
@@ -698,6 +808,18 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
             L = L_end;
         } while (L < ntl);
         s.dev.num_tail_windows = ntw;
+        s.par_windows.assign(pb.windows.begin(), pb.windows.begin() + npw);
+        std::vector<uint4> crecs;
+        std::vector<uint32_t> cbegin;
+        if (ColourGroup(tail.seq, nv, crecs, cbegin))
+        {
+            hipError_t e;
+            if ((e = s.colour_recs.upload(crecs)) != hipSuccess || (e = s.colour_begin.upload(cbegin)) != hipSuccess)
+                return Fail(XPBD_E_DEVICE, std::string("settings upload: ") + hipGetErrorString(e));
+            s.num_colours = (uint32_t)cbegin.size() - 1u;
+            s.colour_records = (uint32_t)(crecs.size() / 2);
+            s.colourable = true;
+        }
     }
 
     s.num_edges = d.num_edges;
@@ -911,6 +1033,116 @@ double StreamedMicroseconds(const Body &b)
            * LevelPassMicroseconds(1024u);
 }
 
+// Teams.  A streamed body without rods, pressure, dynamic or sensor colliders runs on a
+// team of workgroups when it has at least team_min_windows parallel-group windows, or
+// when it asks for the coloured non-parallel group.  One member per window (the
+// windows' levels then run concurrently), at most kMaxTeamBlocks workgroups in all, so
+// that every member of every team holds a CU at once; each member takes a contiguous
+// run of windows of about equal cost.
+constexpr uint32_t kMaxTeamBlocks = 224;
+constexpr uint32_t kMaxTeamSize = 64;
+
+int BuildTeams(xpbd_context *ctx)
+{
+    ctx->teams.clear();
+    ctx->team_blocks.clear();
+    std::vector<uint32_t> keep, members;
+    for (uint32_t i : ctx->streamed)
+    {
+        const Body &b = *ctx->bodies[i];
+        const xpbd_settings &st = *b.settings;
+        const bool eligible = !st.dev.resident && st.rods.empty() && !(b.dev.pressure > 0.0f && st.num_faces > 0)
+                              && ctx->num_dynamic == 0 && ctx->num_sensors == 0;
+        const bool coloured = b.tail_mode == XPBD_TAIL_COLOURED;
+        if (coloured && !(eligible && st.colourable))
+            return Fail(XPBD_E_UNSUPPORTED, "body " + std::to_string(i) + ": the coloured non-parallel group needs a streamed body "
+                                            "(over 8192 vertices) without rods, pressure, skinned constraints in that group, "
+                                            "dynamic or sensor colliders");
+        if (eligible && (coloured || (ctx->team_min_windows > 0 && st.dev.num_par_windows >= ctx->team_min_windows)))
+        {
+            ctx->teams.push_back(TeamDev{});
+            ctx->teams.back().body = i;
+            members.push_back(std::max(1u, std::min(st.dev.num_par_windows, kMaxTeamSize)));
+        }
+        else
+            keep.push_back(i);
+    }
+    ctx->streamed = keep;
+    if (ctx->teams.empty())
+        return XPBD_OK;
+    uint32_t total = 0;
+    for (uint32_t m : members) total += m;
+    while (total > kMaxTeamBlocks)
+    {
+        uint32_t &largest = *std::max_element(members.begin(), members.end());
+        if (largest == 1)
+            return Fail(XPBD_E_CAPACITY, "more large bodies than teams can hold at once");
+        --largest;
+        --total;
+    }
+    const uint32_t nc = (uint32_t)ctx->colliders.size();
+    std::vector<uint32_t> split;
+    uint32_t capacity = 0;
+    for (size_t k = 0; k < ctx->teams.size(); ++k)
+    {
+        TeamDev &t = ctx->teams[k];
+        const Body &b = *ctx->bodies[t.body];
+        const std::vector<WindowDev> &pw = b.settings->par_windows;
+        t.size = members[k];
+        capacity = std::max(capacity, b.settings->dev.window_capacity);
+        // contiguous runs of windows, cost = levels + gather + scatter
+        double all = 0.0;
+        for (const WindowDev &w : pw) all += w.nlev + 2.0;
+        const size_t first = split.size();
+        split.push_back(0);
+        double acc = 0.0;
+        uint32_t w = 0;
+        for (uint32_t m = 1; m < t.size; ++m)
+        {
+            const double goal = all * m / t.size;
+            while (w < pw.size() && acc + 0.5 * (pw[w].nlev + 2.0) < goal)
+                acc += pw[w++].nlev + 2.0;
+            split.push_back(w);
+        }
+        split.push_back((uint32_t)pw.size());
+        t.par_split = reinterpret_cast<const uint32_t *>(first);       // offset until uploaded
+        for (uint32_t m = 0; m < t.size; ++m)
+            ctx->team_blocks.push_back(make_uint2((uint32_t)k, m));
+        if (b.tail_mode == XPBD_TAIL_COLOURED)
+        {
+            t.colour_recs = b.settings->colour_recs.p;
+            t.colour_begin = b.settings->colour_begin.p;
+            t.num_colours = b.settings->num_colours;
+        }
+    }
+    const uint32_t no_leaves = team_lds_bytes(capacity, 0, nc);
+    if (no_leaves > kMaxLdsBytes)
+        return Fail(XPBD_E_CAPACITY, "a team member's window needs more LDS than a workgroup has");
+    ctx->team_max_leaves = std::min(nc, (kMaxLdsBytes - no_leaves) / bundle_leaf_bytes());
+    ctx->team_lds = team_lds_bytes(capacity, ctx->team_max_leaves, nc);
+    HIP_TRY(ctx->d_team_split.upload(split));
+    HIP_TRY(ctx->d_team_sync.alloc(64 * ctx->teams.size()));
+    size_t red = 0;
+    for (const TeamDev &t : ctx->teams) red += 32u * (t.size + 1u);
+    HIP_TRY(ctx->d_team_red.alloc(red));
+    HIP_TRY(ctx->d_team_leaves.alloc((size_t)std::max(1u, nc) * ctx->team_blocks.size() * bundle_leaf_bytes()));
+    size_t member0 = 0;
+    red = 0;
+    for (size_t k = 0; k < ctx->teams.size(); ++k)
+    {
+        TeamDev &t = ctx->teams[k];
+        t.par_split = ctx->d_team_split.p + reinterpret_cast<size_t>(t.par_split);
+        t.sync = ctx->d_team_sync.p + 64 * k;
+        t.red = ctx->d_team_red.p + red;
+        red += 32u * (t.size + 1u);
+        t.leaves = reinterpret_cast<Leaf *>(ctx->d_team_leaves.p + member0 * std::max(1u, nc) * bundle_leaf_bytes());
+        member0 += t.size;
+    }
+    HIP_TRY(ctx->d_teams.upload(ctx->teams));
+    HIP_TRY(ctx->d_team_blocks.upload(ctx->team_blocks));
+    return XPBD_OK;
+}
+
 int BuildLayout(xpbd_context *ctx)
 {
     const uint32_t nc = (uint32_t)ctx->colliders.size();
@@ -939,6 +1171,8 @@ int BuildLayout(xpbd_context *ctx)
             critical = std::max(critical, StreamedMicroseconds(b));
         }
     }
+    if (int rc = BuildTeams(ctx))
+        return rc;
     // same iteration count and inverse-mass class together, deepest level sequences first
     std::stable_sort(resident.begin(), resident.end(), [&](uint32_t a, uint32_t b) {
         const Body &A = *ctx->bodies[a], &B = *ctx->bodies[b];
@@ -1329,6 +1563,47 @@ int xpbd_settings_stats(const xpbd_settings *s, uint32_t out[8])
     return XPBD_OK;
 }
 
+int xpbd_settings_team_stats(const xpbd_settings *s, uint32_t out[4])
+{
+    if (!s || !out)
+        return Fail(XPBD_E_INVALID, "null argument");
+    out[0] = s->dev.resident ? 0u : s->dev.num_par_windows;
+    out[1] = s->dev.resident ? 0u : s->dev.num_tail_windows;
+    out[2] = s->colourable ? 1u : 0u;
+    out[3] = s->num_colours;
+    return XPBD_OK;
+}
+
+int xpbd_context_set_team_min_windows(xpbd_context *ctx, uint32_t windows)
+{
+    if (!ctx)
+        return Fail(XPBD_E_INVALID, "null context");
+    if (ctx->team_min_windows != windows)
+    {
+        ctx->team_min_windows = windows;
+        ctx->layout_dirty = true;
+    }
+    return XPBD_OK;
+}
+
+int xpbd_body_set_tail_mode(xpbd_context *ctx, int32_t body, int32_t mode)
+{
+    if (!ctx || body < 0 || body >= (int32_t)ctx->bodies.size())
+        return Fail(XPBD_E_INVALID, "invalid body");
+    if (mode != XPBD_TAIL_EXACT && mode != XPBD_TAIL_COLOURED)
+        return Fail(XPBD_E_INVALID, "tail mode must be XPBD_TAIL_EXACT or XPBD_TAIL_COLOURED");
+    Body &b = *ctx->bodies[body];
+    if (mode == XPBD_TAIL_COLOURED && (b.settings->dev.resident || !b.settings->colourable))
+        return Fail(XPBD_E_UNSUPPORTED, "the coloured non-parallel group needs a streamed body (over 8192 vertices) whose "
+                                        "non-parallel group holds only volumes, bends, edges and LRAs");
+    if (b.tail_mode != mode)
+    {
+        b.tail_mode = mode;
+        ctx->layout_dirty = true;
+    }
+    return XPBD_OK;
+}
+
 int xpbd_body_create(xpbd_context *ctx, const xpbd_body_desc *d, int32_t *out_body)
 {
     if (!ctx || !d || !out_body || !d->settings)
@@ -1543,9 +1818,17 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
     HIP_TRY(ctx->d_hulls.upload(hulls));
     for (size_t k = 0; k < hull_offsets.size(); ++k)
         v[hull_offsets[k].collider].hull = ctx->d_hulls.p + k;
-    if (v.size() != ctx->colliders.size())
+    if (v.size() != ctx->colliders.size() || (num_dynamic > 0) != (ctx->num_dynamic > 0) || (num_sensors > 0) != (ctx->num_sensors > 0))
         ctx->layout_dirty = true;
     ctx->colliders = v;
+    // per collider: shape type, motion type, sensor flag and the caller's identifier; a
+    // contact settings table holds for the list it was given for
+    ctx->collider_sig.resize(2 * (size_t)n);
+    for (uint32_t i = 0; i < n; ++i)
+    {
+        ctx->collider_sig[2 * i] = (uint32_t)c[i].type | ((uint32_t)c[i].motion_type << 8) | (c[i].is_sensor ? 1u << 16 : 0u);
+        ctx->collider_sig[2 * i + 1] = c[i].user_id;
+    }
     ctx->num_dynamic = num_dynamic;
     ctx->num_sensors = num_sensors;
     HIP_TRY(ctx->d_colliders.upload(v));
@@ -1572,7 +1855,7 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
     const uint32_t nc = (uint32_t)ctx->colliders.size();
     int rc;
     for (size_t i = 0; i < ctx->bodies.size(); ++i)
-        if (ctx->bodies[i]->pair_n && ctx->bodies[i]->pair_n != nc)
+        if (ctx->bodies[i]->pair_n && (ctx->bodies[i]->pair_n != nc || ctx->bodies[i]->pair_sig != ctx->collider_sig))
             return Fail(XPBD_E_INVALID, "body " + std::to_string(i) + " has contact settings for another collider list");
     if (ctx->skin_dirty)
     {
@@ -1739,24 +2022,42 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
     hipStream_t streams[xpbd_context::kMaxAux + 1] = { ctx->stream };
     for (int a = 0; a < num_aux; ++a) streams[a + 1] = ctx->aux[a];
     double queued[xpbd_context::kMaxAux + 1] = {};
-    uint32_t offset = 0;
-    for (const Launch &l : launches)
-    {
-        HIP_TRY(launch_update(l.resident, l.staged, l.rods, ctx->d_bodies.p, ctx->d_states.p, ctx->d_list.p + offset, (uint32_t)l.bodies.size(),
-                              ctx->d_colliders.p, nc, l.max_leaves, l.lds, gx, gy, gz, dt, collision_steps, frames, ctx->d_error.p, ctx->stream));
-        offset += (uint32_t)l.bodies.size();
-        double longest = 0.0;
-        for (uint32_t bi : l.bodies)
-            longest = std::max(longest, StreamedMicroseconds(*ctx->bodies[bi]));
-        queued[0] += longest;
-    }
-    for (const auto &Lp : ctx->bundle_launches)
-    {
-        const auto &L = *Lp;
+    auto least_queued = [&]() {
         int best = 0;
         for (int q = 1; q <= num_aux; ++q)
             if (queued[q] < queued[best])
                 best = q;
+        return best;
+    };
+    // teams first, on the context stream: their members need a CU each at once, which
+    // they get before the other launches take the CUs
+    if (!ctx->teams.empty())
+    {
+        HIP_TRY(hipMemsetAsync(ctx->d_team_sync.p, 0, sizeof(uint32_t) * ctx->d_team_sync.n, ctx->stream));
+        HIP_TRY(launch_team(ctx->d_bodies.p, ctx->d_states.p, ctx->d_teams.p, ctx->d_team_blocks.p, (uint32_t)ctx->team_blocks.size(),
+                            ctx->d_colliders.p, nc, ctx->team_max_leaves, ctx->team_lds, gx, gy, gz, dt, collision_steps, frames,
+                            ctx->d_error.p, ctx->stream));
+        double longest = 0.0;
+        for (const TeamDev &t : ctx->teams)
+            longest = std::max(longest, StreamedMicroseconds(*ctx->bodies[t.body]));
+        queued[0] += longest;
+    }
+    uint32_t offset = 0;
+    for (const Launch &l : launches)
+    {
+        const int q = ctx->teams.empty() ? 0 : least_queued();
+        HIP_TRY(launch_update(l.resident, l.staged, l.rods, ctx->d_bodies.p, ctx->d_states.p, ctx->d_list.p + offset, (uint32_t)l.bodies.size(),
+                              ctx->d_colliders.p, nc, l.max_leaves, l.lds, gx, gy, gz, dt, collision_steps, frames, ctx->d_error.p, streams[q]));
+        offset += (uint32_t)l.bodies.size();
+        double longest = 0.0;
+        for (uint32_t bi : l.bodies)
+            longest = std::max(longest, StreamedMicroseconds(*ctx->bodies[bi]));
+        queued[q] += longest;
+    }
+    for (const auto &Lp : ctx->bundle_launches)
+    {
+        const auto &L = *Lp;
+        const int best = least_queued();
         queued[best] += std::max(L.cost, L.work / 256.0);
         HIP_TRY(launch_bundles(L.staged, L.block, L.uniform, L.table.p, L.count, ctx->d_bodies.p, ctx->d_states.p, ctx->d_colliders.p, nc,
                                L.lds, gx, gy, gz, dt, collision_steps, frames, streams[best]));
@@ -1792,6 +2093,9 @@ int xpbd_synchronize(xpbd_context *ctx)
     if (err)
     {
         HIP_TRY(hipMemset(ctx->d_error.p, 0, sizeof(int32_t)));
+        if (err & 4)
+            return Fail(XPBD_E_DEVICE, "a team of workgroups waited too long at a barrier (its members did not all run at once); "
+                                       "the update's results are invalid");
         return Fail(XPBD_E_CAPACITY, "a body collided with more colliders than the leaf capacity");
     }
     return XPBD_OK;
@@ -1878,6 +2182,33 @@ int xpbd_body_state(xpbd_context *ctx, int32_t body, float out[14])
     memcpy(out + 9, s.bmax, 12);
     out[12] = (float)s.active;
     out[13] = s.sleep_timer;
+    return XPBD_OK;
+}
+
+int xpbd_body_placement(xpbd_context *ctx, int32_t body, int32_t out[4])
+{
+    if (!ctx || body < 0 || body >= (int32_t)ctx->bodies.size() || !out)
+        return Fail(XPBD_E_INVALID, "invalid body");
+    if (ctx->layout_dirty || ctx->layout_colliders != (uint32_t)ctx->colliders.size())
+        return Fail(XPBD_E_INVALID, "the layout is made by the next xpbd_update");
+    out[0] = XPBD_PLACED_STREAMED; out[1] = 1024; out[2] = -1; out[3] = 1;
+    for (size_t k = 0; k < ctx->teams.size(); ++k)
+        if (ctx->teams[k].body == (uint32_t)body)
+        {
+            out[0] = XPBD_PLACED_TEAM; out[2] = (int32_t)k; out[3] = (int32_t)ctx->teams[k].size;
+            return XPBD_OK;
+        }
+    for (const auto &Lp : ctx->bundle_launches)
+        for (const BundleDev &d : Lp->host)
+            for (size_t k = 0; k < ctx->bundles.size(); ++k)
+                if (ctx->bundles[k]->dev.recs == d.recs)
+                    for (uint32_t bi : ctx->bundles[k]->bodies)
+                        if (bi == (uint32_t)body)
+                        {
+                            out[0] = XPBD_PLACED_BUNDLE; out[1] = (int32_t)Lp->block; out[2] = (int32_t)k;
+                            out[3] = (int32_t)ctx->bundles[k]->bodies.size();
+                            return XPBD_OK;
+                        }
     return XPBD_OK;
 }
 
@@ -2222,6 +2553,7 @@ int xpbd_body_set_contact_settings(xpbd_context *ctx, int32_t body, uint32_t num
         row[c] = make_float4(settings[4 * c], settings[4 * c + 1], settings[4 * c + 2], settings[4 * c + 3]);
     HIP_TRY(b.pair.upload(row));
     b.pair_n = num_colliders;
+    b.pair_sig = ctx->collider_sig;
     b.dev.pair = num_colliders ? b.pair.p : nullptr;
     ctx->tables_dirty = true;
     return XPBD_OK;

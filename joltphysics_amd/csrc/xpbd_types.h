@@ -84,6 +84,23 @@ struct BodyDev
     const float4 *pair;         // per collider: contact settings of the pair (see pair_settings), or null
 };
 
+// A large body run by a team of workgroups (xpbd_team.hip).  Member m takes the
+// parallel-group windows [par_split[m], par_split[m + 1]) and a slice of the vertices.
+struct TeamDev
+{
+    uint32_t body;              // index into the context's body table
+    uint32_t size;              // members
+    const uint32_t *par_split;  // size + 1 window offsets
+    uint32_t *sync;             // team barrier counter (zeroed before every launch)
+    float *red;                 // 32 floats per member (partial reductions), then 32 for member 0's result
+    // coloured non-parallel group (null: the group runs in the reference's order on member 0):
+    // 2 uint4 per record, {v0, v1, v2, v3} and {type, p0, p1, 0}
+    const uint4 *colour_recs;
+    const uint32_t *colour_begin;   // num_colours + 1 record offsets
+    uint32_t num_colours;
+    struct Leaf *leaves;        // per member: num_colliders leaves, used when the colliding list overflows LDS
+};
+
 // Per-body dynamic state kept on the device between launches.
 struct BodyStateDev
 {

@@ -203,6 +203,7 @@ class Collider:
     linear_velocity: tuple = (0.0, 0.0, 0.0)
     angular_velocity: tuple = (0.0, 0.0, 0.0)
     is_sensor: bool = False     # Body::IsSensor(): detects vertices, never collides
+    user_id: int | None = None  # caller's identifier (BodyID); None: the object's identity
 
     def param_array(self):
         if self.type == SHAPE_CONVEX_HULL:

@@ -13,6 +13,7 @@ machines.
 """
 from __future__ import annotations
 
+import hashlib
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -363,6 +364,33 @@ def config3_batched(num_bodies=2048, seed=3, frames=600):
     plane.bounds = collider_world_bounds(plane)
     return Scene("batched2048", bodies, [plane], gravity=(0.0, 0.0, -9.81), dt=1.0 / 60.0,
                  collision_steps=2, frames=frames)
+
+
+def batched_check_sample(scene, settings):
+    """64 bodies of the batched scene covering every size: the 12 largest cloths (streamed,
+    above the LDS limit), 24 cloths and 18 cubes evenly spread over their sorted sizes
+    (single-body workgroups and bundles of both launch classes) and 10 icospheres of every
+    subdivision (pressure)."""
+    kinds = {"cloth": [], "cube": [], "ico": []}
+    for i, b in enumerate(scene.bodies):
+        kinds[b.mesh[0]].append(i)
+
+    def spread(ids, n, skip=0):
+        ids = sorted(ids, key=lambda i: (-len(settings[i].vertices), i))[skip:]
+        return [ids[int(k * (len(ids) - 1) / max(n - 1, 1))] for k in range(n)]
+
+    cloths = sorted(kinds["cloth"], key=lambda i: (-len(settings[i].vertices), i))
+    pick = cloths[:12] + spread(kinds["cloth"], 24, skip=12) + spread(kinds["cube"], 18) + spread(kinds["ico"], 10)
+    return sorted(set(pick))
+
+
+def state_digest(r):
+    """SHA-256 of a body's float32 positions, velocities and body position (bit patterns):
+    the form in which reference trajectories of the batched scene are kept."""
+    h = hashlib.sha256()
+    for k in ("pos", "vel", "body_pos"):
+        h.update(np.ascontiguousarray(r[k], np.float32).tobytes())
+    return h.hexdigest()
 
 
 def _uniform_compliance(c):

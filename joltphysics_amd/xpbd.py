@@ -66,7 +66,7 @@ class _ColliderDesc(C.Structure):
                 ("world_bounds", C.c_float * 6), ("hull", C.POINTER(_ConvexHull)),
                 ("motion_type", C.c_int32), ("inv_mass", C.c_float), ("inv_inertia_diagonal", C.c_float * 3),
                 ("inertia_rotation", C.c_float * 4), ("linear_velocity", C.c_float * 3),
-                ("angular_velocity", C.c_float * 3), ("is_sensor", C.c_int32)]
+                ("angular_velocity", C.c_float * 3), ("is_sensor", C.c_int32), ("user_id", C.c_uint32)]
 
 
 class _RolloutDesc(C.Structure):
@@ -94,11 +94,12 @@ class _BuildDesc(C.Structure):
 # Exported symbols of the C ABI, in declaration order of include/jolt_xpbd.h.
 EXPORTED_SYMBOLS = (
     "xpbd_last_error", "xpbd_context_create", "xpbd_context_destroy", "xpbd_context_stream",
-    "xpbd_settings_create", "xpbd_settings_release", "xpbd_settings_stats",
+    "xpbd_settings_create", "xpbd_settings_release", "xpbd_settings_stats", "xpbd_settings_team_stats",
     "xpbd_settings_build", "xpbd_built_counts", "xpbd_built_read", "xpbd_built_free",
     "xpbd_build_last_error", "xpbd_body_create",
-    "xpbd_set_colliders", "xpbd_set_gravity", "xpbd_update", "xpbd_synchronize", "xpbd_body_read",
-    "xpbd_body_write", "xpbd_body_state", "xpbd_body_velocity_deltas", "xpbd_body_contacts", "xpbd_body_sensor_contacts",
+    "xpbd_set_colliders", "xpbd_set_gravity", "xpbd_update", "xpbd_context_set_team_min_windows",
+    "xpbd_body_set_tail_mode", "xpbd_synchronize", "xpbd_body_read",
+    "xpbd_body_write", "xpbd_body_state", "xpbd_body_placement", "xpbd_body_velocity_deltas", "xpbd_body_contacts", "xpbd_body_sensor_contacts",
     "xpbd_body_skin_vertices", "xpbd_body_rod_states", "xpbd_body_set_contact_settings",
     "xpbd_body_set_skin_options", "xpbd_body_save_state", "xpbd_body_restore_state", "xpbd_profile_enable",
     "xpbd_profile_read",
@@ -135,6 +136,10 @@ def load_library(path=LIB_PATH):
     lib.xpbd_settings_create.argtypes = [vp, C.POINTER(_SettingsDesc), C.POINTER(vp)]
     lib.xpbd_settings_release.argtypes = [vp]
     lib.xpbd_settings_stats.argtypes = [vp, C.POINTER(u32)]
+    lib.xpbd_settings_team_stats.argtypes = [vp, C.POINTER(u32)]
+    lib.xpbd_context_set_team_min_windows.argtypes = [vp, u32]
+    lib.xpbd_body_set_tail_mode.argtypes = [vp, i32, i32]
+    lib.xpbd_body_placement.argtypes = [vp, i32, vp]
     lib.xpbd_settings_build.argtypes = [C.POINTER(_BuildDesc), C.POINTER(vp)]
     lib.xpbd_built_counts.argtypes = [vp, C.POINTER(u32)]
     lib.xpbd_built_read.argtypes = [vp] * 8
@@ -228,9 +233,12 @@ class Settings:
     def stats(self):
         out = (C.c_uint32 * 8)()
         _check(self.ctx.lib.xpbd_settings_stats(self.handle, out))
+        team = (C.c_uint32 * 4)()
+        _check(self.ctx.lib.xpbd_settings_team_stats(self.handle, team))
         return dict(parallel_groups=out[0], max_group_levels=out[1], tail_levels=out[2],
                     tail_constraints=out[3], constraints=out[4], resident=bool(out[5]),
-                    parallel_windows=out[6], tail_windows=out[7])
+                    parallel_windows=out[6], tail_windows=out[7], tail_colourable=bool(team[2]),
+                    tail_colours=team[3])
 
 
 class Context:
@@ -299,6 +307,8 @@ class Context:
                 arr[i].world_bounds[k] = float(c.bounds[k])
             arr[i].motion_type = int(c.motion_type)
             arr[i].is_sensor = int(bool(c.is_sensor))
+            # the collider's identity: its own id, else the Python object's
+            arr[i].user_id = (int(c.user_id) if c.user_id is not None else id(c)) & 0xffffffff
             if c.motion_type == MOTION_DYNAMIC:
                 arr[i].inv_mass = float(c.inv_mass)
                 for k in range(3):
@@ -453,6 +463,22 @@ class Context:
         pos = None if pos is None else np.ascontiguousarray(pos, np.float32)
         vel = None if vel is None else np.ascontiguousarray(vel, np.float32)
         _check(self.lib.xpbd_body_write(self.h, body, _ptr(pos), _ptr(vel)))
+
+    def set_team_min_windows(self, windows):
+        """Streamed bodies with at least `windows` parallel-group windows run on a team of
+        workgroups (0: never)."""
+        _check(self.lib.xpbd_context_set_team_min_windows(self.h, int(windows)))
+
+    def set_tail_mode(self, body, coloured):
+        """The non-parallel group in the reference's order (False) or re-coloured (True)."""
+        _check(self.lib.xpbd_body_set_tail_mode(self.h, body, 1 if coloured else 0))
+
+    def placement(self, body):
+        """(kind, threads per workgroup, bundle / team index, bodies in the bundle or team
+        size) of the last update; kind 'bundle', 'streamed' or 'team'."""
+        out = np.zeros(4, np.int32)
+        _check(self.lib.xpbd_body_placement(self.h, body, _ptr(out)))
+        return ("bundle", "streamed", "team")[int(out[0])], int(out[1]), int(out[2]), int(out[3])
 
     def body_state(self, body):
         out = np.zeros(14, np.float32)

@@ -77,17 +77,25 @@ def float_equal(a, b):
     return a.shape == b.shape and bool(np.all(a == b))
 
 
-def reference_run(scene, frames_to_keep, variant="nofma", num_threads=0, record_contacts=False):
-    """Builds settings with the reference's own code and runs PhysicsSystem::Update.
+def reference_run(scene, frames_to_keep, variant="nofma", num_threads=0, record_contacts=False, settings=None,
+                  bodies=None):
+    """Builds settings with the reference's own code (unless `settings` are given: one per
+    body of the scene) and runs PhysicsSystem::Update on the scene's bodies, or only on
+    those listed in `bodies` (soft bodies never interact, so a subset evolves as it does
+    in the whole scene).
 
-    Returns (settings_list, collider_bounds[n,6], {frame: [read(body)...]})."""
+    Returns (settings_list, collider_bounds[n,6], {frame: [read(body)...]}), the reads in
+    the order of `bodies`."""
     from oracle import JoltReference, JoltWorld
     ref = JoltReference(variant)
-    settings = [ref.build_settings(b.mesh) for b in scene.bodies]
-    for b, s in zip(scene.bodies, settings):
-        if b.post is not None:
-            b.post(s)
-    world = JoltWorld(ref, scene.gravity, num_threads=num_threads)
+    if settings is None:
+        settings = [ref.build_settings(b.mesh) for b in scene.bodies]
+        for b, s in zip(scene.bodies, settings):
+            if b.post is not None:
+                b.post(s)
+    chosen = list(range(len(scene.bodies))) if bodies is None else list(bodies)
+    world = JoltWorld(ref, scene.gravity, max_bodies=max(4096, len(chosen) + len(scene.colliders) + 16),
+                      num_threads=num_threads)
     ids = [world.add_collider(c) for c in scene.colliders]
     bounds = np.array([world.collider_bounds(i) for i in ids], np.float32).reshape(-1, 6)
     if record_contacts:
@@ -97,15 +105,16 @@ def reference_run(scene, frames_to_keep, variant="nofma", num_threads=0, record_
         c.device_params, c.com = world.collider_device(i)
         if c.type == SHAPE_CONVEX_HULL:
             c.hull = world.collider_hull(i)
-    for b, s in zip(scene.bodies, settings):
-        world.add_soft_body(s, b.position, b.rotation, b.params)
+    for k in chosen:
+        b = scene.bodies[k]
+        world.add_soft_body(settings[k], b.position, b.rotation, b.params)
     world.optimize_broadphase()
     out = {}
     done = 0
     for f in sorted(frames_to_keep):
         world.update(scene.dt, scene.collision_steps, f - done)
         done = f
-        out[f] = [world.read(i) for i in range(len(scene.bodies))]
+        out[f] = [world.read(i) for i in range(len(chosen))]
         for i, r in enumerate(out[f]):
             r["rod_rot"], r["rod_ang"] = world.rod_states(i)
         if record_contacts:
@@ -234,7 +243,9 @@ class GpuRunner:
         return self.ctx.velocity_deltas(self.ids[body])
 
     def read(self, body):
-        return self.ctx.read(self.ids[body])
+        r = self.ctx.read(self.ids[body])
+        r["contact_collider"], r["contact_point"] = self.ctx.contacts(self.ids[body])
+        return r
 
     def close(self):
         self.ctx.close()

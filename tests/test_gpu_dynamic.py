@@ -67,29 +67,45 @@ def test_two_bodies_on_dynamic_bodies_match_restatement():
 
 
 @pytest.mark.skipif(not have_reference("nofma"), reason="reference library not built")
-def test_contact_validation_settings_bit_exact():
+@pytest.mark.parametrize("name", list(scene_zoo.CUSTOM))
+def test_contact_validation_settings_bit_exact(name):
     """Per-pair contact settings as a contact listener's OnSoftBodyContactValidate gives them
-    (SoftBodyMotionProperties.cpp:151-216, 763): a dynamic box with all three scales changed
-    (soft body mInvMassScale1 0.5, mInvMassScale2 3, mInvInertiaScale2 2), a rejected
+    (SoftBodyMotionProperties.cpp:151-216, 763, 798): a dynamic box with all three scales
+    changed (soft body mInvMassScale1 0.5, mInvMassScale2 3, mInvInertiaScale2 2), a rejected
     dynamic sphere, and a static box with mInvMassScale1 0 (no interaction: skipped);
-    bit-exact against the reference's listener every frame, collider velocities included."""
-    sc = scene_zoo.cloth_on_dynamic_bodies()
+    bit-exact against the reference's listener every checked frame, the colliders' linear and
+    angular velocities included, LDS-resident and streamed.  A run with unit settings shows
+    that the cloth reaches the rejected sphere and the skipped box, so both branches act."""
+    sc = scene_zoo.CUSTOM[name]()
     nc = len(sc.colliders)
-    table = np.tile(np.float32([1.0, 1.0, 1.0, 1.0]), (nc, 1))
     dyn = [k for k, c in enumerate(sc.colliders) if c.motion_type == MOTION_DYNAMIC]
     static = [k for k, c in enumerate(sc.colliders) if c.motion_type != MOTION_DYNAMIC]
     assert len(dyn) >= 2 and len(static) >= 1
+    frames = tuple(range(1, sc.frames + 1))
+    print(f"scene {name}: contact settings, frames {frames[-1]}", flush=True)
+
+    unit = np.tile(np.float32([1.0, 1.0, 1.0, 1.0]), (nc, 1))
+    settings, bounds, _, _ = reference_custom_run(sc, (1,))
+    run = GpuRunner(sc, settings, bounds, contact_settings=unit)
+    out, _ = custom_run(sc, settings, bounds, frames, run)
+    run.close()
+    touched = set()
+    for f in frames:
+        touched |= set(int(k) for k in np.unique(out[f][0]["contact_collider"]) if k >= 0)
+    print(f"  colliders touched with unit settings: {sorted(touched)}", flush=True)
+    assert dyn[0] in touched and dyn[1] in touched and static[0] in touched, touched
+
+    sc = scene_zoo.CUSTOM[name]()
+    table = unit.copy()
     table[dyn[0]] = (0.5, 3.0, 2.0, 1.0)
     table[dyn[1]] = (1.0, 1.0, 1.0, 0.0)
     table[static[0]] = (0.0, 1.0, 1.0, 1.0)
-    frames = (1, 4, sc.frames)
     settings, bounds, ref, ref_vel = reference_custom_run(sc, frames, contact_settings=table)
     run = GpuRunner(sc, settings, bounds, contact_settings=table)
     out, vel = custom_run(sc, settings, bounds, frames, run)
     run.close()
+    _check(out, vel, ref, ref_vel, frames, nc)
     for f in frames:
-        assert float_equal(out[f][0]["pos"], ref[f][0]["pos"]), f"positions differ at frame {f}"
-        assert float_equal(out[f][0]["vel"], ref[f][0]["vel"]), f"velocities differ at frame {f}"
-        for k in range(nc):
-            assert float_equal(np.float32(vel[f][k][0]), np.float32(ref_vel[f][k][0])), f"collider {k} linear velocity, frame {f}"
-            assert float_equal(np.float32(vel[f][k][1]), np.float32(ref_vel[f][k][1])), f"collider {k} angular velocity, frame {f}"
+        hit = out[f][0]["contact_collider"]
+        assert not np.isin(hit, [dyn[1], static[0]]).any(), f"a rejected or skipped collider reports contact, frame {f}"
+    assert any(ref_vel[f][dyn[0]] != ref_vel[1][dyn[0]] for f in frames), "the cloth never reached the scaled box"
