@@ -122,6 +122,25 @@ int xpbd_settings_release(xpbd_settings *settings);
  * non-parallel group, out[4]=total constraints, out[5]=1 if the body keeps all
  * positions in LDS, out[6]/out[7]=parallel / non-parallel windows. */
 int xpbd_settings_stats(const xpbd_settings *settings, uint32_t out[8]);
+/*
+ * The level programs of settings built on the host alone, as xpbd_settings_create would
+ * upload them (no device; for checks of the schedule).  xpbd_schedule_build returns settings
+ * usable only with xpbd_schedule_sizes / xpbd_schedule_read / xpbd_settings_release.
+ * sizes: out[0] records (4 x u32 each, packed as in xpbd_types.h), out[1] level offsets,
+ * out[2] windows (4 x u32: vertex list begin, count (0: identity), first level, levels),
+ * out[3] parallel-group windows (the first ones), out[4] vertex list entries, out[5]
+ * coloured records (8 x u32: v0-v3, type, p0, p1, 0), out[6] colour offsets, out[7] 1 if
+ * resident, out[8] / out[9] records and level offsets of the identity program (body vertex
+ * numbers; the program bundles merge), empty for bodies of 16384 vertices or more.
+ * rec_index: the constraint index (in its type's array) of every record, ~0 for padding.
+ * No reference counterpart.
+ */
+int xpbd_schedule_build(const xpbd_settings_desc *desc, xpbd_settings **out_settings);
+int xpbd_schedule_sizes(const xpbd_settings *settings, uint32_t out[10]);
+int xpbd_schedule_read(const xpbd_settings *settings, uint32_t *recs, uint32_t *rec_index, uint32_t *lvl, uint32_t *windows,
+                       uint32_t *vlist, uint32_t *colour_recs, uint32_t *colour_begin, uint32_t *identity_recs,
+                       uint32_t *identity_rec_index, uint32_t *identity_level_begin);
+
 /* Team statistics of a streamed body's settings (0 for LDS-resident ones): out[0]=parallel-group
  * windows, out[1]=windows of the non-parallel group, out[2]=1 if that group can be coloured
  * (it holds only volumes, bends, edges and LRAs), out[3]=its colours.  No reference

@@ -152,6 +152,15 @@ struct xpbd_settings
     uint32_t num_colours = 0;
     uint32_t colour_records = 0;
     bool colourable = false;
+    // settings built on the host only (xpbd_schedule_build): the programs the device would get
+    struct HostSchedule
+    {
+        std::vector<uint4> recs;
+        std::vector<uint32_t> rec_index, lvl, vlist, colour_begin;
+        std::vector<WindowDev> windows;
+        std::vector<uint4> colour_recs;
+    };
+    std::unique_ptr<HostSchedule> host_schedule;
 };
 
 struct Body
@@ -814,7 +823,12 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
         if (ColourGroup(tail.seq, nv, crecs, cbegin))
         {
             hipError_t e;
-            if ((e = s.colour_recs.upload(crecs)) != hipSuccess || (e = s.colour_begin.upload(cbegin)) != hipSuccess)
+            if (s.host_schedule)
+            {
+                s.host_schedule->colour_recs = crecs;
+                s.host_schedule->colour_begin = cbegin;
+            }
+            else if ((e = s.colour_recs.upload(crecs)) != hipSuccess || (e = s.colour_begin.upload(cbegin)) != hipSuccess)
                 return Fail(XPBD_E_DEVICE, std::string("settings upload: ") + hipGetErrorString(e));
             s.num_colours = (uint32_t)cbegin.size() - 1u;
             s.colour_records = (uint32_t)(crecs.size() / 2);
@@ -879,6 +893,19 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
     }
 
     hipError_t e;
+    if (s.host_schedule)
+    {
+        s.host_schedule->recs = pb.recs;
+        s.host_schedule->rec_index = pb.rec_index;
+        s.host_schedule->lvl = pb.lvl;
+        s.host_schedule->windows = pb.windows;
+        s.host_schedule->vlist = pb.vlist;
+        s.dev.num_records = (uint32_t)pb.recs.size();
+        s.dev.num_levels_total = (uint32_t)pb.lvl.size();
+        s.dev.resident = resident ? 1u : 0u;
+        s.dev.window_capacity = window_capacity;
+        return XPBD_OK;
+    }
     if ((e = s.recs.upload(pb.recs)) != hipSuccess || (e = s.lvl.upload(pb.lvl)) != hipSuccess
         || (e = s.windows.upload(pb.windows)) != hipSuccess || (e = s.vlist.upload(pb.vlist)) != hipSuccess
         || (e = s.faces.upload(f3)) != hipSuccess || (e = s.vf_begin.upload(vf_begin)) != hipSuccess
@@ -1543,6 +1570,59 @@ int xpbd_settings_create(xpbd_context *ctx, const xpbd_settings_desc *desc, xpbd
     if (rc != XPBD_OK)
         return rc;
     *out_settings = s.release();
+    return XPBD_OK;
+}
+
+int xpbd_schedule_build(const xpbd_settings_desc *desc, xpbd_settings **out)
+{
+    if (!desc || !out)
+        return Fail(XPBD_E_INVALID, "null argument");
+    *out = nullptr;
+    std::unique_ptr<xpbd_settings> s(new xpbd_settings);
+    s->host_schedule.reset(new xpbd_settings::HostSchedule);
+    int rc = BuildSchedule(*desc, *s);
+    if (rc != XPBD_OK)
+        return rc;
+    *out = s.release();
+    return XPBD_OK;
+}
+
+int xpbd_schedule_sizes(const xpbd_settings *s, uint32_t out[10])
+{
+    if (!s || !out || !s->host_schedule)
+        return Fail(XPBD_E_INVALID, "not a host schedule");
+    const auto &h = *s->host_schedule;
+    out[0] = (uint32_t)h.recs.size();
+    out[1] = (uint32_t)h.lvl.size();
+    out[2] = (uint32_t)h.windows.size();
+    out[3] = s->dev.num_par_windows;
+    out[4] = (uint32_t)h.vlist.size();
+    out[5] = (uint32_t)(h.colour_recs.size() / 2);
+    out[6] = (uint32_t)h.colour_begin.size();
+    out[7] = s->dev.resident;
+    out[8] = (uint32_t)s->host_recs.size();
+    out[9] = (uint32_t)s->host_level_begin.size();
+    return XPBD_OK;
+}
+
+int xpbd_schedule_read(const xpbd_settings *s, uint32_t *recs, uint32_t *rec_index, uint32_t *lvl, uint32_t *windows,
+                       uint32_t *vlist, uint32_t *colour_recs, uint32_t *colour_begin, uint32_t *identity_recs,
+                       uint32_t *identity_rec_index, uint32_t *identity_level_begin)
+{
+    if (!s || !s->host_schedule)
+        return Fail(XPBD_E_INVALID, "not a host schedule");
+    const auto &h = *s->host_schedule;
+    auto put = [](uint32_t *dst, const void *src, size_t bytes) { if (dst && bytes) memcpy(dst, src, bytes); };
+    put(recs, h.recs.data(), h.recs.size() * sizeof(uint4));
+    put(rec_index, h.rec_index.data(), h.rec_index.size() * 4);
+    put(lvl, h.lvl.data(), h.lvl.size() * 4);
+    put(windows, h.windows.data(), h.windows.size() * sizeof(WindowDev));
+    put(vlist, h.vlist.data(), h.vlist.size() * 4);
+    put(colour_recs, h.colour_recs.data(), h.colour_recs.size() * sizeof(uint4));
+    put(colour_begin, h.colour_begin.data(), h.colour_begin.size() * 4);
+    put(identity_recs, s->host_recs.data(), s->host_recs.size() * sizeof(uint4));
+    put(identity_rec_index, s->host_rec_index.data(), s->host_rec_index.size() * 4);
+    put(identity_level_begin, s->host_level_begin.data(), s->host_level_begin.size() * 4);
     return XPBD_OK;
 }
 

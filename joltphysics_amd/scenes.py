@@ -219,13 +219,48 @@ def collider_world_bounds(c: Collider):
     elif c.type == SHAPE_CONVEX_HULL:
         w = c.hull_points().astype(np.float64) @ R.T + p
         return tuple(np.concatenate([w.min(axis=0), w.max(axis=0)]).astype(np.float32))
-    else:  # plane: square of half extent around the plane point, extruded half extent behind
-        n = np.array(c.params[:3], np.float64)
-        he = np.full(3, 2.0 * c.params[4])
-        centre = -n * c.params[3]
-        lo, hi = R @ centre + p - he, R @ centre + p + he
-        return tuple(np.concatenate([lo, hi]).astype(np.float32))
+    else:
+        return plane_world_bounds(c)
     return tuple(np.concatenate([p - he, p + he]).astype(np.float32))
+
+
+def plane_world_bounds(c: Collider):
+    """Body::GetWorldSpaceBounds of a plane collider, in float32 as the reference computes it:
+    PlaneShape::CalculateLocalBounds (the four corners of the half-extent square around the
+    plane point and the same corners half extent behind it; PlaneShape.cpp:45-85), then
+    AABox::Transformed by the body transform (AABox.h:193-213)."""
+    f = np.float32
+    n = np.array(c.params[:3], f)
+    n = n / f(np.sqrt(f(f(n[0] * n[0] + n[1] * n[1]) + n[2] * n[2])))
+    const, half = f(c.params[3]), f(c.params[4])
+
+    def cross(a, b):
+        return np.array([f(a[1] * b[2]) - f(a[2] * b[1]), f(a[2] * b[0]) - f(a[0] * b[2]),
+                         f(a[0] * b[1]) - f(a[1] * b[0])], f)
+
+    def length(a):
+        return f(np.sqrt(f(f(a[0] * a[0] + a[1] * a[1]) + a[2] * a[2])))
+
+    # sPlaneGetOrthogonalBasis: NormalizedOr(AxisX) of n x Y, then the two crosses
+    p1 = cross(n, np.array([0, 1, 0], f))
+    l2 = f(f(p1[0] * p1[0] + p1[1] * p1[1]) + p1[2] * p1[2])
+    p1 = np.array([1, 0, 0], f) if l2 <= np.finfo(np.float32).tiny else p1 / f(np.sqrt(l2))
+    p2 = cross(p1, n)
+    p2 = p2 / length(p2)
+    p1 = cross(n, p2)
+    p1, p2 = p1 * half, p2 * half
+    point = -n * const
+    corners = [point + p1 + p2, point + p1 - p2, point - p1 - p2, point - p1 + p2]
+    pts = np.array(corners + [v - half * n for v in corners], f)
+    lmin, lmax = pts.min(axis=0), pts.max(axis=0)
+    R = _rot_matrix(c.rot).astype(f)
+    t = np.array(c.pos, f)
+    wmin, wmax = t.copy(), t.copy()
+    for k in range(3):
+        a, b = R[:, k] * lmin[k], R[:, k] * lmax[k]
+        wmin = wmin + np.minimum(a, b)
+        wmax = wmax + np.maximum(a, b)
+    return tuple(np.concatenate([wmin, wmax]).astype(np.float32))
 
 
 # ------------------------------------------------------------------ configs
@@ -279,13 +314,17 @@ def config2_cloth_and_cube(cloth_n=512, cube_cells=48, num_spheres=64, seed=2, f
                  dt=1.0 / 60.0, collision_steps=4, frames=frames)
 
 
-def config5_block(cells=64, spacing=0.05, num_boxes=512, seed=5, frames=1, region=10.0):
+def config5_block(cells=64, spacing=0.05, num_boxes=512, seed=5, frames=1, region=10.0, touching=True):
     """Tetrahedralised cells^3 block (5 tets per cell, spacing 0.05 m, inv-mass 1) with edge,
     volume and long-range-attachment constraints: the top face's first cells x cells vertices
     are pinned (4096 for 64 cells) and every free vertex gets a Euclidean LRA to its nearest
     pinned vertex (max distance = rest distance).  Compliances log-uniform [1e-7, 1e-4] per
     constraint; 512 static boxes (half extents U(0.2, 2) m, random rotation) in a 20 x 20 x 10 m
-    region around the block (`region` = 10: its half width); 8 collision steps x 10 iterations."""
+    region around the block (`region` = 10: its half width); 8 collision steps x 10 iterations.
+    `touching`: a box whose world bounds overlap the block's is moved out along its shallowest
+    axis until the two bounds overlap by 1 cm, so the boxes touch the block instead of starting
+    deep inside it (a block embedded in boxes blows up within the first frame, in the
+    reference as here)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     v, e, t = tet_cube(cells, spacing, inv_mass=1.0, compliance=1e-6)
     g = cells + 1
@@ -317,6 +356,20 @@ def config5_block(cells=64, spacing=0.05, num_boxes=512, seed=5, frames=1, regio
         colliders.append(Collider(SHAPE_BOX, he, pos=p, rot=random_quaternion(rng)))
     for c in colliders:
         c.bounds = collider_world_bounds(c)
+    if touching:
+        lo = v["pos"].min(axis=0).astype(np.float64) + np.array(body.position)
+        hi = v["pos"].max(axis=0).astype(np.float64) + np.array(body.position)
+        for c in colliders:
+            b = np.array(c.bounds, np.float64)
+            if np.any(lo > b[3:]) or np.any(hi < b[:3]):
+                continue
+            # the shift that clears the block's bounds, per axis and side; the shallowest wins
+            shift, axis, side = min(((b[3 + k] - lo[k]) if sd < 0 else (hi[k] - b[k]), k, sd)
+                                    for k in range(3) for sd in (-1, 1))
+            p = list(c.pos)
+            p[axis] += side * (shift - 0.01)
+            c.pos = tuple(float(x) for x in p)
+            c.bounds = collider_world_bounds(c)
     return Scene(f"block{cells}_lra_boxes{num_boxes}", [body], colliders, gravity=(0.0, 0.0, -9.81),
                  dt=1.0 / 60.0, collision_steps=8, frames=frames)
 

@@ -94,7 +94,8 @@ class _BuildDesc(C.Structure):
 # Exported symbols of the C ABI, in declaration order of include/jolt_xpbd.h.
 EXPORTED_SYMBOLS = (
     "xpbd_last_error", "xpbd_context_create", "xpbd_context_destroy", "xpbd_context_stream",
-    "xpbd_settings_create", "xpbd_settings_release", "xpbd_settings_stats", "xpbd_settings_team_stats",
+    "xpbd_settings_create", "xpbd_settings_release", "xpbd_settings_stats", "xpbd_schedule_build",
+    "xpbd_schedule_sizes", "xpbd_schedule_read", "xpbd_settings_team_stats",
     "xpbd_settings_build", "xpbd_built_counts", "xpbd_built_read", "xpbd_built_free",
     "xpbd_build_last_error", "xpbd_body_create",
     "xpbd_set_colliders", "xpbd_set_gravity", "xpbd_update", "xpbd_context_set_team_min_windows",
@@ -137,6 +138,9 @@ def load_library(path=LIB_PATH):
     lib.xpbd_settings_release.argtypes = [vp]
     lib.xpbd_settings_stats.argtypes = [vp, C.POINTER(u32)]
     lib.xpbd_settings_team_stats.argtypes = [vp, C.POINTER(u32)]
+    lib.xpbd_schedule_build.argtypes = [C.POINTER(_SettingsDesc), C.POINTER(vp)]
+    lib.xpbd_schedule_sizes.argtypes = [vp, C.POINTER(u32)]
+    lib.xpbd_schedule_read.argtypes = [vp] * 11
     lib.xpbd_context_set_team_min_windows.argtypes = [vp, u32]
     lib.xpbd_body_set_tail_mode.argtypes = [vp, i32, i32]
     lib.xpbd_body_placement.argtypes = [vp, i32, vp]
@@ -187,6 +191,60 @@ def _ptr(a):
     if a is None or a.size == 0:
         return None
     return a.ctypes.data_as(C.c_void_p)
+
+
+def _settings_desc(s: SettingsArrays):
+    """xpbd_settings_desc of the arrays (and the contiguous copies it points into)."""
+    v = np.ascontiguousarray(s.vertices, VERTEX)
+    f = s.faces
+    if f.dtype != FACE:
+        f4 = np.zeros(len(f), FACE)
+        f4["v"] = f
+        f = f4
+    f = np.ascontiguousarray(f, FACE)
+    e = np.ascontiguousarray(s.edges, EDGE)
+    b = np.ascontiguousarray(s.bends, BEND)
+    t = np.ascontiguousarray(s.volumes, VOLUME)
+    lr = np.ascontiguousarray(s.lras, LRA)
+    g = np.ascontiguousarray(s.groups, GROUP)
+    sk = np.ascontiguousarray(s.skinned, SKINNED)
+    skn = np.ascontiguousarray(s.skinned_normals, np.uint32)
+    ib = np.ascontiguousarray(s.inv_binds, INV_BIND)
+    rs = np.ascontiguousarray(s.rod_stretch_shear, ROD_SS)
+    rb = np.ascontiguousarray(s.rod_bend_twist, ROD_BT)
+    d = _SettingsDesc(_ptr(v), len(v), _ptr(f), len(f), _ptr(e), len(e), _ptr(b), len(b),
+                      _ptr(t), len(t), _ptr(lr), len(lr), _ptr(g), len(g), len(sk), len(rs), len(rb),
+                      _ptr(sk), _ptr(skn), len(skn), _ptr(ib), len(ib), _ptr(rs), _ptr(rb))
+    return d, (v, f, e, b, t, lr, g, sk, skn, ib, rs, rb)
+
+
+def host_schedule(s: SettingsArrays) -> dict:
+    """The level programs the device would run for these settings, built on the host
+    (xpbd_schedule_build; no device needed): records (n, 4) uint32, their constraint
+    indices, level offsets, windows (n, 4), the gathered windows' vertex list, the coloured
+    non-parallel group and the identity program bundles merge."""
+    lib = load_library()
+    d, _keep = _settings_desc(s)
+    h = C.c_void_p()
+    rc = lib.xpbd_schedule_build(C.byref(d), C.byref(h))
+    if rc != XPBD_OK:
+        raise XpbdError(f"xpbd error {rc}: {lib.xpbd_last_error().decode()}")
+    try:
+        n = (C.c_uint32 * 10)()
+        _check(lib.xpbd_schedule_sizes(h, n))
+        out = dict(recs=np.zeros((n[0], 4), np.uint32), rec_index=np.zeros(n[0], np.uint32),
+                   lvl=np.zeros(n[1], np.uint32), windows=np.zeros((n[2], 4), np.uint32),
+                   vlist=np.zeros(n[4], np.uint32), colour_recs=np.zeros((n[5], 8), np.uint32),
+                   colour_begin=np.zeros(n[6], np.uint32), identity_recs=np.zeros((n[8], 4), np.uint32),
+                   identity_rec_index=np.zeros(n[8], np.uint32), identity_level_begin=np.zeros(n[9], np.uint32))
+        _check(lib.xpbd_schedule_read(h, *(_ptr(out[k]) for k in (
+            "recs", "rec_index", "lvl", "windows", "vlist", "colour_recs", "colour_begin", "identity_recs",
+            "identity_rec_index", "identity_level_begin"))))
+        out["num_par_windows"] = int(n[3])
+        out["resident"] = bool(n[7])
+        return out
+    finally:
+        lib.xpbd_settings_release(h)
 
 
 def build_settings(mesh) -> SettingsArrays:
@@ -391,26 +449,7 @@ class Context:
         return lin[:n], ang[:n], upd[:n]
 
     def create_settings(self, s: SettingsArrays) -> Settings:
-        v = np.ascontiguousarray(s.vertices, VERTEX)
-        f = s.faces
-        if f.dtype != FACE:
-            f4 = np.zeros(len(f), FACE)
-            f4["v"] = f
-            f = f4
-        f = np.ascontiguousarray(f, FACE)
-        e = np.ascontiguousarray(s.edges, EDGE)
-        b = np.ascontiguousarray(s.bends, BEND)
-        t = np.ascontiguousarray(s.volumes, VOLUME)
-        lr = np.ascontiguousarray(s.lras, LRA)
-        g = np.ascontiguousarray(s.groups, GROUP)
-        sk = np.ascontiguousarray(s.skinned, SKINNED)
-        skn = np.ascontiguousarray(s.skinned_normals, np.uint32)
-        ib = np.ascontiguousarray(s.inv_binds, INV_BIND)
-        rs = np.ascontiguousarray(s.rod_stretch_shear, ROD_SS)
-        rb = np.ascontiguousarray(s.rod_bend_twist, ROD_BT)
-        d = _SettingsDesc(_ptr(v), len(v), _ptr(f), len(f), _ptr(e), len(e), _ptr(b), len(b),
-                          _ptr(t), len(t), _ptr(lr), len(lr), _ptr(g), len(g), len(sk), len(rs), len(rb),
-                          _ptr(sk), _ptr(skn), len(skn), _ptr(ib), len(ib), _ptr(rs), _ptr(rb))
+        d, _keep = _settings_desc(s)
         h = C.c_void_p()
         _check(self.lib.xpbd_settings_create(self.h, C.byref(d), C.byref(h)))
         st = Settings(self, h, s)

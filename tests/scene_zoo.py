@@ -106,7 +106,7 @@ def cloth_cube_spheres(frames=4):
 def block_lra_boxes(cells=10, num_boxes=64, frames=2):
     """Config 5 at reduced size: tet block with edge, volume and LRA constraints among dense
     static boxes (many leaves per body), 8 collision steps x 10 iterations."""
-    return scenes.config5_block(cells=cells, num_boxes=num_boxes, frames=frames, region=1.5)
+    return scenes.config5_block(cells=cells, num_boxes=num_boxes, frames=frames, region=1.5, touching=False)
 
 
 def block_lra_boxes_streamed(frames=1):
