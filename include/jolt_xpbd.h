@@ -290,6 +290,10 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
  * scheduling choice).
  */
 int xpbd_context_set_team_min_windows(xpbd_context *ctx, uint32_t windows);
+/* Workgroups per team at least (0, the default: one per parallel-group window, two for a body
+ * with the coloured non-parallel group; at most 224 in all).  Members beyond the windows take part in the per-vertex phases and the coloured
+ * non-parallel group.  No reference counterpart. */
+int xpbd_context_set_team_size(xpbd_context *ctx, uint32_t workgroups);
 
 /*
  * How a body runs its non-parallel update group (the reference's last group, applied in

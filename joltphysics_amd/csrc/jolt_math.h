@@ -142,7 +142,8 @@ XHD void get_quaternion(const M44 &m, float q[4])
     }
     int i = 0;
     if (m.c[1][1] > m.c[0][0]) i = 1;
-    if (m.c[2][2] > m.c[i][i]) i = 2;
+    // m(i, i) by selection: an indexed read would put the matrix on the stack
+    if (m.c[2][2] > (i == 1 ? m.c[1][1] : m.c[0][0])) i = 2;
     if (i == 0)
     {
         float s = sqrtf(m.c[0][0] - (m.c[1][1] + m.c[2][2]) + 1.0f);

@@ -241,6 +241,7 @@ struct xpbd_context
     std::vector<uint32_t> streamed;
     // large streamed bodies run by teams of workgroups (xpbd_team.hip)
     uint32_t team_min_windows = 3;  // a streamed body with this many parallel-group windows gets a team (0: never)
+    uint32_t team_size = 0;         // workgroups per team at least (0: one per parallel-group window)
     std::vector<TeamDev> teams;
     std::vector<uint2> team_blocks; // per workgroup of the team launch: team, member
     DevBuf<TeamDev> d_teams;
@@ -1067,7 +1068,7 @@ double StreamedMicroseconds(const Body &b)
 // that every member of every team holds a CU at once; each member takes a contiguous
 // run of windows of about equal cost.
 constexpr uint32_t kMaxTeamBlocks = 224;
-constexpr uint32_t kMaxTeamSize = 64;
+constexpr uint32_t kMaxTeamSize = 224;
 
 int BuildTeams(xpbd_context *ctx)
 {
@@ -1089,7 +1090,9 @@ int BuildTeams(xpbd_context *ctx)
         {
             ctx->teams.push_back(TeamDev{});
             ctx->teams.back().body = i;
-            members.push_back(std::max(1u, std::min(st.dev.num_par_windows, kMaxTeamSize)));
+            // one member per window; a coloured body's colour passes take twice that by default
+            const uint32_t want = ctx->team_size ? ctx->team_size : (coloured ? 2u : 1u) * st.dev.num_par_windows;
+            members.push_back(std::max(1u, std::min(std::max(st.dev.num_par_windows, want), kMaxTeamSize)));
         }
         else
             keep.push_back(i);
@@ -1124,9 +1127,15 @@ int BuildTeams(xpbd_context *ctx)
         split.push_back(0);
         double acc = 0.0;
         uint32_t w = 0;
+        const uint32_t with_windows = std::max(1u, std::min(t.size, (uint32_t)pw.size()));
         for (uint32_t m = 1; m < t.size; ++m)
         {
-            const double goal = all * m / t.size;
+            if (m >= with_windows)
+            {
+                split.push_back((uint32_t)pw.size());
+                continue;
+            }
+            const double goal = all * m / with_windows;
             while (w < pw.size() && acc + 0.5 * (pw[w].nlev + 2.0) < goal)
                 acc += pw[w++].nlev + 2.0;
             split.push_back(w);
@@ -1651,6 +1660,18 @@ int xpbd_settings_team_stats(const xpbd_settings *s, uint32_t out[4])
     out[1] = s->dev.resident ? 0u : s->dev.num_tail_windows;
     out[2] = s->colourable ? 1u : 0u;
     out[3] = s->num_colours;
+    return XPBD_OK;
+}
+
+int xpbd_context_set_team_size(xpbd_context *ctx, uint32_t workgroups)
+{
+    if (!ctx)
+        return Fail(XPBD_E_INVALID, "null context");
+    if (ctx->team_size != workgroups)
+    {
+        ctx->team_size = workgroups;
+        ctx->layout_dirty = true;
+    }
     return XPBD_OK;
 }
 

@@ -98,7 +98,7 @@ EXPORTED_SYMBOLS = (
     "xpbd_schedule_sizes", "xpbd_schedule_read", "xpbd_settings_team_stats",
     "xpbd_settings_build", "xpbd_built_counts", "xpbd_built_read", "xpbd_built_free",
     "xpbd_build_last_error", "xpbd_body_create",
-    "xpbd_set_colliders", "xpbd_set_gravity", "xpbd_update", "xpbd_context_set_team_min_windows",
+    "xpbd_set_colliders", "xpbd_set_gravity", "xpbd_update", "xpbd_context_set_team_min_windows", "xpbd_context_set_team_size",
     "xpbd_body_set_tail_mode", "xpbd_synchronize", "xpbd_body_read",
     "xpbd_body_write", "xpbd_body_state", "xpbd_body_placement", "xpbd_body_velocity_deltas", "xpbd_body_contacts", "xpbd_body_sensor_contacts",
     "xpbd_body_skin_vertices", "xpbd_body_rod_states", "xpbd_body_set_contact_settings",
@@ -142,6 +142,7 @@ def load_library(path=LIB_PATH):
     lib.xpbd_schedule_sizes.argtypes = [vp, C.POINTER(u32)]
     lib.xpbd_schedule_read.argtypes = [vp] * 11
     lib.xpbd_context_set_team_min_windows.argtypes = [vp, u32]
+    lib.xpbd_context_set_team_size.argtypes = [vp, u32]
     lib.xpbd_body_set_tail_mode.argtypes = [vp, i32, i32]
     lib.xpbd_body_placement.argtypes = [vp, i32, vp]
     lib.xpbd_settings_build.argtypes = [C.POINTER(_BuildDesc), C.POINTER(vp)]
@@ -507,6 +508,10 @@ class Context:
         """Streamed bodies with at least `windows` parallel-group windows run on a team of
         workgroups (0: never)."""
         _check(self.lib.xpbd_context_set_team_min_windows(self.h, int(windows)))
+
+    def set_team_size(self, workgroups):
+        """Workgroups per team at least (0: one per parallel-group window)."""
+        _check(self.lib.xpbd_context_set_team_size(self.h, int(workgroups)))
 
     def set_tail_mode(self, body, coloured):
         """The non-parallel group in the reference's order (False) or re-coloured (True)."""
