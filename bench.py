@@ -276,6 +276,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-diff", action="store_true", help="skip the fwd+bwd differentiation measurement")
     ap.add_argument("--no-large", action="store_true", help="skip the large single body measurement")
+    ap.add_argument("--only-large", action="store_true", help="only the large single body measurement (profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (the known-size reference "
                          "for HBM byte counters) and exit")
@@ -294,6 +295,11 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "independent bodies would shard across GPUs by cost with no exchange", file=sys.stderr)
         sys.exit(2)
+
+    if args.only_large:
+        _, _, large = large_body_run(steps=args.steps, warmup=args.warmup)
+        print(json.dumps({"large_body": large}), flush=True)
+        return
 
     scene = make_workload(args.workload)
     t0 = time.time()

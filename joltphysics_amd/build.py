@@ -21,7 +21,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIBNAME = "libjolt_xpbd.so"
 
-HIP_SOURCES = ["xpbd_kernels.hip", "xpbd_bundle.hip", "xpbd_team.hip", "xpbd_rollout.hip", "xpbd_runtime.cpp", "settings_builder.cpp"]
+HIP_SOURCES = ["xpbd_kernels.hip", "xpbd_bundle.hip", "xpbd_team.hip", "xpbd_skin.hip", "xpbd_rollout.hip", "xpbd_runtime.cpp", "settings_builder.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
@@ -68,5 +68,51 @@ def build(verbose=False):
     return out
 
 
+# Host code checked with AddressSanitizer and UndefinedBehaviorSanitizer: the runtime and the
+# settings builder are rebuilt with the sanitizers on their host pass only (the device code of
+# every source, and the kernel objects, are the regular build's).  Used by the CPU tests
+# through JOLT_XPBD_LIBRARY with the sanitizer runtime preloaded.
+HOST_SOURCES = ["xpbd_runtime.cpp", "settings_builder.cpp"]
+SANITIZE = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer"]
+
+
+def sanitizer_runtime():
+    out = subprocess.run([hipcc(), "-print-file-name=libclang_rt.asan-x86_64.so"], capture_output=True, text=True)
+    path = out.stdout.strip()
+    return path if os.path.isabs(path) and os.path.exists(path) else None
+
+
+def asan_library_path():
+    return os.path.join(LIBDIR, "libjolt_xpbd_asan.so")
+
+
+def build_sanitized(verbose=False):
+    """libjolt_xpbd_asan.so: host sources with ASan + UBSan, kernels from the regular build."""
+    objdir = os.path.join(LIBDIR, "obj")
+    cc = hipcc()
+    objs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        if src in HOST_SOURCES:
+            obj = os.path.join(objdir, os.path.splitext(src)[0] + ".asan.o")
+            cmd = [cc, *HIPCC_FLAGS, *SANITIZE, "-O1", "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj,
+                   "-I", os.path.join(ROOT, "include")]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            subprocess.run(cmd, check=True)
+        elif not os.path.exists(obj):
+            raise RuntimeError("build the regular library first")
+        objs.append(obj)
+    out = asan_library_path()
+    cmd = [cc, "--offload-arch=gfx950", "-shared", "-fPIC", "-Xarch_host", "-fsanitize=address", "-Xarch_host",
+           "-fsanitize=undefined", "-o", out + ".tmp", *objs]
+    subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    if "--sanitized" in sys.argv:
+        print(build_sanitized(verbose="-v" in sys.argv))

@@ -33,6 +33,9 @@ hipError_t launch_team(const BodyDev *bodies, BodyStateDev *states, const TeamDe
                        uint32_t num_blocks, const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves,
                        uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
                        int32_t *error_flag, hipStream_t stream);
+hipError_t launch_skin(const float *joints, const SkinInvBindDev *inv_binds, uint32_t num_inv_binds, float *skin,
+                       const SkinnedDev *recs, uint32_t num_recs, const uint32_t *normal_faces, const uint32_t *faces,
+                       float4 *states, uint32_t nv, float4 *x, float4 *v, int32_t hard_skin_all, hipStream_t stream);
 uint32_t rollout_forward_lds_bytes(uint32_t nv, bool f64);
 uint32_t rollout_backward_lds_bytes(uint32_t nv, uint32_t ne);
 hipError_t launch_rollout_forward_f32(const RolloutDev &R, const float *compliance, const float *velocity, float *positions,
@@ -133,7 +136,10 @@ struct xpbd_settings
     std::vector<uint4> host_recs;
     std::vector<uint32_t> host_level_begin;     // nlev + 1 offsets into host_recs
     std::vector<uint32_t> host_rec_index;       // constraint index of each host record (no-op: ~0u)
-    // skinning (host copies, for SkinVertices)
+    // skinning: device tables for SkinVertices, host copies for the checks and state streams
+    DevBuf<SkinnedDev> d_skinned;
+    DevBuf<SkinInvBindDev> d_inv_binds;
+    DevBuf<uint32_t> d_skinned_normals;
     std::vector<SkinnedRec> skinned;
     std::vector<uint32_t> skinned_normals;
     std::vector<InvBindRec> inv_binds;
@@ -178,6 +184,7 @@ struct Body
     BodyDev dev{};
     BodyStateDev state{};
     bool skinned = false;           // has skin entries at dev.skin_base (one per vertex)
+    DevBuf<float> d_joints, d_skin_m;   // SkinVertices: joint matrices, skin matrices (16 floats each)
 };
 
 struct BundleHost
@@ -906,6 +913,36 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
         s.dev.resident = resident ? 1u : 0u;
         s.dev.window_capacity = window_capacity;
         return XPBD_OK;
+    }
+    if (d.num_skinned)
+    {
+        const VertexRec *verts = (const VertexRec *)d.vertices;
+        std::vector<SkinnedDev> sk(d.num_skinned);
+        for (uint32_t k = 0; k < d.num_skinned; ++k)
+        {
+            const SkinnedRec &sr = skinned[k];
+            SkinnedDev &o = sk[k];
+            memcpy(o.bind, verts[sr.vertex].pos, sizeof(o.bind));
+            o.vertex = sr.vertex;
+            for (int j = 0; j < 4; ++j)
+            {
+                o.inv_bind[j] = sr.w[j].inv_bind;
+                o.weight[j] = sr.w[j].weight;
+            }
+            o.normal_info = sr.normal_info;
+            o.max_distance = sr.max_distance;
+            o.pad[0] = o.pad[1] = 0;
+        }
+        std::vector<SkinInvBindDev> ib(d.num_inv_bind_matrices);
+        for (uint32_t k = 0; k < d.num_inv_bind_matrices; ++k)
+        {
+            ib[k].joint = s.inv_binds[k].joint;
+            ib[k].pad[0] = ib[k].pad[1] = ib[k].pad[2] = 0;
+            memcpy(ib[k].m, s.inv_binds[k].m, sizeof(ib[k].m));
+        }
+        if ((e = s.d_skinned.upload(sk)) != hipSuccess || (e = s.d_inv_binds.upload(ib)) != hipSuccess
+            || (e = s.d_skinned_normals.upload(s.skinned_normals)) != hipSuccess)
+            return Fail(XPBD_E_DEVICE, std::string("settings upload: ") + hipGetErrorString(e));
     }
     if ((e = s.recs.upload(pb.recs)) != hipSuccess || (e = s.lvl.upload(pb.lvl)) != hipSuccess
         || (e = s.windows.upload(pb.windows)) != hipSuccess || (e = s.vlist.upload(pb.vlist)) != hipSuccess
@@ -1944,6 +1981,10 @@ int xpbd_set_gravity(xpbd_context *ctx, const float g[3])
     return XPBD_OK;
 }
 
+namespace {
+int PushSkin(xpbd_context *ctx);
+} // namespace
+
 int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t frames)
 {
     if (!ctx)
@@ -1958,22 +1999,9 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
     for (size_t i = 0; i < ctx->bodies.size(); ++i)
         if (ctx->bodies[i]->pair_n && (ctx->bodies[i]->pair_n != nc || ctx->bodies[i]->pair_sig != ctx->collider_sig))
             return Fail(XPBD_E_INVALID, "body " + std::to_string(i) + " has contact settings for another collider list");
-    if (ctx->skin_dirty)
-    {
-        // skin states go up before the layout, whose bundles point at them
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        if (ctx->d_skin.n != ctx->skin_host.size())
-        {
-            HIP_TRY(ctx->d_skin.alloc(ctx->skin_host.size()));
-            for (auto &b : ctx->bodies)
-                if (b->skinned)
-                    b->dev.skin = ctx->d_skin.p;
-            ctx->tables_dirty = true;
-            ctx->layout_dirty = true;
-        }
-        HIP_TRY(hipMemcpy(ctx->d_skin.p, ctx->skin_host.data(), sizeof(float4) * ctx->skin_host.size(), hipMemcpyHostToDevice));
-        ctx->skin_dirty = false;
-    }
+    // skin states go up before the layout, whose bundles point at them
+    if ((rc = PushSkin(ctx)) != XPBD_OK)
+        return rc;
     if (!ctx->skin_host.empty())
         ctx->skin_on_device = true;
     if (ctx->layout_dirty || ctx->layout_colliders != nc)
@@ -2325,6 +2353,27 @@ size_t StateBytes(uint32_t nv, size_t nrods, bool skinned)
 
 namespace {
 // Brings the host copy of the skin states up to date after updates moved them.
+// Skin states whose host copy is newer go to the device (allocated there on first use).
+int PushSkin(xpbd_context *ctx)
+{
+    if (!ctx->skin_dirty)
+        return XPBD_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->d_skin.n != ctx->skin_host.size())
+    {
+        HIP_TRY(ctx->d_skin.alloc(ctx->skin_host.size()));
+        for (auto &b : ctx->bodies)
+            if (b->skinned)
+                b->dev.skin = ctx->d_skin.p;
+        ctx->tables_dirty = true;
+        ctx->layout_dirty = true;
+    }
+    HIP_TRY(hipMemcpy(ctx->d_skin.p, ctx->skin_host.data(), sizeof(float4) * ctx->skin_host.size(), hipMemcpyHostToDevice));
+    ctx->skin_dirty = false;
+    return XPBD_OK;
+}
+
 int PullSkin(xpbd_context *ctx)
 {
     if (!ctx->skin_on_device || ctx->d_skin.n != ctx->skin_host.size())
@@ -2346,85 +2395,30 @@ int xpbd_body_skin_vertices(xpbd_context *ctx, int32_t body, const float *joint_
     const xpbd_settings &st = *b.settings;
     if (!b.skinned)
         return Fail(XPBD_E_INVALID, "body has no skinned constraints");
-    int rc = PullSkin(ctx);
-    if (rc != XPBD_OK)
-        return rc;
-    // skin matrices: joint matrix * inverse bind matrix (Mat44 * Mat44, Mat44.inl:241-252)
-    std::vector<M44> skin_m(st.inv_binds.size());
-    for (size_t k = 0; k < st.inv_binds.size(); ++k)
-    {
-        const InvBindRec &ib = st.inv_binds[k];
+    for (const InvBindRec &ib : st.inv_binds)
         if (ib.joint >= num_joints)
             return Fail(XPBD_E_INVALID, "inverse bind matrix refers to a missing joint");
-        M44 j, m;
-        memcpy(j.c, joint_matrices + 16 * (size_t)ib.joint, sizeof(j.c));
-        memcpy(m.c, ib.m, sizeof(m.c));
-        skin_m[k] = mul(j, m);
-    }
-    float4 *E = ctx->skin_host.data() + 4 * (size_t)b.dev.skin_base;
-    for (const SkinnedRec &sr : st.skinned)
+    int rc = PushSkin(ctx);
+    if (rc != XPBD_OK)
+        return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    // the joint matrices are the call's only input from host memory
+    if (b.d_joints.n < 16 * (size_t)num_joints)
     {
-        const float *bp = st.vertices[sr.vertex].pos;
-        const V3 bind = v3(bp[0], bp[1], bp[2]);
-        V3 pos = v3(0.0f, 0.0f, 0.0f);
-        for (int k = 0; k < 4; ++k)
-        {
-            if (sr.w[k].weight == 0.0f)
-                break;
-            pos = pos + sr.w[k].weight * mul_point(skin_m[sr.w[k].inv_bind], bind);
-        }
-        float4 *e = E + 4 * (size_t)sr.vertex;
-        e[0] = make_float4(e[1].x, e[1].y, e[1].z, 0.0f);
-        e[1] = make_float4(pos.x, pos.y, pos.z, 0.0f);
-    }
-    for (const SkinnedRec &sr : st.skinned)
-    {
-        V3 normal = v3(0.0f, 0.0f, 0.0f);
-        const uint32_t nf = sr.normal_info >> 24;
-        if (nf > 0)
-        {
-            const uint32_t *f = st.skinned_normals.data() + (sr.normal_info & 0xffffffu);
-            for (uint32_t k = 0; k < nf; ++k)
-            {
-                const FaceRec &face = st.face_recs[f[k]];
-                const V3 v0 = v3(E[4 * (size_t)face.v[0] + 1].x, E[4 * (size_t)face.v[0] + 1].y, E[4 * (size_t)face.v[0] + 1].z);
-                const V3 v1 = v3(E[4 * (size_t)face.v[1] + 1].x, E[4 * (size_t)face.v[1] + 1].y, E[4 * (size_t)face.v[1] + 1].z);
-                const V3 v2 = v3(E[4 * (size_t)face.v[2] + 1].x, E[4 * (size_t)face.v[2] + 1].y, E[4 * (size_t)face.v[2] + 1].z);
-                const V3 c = cross(v1 - v0, v2 - v0);
-                const float l2 = len_sq(c);
-                normal = normal + (l2 <= FLT_MIN ? v3(0.0f, 0.0f, 0.0f) : c / sqrtf(l2));   // NormalizedOr(zero)
-            }
-            const float l2 = len_sq(normal);
-            normal = l2 <= FLT_MIN ? v3(0.0f, 0.0f, 0.0f) : normal / sqrtf(l2);
-        }
-        E[4 * (size_t)sr.vertex + 2] = make_float4(normal.x, normal.y, normal.z, 0.0f);
-    }
-    const bool enabled = st.skinned.empty() || E[4 * (size_t)st.skinned[0].vertex + 3].z != 0.0f;
-    if (hard_skin_all || !enabled)
-    {
-        HIP_TRY(hipSetDevice(ctx->device));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        std::vector<float4> x(b.nv), v(b.nv);
-        HIP_TRY(hipMemcpy(x.data(), b.dev.x, sizeof(float4) * b.nv, hipMemcpyDeviceToHost));
-        HIP_TRY(hipMemcpy(v.data(), b.dev.v, sizeof(float4) * b.nv, hipMemcpyDeviceToHost));
-        for (const SkinnedRec &sr : st.skinned)
-        {
-            float4 *e = E + 4 * (size_t)sr.vertex;
-            if (hard_skin_all)
-            {
-                e[0] = make_float4(e[1].x, e[1].y, e[1].z, 0.0f);
-                x[sr.vertex] = make_float4(e[1].x, e[1].y, e[1].z, x[sr.vertex].w);
-                v[sr.vertex] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            }
-            else if (sr.max_distance == 0.0f)
-                x[sr.vertex] = make_float4(e[1].x, e[1].y, e[1].z, x[sr.vertex].w);
-        }
-        HIP_TRY(hipMemcpy(b.dev.x, x.data(), sizeof(float4) * b.nv, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(b.dev.v, v.data(), sizeof(float4) * b.nv, hipMemcpyHostToDevice));
+        HIP_TRY(b.d_joints.alloc(16 * (size_t)std::max(1u, num_joints)));
     }
-    for (uint32_t i = 0; i < b.nv; ++i)
-        E[4 * (size_t)i + 3].w = 1.0f;      // mSkinStatePreviousPositionValid
-    ctx->skin_dirty = true;
+    if (b.d_skin_m.n < 16 * st.inv_binds.size())
+        HIP_TRY(b.d_skin_m.alloc(16 * std::max<size_t>(1, st.inv_binds.size())));
+    if (num_joints)
+    {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        HIP_TRY(hipMemcpy(b.d_joints.p, joint_matrices, sizeof(float) * 16 * num_joints, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(launch_skin(b.d_joints.p, st.d_inv_binds.p, (uint32_t)st.inv_binds.size(), b.d_skin_m.p,
+                        st.d_skinned.p, (uint32_t)st.skinned.size(), st.d_skinned_normals.p, st.faces.p,
+                        ctx->d_skin.p + 4 * (size_t)b.dev.skin_base, b.nv, b.dev.x, b.dev.v, hard_skin_all, ctx->stream));
+    ctx->skin_on_device = true;
     return XPBD_OK;
 }
 

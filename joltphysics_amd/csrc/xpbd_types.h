@@ -101,6 +101,28 @@ struct TeamDev
     struct Leaf *leaves;        // per member: num_colliders leaves, used when the colliding list overflows LDS
 };
 
+// Skinned constraint on the device (SoftBodySharedSettings.h:225-285): the vertex's bind pose
+// position, its vertex, up to four (inverse bind, weight) pairs, its normal faces
+// {count << 24 | first} and its max distance.
+struct SkinnedDev
+{
+    float bind[3];
+    uint32_t vertex;
+    uint32_t inv_bind[4];
+    float weight[4];
+    uint32_t normal_info;
+    float max_distance;
+    uint32_t pad[2];
+};
+
+// Inverse bind matrix (column major) and its joint.
+struct SkinInvBindDev
+{
+    uint32_t joint;
+    uint32_t pad[3];
+    float m[16];
+};
+
 // Per-body dynamic state kept on the device between launches.
 struct BodyStateDev
 {

@@ -21,7 +21,8 @@ from .records import (BEND, EDGE, FACE, GROUP, INV_BIND, LRA, MOTION_DYNAMIC, RO
                       SKINNED, VERTEX, VOLUME, BodyParams, SettingsArrays)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_lib", "libjolt_xpbd.so")
+# JOLT_XPBD_LIBRARY selects another build of the library (the sanitized one in the CPU tests)
+LIB_PATH = os.environ.get("JOLT_XPBD_LIBRARY") or os.path.join(_HERE, "_lib", "libjolt_xpbd.so")
 
 XPBD_OK = 0
 
