@@ -972,3 +972,8 @@ int jo_body_write(void *wp, int body, const float *pos, const float *vel)
 int jo_hardware_concurrency() { return int(std::thread::hardware_concurrency()); }
 
 } // extern "C"
+
+// The world's PhysicsSystem and the BodyID of soft body `body` (index and sequence number), for
+// harnesses built into the same library (adapter_harness.cpp).
+extern "C" void *jo_world_physics_system(void *wp) { return &((World *)wp)->mSystem; }
+extern "C" uint32 jo_world_soft_body_id(void *wp, int body) { return ((World *)wp)->mSoftBodies[body].GetIndexAndSequenceNumber(); }
