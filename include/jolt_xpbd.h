@@ -52,8 +52,49 @@ extern "C" {
  * to the shape's centre of mass; tanAlpha = Tan(ASin(sinAlpha)) with Jolt's approximations) */
 #define XPBD_SHAPE_TAPERED_CAPSULE 5
 /* ConvexHullShape: no params; the hull's own data in xpbd_collider_desc.hull
- * (ConvexHullShape.cpp:1060-1164, unscaled) */
+ * (ConvexHullShape.cpp:1060-1164) */
 #define XPBD_SHAPE_CONVEX_HULL 6
+/* TaperedCylinderShape: params {top, bottom, topRadius, bottomRadius} as the shape holds them
+ * (TaperedCylinderShape.cpp:108-130: top and bottom relative to the shape's centre of mass;
+ * SaveBinaryState writes them); TaperedCylinderShape.cpp:390-506 */
+#define XPBD_SHAPE_TAPERED_CYLINDER 7
+/* TriangleShape (TriangleShape.cpp:272-283) and MeshShape (MeshShape.cpp:833-885): no params;
+ * xpbd_collider_desc.tree holds the triangles as the shape's tree walk visits them
+ * (CollideSoftBodyVerticesVsTriangles.h).  A triangle is a tree whose root is a leaf holding
+ * its one triangle. */
+#define XPBD_SHAPE_MESH 8
+
+/* Child references of an xpbd_tree_node (and xpbd_triangle_tree.root) */
+#define XPBD_TREE_NODE 0x00000000u      /* | node index (bits 0-29) */
+#define XPBD_TREE_LEAF 0x40000000u      /* | triangle count << 24 (1-63) | first triangle (bits 0-23) */
+#define XPBD_TREE_EMPTY 0x80000000u     /* a padding child: never visited */
+
+/* One node of a four-way bounding-volume tree: the bounds of its four children, per axis,
+ * decoded to float exactly as the reference decodes them (NodeCodecQuadTreeHalfFloat.h:245-
+ * 283: half floats widened to float), and the children's references in stored order. */
+typedef struct xpbd_tree_node {
+    float min_x[4], min_y[4], min_z[4];
+    float max_x[4], max_y[4], max_z[4];
+    uint32_t child[4];
+} xpbd_tree_node;
+
+/* A MeshShape's tree as NodeCodecQuadTreeHalfFloat::DecodingContext::WalkTree walks it: the
+ * root reference, the nodes, and the triangles of every leaf in the order the leaf's block
+ * decodes them (TriangleCodecIndexed8BitPackSOA4Flags, vertices in the shape's space, 9
+ * floats per triangle).  Per vertex the device walks it closest child first, with the
+ * reference's sort of the child distances (Vec4::sSort4Reverse) and its pruning, so the
+ * triangle it keeps is the one the reference keeps, ties included. */
+typedef struct xpbd_triangle_tree {
+    uint32_t root;
+    uint32_t num_nodes;
+    const xpbd_tree_node *nodes;
+    uint32_t num_triangles;
+    const float *triangles;         /* num_triangles x 9: v0, v1, v2 */
+} xpbd_triangle_tree;
+
+/* Largest depth of a triangle tree (levels of nodes below the root): the per-vertex walk keeps
+ * its stack in 3 x depth + 4 entries. */
+#define XPBD_TREE_MAX_DEPTH 20
 
 /* A convex hull as ConvexHullShape holds it: points relative to its centre of mass
  * (GetPoint), one plane per face (GetPlanes: normal xyz, constant) and each face's
@@ -264,6 +305,13 @@ typedef struct xpbd_collider_desc {
      * A table of xpbd_body_set_contact_settings holds only while every index of the collider
      * list keeps its identifier, shape type, motion type and sensor flag. */
     uint32_t user_id;
+    /* The shape's scale, as a ScaledShape around it gives it (ScaledShape.cpp:142-150:
+     * CollideSoftBodyVertices of the inner shape with inScale * mScale); {0, 0, 0} stands for
+     * {1, 1, 1}.  Sphere, capsule and tapered capsule take a uniform scale, cylinder and
+     * tapered cylinder one uniform in x and z (IsValidScale of the shape); `position` is the
+     * centre of mass of the scaled shape. */
+    float scale[3];
+    const xpbd_triangle_tree *tree;     /* XPBD_SHAPE_MESH only (copied by xpbd_set_colliders) */
 } xpbd_collider_desc;
 
 int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *colliders, uint32_t num_colliders);

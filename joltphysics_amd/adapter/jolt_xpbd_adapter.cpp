@@ -2,18 +2,11 @@
 // Jolt's public API; everything it hands the device goes through include/jolt_xpbd.h.
 
 #include "jolt_xpbd_adapter.h"
+#include "jolt_xpbd_shapes.h"
 
 #include <Jolt/Core/StreamWrapper.h>
-#include <Jolt/Math/Trigonometry.h>
 #include <Jolt/Physics/Body/BodyLock.h>
 #include <Jolt/Physics/Collision/CollisionCollectorImpl.h>
-#include <Jolt/Physics/Collision/Shape/BoxShape.h>
-#include <Jolt/Physics/Collision/Shape/CapsuleShape.h>
-#include <Jolt/Physics/Collision/Shape/ConvexHullShape.h>
-#include <Jolt/Physics/Collision/Shape/CylinderShape.h>
-#include <Jolt/Physics/Collision/Shape/PlaneShape.h>
-#include <Jolt/Physics/Collision/Shape/SphereShape.h>
-#include <Jolt/Physics/Collision/Shape/TaperedCapsuleShape.h>
 #include <Jolt/Physics/SoftBody/SoftBodyMotionProperties.h>
 #include <Jolt/Physics/StateRecorderImpl.h>
 
@@ -65,14 +58,6 @@ class BodyIDCollector : public JPH::CollideShapeBodyCollector
 public:
     std::vector<JPH::BodyID> mHits;
     void AddHit(const JPH::BodyID &inBodyID) override { mHits.push_back(inBodyID); }
-};
-
-// A convex hull's tables, as the shape holds them, alive until xpbd_set_colliders returns
-struct HullTables
-{
-    std::vector<float> points, planes;
-    std::vector<uint32_t> first, count, index;
-    xpbd_convex_hull hull;
 };
 
 } // namespace
@@ -220,7 +205,7 @@ bool SoftBodyUpdater::GatherColliders(float dt, int collision_steps)
     const bool listener = mSystem.GetSoftBodyContactListener() != nullptr;
     std::vector<xpbd_collider_desc> desc;
     std::vector<JPH::BodyID> ids;
-    std::vector<std::unique_ptr<HullTables>> hulls;
+    std::vector<std::unique_ptr<ShapeData>> shapes;
     std::vector<std::vector<float>> accept(mBodies.size());
     bool any_rejected = false;
     for (JPH::BodyID hit : hits.mHits)
@@ -232,87 +217,11 @@ bool SoftBodyUpdater::GatherColliders(float dt, int collision_steps)
         if (!b.IsRigidBody() || (b.IsSensor() && !listener))
             continue;
         xpbd_collider_desc d = {};
-        const JPH::Shape *shape = b.GetShape();
-        switch (shape->GetSubType())
-        {
-        case JPH::EShapeSubType::Plane:
-            {
-                const auto *s = static_cast<const JPH::PlaneShape *>(shape);
-                const JPH::Plane &pl = s->GetPlane();
-                d.type = XPBD_SHAPE_PLANE;
-                d.params[0] = pl.GetNormal().GetX(); d.params[1] = pl.GetNormal().GetY(); d.params[2] = pl.GetNormal().GetZ();
-                d.params[3] = pl.GetConstant(); d.params[4] = s->GetHalfExtent();
-            }
-            break;
-        case JPH::EShapeSubType::Sphere:
-            d.type = XPBD_SHAPE_SPHERE;
-            d.params[0] = static_cast<const JPH::SphereShape *>(shape)->GetRadius();
-            break;
-        case JPH::EShapeSubType::Box:
-            {
-                const JPH::Vec3 he = static_cast<const JPH::BoxShape *>(shape)->GetHalfExtent();
-                d.type = XPBD_SHAPE_BOX;
-                d.params[0] = he.GetX(); d.params[1] = he.GetY(); d.params[2] = he.GetZ();
-            }
-            break;
-        case JPH::EShapeSubType::Capsule:
-            {
-                const auto *s = static_cast<const JPH::CapsuleShape *>(shape);
-                d.type = XPBD_SHAPE_CAPSULE;
-                d.params[0] = s->GetHalfHeightOfCylinder(); d.params[1] = s->GetRadius();
-            }
-            break;
-        case JPH::EShapeSubType::Cylinder:
-            {
-                const auto *s = static_cast<const JPH::CylinderShape *>(shape);
-                d.type = XPBD_SHAPE_CYLINDER;
-                d.params[0] = s->GetHalfHeight(); d.params[1] = s->GetRadius();
-            }
-            break;
-        case JPH::EShapeSubType::TaperedCapsule:
-            {
-                // centres relative to the centre of mass and the cone angle, as the shape's
-                // constructor derives them (TaperedCapsuleShape.cpp:86-131)
-                const auto *s = static_cast<const JPH::TaperedCapsuleShape *>(shape);
-                const float half_height = s->GetHalfHeight(), top = s->GetTopRadius(), bottom = s->GetBottomRadius();
-                const float top_center = half_height + 0.5f * (bottom - top);
-                const float bottom_center = -half_height + 0.5f * (bottom - top);
-                const float sin_alpha = (bottom - top) / (top_center - bottom_center);
-                d.type = XPBD_SHAPE_TAPERED_CAPSULE;
-                d.params[0] = top_center; d.params[1] = bottom_center; d.params[2] = top; d.params[3] = bottom;
-                d.params[4] = sin_alpha; d.params[5] = JPH::Tan(JPH::ASin(sin_alpha));
-            }
-            break;
-        case JPH::EShapeSubType::ConvexHull:
-            {
-                const auto *s = static_cast<const JPH::ConvexHullShape *>(shape);
-                std::unique_ptr<HullTables> t(new HullTables);
-                for (JPH::uint i = 0; i < s->GetNumPoints(); ++i)
-                {
-                    const JPH::Vec3 p = s->GetPoint(i);
-                    t->points.insert(t->points.end(), { p.GetX(), p.GetY(), p.GetZ() });
-                }
-                for (JPH::uint f = 0; f < s->GetNumFaces(); ++f)
-                {
-                    const JPH::Plane &pl = s->GetPlanes()[f];
-                    t->planes.insert(t->planes.end(), { pl.GetNormal().GetX(), pl.GetNormal().GetY(), pl.GetNormal().GetZ(), pl.GetConstant() });
-                    JPH::uint vi[256];
-                    const JPH::uint n = s->GetFaceVertices(f, 256, vi);
-                    t->first.push_back((uint32_t)t->index.size());
-                    t->count.push_back(n);
-                    t->index.insert(t->index.end(), vi, vi + n);
-                }
-                t->hull = { s->GetNumPoints(), t->points.data(), s->GetNumFaces(), t->planes.data(), t->first.data(),
-                            t->count.data(), (uint32_t)t->index.size(), t->index.data() };
-                d.type = XPBD_SHAPE_CONVEX_HULL;
-                d.hull = &t->hull;
-                hulls.push_back(std::move(t));
-            }
-            break;
-        default:
-            return Fail("a soft body reaches a collider shape the device path does not support (mesh, height field, "
-                        "scaled or compound shape)");
-        }
+        std::unique_ptr<ShapeData> tables(new ShapeData);
+        std::string shape_err;
+        if (!DescribeShape(b.GetShape(), d, *tables, shape_err))
+            return Fail(shape_err);
+        shapes.push_back(std::move(tables));
         const JPH::RVec3 com = b.GetCenterOfMassPosition();
         const JPH::Quat q = b.GetRotation();
         d.position[0] = float(com.GetX()); d.position[1] = float(com.GetY()); d.position[2] = float(com.GetZ());

@@ -21,9 +21,11 @@
 // (JobSoftBodyFinalize), so the rest of Jolt sees the soft bodies as if it had simulated
 // them.
 //
-// Supported colliders: plane, sphere, box, capsule, cylinder, tapered capsule and convex hull
-// bodies, unscaled and uncomposed; Update() fails (false, LastError()) when a soft body's
-// bounds reach any other shape, and the caller keeps the CPU path for it.
+// Supported colliders: plane, sphere, box, capsule, cylinder, tapered capsule, tapered
+// cylinder, convex hull, triangle and mesh bodies, each possibly inside ScaledShapes
+// (jolt_xpbd_shapes.h); Update() fails (false, LastError()) when a soft body's bounds reach
+// any other shape (height field, compound, rotated / translated), and the caller keeps the
+// CPU path for it.
 #pragma once
 
 #include <Jolt/Jolt.h>

@@ -89,7 +89,7 @@ template <> struct BundlePositions<true>
     }
 };
 
-template <bool STAGED, int BLOCK, bool UNIFORM>
+template <bool STAGED, int BLOCK, bool UNIFORM, bool EXT>
 __device__ __forceinline__ void bundle_update(
     const BundleDev &G, const BodyDev *__restrict__ bodies, BodyStateDev *__restrict__ states,
     const ColliderDev *__restrict__ colliders, uint32_t num_colliders,
@@ -275,7 +275,7 @@ __device__ __forceinline__ void bundle_update(
                     float largest = -FLT_MAX;
                     if (st[b].active && leaf.type >= 0)
                         for (uint32_t k = lane; k < bb.nv; k += kWave)
-                            largest = smax(largest, sensor_penetration(leaf, pget(X, bb.voff + k)));
+                            largest = smax(largest, sensor_penetration<EXT>(leaf, pget(X, bb.voff + k)));
                     largest = wave_max(largest);
                     if (lane == 0)
                         L.sensor_hits[rank] = largest > 0.0f ? 1 : 0;
@@ -293,7 +293,7 @@ __device__ __forceinline__ void bundle_update(
                     Pl plane; plane.n = v3(0.0f, 0.0f, 0.0f); plane.c = 0.0f;
                     float largest = -FLT_MAX;
                     int32_t idx = -1;
-                    collide_vertex(leaves + b * nc, n, xyz(p), largest, idx, plane);
+                    collide_vertex<EXT>(leaves + b * nc, n, xyz(p), largest, idx, plane);
                     if (idx >= 0)
                     {
                         G.idx[i] = idx;
@@ -654,13 +654,13 @@ __device__ __forceinline__ void bundle_update(
 // One workgroup per bundle.  STAGED: the records and level offsets are copied
 // into LDS at launch (small programs only, 256-thread launches); UNIFORM: three
 // floats per vertex in LDS and the one inverse mass of the bundle.
-template <bool STAGED, int BLOCK, bool UNIFORM>
+template <bool STAGED, int BLOCK, bool UNIFORM, bool EXT>
 __global__ __launch_bounds__(BLOCK) void bundle_kernel(
     const BundleDev *__restrict__ bundles, const BodyDev *__restrict__ bodies, BodyStateDev *__restrict__ states,
     const ColliderDev *__restrict__ colliders, uint32_t num_colliders,
     float gx, float gy, float gz, float dt, int32_t steps, int32_t frames)
 {
-    bundle_update<STAGED, BLOCK, UNIFORM>(bundles[blockIdx.x], bodies, states, colliders, num_colliders, gx, gy, gz, dt, steps, frames);
+    bundle_update<STAGED, BLOCK, UNIFORM, EXT>(bundles[blockIdx.x], bodies, states, colliders, num_colliders, gx, gy, gz, dt, steps, frames);
 }
 
 uint32_t bundle_lds_bytes(uint32_t nv, uint32_t nrec, uint32_t nlvl, uint32_t nb, bool uniform)
@@ -670,51 +670,58 @@ uint32_t bundle_lds_bytes(uint32_t nv, uint32_t nrec, uint32_t nlvl, uint32_t nb
 
 uint32_t bundle_leaf_bytes() { return (uint32_t)sizeof(Leaf); }
 
-template <bool S, int BLOCK, bool U>
+template <bool S, int BLOCK, bool U, bool EXT>
 static hipError_t launch_bundles_variant(const BundleDev *bundles, uint32_t num_bundles, const BodyDev *bodies, BodyStateDev *states,
                                          const ColliderDev *colliders, uint32_t num_colliders, uint32_t lds_bytes,
                                          float gx, float gy, float gz, float dt, int32_t steps, int32_t frames, hipStream_t stream)
 {
-    static const hipError_t configured = hipFuncSetAttribute((const void *)bundle_kernel<S, BLOCK, U>,
+    static const hipError_t configured = hipFuncSetAttribute((const void *)bundle_kernel<S, BLOCK, U, EXT>,
                                                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
     if (configured != hipSuccess)
         return configured;
-    hipLaunchKernelGGL((bundle_kernel<S, BLOCK, U>), dim3(num_bundles), dim3(BLOCK), lds_bytes, stream,
+    hipLaunchKernelGGL((bundle_kernel<S, BLOCK, U, EXT>), dim3(num_bundles), dim3(BLOCK), lds_bytes, stream,
                        bundles, bodies, states, colliders, num_colliders, gx, gy, gz, dt, steps, frames);
     return hipGetLastError();
 }
 
-template <bool U>
+template <bool U, bool EXT>
 static hipError_t launch_bundles_uniform(bool staged, uint32_t block, const BundleDev *bundles, uint32_t num_bundles,
                                          const BodyDev *bodies, BodyStateDev *states, const ColliderDev *colliders,
                                          uint32_t num_colliders, uint32_t lds_bytes, float gx, float gy, float gz, float dt,
                                          int32_t steps, int32_t frames, hipStream_t stream)
 {
     if (staged && block == 256)
-        return launch_bundles_variant<true, 256, U>(bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
+        return launch_bundles_variant<true, 256, U, EXT>(bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
     if (staged)
         return hipErrorInvalidValue;
     if (block == 1024)
-        return launch_bundles_variant<false, 1024, U>(bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
+        return launch_bundles_variant<false, 1024, U, EXT>(bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
     if (block == 512)
-        return launch_bundles_variant<false, 512, U>(bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
+        return launch_bundles_variant<false, 512, U, EXT>(bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
     if (block == 256)
-        return launch_bundles_variant<false, 256, U>(bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
+        return launch_bundles_variant<false, 256, U, EXT>(bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
     return hipErrorInvalidValue;
 }
 
 // block: 256, 512 or 1024 threads (staged bundles only in 256-thread launches); uniform:
-// the bundles' vertices share one inverse mass and keep three floats per vertex in LDS
-hipError_t launch_bundles(bool staged, uint32_t block, bool uniform, const BundleDev *bundles, uint32_t num_bundles,
+// the bundles' vertices share one inverse mass and keep three floats per vertex in LDS;
+// ext: the collider list holds triangle trees or scaled convex hulls
+hipError_t launch_bundles(bool staged, uint32_t block, bool uniform, bool ext, const BundleDev *bundles, uint32_t num_bundles,
                           const BodyDev *bodies, BodyStateDev *states, const ColliderDev *colliders, uint32_t num_colliders,
                           uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
                           hipStream_t stream)
 {
     if (num_bundles == 0)
         return hipSuccess;
+    if (ext)
+    {
+        if (uniform)
+            return launch_bundles_uniform<true, true>(staged, block, bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
+        return launch_bundles_uniform<false, true>(staged, block, bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
+    }
     if (uniform)
-        return launch_bundles_uniform<true>(staged, block, bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
-    return launch_bundles_uniform<false>(staged, block, bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
+        return launch_bundles_uniform<true, false>(staged, block, bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
+    return launch_bundles_uniform<false, false>(staged, block, bundles, num_bundles, bodies, states, colliders, num_colliders, lds_bytes, gx, gy, gz, dt, steps, frames, stream);
 }
 
 } // namespace xpbd

@@ -6,6 +6,7 @@
 #include <cfloat>
 
 #include "jolt_math.h"
+#include "jolt_xpbd.h"
 #include "xpbd_types.h"
 
 namespace xpbd {
@@ -18,13 +19,20 @@ struct Leaf
 {
     M44 t;          // leaf transform (soft-body local)
     M44 inv;        // InversedRotationTranslation of t (box, capsule)
-    float pn[3];    // plane shape: transformed plane normal | tapered capsule {bottom radius, sin, tan alpha}
-    float pc;       //              and constant
+    float pn[3];    // plane shape: transformed plane normal | tapered capsule {bottom radius, sin, tan alpha} |
+                    // tapered cylinder {bottom radius} | scaled hull, triangle tree: scale
+    float pc;       // plane shape: constant | tapered capsule: sign of the y scale | scaled hull: -1 for an
+                    // inside-out scale, else 1 | triangle tree: normal sign
     float a, b, c;  // sphere {radius} | box half extent | capsule, cylinder {half height, radius} |
-                    // tapered capsule {top centre, bottom centre, top radius}
+                    // tapered capsule {top centre, bottom centre, top radius} (scaled) |
+                    // tapered cylinder {top, bottom, top radius} (scaled) | scaled hull: 1 / scale
     int32_t type;
     float friction, restitution;
-    const HullDev *hull;    // convex hull tables (device memory)
+    union
+    {
+        const HullDev *hull;    // convex hull tables (device memory)
+        const TreeDev *tree;    // triangle tree (device memory)
+    };
     int32_t dyn;            // dynamic collider: rank into the body's DynStateDev row; -1 otherwise
     int32_t collider;       // index of the collider in the context's list
     float inv_mass_scale1;  // SoftBodyContactSettings::mInvMassScale1 of the pair (CollidingShape::mSoftBodyInvMassScale)
@@ -398,15 +406,20 @@ constexpr int kWaitVmcntZero = 0x0f70;
 
 // Workgroup barrier for data that the wavefronts of a workgroup hand to each
 // other through global memory (the level programs exchange data through LDS
-// only).  Every wavefront first waits until its own global stores have
-// completed; after the barrier the CU's vector L1 is invalidated, so the loads
-// that follow read the other wavefronts' stores from L2.  __syncthreads() alone
-// emits neither on gfx950: it orders LDS, and leaves global stores in flight.
+// only): a release fence, the barrier and an acquire fence, all at workgroup
+// scope on the global address space, which the compiler's memory model orders
+// (every wavefront of a workgroup runs on one CU and shares its vector L1, so
+// the fences need no cache maintenance outside threadgroup-split mode, which
+// these kernels do not use).  Each wavefront also waits until its own global
+// stores have completed before the barrier, so the hand-off does not depend on
+// the order in which the CU's memory pipeline completes a store and a later
+// load of another wavefront.
 __device__ __forceinline__ void sync_global()
 {
     __builtin_amdgcn_s_waitcnt(kWaitVmcntZero);
-    __syncthreads();
-    asm volatile("buffer_inv sc0" ::: "memory");
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "global");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "global");
 }
 
 // Record load from global memory through an address-space-1 pointer, so the
@@ -647,6 +660,326 @@ __device__ __forceinline__ void collide_hull(const Leaf &L, uint32_t l, V3 x, fl
     }
 }
 
+// ConvexHullShape::CollideSoftBodyVertices for one vertex, scaled (ConvexHullShape.cpp:1060-
+// 1164, the `is_not_scaled` false branch): each plane rebuilt from the scaled normal and the
+// scaled first point of its face; edges from scaled points, tested with the inside-out sign.
+// L.pn: scale, (L.a, L.b, L.c): 1 / scale, L.pc: -1 for an inside-out scale, else 1.
+__device__ __noinline__ void collide_hull_scaled(const Leaf &L, uint32_t l, V3 x, float &largest, int32_t &idx, Pl &plane)
+{
+    const HullDev &H = *L.hull;
+    const V3 scale = v3(L.pn[0], L.pn[1], L.pn[2]), inv_scale = v3(L.a, L.b, L.c);
+    const float scale_flip = L.pc;
+    const V3 local = mul_point(L.inv, x);
+    float max_distance = -FLT_MAX;
+    V3 max_plane_normal = v3(0.0f, 0.0f, 0.0f);
+    uint32_t max_plane_idx = 0;
+    for (uint32_t f = 0; f < H.num_faces; ++f)
+    {
+        const V3 n = mulv(inv_scale, xyz(H.planes[f]));
+        const V3 plane_normal = n / len(n);                                         // Vec3::Normalized
+        const V3 plane_point = mulv(scale, xyz(H.points[H.index[H.face_first[f]]]));
+        const float distance = dot(plane_normal, local - plane_point);
+        if (distance > max_distance)
+        {
+            max_distance = distance;
+            max_plane_normal = plane_normal;
+            max_plane_idx = f;
+        }
+    }
+    V3 closest_point = -max_distance * max_plane_normal;
+    const bool is_outside = max_distance > 0.0f;
+    if (is_outside)
+    {
+        float closest_point_dist_sq = FLT_MAX;
+        const uint32_t *vi = H.index + H.face_first[max_plane_idx];
+        const uint32_t count = H.face_count[max_plane_idx];
+        for (uint32_t k = 0; k < count; ++k)
+        {
+            const V3 p1 = mulv(scale, xyz(H.points[vi[k]]));
+            const V3 p2 = mulv(scale, xyz(H.points[vi[k + 1 == count ? 0 : k + 1]]));
+            const V3 edge_normal = cross(p2 - p1, max_plane_normal);
+            if (scale_flip * dot(edge_normal, local - p1) > 0.0f)
+            {
+                const V3 closest = closest_point_on_line(p1 - local, p2 - local);
+                const float distance_sq = len_sq(closest);
+                if (distance_sq < closest_point_dist_sq)
+                {
+                    closest_point_dist_sq = distance_sq;
+                    closest_point = closest;
+                }
+            }
+        }
+    }
+    V3 normal = neg(closest_point);
+    const float normal_length = len(normal);
+    float penetration = normal_length;
+    if (is_outside)
+        penetration = -penetration;
+    else
+        normal = neg(normal);
+    if (!(largest >= penetration))
+    {
+        largest = penetration;
+        normal = normal_length > 1.0e-12f ? normal / normal_length : max_plane_normal;
+        plane = plane_transformed(plane_from_point_normal(local + closest_point, normal), L.t);
+        idx = (int32_t)l;
+    }
+}
+
+// ClosestPoint::GetClosestPointOnTriangle (ClosestPoint.h:157-318, MustIncludeC false): the
+// point of triangle (a, b, c) closest to the origin and the feature it lies on (bit 0: a,
+// bit 1: b, bit 2: c; 7: the interior).  Without FMA, CrossPrecise is Cross.
+__device__ __forceinline__ float clamp01(float v)
+{
+    const float lo = v > 0.0f ? v : 0.0f;              // max(v, 0)
+    return lo < 1.0f ? lo : 1.0f;                      // min(., 1)
+}
+
+__device__ __forceinline__ V3 closest_point_on_triangle(V3 in_a, V3 in_b, V3 in_c, uint32_t &out_set)
+{
+    const V3 ba = in_a - in_b, bc = in_c - in_b;
+    const bool swap_ac = dot(bc, bc) < dot(ba, ba);     // the shorter of ab, cb becomes ab
+    const V3 a = swap_ac ? in_c : in_a;
+    const V3 c = swap_ac ? in_a : in_c;
+    const V3 ab = in_b - a;
+    const V3 ac = c - a;
+    const V3 n = cross(ab, ac);
+    const float n_len_sq = len_sq(n);
+    if (n_len_sq < 1.0e-10f)
+    {
+        // degenerate: vertices, then edges ac, bc, ab
+        uint32_t closest_set = 4u;
+        V3 closest_point = in_c;
+        float best_dist_sq = len_sq(in_c);
+        const float a_len_sq = len_sq(in_a);
+        if (a_len_sq < best_dist_sq)
+        {
+            closest_set = 1u; closest_point = in_a; best_dist_sq = a_len_sq;
+        }
+        const float b_len_sq = len_sq(in_b);
+        if (b_len_sq < best_dist_sq)
+        {
+            closest_set = 2u; closest_point = in_b; best_dist_sq = b_len_sq;
+        }
+        const float ac_len_sq = len_sq(ac);
+        if (ac_len_sq > FLT_EPSILON * FLT_EPSILON)
+        {
+            const float v = clamp01(-dot(a, ac) / ac_len_sq);
+            const V3 q = a + v * ac;
+            const float dist_sq = len_sq(q);
+            if (dist_sq < best_dist_sq)
+            {
+                closest_set = 5u; closest_point = q; best_dist_sq = dist_sq;
+            }
+        }
+        const V3 bc2 = in_c - in_b;
+        const float bc_len_sq = len_sq(bc2);
+        if (bc_len_sq > FLT_EPSILON * FLT_EPSILON)
+        {
+            const float v = clamp01(-dot(in_b, bc2) / bc_len_sq);
+            const V3 q = in_b + v * bc2;
+            const float dist_sq = len_sq(q);
+            if (dist_sq < best_dist_sq)
+            {
+                closest_set = 6u; closest_point = q; best_dist_sq = dist_sq;
+            }
+        }
+        const V3 ab2 = in_b - in_a;
+        const float ab_len_sq = len_sq(ab2);
+        if (ab_len_sq > FLT_EPSILON * FLT_EPSILON)
+        {
+            const float v = clamp01(-dot(in_a, ab2) / ab_len_sq);
+            const V3 q = in_a + v * ab2;
+            const float dist_sq = len_sq(q);
+            if (dist_sq < best_dist_sq)
+            {
+                closest_set = 3u; closest_point = q; best_dist_sq = dist_sq;
+            }
+        }
+        out_set = closest_set;
+        return closest_point;
+    }
+    const V3 ap = neg(a);
+    const float d1 = dot(ab, ap), d2 = dot(ac, ap);
+    if (d1 <= 0.0f && d2 <= 0.0f)
+    {
+        out_set = swap_ac ? 4u : 1u;
+        return a;
+    }
+    const V3 bp = neg(in_b);
+    const float d3 = dot(ab, bp), d4 = dot(ac, bp);
+    if (d3 >= 0.0f && d4 <= d3)
+    {
+        out_set = 2u;
+        return in_b;
+    }
+    if (d1 * d4 <= d3 * d2 && d1 >= 0.0f && d3 <= 0.0f)
+    {
+        const float v = d1 / (d1 - d3);
+        out_set = swap_ac ? 6u : 3u;
+        return a + v * ab;
+    }
+    const V3 cp = neg(c);
+    const float d5 = dot(ab, cp), d6 = dot(ac, cp);
+    if (d6 >= 0.0f && d5 <= d6)
+    {
+        out_set = swap_ac ? 1u : 4u;
+        return c;
+    }
+    if (d5 * d2 <= d1 * d6 && d2 >= 0.0f && d6 <= 0.0f)
+    {
+        const float w = d2 / (d2 - d6);
+        out_set = 5u;
+        return a + w * ac;
+    }
+    const float d4_d3 = d4 - d3, d5_d6 = d5 - d6;
+    if (d3 * d6 <= d5 * d4 && d4_d3 >= 0.0f && d5_d6 >= 0.0f)
+    {
+        const float w = d4_d3 / (d4_d3 + d5_d6);
+        out_set = swap_ac ? 3u : 6u;
+        return in_b + w * (c - in_b);
+    }
+    out_set = 7u;
+    return (n * dot((a + in_b) + c, n)) / (3.0f * n_len_sq);
+}
+
+// Vec4::sSort4Reverse (Vec4.inl:480-502): the compare-exchange network that orders four
+// values from high to low, carrying their identifiers; ties keep the network's order.
+__device__ __forceinline__ void sort4_reverse(float (&v)[4], uint32_t (&id)[4])
+{
+    auto exchange = [&](int i, int j) {         // larger of (i, j) to i
+        if (v[j] > v[i])
+        {
+            const float tv = v[i]; v[i] = v[j]; v[j] = tv;
+            const uint32_t ti = id[i]; id[i] = id[j]; id[j] = ti;
+        }
+    };
+    exchange(0, 2); exchange(1, 3);
+    exchange(0, 1); exchange(2, 3);
+    exchange(1, 2);
+}
+
+constexpr int kTreeStack = 3 * XPBD_TREE_MAX_DEPTH + 4;
+
+// CollideSoftBodyVerticesVsTriangles (CollideSoftBodyVerticesVsTriangles.h) over a triangle
+// tree for one vertex, walked as MeshShape::CollideSoftBodyVertices walks its tree
+// (MeshShape.cpp:833-885, NodeCodecQuadTreeHalfFloat.h:245-318): from the root, a node's
+// four children get the squared distance of their scaled bounds to the vertex
+// (AABox4Scale, AABox4DistanceSqToPoint), are sorted from far to near (SortReverseAndStore)
+// and the ones nearer than the closest triangle so far are pushed, nearest on top; a leaf's
+// triangles are tested in order, each scaled, and replace the closest one when strictly
+// nearer; popped entries are skipped unless still nearer than the closest triangle.  Then
+// FinishVertex: an interior closest point gives the triangle's plane (penetration below the
+// 0.1 m triangle thickness), an edge or vertex the plane through the closest point facing
+// the vertex, back-facing edges ignored.  L.pn: scale, L.pc: normal sign (-1 inside out).
+__device__ __noinline__ void collide_tree(const Leaf &L, uint32_t l, V3 x, float &largest, int32_t &idx, Pl &plane)
+{
+    const TreeDev &T = *L.tree;
+    const V3 scale = v3(L.pn[0], L.pn[1], L.pn[2]);
+    const V3 local = mul_point(L.inv, x);
+    float closest_dist_sq = FLT_MAX;
+    V3 cv0 = v3(0.0f, 0.0f, 0.0f), cv1 = cv0, cv2 = cv0, closest = cv0;
+    uint32_t closest_set = 0;
+    uint32_t stack[kTreeStack];
+    float dist_stack[kTreeStack];
+    int top = 0;
+    stack[0] = T.root;
+    dist_stack[0] = 0.0f;
+    do
+    {
+        const uint32_t ref = stack[top];
+        const uint32_t kind = ref & 0xc0000000u;
+        if (kind == XPBD_TREE_NODE)
+        {
+            const TreeNodeDev &N = T.nodes[ref];
+            float d[4];
+            uint32_t id[4];
+            for (int k = 0; k < 4; ++k)
+            {
+                const float sx0 = scale.x * N.min_x[k], sx1 = scale.x * N.max_x[k];
+                const float sy0 = scale.y * N.min_y[k], sy1 = scale.y * N.max_y[k];
+                const float sz0 = scale.z * N.min_z[k], sz1 = scale.z * N.max_z[k];
+                const float cx = smin(smax(local.x, smin(sx0, sx1)), smax(sx0, sx1));
+                const float cy = smin(smax(local.y, smin(sy0, sy1)), smax(sy0, sy1));
+                const float cz = smin(smax(local.z, smin(sz0, sz1)), smax(sz0, sz1));
+                const float dx = cx - local.x, dy = cy - local.y, dz = cz - local.z;
+                d[k] = (dx * dx + dy * dy) + dz * dz;
+                id[k] = N.child[k];
+            }
+            sort4_reverse(d, id);
+            int n = 0;
+            for (int k = 0; k < 4; ++k)
+                n += d[k] < closest_dist_sq ? 1 : 0;
+            // the n nearest (the last n after the sort) go on the stack, farthest first
+            for (int j = 0; j < n; ++j)
+            {
+                stack[top + j] = id[4 - n + j];
+                dist_stack[top + j] = d[4 - n + j];
+            }
+            top += n;
+        }
+        else if (kind == XPBD_TREE_LEAF)
+        {
+            const uint32_t first = ref & 0x00ffffffu, count = (ref >> 24) & 0x3fu;
+            for (uint32_t t = first; t < first + count; ++t)
+            {
+                const float *tv = T.triangles + 9u * t;
+                const V3 v0 = mulv(scale, v3(tv[0], tv[1], tv[2]));
+                const V3 v1 = mulv(scale, v3(tv[3], tv[4], tv[5]));
+                const V3 v2 = mulv(scale, v3(tv[6], tv[7], tv[8]));
+                uint32_t set;
+                const V3 cp = closest_point_on_triangle(v0 - local, v1 - local, v2 - local, set);
+                const float dist_sq = len_sq(cp);
+                if (dist_sq < closest_dist_sq)
+                {
+                    cv0 = v0; cv1 = v1; cv2 = v2;
+                    closest = cp;
+                    closest_dist_sq = dist_sq;
+                    closest_set = set;
+                }
+            }
+        }
+        // the next entry still nearer than the closest triangle
+        do
+            --top;
+        while (top >= 0 && !(dist_stack[top] < closest_dist_sq));
+    } while (top >= 0);
+
+    if (!(closest_dist_sq < FLT_MAX))
+        return;
+    const V3 w0 = mul_point(L.t, cv0), w1 = mul_point(L.t, cv1), w2 = mul_point(L.t, cv2);
+    const V3 triangle_normal = L.pc * normalized_or(cross(w1 - w0, w2 - w0), v3(0.0f, 1.0f, 0.0f));
+    if (closest_set == 7u)
+    {
+        const float penetration = dot(triangle_normal, w0 - x);
+        if (penetration < 0.1f && !(largest >= penetration))
+        {
+            largest = penetration;
+            plane = plane_from_point_normal(w0, triangle_normal);
+            idx = (int32_t)l;
+        }
+    }
+    else
+    {
+        const V3 closest_point = mul_point(L.t, local + closest);
+        const V3 normal = x - closest_point;
+        if (dot(normal, triangle_normal) > 0.0f)
+        {
+            const float normal_length = len(normal);
+            const float penetration = -normal_length;
+            if (!(largest >= penetration))
+            {
+                largest = penetration;
+                plane = plane_from_point_normal(closest_point, normal_length > 0.0f ? normal / normal_length : triangle_normal);
+                idx = (int32_t)l;
+            }
+        }
+    }
+}
+
+// EXT: the kernel variant that also handles triangle trees and scaled convex hulls (their
+// walks cost registers that the other variants keep for the level loop).
+template <bool EXT>
 __device__ __forceinline__ void collide_vertex(const Leaf *leaves, uint32_t num_leaves, V3 x, float &largest, int32_t &idx, Pl &plane)
 {
     for (uint32_t l = 0; l < num_leaves; ++l)
@@ -782,11 +1115,12 @@ __device__ __forceinline__ void collide_vertex(const Leaf *leaves, uint32_t num_
                 }
             }
             break;
-        case 5: // TaperedCapsuleShape.cpp:304-371 (scale 1: the y flip and scale factors are 1)
+        case 5: // TaperedCapsuleShape.cpp:304-371 (centres and radii scaled in make_leaf)
             {
                 const V3 top_center = v3(0.0f, L.a, 0.0f), bottom_center = v3(0.0f, L.b, 0.0f);
                 const float top_radius = L.c, bottom_radius = L.pn[0], sin_alpha = L.pn[1], tan_alpha = L.pn[2];
-                const V3 local = mul_point(L.inv, x);
+                const V3 scale_y_flip = v3(1.0f, L.pc, 1.0f);
+                const V3 local = mulv(scale_y_flip, mul_point(L.inv, x));
                 V3 position, normal;
                 const V3 to_top = local - top_center;
                 const float to_top_len = len(to_top);
@@ -809,7 +1143,85 @@ __device__ __forceinline__ void collide_vertex(const Leaf *leaves, uint32_t num_
                     }
                     position = bottom_center + bottom_radius * normal;
                 }
-                const Pl p = plane_from_point_normal(position, normal);
+                Pl p = plane_from_point_normal(position, normal);
+                const float pen = -signed_distance(p, local);
+                if (!(largest >= pen))
+                {
+                    largest = pen;
+                    p.n = mulv(scale_y_flip, p.n);          // a flipped capsule flips the normal's y
+                    plane = plane_transformed(p, L.t);
+                    idx = (int32_t)l;
+                }
+            }
+            break;
+        case 6:
+            collide_hull(L, l, x, largest, idx, plane);
+            break;
+        case 7: // TaperedCylinderShape.cpp:390-506 (top, bottom and radii scaled in make_leaf)
+            {
+                const float top = L.a, bottom = L.b, top_radius = L.c, bottom_radius = L.pn[0];
+                const V3 top_3d = v3(0.0f, top, 0.0f), bottom_3d = v3(0.0f, bottom, 0.0f);
+                const V3 local = mul_point(L.inv, x);
+                const V3 normal_xz = normalized_or(mulv(v3(1.0f, 0.0f, 1.0f), local), v3(1.0f, 0.0f, 0.0f));
+                const float tan_alpha = (bottom_radius - top_radius) / (top - bottom);
+                const V3 sn = v3(normal_xz.x, tan_alpha, normal_xz.z);
+                const V3 side_normal = sn / len(sn);
+                const V3 side_support_top = normal_xz * top_radius + top_3d;
+                const float side_penetration = dot(side_support_top - local, side_normal);
+                const float top_penetration = top - local.y;
+                const float bottom_penetration = local.y - bottom;
+                const float min_top_bottom_penetration = smin(top_penetration, bottom_penetration);
+                V3 point, normal;
+                if (side_penetration < 0.0f || min_top_bottom_penetration < 0.0f)
+                {
+                    const V3 side_support_bottom = normal_xz * bottom_radius + bottom_3d;
+                    const V3 bottom_to_top = side_support_top - side_support_bottom;
+                    const float fraction = dot(local - side_support_bottom, bottom_to_top);
+                    const float distance_to_axis = dot(normal_xz, local);
+                    const bool inside_top_radius = distance_to_axis <= top_radius;
+                    const bool inside_bottom_radius = distance_to_axis <= bottom_radius;
+                    if (fraction >= len_sq(bottom_to_top) && !inside_top_radius)
+                    {
+                        point = side_support_top;
+                        normal = normalized_or(local - point, v3(0.0f, 1.0f, 0.0f));
+                    }
+                    else if (fraction < 0.0f && !inside_bottom_radius)
+                    {
+                        point = side_support_bottom;
+                        normal = normalized_or(local - point, v3(0.0f, 1.0f, 0.0f));
+                    }
+                    else if (top_penetration < 0.0f && inside_top_radius)
+                    {
+                        point = top_3d;
+                        normal = v3(0.0f, 1.0f, 0.0f);
+                    }
+                    else if (bottom_penetration < 0.0f && inside_bottom_radius)
+                    {
+                        point = bottom_3d;
+                        normal = v3(0.0f, -1.0f, 0.0f);
+                    }
+                    else
+                    {
+                        point = side_support_top;
+                        normal = side_normal;
+                    }
+                }
+                else if (side_penetration < min_top_bottom_penetration)
+                {
+                    point = side_support_top;
+                    normal = side_normal;
+                }
+                else if (top_penetration < bottom_penetration)
+                {
+                    point = top_3d;
+                    normal = v3(0.0f, 1.0f, 0.0f);
+                }
+                else
+                {
+                    point = bottom_3d;
+                    normal = v3(0.0f, -1.0f, 0.0f);
+                }
+                const Pl p = plane_from_point_normal(point, normal);
                 const float pen = -signed_distance(p, local);
                 if (!(largest >= pen))
                 {
@@ -819,8 +1231,13 @@ __device__ __forceinline__ void collide_vertex(const Leaf *leaves, uint32_t num_
                 }
             }
             break;
-        case 6:
-            collide_hull(L, l, x, largest, idx, plane);
+        case 8:
+            if constexpr (EXT)
+                collide_tree(L, l, x, largest, idx, plane);
+            break;
+        case 9:
+            if constexpr (EXT)
+                collide_hull_scaled(L, l, x, largest, idx, plane);
             break;
         }
     }
@@ -943,6 +1360,7 @@ __device__ __forceinline__ float4 contact_report(V3 x, float4 pl, bool has_conta
 
 // DetermineSensorCollisions (SoftBodyMotionProperties.cpp:267-289) for one vertex: the
 // largest penetration of the sensor's leaf over the vertices decides its contact.
+template <bool EXT>
 __device__ __forceinline__ float sensor_penetration(const Leaf &leaf, float4 p)
 {
     if (!(p.w > 0.0f))
@@ -952,7 +1370,7 @@ __device__ __forceinline__ float sensor_penetration(const Leaf &leaf, float4 p)
     Pl plane;
     plane.n = v3(0.0f, 0.0f, 0.0f);
     plane.c = 0.0f;
-    collide_vertex(&leaf, 1, xyz(p), largest, idx, plane);
+    collide_vertex<EXT>(&leaf, 1, xyz(p), largest, idx, plane);
     return largest;
 }
 
@@ -962,6 +1380,16 @@ __device__ __forceinline__ void clear_dynamic(DynStateDev &D)
     st3(D.acc_ang, v3(0.0f, 0.0f, 0.0f));
     D.any_updated = 0;
     D.updated = 0;
+}
+
+// ScaleHelpers::IsNotScaled and IsInsideOut (ScaleHelpers.h:18-33)
+__device__ __forceinline__ bool not_scaled(V3 scale)
+{
+    return len_sq(scale - v3(1.0f, 1.0f, 1.0f)) <= 1.0e-8f;                       // Vec3::IsClose
+}
+__device__ __forceinline__ bool inside_out(V3 scale)
+{
+    return (((scale.x < 0.0f) ? 1 : 0) + ((scale.y < 0.0f) ? 1 : 0) + ((scale.z < 0.0f) ? 1 : 0)) % 2 == 1;
 }
 
 // Leaf of collider c in the soft body's local space (SoftBodyMotionProperties.cpp:174-186:
@@ -988,39 +1416,81 @@ __device__ __forceinline__ void make_leaf(const ColliderDev &c, const M44 &inv_c
     }
     out.friction = sqrtf(body_friction * c.friction);                                 // ContactConstraintManager.h:554
     out.restitution = body_restitution > c.restitution ? body_restitution : c.restitution; // ContactConstraintManager.h:555
+    // the shape's scale (ScaledShape.cpp:142-145), each shape scaling as its
+    // CollideSoftBodyVertices does
+    const V3 scale = v3(c.scale[0], c.scale[1], c.scale[2]);
+    const V3 abs_scale = vabs(scale);                                               // Vec3::Abs
     switch (c.type)
     {
     case 0:
         {
-            // Plane::Scaled with unit scale (Plane.h:49-54) then GetTransformed
-            V3 n = v3(c.params[0], c.params[1], c.params[2]);
-            V3 sn = v3(n.x / 1.0f, n.y / 1.0f, n.z / 1.0f);
-            float snl = len(sn);
+            // Plane::Scaled (Plane.h:49-54) then GetTransformed (PlaneShape.cpp:246)
+            const V3 n = v3(c.params[0], c.params[1], c.params[2]);
+            const V3 sn = v3(n.x / scale.x, n.y / scale.y, n.z / scale.z);
+            const float snl = len(sn);
             Pl p; p.n = sn / snl; p.c = c.params[3] / snl;
-            Pl tp = plane_transformed(p, t);
+            const Pl tp = plane_transformed(p, t);
             out.pn[0] = tp.n.x; out.pn[1] = tp.n.y; out.pn[2] = tp.n.z; out.pc = tp.c;
         }
         break;
     case 1:
-        out.a = fabsf(1.0f) * c.params[0];
+        out.a = abs_scale.x * c.params[0];                                          // SphereShape.cpp:53-59
         break;
     case 2:
         out.inv = inversed_rotation_translation(t);
-        out.a = 1.0f * c.params[0]; out.b = 1.0f * c.params[1]; out.c = 1.0f * c.params[2];
+        out.a = abs_scale.x * c.params[0]; out.b = abs_scale.y * c.params[1]; out.c = abs_scale.z * c.params[2];   // BoxShape.cpp:237
         break;
     case 3:
+        {
+            out.inv = inversed_rotation_translation(t);
+            const float s = fabsf(scale.x);                                         // CapsuleShape.cpp:331-333
+            out.a = s * c.params[0]; out.b = s * c.params[1];
+        }
+        break;
     case 4:
         out.inv = inversed_rotation_translation(t);
-        out.a = 1.0f * c.params[0]; out.b = 1.0f * c.params[1];
+        out.a = abs_scale.y * c.params[0]; out.b = abs_scale.x * c.params[1];      // CylinderShape.cpp:324-326
         break;
     case 5:
+        // TaperedCapsuleShape.cpp:311-318: centres by |scale.y|, radii by |scale.x|, y flip
         out.inv = inversed_rotation_translation(t);
-        out.a = c.params[0]; out.b = c.params[1]; out.c = c.params[2];
-        out.pn[0] = c.params[3]; out.pn[1] = c.params[4]; out.pn[2] = c.params[5];
+        out.a = abs_scale.y * c.params[0]; out.b = abs_scale.y * c.params[1];
+        out.c = abs_scale.x * c.params[2]; out.pn[0] = abs_scale.x * c.params[3];
+        out.pn[1] = c.params[4]; out.pn[2] = c.params[5];
+        out.pc = sign_of(scale.y);
+        break;
+    case 7:
+        {
+            // TaperedCylinderShape::GetScaled (TaperedCylinderShape.cpp:181-199)
+            out.inv = inversed_rotation_translation(t);
+            float top = scale.y * c.params[0], bottom = scale.y * c.params[1];
+            float top_radius = abs_scale.x * c.params[2], bottom_radius = abs_scale.x * c.params[3];
+            if (bottom > top)
+            {
+                const float ft = top; top = bottom; bottom = ft;
+                const float fr = top_radius; top_radius = bottom_radius; bottom_radius = fr;
+            }
+            out.a = top; out.b = bottom; out.c = top_radius; out.pn[0] = bottom_radius;
+        }
+        break;
+    case 8:
+        // CollideSoftBodyVerticesVsTriangles (scale, normal sign of an inside-out scale)
+        out.inv = inversed_rotation_translation(t);
+        out.tree = c.tree;
+        out.pn[0] = scale.x; out.pn[1] = scale.y; out.pn[2] = scale.z;
+        out.pc = inside_out(scale) ? -1.0f : 1.0f;
         break;
     default:
         out.inv = inversed_rotation_translation(t);
         out.hull = c.hull;
+        if (!not_scaled(scale))
+        {
+            // the scaled branch of ConvexHullShape.cpp:1063-1066, 1091-1106, 1126-1134
+            out.type = 9;
+            out.pn[0] = scale.x; out.pn[1] = scale.y; out.pn[2] = scale.z;
+            out.a = 1.0f / scale.x; out.b = 1.0f / scale.y; out.c = 1.0f / scale.z;   // Vec3::Reciprocal
+            out.pc = inside_out(scale) ? -1.0f : 1.0f;
+        }
         break;
     }
 }

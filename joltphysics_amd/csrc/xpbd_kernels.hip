@@ -81,7 +81,7 @@ __device__ __forceinline__ void run_levels(const uint4 *recs, const uint32_t *lv
 // ---------------------------------------------------------------- kernel
 // Advances each body of `body_list` by `frames` x `steps` collision steps.  RODS: the
 // variant for bodies with Cosserat rods.
-template <bool RESIDENT, bool STAGED, int kBlock, bool RODS>
+template <bool RESIDENT, bool STAGED, int kBlock, bool RODS, bool EXT>
 __global__ __launch_bounds__(kBlock) void update_kernel(
     const BodyDev *__restrict__ bodies, BodyStateDev *__restrict__ states, const uint32_t *__restrict__ body_list,
     const ColliderDev *__restrict__ colliders, uint32_t num_colliders, uint32_t max_leaves,
@@ -270,7 +270,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                             make_leaf(colliders[ci], inv_com_s, B.friction, B.restitution, sh_sensor_leaf, nullptr);
                         __syncthreads();
                         for (uint32_t i = tid; i < nv; i += kBlock)
-                            largest = smax(largest, sensor_penetration(sh_sensor_leaf, P[i]));
+                            largest = smax(largest, sensor_penetration<EXT>(sh_sensor_leaf, P[i]));
                     }
                     largest = wave_max(largest);
                     if (lane == 0)
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                         Pl plane; plane.n = v3(pl.x, pl.y, pl.z); plane.c = pl.w;
                         float largest = -FLT_MAX;
                         int32_t idx = -1;
-                        collide_vertex(leaves, num_leaves, xyz(p), largest, idx, plane);
+                        collide_vertex<EXT>(leaves, num_leaves, xyz(p), largest, idx, plane);
                         if (idx >= 0)
                         {
                             B.idx[i] = idx;
@@ -673,7 +673,7 @@ uint32_t kernel_lds_bytes(bool resident, bool staged, const SettingsDev &s, uint
                       staged ? s.num_levels_total : 0u, max_leaves, nc, nf).total;
 }
 
-template <bool R, bool S, int BLOCK, bool RODS>
+template <bool R, bool S, int BLOCK, bool RODS, bool EXT>
 static hipError_t launch_variant(const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list, uint32_t num_blocks,
                                  const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves, uint32_t lds_bytes,
                                  float gx, float gy, float gz, float dt, int32_t steps, int32_t frames, int32_t *error_flag,
@@ -682,33 +682,33 @@ static hipError_t launch_variant(const BodyDev *bodies, BodyStateDev *states, co
     static bool configured = false;
     if (!configured)
     {
-        hipError_t e = hipFuncSetAttribute((const void *)update_kernel<R, S, BLOCK, RODS>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+        hipError_t e = hipFuncSetAttribute((const void *)update_kernel<R, S, BLOCK, RODS, EXT>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
         if (e != hipSuccess)
             return e;
         configured = true;
     }
-    hipLaunchKernelGGL((update_kernel<R, S, BLOCK, RODS>), dim3(num_blocks), dim3(BLOCK), lds_bytes, stream,
+    hipLaunchKernelGGL((update_kernel<R, S, BLOCK, RODS, EXT>), dim3(num_blocks), dim3(BLOCK), lds_bytes, stream,
                        bodies, states, body_list, colliders, num_colliders, max_leaves, gx, gy, gz, dt, steps, frames, error_flag);
     return hipGetLastError();
 }
 
-template <bool RODS>
+template <bool RODS, bool EXT>
 static hipError_t launch_update_rods(bool resident, bool staged, const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list,
                                      uint32_t num_blocks, const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves,
                                      uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
                                      int32_t *error_flag, hipStream_t stream)
 {
     if (resident && staged)
-        return launch_variant<true, true, 256, RODS>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
+        return launch_variant<true, true, 256, RODS, EXT>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
     if (resident)
-        return launch_variant<true, false, 256, RODS>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
-    return launch_variant<false, false, 1024, RODS>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
+        return launch_variant<true, false, 256, RODS, EXT>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
+    return launch_variant<false, false, 1024, RODS, EXT>(bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
 }
 
 // Resident single-body launches use 256-thread workgroups; streamed bodies use
 // 1024 threads so one pass covers the wide merged levels of an 8192-vertex window.
 // Bodies with Cosserat rods run in the variants that hold the rod code.
-hipError_t launch_update(bool resident, bool staged, bool rods, const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list,
+hipError_t launch_update(bool resident, bool staged, bool rods, bool ext, const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list,
                          uint32_t num_blocks, const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves,
                          uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
                          int32_t *error_flag, hipStream_t stream)
@@ -716,8 +716,10 @@ hipError_t launch_update(bool resident, bool staged, bool rods, const BodyDev *b
     if (num_blocks == 0)
         return hipSuccess;
     if (rods)
-        return launch_update_rods<true>(resident, staged, bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
-    return launch_update_rods<false>(resident, staged, bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
+        return ext ? launch_update_rods<true, true>(resident, staged, bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream)
+                   : launch_update_rods<true, false>(resident, staged, bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
+    return ext ? launch_update_rods<false, true>(resident, staged, bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream)
+               : launch_update_rods<false, false>(resident, staged, bodies, states, body_list, num_blocks, colliders, num_colliders, max_leaves, lds_bytes, gx, gy, gz, dt, steps, frames, error_flag, stream);
 }
 
 } // namespace xpbd

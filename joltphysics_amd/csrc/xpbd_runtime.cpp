@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -20,16 +21,16 @@ namespace xpbd {
 uint32_t kernel_lds_bytes(bool resident, bool staged, const SettingsDev &s, uint32_t max_leaves, uint32_t nc, uint32_t nf);
 uint32_t bundle_lds_bytes(uint32_t nv, uint32_t nrec, uint32_t nlvl, uint32_t nb, bool uniform);
 uint32_t bundle_leaf_bytes();
-hipError_t launch_bundles(bool staged, uint32_t block, bool uniform, const BundleDev *bundles, uint32_t num_bundles,
+hipError_t launch_bundles(bool staged, uint32_t block, bool uniform, bool ext, const BundleDev *bundles, uint32_t num_bundles,
                           const BodyDev *bodies, BodyStateDev *states, const ColliderDev *colliders, uint32_t num_colliders,
                           uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
                           hipStream_t stream);
-hipError_t launch_update(bool resident, bool staged, bool rods, const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list,
+hipError_t launch_update(bool resident, bool staged, bool rods, bool ext, const BodyDev *bodies, BodyStateDev *states, const uint32_t *body_list,
                          uint32_t num_blocks, const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves,
                          uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
                          int32_t *error_flag, hipStream_t stream);
 uint32_t team_lds_bytes(uint32_t window_capacity, uint32_t max_leaves, uint32_t nc);
-hipError_t launch_team(const BodyDev *bodies, BodyStateDev *states, const TeamDev *teams, const uint2 *block_map,
+hipError_t launch_team(bool ext, const BodyDev *bodies, BodyStateDev *states, const TeamDev *teams, const uint2 *block_map,
                        uint32_t num_blocks, const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves,
                        uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
                        int32_t *error_flag, hipStream_t stream);
@@ -220,6 +221,10 @@ struct xpbd_context
     DevBuf<float4> d_hull_vec;      // convex hulls: planes, then points, per hull
     DevBuf<uint32_t> d_hull_idx;    // convex hulls: face first, face count, vertex index, per hull
     DevBuf<HullDev> d_hulls;
+    DevBuf<TreeNodeDev> d_tree_nodes;   // triangle trees: nodes of every tree, references rebased
+    DevBuf<float> d_tree_tris;          // triangle trees: 9 floats per triangle, every tree
+    DevBuf<TreeDev> d_trees;
+    bool ext_colliders = false;         // a triangle tree or a scaled convex hull among the colliders
     // skinned vertex states of all skinned bodies, 4 float4 per vertex: previous and current
     // skinned position, normal, {max distance multiplier, back stop radius, enabled, previous
     // position valid}; kept on the host, uploaded before an update when changed
@@ -1866,6 +1871,70 @@ int xpbd_body_create(xpbd_context *ctx, const xpbd_body_desc *d, int32_t *out_bo
     return XPBD_OK;
 }
 
+namespace {
+// A triangle tree's references stay inside its nodes and triangles, every node is reached
+// once from the root (a tree, no cycles), and the depth is within the device walk's stack.
+bool ValidateTree(const xpbd_triangle_tree &t, std::string &err)
+{
+    if (t.num_nodes >= (1u << 30) || t.num_triangles >= (1u << 24))
+    {
+        err = "too many nodes or triangles";
+        return false;
+    }
+    std::vector<uint8_t> seen(t.num_nodes, 0);
+    struct Item { uint32_t ref, depth; };
+    std::vector<Item> todo{ { t.root, 0 } };
+    while (!todo.empty())
+    {
+        const Item it = todo.back();
+        todo.pop_back();
+        const uint32_t kind = it.ref & 0xc0000000u;
+        if (kind == XPBD_TREE_EMPTY)
+            continue;
+        if (kind == XPBD_TREE_LEAF)
+        {
+            const uint32_t first = it.ref & 0x00ffffffu, count = (it.ref >> 24) & 0x3fu;
+            if (count == 0 || (uint64_t)first + count > t.num_triangles)
+            {
+                err = "leaf outside the triangle table";
+                return false;
+            }
+            continue;
+        }
+        if (kind != XPBD_TREE_NODE || it.ref >= t.num_nodes)
+        {
+            err = "reference outside the node table";
+            return false;
+        }
+        if (seen[it.ref])
+        {
+            err = "a node is reached twice";
+            return false;
+        }
+        seen[it.ref] = 1;
+        if (it.depth + 1 > XPBD_TREE_MAX_DEPTH)
+        {
+            err = "deeper than XPBD_TREE_MAX_DEPTH";
+            return false;
+        }
+        for (int q = 0; q < 4; ++q)
+            todo.push_back({ t.nodes[it.ref].child[q], it.depth + 1 });
+    }
+    return true;
+}
+
+// A tree reference moved into the pools, whose nodes and triangles start at the given offsets
+uint32_t RebaseTreeRef(uint32_t ref, uint32_t node_base, uint32_t tri_base)
+{
+    const uint32_t kind = ref & 0xc0000000u;
+    if (kind == XPBD_TREE_NODE)
+        return ref + node_base;
+    if (kind == XPBD_TREE_LEAF)
+        return (ref & 0xff000000u) | ((ref & 0x00ffffffu) + tri_base);
+    return ref;
+}
+} // namespace
+
 int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t n)
 {
     if (!ctx || (n && !c))
@@ -1878,11 +1947,46 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
     std::vector<uint32_t> hull_idx;
     struct HullOffsets { uint32_t planes, points, face_first, face_count, index, num_faces, collider; };
     std::vector<HullOffsets> hull_offsets;
+    // triangle trees: validated here, nodes and triangles flattened into two pools
+    std::vector<TreeNodeDev> tree_nodes;
+    std::vector<float> tree_tris;
+    struct TreeOffsets { uint32_t nodes, tris, root, collider; };
+    std::vector<TreeOffsets> tree_offsets;
     for (uint32_t i = 0; i < n; ++i)
     {
-        if (c[i].type < XPBD_SHAPE_PLANE || c[i].type > XPBD_SHAPE_CONVEX_HULL)
+        if (c[i].type < XPBD_SHAPE_PLANE || c[i].type > XPBD_SHAPE_MESH)
             return Fail(XPBD_E_UNSUPPORTED, "collider shape type not supported (plane, sphere, box, capsule, cylinder, "
-                                            "tapered capsule, convex hull only)");
+                                            "tapered capsule, convex hull, tapered cylinder, mesh only)");
+        const bool unit = c[i].scale[0] == 0.0f && c[i].scale[1] == 0.0f && c[i].scale[2] == 0.0f;
+        const float sc[3] = { unit ? 1.0f : c[i].scale[0], unit ? 1.0f : c[i].scale[1], unit ? 1.0f : c[i].scale[2] };
+        for (int k = 0; k < 3; ++k)
+            if (!(std::fabs(sc[k]) >= 1.0e-6f) || !std::isfinite(sc[k]))
+                return Fail(XPBD_E_INVALID, "collider scale components must be finite and at least 1e-6 in magnitude");
+        if (c[i].type == XPBD_SHAPE_MESH)
+        {
+            const xpbd_triangle_tree *t = c[i].tree;
+            if (!t || !t->triangles || (t->num_nodes && !t->nodes))
+                return Fail(XPBD_E_INVALID, "mesh collider without tree data");
+            std::string err;
+            if (!ValidateTree(*t, err))
+                return Fail(XPBD_E_INVALID, "mesh collider tree: " + err);
+            TreeOffsets o;
+            o.nodes = (uint32_t)tree_nodes.size();
+            o.tris = (uint32_t)tree_tris.size() / 9u;
+            for (uint32_t k = 0; k < t->num_nodes; ++k)
+            {
+                TreeNodeDev d;
+                static_assert(sizeof(TreeNodeDev) == sizeof(xpbd_tree_node), "tree node layouts");
+                memcpy(&d, &t->nodes[k], sizeof(d));
+                for (int q = 0; q < 4; ++q)
+                    d.child[q] = RebaseTreeRef(d.child[q], o.nodes, o.tris);
+                tree_nodes.push_back(d);
+            }
+            tree_tris.insert(tree_tris.end(), t->triangles, t->triangles + 9 * (size_t)t->num_triangles);
+            o.root = RebaseTreeRef(t->root, o.nodes, o.tris);
+            o.collider = i;
+            tree_offsets.push_back(o);
+        }
         if (c[i].type == XPBD_SHAPE_CONVEX_HULL)
         {
             const xpbd_convex_hull *h = c[i].hull;
@@ -1920,6 +2024,8 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
             return Fail(XPBD_E_INVALID, "collider motion type must be 0 (static), 1 (kinematic) or 2 (dynamic)");
         v[i].type = c[i].type;
         v[i].hull = nullptr;
+        v[i].tree = nullptr;
+        memcpy(v[i].scale, sc, sizeof(v[i].scale));
         v[i].motion = c[i].motion_type;
         v[i].dyn = -1;
         v[i].sensor = c[i].is_sensor ? (int32_t)num_sensors++ : -1;
@@ -1939,6 +2045,8 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
         v[i].restitution = c[i].restitution;
         memcpy(v[i].bounds, c[i].world_bounds, sizeof(v[i].bounds));
     }
+    if (tree_nodes.size() >= (1u << 30) || tree_tris.size() / 9u >= (1u << 24))
+        return Fail(XPBD_E_CAPACITY, "the mesh colliders hold more than 2^30 nodes or 2^24 triangles together");
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     HIP_TRY(ctx->d_hull_vec.upload(hull_vec));
     HIP_TRY(ctx->d_hull_idx.upload(hull_idx));
@@ -1956,6 +2064,18 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
     HIP_TRY(ctx->d_hulls.upload(hulls));
     for (size_t k = 0; k < hull_offsets.size(); ++k)
         v[hull_offsets[k].collider].hull = ctx->d_hulls.p + k;
+    HIP_TRY(ctx->d_tree_nodes.upload(tree_nodes));
+    HIP_TRY(ctx->d_tree_tris.upload(tree_tris));
+    std::vector<TreeDev> trees(tree_offsets.size());
+    for (size_t k = 0; k < tree_offsets.size(); ++k)
+    {
+        trees[k].nodes = ctx->d_tree_nodes.p;
+        trees[k].triangles = ctx->d_tree_tris.p;
+        trees[k].root = tree_offsets[k].root;
+    }
+    HIP_TRY(ctx->d_trees.upload(trees));
+    for (size_t k = 0; k < tree_offsets.size(); ++k)
+        v[tree_offsets[k].collider].tree = ctx->d_trees.p + k;
     if (v.size() != ctx->colliders.size() || (num_dynamic > 0) != (ctx->num_dynamic > 0) || (num_sensors > 0) != (ctx->num_sensors > 0))
         ctx->layout_dirty = true;
     ctx->colliders = v;
@@ -1969,6 +2089,12 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
     }
     ctx->num_dynamic = num_dynamic;
     ctx->num_sensors = num_sensors;
+    ctx->ext_colliders = false;
+    for (const ColliderDev &d : v)
+        ctx->ext_colliders |= d.type == XPBD_SHAPE_MESH
+                              || (d.type == XPBD_SHAPE_CONVEX_HULL
+                                  && (d.scale[0] - 1.0f) * (d.scale[0] - 1.0f) + (d.scale[1] - 1.0f) * (d.scale[1] - 1.0f)
+                                         + (d.scale[2] - 1.0f) * (d.scale[2] - 1.0f) > 1.0e-8f);
     HIP_TRY(ctx->d_colliders.upload(v));
     return XPBD_OK;
 }
@@ -2163,7 +2289,7 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
     if (!ctx->teams.empty())
     {
         HIP_TRY(hipMemsetAsync(ctx->d_team_sync.p, 0, sizeof(uint32_t) * ctx->d_team_sync.n, ctx->stream));
-        HIP_TRY(launch_team(ctx->d_bodies.p, ctx->d_states.p, ctx->d_teams.p, ctx->d_team_blocks.p, (uint32_t)ctx->team_blocks.size(),
+        HIP_TRY(launch_team(ctx->ext_colliders, ctx->d_bodies.p, ctx->d_states.p, ctx->d_teams.p, ctx->d_team_blocks.p, (uint32_t)ctx->team_blocks.size(),
                             ctx->d_colliders.p, nc, ctx->team_max_leaves, ctx->team_lds, gx, gy, gz, dt, collision_steps, frames,
                             ctx->d_error.p, ctx->stream));
         double longest = 0.0;
@@ -2175,7 +2301,7 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
     for (const Launch &l : launches)
     {
         const int q = ctx->teams.empty() ? 0 : least_queued();
-        HIP_TRY(launch_update(l.resident, l.staged, l.rods, ctx->d_bodies.p, ctx->d_states.p, ctx->d_list.p + offset, (uint32_t)l.bodies.size(),
+        HIP_TRY(launch_update(l.resident, l.staged, l.rods, ctx->ext_colliders, ctx->d_bodies.p, ctx->d_states.p, ctx->d_list.p + offset, (uint32_t)l.bodies.size(),
                               ctx->d_colliders.p, nc, l.max_leaves, l.lds, gx, gy, gz, dt, collision_steps, frames, ctx->d_error.p, streams[q]));
         offset += (uint32_t)l.bodies.size();
         double longest = 0.0;
@@ -2188,7 +2314,7 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
         const auto &L = *Lp;
         const int best = least_queued();
         queued[best] += std::max(L.cost, L.work / 256.0);
-        HIP_TRY(launch_bundles(L.staged, L.block, L.uniform, L.table.p, L.count, ctx->d_bodies.p, ctx->d_states.p, ctx->d_colliders.p, nc,
+        HIP_TRY(launch_bundles(L.staged, L.block, L.uniform, ctx->ext_colliders, L.table.p, L.count, ctx->d_bodies.p, ctx->d_states.p, ctx->d_colliders.p, nc,
                                L.lds, gx, gy, gz, dt, collision_steps, frames, streams[best]));
     }
     for (int a = 0; a < num_aux; ++a)

@@ -102,6 +102,7 @@ __device__ __forceinline__ void project_coloured(const GlobalPositions &X, uint4
     }
 }
 
+template <bool EXT>
 __global__ __launch_bounds__(kTeamBlock) void team_kernel(
     const BodyDev *__restrict__ bodies, BodyStateDev *__restrict__ states, const TeamDev *__restrict__ teams,
     const uint2 *__restrict__ block_map, const ColliderDev *__restrict__ colliders, uint32_t num_colliders,
@@ -243,7 +244,7 @@ __global__ __launch_bounds__(kTeamBlock) void team_kernel(
                     Pl plane; plane.n = v3(pl.x, pl.y, pl.z); plane.c = pl.w;
                     float largest = -FLT_MAX;
                     int32_t idx = -1;
-                    collide_vertex(leaves, num_leaves, xyz(p), largest, idx, plane);
+                    collide_vertex<EXT>(leaves, num_leaves, xyz(p), largest, idx, plane);
                     if (idx >= 0)
                     {
                         B.idx[i] = idx;
@@ -506,20 +507,34 @@ uint32_t team_lds_bytes(uint32_t window_capacity, uint32_t max_leaves, uint32_t 
     return team_lds_layout(window_capacity, max_leaves, nc).total;
 }
 
-hipError_t launch_team(const BodyDev *bodies, BodyStateDev *states, const TeamDev *teams, const uint2 *block_map,
-                       uint32_t num_blocks, const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves,
-                       uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
-                       int32_t *error_flag, hipStream_t stream)
+template <bool EXT>
+static hipError_t launch_team_variant(const BodyDev *bodies, BodyStateDev *states, const TeamDev *teams, const uint2 *block_map,
+                                      uint32_t num_blocks, const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves,
+                                      uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
+                                      int32_t *error_flag, hipStream_t stream)
 {
     static const hipError_t configured =
-        hipFuncSetAttribute((const void *)team_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+        hipFuncSetAttribute((const void *)team_kernel<EXT>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
     if (configured != hipSuccess)
         return configured;
     if (num_blocks == 0)
         return hipSuccess;
-    hipLaunchKernelGGL(team_kernel, dim3(num_blocks), dim3(kTeamBlock), lds_bytes, stream, bodies, states, teams, block_map,
+    hipLaunchKernelGGL((team_kernel<EXT>), dim3(num_blocks), dim3(kTeamBlock), lds_bytes, stream, bodies, states, teams, block_map,
                        colliders, num_colliders, max_leaves, gx, gy, gz, dt, steps, frames, error_flag);
     return hipGetLastError();
+}
+
+// ext: the collider list holds triangle trees or scaled convex hulls
+hipError_t launch_team(bool ext, const BodyDev *bodies, BodyStateDev *states, const TeamDev *teams, const uint2 *block_map,
+                       uint32_t num_blocks, const ColliderDev *colliders, uint32_t num_colliders, uint32_t max_leaves,
+                       uint32_t lds_bytes, float gx, float gy, float gz, float dt, int32_t steps, int32_t frames,
+                       int32_t *error_flag, hipStream_t stream)
+{
+    if (ext)
+        return launch_team_variant<true>(bodies, states, teams, block_map, num_blocks, colliders, num_colliders, max_leaves, lds_bytes,
+                                         gx, gy, gz, dt, steps, frames, error_flag, stream);
+    return launch_team_variant<false>(bodies, states, teams, block_map, num_blocks, colliders, num_colliders, max_leaves, lds_bytes,
+                                      gx, gy, gz, dt, steps, frames, error_flag, stream);
 }
 
 } // namespace xpbd

@@ -146,10 +146,28 @@ struct HullDev
     uint32_t num_faces;
 };
 
+// A triangle tree (mesh or single triangle) as the reference walks it: four-way nodes with
+// float child bounds and child references (XPBD_TREE_* encoding), triangles 9 floats each.
+struct TreeNodeDev
+{
+    float min_x[4], min_y[4], min_z[4];
+    float max_x[4], max_y[4], max_z[4];
+    uint32_t child[4];
+};
+
+struct TreeDev
+{
+    const TreeNodeDev *nodes;
+    const float *triangles;
+    uint32_t root;
+};
+
 struct ColliderDev
 {
     int32_t type;
     float params[8];
+    float scale[3];             // the leaf shape's scale (ScaledShape)
+    const TreeDev *tree;        // XPBD_SHAPE_MESH
     const HullDev *hull;        // XPBD_SHAPE_CONVEX_HULL
     float pos[3];
     float rot[4];

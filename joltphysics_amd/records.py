@@ -55,6 +55,12 @@ LRA_NONE, LRA_EUCLIDEAN, LRA_GEODESIC = 0, 1, 2
 # Collider shape types of the C ABI
 SHAPE_PLANE, SHAPE_SPHERE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_CYLINDER, SHAPE_TAPERED_CAPSULE = 0, 1, 2, 3, 4, 5
 SHAPE_CONVEX_HULL = 6
+SHAPE_TAPERED_CYLINDER = 7      # device params {top, bottom, top radius, bottom radius}
+SHAPE_MESH = 8                  # triangle tree (MeshShape or TriangleShape)
+
+# xpbd_tree_node: the four children's bounds per axis and their references
+TREE_NODE = np.dtype([("min", "<f4", (3, 4)), ("max", "<f4", (3, 4)), ("child", "<u4", 4)])
+TREE_NODE_REF, TREE_LEAF_REF, TREE_EMPTY_REF = 0x00000000, 0x40000000, 0x80000000
 # EMotionType (MotionType.h:10-15)
 MOTION_STATIC, MOTION_KINEMATIC, MOTION_DYNAMIC = 0, 1, 2
 
@@ -170,6 +176,16 @@ class ConvexHull:
 
 
 @dataclass
+class TriangleTree:
+    """A mesh's bounding-volume tree in the form the device walks it (xpbd_triangle_tree):
+    root reference, nodes (TREE_NODE), triangles (T, 9) float32 in leaf order."""
+
+    root: int
+    nodes: np.ndarray
+    triangles: np.ndarray
+
+
+@dataclass
 class Collider:
     """A static collider body: shape, world transform (body position and rotation),
     material, world bounds.
@@ -204,6 +220,13 @@ class Collider:
     angular_velocity: tuple = (0.0, 0.0, 0.0)
     is_sensor: bool = False     # Body::IsSensor(): detects vertices, never collides
     user_id: int | None = None  # caller's identifier (BodyID); None: the object's identity
+    scale: tuple = (1.0, 1.0, 1.0)  # ScaledShape around the shape
+    # SHAPE_MESH construction: a MeshShape from (vertices (N, 3), indices (M, 3), max triangles
+    # per leaf or 0) or a TriangleShape from `triangle` (9 floats); `tree` is what the device
+    # consumes (taken from the reference's shape)
+    mesh: tuple | None = None
+    triangle: tuple | None = None
+    tree: TriangleTree | None = None
 
     def param_array(self):
         if self.type == SHAPE_CONVEX_HULL:

@@ -19,8 +19,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .records import (BEND_DIHEDRAL, BEND_NONE, EDGE, LRA, LRA_NONE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_CYLINDER,
-                      SHAPE_CONVEX_HULL, SHAPE_PLANE, SHAPE_SPHERE, SHAPE_TAPERED_CAPSULE, VERTEX, VOLUME, BodyParams, Collider,
-                      tapered_capsule_device_params)
+                      SHAPE_CONVEX_HULL, SHAPE_MESH, SHAPE_PLANE, SHAPE_SPHERE, SHAPE_TAPERED_CAPSULE,
+                      SHAPE_TAPERED_CYLINDER, VERTEX, VOLUME, BodyParams, Collider, tapered_capsule_device_params)
 
 # Build flags (same bit meaning on the product and oracle builders)
 CREATE_CONSTRAINTS = 1
@@ -195,9 +195,25 @@ def _rot_matrix(q):
 
 
 def collider_world_bounds(c: Collider):
-    """World-space AABB of a static collider body (conservative for planes)."""
+    """World-space AABB of a static collider body (conservative for planes, scaled shapes
+    and tapered cylinders)."""
     p = np.array(c.pos, np.float64)
     R = _rot_matrix(c.rot)
+    scale = np.abs(np.array(c.scale, np.float64))
+    if c.type == SHAPE_MESH:
+        pts = np.asarray(c.triangle if c.triangle is not None else c.mesh[0], np.float64).reshape(-1, 3)
+        w = (pts * np.array(c.scale, np.float64)) @ R.T + p
+        return tuple(np.concatenate([w.min(axis=0), w.max(axis=0)]).astype(np.float32))
+    if c.type == SHAPE_TAPERED_CYLINDER:
+        hh, r = c.params[0], max(c.params[1], c.params[2])
+        he = np.abs(R) @ (scale * np.array([r, 2.0 * hh, r]))
+        return tuple(np.concatenate([p - he, p + he]).astype(np.float32))
+    if (scale != 1.0).any() and c.type != SHAPE_PLANE:
+        # the unscaled bounds' extent around the body position, grown by the scale
+        unscaled = Collider(c.type, c.params, pos=(0.0, 0.0, 0.0), rot=(0.0, 0.0, 0.0, 1.0))
+        b = np.array(collider_world_bounds(unscaled), np.float64)
+        he = np.abs(R) @ (scale.max() * np.maximum(np.abs(b[:3]), np.abs(b[3:])))
+        return tuple(np.concatenate([p - he, p + he]).astype(np.float32))
     if c.type == SHAPE_SPHERE:
         he = np.full(3, c.params[0])
         return tuple(np.concatenate([p - he, p + he]).astype(np.float32))

@@ -18,7 +18,7 @@ import sys
 import numpy as np
 
 from .records import (BEND, EDGE, FACE, GROUP, INV_BIND, LRA, MOTION_DYNAMIC, ROD_BT, ROD_SS, SHAPE_CONVEX_HULL,
-                      SKINNED, VERTEX, VOLUME, BodyParams, SettingsArrays)
+                      SHAPE_MESH, SKINNED, TREE_NODE, VERTEX, VOLUME, BodyParams, SettingsArrays)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 # JOLT_XPBD_LIBRARY selects another build of the library (the sanitized one in the CPU tests)
@@ -61,13 +61,19 @@ class _ConvexHull(C.Structure):
                 ("num_vertex_index", C.c_uint32), ("vertex_index", C.c_void_p)]
 
 
+class _TriangleTree(C.Structure):
+    _fields_ = [("root", C.c_uint32), ("num_nodes", C.c_uint32), ("nodes", C.c_void_p),
+                ("num_triangles", C.c_uint32), ("triangles", C.c_void_p)]
+
+
 class _ColliderDesc(C.Structure):
     _fields_ = [("type", C.c_int32), ("params", C.c_float * 8), ("position", C.c_float * 3),
                 ("rotation", C.c_float * 4), ("friction", C.c_float), ("restitution", C.c_float),
                 ("world_bounds", C.c_float * 6), ("hull", C.POINTER(_ConvexHull)),
                 ("motion_type", C.c_int32), ("inv_mass", C.c_float), ("inv_inertia_diagonal", C.c_float * 3),
                 ("inertia_rotation", C.c_float * 4), ("linear_velocity", C.c_float * 3),
-                ("angular_velocity", C.c_float * 3), ("is_sensor", C.c_int32), ("user_id", C.c_uint32)]
+                ("angular_velocity", C.c_float * 3), ("is_sensor", C.c_int32), ("user_id", C.c_uint32),
+                ("scale", C.c_float * 3), ("tree", C.POINTER(_TriangleTree))]
 
 
 class _RolloutDesc(C.Structure):
@@ -351,6 +357,16 @@ class Context:
                                 a[3].ctypes.data, len(a[4]), a[4].ctypes.data)
                 keep.append((a, d))
                 arr[i].hull = C.pointer(d)
+            if c.type == SHAPE_MESH:
+                if c.tree is None:
+                    raise XpbdError("mesh collider without tree data")
+                t = c.tree
+                a = (np.ascontiguousarray(t.nodes, TREE_NODE), np.ascontiguousarray(t.triangles, np.float32).reshape(-1, 9))
+                d = _TriangleTree(int(t.root), len(a[0]), a[0].ctypes.data if len(a[0]) else None, len(a[1]), a[1].ctypes.data)
+                keep.append((a, d))
+                arr[i].tree = C.pointer(d)
+            for k in range(3):
+                arr[i].scale[k] = float(c.scale[k])
             p = c.device_param_array()
             for k in range(8):
                 arr[i].params[k] = float(p[k])

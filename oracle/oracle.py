@@ -17,8 +17,8 @@ import os
 import numpy as np
 
 from joltphysics_amd.records import (BEND, EDGE, FACE, GROUP, INV_BIND, LRA, MOTION_DYNAMIC, ROD_BT, ROD_SS,
-                                     SHAPE_CONVEX_HULL, SKINNED,
-                                     VERTEX, VOLUME, ConvexHull, SettingsArrays)
+                                     SHAPE_CONVEX_HULL, SHAPE_MESH, SKINNED, TREE_NODE,
+                                     VERTEX, VOLUME, ConvexHull, SettingsArrays, TriangleTree)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(HERE, "_ref")
@@ -99,6 +99,14 @@ class JoltReference:
             lib.jo_world_collider_device.argtypes = [_vp, _i, _vp, _vp]
             lib.jo_world_add_hull.restype = _i
             lib.jo_world_add_hull.argtypes = [_vp, _vp, _i, _vp, _vp, _f, _f]
+            lib.jo_world_set_next_scale.restype = None
+            lib.jo_world_set_next_scale.argtypes = [_vp, _vp]
+            lib.jo_world_add_triangle.restype = _i
+            lib.jo_world_add_triangle.argtypes = [_vp, _vp, _vp, _vp, _f, _f]
+            lib.jo_world_add_mesh.restype = _i
+            lib.jo_world_add_mesh.argtypes = [_vp, _vp, _i, _vp, _i, _i, _vp, _vp, _f, _f]
+            lib.jo_world_collider_describe.restype = _i
+            lib.jo_world_collider_describe.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
             lib.jo_world_hull_data.restype = _i
             lib.jo_world_add_dynamic.restype = _i
             lib.jo_world_add_dynamic.argtypes = [_vp, _i, _vp, _vp, _vp, _f, _f, _vp, _vp]
@@ -239,6 +247,24 @@ class JoltWorld:
     def add_collider(self, c):
         pos = np.array(c.pos, np.float32)
         rot = np.array(c.rot, np.float32)
+        if tuple(float(x) for x in c.scale) != (1.0, 1.0, 1.0):
+            self.lib.jo_world_set_next_scale(self.h, _ptr(np.array(c.scale, np.float32)))
+        if c.type == SHAPE_MESH and c.motion_type != MOTION_DYNAMIC:
+            if c.triangle is not None:
+                v = np.array(c.triangle, np.float32)
+                i = self.lib.jo_world_add_triangle(self.h, _ptr(v), _ptr(pos), _ptr(rot), float(c.friction), float(c.restitution))
+            else:
+                verts = np.ascontiguousarray(c.mesh[0], np.float32).reshape(-1, 3)
+                idx = np.ascontiguousarray(c.mesh[1], np.uint32).reshape(-1, 3)
+                per_leaf = int(c.mesh[2]) if len(c.mesh) > 2 else 0
+                i = self.lib.jo_world_add_mesh(self.h, _ptr(verts), len(verts), _ptr(idx), len(idx), per_leaf, _ptr(pos), _ptr(rot),
+                                               float(c.friction), float(c.restitution))
+            if i < 0:
+                raise RuntimeError("add_collider failed")
+            if c.is_sensor:
+                self.lib.jo_world_set_sensor(self.h, i)
+            self._num_colliders = i + 1
+            return i
         if c.motion_type == MOTION_DYNAMIC:
             p = c.param_array()
             lin = np.array(c.linear_velocity, np.float32)
@@ -264,6 +290,21 @@ class JoltWorld:
         out = np.zeros(6, np.float32)
         self.lib.jo_world_collider_bounds(self.h, i, _ptr(out))
         return tuple(float(x) for x in out)
+
+    def collider_tree(self, i):
+        """The triangle tree of mesh / triangle collider i as the device takes it."""
+        t, params, scale, com = C.c_int(), np.zeros(8, np.float32), np.zeros(3, np.float32), np.zeros(3, np.float32)
+        sizes = np.zeros(2, np.int32)
+        if self.lib.jo_world_collider_describe(self.h, i, C.byref(t), _ptr(params), _ptr(scale), _ptr(com), _ptr(sizes),
+                                               None, None, None) != 0:
+            raise RuntimeError("collider_describe failed")
+        root = C.c_uint32()
+        nodes = np.zeros(int(sizes[0]), TREE_NODE)
+        tris = np.zeros((int(sizes[1]), 9), np.float32)
+        self.lib.jo_world_collider_describe(self.h, i, C.byref(t), _ptr(params), _ptr(scale), _ptr(com), _ptr(sizes),
+                                            C.byref(root), nodes.ctypes.data if len(nodes) else _ptr(np.zeros(28, np.uint32)),
+                                            _ptr(tris) if len(tris) else _ptr(np.zeros(9, np.float32)))
+        return TriangleTree(int(root.value), nodes, tris)
 
     def collider_device(self, i):
         """(shape parameters as the shape holds them [8], centre-of-mass position [3])."""
@@ -421,6 +462,10 @@ class Restatement:
             lib.xr_body_velocity_deltas.restype = _i
             lib.xr_body_velocity_deltas.argtypes = [_vp, _i, _i, _vp, _vp, _vp]
             lib.xr_world_set_hull.argtypes = [_vp, _i, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _u32]
+            lib.xr_world_set_collider_scale.restype = _i
+            lib.xr_world_set_collider_scale.argtypes = [_vp, _i, _vp]
+            lib.xr_world_set_tree.restype = _i
+            lib.xr_world_set_tree.argtypes = [_vp, _i, _u32, _vp, _vp, _u32, _vp, _u32]
             lib.xr_world_add_soft_body.restype = _i
             lib.xr_world_add_soft_body.argtypes = [_vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32,
                                                    _vp, _u32, _vp, _u32, _vp, _vp, _vp]
@@ -461,6 +506,18 @@ class Restatement:
             if self.lib.xr_world_set_hull(self.h, i, _ptr(a[0]), len(a[0]), _ptr(a[1]), len(a[1]), _ptr(a[2]),
                                           _ptr(a[3]), _ptr(a[4]), len(a[4])) != 0:
                 raise RuntimeError("invalid convex hull data")
+        if i >= 0 and c.type == SHAPE_MESH:
+            if c.tree is None:
+                raise RuntimeError("mesh collider without tree data")
+            nodes = np.ascontiguousarray(c.tree.nodes, TREE_NODE)
+            bounds = np.ascontiguousarray(np.concatenate([nodes["min"].reshape(-1, 12), nodes["max"].reshape(-1, 12)], axis=1), np.float32)
+            children = np.ascontiguousarray(nodes["child"], np.uint32)
+            tris = np.ascontiguousarray(c.tree.triangles, np.float32)
+            if self.lib.xr_world_set_tree(self.h, i, int(c.tree.root), _ptr(bounds) if len(nodes) else None,
+                                          _ptr(children) if len(nodes) else None, len(nodes), _ptr(tris), len(tris)) != 0:
+                raise RuntimeError("invalid triangle tree")
+        if i >= 0 and tuple(float(x) for x in c.scale) != (1.0, 1.0, 1.0):
+            self.lib.xr_world_set_collider_scale(self.h, i, _ptr(np.array(c.scale, np.float32)))
         if i >= 0 and c.is_sensor:
             self.lib.xr_world_set_sensor(self.h, i)
         if i >= 0 and c.motion_type != 0:

@@ -287,6 +287,13 @@ struct Collider
 	float inv_mass = 0, inv_inertia_diag[3] = { 0, 0, 0 }, inertia_rot[4] = { 0, 0, 0, 1 };
 	float lin_vel[3] = { 0, 0, 0 }, ang_vel[3] = { 0, 0, 0 };
 	int is_sensor = 0;     // sensors never collide (DetermineCollidingShapes keeps them apart)
+	float scale[3] = { 1, 1, 1 };   // ScaledShape around the shape (ScaledShape.cpp:142-145)
+	// triangle tree (type 8) as MeshShape walks it: per node the four children's bounds
+	// (min xyz, max xyz, 4 each) and references; triangles 9 floats each
+	std::vector<float> tree_bounds;
+	std::vector<u32> tree_child;
+	std::vector<float> tree_tris;
+	u32 tree_root = 0;
 };
 
 template <class F> struct Leaf
@@ -352,6 +359,165 @@ template <class F> struct World
 	std::vector<Body<F>> bodies;
 };
 
+// ClosestPoint::GetClosestPointOnTriangle (ClosestPoint.h:157-318, MustIncludeC false): the
+// point of triangle (a, b, c) nearest the origin and its feature (bit 0 a, bit 1 b, bit 2 c,
+// 7 the interior).  Without FMA, CrossPrecise is Cross.
+template <class F> V3<F> ClosestPointOnTriangle(V3<F> in_a, V3<F> in_b, V3<F> in_c, u32 &out_set)
+{
+	auto clamp01 = [](F v) { F lo = v > F(0) ? v : F(0); return lo < F(1) ? lo : F(1); };
+	V3<F> ba = in_a - in_b, bc = in_c - in_b;
+	bool swap_ac = Dot(bc, bc) < Dot(ba, ba);
+	V3<F> a = swap_ac ? in_c : in_a;
+	V3<F> c = swap_ac ? in_a : in_c;
+	V3<F> ab = in_b - a;
+	V3<F> ac = c - a;
+	V3<F> n = Cross(ab, ac);
+	F n_len_sq = LenSq(n);
+	if (n_len_sq < F(1.0e-10f))
+	{
+		u32 closest_set = 4;
+		V3<F> closest_point = in_c;
+		F best = LenSq(in_c);
+		F a_len_sq = LenSq(in_a);
+		if (a_len_sq < best) { closest_set = 1; closest_point = in_a; best = a_len_sq; }
+		F b_len_sq = LenSq(in_b);
+		if (b_len_sq < best) { closest_set = 2; closest_point = in_b; best = b_len_sq; }
+		const F eps_sq = F(FLT_EPSILON) * F(FLT_EPSILON);
+		F ac_len_sq = LenSq(ac);
+		if (ac_len_sq > eps_sq)
+		{
+			F v = clamp01(-Dot(a, ac) / ac_len_sq);
+			V3<F> q = a + v * ac;
+			F d = LenSq(q);
+			if (d < best) { closest_set = 5; closest_point = q; best = d; }
+		}
+		V3<F> bc2 = in_c - in_b;
+		F bc_len_sq = LenSq(bc2);
+		if (bc_len_sq > eps_sq)
+		{
+			F v = clamp01(-Dot(in_b, bc2) / bc_len_sq);
+			V3<F> q = in_b + v * bc2;
+			F d = LenSq(q);
+			if (d < best) { closest_set = 6; closest_point = q; best = d; }
+		}
+		V3<F> ab2 = in_b - in_a;
+		F ab_len_sq = LenSq(ab2);
+		if (ab_len_sq > eps_sq)
+		{
+			F v = clamp01(-Dot(in_a, ab2) / ab_len_sq);
+			V3<F> q = in_a + v * ab2;
+			F d = LenSq(q);
+			if (d < best) { closest_set = 3; closest_point = q; best = d; }
+		}
+		out_set = closest_set;
+		return closest_point;
+	}
+	V3<F> ap = Neg(a);
+	F d1 = Dot(ab, ap), d2 = Dot(ac, ap);
+	if (d1 <= F(0) && d2 <= F(0)) { out_set = swap_ac ? 4 : 1; return a; }
+	V3<F> bp = Neg(in_b);
+	F d3 = Dot(ab, bp), d4 = Dot(ac, bp);
+	if (d3 >= F(0) && d4 <= d3) { out_set = 2; return in_b; }
+	if (d1 * d4 <= d3 * d2 && d1 >= F(0) && d3 <= F(0)) { F v = d1 / (d1 - d3); out_set = swap_ac ? 6 : 3; return a + v * ab; }
+	V3<F> cp = Neg(c);
+	F d5 = Dot(ab, cp), d6 = Dot(ac, cp);
+	if (d6 >= F(0) && d5 <= d6) { out_set = swap_ac ? 1 : 4; return c; }
+	if (d5 * d2 <= d1 * d6 && d2 >= F(0) && d6 <= F(0)) { F w = d2 / (d2 - d6); out_set = 5; return a + w * ac; }
+	F d4_d3 = d4 - d3, d5_d6 = d5 - d6;
+	if (d3 * d6 <= d5 * d4 && d4_d3 >= F(0) && d5_d6 >= F(0)) { F w = d4_d3 / (d4_d3 + d5_d6); out_set = swap_ac ? 3 : 6; return in_b + w * (c - in_b); }
+	out_set = 7;
+	return (n * Dot((a + in_b) + c, n)) / (F(3.0f) * n_len_sq);
+}
+
+template <class F> struct TriangleHit
+{
+	V3<F> v0, v1, v2, closest;
+	F dist_sq = F(FLT_MAX);
+	u32 set = 0;
+};
+
+// The closest triangle of a triangle tree to `local` as MeshShape's walk finds it
+// (MeshShape.cpp:833-885, NodeCodecQuadTreeHalfFloat.h:245-318, SortReverseAndStore.h:15-33):
+// recursion restated as the reference's explicit stack, children sorted far to near by the
+// Vec4::sSort4Reverse network, pruned against the closest distance so far.
+template <class F> TriangleHit<F> WalkTriangleTree(const Collider &c, V3<F> scale, V3<F> local)
+{
+	TriangleHit<F> hit;
+	std::vector<u32> stack{ c.tree_root };
+	std::vector<F> dist{ F(0) };
+	int top = 0;
+	do
+	{
+		const u32 ref = stack[top];
+		const u32 kind = ref & 0xc0000000u;
+		if (kind == 0u)
+		{
+			const float *bd = &c.tree_bounds[24 * size_t(ref)];
+			F d[4];
+			u32 id[4];
+			for (int k = 0; k < 4; ++k)
+			{
+				F lo[3], hi[3];
+				const F sc[3] = { scale.x, scale.y, scale.z };
+				for (int a = 0; a < 3; ++a)
+				{
+					F s0 = sc[a] * F(bd[4 * a + k]), s1 = sc[a] * F(bd[12 + 4 * a + k]);
+					lo[a] = SMin(s0, s1);
+					hi[a] = SMax(s0, s1);
+				}
+				F cx = SMin(SMax(local.x, lo[0]), hi[0]) - local.x;
+				F cy = SMin(SMax(local.y, lo[1]), hi[1]) - local.y;
+				F cz = SMin(SMax(local.z, lo[2]), hi[2]) - local.z;
+				d[k] = (cx * cx + cy * cy) + cz * cz;
+				id[k] = c.tree_child[4 * size_t(ref) + k];
+			}
+			auto exchange = [&](int p, int q) { if (d[q] > d[p]) { std::swap(d[p], d[q]); std::swap(id[p], id[q]); } };
+			exchange(0, 2); exchange(1, 3);
+			exchange(0, 1); exchange(2, 3);
+			exchange(1, 2);
+			int n = 0;
+			for (int k = 0; k < 4; ++k)
+				n += d[k] < hit.dist_sq ? 1 : 0;
+			if (size_t(top + n) > stack.size())
+			{
+				stack.resize(top + n);
+				dist.resize(top + n);
+			}
+			for (int j = 0; j < n; ++j)
+			{
+				stack[top + j] = id[4 - n + j];
+				dist[top + j] = d[4 - n + j];
+			}
+			top += n;
+		}
+		else if (kind == 0x40000000u)
+		{
+			const u32 first = ref & 0x00ffffffu, count = (ref >> 24) & 0x3fu;
+			for (u32 t = first; t < first + count; ++t)
+			{
+				const float *tv = &c.tree_tris[9 * size_t(t)];
+				V3<F> v0 = Mul(scale, V3<F>{ F(tv[0]), F(tv[1]), F(tv[2]) });
+				V3<F> v1 = Mul(scale, V3<F>{ F(tv[3]), F(tv[4]), F(tv[5]) });
+				V3<F> v2 = Mul(scale, V3<F>{ F(tv[6]), F(tv[7]), F(tv[8]) });
+				u32 set;
+				V3<F> cp = ClosestPointOnTriangle(v0 - local, v1 - local, v2 - local, set);
+				F d2 = LenSq(cp);
+				if (d2 < hit.dist_sq)
+				{
+					hit.v0 = v0; hit.v1 = v1; hit.v2 = v2;
+					hit.closest = cp;
+					hit.dist_sq = d2;
+					hit.set = set;
+				}
+			}
+		}
+		do
+			--top;
+		while (top >= 0 && !(dist[top] < hit.dist_sq));
+	} while (top >= 0);
+	return hit;
+}
+
 // ------------------------------------------------------------------ shape collision
 // Restates Shape::CollideSoftBodyVertices for the four primitive shapes, with
 // UpdatePenetration / SetCollision of CollideSoftBodyVertexIterator.h:83-97.
@@ -360,6 +526,8 @@ template <class F> void CollideVertices(Body<F> &b, const Leaf<F> &leaf, int sha
 	const Collider &c = *leaf.col;
 	const M44<F> &T = leaf.transform;
 	const size_t nv = b.x.size();
+	const V3<F> scale{ F(c.scale[0]), F(c.scale[1]), F(c.scale[2]) };
+	const V3<F> abs_scale = VAbs(scale);
 	auto update = [&](size_t i, F pen) {
 		if (b.largest[i] >= pen)
 			return false;
@@ -368,11 +536,10 @@ template <class F> void CollideVertices(Body<F> &b, const Leaf<F> &leaf, int sha
 	};
 	switch (c.type)
 	{
-	case 0: // PlaneShape.cpp:243-256 with Plane::Scaled (scale 1, Plane.h:49-54) and GetTransformed
+	case 0: // PlaneShape.cpp:243-256 with Plane::Scaled (Plane.h:49-54) and GetTransformed
 		{
 			PlaneT<F> p{ { F(c.params[0]), F(c.params[1]), F(c.params[2]) }, F(c.params[3]) };
-			V3<F> one{ 1, 1, 1 };
-			V3<F> sn{ p.n.x / one.x, p.n.y / one.y, p.n.z / one.z };
+			V3<F> sn{ p.n.x / scale.x, p.n.y / scale.y, p.n.z / scale.z };
 			F snl = Len(sn);
 			PlaneT<F> scaled{ sn / snl, p.c / snl };
 			PlaneT<F> plane = PlaneTransformed(scaled, T);
@@ -387,7 +554,7 @@ template <class F> void CollideVertices(Body<F> &b, const Leaf<F> &leaf, int sha
 	case 1: // SphereShape.cpp:279-301
 		{
 			V3<F> center = Col3(T, 3);
-			F radius = F(c.params[0]);
+			F radius = abs_scale.x * F(c.params[0]);	// SphereShape::GetScaledRadius
 			for (size_t i = 0; i < nv; ++i)
 				if (b.w[i] > F(0))
 				{
@@ -408,7 +575,7 @@ template <class F> void CollideVertices(Body<F> &b, const Leaf<F> &leaf, int sha
 		{
 			M44<F> inv = InvRT(T);
 			V3<F> he{ F(c.params[0]), F(c.params[1]), F(c.params[2]) };
-			he = Mul(VAbs(V3<F>{ 1, 1, 1 }), he);
+			he = Mul(abs_scale, he);
 			for (size_t i = 0; i < nv; ++i)
 				if (b.w[i] > F(0))
 				{
@@ -443,12 +610,12 @@ template <class F> void CollideVertices(Body<F> &b, const Leaf<F> &leaf, int sha
 				}
 		}
 		break;
-	case 3: // CapsuleShape.cpp:324-375 (scale 1)
+	case 3: // CapsuleShape.cpp:324-375
 		{
 			M44<F> inv = InvRT(T);
-			F scale = std::abs(F(1));
-			F hh = scale * F(c.params[0]);
-			F radius = scale * F(c.params[1]);
+			F s_xz = std::abs(scale.x);
+			F hh = s_xz * F(c.params[0]);
+			F radius = s_xz * F(c.params[1]);
 			for (size_t i = 0; i < nv; ++i)
 				if (b.w[i] > F(0))
 				{
@@ -483,11 +650,11 @@ template <class F> void CollideVertices(Body<F> &b, const Leaf<F> &leaf, int sha
 				}
 		}
 		break;
-	case 4: // CylinderShape.cpp:317-366 (scale 1)
+	case 4: // CylinderShape.cpp:317-366
 		{
 			M44<F> inv = InvRT(T);
-			F half_height = std::abs(F(1)) * F(c.params[0]);
-			F radius = std::abs(F(1)) * F(c.params[1]);
+			F half_height = abs_scale.y * F(c.params[0]);
+			F radius = abs_scale.x * F(c.params[1]);
 			for (size_t i = 0; i < nv; ++i)
 				if (b.w[i] > F(0))
 				{
@@ -522,16 +689,17 @@ template <class F> void CollideVertices(Body<F> &b, const Leaf<F> &leaf, int sha
 				}
 		}
 		break;
-	case 5: // TaperedCapsuleShape.cpp:304-371 (scale 1)
+	case 5: // TaperedCapsuleShape.cpp:304-371
 		{
 			M44<F> inv = InvRT(T);
-			V3<F> top_center{ 0, F(c.params[0]), 0 }, bottom_center{ 0, F(c.params[1]), 0 };
-			F top_radius = F(c.params[2]), bottom_radius = F(c.params[3]);
+			V3<F> top_center{ 0, abs_scale.y * F(c.params[0]), 0 }, bottom_center{ 0, abs_scale.y * F(c.params[1]), 0 };
+			F top_radius = abs_scale.x * F(c.params[2]), bottom_radius = abs_scale.x * F(c.params[3]);
 			F sin_alpha = F(c.params[4]), tan_alpha = F(c.params[5]);
+			V3<F> scale_y_flip{ 1, SignOf(scale.y), 1 };
 			for (size_t i = 0; i < nv; ++i)
 				if (b.w[i] > F(0))
 				{
-					V3<F> local = MulPoint(inv, b.x[i]);
+					V3<F> local = Mul(scale_y_flip, MulPoint(inv, b.x[i]));
 					V3<F> position, normal;
 					V3<F> to_top = local - top_center;
 					F to_top_len = Len(to_top);
@@ -558,16 +726,21 @@ template <class F> void CollideVertices(Body<F> &b, const Leaf<F> &leaf, int sha
 					F pen = -SignedDistance(plane, local);
 					if (update(i, pen))
 					{
+						plane.n = Mul(scale_y_flip, plane.n);
 						b.plane[i] = PlaneTransformed(plane, T);
 						b.shape_idx[i] = shape_index;
 					}
 				}
 		}
 		break;
-	case 6: // ConvexHullShape.cpp:1060-1164 (scale 1: the scaled-plane branch and scale_flip drop out)
+	case 6: // ConvexHullShape.cpp:1060-1164
 		{
 			M44<F> inv = InvRT(T);
 			const size_t nf = c.face_count.size();
+			const V3<F> inv_scale{ F(1) / scale.x, F(1) / scale.y, F(1) / scale.z };	// Vec3::Reciprocal
+			const bool is_not_scaled = LenSq(scale - V3<F>{ 1, 1, 1 }) <= F(1.0e-8f);	// ScaleHelpers::IsNotScaled
+			const int negatives = int(scale.x < F(0)) + int(scale.y < F(0)) + int(scale.z < F(0));
+			const F scale_flip = (negatives & 1) ? F(-1) : F(1);						// ScaleHelpers::IsInsideOut
 			auto point = [&](u32 k) { return V3<F>{ F(c.hull_points[3 * k]), F(c.hull_points[3 * k + 1]), F(c.hull_points[3 * k + 2]) }; };
 			for (size_t i = 0; i < nv; ++i)
 				if (b.w[i] > F(0))
@@ -579,12 +752,27 @@ template <class F> void CollideVertices(Body<F> &b, const Leaf<F> &leaf, int sha
 					size_t max_plane_idx = 0;
 					for (size_t f = 0; f < nf; ++f)
 					{
-						PlaneT<F> p{ { F(c.hull_planes[4 * f]), F(c.hull_planes[4 * f + 1]), F(c.hull_planes[4 * f + 2]) }, F(c.hull_planes[4 * f + 3]) };
-						F distance = SignedDistance(p, local);
+						V3<F> n{ F(c.hull_planes[4 * f]), F(c.hull_planes[4 * f + 1]), F(c.hull_planes[4 * f + 2]) };
+						F distance;
+						V3<F> plane_normal;
+						if (is_not_scaled)
+						{
+							PlaneT<F> p{ n, F(c.hull_planes[4 * f + 3]) };
+							distance = SignedDistance(p, local);
+							plane_normal = n;
+						}
+						else
+						{
+							// the plane rebuilt from the scaled normal and the scaled first point of the face
+							V3<F> sn = Mul(inv_scale, n);
+							plane_normal = sn / Len(sn);
+							V3<F> plane_point = Mul(scale, point(c.index[c.face_first[f]]));
+							distance = Dot(plane_normal, local - plane_point);
+						}
 						if (distance > max_distance)
 						{
 							max_distance = distance;
-							max_plane_normal = p.n;
+							max_plane_normal = plane_normal;
 							max_plane_idx = f;
 						}
 					}
@@ -597,10 +785,10 @@ template <class F> void CollideVertices(Body<F> &b, const Leaf<F> &leaf, int sha
 						const u32 first = c.face_first[max_plane_idx], count = c.face_count[max_plane_idx];
 						for (u32 k = 0; k < count; ++k)
 						{
-							V3<F> p1 = point(c.index[first + k]);
-							V3<F> p2 = point(c.index[first + (k + 1 == count ? 0 : k + 1)]);
+							V3<F> p1 = Mul(scale, point(c.index[first + k]));
+							V3<F> p2 = Mul(scale, point(c.index[first + (k + 1 == count ? 0 : k + 1)]));
 							V3<F> edge_normal = Cross(p2 - p1, max_plane_normal);
-							if (Dot(edge_normal, local - p1) > F(0))
+							if (scale_flip * Dot(edge_normal, local - p1) > F(0))
 							{
 								V3<F> closest = ClosestPointOnLine(p1 - local, p2 - local);
 								F distance_sq = LenSq(closest);
@@ -624,6 +812,134 @@ template <class F> void CollideVertices(Body<F> &b, const Leaf<F> &leaf, int sha
 						normal = normal_length > F(1.0e-12f) ? normal / normal_length : max_plane_normal;
 						b.plane[i] = PlaneTransformed(PlaneFromPointNormal(local + closest_point, normal), T);
 						b.shape_idx[i] = shape_index;
+					}
+				}
+		}
+		break;
+	case 7: // TaperedCylinderShape.cpp:390-506 with GetScaled (:181-199)
+		{
+			M44<F> inv = InvRT(T);
+			F top = scale.y * F(c.params[0]), bottom = scale.y * F(c.params[1]);
+			F top_radius = abs_scale.x * F(c.params[2]), bottom_radius = abs_scale.x * F(c.params[3]);
+			if (bottom > top)
+			{
+				std::swap(top, bottom);
+				std::swap(top_radius, bottom_radius);
+			}
+			const V3<F> top_3d{ 0, top, 0 }, bottom_3d{ 0, bottom, 0 };
+			for (size_t i = 0; i < nv; ++i)
+				if (b.w[i] > F(0))
+				{
+					V3<F> local = MulPoint(inv, b.x[i]);
+					V3<F> normal_xz = NormalizedOr(Mul(V3<F>{ 1, 0, 1 }, local), V3<F>{ 1, 0, 0 });
+					F tan_alpha = (bottom_radius - top_radius) / (top - bottom);
+					V3<F> sn{ normal_xz.x, tan_alpha, normal_xz.z };
+					V3<F> side_normal = sn / Len(sn);
+					V3<F> side_support_top = normal_xz * top_radius + top_3d;
+					F side_penetration = Dot(side_support_top - local, side_normal);
+					F top_penetration = top - local.y;
+					F bottom_penetration = local.y - bottom;
+					F min_top_bottom_penetration = std::min(top_penetration, bottom_penetration);
+					V3<F> point, normal;
+					if (side_penetration < F(0) || min_top_bottom_penetration < F(0))
+					{
+						V3<F> side_support_bottom = normal_xz * bottom_radius + bottom_3d;
+						V3<F> bottom_to_top = side_support_top - side_support_bottom;
+						F fraction = Dot(local - side_support_bottom, bottom_to_top);
+						F distance_to_axis = Dot(normal_xz, local);
+						bool inside_top_radius = distance_to_axis <= top_radius;
+						bool inside_bottom_radius = distance_to_axis <= bottom_radius;
+						if (fraction >= LenSq(bottom_to_top) && !inside_top_radius)
+						{
+							point = side_support_top;
+							normal = NormalizedOr(local - point, V3<F>{ 0, 1, 0 });
+						}
+						else if (fraction < F(0) && !inside_bottom_radius)
+						{
+							point = side_support_bottom;
+							normal = NormalizedOr(local - point, V3<F>{ 0, 1, 0 });
+						}
+						else if (top_penetration < F(0) && inside_top_radius)
+						{
+							point = top_3d;
+							normal = V3<F>{ 0, 1, 0 };
+						}
+						else if (bottom_penetration < F(0) && inside_bottom_radius)
+						{
+							point = bottom_3d;
+							normal = V3<F>{ 0, -1, 0 };
+						}
+						else
+						{
+							point = side_support_top;
+							normal = side_normal;
+						}
+					}
+					else if (side_penetration < min_top_bottom_penetration)
+					{
+						point = side_support_top;
+						normal = side_normal;
+					}
+					else if (top_penetration < bottom_penetration)
+					{
+						point = top_3d;
+						normal = V3<F>{ 0, 1, 0 };
+					}
+					else
+					{
+						point = bottom_3d;
+						normal = V3<F>{ 0, -1, 0 };
+					}
+					PlaneT<F> plane = PlaneFromPointNormal(point, normal);
+					F pen = -SignedDistance(plane, local);
+					if (update(i, pen))
+					{
+						b.plane[i] = PlaneTransformed(plane, T);
+						b.shape_idx[i] = shape_index;
+					}
+				}
+		}
+		break;
+	case 8: // MeshShape.cpp:833-885 / TriangleShape.cpp:272-283 through CollideSoftBodyVerticesVsTriangles
+		{
+			M44<F> inv = InvRT(T);
+			const int negatives = int(scale.x < F(0)) + int(scale.y < F(0)) + int(scale.z < F(0));
+			const F normal_sign = (negatives & 1) ? F(-1) : F(1);
+			for (size_t i = 0; i < nv; ++i)
+				if (b.w[i] > F(0))
+				{
+					const V3<F> local = MulPoint(inv, b.x[i]);
+					TriangleHit<F> hit = WalkTriangleTree(c, scale, local);
+					if (!(hit.dist_sq < F(FLT_MAX)))
+						continue;
+					// FinishVertex: back to the soft body's space
+					V3<F> v0 = MulPoint(T, hit.v0), v1 = MulPoint(T, hit.v1), v2 = MulPoint(T, hit.v2);
+					V3<F> triangle_normal = normal_sign * NormalizedOr(Cross(v1 - v0, v2 - v0), V3<F>{ 0, 1, 0 });
+					if (hit.set == 7u)
+					{
+						// interior: the triangle's plane, at most sTriangleThickness (0.1) deep
+						F penetration = Dot(triangle_normal, v0 - b.x[i]);
+						if (penetration < F(0.1f) && update(i, penetration))
+						{
+							b.plane[i] = PlaneFromPointNormal(v0, triangle_normal);
+							b.shape_idx[i] = shape_index;
+						}
+					}
+					else
+					{
+						// edge or vertex: the plane through the closest point facing the vertex
+						V3<F> closest_point = MulPoint(T, local + hit.closest);
+						V3<F> normal = b.x[i] - closest_point;
+						if (Dot(normal, triangle_normal) > F(0))
+						{
+							F normal_length = Len(normal);
+							F penetration = -normal_length;
+							if (update(i, penetration))
+							{
+								b.plane[i] = PlaneFromPointNormal(closest_point, normal_length > F(0) ? normal / normal_length : triangle_normal);
+								b.shape_idx[i] = shape_index;
+							}
+						}
 					}
 				}
 		}
@@ -1187,11 +1503,11 @@ void xr_world_set_gravity(void *wp, float gx, float gy, float gz)
 int xr_world_add_collider(void *wp, int type, const float *params, const float *pos, const float *rot, float friction, float restitution, const float *bounds)
 {
 	XrWorld *w = (XrWorld *)wp;
-	if (type < 0 || type > 6)
+	if (type < 0 || type > 8)
 		return -1;
 	Collider c{};
 	c.type = type;
-	int np = type == 0 ? 5 : type == 1 ? 1 : type == 2 ? 3 : type == 5 ? 6 : type == 6 ? 0 : 2;
+	int np = type == 0 ? 5 : type == 1 ? 1 : type == 2 ? 3 : type == 5 ? 6 : type == 6 || type == 8 ? 0 : type == 7 ? 4 : 2;
 	for (int i = 0; i < np; ++i) c.params[i] = params[i];
 	for (int i = 0; i < 3; ++i) c.pos[i] = pos[i];
 	for (int i = 0; i < 4; ++i) c.rot[i] = rot[i];
@@ -1201,6 +1517,39 @@ int xr_world_add_collider(void *wp, int type, const float *params, const float *
 	w->f.colliders.push_back(c);
 	w->d.colliders.push_back(c);
 	return int(w->f.colliders.size() - 1);
+}
+
+// Scale of collider `collider` (a ScaledShape around its shape)
+int xr_world_set_collider_scale(void *wp, int collider, const float *scale3)
+{
+	XrWorld *w = (XrWorld *)wp;
+	if (collider < 0 || size_t(collider) >= w->f.colliders.size())
+		return -1;
+	for (int k = 0; k < 3; ++k)
+	{
+		w->f.colliders[collider].scale[k] = scale3[k];
+		w->d.colliders[collider].scale[k] = scale3[k];
+	}
+	return 0;
+}
+
+// Triangle tree of mesh collider `collider` (type 8): the root reference, nn nodes of 24 bound
+// floats (min x, y, z, max x, y, z of the four children) and 4 child references each, nt
+// triangles of 9 floats.
+int xr_world_set_tree(void *wp, int collider, u32 root, const float *bounds, const u32 *children, u32 nn, const float *tris, u32 nt)
+{
+	XrWorld *w = (XrWorld *)wp;
+	if (collider < 0 || size_t(collider) >= w->f.colliders.size() || w->f.colliders[collider].type != 8)
+		return -1;
+	Collider *targets[2] = { &w->f.colliders[collider], &w->d.colliders[collider] };
+	for (Collider *c : targets)
+	{
+		c->tree_root = root;
+		c->tree_bounds.assign(bounds, bounds + 24 * size_t(nn));
+		c->tree_child.assign(children, children + 4 * size_t(nn));
+		c->tree_tris.assign(tris, tris + 9 * size_t(nt));
+	}
+	return 0;
 }
 
 // Hull data of convex hull collider `collider` (type 6): np points relative to the

@@ -8,8 +8,8 @@ import numpy as np
 
 from joltphysics_amd import scenes
 from joltphysics_amd.records import (BEND_DIHEDRAL, BEND_DISTANCE, LRA_GEODESIC, SHAPE_BOX, SHAPE_CAPSULE,
-                                     MOTION_DYNAMIC, SHAPE_CONVEX_HULL, SHAPE_CYLINDER, SHAPE_PLANE, SHAPE_SPHERE,
-                                     SHAPE_TAPERED_CAPSULE, BodyParams,
+                                     MOTION_DYNAMIC, SHAPE_CONVEX_HULL, SHAPE_CYLINDER, SHAPE_MESH, SHAPE_PLANE,
+                                     SHAPE_SPHERE, SHAPE_TAPERED_CAPSULE, SHAPE_TAPERED_CYLINDER, BodyParams,
                                      Collider)
 from joltphysics_amd.scenes import (CALC_EDGE_LENGTHS, CALC_VOLUMES, CREATE_CONSTRAINTS, OPTIMIZE, BodySpec,
                                     MeshSpec, Scene)
@@ -166,6 +166,76 @@ def cloth_hulls_streamed():
     return cloth_hulls(n=120, frames=8, spacing=0.02, height=0.46, iterations=4, collision_steps=1)
 
 
+def cloth_scaled_shapes(n=40, frames=30):
+    """n x n cloth (edge compliance 1e-4 and distance bends: a free sheet that stays calm in
+    the reference) draping over shapes inside ScaledShapes: a box scaled non-uniformly, a
+    sphere and a capsule scaled uniformly, a cylinder scaled in y, a tapered capsule
+    flipped by a negative y scale, a convex hull scaled non-uniformly and inside out (one
+    negative component), and two tapered cylinders (one unscaled, one flipped and widened)."""
+    v, f = scenes.cloth(n, 0.05, 60.0, jitter=1e-4, seed=14)
+    mesh = MeshSpec(vertices=v, faces=f, attrs=scenes.attrs(1e-4, 1e-4, 1e-3), bend_type=BEND_DISTANCE)
+    body = BodySpec(mesh=mesh, position=(0.0, 0.0, 0.65), params=BodyParams(num_iterations=5, allow_sleeping=False))
+    cloud, _ = _hull_points()
+    cols = [Collider(SHAPE_BOX, (0.2, 0.15, 0.1), pos=(-0.6, -0.55, 0.2), rot=_quat((0.2, 1, 0.1), 0.4), friction=0.4,
+                     scale=(1.4, 0.6, 1.8)),
+            Collider(SHAPE_SPHERE, (0.12,), pos=(-0.05, -0.6, 0.25), scale=(1.6, 1.6, 1.6), friction=0.3),
+            Collider(SHAPE_CAPSULE, (0.15, 0.08), pos=(0.55, -0.55, 0.25), rot=_quat((1, 0, 0.3), 1.2), scale=(1.3, 1.3, 1.3)),
+            Collider(SHAPE_CYLINDER, (0.12, 0.15), pos=(-0.55, 0.05, 0.2), rot=_quat((0.3, 0, 1), 0.3), scale=(1.2, 1.8, 1.2),
+                     friction=0.5),
+            Collider(SHAPE_TAPERED_CAPSULE, (0.2, 0.12, 0.06), pos=(0.05, 0.05, 0.28), rot=_quat((0.1, 0, 1), 1.4),
+                     scale=(1.1, -1.1, 1.1), restitution=0.2),
+            Collider(SHAPE_CONVEX_HULL, tuple(cloud), pos=(0.55, 0.05, 0.22), rot=_quat((0.3, 1, 0.2), 0.7),
+                     scale=(1.3, -0.8, 1.1)),
+            Collider(SHAPE_TAPERED_CYLINDER, (0.15, 0.05, 0.18), pos=(-0.5, 0.6, 0.2), rot=_quat((1, 0.2, 0), 0.3), friction=0.6),
+            Collider(SHAPE_TAPERED_CYLINDER, (0.12, 0.16, 0.04), pos=(0.3, 0.6, 0.22), rot=_quat((0.2, 1, 0), 0.5),
+                     scale=(1.3, -1.2, 1.3))]
+    return Scene("cloth_scaled_shapes", [body], _bounded(cols), gravity=(0.0, 0.0, -9.81), dt=1 / 60, collision_steps=2,
+                 frames=frames)
+
+
+def _terrain(cells, size, seed):
+    """A (cells+1)^2-vertex height field as an indexed triangle list in the x-y plane (z up),
+    heights random around 0: the grid is axis aligned, so many vertices of an aligned cloth
+    are equally near two triangles and the walk order decides."""
+    rng = np.random.default_rng(seed)
+    xs = np.linspace(-size / 2, size / 2, cells + 1)
+    X, Y = np.meshgrid(xs, xs, indexing="xy")
+    Z = rng.uniform(-0.04, 0.04, size=X.shape)
+    Z[::3, ::3] = 0.0
+    verts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1).astype(np.float32)
+    idx = np.arange((cells + 1) ** 2).reshape(cells + 1, cells + 1)
+    a, b = idx[:-1, :-1].ravel(), idx[:-1, 1:].ravel()
+    c, d = idx[1:, :-1].ravel(), idx[1:, 1:].ravel()
+    tris = np.concatenate([np.stack([a, b, d], 1), np.stack([a, d, c], 1)]).astype(np.uint32)
+    return verts, tris
+
+
+def cloth_on_mesh(n=40, frames=30, spacing=0.05, height=0.3, iterations=5, collision_steps=2, terrain_cells=24):
+    """n x n cloth (calm parameters as above) falling onto a bumpy terrain MeshShape (axis
+    aligned grid, flat patches),
+    beside a slanted TriangleShape and a scaled, rotated copy of a small mesh (a pyramid of
+    four triangles, inside out through a negative scale)."""
+    v, f = scenes.cloth(n, spacing, 60.0, jitter=1e-4, seed=15)
+    mesh = MeshSpec(vertices=v, faces=f, attrs=scenes.attrs(1e-4, 1e-4, 1e-3), bend_type=BEND_DISTANCE)
+    body = BodySpec(mesh=mesh, position=(0.0, 0.0, height),
+                    params=BodyParams(num_iterations=iterations, allow_sleeping=False))
+    terrain = _terrain(terrain_cells, 3.0, 16)
+    pyramid_v = np.array([[-0.2, -0.2, 0.0], [0.2, -0.2, 0.0], [0.2, 0.2, 0.0], [-0.2, 0.2, 0.0], [0.0, 0.0, 0.18]], np.float32)
+    pyramid_t = np.array([[0, 1, 4], [1, 2, 4], [2, 3, 4], [3, 0, 4]], np.uint32)
+    cols = [Collider(SHAPE_MESH, (), pos=(0.0, 0.0, 0.0), friction=0.5, mesh=(terrain[0], terrain[1], 0)),
+            Collider(SHAPE_MESH, (), pos=(0.45, 0.4, 0.12), rot=_quat((1, 0.3, 0), 0.5), friction=0.3,
+                     triangle=(-0.2, -0.15, 0.0, 0.25, -0.1, 0.02, 0.0, 0.22, 0.05)),
+            Collider(SHAPE_MESH, (), pos=(-0.4, -0.35, 0.05), rot=_quat((0, 0, 1), 0.4), restitution=0.1,
+                     scale=(1.2, 1.2, -1.5), mesh=(pyramid_v, pyramid_t, 2))]
+    return Scene("cloth_on_mesh", [body], _bounded(cols), gravity=(0.0, 0.0, -9.81), dt=1 / 60,
+                 collision_steps=collision_steps, frames=frames)
+
+
+def cloth_on_mesh_streamed():
+    """The mesh scene with a 120 x 120 cloth (14,400 vertices, past the LDS budget), a few frames."""
+    return cloth_on_mesh(n=120, frames=6, spacing=0.02, height=0.12, iterations=4, collision_steps=1)
+
+
 ALL = {
     "cloth100_sphere": big_cloth_on_sphere,
     "tetcube_plane_box": tet_cube_on_plane,
@@ -181,6 +251,9 @@ ALL = {
     "cloth_sensors_streamed": lambda: cloth_sensors(n=120, frames=6, spacing=0.01),   # 14,400 vertices
     "block_lra_boxes": block_lra_boxes,
     "block_lra_boxes_streamed": block_lra_boxes_streamed,
+    "cloth_scaled_shapes": cloth_scaled_shapes,
+    "cloth_on_mesh": cloth_on_mesh,
+    "cloth_on_mesh_streamed": cloth_on_mesh_streamed,
 }
 
 

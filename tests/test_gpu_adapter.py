@@ -33,6 +33,8 @@ SCENES = {
     "pressure_sphere": scene_zoo.pressure_sphere,
     "batched": scene_zoo.batched,
     "cloth100_sphere": scene_zoo.big_cloth_on_sphere,
+    "cloth_scaled_shapes": scene_zoo.cloth_scaled_shapes,
+    "cloth_on_mesh": scene_zoo.cloth_on_mesh,
 }
 
 

@@ -1,0 +1,283 @@
+"""Pins of the oracle's scaled shapes, tapered cylinder and triangle trees (the C++ restatement,
+oracle/xpbd_restatement.cpp), chosen so that each fails on a plausible mistake:
+
+* closed forms: a ScaledShape's box, sphere and capsule against the unscaled shape of the
+  scaled size; a cube's convex hull scaled (2, 1, 1), also mirrored (inside out), against the
+  box of the same size; the tapered cylinder's top, bottom, side and rim regions; a
+  triangle's interior, edge and vertex regions, its 0.1 m thickness behind the face, and the
+  normal sign of an inside-out scale;
+* brute force: the tree walk on random points finds the triangle a scan over all triangles
+  finds (double precision, a two-level tree built here with exact bounds);
+* against the reference (when built): a MeshShape flattened by the Jolt-side converter holds
+  every input triangle once, with its winding, to the codec's quantisation, inside its
+  node's bounds.
+"""
+from __future__ import annotations
+
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import ROOT_DIR, run_restatement
+from joltphysics_amd import scenes
+from joltphysics_amd.records import (GROUP, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_CONVEX_HULL, SHAPE_MESH, SHAPE_SPHERE,
+                                     SHAPE_TAPERED_CYLINDER, TREE_EMPTY_REF, TREE_LEAF_REF, TREE_NODE, VERTEX, BodyParams,
+                                     Collider, ConvexHull, SettingsArrays, TriangleTree)
+
+
+@pytest.fixture(scope="session", autouse=True)
+def restatement_built():
+    from oracle import have_restatement
+    if not have_restatement():
+        subprocess.run(["make", "-C", os.path.join(ROOT_DIR, "oracle"), "restatement"], check=True)
+
+
+def _probe_plane(collider, point):
+    """Collision plane stored for one free vertex at `point` after one collision step without
+    gravity (FP64 restatement; the vertex radius keeps every collider in the colliding list)."""
+    v = np.zeros(1, VERTEX)
+    v["pos"][0] = point
+    v["inv_mass"] = 1.0
+    g = np.zeros(1, GROUP)
+    s = SettingsArrays(v, groups=g)
+    params = BodyParams(num_iterations=1, linear_damping=0.0, update_position=False, allow_sleeping=False,
+                        vertex_radius=5.0)
+    if collider.bounds is None:
+        collider.bounds = (-100.0, -100.0, -100.0, 100.0, 100.0, 100.0)
+    body = scenes.BodySpec(mesh=None, params=params)
+    sc = scenes.Scene("probe", [body], [collider], gravity=(0.0, 0.0, 0.0), dt=1 / 60, collision_steps=1, frames=1)
+    out = run_restatement(sc, [s], np.array([collider.bounds], np.float32), (1,), precision=64)[1][0]
+    return out["plane"][0]
+
+
+def _at_origin(c):
+    c.com = (0.0, 0.0, 0.0)
+    return c
+
+
+# ------------------------------------------------------------------ scaled shapes
+@pytest.mark.parametrize("point", [(1.4, 0.1, 0.0), (0.2, 0.7, 0.1), (0.3, 0.1, -1.5), (0.9, 0.3, 0.2)])
+def test_scaled_box_is_the_box_of_the_scaled_size(point):
+    scaled = _probe_plane(_at_origin(Collider(SHAPE_BOX, (0.5, 0.25, 0.5), scale=(2.0, -2.0, 1.5))), point)
+    plain = _probe_plane(_at_origin(Collider(SHAPE_BOX, (1.0, 0.5, 0.75))), point)
+    np.testing.assert_allclose(scaled, plain, atol=1e-12)
+
+
+def test_scaled_sphere_and_capsule():
+    p = _probe_plane(_at_origin(Collider(SHAPE_SPHERE, (0.5,), scale=(-2.0, -2.0, -2.0))), (3.0, 0.0, 0.0))
+    np.testing.assert_allclose(p, [1, 0, 0, -1.0], atol=1e-12)           # radius |-2| * 0.5
+    p = _probe_plane(_at_origin(Collider(SHAPE_CAPSULE, (0.5, 0.2), scale=(3.0, 3.0, 3.0))), (2.0, 0.3, 0.0))
+    np.testing.assert_allclose(p, [1, 0, 0, -0.6], atol=1e-12)           # cylinder part, radius 0.6
+
+
+_CUBE_POINTS = np.array([[x, y, z] for x in (-0.5, 0.5) for y in (-0.5, 0.5) for z in (-0.5, 0.5)])
+_CUBE_FACES = [[0, 1, 3, 2], [4, 6, 7, 5], [0, 4, 5, 1], [2, 3, 7, 6], [0, 2, 6, 4], [1, 5, 7, 3]]
+
+
+@pytest.mark.parametrize("scale", [(2.0, 1.0, 1.0), (-2.0, 1.0, 1.0), (2.0, -1.0, -1.0)])
+@pytest.mark.parametrize("point", [(1.6, 0.1, 0.2), (0.3, 0.2, 1.5), (0.8, 0.1, 0.05), (1.7, 1.2, 0.1), (-1.6, -0.9, 0.9)])
+def test_scaled_cube_hull_is_the_scaled_box(scale, point):
+    # a mirrored cube is the same cube: the inside-out edge test (scale_flip) must keep it so
+    h = ConvexHull.from_polytope(_CUBE_POINTS, _CUBE_FACES)
+    hull = _at_origin(Collider(SHAPE_CONVEX_HULL, tuple(_CUBE_POINTS.ravel()), hull=h, scale=scale))
+    box = _at_origin(Collider(SHAPE_BOX, (1.0, 0.5, 0.5)))
+    np.testing.assert_allclose(_probe_plane(hull, point), _probe_plane(box, point), atol=1e-9)
+
+
+# ------------------------------------------------------------------ tapered cylinder
+def _tapered_cylinder(half_height=0.5, top_radius=0.2, bottom_radius=0.4, scale=(1.0, 1.0, 1.0)):
+    # top and bottom relative to the centre of mass (TaperedCylinderShape.cpp:108-130)
+    h = 2.0 * half_height
+    tr, br = top_radius, bottom_radius
+    com = h * (3 * tr * tr + 2 * br * tr + br * br) / (4.0 * (tr * tr + br * tr + br * br))
+    c = Collider(SHAPE_TAPERED_CYLINDER, (half_height, top_radius, bottom_radius), scale=scale)
+    top, bottom = float(np.float32(h - com)), float(np.float32(-com))      # as the device takes them
+    c.device_params = (top, bottom, tr, br)
+    return _at_origin(c), top, bottom
+
+
+def test_tapered_cylinder_caps_side_and_rim():
+    c, top, bottom = _tapered_cylinder()
+    np.testing.assert_allclose(_probe_plane(c, (0.05, top + 1.0, 0.0)), [0, 1, 0, -top], atol=1e-12)
+    np.testing.assert_allclose(_probe_plane(c, (0.05, bottom - 1.0, 0.0)), [0, -1, 0, bottom], atol=1e-12)
+    # beside the side at mid height: outward normal (x, tan a, 0) normalised, tan a = (rb - rt) / height,
+    # through the side's top support point
+    tan_a = (float(np.float32(0.4)) - float(np.float32(0.2))) / (top - bottom)
+    n = np.array([1.0, tan_a, 0.0]) / math.hypot(1.0, tan_a)
+    p = _probe_plane(c, (1.0, 0.5 * (top + bottom), 0.0))
+    np.testing.assert_allclose(p[:3], n, atol=1e-12)
+    assert p[3] == pytest.approx(-np.dot(n, [float(np.float32(0.2)), top, 0.0]), abs=1e-12)
+    # beyond the top rim (outside the top radius, above the side's extension): normal from the
+    # rim point (0.2, top, 0) to the vertex
+    rim = np.array([float(np.float32(0.2)), top, 0.0])
+    q = (rim + np.array([0.3, 0.5, 0.0])).astype(np.float32).astype(np.float64)     # vertices are float
+    p = _probe_plane(c, tuple(q))
+    m = (q - rim) / np.linalg.norm(q - rim)
+    np.testing.assert_allclose(p[:3], m, atol=1e-12)
+    assert p[3] == pytest.approx(-np.dot(m, rim), abs=1e-12)
+
+
+def test_tapered_cylinder_negative_y_scale_flips_top_and_bottom():
+    # scale y -1 turns the shape upside down: GetScaled swaps top / bottom and the radii
+    c, top, bottom = _tapered_cylinder(scale=(1.0, -1.0, 1.0))
+    np.testing.assert_allclose(_probe_plane(c, (0.05, -bottom + 1.0, 0.0)), [0, 1, 0, bottom], atol=1e-12)
+    p = _probe_plane(c, (1.0, 0.0, 0.0))
+    assert p[1] < 0.0                       # the wide end is up now: the side normal points down
+
+
+# ------------------------------------------------------------------ triangles
+def _tree(triangles, leaf_size=3):
+    """A two-level tree over `triangles` (T, 9): one root with up to four leaves, bounds the
+    leaves' exact float extents, unused children empty."""
+    tris = np.asarray(triangles, np.float32).reshape(-1, 9)
+    groups = [list(range(k, min(k + leaf_size, len(tris)))) for k in range(0, len(tris), leaf_size)]
+    assert len(groups) <= 4
+    node = np.zeros(1, TREE_NODE)
+    for k in range(4):
+        if k < len(groups):
+            pts = tris[groups[k]].reshape(-1, 3)
+            node["min"][0][:, k] = pts.min(axis=0)
+            node["max"][0][:, k] = pts.max(axis=0)
+            node["child"][0][k] = TREE_LEAF_REF | (len(groups[k]) << 24) | groups[k][0]
+        else:
+            node["min"][0][:, k] = 65504.0
+            node["max"][0][:, k] = 65504.0
+            node["child"][0][k] = TREE_EMPTY_REF
+    return TriangleTree(0, node, tris)
+
+
+def _mesh(triangles, scale=(1.0, 1.0, 1.0)):
+    return _at_origin(Collider(SHAPE_MESH, (), tree=_tree(triangles), scale=scale))
+
+
+_TRI = [0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0]         # normal +z
+
+
+def _f32(p):
+    """A point as a vertex holds it (float)."""
+    return np.asarray(p, np.float32).astype(np.float64)
+
+
+def test_triangle_interior_edge_vertex_and_thickness():
+    # above the interior: the triangle's plane
+    np.testing.assert_allclose(_probe_plane(_mesh(_TRI), (0.2, 0.3, 0.5)), [0, 0, 1, 0], atol=1e-12)
+    # 0.05 behind the face (within the 0.1 thickness): still the triangle's plane
+    np.testing.assert_allclose(_probe_plane(_mesh(_TRI), (0.2, 0.3, -0.05)), [0, 0, 1, 0], atol=1e-12)
+    # beside the hypotenuse, above: plane through the closest edge point, normal towards the vertex
+    q = _f32((0.8, 0.8, 0.2))
+    e = np.array([0.5, 0.5, 0.0])                       # q's projection onto x + y = 1 (q.x == q.y)
+    n = (q - e) / np.linalg.norm(q - e)
+    p = _probe_plane(_mesh(_TRI), tuple(q))
+    np.testing.assert_allclose(p[:3], n, atol=1e-12)
+    assert p[3] == pytest.approx(-np.dot(n, e), abs=1e-12)
+    # beyond the vertex (1, 0, 0)
+    q = _f32((1.5, -0.4, 0.3))
+    n = (q - [1.0, 0.0, 0.0]) / np.linalg.norm(q - [1.0, 0.0, 0.0])
+    np.testing.assert_allclose(_probe_plane(_mesh(_TRI), tuple(q))[:3], n, atol=1e-12)
+
+
+def test_triangle_behind_the_thickness_and_back_facing_edges_are_ignored():
+    # 0.2 behind the interior (past the thickness) and below a back-facing edge region: no
+    # plane is stored for the vertex (the probe returns the initial zero plane)
+    assert np.all(_probe_plane(_mesh(_TRI), (0.2, 0.3, -0.2)) == 0.0)
+    assert np.all(_probe_plane(_mesh(_TRI), (0.8, 0.8, -0.2)) == 0.0)
+
+
+def test_inside_out_scale_flips_the_triangle_normal():
+    # x scale -1 mirrors the triangle to x < 0 and reverses its winding; the normal sign of the
+    # inside-out scale keeps its front facing +z, the mirror image of the unscaled front
+    np.testing.assert_allclose(_probe_plane(_mesh(_TRI, scale=(-1.0, 1.0, 1.0)), (-0.2, 0.3, 0.5)), [0, 0, 1, 0], atol=1e-12)
+    # z scale -1 mirrors it across its own plane: same vertices and winding, so the sign turns
+    # the front to -z (the mirror image of +z); a point on the +z side is now behind it
+    np.testing.assert_allclose(_probe_plane(_mesh(_TRI, scale=(1.0, 1.0, -1.0)), (0.2, 0.3, -0.5)), [0, 0, -1, 0], atol=1e-12)
+    assert np.all(_probe_plane(_mesh(_TRI, scale=(1.0, 1.0, -1.0)), (0.2, 0.3, 0.5)) == 0.0)
+
+
+def _closest_on_triangle(p, a, b, c):
+    """Textbook closest point of triangle abc to p (Ericson, Real-Time Collision Detection 5.1.5)."""
+    ab, ac, ap = b - a, c - a, p - a
+    d1, d2 = ab @ ap, ac @ ap
+    if d1 <= 0 and d2 <= 0:
+        return a
+    bp = p - b
+    d3, d4 = ab @ bp, ac @ bp
+    if d3 >= 0 and d4 <= d3:
+        return b
+    vc = d1 * d4 - d3 * d2
+    if vc <= 0 and d1 >= 0 and d3 <= 0:
+        return a + d1 / (d1 - d3) * ab
+    cp = p - c
+    d5, d6 = ab @ cp, ac @ cp
+    if d6 >= 0 and d5 <= d6:
+        return c
+    vb = d5 * d2 - d1 * d6
+    if vb <= 0 and d2 >= 0 and d6 <= 0:
+        return a + d2 / (d2 - d6) * ac
+    va = d3 * d6 - d5 * d4
+    if va <= 0 and (d4 - d3) >= 0 and (d5 - d6) >= 0:
+        return b + (d4 - d3) / ((d4 - d3) + (d5 - d6)) * (c - b)
+    denom = 1.0 / (va + vb + vc)
+    return a + ab * (vb * denom) + ac * (vc * denom)
+
+
+def test_tree_walk_finds_the_nearest_triangle():
+    rng = np.random.default_rng(31)
+    tris = rng.uniform(-1.0, 1.0, size=(12, 9)).astype(np.float32)
+    c = _mesh(tris)
+    for _ in range(40):
+        q = _f32(rng.uniform(-1.5, 1.5, size=3))
+        p = _probe_plane(c, tuple(q))
+        # brute force: the nearest triangle and point; the stored plane passes through the
+        # nearest point (edge / vertex regions) or contains the triangle (interior)
+        best = min(range(len(tris)), key=lambda t: np.linalg.norm(q - _closest_on_triangle(q, *tris[t].astype(np.float64).reshape(3, 3))))
+        a, b, cc = tris[best].astype(np.float64).reshape(3, 3)
+        x = _closest_on_triangle(q, a, b, cc)
+        if np.all(p == 0.0):
+            continue            # back facing or beyond the thickness: nothing stored
+        assert abs(p[:3] @ x + p[3]) < 1e-9, (q, best)
+
+
+# ------------------------------------------------------------------ the reference's trees
+def _have_reference():
+    from oracle import have_reference
+    return have_reference("nofma")
+
+
+@pytest.mark.skipif(not _have_reference(), reason="reference library not built (needs the reference sources)")
+@pytest.mark.parametrize("cells,per_leaf", [(6, 0), (24, 0), (24, 2), (40, 8)])
+def test_flattened_mesh_holds_every_triangle_once(cells, per_leaf):
+    import scene_zoo
+    from oracle import JoltReference, JoltWorld
+    verts, tris = scene_zoo._terrain(cells, 3.0, 7)
+    world = JoltWorld(JoltReference("nofma"), (0.0, 0.0, -9.81))
+    try:
+        i = world.add_collider(Collider(SHAPE_MESH, (), mesh=(verts, tris, per_leaf)))
+        tree = world.collider_tree(i)
+    finally:
+        world.close()
+    # the triangle codec stores vertices quantised within the mesh bounds (about 1e-6 of the
+    # extent here): match each input triangle, in one of its cyclic orders, to a decoded one
+    got = np.asarray(tree.triangles, np.float64)
+    assert len(got) == len(tris)
+    used = np.zeros(len(got), bool)
+    for t in tris:
+        w = verts[t].astype(np.float64)
+        cands = [np.roll(w, -k, axis=0).ravel() for k in range(3)]
+        err = np.min([np.abs(got - c).max(axis=1) for c in cands], axis=0)
+        err[used] = np.inf
+        j = int(np.argmin(err))
+        assert err[j] < 1e-5, "an input triangle is missing or its winding changed"
+        used[j] = True
+    # every node's child bounds contain what hangs below them, to the quantisation (the tree's
+    # bounds come from the input vertices, its triangles from the quantised ones)
+    nodes = tree.nodes
+    for k in range(len(nodes)):
+        for ch in range(4):
+            ref = int(nodes["child"][k][ch])
+            if ref & 0xc0000000 == TREE_LEAF_REF:
+                first, count = ref & 0xffffff, (ref >> 24) & 0x3f
+                pts = tree.triangles[first:first + count].reshape(-1, 3)
+                assert np.all(pts >= nodes["min"][k][:, ch] - 1e-5) and np.all(pts <= nodes["max"][k][:, ch] + 1e-5)
