@@ -58,7 +58,8 @@ extern "C" {
  * (TaperedCylinderShape.cpp:108-130: top and bottom relative to the shape's centre of mass;
  * SaveBinaryState writes them); TaperedCylinderShape.cpp:390-506 */
 #define XPBD_SHAPE_TAPERED_CYLINDER 7
-/* TriangleShape (TriangleShape.cpp:272-283) and MeshShape (MeshShape.cpp:833-885): no params;
+/* TriangleShape (TriangleShape.cpp:272-283), MeshShape (MeshShape.cpp:833-885) and
+ * HeightFieldShape (HeightFieldShape.cpp:2242-2293): no params;
  * xpbd_collider_desc.tree holds the triangles as the shape's tree walk visits them
  * (CollideSoftBodyVerticesVsTriangles.h).  A triangle is a tree whose root is a leaf holding
  * its one triangle. */
@@ -90,7 +91,13 @@ typedef struct xpbd_triangle_tree {
     const xpbd_tree_node *nodes;
     uint32_t num_triangles;
     const float *triangles;         /* num_triangles x 9: v0, v1, v2 */
+    uint32_t flags;                 /* XPBD_TREE_HEIGHT_FIELD or 0 */
 } xpbd_triangle_tree;
+
+/* xpbd_triangle_tree.flags: a HeightFieldShape's range-block hierarchy (HeightFieldShape.cpp:
+ * 1745-2093, 2242-2293): a child whose unscaled bounds have min y > max y (a block without
+ * collision) gets distance FLT_MAX and is never visited. */
+#define XPBD_TREE_HEIGHT_FIELD 1u
 
 /* Largest depth of a triangle tree (levels of nodes below the root): the per-vertex walk keeps
  * its stack in 3 x depth + 4 entries. */

@@ -22,10 +22,10 @@
 // them.
 //
 // Supported colliders: plane, sphere, box, capsule, cylinder, tapered capsule, tapered
-// cylinder, convex hull, triangle and mesh bodies, each possibly inside ScaledShapes
-// (jolt_xpbd_shapes.h); Update() fails (false, LastError()) when a soft body's bounds reach
-// any other shape (height field, compound, rotated / translated), and the caller keeps the
-// CPU path for it.
+// cylinder, convex hull, triangle, mesh and height field bodies, each possibly inside
+// ScaledShapes (jolt_xpbd_shapes.h); Update() fails (false, LastError()) when a soft body's
+// bounds reach any other shape (compound, rotated / translated, offset centre of mass), and
+// the caller keeps the CPU path for it.
 #pragma once
 
 #include <Jolt/Jolt.h>

@@ -19,6 +19,7 @@
 #include <Jolt/Physics/Collision/Shape/CapsuleShape.h>
 #include <Jolt/Physics/Collision/Shape/ConvexHullShape.h>
 #include <Jolt/Physics/Collision/Shape/CylinderShape.h>
+#include <Jolt/Physics/Collision/Shape/HeightFieldShape.h>
 #include <Jolt/Physics/Collision/Shape/MeshShape.h>
 #include <Jolt/Physics/Collision/Shape/PlaneShape.h>
 #include <Jolt/Physics/Collision/Shape/ScaledShape.h>
@@ -194,6 +195,179 @@ inline bool FlattenMesh(const JPH::MeshShape &mesh, ShapeData &out, std::string 
     out.tree.nodes = out.nodes.data();
     out.tree.num_triangles = uint32_t(out.triangles.size() / 9);
     out.tree.triangles = out.triangles.data();
+    out.tree.flags = 0;
+    return true;
+}
+
+// A HeightFieldShape's range-block hierarchy as HeightFieldShape::DecodingContext::
+// WalkHeightField visits it (HeightFieldShape.cpp:1757-2093): a node per range block above the
+// finest level, its four children's bounds from the quantised range data (internal cells in
+// x and z, min / max height in y); at the finest level a node per block whose four children
+// are the block's quarters, bounds from their decoded samples (inside out when none has
+// collision), each a leaf of its triangles in the walk's order (rows, columns, two triangles
+// per cell, cells and triangles touching a sample without collision left out).  Positions
+// come from HeightFieldShape::GetPosition / IsNoCollision, which decode a sample exactly as
+// the walk does; the range data from SaveBinaryState (HeightFieldShape.cpp:2660-2682).  Only
+// block counts that are powers of two (the shape's efficient case) are taken.
+inline bool FlattenHeightField(const JPH::HeightFieldShape &hf, ShapeData &out, std::string &err)
+{
+    using namespace JPH;
+    const std::string s = shapes_detail::SavedState(hf);
+    size_t at = shapes_detail::kShapeStateBytes;
+    auto get = [&](void *dst, size_t n) -> bool {
+        if (at + n > s.size())
+            return false;
+        memcpy(dst, s.data() + at, n);
+        at += n;
+        return true;
+    };
+    float offset[3], scale[3];
+    uint32_t sample_count, block_size, mat_count, mat_bits;
+    uint8_t bits_per_sample, has_heights;
+    uint16_t min_sample, max_sample;
+    if (!get(offset, 12) || !get(scale, 12) || !get(&sample_count, 4) || !get(&block_size, 4) || !get(&bits_per_sample, 1)
+        || !get(&min_sample, 2) || !get(&max_sample, 2) || !get(&mat_count, 4))
+    {
+        err = "height field state too short";
+        return false;
+    }
+    at += mat_count;
+    if (!get(&mat_bits, 4) || !get(&has_heights, 1))
+    {
+        err = "height field state too short";
+        return false;
+    }
+    out.nodes.clear();
+    out.triangles.clear();
+    out.tree.flags = XPBD_TREE_HEIGHT_FIELD;
+    out.tree.root = XPBD_TREE_EMPTY;
+    out.tree.num_nodes = 0;
+    out.tree.nodes = nullptr;
+    out.tree.num_triangles = 0;
+    out.tree.triangles = nullptr;
+    if (!has_heights)
+        return true;                // no samples: nothing collides (HeightFieldShape.cpp:1761-1762)
+    if (block_size == 0 || sample_count % block_size != 0)
+    {
+        err = "height field block size";
+        return false;
+    }
+    const uint32_t num_blocks = sample_count / block_size;
+    if (num_blocks < 2 || (num_blocks & (num_blocks - 1)) != 0)
+    {
+        err = "height field with a block count that is not a power of two";
+        return false;
+    }
+    const uint32_t max_level = 32 - CountLeadingZeros(num_blocks - 1);
+    static const uint32_t grid_offsets[] = { 0, 1, 5, 21, 85, 341, 1365, 5461, 21845, 87381, 349525, 1398101, 5592405, 22369621 };
+    if (max_level >= sizeof(grid_offsets) / sizeof(grid_offsets[0]))
+    {
+        err = "height field too large";
+        return false;
+    }
+    const char *range_blocks = s.data() + at;       // uint16 min[4], max[4] per range block
+    const size_t range_bytes = s.size() - at;
+    const uint32_t max_stride = (num_blocks + 1) >> 1;
+    const Vec4 ox = Vec4::sReplicate(offset[0]), oy = Vec4::sReplicate(offset[1]), oz = Vec4::sReplicate(offset[2]);
+    const Vec4 sx = Vec4::sReplicate(scale[0]), sy = Vec4::sReplicate(scale[1]), sz = Vec4::sReplicate(scale[2]);
+    const UVec4 sample_count_min_1 = UVec4::sReplicate(sample_count - 1);
+    const float large = 1.0e30f;                    // cLargeFloat
+
+    struct Item { uint32_t level, x, y, node; };
+    std::vector<Item> todo;
+    auto new_node = [&]() { out.nodes.emplace_back(); return uint32_t(out.nodes.size() - 1); };
+    out.tree.root = XPBD_TREE_NODE | new_node();
+    todo.push_back({ 0, 0, 0, out.tree.root });
+    while (!todo.empty())
+    {
+        const Item it = todo.back();
+        todo.pop_back();
+        xpbd_tree_node n;
+        if (it.level < max_level)
+        {
+            // an internal range block: HeightFieldShape.cpp:2038-2062
+            const uint32_t stride = std::min(1u << it.level, max_stride);
+            const size_t offset16 = size_t(grid_offsets[it.level] + stride * it.y + it.x) * 16;
+            if (offset16 + 16 > range_bytes)
+            {
+                err = "height field range data truncated";
+                return false;
+            }
+            uint16_t mm[8];
+            memcpy(mm, range_blocks + offset16, 16);
+            const Vec4 bounds_miny = oy + sy * Vec4(float(mm[0]), float(mm[1]), float(mm[2]), float(mm[3]));
+            const Vec4 bounds_maxy = oy + sy * Vec4(float(mm[4]), float(mm[5]), float(mm[6]), float(mm[7]));
+            const UVec4 cell = UVec4::sReplicate(block_size << (max_level - it.level - 1));
+            const UVec4 two_x = UVec4::sReplicate(2 * it.x), two_y = UVec4::sReplicate(2 * it.y);
+            const Vec4 bounds_minx = ox + sx * (cell * (two_x + UVec4(0, 1, 0, 1))).ToFloat();
+            const Vec4 bounds_maxx = ox + sx * UVec4::sMin(cell * (two_x + UVec4(1, 2, 1, 2)), sample_count_min_1).ToFloat();
+            const Vec4 bounds_minz = oz + sz * (cell * (two_y + UVec4(0, 0, 1, 1))).ToFloat();
+            const Vec4 bounds_maxz = oz + sz * UVec4::sMin(cell * (two_y + UVec4(1, 1, 2, 2)), sample_count_min_1).ToFloat();
+            for (int k = 0; k < 4; ++k)
+            {
+                n.min_x[k] = bounds_minx[k]; n.min_y[k] = bounds_miny[k]; n.min_z[k] = bounds_minz[k];
+                n.max_x[k] = bounds_maxx[k]; n.max_y[k] = bounds_maxy[k]; n.max_z[k] = bounds_maxz[k];
+                const uint32_t child = new_node();
+                n.child[k] = XPBD_TREE_NODE | child;
+                todo.push_back({ it.level + 1, 2 * it.x + (k & 1), 2 * it.y + (k >> 1), child });
+            }
+        }
+        else
+        {
+            // a block at the finest level: HeightFieldShape.cpp:1807-2036
+            const uint32_t min_x = it.x * block_size, min_y = it.y * block_size;
+            uint32_t max_x = min_x + block_size, max_y = min_y + block_size;
+            if (it.x >= num_blocks - 1)
+                --max_x;                    // no next block: one less triangle
+            if (it.y >= num_blocks - 1)
+                --max_y;
+            const uint32_t half = block_size >> 1;
+            const uint32_t bsx = max_x - min_x - half, bsy = max_y - min_y - half;
+            const uint32_t range[4][4] = { { 0, 0, half, half }, { half, 0, bsx, half }, { 0, half, half, bsy }, { half, half, bsx, bsy } };
+            for (int k = 0; k < 4; ++k)
+            {
+                Vec3 vmin = Vec3::sReplicate(large), vmax = Vec3::sReplicate(-large);
+                for (uint32_t vy = min_y + range[k][1]; vy <= min_y + range[k][1] + range[k][3]; ++vy)
+                    for (uint32_t vx = min_x + range[k][0]; vx <= min_x + range[k][0] + range[k][2]; ++vx)
+                        if (!hf.IsNoCollision(vx, vy))
+                        {
+                            const Vec3 p = hf.GetPosition(vx, vy);
+                            vmin = Vec3::sMin(vmin, p);
+                            vmax = Vec3::sMax(vmax, p);
+                        }
+                n.min_x[k] = vmin.GetX(); n.min_y[k] = vmin.GetY(); n.min_z[k] = vmin.GetZ();
+                n.max_x[k] = vmax.GetX(); n.max_y[k] = vmax.GetY(); n.max_z[k] = vmax.GetZ();
+                // the quarter's triangles (HeightFieldShape.cpp:1959-2023)
+                const uint32_t first = uint32_t(out.triangles.size() / 9);
+                auto put = [&](uint32_t ax, uint32_t ay, uint32_t bx, uint32_t by, uint32_t cx, uint32_t cy) {
+                    for (const Vec3 &p : { hf.GetPosition(ax, ay), hf.GetPosition(bx, by), hf.GetPosition(cx, cy) })
+                        out.triangles.insert(out.triangles.end(), { p.GetX(), p.GetY(), p.GetZ() });
+                };
+                for (uint32_t vy = min_y + range[k][1]; vy < min_y + range[k][1] + range[k][3]; ++vy)
+                    for (uint32_t vx = min_x + range[k][0]; vx < min_x + range[k][0] + range[k][2]; ++vx)
+                    {
+                        if (hf.IsNoCollision(vx, vy) || hf.IsNoCollision(vx + 1, vy + 1))
+                            continue;
+                        if (!hf.IsNoCollision(vx, vy + 1))
+                            put(vx, vy, vx, vy + 1, vx + 1, vy + 1);
+                        if (!hf.IsNoCollision(vx + 1, vy))
+                            put(vx, vy, vx + 1, vy + 1, vx + 1, vy);
+                    }
+                const uint32_t count = uint32_t(out.triangles.size() / 9) - first;
+                if (count > 63)
+                {
+                    err = "height field block too large";
+                    return false;
+                }
+                n.child[k] = count == 0 ? XPBD_TREE_EMPTY : (XPBD_TREE_LEAF | (count << 24) | first);
+            }
+        }
+        out.nodes[it.node] = n;
+    }
+    out.tree.num_nodes = uint32_t(out.nodes.size());
+    out.tree.nodes = out.nodes.data();
+    out.tree.num_triangles = uint32_t(out.triangles.size() / 9);
+    out.tree.triangles = out.triangles.data();
     return true;
 }
 
@@ -210,6 +384,7 @@ inline void TriangleTree(const JPH::TriangleShape &shape, ShapeData &out)
     out.tree.nodes = nullptr;
     out.tree.num_triangles = 1;
     out.tree.triangles = out.triangles.data();
+    out.tree.flags = 0;
 }
 
 // Shape type, parameters, scale and tables of `shape` into `d` (position, rotation, material
@@ -321,9 +496,15 @@ inline bool DescribeShape(const JPH::Shape *shape, xpbd_collider_desc &d, ShapeD
         d.type = XPBD_SHAPE_MESH;
         d.tree = &data.tree;
         return true;
+    case JPH::EShapeSubType::HeightField:
+        if (!FlattenHeightField(*static_cast<const JPH::HeightFieldShape *>(shape), data, err))
+            return false;
+        d.type = XPBD_SHAPE_MESH;
+        d.tree = &data.tree;
+        return true;
     default:
-        err = "a soft body reaches a collider shape the device path does not support (height field, compound or "
-              "rotated / translated shapes)";
+        err = "a soft body reaches a collider shape the device path does not support (compound, rotated / "
+              "translated or offset centre of mass shapes)";
         return false;
     }
 }

@@ -872,6 +872,7 @@ constexpr int kTreeStack = 3 * XPBD_TREE_MAX_DEPTH + 4;
 // FinishVertex: an interior closest point gives the triangle's plane (penetration below the
 // 0.1 m triangle thickness), an edge or vertex the plane through the closest point facing
 // the vertex, back-facing edges ignored.  L.pn: scale, L.pc: normal sign (-1 inside out).
+// A height field's hierarchy walks the same way (HeightFieldShape.cpp:1757-2093, 2242-2293).
 __device__ __noinline__ void collide_tree(const Leaf &L, uint32_t l, V3 x, float &largest, int32_t &idx, Pl &plane)
 {
     const TreeDev &T = *L.tree;
@@ -904,6 +905,9 @@ __device__ __noinline__ void collide_tree(const Leaf &L, uint32_t l, V3 x, float
                 const float cz = smin(smax(local.z, smin(sz0, sz1)), smax(sz0, sz1));
                 const float dx = cx - local.x, dy = cy - local.y, dz = cz - local.z;
                 d[k] = (dx * dx + dy * dy) + dz * dz;
+                // a height field's block without collision (HeightFieldShape.cpp:2268-2269)
+                if (T.height_field && !(N.min_y[k] <= N.max_y[k]))
+                    d[k] = FLT_MAX;
                 id[k] = N.child[k];
             }
             sort4_reverse(d, id);

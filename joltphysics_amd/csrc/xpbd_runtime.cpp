@@ -1950,7 +1950,7 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
     // triangle trees: validated here, nodes and triangles flattened into two pools
     std::vector<TreeNodeDev> tree_nodes;
     std::vector<float> tree_tris;
-    struct TreeOffsets { uint32_t nodes, tris, root, collider; };
+    struct TreeOffsets { uint32_t nodes, tris, root, collider, flags; };
     std::vector<TreeOffsets> tree_offsets;
     for (uint32_t i = 0; i < n; ++i)
     {
@@ -1985,6 +1985,9 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
             tree_tris.insert(tree_tris.end(), t->triangles, t->triangles + 9 * (size_t)t->num_triangles);
             o.root = RebaseTreeRef(t->root, o.nodes, o.tris);
             o.collider = i;
+            if (t->flags & ~XPBD_TREE_HEIGHT_FIELD)
+                return Fail(XPBD_E_INVALID, "mesh collider tree: unknown flags");
+            o.flags = t->flags;
             tree_offsets.push_back(o);
         }
         if (c[i].type == XPBD_SHAPE_CONVEX_HULL)
@@ -2072,6 +2075,7 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
         trees[k].nodes = ctx->d_tree_nodes.p;
         trees[k].triangles = ctx->d_tree_tris.p;
         trees[k].root = tree_offsets[k].root;
+        trees[k].height_field = (tree_offsets[k].flags & XPBD_TREE_HEIGHT_FIELD) ? 1u : 0u;
     }
     HIP_TRY(ctx->d_trees.upload(trees));
     for (size_t k = 0; k < tree_offsets.size(); ++k)

@@ -160,6 +160,7 @@ struct TreeDev
     const TreeNodeDev *nodes;
     const float *triangles;
     uint32_t root;
+    uint32_t height_field;      // children with unscaled min y > max y are never visited
 };
 
 struct ColliderDev

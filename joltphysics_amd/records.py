@@ -61,6 +61,7 @@ SHAPE_MESH = 8                  # triangle tree (MeshShape or TriangleShape)
 # xpbd_tree_node: the four children's bounds per axis and their references
 TREE_NODE = np.dtype([("min", "<f4", (3, 4)), ("max", "<f4", (3, 4)), ("child", "<u4", 4)])
 TREE_NODE_REF, TREE_LEAF_REF, TREE_EMPTY_REF = 0x00000000, 0x40000000, 0x80000000
+TREE_HEIGHT_FIELD = 1
 # EMotionType (MotionType.h:10-15)
 MOTION_STATIC, MOTION_KINEMATIC, MOTION_DYNAMIC = 0, 1, 2
 
@@ -183,6 +184,7 @@ class TriangleTree:
     root: int
     nodes: np.ndarray
     triangles: np.ndarray
+    flags: int = 0              # TREE_HEIGHT_FIELD: a height field's hierarchy
 
 
 @dataclass
@@ -222,10 +224,12 @@ class Collider:
     user_id: int | None = None  # caller's identifier (BodyID); None: the object's identity
     scale: tuple = (1.0, 1.0, 1.0)  # ScaledShape around the shape
     # SHAPE_MESH construction: a MeshShape from (vertices (N, 3), indices (M, 3), max triangles
-    # per leaf or 0) or a TriangleShape from `triangle` (9 floats); `tree` is what the device
-    # consumes (taken from the reference's shape)
+    # per leaf or 0), a TriangleShape from `triangle` (9 floats) or a HeightFieldShape from
+    # `height_field` (samples (n, n), block size, bits per sample, offset (3), scale (3)); `tree`
+    # is what the device consumes (taken from the reference's shape)
     mesh: tuple | None = None
     triangle: tuple | None = None
+    height_field: tuple | None = None
     tree: TriangleTree | None = None
 
     def param_array(self):

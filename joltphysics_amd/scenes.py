@@ -200,6 +200,15 @@ def collider_world_bounds(c: Collider):
     p = np.array(c.pos, np.float64)
     R = _rot_matrix(c.rot)
     scale = np.abs(np.array(c.scale, np.float64))
+    if c.type == SHAPE_MESH and c.height_field is not None:
+        samples, _, _, off, sc = c.height_field
+        n = np.asarray(samples).shape[0]
+        h = np.asarray(samples, np.float64)
+        h = h[h < 1e30]
+        pts = np.array([[0.0, h.min(), 0.0], [n - 1.0, h.max(), n - 1.0]]) * np.array(sc, np.float64) + np.array(off, np.float64)
+        corners = np.array([[x, y, z] for x in pts[:, 0] for y in pts[:, 1] for z in pts[:, 2]])
+        w = (corners * np.array(c.scale, np.float64)) @ R.T + p
+        return tuple(np.concatenate([w.min(axis=0), w.max(axis=0)]).astype(np.float32))
     if c.type == SHAPE_MESH:
         pts = np.asarray(c.triangle if c.triangle is not None else c.mesh[0], np.float64).reshape(-1, 3)
         w = (pts * np.array(c.scale, np.float64)) @ R.T + p

@@ -63,7 +63,7 @@ class _ConvexHull(C.Structure):
 
 class _TriangleTree(C.Structure):
     _fields_ = [("root", C.c_uint32), ("num_nodes", C.c_uint32), ("nodes", C.c_void_p),
-                ("num_triangles", C.c_uint32), ("triangles", C.c_void_p)]
+                ("num_triangles", C.c_uint32), ("triangles", C.c_void_p), ("flags", C.c_uint32)]
 
 
 class _ColliderDesc(C.Structure):
@@ -362,7 +362,8 @@ class Context:
                     raise XpbdError("mesh collider without tree data")
                 t = c.tree
                 a = (np.ascontiguousarray(t.nodes, TREE_NODE), np.ascontiguousarray(t.triangles, np.float32).reshape(-1, 9))
-                d = _TriangleTree(int(t.root), len(a[0]), a[0].ctypes.data if len(a[0]) else None, len(a[1]), a[1].ctypes.data)
+                d = _TriangleTree(int(t.root), len(a[0]), a[0].ctypes.data if len(a[0]) else None, len(a[1]),
+                                  a[1].ctypes.data if len(a[1]) else None, int(t.flags))
                 keep.append((a, d))
                 arr[i].tree = C.pointer(d)
             for k in range(3):

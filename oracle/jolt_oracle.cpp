@@ -43,6 +43,7 @@
 #include <Jolt/Physics/Collision/Shape/TaperedCapsuleShape.h>
 #include <Jolt/Physics/Collision/Shape/ConvexHullShape.h>
 #include <Jolt/Physics/Collision/Shape/MeshShape.h>
+#include <Jolt/Physics/Collision/Shape/HeightFieldShape.h>
 #include <Jolt/Physics/Collision/Shape/ScaledShape.h>
 #include <Jolt/Physics/Collision/Shape/TaperedCylinderShape.h>
 #include <Jolt/Physics/Collision/Shape/TriangleShape.h>
@@ -695,12 +696,26 @@ int jo_world_add_mesh(void *wp, const float *vertices, int nv, const uint32 *ind
 	return AddStaticShape((World *)wp, r.Get(), 8, pos, rot, friction, restitution);
 }
 
+// Static height field collider (HeightFieldShapeSettings): n x n samples (row major, FLT_MAX for
+// no collision), block size, bits per sample, offset[3], scale[3] of the shape.
+int jo_world_add_height_field(void *wp, const float *samples, int n, int block_size, int bits_per_sample, const float *offset3,
+							  const float *scale3, const float *pos, const float *rot, float friction, float restitution)
+{
+	HeightFieldShapeSettings hs(samples, Vec3(offset3[0], offset3[1], offset3[2]), Vec3(scale3[0], scale3[1], scale3[2]), uint32(n));
+	hs.mBlockSize = uint32(block_size);
+	hs.mBitsPerSample = uint32(bits_per_sample);
+	Shape::ShapeResult r = hs.Create();
+	if (r.HasError() || r.Get()->GetSubType() != EShapeSubType::HeightField)
+		return -1;
+	return AddStaticShape((World *)wp, r.Get(), 8, pos, rot, friction, restitution);
+}
+
 // The collider's shape as the device takes it (jolt_xpbd_shapes.h): type, params[8], scale[3],
 // centre of mass com[3]; for a mesh or triangle with non-null arrays also its tree: the root
 // reference, the nodes (28 words each: 24 bound floats, 4 child references) and triangles (9
 // floats each).  sizes[2] = {nodes, triangles}.
 int jo_world_collider_describe(void *wp, int collider, int *type, float *params8, float *scale3, float *com3,
-							   int *sizes, uint32 *root, uint32 *nodes, float *triangles)
+							   int *sizes, uint32 *root, uint32 *nodes, float *triangles, uint32 *tree_flags)
 {
 	World *w = (World *)wp;
 	BodyLockRead lock(w->mSystem.GetBodyLockInterface(), w->mColliders[collider]);
@@ -721,6 +736,7 @@ int jo_world_collider_describe(void *wp, int collider, int *type, float *params8
 	if (d.type == XPBD_SHAPE_MESH && nodes != nullptr)
 	{
 		*root = data.tree.root;
+		*tree_flags = data.tree.flags;
 		memcpy(nodes, data.nodes.data(), data.nodes.size() * sizeof(xpbd_tree_node));
 		memcpy(triangles, data.triangles.data(), data.triangles.size() * sizeof(float));
 	}

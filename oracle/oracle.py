@@ -106,7 +106,9 @@ class JoltReference:
             lib.jo_world_add_mesh.restype = _i
             lib.jo_world_add_mesh.argtypes = [_vp, _vp, _i, _vp, _i, _i, _vp, _vp, _f, _f]
             lib.jo_world_collider_describe.restype = _i
-            lib.jo_world_collider_describe.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+            lib.jo_world_collider_describe.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+            lib.jo_world_add_height_field.restype = _i
+            lib.jo_world_add_height_field.argtypes = [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _f, _f]
             lib.jo_world_hull_data.restype = _i
             lib.jo_world_add_dynamic.restype = _i
             lib.jo_world_add_dynamic.argtypes = [_vp, _i, _vp, _vp, _vp, _f, _f, _vp, _vp]
@@ -253,6 +255,12 @@ class JoltWorld:
             if c.triangle is not None:
                 v = np.array(c.triangle, np.float32)
                 i = self.lib.jo_world_add_triangle(self.h, _ptr(v), _ptr(pos), _ptr(rot), float(c.friction), float(c.restitution))
+            elif c.height_field is not None:
+                samples, block, bits, off, sc = c.height_field
+                smp = np.ascontiguousarray(samples, np.float32)
+                i = self.lib.jo_world_add_height_field(self.h, _ptr(smp), int(smp.shape[0]), int(block), int(bits),
+                                                       _ptr(np.array(off, np.float32)), _ptr(np.array(sc, np.float32)),
+                                                       _ptr(pos), _ptr(rot), float(c.friction), float(c.restitution))
             else:
                 verts = np.ascontiguousarray(c.mesh[0], np.float32).reshape(-1, 3)
                 idx = np.ascontiguousarray(c.mesh[1], np.uint32).reshape(-1, 3)
@@ -296,15 +304,15 @@ class JoltWorld:
         t, params, scale, com = C.c_int(), np.zeros(8, np.float32), np.zeros(3, np.float32), np.zeros(3, np.float32)
         sizes = np.zeros(2, np.int32)
         if self.lib.jo_world_collider_describe(self.h, i, C.byref(t), _ptr(params), _ptr(scale), _ptr(com), _ptr(sizes),
-                                               None, None, None) != 0:
+                                               None, None, None, None) != 0:
             raise RuntimeError("collider_describe failed")
-        root = C.c_uint32()
+        root, flags = C.c_uint32(), C.c_uint32()
         nodes = np.zeros(int(sizes[0]), TREE_NODE)
         tris = np.zeros((int(sizes[1]), 9), np.float32)
         self.lib.jo_world_collider_describe(self.h, i, C.byref(t), _ptr(params), _ptr(scale), _ptr(com), _ptr(sizes),
                                             C.byref(root), nodes.ctypes.data if len(nodes) else _ptr(np.zeros(28, np.uint32)),
-                                            _ptr(tris) if len(tris) else _ptr(np.zeros(9, np.float32)))
-        return TriangleTree(int(root.value), nodes, tris)
+                                            _ptr(tris) if len(tris) else _ptr(np.zeros(9, np.float32)), C.byref(flags))
+        return TriangleTree(int(root.value), nodes, tris, int(flags.value))
 
     def collider_device(self, i):
         """(shape parameters as the shape holds them [8], centre-of-mass position [3])."""
@@ -465,7 +473,7 @@ class Restatement:
             lib.xr_world_set_collider_scale.restype = _i
             lib.xr_world_set_collider_scale.argtypes = [_vp, _i, _vp]
             lib.xr_world_set_tree.restype = _i
-            lib.xr_world_set_tree.argtypes = [_vp, _i, _u32, _vp, _vp, _u32, _vp, _u32]
+            lib.xr_world_set_tree.argtypes = [_vp, _i, _u32, _vp, _vp, _u32, _vp, _u32, _u32]
             lib.xr_world_add_soft_body.restype = _i
             lib.xr_world_add_soft_body.argtypes = [_vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32,
                                                    _vp, _u32, _vp, _u32, _vp, _vp, _vp]
@@ -514,7 +522,8 @@ class Restatement:
             children = np.ascontiguousarray(nodes["child"], np.uint32)
             tris = np.ascontiguousarray(c.tree.triangles, np.float32)
             if self.lib.xr_world_set_tree(self.h, i, int(c.tree.root), _ptr(bounds) if len(nodes) else None,
-                                          _ptr(children) if len(nodes) else None, len(nodes), _ptr(tris), len(tris)) != 0:
+                                          _ptr(children) if len(nodes) else None, len(nodes), _ptr(tris) if len(tris) else None,
+                                          len(tris), int(c.tree.flags)) != 0:
                 raise RuntimeError("invalid triangle tree")
         if i >= 0 and tuple(float(x) for x in c.scale) != (1.0, 1.0, 1.0):
             self.lib.xr_world_set_collider_scale(self.h, i, _ptr(np.array(c.scale, np.float32)))

@@ -294,6 +294,7 @@ struct Collider
 	std::vector<u32> tree_child;
 	std::vector<float> tree_tris;
 	u32 tree_root = 0;
+	bool tree_height_field = false;	// invalid child bounds (min y > max y) are never visited
 };
 
 template <class F> struct Leaf
@@ -469,6 +470,9 @@ template <class F> TriangleHit<F> WalkTriangleTree(const Collider &c, V3<F> scal
 				F cy = SMin(SMax(local.y, lo[1]), hi[1]) - local.y;
 				F cz = SMin(SMax(local.z, lo[2]), hi[2]) - local.z;
 				d[k] = (cx * cx + cy * cy) + cz * cz;
+				// a height field's block without collision (HeightFieldShape.cpp:2268-2269)
+				if (c.tree_height_field && !(bd[4 + k] <= bd[16 + k]))
+					d[k] = F(FLT_MAX);
 				id[k] = c.tree_child[4 * size_t(ref) + k];
 			}
 			auto exchange = [&](int p, int q) { if (d[q] > d[p]) { std::swap(d[p], d[q]); std::swap(id[p], id[q]); } };
@@ -1535,8 +1539,9 @@ int xr_world_set_collider_scale(void *wp, int collider, const float *scale3)
 
 // Triangle tree of mesh collider `collider` (type 8): the root reference, nn nodes of 24 bound
 // floats (min x, y, z, max x, y, z of the four children) and 4 child references each, nt
-// triangles of 9 floats.
-int xr_world_set_tree(void *wp, int collider, u32 root, const float *bounds, const u32 *children, u32 nn, const float *tris, u32 nt)
+// triangles of 9 floats; flags bit 0: a height field's hierarchy.
+int xr_world_set_tree(void *wp, int collider, u32 root, const float *bounds, const u32 *children, u32 nn, const float *tris, u32 nt,
+					  u32 flags)
 {
 	XrWorld *w = (XrWorld *)wp;
 	if (collider < 0 || size_t(collider) >= w->f.colliders.size() || w->f.colliders[collider].type != 8)
@@ -1548,6 +1553,7 @@ int xr_world_set_tree(void *wp, int collider, u32 root, const float *bounds, con
 		c->tree_bounds.assign(bounds, bounds + 24 * size_t(nn));
 		c->tree_child.assign(children, children + 4 * size_t(nn));
 		c->tree_tris.assign(tris, tris + 9 * size_t(nt));
+		c->tree_height_field = (flags & 1u) != 0;
 	}
 	return 0;
 }

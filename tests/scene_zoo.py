@@ -231,6 +231,40 @@ def cloth_on_mesh(n=40, frames=30, spacing=0.05, height=0.3, iterations=5, colli
                  collision_steps=collision_steps, frames=frames)
 
 
+def _height_samples(n, seed, hole=True):
+    """n x n height samples: smooth bumps of a few centimetres, and a square hole of samples
+    without collision (FLT_MAX)."""
+    rng = np.random.default_rng(seed)
+    x = np.linspace(0.0, 1.0, n)
+    X, Y = np.meshgrid(x, x, indexing="xy")
+    h = 0.03 * np.sin(6.0 * X + rng.uniform(0, 6)) * np.cos(5.0 * Y + rng.uniform(0, 6)) + rng.uniform(-0.005, 0.005, X.shape)
+    h = h.astype(np.float32)
+    if hole:
+        h[n // 2 - 2:n // 2 + 2, n // 4:n // 4 + 3] = np.float32(3.4028235e38)
+    return h
+
+
+def cloth_on_height_field(n=40, frames=30, spacing=0.05, height=0.25, iterations=5, collision_steps=2, samples=32, block=4):
+    """n x n cloth (calm parameters) falling onto a HeightFieldShape terrain (samples^2, blocks of
+    block^2, 8-bit samples; a square hole without collision), turned so that its y axis is up
+    (+z) and scaled by a ScaledShape."""
+    v, f = scenes.cloth(n, spacing, 60.0, jitter=1e-4, seed=17)
+    mesh = MeshSpec(vertices=v, faces=f, attrs=scenes.attrs(1e-4, 1e-4, 1e-3), bend_type=BEND_DISTANCE)
+    body = BodySpec(mesh=mesh, position=(0.05, -0.03, height),
+                    params=BodyParams(num_iterations=iterations, allow_sleeping=False))
+    size = 3.0
+    hf = (_height_samples(samples, 18), block, 8, (-size / 2, 0.0, -size / 2), (size / (samples - 1), 1.0, size / (samples - 1)))
+    cols = [Collider(SHAPE_MESH, (), pos=(0.0, 0.0, 0.0), rot=_quat((1, 0, 0), 1.5707963267948966), friction=0.5,
+                     scale=(1.0, 1.2, 1.0), height_field=hf)]
+    return Scene("cloth_on_height_field", [body], _bounded(cols), gravity=(0.0, 0.0, -9.81), dt=1 / 60,
+                 collision_steps=collision_steps, frames=frames)
+
+
+def cloth_on_height_field_streamed():
+    """The height field scene with a 120 x 120 cloth (14,400 vertices), a few frames."""
+    return cloth_on_height_field(n=120, frames=8, spacing=0.02, height=0.05, iterations=4, collision_steps=1)
+
+
 def cloth_on_mesh_streamed():
     """The mesh scene with a 120 x 120 cloth (14,400 vertices, past the LDS budget), a few frames."""
     return cloth_on_mesh(n=120, frames=6, spacing=0.02, height=0.12, iterations=4, collision_steps=1)
@@ -254,6 +288,8 @@ ALL = {
     "cloth_scaled_shapes": cloth_scaled_shapes,
     "cloth_on_mesh": cloth_on_mesh,
     "cloth_on_mesh_streamed": cloth_on_mesh_streamed,
+    "cloth_on_height_field": cloth_on_height_field,
+    "cloth_on_height_field_streamed": cloth_on_height_field_streamed,
 }
 
 

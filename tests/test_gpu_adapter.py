@@ -35,6 +35,7 @@ SCENES = {
     "cloth100_sphere": scene_zoo.big_cloth_on_sphere,
     "cloth_scaled_shapes": scene_zoo.cloth_scaled_shapes,
     "cloth_on_mesh": scene_zoo.cloth_on_mesh,
+    "cloth_on_height_field": scene_zoo.cloth_on_height_field,
 }
 
 

@@ -281,3 +281,53 @@ def test_flattened_mesh_holds_every_triangle_once(cells, per_leaf):
                 first, count = ref & 0xffffff, (ref >> 24) & 0x3f
                 pts = tree.triangles[first:first + count].reshape(-1, 3)
                 assert np.all(pts >= nodes["min"][k][:, ch] - 1e-5) and np.all(pts <= nodes["max"][k][:, ch] + 1e-5)
+
+
+@pytest.mark.skipif(not _have_reference(), reason="reference library not built (needs the reference sources)")
+@pytest.mark.parametrize("samples,block", [(16, 2), (32, 4), (64, 8)])
+def test_flattened_height_field_keeps_the_cells_with_collision(samples, block):
+    # per cell (x, y) the walk emits its first triangle when samples (x, y), (x+1, y+1) and
+    # (x, y+1) have collision, its second when (x, y), (x+1, y+1) and (x+1, y) do
+    import scene_zoo
+    from oracle import JoltReference, JoltWorld
+    h = scene_zoo._height_samples(samples, 9)
+    valid = h < 1e30
+    want = 0
+    for y in range(samples - 1):
+        for x in range(samples - 1):
+            if valid[y, x] and valid[y + 1, x + 1]:
+                want += int(valid[y + 1, x]) + int(valid[y, x + 1])
+    world = JoltWorld(JoltReference("nofma"), (0.0, 0.0, -9.81))
+    try:
+        hf = (h, block, 8, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+        tree = world.collider_tree(world.add_collider(Collider(SHAPE_MESH, (), height_field=hf)))
+    finally:
+        world.close()
+    assert tree.flags == 1
+    assert len(tree.triangles) == want
+    # every triangle lies on the sample grid, first vertex at integer (x, z), front face up (+y)
+    t = tree.triangles.reshape(-1, 3, 3).astype(np.float64)
+    assert np.all(t[:, :, 0] == np.round(t[:, :, 0])) and np.all(t[:, :, 2] == np.round(t[:, :, 2]))
+    n = np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0])
+    assert np.all(n[:, 1] > 0.0)
+
+
+@pytest.mark.skipif(not _have_reference(), reason="reference library not built (needs the reference sources)")
+def test_flat_height_field_gives_the_plane_below_and_nothing_over_a_hole():
+    from oracle import JoltReference, JoltWorld
+    n = 16
+    h = np.full((n, n), 0.25, np.float32)
+    h[6:9, 6:9] = np.float32(3.4028235e38)
+    world = JoltWorld(JoltReference("nofma"), (0.0, 0.0, -9.81))
+    try:
+        c = Collider(SHAPE_MESH, (), height_field=(h, 2, 8, (0.0, 0.0, 0.0), (0.5, 1.0, 0.5)))
+        tree = world.collider_tree(world.add_collider(c))
+    finally:
+        world.close()
+    c = _at_origin(Collider(SHAPE_MESH, (), tree=tree))
+    y0 = tree.triangles[0][1]                    # the quantised height of the flat field
+    assert abs(y0 - 0.25) < 1e-3
+    np.testing.assert_allclose(_probe_plane(c, (2.3, 1.0, 1.7)), [0, 1, 0, -y0], atol=1e-7)
+    # over the middle of the hole the nearest triangle is at its rim: an edge plane, not y-up
+    p = _probe_plane(c, (3.6, 0.3, 3.6))
+    assert abs(p[1]) < 0.999
