@@ -274,6 +274,43 @@ typedef struct xpbd_body_desc {
 
 int xpbd_body_create(xpbd_context *ctx, const xpbd_body_desc *desc, int32_t *out_body);
 
+/* Placement steps from a collider body's centre of mass down to one leaf shape of a compound or
+ * decorated shape, as CollectTransformedShapes takes them (SoftBodyMotionProperties.cpp:174-188):
+ * each collision step the device starts from the body's position p and rotation q relative to the
+ * soft body and applies the steps in order; the leaf's transform is then (q, p). */
+#define XPBD_OP_CULL 1          /* leave the part out unless the soft body's local bounds, as an oriented box
+                                   in the current (p, q) space, overlap the box v[0..5] (min xyz, max xyz): a
+                                   compound child's bounds (or a static compound tree node's, one step per
+                                   level), times the compound's scale (CompoundShapeVisitors.h:226-256,
+                                   AABox4.h:86-185) */
+#define XPBD_OP_TRANSLATE 2     /* p = p + q * v[0..2]: a compound child's scaled centre of mass
+                                   (CompoundShapeVisitors.h:266) */
+#define XPBD_OP_ROTATE 3        /* q = q * v[0..3] (x y z w): a compound child's or a RotatedTranslatedShape's
+                                   rotation (CompoundShapeVisitors.h:267, RotatedTranslatedShape.cpp:175) */
+#define XPBD_OP_OFFSET 4        /* p = p - q * v[0..2]: an OffsetCenterOfMassShape's scaled offset
+                                   (OffsetCenterOfMassShape.cpp:141) */
+#define XPBD_MAX_PART_OPS 1024
+
+typedef struct xpbd_part_op {
+    uint32_t kind;
+    float v[6];
+} xpbd_part_op;
+
+/* One leaf shape of a collider body, in the order the reference collects them (StaticCompoundShape
+ * walks its tree last child first, MutableCompoundShape its sub shapes in index order;
+ * RotatedTranslatedShape, OffsetCenterOfMassShape and ScaledShape pass through to their inner
+ * shape).  type, params, scale, hull and tree as in xpbd_collider_desc; ops: the placement steps
+ * (copied by xpbd_set_colliders). */
+typedef struct xpbd_collider_part {
+    int32_t type;
+    float params[8];
+    float scale[3];
+    const xpbd_convex_hull *hull;
+    const xpbd_triangle_tree *tree;
+    uint32_t num_ops;
+    const xpbd_part_op *ops;
+} xpbd_collider_part;
+
 /*
  * Static colliders of the world, in broadphase order (the order in which
  * PhysicsSystem::GetBroadPhaseQuery().CollideAABox reports them,
@@ -319,6 +356,12 @@ typedef struct xpbd_collider_desc {
      * centre of mass of the scaled shape. */
     float scale[3];
     const xpbd_triangle_tree *tree;     /* XPBD_SHAPE_MESH only (copied by xpbd_set_colliders) */
+    /* A compound or decorated body (num_parts > 0): its leaf shapes, copied by xpbd_set_colliders;
+     * type, params, scale, hull and tree above are then unused.  The body's leaves share its
+     * motion, material and contact settings; a vertex's contact names the body
+     * (CollidingShape, SoftBodyMotionProperties.h:200-212).  num_parts 0: the one shape above. */
+    uint32_t num_parts;
+    const xpbd_collider_part *parts;
 } xpbd_collider_desc;
 
 int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *colliders, uint32_t num_colliders);

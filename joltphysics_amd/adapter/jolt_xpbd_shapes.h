@@ -20,14 +20,20 @@
 #include <Jolt/Physics/Collision/Shape/ConvexHullShape.h>
 #include <Jolt/Physics/Collision/Shape/CylinderShape.h>
 #include <Jolt/Physics/Collision/Shape/HeightFieldShape.h>
+#include <Jolt/Geometry/AABox4.h>
 #include <Jolt/Physics/Collision/Shape/MeshShape.h>
+#include <Jolt/Physics/Collision/Shape/MutableCompoundShape.h>
+#include <Jolt/Physics/Collision/Shape/OffsetCenterOfMassShape.h>
 #include <Jolt/Physics/Collision/Shape/PlaneShape.h>
+#include <Jolt/Physics/Collision/Shape/RotatedTranslatedShape.h>
 #include <Jolt/Physics/Collision/Shape/ScaledShape.h>
 #include <Jolt/Physics/Collision/Shape/SphereShape.h>
+#include <Jolt/Physics/Collision/Shape/StaticCompoundShape.h>
 #include <Jolt/Physics/Collision/Shape/TaperedCapsuleShape.h>
 #include <Jolt/Physics/Collision/Shape/TaperedCylinderShape.h>
 #include <Jolt/Physics/Collision/Shape/TriangleShape.h>
 
+#include <array>
 #include <cstring>
 #include <memory>
 #include <sstream>
@@ -48,6 +54,10 @@ struct ShapeData
     std::vector<xpbd_tree_node> nodes;
     std::vector<float> triangles;
     xpbd_triangle_tree tree{};
+    // a compound or decorated shape: its leaves, their placement steps and tables
+    std::vector<xpbd_collider_part> parts;
+    std::vector<std::vector<xpbd_part_op>> part_ops;
+    std::vector<std::unique_ptr<ShapeData>> part_data;
 };
 
 namespace shapes_detail {
@@ -86,6 +96,23 @@ inline bool TaperedCylinderParams(const JPH::TaperedCylinderShape &shape, float 
     if (s.size() < at + 16)
         return false;
     memcpy(out, s.data() + at, 16);
+    return true;
+}
+
+// TaperedCapsuleShape's {top centre, bottom centre, top radius, bottom radius, sin alpha, tan
+// alpha} as the shape holds them: SaveBinaryState writes after ConvexShape's bytes the centre of
+// mass (3 floats), the top and bottom radius, top and bottom centre, convex radius, sin and tan
+// of the cone angle (TaperedCapsuleShape.cpp:406-418).  GetHalfHeight() is rebuilt from the
+// centres, so the centres cannot be recomputed from it exactly.
+inline bool TaperedCapsuleParams(const JPH::TaperedCapsuleShape &shape, float out[6])
+{
+    const std::string s = shapes_detail::SavedState(shape);
+    const size_t at = shapes_detail::kConvexShapeStateBytes + 12;
+    if (s.size() < at + 28)
+        return false;
+    float v[7];     // top radius, bottom radius, top centre, bottom centre, convex radius, sin, tan
+    memcpy(v, s.data() + at, 28);
+    out[0] = v[2]; out[1] = v[3]; out[2] = v[0]; out[3] = v[1]; out[4] = v[5]; out[5] = v[6];
     return true;
 }
 
@@ -387,74 +414,65 @@ inline void TriangleTree(const JPH::TriangleShape &shape, ShapeData &out)
     out.tree.flags = 0;
 }
 
-// Shape type, parameters, scale and tables of `shape` into `d` (position, rotation, material
-// and motion are the caller's).  ScaledShape wrappers multiply into d.scale (ScaledShape.cpp:
-// 142-150).  False (and `err`) for shapes the device does not collide with.
-inline bool DescribeShape(const JPH::Shape *shape, xpbd_collider_desc &d, ShapeData &data, std::string &err)
+// Type, parameters, scale and tables of the leaf shape `shape` seen at scale `scale` (convex
+// shapes, triangles, meshes and height fields; hull and tree point into `data`).  False (and `err`)
+// for shapes the device does not collide with.
+inline bool DescribeLeaf(const JPH::Shape *shape, JPH::Vec3Arg scale, int32_t &type, float params[8], float scale_out[3],
+                         const xpbd_convex_hull *&hull, const xpbd_triangle_tree *&tree, ShapeData &data, std::string &err)
 {
-    JPH::Vec3 scale = JPH::Vec3::sOne();
-    while (shape->GetSubType() == JPH::EShapeSubType::Scaled)
-    {
-        const auto *s = static_cast<const JPH::ScaledShape *>(shape);
-        scale = scale * s->GetScale();
-        shape = s->GetInnerShape();
-    }
-    d.scale[0] = scale.GetX(); d.scale[1] = scale.GetY(); d.scale[2] = scale.GetZ();
-    d.hull = nullptr;
-    d.tree = nullptr;
+    scale_out[0] = scale.GetX(); scale_out[1] = scale.GetY(); scale_out[2] = scale.GetZ();
+    hull = nullptr;
+    tree = nullptr;
+    for (int k = 0; k < 8; ++k)
+        params[k] = 0.0f;
     switch (shape->GetSubType())
     {
     case JPH::EShapeSubType::Plane:
         {
             const auto *s = static_cast<const JPH::PlaneShape *>(shape);
             const JPH::Plane &pl = s->GetPlane();
-            d.type = XPBD_SHAPE_PLANE;
-            d.params[0] = pl.GetNormal().GetX(); d.params[1] = pl.GetNormal().GetY(); d.params[2] = pl.GetNormal().GetZ();
-            d.params[3] = pl.GetConstant(); d.params[4] = s->GetHalfExtent();
+            type = XPBD_SHAPE_PLANE;
+            params[0] = pl.GetNormal().GetX(); params[1] = pl.GetNormal().GetY(); params[2] = pl.GetNormal().GetZ();
+            params[3] = pl.GetConstant(); params[4] = s->GetHalfExtent();
         }
         return true;
     case JPH::EShapeSubType::Sphere:
-        d.type = XPBD_SHAPE_SPHERE;
-        d.params[0] = static_cast<const JPH::SphereShape *>(shape)->GetRadius();
+        type = XPBD_SHAPE_SPHERE;
+        params[0] = static_cast<const JPH::SphereShape *>(shape)->GetRadius();
         return true;
     case JPH::EShapeSubType::Box:
         {
             const JPH::Vec3 he = static_cast<const JPH::BoxShape *>(shape)->GetHalfExtent();
-            d.type = XPBD_SHAPE_BOX;
-            d.params[0] = he.GetX(); d.params[1] = he.GetY(); d.params[2] = he.GetZ();
+            type = XPBD_SHAPE_BOX;
+            params[0] = he.GetX(); params[1] = he.GetY(); params[2] = he.GetZ();
         }
         return true;
     case JPH::EShapeSubType::Capsule:
         {
             const auto *s = static_cast<const JPH::CapsuleShape *>(shape);
-            d.type = XPBD_SHAPE_CAPSULE;
-            d.params[0] = s->GetHalfHeightOfCylinder(); d.params[1] = s->GetRadius();
+            type = XPBD_SHAPE_CAPSULE;
+            params[0] = s->GetHalfHeightOfCylinder(); params[1] = s->GetRadius();
         }
         return true;
     case JPH::EShapeSubType::Cylinder:
         {
             const auto *s = static_cast<const JPH::CylinderShape *>(shape);
-            d.type = XPBD_SHAPE_CYLINDER;
-            d.params[0] = s->GetHalfHeight(); d.params[1] = s->GetRadius();
+            type = XPBD_SHAPE_CYLINDER;
+            params[0] = s->GetHalfHeight(); params[1] = s->GetRadius();
         }
         return true;
     case JPH::EShapeSubType::TaperedCapsule:
+        // centres relative to the centre of mass and the cone angle, as the shape holds them
+        type = XPBD_SHAPE_TAPERED_CAPSULE;
+        if (!TaperedCapsuleParams(*static_cast<const JPH::TaperedCapsuleShape *>(shape), params))
         {
-            // centres relative to the centre of mass and the cone angle, as the shape's
-            // constructor derives them (TaperedCapsuleShape.cpp:86-131)
-            const auto *s = static_cast<const JPH::TaperedCapsuleShape *>(shape);
-            const float half_height = s->GetHalfHeight(), top = s->GetTopRadius(), bottom = s->GetBottomRadius();
-            const float top_center = half_height + 0.5f * (bottom - top);
-            const float bottom_center = -half_height + 0.5f * (bottom - top);
-            const float sin_alpha = (bottom - top) / (top_center - bottom_center);
-            d.type = XPBD_SHAPE_TAPERED_CAPSULE;
-            d.params[0] = top_center; d.params[1] = bottom_center; d.params[2] = top; d.params[3] = bottom;
-            d.params[4] = sin_alpha; d.params[5] = JPH::Tan(JPH::ASin(sin_alpha));
+            err = "tapered capsule state";
+            return false;
         }
         return true;
     case JPH::EShapeSubType::TaperedCylinder:
-        d.type = XPBD_SHAPE_TAPERED_CYLINDER;
-        if (!TaperedCylinderParams(*static_cast<const JPH::TaperedCylinderShape *>(shape), d.params))
+        type = XPBD_SHAPE_TAPERED_CYLINDER;
+        if (!TaperedCylinderParams(*static_cast<const JPH::TaperedCylinderShape *>(shape), params))
         {
             err = "tapered cylinder state";
             return false;
@@ -481,32 +499,312 @@ inline bool DescribeShape(const JPH::Shape *shape, xpbd_collider_desc &d, ShapeD
             }
             data.hull = { s->GetNumPoints(), data.points.data(), s->GetNumFaces(), data.planes.data(), data.first.data(),
                           data.count.data(), (uint32_t)data.index.size(), data.index.data() };
-            d.type = XPBD_SHAPE_CONVEX_HULL;
-            d.hull = &data.hull;
+            type = XPBD_SHAPE_CONVEX_HULL;
+            hull = &data.hull;
         }
         return true;
     case JPH::EShapeSubType::Triangle:
         TriangleTree(*static_cast<const JPH::TriangleShape *>(shape), data);
-        d.type = XPBD_SHAPE_MESH;
-        d.tree = &data.tree;
+        type = XPBD_SHAPE_MESH;
+        tree = &data.tree;
         return true;
     case JPH::EShapeSubType::Mesh:
         if (!FlattenMesh(*static_cast<const JPH::MeshShape *>(shape), data, err))
             return false;
-        d.type = XPBD_SHAPE_MESH;
-        d.tree = &data.tree;
+        type = XPBD_SHAPE_MESH;
+        tree = &data.tree;
         return true;
     case JPH::EShapeSubType::HeightField:
         if (!FlattenHeightField(*static_cast<const JPH::HeightFieldShape *>(shape), data, err))
             return false;
-        d.type = XPBD_SHAPE_MESH;
-        d.tree = &data.tree;
+        type = XPBD_SHAPE_MESH;
+        tree = &data.tree;
         return true;
     default:
-        err = "a soft body reaches a collider shape the device path does not support (compound, rotated / "
-              "translated or offset centre of mass shapes)";
+        err = "a soft body reaches a collider shape the device path does not support";
         return false;
     }
+}
+
+namespace shapes_detail {
+
+inline xpbd_part_op Op(uint32_t kind, std::initializer_list<float> v)
+{
+    xpbd_part_op op{};
+    op.kind = kind;
+    int k = 0;
+    for (float x : v)
+        op.v[k++] = x;
+    return op;
+}
+
+// One bounds step: a child's (or tree node's) box, scaled by the compound's scale the way the
+// compound's visitor scales it (AABox4Scale, AABox4.h:37-52)
+inline xpbd_part_op CullOp(JPH::Vec3Arg scale, const float mn[3], const float mx[3])
+{
+    JPH::Vec4 bmin_x(mn[0], 0, 0, 0), bmin_y(mn[1], 0, 0, 0), bmin_z(mn[2], 0, 0, 0);
+    JPH::Vec4 bmax_x(mx[0], 0, 0, 0), bmax_y(mx[1], 0, 0, 0), bmax_z(mx[2], 0, 0, 0);
+    JPH::Vec4 omin_x, omin_y, omin_z, omax_x, omax_y, omax_z;
+    JPH::AABox4Scale(scale, bmin_x, bmin_y, bmin_z, bmax_x, bmax_y, bmax_z, omin_x, omin_y, omin_z, omax_x, omax_y, omax_z);
+    return Op(XPBD_OP_CULL, { omin_x.GetX(), omin_y.GetX(), omin_z.GetX(), omax_x.GetX(), omax_y.GetX(), omax_z.GetX() });
+}
+
+// StaticCompoundShape's tree (the bytes SaveBinaryState writes after CompoundShape's: node count,
+// then per node 4 x 6 half-float bounds and 4 child words, StaticCompoundShape.h:100-124; IS_SUBSHAPE
+// 0x80000000, INVALID_NODE 0x7fffffff): for each sub shape in the order WalkTree visits them (a
+// node's children pushed in order, the last one popped first, StaticCompoundShape.cpp:358-419), the
+// bounds of every entry on its way down from the root node
+struct StaticLeaf
+{
+    uint32_t sub_shape;
+    std::vector<std::array<float, 6>> path;     // min xyz, max xyz per level, root first
+};
+
+inline bool StaticCompoundOrder(const JPH::StaticCompoundShape &shape, std::vector<StaticLeaf> &out, std::string &err)
+{
+    const std::string state = SavedState(shape);
+    // CompoundShape::SaveBinaryState (CompoundShape.cpp:394-405): Shape's header, centre of mass,
+    // local bounds, inner radius, sub shape count and per sub shape: user data, position, rotation
+    size_t at = kShapeStateBytes + 12 + 24 + 4;
+    uint32_t num_sub = 0;
+    if (state.size() < at + 4)
+    {
+        err = "static compound state";
+        return false;
+    }
+    memcpy(&num_sub, state.data() + at, 4);
+    at += 4 + (size_t)num_sub * (4 + 12 + 12);
+    uint32_t num_nodes = 0;
+    if (state.size() < at + 4)
+    {
+        err = "static compound state";
+        return false;
+    }
+    memcpy(&num_nodes, state.data() + at, 4);
+    at += 4;
+    constexpr size_t kNodeBytes = 24 * 2 + 4 * 4;
+    if (num_nodes == 0 || state.size() < at + (size_t)num_nodes * kNodeBytes)
+    {
+        err = "static compound tree state";
+        return false;
+    }
+    struct Node { float mn[3][4], mx[3][4]; uint32_t child[4]; };
+    std::vector<Node> nodes(num_nodes);
+    for (uint32_t n = 0; n < num_nodes; ++n)
+    {
+        const char *p = state.data() + at + (size_t)n * kNodeBytes;
+        JPH::HalfFloat h[24];
+        memcpy(h, p, sizeof(h));
+        for (int a = 0; a < 3; ++a)
+        {
+            Widen4(h + 4 * a, nodes[n].mn[a]);
+            Widen4(h + 12 + 4 * a, nodes[n].mx[a]);
+        }
+        memcpy(nodes[n].child, p + 48, 16);
+    }
+    constexpr uint32_t kSubShape = 0x80000000u, kInvalid = 0x7fffffffu;
+    struct Entry { uint32_t ref; std::vector<std::array<float, 6>> path; };
+    std::vector<Entry> stack{ { 0u, {} } };
+    while (!stack.empty())
+    {
+        Entry e = std::move(stack.back());
+        stack.pop_back();
+        if (e.ref == kInvalid)
+            continue;
+        if (e.ref & kSubShape)
+        {
+            out.push_back({ e.ref ^ kSubShape, std::move(e.path) });
+            continue;
+        }
+        if (e.ref >= num_nodes || e.path.size() > 64)
+        {
+            err = "static compound tree state";
+            return false;
+        }
+        const Node &nd = nodes[e.ref];
+        for (int k = 0; k < 4; ++k)
+        {
+            if (nd.child[k] == kInvalid)
+                continue;
+            Entry c{ nd.child[k], e.path };
+            c.path.push_back({ nd.mn[0][k], nd.mn[1][k], nd.mn[2][k], nd.mx[0][k], nd.mx[1][k], nd.mx[2][k] });
+            stack.push_back(std::move(c));
+        }
+    }
+    if (out.size() != num_sub)
+    {
+        err = "static compound tree does not reach every sub shape once";
+        return false;
+    }
+    return true;
+}
+
+// MutableCompoundShape's per sub shape bounds (the Bounds blocks SaveBinaryState writes after
+// CompoundShape's: 6 x 4 floats per block of 4 sub shapes, MutableCompoundShape.cpp:560-568)
+inline bool MutableCompoundBounds(const JPH::MutableCompoundShape &shape, std::vector<std::array<float, 6>> &out, std::string &err)
+{
+    const std::string state = SavedState(shape);
+    size_t at = kShapeStateBytes + 12 + 24 + 4;
+    uint32_t num_sub = 0;
+    if (state.size() < at + 4)
+    {
+        err = "mutable compound state";
+        return false;
+    }
+    memcpy(&num_sub, state.data() + at, 4);
+    at += 4 + (size_t)num_sub * (4 + 12 + 12);
+    const uint32_t num_blocks = (num_sub + 3) / 4;
+    if (state.size() < at + (size_t)num_blocks * 96)
+    {
+        err = "mutable compound bounds state";
+        return false;
+    }
+    out.resize(num_sub);
+    for (uint32_t i = 0; i < num_sub; ++i)
+    {
+        const char *block = state.data() + at + (size_t)(i / 4) * 96;
+        for (int a = 0; a < 6; ++a)
+            memcpy(&out[i][a], block + 16 * a + 4 * (i % 4), 4);
+    }
+    return true;
+}
+
+} // namespace shapes_detail
+
+// The leaves of `shape` at scale `scale`, after the placement steps `ops`, in the order
+// CollectTransformedShapes reaches them: ScaledShape multiplies the scale (ScaledShape.cpp:147-154),
+// RotatedTranslatedShape rotates (RotatedTranslatedShape.cpp:169-176), OffsetCenterOfMassShape
+// shifts (OffsetCenterOfMassShape.cpp:135-142) and compounds test each child's bounds, then move
+// to and rotate into the child (CompoundShapeVisitors.h:226-272).
+inline bool CollectParts(const JPH::Shape *shape, JPH::Vec3Arg scale, std::vector<xpbd_part_op> &ops, ShapeData &data, std::string &err)
+{
+    using namespace shapes_detail;
+    switch (shape->GetSubType())
+    {
+    case JPH::EShapeSubType::Scaled:
+        {
+            const auto *s = static_cast<const JPH::ScaledShape *>(shape);
+            return CollectParts(s->GetInnerShape(), scale * s->GetScale(), ops, data, err);
+        }
+    case JPH::EShapeSubType::RotatedTranslated:
+        {
+            const auto *s = static_cast<const JPH::RotatedTranslatedShape *>(shape);
+            const JPH::Quat r = s->GetRotation();
+            ops.push_back(Op(XPBD_OP_ROTATE, { r.GetX(), r.GetY(), r.GetZ(), r.GetW() }));
+            const bool ok = CollectParts(s->GetInnerShape(), s->TransformScale(scale), ops, data, err);
+            ops.pop_back();
+            return ok;
+        }
+    case JPH::EShapeSubType::OffsetCenterOfMass:
+        {
+            const auto *s = static_cast<const JPH::OffsetCenterOfMassShape *>(shape);
+            const JPH::Vec3 off = scale * s->GetOffset();
+            ops.push_back(Op(XPBD_OP_OFFSET, { off.GetX(), off.GetY(), off.GetZ() }));
+            const bool ok = CollectParts(s->GetInnerShape(), scale, ops, data, err);
+            ops.pop_back();
+            return ok;
+        }
+    case JPH::EShapeSubType::StaticCompound:
+    case JPH::EShapeSubType::MutableCompound:
+        {
+            const auto *cs = static_cast<const JPH::CompoundShape *>(shape);
+            // (sub shape, bounds steps) in collection order
+            std::vector<std::pair<uint32_t, std::vector<std::array<float, 6>>>> order;
+            if (shape->GetSubType() == JPH::EShapeSubType::StaticCompound)
+            {
+                std::vector<StaticLeaf> leaves;
+                if (!StaticCompoundOrder(*static_cast<const JPH::StaticCompoundShape *>(shape), leaves, err))
+                    return false;
+                for (StaticLeaf &l : leaves)
+                    order.emplace_back(l.sub_shape, std::move(l.path));
+            }
+            else
+            {
+                std::vector<std::array<float, 6>> bounds;
+                if (!MutableCompoundBounds(*static_cast<const JPH::MutableCompoundShape *>(shape), bounds, err))
+                    return false;
+                for (uint32_t i = 0; i < (uint32_t)bounds.size(); ++i)
+                    order.push_back({ i, { bounds[i] } });
+            }
+            for (const auto &entry : order)
+            {
+                const JPH::CompoundShape::SubShape &sub = cs->GetSubShape(entry.first);
+                const size_t depth = ops.size();
+                for (const std::array<float, 6> &b : entry.second)
+                    ops.push_back(CullOp(scale, b.data(), b.data() + 3));
+                const JPH::Vec3 pos = scale * sub.GetPositionCOM();
+                const JPH::Quat rot = sub.GetRotation();
+                ops.push_back(Op(XPBD_OP_TRANSLATE, { pos.GetX(), pos.GetY(), pos.GetZ() }));
+                ops.push_back(Op(XPBD_OP_ROTATE, { rot.GetX(), rot.GetY(), rot.GetZ(), rot.GetW() }));
+                const bool ok = CollectParts(sub.mShape, sub.TransformScale(scale), ops, data, err);
+                ops.resize(depth);
+                if (!ok)
+                    return false;
+            }
+            return true;
+        }
+    default:
+        {
+            if (ops.size() > XPBD_MAX_PART_OPS)
+            {
+                err = "a compound leaf lies more than XPBD_MAX_PART_OPS placement steps deep";
+                return false;
+            }
+            data.part_data.push_back(std::make_unique<ShapeData>());
+            xpbd_collider_part part{};
+            if (!DescribeLeaf(shape, scale, part.type, part.params, part.scale, part.hull, part.tree, *data.part_data.back(), err))
+                return false;
+            data.part_ops.push_back(ops);
+            data.parts.push_back(part);
+            return true;
+        }
+    }
+}
+
+// Shape type, parameters, scale and tables of `shape` into `d` (position, rotation, material
+// and motion are the caller's).  A leaf shape inside ScaledShape wrappers fills type, params,
+// scale, hull / tree; compounds and RotatedTranslated / OffsetCenterOfMass shapes fill
+// num_parts / parts.  False (and `err`) for shapes the device does not collide with.
+inline bool DescribeShape(const JPH::Shape *shape, xpbd_collider_desc &d, ShapeData &data, std::string &err)
+{
+    JPH::Vec3 scale = JPH::Vec3::sOne();
+    const JPH::Shape *leaf = shape;
+    while (leaf->GetSubType() == JPH::EShapeSubType::Scaled)
+    {
+        const auto *s = static_cast<const JPH::ScaledShape *>(leaf);
+        scale = scale * s->GetScale();
+        leaf = s->GetInnerShape();
+    }
+    d.num_parts = 0;
+    d.parts = nullptr;
+    const JPH::EShapeType t = leaf->GetType();
+    if (t != JPH::EShapeType::Compound && t != JPH::EShapeType::Decorated)
+        return DescribeLeaf(leaf, scale, d.type, d.params, d.scale, d.hull, d.tree, data, err);
+    data.parts.clear();
+    data.part_ops.clear();
+    data.part_data.clear();
+    std::vector<xpbd_part_op> ops;
+    if (!CollectParts(shape, JPH::Vec3::sOne(), ops, data, err))
+        return false;
+    if (data.parts.empty())
+    {
+        err = "compound shape without leaves";
+        return false;
+    }
+    for (size_t k = 0; k < data.parts.size(); ++k)
+    {
+        data.parts[k].num_ops = (uint32_t)data.part_ops[k].size();
+        data.parts[k].ops = data.part_ops[k].data();
+    }
+    d.type = data.parts[0].type;
+    for (int k = 0; k < 8; ++k)
+        d.params[k] = 0.0f;
+    d.scale[0] = d.scale[1] = d.scale[2] = 1.0f;
+    d.hull = nullptr;
+    d.tree = nullptr;
+    d.num_parts = (uint32_t)data.parts.size();
+    d.parts = data.parts.data();
+    return true;
 }
 
 } // namespace JoltXpbd

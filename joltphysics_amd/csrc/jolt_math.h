@@ -196,6 +196,15 @@ XHD Q4 mul_imaginary(V3 v, Q4 q)
               (v.z * q.w + v.x * q.y) - v.y * q.x,
               -(v.x * q.x + v.y * q.y) - v.z * q.z);
 }
+// Quat * Vec3 (Quat.inl:383-405): p + 2 (w (q x p) + q x (q x p)), each cross product formed
+// as the SSE build forms it (a swizzled difference of products)
+XHD V3 quat_rotate(Q4 q, V3 p)
+{
+    const V3 a = v3(p.z * q.y - q.z * p.y, p.x * q.z - q.x * p.z, p.y * q.x - q.y * p.x);
+    const V3 b = v3(a.z * q.y - q.z * a.y, a.x * q.z - q.x * a.z, a.y * q.x - q.y * a.x);
+    const V3 v = q.w * a + b;
+    return p + (v + v);
+}
 // RotateAxisZ: q * (0, 0, 1) written out (Quat.inl:434-440)
 XHD V3 rotate_axis_z(Q4 q)
 {

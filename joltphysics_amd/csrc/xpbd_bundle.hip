@@ -231,16 +231,17 @@ __device__ __forceinline__ void bundle_update(
                     wmax = wmax + vmax(a, bq);
                 }
                 const ColliderDev &cd = colliders[c];
-                const float4 pr = pair_settings(L.pair, c);
+                const float4 pr = pair_settings(L.pair, cd.owner);
                 bool overlap = !(wmin.x > cd.bounds[3] || wmin.y > cd.bounds[4] || wmin.z > cd.bounds[5]
                                  || wmax.x < cd.bounds[0] || wmax.y < cd.bounds[1] || wmax.z < cd.bounds[2]);
-                if (!overlap || pair_skipped(pr, cd.sensor >= 0, cd.motion == 2))
+                const M44 inv_com = inversed_rotation_translation(com);
+                if (!overlap || pair_skipped(pr, cd.sensor >= 0, cd.motion == 2) || !part_collected<EXT>(cd, inv_com, lmin, lmax))
                 {
                     out.type = -1;
                     continue;
                 }
-                make_leaf(cd, inversed_rotation_translation(com), L.friction, L.restitution, out, L.dyn, pr);
-                out.collider = (int32_t)c;
+                make_leaf<EXT>(cd, inv_com, L.friction, L.restitution, out, L.dyn, pr);
+                out.collider = cd.owner;
             }
             sync_global();
             // colliding list of each body in collider (broadphase) order
@@ -277,8 +278,9 @@ __device__ __forceinline__ void bundle_update(
                         for (uint32_t k = lane; k < bb.nv; k += kWave)
                             largest = smax(largest, sensor_penetration<EXT>(leaf, pget(X, bb.voff + k)));
                     largest = wave_max(largest);
+                    // a compound sensor is hit when one of its leaves is
                     if (lane == 0)
-                        L.sensor_hits[rank] = largest > 0.0f ? 1 : 0;
+                        L.sensor_hits[rank] = largest > 0.0f || (!colliders[c].first && L.sensor_hits[rank]) ? 1 : 0;
                 }
             }
 
@@ -441,7 +443,7 @@ __device__ __forceinline__ void bundle_update(
                                 V3 n = plane.n;
                                 x = x + n * projected_distance;
                                 if (cs.dyn >= 0)
-                                    dynamic_contact(L.dyn[cs.dyn], col3(cs.t, 3), x, v, prev_v, n, projected_distance, p.w,
+                                    dynamic_contact(L.dyn[cs.dyn], ld3(L.dyn[cs.dyn].com), x, v, prev_v, n, projected_distance, p.w,
                                                     cs.friction, cs.restitution, L.dt_sub, L.restitution_threshold,
                                                     cs.inv_mass_scale1);
                                 else if (cs.inv_mass_scale1 > 0.0f)

@@ -1291,6 +1291,7 @@ __device__ __forceinline__ void init_dynamic(const ColliderDev &c, const M44 &in
     const V3 lin = mul3x3(inv_com, ld3(c.lin_vel)), ang = mul3x3(inv_com, ld3(c.ang_vel));
     st3(D.lin, lin); st3(D.orig_lin, lin);
     st3(D.ang, ang); st3(D.orig_ang, ang);
+    st3(D.com, col3(rel, 3));
     D.updated = 0;
 }
 
@@ -1396,18 +1397,125 @@ __device__ __forceinline__ bool inside_out(V3 scale)
     return (((scale.x < 0.0f) ? 1 : 0) + ((scale.y < 0.0f) ? 1 : 0) + ((scale.z < 0.0f) ? 1 : 0)) % 2 == 1;
 }
 
+// AABox4VsBox (AABox4.h:86-185) for one box b[0..5] (min xyz, max xyz) against the soft body's
+// local bounds [lmin, lmax] seen from a compound at rotation q and position p: the oriented box
+// OrientedBox(Mat44::sInverseRotationTranslation(q, p), bounds) (CompoundShapeVisitors.h:229,
+// Mat44.inl:156-161, OrientedBox.h:27), epsilon 1e-6.  The reference evaluates every axis and
+// ANDs the results; stopping at the first separating axis gives the same answer.
+__device__ __forceinline__ bool box_overlaps(Q4 q, V3 p, V3 lmin, V3 lmax, const float *b)
+{
+    const M44 m = rotation_translation(-q.x, -q.y, -q.z, q.w, v3(0.0f, 0.0f, 0.0f));    // sRotation(Conjugated)
+    const V3 center = 0.5f * (lmin + lmax), he = 0.5f * (lmax - lmin);                // AABox::GetCenter / GetExtent
+    const V3 tr = neg(mul3x3(m, p)) + mul3x3(m, center);                              // PreTranslated(center)
+    // R(r, c) = m.c[c][r]; ar[i][j] = |R(j, i)| + epsilon
+    float ar[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            ar[i][j] = fabsf(m.c[i][j]) + 1.0e-6f;
+    float t[3], a[3];
+    for (int i = 0; i < 3; ++i)
+    {
+        t[i] = comp(tr, i) - 0.5f * (b[i] + b[3 + i]);
+        a[i] = 0.5f * (b[3 + i] - b[i]);
+    }
+    // L = A0, A1, A2
+    for (int i = 0; i < 3; ++i)
+        if (!(fabsf(t[i]) <= a[i] + ((he.x * ar[0][i] + he.y * ar[1][i]) + he.z * ar[2][i])))
+            return false;
+    // L = B0, B1, B2
+    for (int i = 0; i < 3; ++i)
+    {
+        const float ra = (a[0] * ar[i][0] + a[1] * ar[i][1]) + a[2] * ar[i][2];
+        const float d = (t[0] * m.c[i][0] + t[1] * m.c[i][1]) + t[2] * m.c[i][2];
+        if (!(fabsf(d) <= ra + comp(he, i)))
+            return false;
+    }
+    // L = Ai x Bj: ra over the two other axes of A, rb over the two other axes of B
+    #define XPBD_CROSS_AXIS(ra, rb, d) if (!(fabsf(d) <= (ra) + (rb))) return false;
+    XPBD_CROSS_AXIS(a[1] * ar[0][2] + a[2] * ar[0][1], he.y * ar[2][0] + he.z * ar[1][0], t[2] * m.c[0][1] - t[1] * m.c[0][2])
+    XPBD_CROSS_AXIS(a[1] * ar[1][2] + a[2] * ar[1][1], he.x * ar[2][0] + he.z * ar[0][0], t[2] * m.c[1][1] - t[1] * m.c[1][2])
+    XPBD_CROSS_AXIS(a[1] * ar[2][2] + a[2] * ar[2][1], he.x * ar[1][0] + he.y * ar[0][0], t[2] * m.c[2][1] - t[1] * m.c[2][2])
+    XPBD_CROSS_AXIS(a[0] * ar[0][2] + a[2] * ar[0][0], he.y * ar[2][1] + he.z * ar[1][1], t[0] * m.c[0][2] - t[2] * m.c[0][0])
+    XPBD_CROSS_AXIS(a[0] * ar[1][2] + a[2] * ar[1][0], he.x * ar[2][1] + he.z * ar[0][1], t[0] * m.c[1][2] - t[2] * m.c[1][0])
+    XPBD_CROSS_AXIS(a[0] * ar[2][2] + a[2] * ar[2][0], he.x * ar[1][1] + he.y * ar[0][1], t[0] * m.c[2][2] - t[2] * m.c[2][0])
+    XPBD_CROSS_AXIS(a[0] * ar[0][1] + a[1] * ar[0][0], he.y * ar[2][2] + he.z * ar[1][2], t[1] * m.c[0][0] - t[0] * m.c[0][1])
+    XPBD_CROSS_AXIS(a[0] * ar[1][1] + a[1] * ar[1][0], he.x * ar[2][2] + he.z * ar[0][2], t[1] * m.c[1][0] - t[0] * m.c[1][1])
+    XPBD_CROSS_AXIS(a[0] * ar[2][1] + a[1] * ar[2][0], he.x * ar[1][2] + he.y * ar[0][2], t[1] * m.c[2][0] - t[0] * m.c[2][1])
+    #undef XPBD_CROSS_AXIS
+    return true;
+}
+
+// The leaf's rotation q and position p relative to the soft body: the collider body's centre of
+// mass transform com = (inverse soft body transform * body transform), split into quaternion and
+// translation as SoftBodyMotionProperties.cpp:174-188 passes it to CollectTransformedShapes, then
+// the part's placement steps (CompoundShapeVisitors.h:258-272, RotatedTranslatedShape.cpp:169-176,
+// OffsetCenterOfMassShape.cpp:135-142).  With `cull`, false when a compound's bounds test leaves
+// the part out (the soft body's local bounds [lmin, lmax]).
+template <bool EXT>
+__device__ __forceinline__ bool part_pose(const ColliderDev &c, const M44 &rel, Q4 &q, V3 &p, bool cull, V3 lmin, V3 lmax)
+{
+    float qf[4];
+    get_quaternion(rel, qf);
+    q = q4(qf[0], qf[1], qf[2], qf[3]);
+    p = col3(rel, 3);
+    if constexpr (EXT)
+        for (uint32_t k = 0; k < c.num_ops; ++k)
+        {
+            const PartOpDev &op = c.ops[k];
+            switch (op.kind)
+            {
+            case XPBD_OP_CULL:
+                if (cull && !box_overlaps(q, p, lmin, lmax, op.v))
+                    return false;
+                break;
+            case XPBD_OP_TRANSLATE:
+                p = p + quat_rotate(q, v3(op.v[0], op.v[1], op.v[2]));
+                break;
+            case XPBD_OP_ROTATE:
+                q = quat_mul(q, q4(op.v[0], op.v[1], op.v[2], op.v[3]));
+                break;
+            default:    // XPBD_OP_OFFSET
+                p = p - quat_rotate(q, v3(op.v[0], op.v[1], op.v[2]));
+                break;
+            }
+        }
+    return true;
+}
+
+__device__ __forceinline__ M44 collider_relative(const ColliderDev &c, const M44 &inv_com)
+{
+    return mul(inv_com, rotation_translation(c.rot[0], c.rot[1], c.rot[2], c.rot[3], v3(c.pos[0], c.pos[1], c.pos[2])));
+}
+
+// Whether the soft body collects leaf c of an overlapping body this step: always for a simple
+// body; for a compound's leaf, when every bounds test on its way passes.
+template <bool EXT>
+__device__ __forceinline__ bool part_collected(const ColliderDev &c, const M44 &inv_com, V3 lmin, V3 lmax)
+{
+    if constexpr (EXT)
+    {
+        if (c.num_ops == 0)
+            return true;
+        Q4 q;
+        V3 p;
+        return part_pose<EXT>(c, collider_relative(c, inv_com), q, p, true, lmin, lmax);
+    }
+    return true;
+}
+
 // Leaf of collider c in the soft body's local space (SoftBodyMotionProperties.cpp:174-186:
 // com = (inverse soft body transform * collider transform), then the leaf
-// transform is rebuilt from com's quaternion and translation).  A dynamic collider
-// also initialises its entry of the body's DynStateDev row.
+// transform is rebuilt from com's quaternion and translation, after a compound's placement
+// steps).  A dynamic collider also initialises its entry of the body's DynStateDev row.
+template <bool EXT>
 __device__ __forceinline__ void make_leaf(const ColliderDev &c, const M44 &inv_com, float body_friction, float body_restitution, Leaf &out,
                                           DynStateDev *dyn_row, float4 pair = make_float4(1.0f, 1.0f, 1.0f, 1.0f))
 {
-    M44 ccom = rotation_translation(c.rot[0], c.rot[1], c.rot[2], c.rot[3], v3(c.pos[0], c.pos[1], c.pos[2]));
-    M44 rel = mul(inv_com, ccom);
-    float q[4];
-    get_quaternion(rel, q);
-    M44 t = rotation_translation(q[0], q[1], q[2], q[3], col3(rel, 3));
+    const M44 rel = collider_relative(c, inv_com);
+    Q4 q;
+    V3 pos;
+    part_pose<EXT>(c, rel, q, pos, false, v3(0.0f, 0.0f, 0.0f), v3(0.0f, 0.0f, 0.0f));
+    M44 t = rotation_translation(q.x, q.y, q.z, q.w, pos);
     out.t = t;
     out.type = c.type;
     out.dyn = -1;

@@ -197,13 +197,15 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                     wmin = wmin + vmin(a, b);
                     wmax = wmax + vmax(a, b);
                 }
+                const M44 inv_com = inversed_rotation_translation(com);
                 for (uint32_t ci = tid; ci < num_colliders; ci += kBlock)
                 {
                     const ColliderDev &cd = colliders[ci];
                     // AABox::Overlaps (Geometry/AABox.h:164-167)
                     bool overlap = !(wmin.x > cd.bounds[3] || wmin.y > cd.bounds[4] || wmin.z > cd.bounds[5]
                                      || wmax.x < cd.bounds[0] || wmax.y < cd.bounds[1] || wmax.z < cd.bounds[2]);
-                    slots[ci] = overlap && !pair_skipped(pair_settings(B.pair, ci), cd.sensor >= 0, cd.motion == 2) ? 1 : 0;
+                    slots[ci] = overlap && !pair_skipped(pair_settings(B.pair, cd.owner), cd.sensor >= 0, cd.motion == 2)
+                                && part_collected<EXT>(cd, inv_com, lmin, lmax) ? 1 : 0;
                 }
                 __syncthreads();
                 if (tid == 0)
@@ -237,12 +239,12 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                     sh_num_dyn_leaves = has_dynamic ? nd : 0u;
                 }
                 __syncthreads();
-                const M44 inv_com = inversed_rotation_translation(com);
                 for (uint32_t ci = tid; ci < num_colliders; ci += kBlock)
                     if (slots[ci] >= 0)
                     {
-                        make_leaf(colliders[ci], inv_com, B.friction, B.restitution, leaves[slots[ci]], dyn, pair_settings(B.pair, ci));
-                        leaves[slots[ci]].collider = (int32_t)ci;
+                        const ColliderDev &cd = colliders[ci];
+                        make_leaf<EXT>(cd, inv_com, B.friction, B.restitution, leaves[slots[ci]], dyn, pair_settings(B.pair, cd.owner));
+                        leaves[slots[ci]].collider = cd.owner;
                     }
                 // dynamic states are read by the in-order contact pass of another wavefront
                 if (has_dynamic)
@@ -267,7 +269,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                     if (slots[ci] == -2)
                     {
                         if (tid == 0)
-                            make_leaf(colliders[ci], inv_com_s, B.friction, B.restitution, sh_sensor_leaf, nullptr);
+                            make_leaf<EXT>(colliders[ci], inv_com_s, B.friction, B.restitution, sh_sensor_leaf, nullptr);
                         __syncthreads();
                         for (uint32_t i = tid; i < nv; i += kBlock)
                             largest = smax(largest, sensor_penetration<EXT>(sh_sensor_leaf, P[i]));
@@ -280,7 +282,8 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                     {
                         for (int w = 1; w < kBlock / kWave; ++w)
                             largest = smax(largest, red[w * 32]);
-                        B.sensor_hits[rank] = largest > 0.0f ? 1 : 0;
+                        // a compound sensor is hit when one of its leaves is
+                        B.sensor_hits[rank] = largest > 0.0f || (!colliders[ci].first && B.sensor_hits[rank]) ? 1 : 0;
                     }
                     __syncthreads();
                 }
@@ -470,7 +473,7 @@ __global__ __launch_bounds__(kBlock) void update_kernel(
                                 V3 n = plane.n;
                                 x = x + n * projected_distance;
                                 if (cs.dyn >= 0)
-                                    dynamic_contact(dyn[cs.dyn], col3(cs.t, 3), x, v, prev_v, n, projected_distance, p.w,
+                                    dynamic_contact(dyn[cs.dyn], ld3(dyn[cs.dyn].com), x, v, prev_v, n, projected_distance, p.w,
                                                     cs.friction, cs.restitution, dt_sub, restitution_threshold,
                                                     cs.inv_mass_scale1);
                                 else if (cs.inv_mass_scale1 > 0.0f)

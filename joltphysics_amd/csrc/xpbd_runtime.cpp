@@ -215,16 +215,18 @@ struct xpbd_context
     DevBuf<uint32_t> d_list;
     bool tables_dirty = true;
     bool states_on_device = false;
-    std::vector<ColliderDev> colliders;
-    std::vector<uint32_t> collider_sig;    // per collider: type | motion << 8 | sensor << 16, user id
+    std::vector<ColliderDev> colliders;     // leaf shapes of the collider bodies, body by body
+    std::vector<uint32_t> collider_first;  // per collider body: its first leaf in `colliders`
+    std::vector<uint32_t> collider_sig;    // per collider body: type | motion << 8 | sensor << 16 | parts << 17, user id
     DevBuf<ColliderDev> d_colliders;
+    DevBuf<PartOpDev> d_part_ops;          // placement steps of compound leaves
     DevBuf<float4> d_hull_vec;      // convex hulls: planes, then points, per hull
     DevBuf<uint32_t> d_hull_idx;    // convex hulls: face first, face count, vertex index, per hull
     DevBuf<HullDev> d_hulls;
     DevBuf<TreeNodeDev> d_tree_nodes;   // triangle trees: nodes of every tree, references rebased
     DevBuf<float> d_tree_tris;          // triangle trees: 9 floats per triangle, every tree
     DevBuf<TreeDev> d_trees;
-    bool ext_colliders = false;         // a triangle tree or a scaled convex hull among the colliders
+    bool ext_colliders = false;         // a triangle tree, a scaled convex hull or a compound leaf among the colliders
     // skinned vertex states of all skinned bodies, 4 float4 per vertex: previous and current
     // skinned position, normal, {max distance multiplier, back stop radius, enabled, previous
     // position valid}; kept on the host, uploaded before an update when changed
@@ -1940,7 +1942,9 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
     if (!ctx || (n && !c))
         return Fail(XPBD_E_INVALID, "null argument");
     HIP_TRY(hipSetDevice(ctx->device));
-    std::vector<ColliderDev> v(n);
+    std::vector<ColliderDev> v;
+    v.reserve(n);
+    std::vector<uint32_t> first(n);
     uint32_t num_dynamic = 0, num_sensors = 0;
     // convex hull tables: validated here, flattened into two pools, pointers set after upload
     std::vector<float4> hull_vec;
@@ -1952,24 +1956,47 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
     std::vector<float> tree_tris;
     struct TreeOffsets { uint32_t nodes, tris, root, collider, flags; };
     std::vector<TreeOffsets> tree_offsets;
-    for (uint32_t i = 0; i < n; ++i)
-    {
-        if (c[i].type < XPBD_SHAPE_PLANE || c[i].type > XPBD_SHAPE_MESH)
-            return Fail(XPBD_E_UNSUPPORTED, "collider shape type not supported (plane, sphere, box, capsule, cylinder, "
-                                            "tapered capsule, convex hull, tapered cylinder, mesh only)");
-        const bool unit = c[i].scale[0] == 0.0f && c[i].scale[1] == 0.0f && c[i].scale[2] == 0.0f;
-        const float sc[3] = { unit ? 1.0f : c[i].scale[0], unit ? 1.0f : c[i].scale[1], unit ? 1.0f : c[i].scale[2] };
+    // compound leaves' placement steps, one pool
+    std::vector<PartOpDev> op_pool;
+    std::vector<uint32_t> op_offset;        // per leaf
+    // one leaf: the shape of a simple body, or a part of a compound one
+    struct Shape { int32_t type; const float *params, *scale; const xpbd_convex_hull *hull; const xpbd_triangle_tree *tree;
+                   uint32_t num_ops; const xpbd_part_op *ops; };
+    auto add_leaf = [&](const ColliderDev &body, const Shape &sh, bool is_first, std::string &err) -> int {
+        const uint32_t leaf = (uint32_t)v.size();
+        if (sh.type < XPBD_SHAPE_PLANE || sh.type > XPBD_SHAPE_MESH)
+        {
+            err = "collider shape type not supported (plane, sphere, box, capsule, cylinder, tapered capsule, convex hull, "
+                  "tapered cylinder, mesh only)";
+            return XPBD_E_UNSUPPORTED;
+        }
+        const bool unit = sh.scale[0] == 0.0f && sh.scale[1] == 0.0f && sh.scale[2] == 0.0f;
+        const float sc[3] = { unit ? 1.0f : sh.scale[0], unit ? 1.0f : sh.scale[1], unit ? 1.0f : sh.scale[2] };
         for (int k = 0; k < 3; ++k)
             if (!(std::fabs(sc[k]) >= 1.0e-6f) || !std::isfinite(sc[k]))
-                return Fail(XPBD_E_INVALID, "collider scale components must be finite and at least 1e-6 in magnitude");
-        if (c[i].type == XPBD_SHAPE_MESH)
+            {
+                err = "collider scale components must be finite and at least 1e-6 in magnitude";
+                return XPBD_E_INVALID;
+            }
+        if (sh.type == XPBD_SHAPE_MESH)
         {
-            const xpbd_triangle_tree *t = c[i].tree;
+            const xpbd_triangle_tree *t = sh.tree;
             if (!t || !t->triangles || (t->num_nodes && !t->nodes))
-                return Fail(XPBD_E_INVALID, "mesh collider without tree data");
-            std::string err;
-            if (!ValidateTree(*t, err))
-                return Fail(XPBD_E_INVALID, "mesh collider tree: " + err);
+            {
+                err = "mesh collider without tree data";
+                return XPBD_E_INVALID;
+            }
+            std::string terr;
+            if (!ValidateTree(*t, terr))
+            {
+                err = "mesh collider tree: " + terr;
+                return XPBD_E_INVALID;
+            }
+            if (t->flags & ~XPBD_TREE_HEIGHT_FIELD)
+            {
+                err = "mesh collider tree: unknown flags";
+                return XPBD_E_INVALID;
+            }
             TreeOffsets o;
             o.nodes = (uint32_t)tree_nodes.size();
             o.tris = (uint32_t)tree_tris.size() / 9u;
@@ -1984,28 +2011,38 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
             }
             tree_tris.insert(tree_tris.end(), t->triangles, t->triangles + 9 * (size_t)t->num_triangles);
             o.root = RebaseTreeRef(t->root, o.nodes, o.tris);
-            o.collider = i;
-            if (t->flags & ~XPBD_TREE_HEIGHT_FIELD)
-                return Fail(XPBD_E_INVALID, "mesh collider tree: unknown flags");
+            o.collider = leaf;
             o.flags = t->flags;
             tree_offsets.push_back(o);
         }
-        if (c[i].type == XPBD_SHAPE_CONVEX_HULL)
+        if (sh.type == XPBD_SHAPE_CONVEX_HULL)
         {
-            const xpbd_convex_hull *h = c[i].hull;
+            const xpbd_convex_hull *h = sh.hull;
             if (!h || !h->points || !h->planes || !h->face_first || !h->face_count || !h->vertex_index)
-                return Fail(XPBD_E_INVALID, "convex hull collider without hull data");
+            {
+                err = "convex hull collider without hull data";
+                return XPBD_E_INVALID;
+            }
             if (h->num_faces == 0 || h->num_points == 0)
-                return Fail(XPBD_E_INVALID, "convex hull without faces or points");
+            {
+                err = "convex hull without faces or points";
+                return XPBD_E_INVALID;
+            }
             for (uint32_t f = 0; f < h->num_faces; ++f)
             {
                 const uint64_t end = (uint64_t)h->face_first[f] + h->face_count[f];
                 if (h->face_count[f] < 2 || end > h->num_vertex_index)
-                    return Fail(XPBD_E_INVALID, "convex hull face outside the vertex index table");
+                {
+                    err = "convex hull face outside the vertex index table";
+                    return XPBD_E_INVALID;
+                }
             }
             for (uint32_t k = 0; k < h->num_vertex_index; ++k)
                 if (h->vertex_index[k] >= h->num_points)
-                    return Fail(XPBD_E_INVALID, "convex hull vertex index out of range");
+                {
+                    err = "convex hull vertex index out of range";
+                    return XPBD_E_INVALID;
+                }
             HullOffsets o;
             o.planes = (uint32_t)hull_vec.size();
             for (uint32_t f = 0; f < h->num_faces; ++f)
@@ -2020,36 +2057,87 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
             o.index = (uint32_t)hull_idx.size();
             hull_idx.insert(hull_idx.end(), h->vertex_index, h->vertex_index + h->num_vertex_index);
             o.num_faces = h->num_faces;
-            o.collider = i;
+            o.collider = leaf;
             hull_offsets.push_back(o);
         }
+        if (sh.num_ops > XPBD_MAX_PART_OPS || (sh.num_ops && !sh.ops))
+        {
+            err = "collider part: more than XPBD_MAX_PART_OPS placement steps, or none given";
+            return XPBD_E_INVALID;
+        }
+        op_offset.push_back((uint32_t)op_pool.size());
+        for (uint32_t k = 0; k < sh.num_ops; ++k)
+        {
+            const xpbd_part_op &op = sh.ops[k];
+            if (op.kind < XPBD_OP_CULL || op.kind > XPBD_OP_OFFSET)
+            {
+                err = "collider part: unknown placement step";
+                return XPBD_E_INVALID;
+            }
+            PartOpDev d;
+            static_assert(sizeof(PartOpDev) == sizeof(xpbd_part_op), "placement step layouts");
+            memcpy(&d, &op, sizeof(d));
+            op_pool.push_back(d);
+        }
+        ColliderDev d = body;
+        d.type = sh.type;
+        d.hull = nullptr;
+        d.tree = nullptr;
+        memcpy(d.scale, sc, sizeof(d.scale));
+        memcpy(d.params, sh.params, sizeof(d.params));
+        d.first = is_first ? 1 : 0;
+        d.num_ops = sh.num_ops;
+        d.ops = nullptr;
+        v.push_back(d);
+        return XPBD_OK;
+    };
+    for (uint32_t i = 0; i < n; ++i)
+    {
         if (c[i].motion_type < 0 || c[i].motion_type > 2)
             return Fail(XPBD_E_INVALID, "collider motion type must be 0 (static), 1 (kinematic) or 2 (dynamic)");
-        v[i].type = c[i].type;
-        v[i].hull = nullptr;
-        v[i].tree = nullptr;
-        memcpy(v[i].scale, sc, sizeof(v[i].scale));
-        v[i].motion = c[i].motion_type;
-        v[i].dyn = -1;
-        v[i].sensor = c[i].is_sensor ? (int32_t)num_sensors++ : -1;
+        if (c[i].num_parts && !c[i].parts)
+            return Fail(XPBD_E_INVALID, "compound collider without parts");
+        // the body's own fields, shared by its leaves
+        ColliderDev body{};
+        body.motion = c[i].motion_type;
+        body.dyn = -1;
+        body.sensor = c[i].is_sensor ? (int32_t)num_sensors++ : -1;
         if (c[i].motion_type == 2)
         {
-            v[i].dyn = (int32_t)num_dynamic++;
-            v[i].inv_mass = c[i].inv_mass;
-            memcpy(v[i].inv_inertia_diag, c[i].inv_inertia_diagonal, sizeof(v[i].inv_inertia_diag));
-            memcpy(v[i].inertia_rot, c[i].inertia_rotation, sizeof(v[i].inertia_rot));
-            memcpy(v[i].lin_vel, c[i].linear_velocity, sizeof(v[i].lin_vel));
-            memcpy(v[i].ang_vel, c[i].angular_velocity, sizeof(v[i].ang_vel));
+            body.dyn = (int32_t)num_dynamic++;
+            body.inv_mass = c[i].inv_mass;
+            memcpy(body.inv_inertia_diag, c[i].inv_inertia_diagonal, sizeof(body.inv_inertia_diag));
+            memcpy(body.inertia_rot, c[i].inertia_rotation, sizeof(body.inertia_rot));
+            memcpy(body.lin_vel, c[i].linear_velocity, sizeof(body.lin_vel));
+            memcpy(body.ang_vel, c[i].angular_velocity, sizeof(body.ang_vel));
         }
-        memcpy(v[i].params, c[i].params, sizeof(v[i].params));
-        memcpy(v[i].pos, c[i].position, sizeof(v[i].pos));
-        memcpy(v[i].rot, c[i].rotation, sizeof(v[i].rot));
-        v[i].friction = c[i].friction;
-        v[i].restitution = c[i].restitution;
-        memcpy(v[i].bounds, c[i].world_bounds, sizeof(v[i].bounds));
+        memcpy(body.pos, c[i].position, sizeof(body.pos));
+        memcpy(body.rot, c[i].rotation, sizeof(body.rot));
+        body.friction = c[i].friction;
+        body.restitution = c[i].restitution;
+        memcpy(body.bounds, c[i].world_bounds, sizeof(body.bounds));
+        body.owner = (int32_t)i;
+        first[i] = (uint32_t)v.size();
+        std::string err;
+        if (c[i].num_parts == 0)
+        {
+            const Shape sh{ c[i].type, c[i].params, c[i].scale, c[i].hull, c[i].tree, 0u, nullptr };
+            if (int rc = add_leaf(body, sh, true, err))
+                return Fail(rc, err);
+        }
+        else
+            for (uint32_t k = 0; k < c[i].num_parts; ++k)
+            {
+                const xpbd_collider_part &pt = c[i].parts[k];
+                const Shape sh{ pt.type, pt.params, pt.scale, pt.hull, pt.tree, pt.num_ops, pt.ops };
+                if (int rc = add_leaf(body, sh, k == 0, err))
+                    return Fail(rc, err);
+            }
     }
     if (tree_nodes.size() >= (1u << 30) || tree_tris.size() / 9u >= (1u << 24))
         return Fail(XPBD_E_CAPACITY, "the mesh colliders hold more than 2^30 nodes or 2^24 triangles together");
+    if (v.size() >= (1u << 24))
+        return Fail(XPBD_E_CAPACITY, "more than 2^24 collider leaf shapes");
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     HIP_TRY(ctx->d_hull_vec.upload(hull_vec));
     HIP_TRY(ctx->d_hull_idx.upload(hull_idx));
@@ -2080,22 +2168,27 @@ int xpbd_set_colliders(xpbd_context *ctx, const xpbd_collider_desc *c, uint32_t 
     HIP_TRY(ctx->d_trees.upload(trees));
     for (size_t k = 0; k < tree_offsets.size(); ++k)
         v[tree_offsets[k].collider].tree = ctx->d_trees.p + k;
+    HIP_TRY(ctx->d_part_ops.upload(op_pool));
+    for (size_t k = 0; k < v.size(); ++k)
+        v[k].ops = v[k].num_ops ? ctx->d_part_ops.p + op_offset[k] : nullptr;
     if (v.size() != ctx->colliders.size() || (num_dynamic > 0) != (ctx->num_dynamic > 0) || (num_sensors > 0) != (ctx->num_sensors > 0))
         ctx->layout_dirty = true;
     ctx->colliders = v;
-    // per collider: shape type, motion type, sensor flag and the caller's identifier; a
-    // contact settings table holds for the list it was given for
+    ctx->collider_first = first;
+    // per collider body: shape type, motion type, sensor flag, compound flag and the caller's
+    // identifier; a contact settings table holds for the list it was given for
     ctx->collider_sig.resize(2 * (size_t)n);
     for (uint32_t i = 0; i < n; ++i)
     {
-        ctx->collider_sig[2 * i] = (uint32_t)c[i].type | ((uint32_t)c[i].motion_type << 8) | (c[i].is_sensor ? 1u << 16 : 0u);
+        ctx->collider_sig[2 * i] = (uint32_t)(c[i].num_parts ? 0 : c[i].type) | ((uint32_t)c[i].motion_type << 8)
+                                   | (c[i].is_sensor ? 1u << 16 : 0u) | (std::min(c[i].num_parts, 0x7fffu) << 17);
         ctx->collider_sig[2 * i + 1] = c[i].user_id;
     }
     ctx->num_dynamic = num_dynamic;
     ctx->num_sensors = num_sensors;
     ctx->ext_colliders = false;
     for (const ColliderDev &d : v)
-        ctx->ext_colliders |= d.type == XPBD_SHAPE_MESH
+        ctx->ext_colliders |= d.type == XPBD_SHAPE_MESH || d.num_ops > 0
                               || (d.type == XPBD_SHAPE_CONVEX_HULL
                                   && (d.scale[0] - 1.0f) * (d.scale[0] - 1.0f) + (d.scale[1] - 1.0f) * (d.scale[1] - 1.0f)
                                          + (d.scale[2] - 1.0f) * (d.scale[2] - 1.0f) > 1.0e-8f);
@@ -2733,7 +2826,7 @@ int xpbd_body_sensor_contacts(xpbd_context *ctx, int32_t body, uint32_t num_coll
 {
     if (!ctx || body < 0 || body >= (int32_t)ctx->bodies.size() || !hits)
         return Fail(XPBD_E_INVALID, "invalid body or output");
-    if (num_colliders != ctx->colliders.size())
+    if (num_colliders != ctx->collider_first.size())
         return Fail(XPBD_E_INVALID, "num_colliders differs from the last xpbd_set_colliders");
     int rc = xpbd_synchronize(ctx);
     if (rc != XPBD_OK)
@@ -2744,7 +2837,7 @@ int xpbd_body_sensor_contacts(xpbd_context *ctx, int32_t body, uint32_t num_coll
                           hipMemcpyDeviceToHost));
     for (uint32_t c = 0; c < num_colliders; ++c)
     {
-        const int32_t k = ctx->colliders[c].sensor;
+        const int32_t k = ctx->colliders[ctx->collider_first[c]].sensor;
         hits[c] = k >= 0 && (uint32_t)k < row.size() ? row[k] : 0;
     }
     return XPBD_OK;
@@ -2762,7 +2855,7 @@ int xpbd_body_set_contact_settings(xpbd_context *ctx, int32_t body, uint32_t num
         ctx->tables_dirty = true;
         return XPBD_OK;
     }
-    if (num_colliders != ctx->colliders.size())
+    if (num_colliders != ctx->collider_first.size())
         return Fail(XPBD_E_INVALID, "num_colliders differs from the last xpbd_set_colliders");
     for (uint32_t c = 0; c < num_colliders; ++c)
     {
@@ -2813,7 +2906,7 @@ int xpbd_body_velocity_deltas(xpbd_context *ctx, int32_t body, uint32_t num_coll
 {
     if (!ctx || body < 0 || body >= (int32_t)ctx->bodies.size())
         return Fail(XPBD_E_INVALID, "invalid body");
-    if (num_colliders != ctx->colliders.size())
+    if (num_colliders != ctx->collider_first.size())
         return Fail(XPBD_E_INVALID, "num_colliders differs from the last xpbd_set_colliders");
     int rc = xpbd_synchronize(ctx);
     if (rc != XPBD_OK)
@@ -2824,7 +2917,7 @@ int xpbd_body_velocity_deltas(xpbd_context *ctx, int32_t body, uint32_t num_coll
                           hipMemcpyDeviceToHost));
     for (uint32_t c = 0; c < num_colliders; ++c)
     {
-        const int32_t k = ctx->colliders[c].dyn;
+        const int32_t k = ctx->colliders[ctx->collider_first[c]].dyn;
         const bool have = k >= 0 && (uint32_t)k < row.size();
         for (int j = 0; j < 3; ++j)
         {

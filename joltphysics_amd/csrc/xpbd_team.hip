@@ -183,13 +183,15 @@ __global__ __launch_bounds__(kTeamBlock) void team_kernel(
                     wmin = wmin + vmin(a, b);
                     wmax = wmax + vmax(a, b);
                 }
+                const M44 inv_com = inversed_rotation_translation(com);
                 for (uint32_t ci = tid; ci < num_colliders; ci += kTeamBlock)
                 {
                     const ColliderDev &cd = colliders[ci];
                     // AABox::Overlaps (Geometry/AABox.h:164-167)
                     bool overlap = !(wmin.x > cd.bounds[3] || wmin.y > cd.bounds[4] || wmin.z > cd.bounds[5]
                                      || wmax.x < cd.bounds[0] || wmax.y < cd.bounds[1] || wmax.z < cd.bounds[2]);
-                    slots[ci] = overlap && !pair_skipped(pair_settings(B.pair, ci), false, false) ? 1 : 0;
+                    slots[ci] = overlap && !pair_skipped(pair_settings(B.pair, cd.owner), false, false)
+                                && part_collected<EXT>(cd, inv_com, lmin, lmax) ? 1 : 0;
                 }
                 __syncthreads();
                 if (tid == 0)
@@ -203,12 +205,12 @@ __global__ __launch_bounds__(kTeamBlock) void team_kernel(
                 // the colliding list in LDS when it fits, else in this member's slice of the
                 // team's global leaf table (a body among more colliders than LDS holds)
                 leaves = sh_num_leaves <= max_leaves ? lds_leaves : TD.leaves + (size_t)member * num_colliders;
-                const M44 inv_com = inversed_rotation_translation(com);
                 for (uint32_t ci = tid; ci < num_colliders; ci += kTeamBlock)
                     if (slots[ci] >= 0)
                     {
-                        make_leaf(colliders[ci], inv_com, B.friction, B.restitution, leaves[slots[ci]], nullptr, pair_settings(B.pair, ci));
-                        leaves[slots[ci]].collider = (int32_t)ci;
+                        const ColliderDev &cd = colliders[ci];
+                        make_leaf<EXT>(cd, inv_com, B.friction, B.restitution, leaves[slots[ci]], nullptr, pair_settings(B.pair, cd.owner));
+                        leaves[slots[ci]].collider = cd.owner;
                     }
                 if (sh_num_leaves <= max_leaves)
                     __syncthreads();

@@ -163,6 +163,16 @@ struct TreeDev
     uint32_t height_field;      // children with unscaled min y > max y are never visited
 };
 
+// One placement step of a compound's leaf (xpbd_part_op)
+struct PartOpDev
+{
+    uint32_t kind;
+    float v[6];
+};
+
+// One leaf shape of a collider body: a simple body has one, a compound or decorated body one per
+// leaf shape (consecutive, in the order the reference collects them), each holding the body's
+// pose, material and motion.
 struct ColliderDev
 {
     int32_t type;
@@ -182,6 +192,10 @@ struct ColliderDev
     float inertia_rot[4];       // MotionProperties::GetInertiaRotation
     float lin_vel[3], ang_vel[3];   // world space
     int32_t sensor;             // rank among the sensor colliders, -1 for a colliding body
+    int32_t owner;              // index of the collider body in the caller's list
+    int32_t first;              // 1 for the body's first leaf
+    uint32_t num_ops;           // placement steps from the body's centre of mass to the leaf
+    const PartOpDev *ops;
 };
 
 // State of one dynamic collider as one soft body sees it during a collision step
@@ -193,6 +207,7 @@ struct DynStateDev
     float inv_mass;
     float lin[3], ang[3];       // mLinearVelocity / mAngularVelocity (soft-body local)
     float orig_lin[3], orig_ang[3];
+    float com[3];               // mCenterOfMassTransform's translation (soft-body local)
     float acc_lin[3], acc_ang[3];   // world-space velocity changes, summed over the call's steps
     int32_t updated;            // mUpdateVelocities of the current step
     int32_t any_updated;        // some step of the call set mUpdateVelocities

@@ -57,11 +57,16 @@ SHAPE_PLANE, SHAPE_SPHERE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_CYLINDER, SHAPE_TAPER
 SHAPE_CONVEX_HULL = 6
 SHAPE_TAPERED_CYLINDER = 7      # device params {top, bottom, top radius, bottom radius}
 SHAPE_MESH = 8                  # triangle tree (MeshShape or TriangleShape)
+SHAPE_COMPOUND = 100            # compound / decorated body: `shape` builds it, `parts` describe its leaves
 
 # xpbd_tree_node: the four children's bounds per axis and their references
 TREE_NODE = np.dtype([("min", "<f4", (3, 4)), ("max", "<f4", (3, 4)), ("child", "<u4", 4)])
 TREE_NODE_REF, TREE_LEAF_REF, TREE_EMPTY_REF = 0x00000000, 0x40000000, 0x80000000
 TREE_HEIGHT_FIELD = 1
+
+# placement steps of a compound's leaf (xpbd_part_op)
+PART_OP = np.dtype([("kind", "<u4"), ("v", "<f4", 6)])
+OP_CULL, OP_TRANSLATE, OP_ROTATE, OP_OFFSET = 1, 2, 3, 4
 # EMotionType (MotionType.h:10-15)
 MOTION_STATIC, MOTION_KINEMATIC, MOTION_DYNAMIC = 0, 1, 2
 
@@ -188,6 +193,69 @@ class TriangleTree:
 
 
 @dataclass
+class ShapeSpec:
+    """A shape of a compound or decorated collider, as the reference's shape classes build it:
+    kind "basic" (`type`, `params` as Collider takes them), "hull" (`params`: points xyz),
+    "triangle" (`params`: three vertices xyz),
+    "scaled" (`inner`, `vec` = scale), "rotated_translated" (`inner`, `vec` = position, `rot`),
+    "offset_com" (`inner`, `vec` = offset), "static_compound" / "mutable_compound"
+    (`children`: [(ShapeSpec, position, rotation)])."""
+
+    kind: str
+    type: int = 0
+    params: tuple = ()
+    inner: "ShapeSpec | None" = None
+    vec: tuple = ()
+    rot: tuple = (0.0, 0.0, 0.0, 1.0)
+    children: list = field(default_factory=list)
+
+    @staticmethod
+    def basic(type, *params):
+        return ShapeSpec("basic", type=type, params=tuple(params))
+
+    @staticmethod
+    def hull(points):
+        return ShapeSpec("hull", params=tuple(float(x) for x in np.asarray(points, np.float32).ravel()))
+
+    @staticmethod
+    def triangle(v9):
+        return ShapeSpec("triangle", params=tuple(float(x) for x in np.asarray(v9, np.float32).ravel()))
+
+    @staticmethod
+    def scaled(inner, scale):
+        return ShapeSpec("scaled", inner=inner, vec=tuple(scale))
+
+    @staticmethod
+    def rotated_translated(inner, position, rotation):
+        return ShapeSpec("rotated_translated", inner=inner, vec=tuple(position), rot=tuple(rotation))
+
+    @staticmethod
+    def offset_com(inner, offset):
+        return ShapeSpec("offset_com", inner=inner, vec=tuple(offset))
+
+    @staticmethod
+    def static_compound(children):
+        return ShapeSpec("static_compound", children=list(children))
+
+    @staticmethod
+    def mutable_compound(children):
+        return ShapeSpec("mutable_compound", children=list(children))
+
+
+@dataclass
+class ColliderPart:
+    """One leaf of a compound collider as the device takes it (xpbd_collider_part): shape type,
+    params (8), scale, placement steps (PART_OP), hull or tree."""
+
+    type: int
+    params: tuple
+    scale: tuple = (1.0, 1.0, 1.0)
+    ops: np.ndarray = field(default_factory=lambda: np.zeros(0, PART_OP))
+    hull: ConvexHull | None = None
+    tree: TriangleTree | None = None
+
+
+@dataclass
 class Collider:
     """A static collider body: shape, world transform (body position and rotation),
     material, world bounds.
@@ -231,6 +299,10 @@ class Collider:
     triangle: tuple | None = None
     height_field: tuple | None = None
     tree: TriangleTree | None = None
+    # SHAPE_COMPOUND: the shape to build (ShapeSpec) and its leaves as the device takes them
+    # (ColliderPart list, taken from the reference's shape)
+    shape: ShapeSpec | None = None
+    parts: list | None = None
 
     def param_array(self):
         if self.type == SHAPE_CONVEX_HULL:

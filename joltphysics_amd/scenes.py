@@ -18,8 +18,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .records import (BEND_DIHEDRAL, BEND_NONE, EDGE, LRA, LRA_NONE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_CYLINDER,
-                      SHAPE_CONVEX_HULL, SHAPE_MESH, SHAPE_PLANE, SHAPE_SPHERE, SHAPE_TAPERED_CAPSULE,
+from .records import (BEND_DIHEDRAL, BEND_NONE, EDGE, LRA, LRA_NONE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_COMPOUND,
+                      SHAPE_CYLINDER, SHAPE_CONVEX_HULL, SHAPE_MESH, SHAPE_PLANE, SHAPE_SPHERE, SHAPE_TAPERED_CAPSULE,
                       SHAPE_TAPERED_CYLINDER, VERTEX, VOLUME, BodyParams, Collider, tapered_capsule_device_params)
 
 # Build flags (same bit meaning on the product and oracle builders)
@@ -194,12 +194,43 @@ def _rot_matrix(q):
                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
 
 
+def spec_radius(spec):
+    """Radius of a ball around the shape's origin that holds a ShapeSpec (conservative)."""
+    if spec.kind == "basic":
+        t, p = spec.type, tuple(float(x) for x in spec.params)
+        if t == SHAPE_SPHERE:
+            return p[0]
+        if t == SHAPE_BOX:
+            return float(np.linalg.norm(p[:3]))
+        if t == SHAPE_CAPSULE:
+            return p[0] + p[1]
+        if t == SHAPE_CYLINDER:
+            return float(np.hypot(p[0], p[1]))
+        if t == SHAPE_TAPERED_CAPSULE:
+            return p[0] + 2.0 * max(p[1], p[2])
+        if t == SHAPE_TAPERED_CYLINDER:
+            return float(np.hypot(p[0], max(p[1], p[2])))
+        raise ValueError(f"shape type {t} in a compound")
+    if spec.kind in ("hull", "triangle"):
+        return float(np.linalg.norm(np.asarray(spec.params, np.float64).reshape(-1, 3), axis=1).max())
+    if spec.kind == "scaled":
+        return float(np.abs(spec.vec).max()) * spec_radius(spec.inner)
+    if spec.kind == "rotated_translated":
+        return float(np.linalg.norm(spec.vec)) + spec_radius(spec.inner)
+    if spec.kind == "offset_com":
+        return spec_radius(spec.inner)
+    return max(float(np.linalg.norm(pos)) + spec_radius(child) for child, pos, _ in spec.children)
+
+
 def collider_world_bounds(c: Collider):
-    """World-space AABB of a static collider body (conservative for planes, scaled shapes
-    and tapered cylinders)."""
+    """World-space AABB of a static collider body (conservative for planes, scaled shapes,
+    tapered cylinders and compounds)."""
     p = np.array(c.pos, np.float64)
     R = _rot_matrix(c.rot)
     scale = np.abs(np.array(c.scale, np.float64))
+    if c.type == SHAPE_COMPOUND:
+        r = spec_radius(c.shape)
+        return tuple(np.concatenate([p - r, p + r]).astype(np.float32))
     if c.type == SHAPE_MESH and c.height_field is not None:
         samples, _, _, off, sc = c.height_field
         n = np.asarray(samples).shape[0]

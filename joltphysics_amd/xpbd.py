@@ -17,8 +17,8 @@ import sys
 
 import numpy as np
 
-from .records import (BEND, EDGE, FACE, GROUP, INV_BIND, LRA, MOTION_DYNAMIC, ROD_BT, ROD_SS, SHAPE_CONVEX_HULL,
-                      SHAPE_MESH, SKINNED, TREE_NODE, VERTEX, VOLUME, BodyParams, SettingsArrays)
+from .records import (BEND, EDGE, FACE, GROUP, INV_BIND, LRA, MOTION_DYNAMIC, PART_OP, ROD_BT, ROD_SS, SHAPE_COMPOUND,
+                      SHAPE_CONVEX_HULL, SHAPE_MESH, SKINNED, TREE_NODE, VERTEX, VOLUME, BodyParams, SettingsArrays)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 # JOLT_XPBD_LIBRARY selects another build of the library (the sanitized one in the CPU tests)
@@ -66,6 +66,12 @@ class _TriangleTree(C.Structure):
                 ("num_triangles", C.c_uint32), ("triangles", C.c_void_p), ("flags", C.c_uint32)]
 
 
+class _ColliderPart(C.Structure):
+    _fields_ = [("type", C.c_int32), ("params", C.c_float * 8), ("scale", C.c_float * 3),
+                ("hull", C.POINTER(_ConvexHull)), ("tree", C.POINTER(_TriangleTree)),
+                ("num_ops", C.c_uint32), ("ops", C.c_void_p)]
+
+
 class _ColliderDesc(C.Structure):
     _fields_ = [("type", C.c_int32), ("params", C.c_float * 8), ("position", C.c_float * 3),
                 ("rotation", C.c_float * 4), ("friction", C.c_float), ("restitution", C.c_float),
@@ -73,7 +79,26 @@ class _ColliderDesc(C.Structure):
                 ("motion_type", C.c_int32), ("inv_mass", C.c_float), ("inv_inertia_diagonal", C.c_float * 3),
                 ("inertia_rotation", C.c_float * 4), ("linear_velocity", C.c_float * 3),
                 ("angular_velocity", C.c_float * 3), ("is_sensor", C.c_int32), ("user_id", C.c_uint32),
-                ("scale", C.c_float * 3), ("tree", C.POINTER(_TriangleTree))]
+                ("scale", C.c_float * 3), ("tree", C.POINTER(_TriangleTree)),
+                ("num_parts", C.c_uint32), ("parts", C.POINTER(_ColliderPart))]
+
+
+def _hull_struct(h, keep):
+    a = (np.ascontiguousarray(h.points, np.float32), np.ascontiguousarray(h.planes, np.float32),
+         np.ascontiguousarray(h.face_first, np.uint32), np.ascontiguousarray(h.face_count, np.uint32),
+         np.ascontiguousarray(h.vertex_index, np.uint32))
+    d = _ConvexHull(len(a[0]), a[0].ctypes.data, len(a[1]), a[1].ctypes.data, a[2].ctypes.data,
+                    a[3].ctypes.data, len(a[4]), a[4].ctypes.data)
+    keep.append((a, d))
+    return C.pointer(d)
+
+
+def _tree_struct(t, keep):
+    a = (np.ascontiguousarray(t.nodes, TREE_NODE), np.ascontiguousarray(t.triangles, np.float32).reshape(-1, 9))
+    d = _TriangleTree(int(t.root), len(a[0]), a[0].ctypes.data if len(a[0]) else None, len(a[1]),
+                      a[1].ctypes.data if len(a[1]) else None, int(t.flags))
+    keep.append((a, d))
+    return C.pointer(d)
 
 
 class _RolloutDesc(C.Structure):
@@ -345,32 +370,42 @@ class Context:
         arr = (_ColliderDesc * max(n, 1))()
         keep = []   # hull arrays stay alive until the call has copied them
         for i, c in enumerate(colliders):
-            arr[i].type = int(c.type)
-            if c.type == SHAPE_CONVEX_HULL:
-                if c.hull is None:
-                    raise XpbdError("convex hull collider without hull data")
-                h = c.hull
-                a = (np.ascontiguousarray(h.points, np.float32), np.ascontiguousarray(h.planes, np.float32),
-                     np.ascontiguousarray(h.face_first, np.uint32), np.ascontiguousarray(h.face_count, np.uint32),
-                     np.ascontiguousarray(h.vertex_index, np.uint32))
-                d = _ConvexHull(len(a[0]), a[0].ctypes.data, len(a[1]), a[1].ctypes.data, a[2].ctypes.data,
-                                a[3].ctypes.data, len(a[4]), a[4].ctypes.data)
-                keep.append((a, d))
-                arr[i].hull = C.pointer(d)
-            if c.type == SHAPE_MESH:
-                if c.tree is None:
-                    raise XpbdError("mesh collider without tree data")
-                t = c.tree
-                a = (np.ascontiguousarray(t.nodes, TREE_NODE), np.ascontiguousarray(t.triangles, np.float32).reshape(-1, 9))
-                d = _TriangleTree(int(t.root), len(a[0]), a[0].ctypes.data if len(a[0]) else None, len(a[1]),
-                                  a[1].ctypes.data if len(a[1]) else None, int(t.flags))
-                keep.append((a, d))
-                arr[i].tree = C.pointer(d)
-            for k in range(3):
-                arr[i].scale[k] = float(c.scale[k])
-            p = c.device_param_array()
-            for k in range(8):
-                arr[i].params[k] = float(p[k])
+            if c.type == SHAPE_COMPOUND:
+                if not c.parts:
+                    raise XpbdError("compound collider without parts (take them from the reference's shape)")
+                parts = (_ColliderPart * len(c.parts))()
+                for k, pt in enumerate(c.parts):
+                    parts[k].type = int(pt.type)
+                    for j in range(8):
+                        parts[k].params[j] = float(pt.params[j]) if j < len(pt.params) else 0.0
+                    for j in range(3):
+                        parts[k].scale[j] = float(pt.scale[j])
+                    if pt.type == SHAPE_CONVEX_HULL:
+                        parts[k].hull = _hull_struct(pt.hull, keep)
+                    if pt.type == SHAPE_MESH:
+                        parts[k].tree = _tree_struct(pt.tree, keep)
+                    ops = np.ascontiguousarray(pt.ops, PART_OP)
+                    keep.append(ops)
+                    parts[k].num_ops = len(ops)
+                    parts[k].ops = ops.ctypes.data if len(ops) else None
+                keep.append(parts)
+                arr[i].num_parts = len(c.parts)
+                arr[i].parts = parts
+            else:
+                arr[i].type = int(c.type)
+                if c.type == SHAPE_CONVEX_HULL:
+                    if c.hull is None:
+                        raise XpbdError("convex hull collider without hull data")
+                    arr[i].hull = _hull_struct(c.hull, keep)
+                if c.type == SHAPE_MESH:
+                    if c.tree is None:
+                        raise XpbdError("mesh collider without tree data")
+                    arr[i].tree = _tree_struct(c.tree, keep)
+                for k in range(3):
+                    arr[i].scale[k] = float(c.scale[k])
+                p = c.device_param_array()
+                for k in range(8):
+                    arr[i].params[k] = float(p[k])
             com = c.com_position()
             for k in range(3):
                 arr[i].position[k] = float(com[k])

@@ -198,6 +198,7 @@ struct World
 	std::vector<int> mColliderTypes;
 	std::unique_ptr<ContactRecorder> mRecorder;
 	Vec3 mNextScale = Vec3::sOne();					// ScaledShape for the next collider added
+	std::vector<Ref<Shape>> mShapes;				// shapes built by jo_shape_* for compound colliders
 };
 
 // The next collider's shape inside a ScaledShape of this scale (unless it is one)
@@ -669,6 +670,221 @@ static int AddStaticShape(World *w, Ref<Shape> shape, int type, const float *pos
 	return int(w->mColliders.size() - 1);
 }
 
+// ---- shapes built piece by piece (compound colliders): each jo_shape_* returns a handle into the
+// world's shape table, or -1
+static int KeepShape(World *w, Ref<Shape> shape)
+{
+	if (shape == nullptr)
+		return -1;
+	w->mShapes.push_back(shape);
+	return int(w->mShapes.size() - 1);
+}
+
+static const Shape *HeldShape(World *w, int handle)
+{
+	return handle >= 0 && size_t(handle) < w->mShapes.size() ? w->mShapes[handle].GetPtr() : nullptr;
+}
+
+// type and params as jo_world_add_collider takes them
+int jo_shape_basic(void *wp, int type, const float *params)
+{
+	return KeepShape((World *)wp, MakeShape(type, params));
+}
+
+// ConvexHullShapeSettings of `n` points (xyz), convex radius 0
+int jo_shape_hull(void *wp, const float *points, int n)
+{
+	std::vector<Vec3> pts;
+	for (int i = 0; i < n; ++i)
+		pts.push_back(Vec3(points[3 * i], points[3 * i + 1], points[3 * i + 2]));
+	ConvexHullShapeSettings hs(pts.data(), n, 0.0f);
+	Shape::ShapeResult r = hs.Create();
+	if (r.HasError() || r.Get()->GetSubType() != EShapeSubType::ConvexHull)
+		return -1;
+	return KeepShape((World *)wp, r.Get());
+}
+
+// TriangleShape (convex radius 0) of three vertices xyz
+int jo_shape_triangle(void *wp, const float *v9)
+{
+	return KeepShape((World *)wp, new TriangleShape(Vec3(v9[0], v9[1], v9[2]), Vec3(v9[3], v9[4], v9[5]), Vec3(v9[6], v9[7], v9[8])));
+}
+
+int jo_shape_scaled(void *wp, int inner, const float *scale3)
+{
+	World *w = (World *)wp;
+	const Shape *s = HeldShape(w, inner);
+	if (s == nullptr)
+		return -1;
+	return KeepShape(w, new ScaledShape(s, Vec3(scale3[0], scale3[1], scale3[2])));
+}
+
+int jo_shape_rotated_translated(void *wp, int inner, const float *pos3, const float *rot4)
+{
+	World *w = (World *)wp;
+	const Shape *s = HeldShape(w, inner);
+	if (s == nullptr)
+		return -1;
+	RotatedTranslatedShapeSettings rs(Vec3(pos3[0], pos3[1], pos3[2]), Quat(rot4[0], rot4[1], rot4[2], rot4[3]), s);
+	Shape::ShapeResult r = rs.Create();
+	return r.HasError() ? -1 : KeepShape(w, r.Get());
+}
+
+int jo_shape_offset_com(void *wp, int inner, const float *offset3)
+{
+	World *w = (World *)wp;
+	const Shape *s = HeldShape(w, inner);
+	if (s == nullptr)
+		return -1;
+	OffsetCenterOfMassShapeSettings os(Vec3(offset3[0], offset3[1], offset3[2]), s);
+	Shape::ShapeResult r = os.Create();
+	return r.HasError() ? -1 : KeepShape(w, r.Get());
+}
+
+// StaticCompoundShape (is_mutable 0) or MutableCompoundShape of n children: handles, positions
+// (xyz each) and rotations (xyzw each)
+int jo_shape_compound(void *wp, int is_mutable, int n, const int *children, const float *pos3n, const float *rot4n)
+{
+	World *w = (World *)wp;
+	StaticCompoundShapeSettings ss;
+	MutableCompoundShapeSettings ms;
+	CompoundShapeSettings &cs = is_mutable ? static_cast<CompoundShapeSettings &>(ms) : static_cast<CompoundShapeSettings &>(ss);
+	for (int i = 0; i < n; ++i)
+	{
+		const Shape *s = HeldShape(w, children[i]);
+		if (s == nullptr)
+			return -1;
+		cs.AddShape(Vec3(pos3n[3 * i], pos3n[3 * i + 1], pos3n[3 * i + 2]),
+					Quat(rot4n[4 * i], rot4n[4 * i + 1], rot4n[4 * i + 2], rot4n[4 * i + 3]), s);
+	}
+	Shape::ShapeResult r = cs.Create();
+	return r.HasError() ? -1 : KeepShape(w, r.Get());
+}
+
+// A collider body of a built shape (static, or dynamic with the shape's mass properties at
+// density 1000 and world velocities lin[3], ang[3])
+int jo_world_add_shape(void *wp, int shape, const float *pos, const float *rot, float friction, float restitution, int dynamic,
+					   const float *lin, const float *ang)
+{
+	World *w = (World *)wp;
+	const Shape *s = HeldShape(w, shape);
+	if (s == nullptr)
+		return -1;
+	BodyCreationSettings bcs(s, RVec3(pos[0], pos[1], pos[2]), Quat(rot[0], rot[1], rot[2], rot[3]),
+							 dynamic ? EMotionType::Dynamic : EMotionType::Static, dynamic ? Layers::MOVING : Layers::STATIC);
+	bcs.mFriction = friction;
+	bcs.mRestitution = restitution;
+	if (dynamic)
+	{
+		bcs.mLinearVelocity = Vec3(lin[0], lin[1], lin[2]);
+		bcs.mAngularVelocity = Vec3(ang[0], ang[1], ang[2]);
+	}
+	BodyID id = w->mSystem.GetBodyInterface().CreateAndAddBody(bcs, EActivation::DontActivate);
+	if (id.IsInvalid())
+		return -1;
+	w->mColliders.push_back(id);
+	w->mColliderParams.push_back(std::array<float, 8>{});
+	w->mColliderTypes.push_back(9);
+	return int(w->mColliders.size() - 1);
+}
+
+// The leaves of compound collider `collider` as the device takes them (jolt_xpbd_shapes.h).
+// Returns the number of leaves, or -1.  For part >= 0 also sizes[6] = {placement steps, tree
+// nodes, triangles, hull points, hull faces, hull face vertex indices}, and with non-null arrays
+// its type, params8, scale3, steps (7 words each: kind, 6 floats), tree (root, flags, nodes of 28
+// words, triangles of 9 floats) and hull (points xyz, planes, face first / count, indices).
+int jo_world_collider_part(void *wp, int collider, int part, int *sizes, int *type, float *params8, float *scale3, void *ops,
+						   uint32 *root, uint32 *tree_flags, void *nodes, float *triangles, float *hull_points, float *hull_planes,
+						   uint32 *face_first, uint32 *face_count, uint32 *hull_index)
+{
+	World *w = (World *)wp;
+	BodyLockRead lock(w->mSystem.GetBodyLockInterface(), w->mColliders[collider]);
+	if (!lock.Succeeded())
+		return -1;
+	xpbd_collider_desc d = {};
+	JoltXpbd::ShapeData data;
+	std::string err;
+	if (!JoltXpbd::DescribeShape(lock.GetBody().GetShape(), d, data, err))
+		return -1;
+	const int n = int(d.num_parts);
+	if (part < 0)
+		return n;
+	if (part >= n)
+		return -1;
+	const xpbd_collider_part &pt = d.parts[part];
+	const JoltXpbd::ShapeData &pd = *data.part_data[part];
+	sizes[0] = int(pt.num_ops);
+	sizes[1] = pt.tree ? int(pt.tree->num_nodes) : 0;
+	sizes[2] = pt.tree ? int(pt.tree->num_triangles) : 0;
+	sizes[3] = pt.hull ? int(pt.hull->num_points) : 0;
+	sizes[4] = pt.hull ? int(pt.hull->num_faces) : 0;
+	sizes[5] = pt.hull ? int(pt.hull->num_vertex_index) : 0;
+	if (type == nullptr)
+		return n;
+	*type = pt.type;
+	for (int k = 0; k < 8; ++k) params8[k] = pt.params[k];
+	for (int k = 0; k < 3; ++k) scale3[k] = pt.scale[k];
+	if (pt.num_ops)
+		memcpy(ops, pt.ops, pt.num_ops * sizeof(xpbd_part_op));
+	if (pt.tree)
+	{
+		*root = pt.tree->root;
+		*tree_flags = pt.tree->flags;
+		memcpy(nodes, pd.nodes.data(), pd.nodes.size() * sizeof(xpbd_tree_node));
+		memcpy(triangles, pd.triangles.data(), pd.triangles.size() * sizeof(float));
+	}
+	if (pt.hull)
+	{
+		memcpy(hull_points, pd.points.data(), pd.points.size() * sizeof(float));
+		memcpy(hull_planes, pd.planes.data(), pd.planes.size() * sizeof(float));
+		memcpy(face_first, pd.first.data(), pd.first.size() * sizeof(uint32));
+		memcpy(face_count, pd.count.data(), pd.count.size() * sizeof(uint32));
+		memcpy(hull_index, pd.index.data(), pd.index.size() * sizeof(uint32));
+	}
+	return n;
+}
+
+// The leaf shapes a soft body at (soft_pos3, soft_rot4) collects from collider `collider` whose
+// local bounds are box6 (min xyz, max xyz): CollectTransformedShapes as DetermineCollidingShapes
+// calls it (SoftBodyMotionProperties.cpp:174-188).  Writes per leaf the 16 floats of
+// Mat44::sRotationTranslation(rotation, position), column major, and its scale (3); returns the
+// number of leaves (at most max_leaves written), or -1.
+int jo_collect_leaves(void *wp, int collider, const float *soft_pos3, const float *soft_rot4, const float *box6, float *out,
+					  int max_leaves)
+{
+	World *w = (World *)wp;
+	BodyLockRead lock(w->mSystem.GetBodyLockInterface(), w->mColliders[collider]);
+	if (!lock.Succeeded())
+		return -1;
+	const Body &body = lock.GetBody();
+	RMat44 soft = RMat44::sRotationTranslation(Quat(soft_rot4[0], soft_rot4[1], soft_rot4[2], soft_rot4[3]),
+											   RVec3(soft_pos3[0], soft_pos3[1], soft_pos3[2]));
+	Mat44 com = (soft.InversedRotationTranslation() * body.GetCenterOfMassTransform()).ToMat44();
+	struct Collector : public TransformedShapeCollector
+	{
+		void AddHit(const TransformedShape &inResult) override { mHits.push_back(inResult); }
+		std::vector<TransformedShape> mHits;
+	} collector;
+	AABox box(Vec3(box6[0], box6[1], box6[2]), Vec3(box6[3], box6[4], box6[5]));
+	body.GetShape()->CollectTransformedShapes(box, com.GetTranslation(), com.GetQuaternion(), Vec3::sOne(), SubShapeIDCreator(), collector,
+											  ShapeFilter());
+	int n = 0;
+	for (const TransformedShape &ts : collector.mHits)
+	{
+		if (n < max_leaves)
+		{
+			Mat44 m = Mat44::sRotationTranslation(ts.mShapeRotation, Vec3(ts.mShapePositionCOM));
+			for (int c = 0; c < 4; ++c)
+				for (int r = 0; r < 4; ++r)
+					out[19 * n + 4 * c + r] = m(r, c);
+			Vec3 sc = ts.GetShapeScale();
+			out[19 * n + 16] = sc.GetX(); out[19 * n + 17] = sc.GetY(); out[19 * n + 18] = sc.GetZ();
+		}
+		++n;
+	}
+	return n;
+}
+
 // Static triangle collider (TriangleShape, convex radius 0): v9 = three vertices xyz.
 int jo_world_add_triangle(void *wp, const float *v9, const float *pos, const float *rot, float friction, float restitution)
 {
@@ -856,10 +1072,10 @@ int jo_world_hull_data(void *wp, int collider, int *sizes, float *points, float 
 }
 
 // The collider as the soft-body update consumes it: the body's centre-of-mass
-// position and the shape's parameters as the shape holds them.  For the tapered
-// capsule these are the constructor's (TaperedCapsuleShape.cpp:117-131): top and
-// bottom centre relative to the centre of mass, the radii, sin and tan of the
-// cone angle (Jolt's Tan and ASin).
+// position and the shape's parameters as the shape holds them (jolt_xpbd_shapes.h).
+// For the tapered capsule: top and bottom centre relative to the centre of mass, the
+// radii, sin and tan of the cone angle, as its constructor stored them
+// (TaperedCapsuleShape.cpp:117-131).
 int jo_world_collider_device(void *wp, int collider, float *params8, float *com3)
 {
 	World *w = (World *)wp;
@@ -870,16 +1086,7 @@ int jo_world_collider_device(void *wp, int collider, float *params8, float *com3
 	com3[0] = float(com.GetX()); com3[1] = float(com.GetY()); com3[2] = float(com.GetZ());
 	const std::array<float, 8> &p = w->mColliderParams[collider];
 	for (int k = 0; k < 8; ++k) params8[k] = p[k];
-	if (w->mColliderTypes[collider] == 5)
-	{
-		const float half_height = p[0], top_radius = p[1], bottom_radius = p[2];
-		const float top_center = half_height + 0.5f * (bottom_radius - top_radius);
-		const float bottom_center = -half_height + 0.5f * (bottom_radius - top_radius);
-		const float sin_alpha = (bottom_radius - top_radius) / (top_center - bottom_center);
-		params8[0] = top_center; params8[1] = bottom_center; params8[2] = top_radius; params8[3] = bottom_radius;
-		params8[4] = sin_alpha; params8[5] = Tan(ASin(sin_alpha)); params8[6] = 0.0f; params8[7] = 0.0f;
-	}
-	if (w->mColliderTypes[collider] == 7 || w->mColliderTypes[collider] == 8)
+	if (w->mColliderTypes[collider] == 5 || w->mColliderTypes[collider] == 7 || w->mColliderTypes[collider] == 8)
 	{
 		xpbd_collider_desc d = {};
 		JoltXpbd::ShapeData data;

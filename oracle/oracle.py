@@ -16,9 +16,9 @@ import os
 
 import numpy as np
 
-from joltphysics_amd.records import (BEND, EDGE, FACE, GROUP, INV_BIND, LRA, MOTION_DYNAMIC, ROD_BT, ROD_SS,
-                                     SHAPE_CONVEX_HULL, SHAPE_MESH, SKINNED, TREE_NODE,
-                                     VERTEX, VOLUME, ConvexHull, SettingsArrays, TriangleTree)
+from joltphysics_amd.records import (BEND, EDGE, FACE, GROUP, INV_BIND, LRA, MOTION_DYNAMIC, PART_OP, ROD_BT, ROD_SS,
+                                     SHAPE_COMPOUND, SHAPE_CONVEX_HULL, SHAPE_MESH, SKINNED, TREE_NODE,
+                                     VERTEX, VOLUME, ColliderPart, ConvexHull, SettingsArrays, TriangleTree)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REF_DIR = os.path.join(HERE, "_ref")
@@ -110,6 +110,14 @@ class JoltReference:
             lib.jo_world_add_height_field.restype = _i
             lib.jo_world_add_height_field.argtypes = [_vp, _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _f, _f]
             lib.jo_world_hull_data.restype = _i
+            for name, args in (("jo_shape_basic", [_vp, _i, _vp]), ("jo_shape_hull", [_vp, _vp, _i]),
+                               ("jo_shape_scaled", [_vp, _i, _vp]), ("jo_shape_triangle", [_vp, _vp]), ("jo_shape_rotated_translated", [_vp, _i, _vp, _vp]),
+                               ("jo_shape_offset_com", [_vp, _i, _vp]), ("jo_shape_compound", [_vp, _i, _i, _vp, _vp, _vp]),
+                               ("jo_world_add_shape", [_vp, _i, _vp, _vp, _f, _f, _i, _vp, _vp]),
+                               ("jo_world_collider_part", [_vp, _i, _i] + [_vp] * 14),
+                               ("jo_collect_leaves", [_vp, _i, _vp, _vp, _vp, _vp, _i])):
+                getattr(lib, name).restype = _i
+                getattr(lib, name).argtypes = args
             lib.jo_world_add_dynamic.restype = _i
             lib.jo_world_add_dynamic.argtypes = [_vp, _i, _vp, _vp, _vp, _f, _f, _vp, _vp]
             lib.jo_world_collider_motion.restype = _i
@@ -246,9 +254,92 @@ class JoltWorld:
     def __del__(self):
         self.close()
 
+    def _build_shape(self, spec):
+        """Builds a ShapeSpec with the reference's shape classes; returns its handle."""
+        f3 = lambda v: np.array(v, np.float32)
+        if spec.kind == "basic":
+            p = np.zeros(8, np.float32)
+            p[:len(spec.params)] = spec.params
+            h = self.lib.jo_shape_basic(self.h, int(spec.type), _ptr(p))
+        elif spec.kind == "hull":
+            pts = np.ascontiguousarray(np.array(spec.params, np.float32).reshape(-1, 3))
+            h = self.lib.jo_shape_hull(self.h, _ptr(pts), len(pts))
+        elif spec.kind == "triangle":
+            h = self.lib.jo_shape_triangle(self.h, _ptr(np.array(spec.params, np.float32)))
+        elif spec.kind == "scaled":
+            h = self.lib.jo_shape_scaled(self.h, self._build_shape(spec.inner), _ptr(f3(spec.vec)))
+        elif spec.kind == "rotated_translated":
+            h = self.lib.jo_shape_rotated_translated(self.h, self._build_shape(spec.inner), _ptr(f3(spec.vec)), _ptr(f3(spec.rot)))
+        elif spec.kind == "offset_com":
+            h = self.lib.jo_shape_offset_com(self.h, self._build_shape(spec.inner), _ptr(f3(spec.vec)))
+        elif spec.kind in ("static_compound", "mutable_compound"):
+            kids = np.array([self._build_shape(c[0]) for c in spec.children], np.int32)
+            pos = np.ascontiguousarray(np.array([c[1] for c in spec.children], np.float32).reshape(-1, 3))
+            rot = np.ascontiguousarray(np.array([c[2] for c in spec.children], np.float32).reshape(-1, 4))
+            h = self.lib.jo_shape_compound(self.h, int(spec.kind == "mutable_compound"), len(kids), _ptr(kids), _ptr(pos), _ptr(rot))
+        else:
+            raise ValueError(f"unknown shape kind {spec.kind}")
+        if h < 0:
+            raise RuntimeError(f"building a {spec.kind} shape failed")
+        return h
+
+    def collect_leaves(self, i, soft_pos, soft_rot, box):
+        """The reference's CollectTransformedShapes of collider i for a soft body at (soft_pos,
+        soft_rot) with local bounds box (min xyz, max xyz): [(4x4 leaf transform, scale)]."""
+        f = lambda v: np.array(v, np.float32)
+        out = np.zeros((256, 19), np.float32)
+        n = self.lib.jo_collect_leaves(self.h, i, _ptr(f(soft_pos)), _ptr(f(soft_rot)), _ptr(f(box)), _ptr(out), 256)
+        if n < 0 or n > 256:
+            raise RuntimeError("collect_leaves failed")
+        return [(out[k, :16].reshape(4, 4).T.copy(), tuple(out[k, 16:])) for k in range(n)]
+
+    def collider_parts(self, i):
+        """The leaves of compound collider i as the device takes them (ColliderPart list)."""
+        n = self.lib.jo_world_collider_part(self.h, i, -1, *([None] * 14))
+        if n < 0:
+            raise RuntimeError("collider_part failed")
+        parts = []
+        for k in range(n):
+            sizes = np.zeros(6, np.int32)
+            self.lib.jo_world_collider_part(self.h, i, k, _ptr(sizes), *([None] * 13))
+            nops, nn, nt, npt, nf, ni = (int(x) for x in sizes)
+            t = C.c_int()
+            params, scale = np.zeros(8, np.float32), np.zeros(3, np.float32)
+            ops = np.zeros(max(nops, 1), PART_OP)
+            root, flags = C.c_uint32(), C.c_uint32()
+            nodes = np.zeros(max(nn, 1), TREE_NODE)
+            tris = np.zeros((max(nt, 1), 9), np.float32)
+            pts, planes = np.zeros((max(npt, 1), 3), np.float32), np.zeros((max(nf, 1), 4), np.float32)
+            first, count, index = np.zeros(max(nf, 1), np.uint32), np.zeros(max(nf, 1), np.uint32), np.zeros(max(ni, 1), np.uint32)
+            if self.lib.jo_world_collider_part(self.h, i, k, _ptr(sizes), C.byref(t), _ptr(params), _ptr(scale), ops.ctypes.data,
+                                               C.byref(root), C.byref(flags), nodes.ctypes.data, _ptr(tris), _ptr(pts),
+                                               _ptr(planes), _ptr(first), _ptr(count), _ptr(index)) != n:
+                raise RuntimeError("collider_part failed")
+            hull = tree = None
+            if t.value == SHAPE_CONVEX_HULL:
+                hull = ConvexHull(pts[:npt], planes[:nf], first[:nf], count[:nf], index[:ni])
+            if t.value == SHAPE_MESH:
+                tree = TriangleTree(int(root.value), nodes[:nn], tris[:nt], int(flags.value))
+            parts.append(ColliderPart(int(t.value), tuple(float(x) for x in params), tuple(float(x) for x in scale),
+                                      ops[:nops].copy(), hull, tree))
+        return parts
+
     def add_collider(self, c):
         pos = np.array(c.pos, np.float32)
         rot = np.array(c.rot, np.float32)
+        if c.type == SHAPE_COMPOUND:
+            h = self._build_shape(c.shape)
+            dyn = c.motion_type == MOTION_DYNAMIC
+            lin = np.array(c.linear_velocity, np.float32)
+            ang = np.array(c.angular_velocity, np.float32)
+            i = self.lib.jo_world_add_shape(self.h, h, _ptr(pos), _ptr(rot), float(c.friction), float(c.restitution), int(dyn),
+                                            _ptr(lin), _ptr(ang))
+            if i < 0:
+                raise RuntimeError("add_collider failed")
+            if c.is_sensor:
+                self.lib.jo_world_set_sensor(self.h, i)
+            self._num_colliders = i + 1
+            return i
         if tuple(float(x) for x in c.scale) != (1.0, 1.0, 1.0):
             self.lib.jo_world_set_next_scale(self.h, _ptr(np.array(c.scale, np.float32)))
         if c.type == SHAPE_MESH and c.motion_type != MOTION_DYNAMIC:
@@ -474,6 +565,14 @@ class Restatement:
             lib.xr_world_set_collider_scale.argtypes = [_vp, _i, _vp]
             lib.xr_world_set_tree.restype = _i
             lib.xr_world_set_tree.argtypes = [_vp, _i, _u32, _vp, _vp, _u32, _vp, _u32, _u32]
+            lib.xr_world_set_part_tree.restype = _i
+            lib.xr_world_set_part_tree.argtypes = [_vp, _i, _i, _u32, _vp, _vp, _u32, _vp, _u32, _u32]
+            lib.xr_world_set_part_hull.restype = _i
+            lib.xr_world_set_part_hull.argtypes = [_vp, _i, _i, _vp, _u32, _vp, _u32, _vp, _vp, _vp, _u32]
+            lib.xr_collect_leaves.restype = _i
+            lib.xr_collect_leaves.argtypes = [_vp, _i, _vp, _vp, _vp, _vp, _i]
+            lib.xr_world_add_part.restype = _i
+            lib.xr_world_add_part.argtypes = [_vp, _i, _i, _vp, _vp, _u32, _vp]
             lib.xr_world_add_soft_body.restype = _i
             lib.xr_world_add_soft_body.argtypes = [_vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _u32,
                                                    _vp, _u32, _vp, _u32, _vp, _vp, _vp]
@@ -499,33 +598,55 @@ class Restatement:
     def __del__(self):
         self.close()
 
+    def _set_hull(self, i, part, h):
+        a = (np.ascontiguousarray(h.points, np.float32), np.ascontiguousarray(h.planes, np.float32),
+             np.ascontiguousarray(h.face_first, np.uint32), np.ascontiguousarray(h.face_count, np.uint32),
+             np.ascontiguousarray(h.vertex_index, np.uint32))
+        if self.lib.xr_world_set_part_hull(self.h, i, part, _ptr(a[0]), len(a[0]), _ptr(a[1]), len(a[1]), _ptr(a[2]),
+                                           _ptr(a[3]), _ptr(a[4]), len(a[4])) != 0:
+            raise RuntimeError("invalid convex hull data")
+
+    def _set_tree(self, i, part, tree):
+        if tree is None:
+            raise RuntimeError("mesh collider without tree data")
+        nodes = np.ascontiguousarray(tree.nodes, TREE_NODE)
+        bounds = np.ascontiguousarray(np.concatenate([nodes["min"].reshape(-1, 12), nodes["max"].reshape(-1, 12)], axis=1), np.float32)
+        children = np.ascontiguousarray(nodes["child"], np.uint32)
+        tris = np.ascontiguousarray(tree.triangles, np.float32)
+        if self.lib.xr_world_set_part_tree(self.h, i, part, int(tree.root), _ptr(bounds) if len(nodes) else None,
+                                           _ptr(children) if len(nodes) else None, len(nodes), _ptr(tris) if len(tris) else None,
+                                           len(tris), int(tree.flags)) != 0:
+            raise RuntimeError("invalid triangle tree")
+
     def add_collider(self, c, bounds=None):
-        p = c.device_param_array()
+        compound = c.type == SHAPE_COMPOUND
+        p = np.zeros(8, np.float32) if compound else c.device_param_array()
         pos = np.array(c.com_position(), np.float32)
         rot = np.array(c.rot, np.float32)
         b = np.array(bounds if bounds is not None else c.bounds, np.float32)
-        i = self.lib.xr_world_add_collider(self.h, int(c.type), _ptr(p), _ptr(pos), _ptr(rot),
+        # a compound body holds no shape of its own: its leaves follow as parts
+        i = self.lib.xr_world_add_collider(self.h, 1 if compound else int(c.type), _ptr(p), _ptr(pos), _ptr(rot),
                                            float(c.friction), float(c.restitution), _ptr(b))
-        if i >= 0 and c.type == SHAPE_CONVEX_HULL:
-            h = c.hull
-            a = (np.ascontiguousarray(h.points, np.float32), np.ascontiguousarray(h.planes, np.float32),
-                 np.ascontiguousarray(h.face_first, np.uint32), np.ascontiguousarray(h.face_count, np.uint32),
-                 np.ascontiguousarray(h.vertex_index, np.uint32))
-            if self.lib.xr_world_set_hull(self.h, i, _ptr(a[0]), len(a[0]), _ptr(a[1]), len(a[1]), _ptr(a[2]),
-                                          _ptr(a[3]), _ptr(a[4]), len(a[4])) != 0:
-                raise RuntimeError("invalid convex hull data")
-        if i >= 0 and c.type == SHAPE_MESH:
-            if c.tree is None:
-                raise RuntimeError("mesh collider without tree data")
-            nodes = np.ascontiguousarray(c.tree.nodes, TREE_NODE)
-            bounds = np.ascontiguousarray(np.concatenate([nodes["min"].reshape(-1, 12), nodes["max"].reshape(-1, 12)], axis=1), np.float32)
-            children = np.ascontiguousarray(nodes["child"], np.uint32)
-            tris = np.ascontiguousarray(c.tree.triangles, np.float32)
-            if self.lib.xr_world_set_tree(self.h, i, int(c.tree.root), _ptr(bounds) if len(nodes) else None,
-                                          _ptr(children) if len(nodes) else None, len(nodes), _ptr(tris) if len(tris) else None,
-                                          len(tris), int(c.tree.flags)) != 0:
-                raise RuntimeError("invalid triangle tree")
-        if i >= 0 and tuple(float(x) for x in c.scale) != (1.0, 1.0, 1.0):
+        if i >= 0 and compound:
+            if not c.parts:
+                raise RuntimeError("compound collider without parts")
+            for pt in c.parts:
+                prm = np.zeros(8, np.float32)
+                prm[:len(pt.params)] = pt.params
+                ops = np.ascontiguousarray(pt.ops, PART_OP)
+                k = self.lib.xr_world_add_part(self.h, i, int(pt.type), _ptr(prm), _ptr(np.array(pt.scale, np.float32)), len(ops),
+                                               ops.ctypes.data if len(ops) else None)
+                if k < 0:
+                    raise RuntimeError("invalid compound part")
+                if pt.type == SHAPE_CONVEX_HULL:
+                    self._set_hull(i, k, pt.hull)
+                if pt.type == SHAPE_MESH:
+                    self._set_tree(i, k, pt.tree)
+        if i >= 0 and not compound and c.type == SHAPE_CONVEX_HULL:
+            self._set_hull(i, -1, c.hull)
+        if i >= 0 and not compound and c.type == SHAPE_MESH:
+            self._set_tree(i, -1, c.tree)
+        if i >= 0 and not compound and tuple(float(x) for x in c.scale) != (1.0, 1.0, 1.0):
             self.lib.xr_world_set_collider_scale(self.h, i, _ptr(np.array(c.scale, np.float32)))
         if i >= 0 and c.is_sensor:
             self.lib.xr_world_set_sensor(self.h, i)
@@ -537,6 +658,16 @@ class Restatement:
                 raise RuntimeError("invalid collider motion")
         self._num_colliders = max(getattr(self, "_num_colliders", 0), i + 1)
         return i
+
+    def collect_leaves(self, i, soft_pos, soft_rot, box):
+        """The restatement's leaf transforms (4x4 each) of collider i for a soft body at
+        (soft_pos, soft_rot) with local bounds box (min xyz, max xyz)."""
+        f = lambda v: np.array(v, np.float32)
+        out = np.zeros((256, 16), np.float32)
+        n = self.lib.xr_collect_leaves(self.h, i, _ptr(f(soft_pos)), _ptr(f(soft_rot)), _ptr(f(box)), _ptr(out), 256)
+        if n < 0 or n > 256:
+            raise RuntimeError("collect_leaves failed")
+        return [out[k].reshape(4, 4).T.copy() for k in range(n)]
 
     def set_motion(self, i, c):
         """Updates collider i's motion (velocities) from Collider c between update calls."""

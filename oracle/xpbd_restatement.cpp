@@ -223,6 +223,29 @@ template <class F> void GetQuaternion(const M44<F> &m, F q[4])
 	}
 }
 
+// Quat * Quat (Quat.inl:7-48, no FMA): x = (a w + b z) + (d x - c y), y = (b w + c x) + (d y - a z),
+// z = (c w + a y) + (d z - b x), w = -(a x + b y) + (d w - c z) for lhs {a b c d}, rhs {x y z w}
+template <class F> void QuatMul(const F l[4], const F r[4], F out[4])
+{
+	const F a = l[0], b = l[1], c = l[2], d = l[3];
+	const F x = r[0], y = r[1], z = r[2], w = r[3];
+	out[0] = (a * w + b * z) + (d * x - c * y);
+	out[1] = (b * w + c * x) + (d * y - a * z);
+	out[2] = (c * w + a * y) + (d * z - b * x);
+	out[3] = -(a * x + b * y) + (d * w - c * z);
+}
+
+// Quat * Vec3 (Quat.inl:383-405): v + 2 (w (u x v) + u x (u x v)) for u = the vector part, both
+// cross products as the difference of the swizzled products the SSE code forms
+template <class F> V3<F> QuatRotate(const F q[4], V3<F> v)
+{
+	auto cross = [&](V3<F> a) { return V3<F>{ a.z * q[1] - q[2] * a.y, a.x * q[2] - q[0] * a.z, a.y * q[0] - q[1] * a.x }; };
+	const V3<F> uv = cross(v);
+	const V3<F> uuv = cross(uv);
+	const V3<F> t = q[3] * uv + uuv;
+	return v + (t + t);
+}
+
 // ------------------------------------------------------------------ planes / boxes
 template <class F> struct PlaneT { V3<F> n; F c; };
 
@@ -295,12 +318,19 @@ struct Collider
 	std::vector<float> tree_tris;
 	u32 tree_root = 0;
 	bool tree_height_field = false;	// invalid child bounds (min y > max y) are never visited
+	// a compound or decorated body: its leaf shapes in collection order, each with the steps
+	// that place it relative to the body's centre of mass (kind 1 bounds test against v[0..5],
+	// 2 p += q * v, 3 q = q * v, 4 p -= q * v); the leaves carry no pose or material of their own
+	struct Op { u32 kind; float v[6]; };
+	std::vector<Op> ops;
+	std::vector<Collider> parts;
 };
 
 template <class F> struct Leaf
 {
-	M44<F> transform;		// leaf shape transform in soft-body local space
-	const Collider *col;
+	// the body's leaf shapes (LeafShape, SoftBodyMotionProperties.h:187-195): transform in
+	// soft-body local space and shape
+	std::vector<std::pair<M44<F>, const Collider *>> shapes;
 	size_t col_index;
 	F friction, restitution;
 	// CollidingShape fields of a dynamic body (SoftBodyMotionProperties.h:190-212)
@@ -525,10 +555,8 @@ template <class F> TriangleHit<F> WalkTriangleTree(const Collider &c, V3<F> scal
 // ------------------------------------------------------------------ shape collision
 // Restates Shape::CollideSoftBodyVertices for the four primitive shapes, with
 // UpdatePenetration / SetCollision of CollideSoftBodyVertexIterator.h:83-97.
-template <class F> void CollideVertices(Body<F> &b, const Leaf<F> &leaf, int shape_index)
+template <class F> void CollideVertices(Body<F> &b, const Collider &c, const M44<F> &T, int shape_index)
 {
-	const Collider &c = *leaf.col;
-	const M44<F> &T = leaf.transform;
 	const size_t nv = b.x.size();
 	const V3<F> scale{ F(c.scale[0]), F(c.scale[1]), F(c.scale[2]) };
 	const V3<F> abs_scale = VAbs(scale);
@@ -1246,6 +1274,92 @@ template <class F> V3<F> UpdateState(Body<F> &b, F step_dt, const M44<F> &com)
 	return delta_world;
 }
 
+// AABox4VsBox (Geometry/AABox4.h:86-185) for a single box: do the bounds [bmin, bmax] of a compound
+// child overlap the oriented box `orientation` * [-half, half]?  Separating axis test over the
+// box axes A, the oriented box axes B and their 9 cross products, with 1e-6 added to every
+// |rotation| entry.  R(i, j) below is row i, column j of the orientation.
+template <class F> bool BoxVsOrientedBox(const M44<F> &orientation, V3<F> half, const F bmin[3], const F bmax[3])
+{
+	auto R = [&](int row, int col) { return orientation.c[col][row]; };
+	const F eps = F(1.0e-6f);
+	F abs_r[3][3];		// abs_r[axis of B][component]
+	for (int i = 0; i < 3; ++i)
+		for (int j = 0; j < 3; ++j)
+			abs_r[i][j] = std::abs(R(j, i)) + eps;
+	F t[3], a[3];
+	for (int i = 0; i < 3; ++i)
+	{
+		t[i] = orientation.c[3][i] - F(0.5f) * (bmin[i] + bmax[i]);
+		a[i] = F(0.5f) * (bmax[i] - bmin[i]);
+	}
+	const F h[3] = { half.x, half.y, half.z };
+	bool overlap = true;
+	auto test = [&](F dist, F ra, F rb) { overlap = overlap && std::abs(dist) <= ra + rb; };
+	for (int i = 0; i < 3; ++i)
+		test(t[i], a[i], (h[0] * abs_r[0][i] + h[1] * abs_r[1][i]) + h[2] * abs_r[2][i]);
+	for (int i = 0; i < 3; ++i)
+		test((t[0] * R(0, i) + t[1] * R(1, i)) + t[2] * R(2, i), (a[0] * abs_r[i][0] + a[1] * abs_r[i][1]) + a[2] * abs_r[i][2], h[i]);
+	// A_i x B_j for i, j = 0..2: with (i1, i2) and (j1, j2) the other two axes of each box
+	const int other[3][2] = { { 1, 2 }, { 0, 2 }, { 0, 1 } };
+	for (int i = 0; i < 3; ++i)
+		for (int j = 0; j < 3; ++j)
+		{
+			const int i1 = other[i][0], i2 = other[i][1], j1 = other[j][0], j2 = other[j][1];
+			const F ra = a[i1] * abs_r[j][i2] + a[i2] * abs_r[j][i1];
+			const F rb = h[j1] * abs_r[j2][i] + h[j2] * abs_r[j1][i];
+			// the distance along A_i x B_j: t[i2] R(i1, j) - t[i1] R(i2, j), signs as the
+			// reference orders them (t2 R1j - t1 R2j, t0 R2j - t2 R0j, t1 R0j - t0 R1j)
+			const F dist = i == 1 ? t[0] * R(2, j) - t[2] * R(0, j) : t[i2] * R(i1, j) - t[i1] * R(i2, j);
+			test(dist, ra, rb);
+		}
+	return overlap;
+}
+
+// CollectTransformedShapes from a compound or decorated body to one of its leaves
+// (CompoundShapeVisitors.h:226-272, StaticCompoundShape.cpp:543-570, MutableCompoundShape.cpp:443-470,
+// RotatedTranslatedShape.cpp:169-176, OffsetCenterOfMassShape.cpp:135-142): starting from the
+// body's rotation q and position p relative to the soft body, apply the part's steps; a bounds
+// step builds the compound's view of the soft body's local bounds `box`,
+// OrientedBox(Mat44::sInverseRotationTranslation(q, p), box) (Mat44.inl:156-161, OrientedBox.h:27),
+// and drops the part when the child's bounds miss it.
+template <class F> bool PlacePart(const Collider &part, const Box3<F> &box, F q[4], V3<F> &p)
+{
+	for (const Collider::Op &op : part.ops)
+		switch (op.kind)
+		{
+		case 1:
+			{
+				const F conj[4] = { -q[0], -q[1], -q[2], q[3] };
+				M44<F> m = RotationTranslation(conj, V3<F>{ 0, 0, 0 });
+				const V3<F> mt = Neg(Mul3x3(m, p));
+				const V3<F> center = F(0.5f) * (box.mn + box.mx);
+				const V3<F> half = F(0.5f) * (box.mx - box.mn);
+				const V3<F> origin = mt + Mul3x3(m, center);
+				m.c[3][0] = origin.x; m.c[3][1] = origin.y; m.c[3][2] = origin.z;
+				const F bmin[3] = { F(op.v[0]), F(op.v[1]), F(op.v[2]) }, bmax[3] = { F(op.v[3]), F(op.v[4]), F(op.v[5]) };
+				if (!BoxVsOrientedBox(m, half, bmin, bmax))
+					return false;
+			}
+			break;
+		case 2:
+			p = p + QuatRotate(q, V3<F>{ F(op.v[0]), F(op.v[1]), F(op.v[2]) });
+			break;
+		case 3:
+			{
+				const F r[4] = { F(op.v[0]), F(op.v[1]), F(op.v[2]), F(op.v[3]) };
+				F out[4];
+				QuatMul(q, r, out);
+				for (int k = 0; k < 4; ++k)
+					q[k] = out[k];
+			}
+			break;
+		default:
+			p = p - QuatRotate(q, V3<F>{ F(op.v[0]), F(op.v[1]), F(op.v[2]) });
+			break;
+		}
+	return true;
+}
+
 // One collision step of one body: InitializeUpdateContext + DetermineCollidingShapes
 // + ParallelUpdate state machine + JobSoftBodyFinalize position update
 // (SoftBodyMotionProperties.cpp:937-960, 120-255, 969-1115; PhysicsSystem.cpp:2842-2864).
@@ -1280,8 +1394,18 @@ template <class F> void StepBody(World<F> &w, Body<F> &b, F step_dt)
 		F lq[4];
 		GetQuaternion(rel, lq);
 		Leaf<F> leaf;
-		leaf.transform = RotationTranslation(lq, Col3(rel, 3));
-		leaf.col = &c;
+		if (c.parts.empty())
+			leaf.shapes.push_back({ RotationTranslation(lq, Col3(rel, 3)), &c });
+		else
+			for (const Collider &part : c.parts)
+			{
+				F pq[4] = { lq[0], lq[1], lq[2], lq[3] };
+				V3<F> pp = Col3(rel, 3);
+				if (PlacePart(part, lb, pq, pp))
+					leaf.shapes.push_back({ RotationTranslation(pq, pp), &part });
+			}
+		if (leaf.shapes.empty())
+			continue;		// no leaf collected: the body is left out (SoftBodyMotionProperties.cpp:189-190)
 		leaf.col_index = size_t(&c - w.colliders.data());
 		leaf.friction = std::sqrt(F(b.friction) * F(c.friction));		// ContactConstraintManager.h:554
 		leaf.restitution = std::max(F(b.restitution), F(c.restitution)); // ContactConstraintManager.h:555
@@ -1315,7 +1439,8 @@ template <class F> void StepBody(World<F> &w, Body<F> &b, F step_dt)
 
 	// DetermineCollisionPlanes (SoftBodyMotionProperties.cpp:257-265)
 	for (size_t s = 0; s < b.leaves.size(); ++s)
-		CollideVertices(b, b.leaves[s], int(s));
+		for (const auto &shape : b.leaves[s].shapes)
+			CollideVertices(b, *shape.second, shape.first, int(s));
 
 	// Iterations: StartFirstIteration, then ParallelApplyConstraints loop
 	ApplyPressure(b);
@@ -1537,16 +1662,32 @@ int xr_world_set_collider_scale(void *wp, int collider, const float *scale3)
 	return 0;
 }
 
-// Triangle tree of mesh collider `collider` (type 8): the root reference, nn nodes of 24 bound
-// floats (min x, y, z, max x, y, z of the four children) and 4 child references each, nt
-// triangles of 9 floats; flags bit 0: a height field's hierarchy.
-int xr_world_set_tree(void *wp, int collider, u32 root, const float *bounds, const u32 *children, u32 nn, const float *tris, u32 nt,
-					  u32 flags)
+// The collider itself (part < 0) or one of its parts, in the float and the double world
+static bool Targets(XrWorld *w, int collider, int part, Collider *out[2])
 {
-	XrWorld *w = (XrWorld *)wp;
-	if (collider < 0 || size_t(collider) >= w->f.colliders.size() || w->f.colliders[collider].type != 8)
+	if (collider < 0 || size_t(collider) >= w->f.colliders.size())
+		return false;
+	Collider &f = w->f.colliders[collider], &d = w->d.colliders[collider];
+	if (part < 0)
+	{
+		out[0] = &f; out[1] = &d;
+		return true;
+	}
+	if (size_t(part) >= f.parts.size())
+		return false;
+	out[0] = &f.parts[part]; out[1] = &d.parts[part];
+	return true;
+}
+
+// Triangle tree of a mesh shape (type 8), of collider `collider` (part < 0) or of one of its parts:
+// the root reference, nn nodes of 24 bound floats (min x, y, z, max x, y, z of the four children)
+// and 4 child references each, nt triangles of 9 floats; flags bit 0: a height field's hierarchy.
+int xr_world_set_part_tree(void *wp, int collider, int part, u32 root, const float *bounds, const u32 *children, u32 nn,
+						   const float *tris, u32 nt, u32 flags)
+{
+	Collider *targets[2];
+	if (!Targets((XrWorld *)wp, collider, part, targets) || targets[0]->type != 8)
 		return -1;
-	Collider *targets[2] = { &w->f.colliders[collider], &w->d.colliders[collider] };
 	for (Collider *c : targets)
 	{
 		c->tree_root = root;
@@ -1558,14 +1699,20 @@ int xr_world_set_tree(void *wp, int collider, u32 root, const float *bounds, con
 	return 0;
 }
 
-// Hull data of convex hull collider `collider` (type 6): np points relative to the
-// centre of mass (xyz), nf planes (normal xyz, constant), per face first entry and
-// vertex count into the ni concatenated face vertex indices.
-int xr_world_set_hull(void *wp, int collider, const float *points, u32 np, const float *planes, u32 nf,
-					  const u32 *face_first, const u32 *face_count, const u32 *index, u32 ni)
+int xr_world_set_tree(void *wp, int collider, u32 root, const float *bounds, const u32 *children, u32 nn, const float *tris, u32 nt,
+					  u32 flags)
 {
-	XrWorld *w = (XrWorld *)wp;
-	if (collider < 0 || size_t(collider) >= w->f.colliders.size() || w->f.colliders[collider].type != 6)
+	return xr_world_set_part_tree(wp, collider, -1, root, bounds, children, nn, tris, nt, flags);
+}
+
+// Hull data of a convex hull shape (type 6), of collider `collider` (part < 0) or of one of its
+// parts: np points relative to the centre of mass (xyz), nf planes (normal xyz, constant), per
+// face first entry and vertex count into the ni concatenated face vertex indices.
+int xr_world_set_part_hull(void *wp, int collider, int part, const float *points, u32 np, const float *planes, u32 nf,
+						   const u32 *face_first, const u32 *face_count, const u32 *index, u32 ni)
+{
+	Collider *targets[2];
+	if (!Targets((XrWorld *)wp, collider, part, targets) || targets[0]->type != 6)
 		return -1;
 	for (u32 f = 0; f < nf; ++f)
 		if (face_count[f] < 2 || size_t(face_first[f]) + face_count[f] > ni)
@@ -1573,7 +1720,6 @@ int xr_world_set_hull(void *wp, int collider, const float *points, u32 np, const
 	for (u32 k = 0; k < ni; ++k)
 		if (index[k] >= np)
 			return -1;
-	Collider *targets[2] = { &w->f.colliders[collider], &w->d.colliders[collider] };
 	for (Collider *c : targets)
 	{
 		c->hull_points.assign(points, points + 3 * np);
@@ -1583,6 +1729,73 @@ int xr_world_set_hull(void *wp, int collider, const float *points, u32 np, const
 		c->index.assign(index, index + ni);
 	}
 	return 0;
+}
+
+int xr_world_set_hull(void *wp, int collider, const float *points, u32 np, const float *planes, u32 nf,
+					  const u32 *face_first, const u32 *face_count, const u32 *index, u32 ni)
+{
+	return xr_world_set_part_hull(wp, collider, -1, points, np, planes, nf, face_first, face_count, index, ni);
+}
+
+// The leaf transforms (16 floats each, column major) that a soft body at (soft_pos3, soft_rot4)
+// with local bounds box6 (min xyz, max xyz) collects from collider `collider`, in the float
+// world; the number of leaves (at most max_leaves written), or -1.
+int xr_collect_leaves(void *wp, int collider, const float *soft_pos3, const float *soft_rot4, const float *box6, float *out,
+					  int max_leaves)
+{
+	XrWorld *w = (XrWorld *)wp;
+	if (collider < 0 || size_t(collider) >= w->f.colliders.size())
+		return -1;
+	const Collider &c = w->f.colliders[collider];
+	const float sq[4] = { soft_rot4[0], soft_rot4[1], soft_rot4[2], soft_rot4[3] };
+	M44<float> com = RotationTranslation(sq, V3<float>{ soft_pos3[0], soft_pos3[1], soft_pos3[2] });
+	const float cq[4] = { c.rot[0], c.rot[1], c.rot[2], c.rot[3] };
+	M44<float> rel = Mul(InvRT(com), RotationTranslation(cq, V3<float>{ c.pos[0], c.pos[1], c.pos[2] }));
+	float lq[4];
+	GetQuaternion(rel, lq);
+	const Box3<float> box{ { box6[0], box6[1], box6[2] }, { box6[3], box6[4], box6[5] } };
+	std::vector<M44<float>> leaves;
+	if (c.parts.empty())
+		leaves.push_back(RotationTranslation(lq, Col3(rel, 3)));
+	for (const Collider &part : c.parts)
+	{
+		float pq[4] = { lq[0], lq[1], lq[2], lq[3] };
+		V3<float> pp = Col3(rel, 3);
+		if (PlacePart(part, box, pq, pp))
+			leaves.push_back(RotationTranslation(pq, pp));
+	}
+	for (size_t n = 0; n < leaves.size() && int(n) < max_leaves; ++n)
+		for (int col = 0; col < 4; ++col)
+			for (int r = 0; r < 4; ++r)
+				out[16 * n + 4 * col + r] = leaves[n].c[col][r];
+	return int(leaves.size());
+}
+
+// A leaf shape of compound collider `collider`, appended in collection order: type and params as
+// xr_world_add_collider takes them, scale[3], and num_ops placement steps of 7 words each (kind as
+// a 32-bit integer, then 6 floats).  Returns the part's index.
+int xr_world_add_part(void *wp, int collider, int type, const float *params, const float *scale3, u32 num_ops, const void *ops)
+{
+	XrWorld *w = (XrWorld *)wp;
+	if (collider < 0 || size_t(collider) >= w->f.colliders.size() || type < 0 || type > 8)
+		return -1;
+	Collider part{};
+	part.type = type;
+	for (int i = 0; i < 8; ++i) part.params[i] = params[i];
+	for (int i = 0; i < 3; ++i) part.scale[i] = scale3[i];
+	const unsigned char *bytes = (const unsigned char *)ops;
+	for (u32 k = 0; k < num_ops; ++k)
+	{
+		Collider::Op op;
+		memcpy(&op.kind, bytes + 28 * size_t(k), 4);
+		memcpy(op.v, bytes + 28 * size_t(k) + 4, 24);
+		if (op.kind < 1 || op.kind > 4)
+			return -1;
+		part.ops.push_back(op);
+	}
+	w->f.colliders[collider].parts.push_back(part);
+	w->d.colliders[collider].parts.push_back(part);
+	return int(w->f.colliders[collider].parts.size() - 1);
 }
 
 // p: {numIterations, linearDamping, maxLinearVelocity, restitution, friction,

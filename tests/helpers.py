@@ -5,8 +5,8 @@ import os
 
 import numpy as np
 
-from joltphysics_amd.records import (BEND, EDGE, FACE, GROUP, LRA, MOTION_DYNAMIC, SHAPE_CONVEX_HULL, SHAPE_MESH, VERTEX,
-                                     VOLUME, SettingsArrays)
+from joltphysics_amd.records import (BEND, EDGE, FACE, GROUP, LRA, MOTION_DYNAMIC, SHAPE_COMPOUND, SHAPE_CONVEX_HULL,
+                                     SHAPE_MESH, VERTEX, VOLUME, SettingsArrays)
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 ROOT_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -107,6 +107,8 @@ def reference_run(scene, frames_to_keep, variant="nofma", num_threads=0, record_
             c.hull = world.collider_hull(i)
         if c.type == SHAPE_MESH:
             c.tree = world.collider_tree(i)
+        if c.type == SHAPE_COMPOUND:
+            c.parts = world.collider_parts(i)
     for k in chosen:
         b = scene.bodies[k]
         world.add_soft_body(settings[k], b.position, b.rotation, b.params)
@@ -154,6 +156,8 @@ def reference_custom_run(scene, frames_to_keep, variant="nofma", contact_setting
             c.hull = world.collider_hull(i)
         if c.type == SHAPE_MESH:
             c.tree = world.collider_tree(i)
+        if c.type == SHAPE_COMPOUND:
+            c.parts = world.collider_parts(i)
         motion, m = world.collider_motion(i)
         assert motion == c.motion_type
         if motion == MOTION_DYNAMIC:

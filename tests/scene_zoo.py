@@ -8,9 +8,9 @@ import numpy as np
 
 from joltphysics_amd import scenes
 from joltphysics_amd.records import (BEND_DIHEDRAL, BEND_DISTANCE, LRA_GEODESIC, SHAPE_BOX, SHAPE_CAPSULE,
-                                     MOTION_DYNAMIC, SHAPE_CONVEX_HULL, SHAPE_CYLINDER, SHAPE_MESH, SHAPE_PLANE,
+                                     MOTION_DYNAMIC, SHAPE_COMPOUND, SHAPE_CONVEX_HULL, SHAPE_CYLINDER, SHAPE_MESH, SHAPE_PLANE,
                                      SHAPE_SPHERE, SHAPE_TAPERED_CAPSULE, SHAPE_TAPERED_CYLINDER, BodyParams,
-                                     Collider)
+                                     Collider, ShapeSpec)
 from joltphysics_amd.scenes import (CALC_EDGE_LENGTHS, CALC_VOLUMES, CREATE_CONSTRAINTS, OPTIMIZE, BodySpec,
                                     MeshSpec, Scene)
 
@@ -270,6 +270,76 @@ def cloth_on_mesh_streamed():
     return cloth_on_mesh(n=120, frames=6, spacing=0.02, height=0.12, iterations=4, collision_steps=1)
 
 
+def _compound_colliders(sensor=True):
+    """Compound and decorated colliders: a static compound of 25 boxes, spheres and capsules in a
+    wide grid (a 2-level tree whose outer children lie outside the cloth's bounds), a mutable
+    compound cluster (sphere, box, cylinder, tapered capsule, scaled hull), a rotated-translated
+    capsule, an offset-centre-of-mass box, a uniformly scaled static compound nesting a mutable
+    compound, a rotated-translated box, an offset-centre-of-mass sphere and a triangle, a
+    non-uniformly scaled compound of boxes and a hull, and a sensor compound of two spheres."""
+    S = ShapeSpec
+    cloud, _ = _hull_points()
+    grid = []
+    for k, (x, y) in enumerate((x, y) for x in np.linspace(-1.6, 1.6, 5) for y in np.linspace(-1.6, 1.6, 5)):
+        if k % 3 == 0:
+            child = S.basic(SHAPE_BOX, 0.12, 0.09, 0.05)
+        elif k % 3 == 1:
+            child = S.basic(SHAPE_SPHERE, 0.1)
+        else:
+            child = S.basic(SHAPE_CAPSULE, 0.08, 0.05)
+        grid.append((child, (float(x), float(y), 0.02 * (k % 4)), _quat((k % 3, 1, 0.5), 0.3 * k)))
+    cluster = S.mutable_compound([
+        (S.basic(SHAPE_SPHERE, 0.09), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0)),
+        (S.basic(SHAPE_BOX, 0.1, 0.06, 0.04), (0.16, 0.02, 0.03), _quat((0, 0, 1), 0.5)),
+        (S.basic(SHAPE_CYLINDER, 0.08, 0.05), (-0.1, 0.14, 0.0), _quat((1, 0, 0), 1.2)),
+        (S.basic(SHAPE_TAPERED_CAPSULE, 0.06, 0.03, 0.06), (0.05, -0.16, 0.02), _quat((0.3, 1, 0), 0.8)),
+        (S.scaled(S.hull(cloud), (0.5, 0.4, 0.35)), (-0.14, -0.08, 0.04), _quat((1, 1, 0), 0.4))])
+    nested = S.scaled(S.static_compound([
+        (S.mutable_compound([(S.basic(SHAPE_SPHERE, 0.05), (0.0, 0.05, 0.0), (0.0, 0.0, 0.0, 1.0)),
+                             (S.basic(SHAPE_SPHERE, 0.06), (0.0, -0.06, 0.02), (0.0, 0.0, 0.0, 1.0))]),
+         (0.12, 0.0, 0.02), _quat((0, 1, 0), 0.6)),
+        (S.rotated_translated(S.basic(SHAPE_BOX, 0.07, 0.05, 0.03), (0.02, 0.01, 0.0), _quat((1, 0, 1), 0.7)),
+         (-0.1, 0.08, 0.0), (0.0, 0.0, 0.0, 1.0)),
+        (S.offset_com(S.basic(SHAPE_SPHERE, 0.06), (0.02, -0.01, 0.03)), (-0.08, -0.1, 0.01), _quat((0, 0, 1), 0.2)),
+        (S.triangle((-0.1, -0.08, 0.0, 0.12, -0.05, 0.02, 0.0, 0.11, 0.04)), (0.0, 0.0, 0.08), _quat((1, 0.2, 0), 0.3))]),
+        (1.2, 1.2, 1.2))
+    stretched = S.scaled(S.static_compound([
+        (S.basic(SHAPE_BOX, 0.06, 0.05, 0.04), (0.08, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0)),
+        (S.hull(np.asarray(cloud, np.float32).reshape(-1, 3) * 0.4), (-0.08, 0.02, 0.0), (0.0, 0.0, 0.0, 1.0)),
+        (S.basic(SHAPE_BOX, 0.03, 0.08, 0.03), (0.0, 0.1, 0.02), (0.0, 0.0, 0.0, 1.0))]), (1.3, 0.8, -1.1))
+    cols = [Collider(SHAPE_COMPOUND, (), pos=(0.0, 0.0, 0.05), friction=0.4, shape=S.static_compound(grid)),
+            Collider(SHAPE_COMPOUND, (), pos=(0.45, -0.45, 0.2), rot=_quat((0.2, 0.1, 1), 0.6), restitution=0.2,
+                     shape=cluster),
+            Collider(SHAPE_COMPOUND, (), pos=(-0.5, 0.45, 0.15), rot=_quat((0, 1, 0.2), 0.3), friction=0.6,
+                     shape=S.rotated_translated(S.basic(SHAPE_CAPSULE, 0.15, 0.06), (0.05, 0.0, 0.04), _quat((1, 0, 0), 1.4))),
+            Collider(SHAPE_COMPOUND, (), pos=(-0.45, -0.5, 0.18), rot=_quat((1, 1, 0), 0.25),
+                     shape=S.offset_com(S.basic(SHAPE_BOX, 0.12, 0.1, 0.06), (0.05, -0.02, 0.03))),
+            Collider(SHAPE_COMPOUND, (), pos=(0.45, 0.45, 0.18), rot=_quat((0.1, 0.3, 1), 1.1), friction=0.5, shape=nested),
+            Collider(SHAPE_COMPOUND, (), pos=(0.0, 0.5, 0.2), rot=_quat((0, 0, 1), 0.4), shape=stretched)]
+    if sensor:
+        cols.append(Collider(SHAPE_COMPOUND, (), pos=(0.0, -0.4, 0.35), is_sensor=True,
+                             shape=S.static_compound([(S.basic(SHAPE_SPHERE, 0.08), (0.1, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0)),
+                                                      (S.basic(SHAPE_SPHERE, 0.08), (-0.1, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0))])))
+    return _bounded(cols)
+
+
+def cloth_on_compounds(n=40, frames=30, spacing=0.05, height=0.5, sensor=True):
+    """n x n cloth (calm parameters) falling onto compound and decorated colliders."""
+    v, f = scenes.cloth(n, spacing, 60.0, jitter=1e-4, seed=19)
+    mesh = MeshSpec(vertices=v, faces=f, attrs=scenes.attrs(1e-4, 1e-4, 1e-3), bend_type=BEND_DISTANCE)
+    body = BodySpec(mesh=mesh, position=(0.0, 0.0, height), params=BodyParams(num_iterations=5, allow_sleeping=False))
+    return Scene("cloth_on_compounds", [body], _compound_colliders(sensor), gravity=(0.0, 0.0, -9.81), dt=1 / 60,
+                 collision_steps=2, frames=frames)
+
+
+def cloth_on_compounds_streamed():
+    """The compound scene with a 120 x 120 cloth (14,400 vertices, streamed windows; no sensor, so
+    that the body can run on a team)."""
+    sc = cloth_on_compounds(n=120, frames=10, spacing=0.016, height=0.3, sensor=False)
+    sc.name = "cloth_on_compounds_streamed"
+    return sc
+
+
 ALL = {
     "cloth100_sphere": big_cloth_on_sphere,
     "tetcube_plane_box": tet_cube_on_plane,
@@ -290,6 +360,8 @@ ALL = {
     "cloth_on_mesh_streamed": cloth_on_mesh_streamed,
     "cloth_on_height_field": cloth_on_height_field,
     "cloth_on_height_field_streamed": cloth_on_height_field_streamed,
+    "cloth_on_compounds": cloth_on_compounds,
+    "cloth_on_compounds_streamed": cloth_on_compounds_streamed,
 }
 
 
@@ -326,10 +398,37 @@ def two_cloths_on_dynamic_bodies():
                  collision_steps=1, frames=a.frames)
 
 
+def cloth_on_dynamic_compounds(n=24, frames=12, spacing=0.04):
+    """n x n cloth falling onto a dynamic static compound (box and sphere, its centre of mass
+    between them), a dynamic rotated-translated capsule and a static mutable compound,
+    advanced with CustomUpdate frame by frame: the compounds' leaves share their body's
+    velocities, and the contact impulses act about the body's centre of mass."""
+    S = ShapeSpec
+    v, f = scenes.cloth(n, spacing, 60.0, jitter=1e-4, seed=23)
+    mesh = MeshSpec(vertices=v, faces=f, attrs=scenes.attrs(1e-6, 1e-6, 1e-4), bend_type=BEND_DIHEDRAL)
+    body = BodySpec(mesh=mesh, position=(0.0, 0.0, 0.42), params=BodyParams(num_iterations=5, allow_sleeping=False,
+                                                                            friction=0.5, restitution=0.2))
+    dumbbell = S.static_compound([(S.basic(SHAPE_BOX, 0.12, 0.08, 0.06), (-0.1, 0.0, 0.0), _quat((0, 0, 1), 0.3)),
+                                  (S.basic(SHAPE_SPHERE, 0.09), (0.14, 0.02, 0.01), (0.0, 0.0, 0.0, 1.0))])
+    cols = _bounded([Collider(SHAPE_COMPOUND, (), pos=(-0.1, 0.05, 0.2), rot=_quat((0.2, 1, 0.3), 0.5), friction=0.6,
+                              restitution=0.3, motion_type=MOTION_DYNAMIC, linear_velocity=(0.0, 0.0, 0.4),
+                              angular_velocity=(0.2, -0.1, 0.3), shape=dumbbell),
+                     Collider(SHAPE_COMPOUND, (), pos=(0.25, -0.15, 0.22), friction=0.4, motion_type=MOTION_DYNAMIC,
+                              linear_velocity=(0.3, 0.0, 0.0),
+                              shape=S.rotated_translated(S.basic(SHAPE_CAPSULE, 0.12, 0.05), (0.03, 0.0, 0.02),
+                                                         _quat((1, 0, 0), 1.3))),
+                     Collider(SHAPE_COMPOUND, (), pos=(0.2, 0.25, 0.25), friction=0.3,
+                              shape=S.mutable_compound([(S.basic(SHAPE_BOX, 0.1, 0.1, 0.04), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0)),
+                                                        (S.basic(SHAPE_SPHERE, 0.05), (0.08, 0.08, 0.04), (0.0, 0.0, 0.0, 1.0))]))])
+    return Scene("cloth_on_dynamic_compounds", [body], cols, gravity=(0.0, 0.0, -9.81), dt=1 / 60, collision_steps=1,
+                 frames=frames)
+
+
 # advanced with CustomUpdate (one soft body) or, for several, with the restatement only
 CUSTOM = {
     "cloth_on_dynamic_bodies": cloth_on_dynamic_bodies,
     "cloth_on_dynamic_bodies_streamed": cloth_on_dynamic_bodies_streamed,
+    "cloth_on_dynamic_compounds": cloth_on_dynamic_compounds,
 }
 
 

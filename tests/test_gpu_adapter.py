@@ -36,6 +36,7 @@ SCENES = {
     "cloth_scaled_shapes": scene_zoo.cloth_scaled_shapes,
     "cloth_on_mesh": scene_zoo.cloth_on_mesh,
     "cloth_on_height_field": scene_zoo.cloth_on_height_field,
+    "cloth_on_compounds": scene_zoo.cloth_on_compounds,
 }
 
 

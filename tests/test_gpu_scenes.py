@@ -83,3 +83,32 @@ def test_convex_hull_tables_are_validated():
     with pytest.raises(XpbdError, match="face"):
         ctx.set_colliders([c])
     ctx.close()
+
+
+def test_compound_parts_are_validated():
+    """xpbd_set_colliders rejects compound leaves with an unknown placement step, a leaf shape
+    type it does not collide with, or more steps than XPBD_MAX_PART_OPS."""
+    from joltphysics_amd.records import PART_OP, SHAPE_COMPOUND, SHAPE_SPHERE, Collider, ColliderPart
+    from joltphysics_amd.xpbd import Context, XpbdError
+    ops = np.zeros(2, PART_OP)
+    ops["kind"] = (2, 3)
+    ops["v"][1][3] = 1.0
+    part = ColliderPart(SHAPE_SPHERE, (0.2,), ops=ops)
+    c = Collider(SHAPE_COMPOUND, (), parts=[part], bounds=(-1, -1, -1, 1, 1, 1))
+    ctx = Context(0)
+    try:
+        ctx.set_colliders([c])
+        part.ops = ops.copy()
+        part.ops["kind"][0] = 9
+        with pytest.raises(XpbdError, match="placement step"):
+            ctx.set_colliders([c])
+        part.ops = np.zeros(1025, PART_OP)
+        part.ops["kind"] = 2
+        with pytest.raises(XpbdError, match="XPBD_MAX_PART_OPS"):
+            ctx.set_colliders([c])
+        part.ops = ops
+        part.type = 42
+        with pytest.raises(XpbdError, match="not supported"):
+            ctx.set_colliders([c])
+    finally:
+        ctx.close()

@@ -25,7 +25,7 @@ from joltphysics_amd.xpbd import Context, build_settings
 pytestmark = pytest.mark.gpu
 
 TEAM_SCENES = ("cloth100_sphere", "cloth_hulls_streamed", "block_lra_boxes_streamed", "cloth_on_mesh_streamed",
-               "cloth_on_height_field_streamed")
+               "cloth_on_height_field_streamed", "cloth_on_compounds_streamed")
 
 
 def _run_team(sc, settings, bounds, frames, coloured=False):
