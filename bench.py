@@ -131,12 +131,17 @@ def measured_traffic(workload):
     return None, None
 
 
-def parity_check(ctx, ids, scene, settings, frame):
+DIGESTS = {"exact": "batched2048_sample_digests.json", "coloured": "batched2048_coloured_sample_digests.json"}
+
+
+def parity_check(ctx, ids, scene, settings, frame, schedule="exact"):
     """Compares the GPU state of the batched scene's 64 check bodies after `frame` frames
     from the initial state with digests of the reference's state (tests/golden/
     batched2048_sample_digests.json, written from the reference built with FMA
-    contraction off)."""
-    path = os.path.join(ROOT, "tests", "golden", "batched2048_sample_digests.json")
+    contraction off; for the coloured schedule batched2048_coloured_sample_digests.json, the
+    reference on the same settings with each run of each update group sorted by colour)."""
+    name = DIGESTS[schedule]
+    path = os.path.join(ROOT, "tests", "golden", name)
     if not os.path.exists(path):
         return {"checked": False, "reason": "no reference digests for this workload"}
     with open(path) as f:
@@ -154,7 +159,38 @@ def parity_check(ctx, ids, scene, settings, frame):
     mismatched = [k for k, g, w in zip(sample, got, want) if g != w]
     return {"checked": True, "frame": frame, "bodies": len(sample), "bit_exact": not mismatched,
             "mismatched_bodies": mismatched,
-            "against": "reference PhysicsSystem::Update (FMA off): tests/golden/batched2048_sample_digests.json"}
+            "against": "reference PhysicsSystem::Update (FMA off)" + (" on recoloured settings" if schedule != "exact" else "")
+                       + ": tests/golden/" + name}
+
+
+def timed_run(scene, settings, schedule, warmup, steps):
+    """Uploads the workload with the given schedule, runs `warmup` frames, checks the sampled
+    bodies against the reference's digests, then times `steps` frames in one xpbd_update call.
+    Returns (wall seconds, launches, kernel milliseconds, parity)."""
+    ctx = Context(0)
+    ctx.set_gravity(scene.gravity)
+    ctx.set_colliders(scene.colliders)
+    t0 = time.time()
+    ids = []
+    for b, s in zip(scene.bodies, settings):
+        st = ctx.create_settings(s, schedule)
+        ids.append(ctx.create_body(st, b.position, b.rotation, b.params))
+    log(f"uploaded ({schedule} schedule) in {time.time() - t0:.1f}s")
+    if warmup > 0:
+        ctx.update(scene.dt, scene.collision_steps, warmup)
+    ctx.synchronize()
+    parity = parity_check(ctx, ids, scene, settings, warmup, schedule)
+    log(f"parity after the warm-up: {parity}")
+    ctx.profile_enable(True)
+    t0 = time.perf_counter()
+    ctx.update(scene.dt, scene.collision_steps, steps)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    n_launch, kernel_ms = ctx.profile_read()
+    ctx.profile_enable(False)
+    ctx.close()
+    log(f"{schedule} schedule: {1000.0 * (t1 - t0) / steps:.3f} ms per frame")
+    return t1 - t0, n_launch, kernel_ms, parity
 
 
 def differentiation_run(warmup=1, repeats=3):
@@ -277,6 +313,9 @@ def main():
     ap.add_argument("--no-diff", action="store_true", help="skip the fwd+bwd differentiation measurement")
     ap.add_argument("--no-large", action="store_true", help="skip the large single body measurement")
     ap.add_argument("--only-large", action="store_true", help="only the large single body measurement (profiling)")
+    ap.add_argument("--schedule", default="exact", choices=("exact", "coloured", "coloured_tail"),
+                    help="constraint schedule of the timed workload (exact: the reference's order)")
+    ap.add_argument("--no-throughput", action="store_true", help="skip the coloured-schedule measurement")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB three times (the known-size reference "
                          "for HBM byte counters) and exit")
@@ -307,39 +346,16 @@ def main():
     log(f"workload {scene.name}: {len(settings)} bodies, {sum(len(s.vertices) for s in settings)} vertices, "
         f"{sum(s.num_constraints() for s in settings)} constraints, settings built in {time.time() - t0:.1f}s")
 
-    ctx = Context(0)
-    ctx.set_gravity(scene.gravity)
-    ctx.set_colliders(scene.colliders)
-    t0 = time.time()
-    ids = []
-    for b, s in zip(scene.bodies, settings):
-        st = ctx.create_settings(s)
-        ids.append(ctx.create_body(st, b.position, b.rotation, b.params))
-    log(f"uploaded in {time.time() - t0:.1f}s")
-
-    if args.warmup > 0:
-        ctx.update(scene.dt, scene.collision_steps, args.warmup)
-    ctx.synchronize()
-    parity = parity_check(ctx, ids, scene, settings, args.warmup)
-    log(f"parity after the warm-up: {parity}")
-    try:
-        import torch
-        if torch.cuda.is_available():
-            torch.cuda.synchronize()
-    except Exception:
-        torch = None
-
-    ctx.profile_enable(True)
-    t0 = time.perf_counter()
-    ctx.update(scene.dt, scene.collision_steps, args.steps)
-    ctx.synchronize()
-    t1 = time.perf_counter()
-    n_launch, kernel_ms = ctx.profile_read()
-    ctx.profile_enable(False)
-    wall = t1 - t0
-
+    wall, n_launch, kernel_ms, parity = timed_run(scene, settings, args.schedule, args.warmup, args.steps)
     solves = scenes.solves_per_frame(scene, settings)
     value = solves * args.steps / wall
+    throughput = None
+    if args.schedule == "exact" and not args.no_throughput:
+        # the coloured schedule on the same workload: fewer, wider level passes
+        t_wall, t_launch, t_kernel_ms, t_parity = timed_run(scene, settings, "coloured", args.warmup, args.steps)
+        throughput = {"schedule": "coloured", "value": solves * args.steps / t_wall, "unit": "constraint-solves/s",
+                      "ms_per_step": 1000.0 * t_wall / args.steps, "launch_ms": round(t_kernel_ms / max(t_launch, 1), 3),
+                      "parity": t_parity}
     bytes_frame = algorithmic_bytes_per_frame(scene, settings)
     achieved = bytes_frame * args.steps / (kernel_ms / 1000.0) / 1e9
     traffic, traffic_src = measured_traffic(scene.name)
@@ -351,7 +367,6 @@ def main():
                           "bodies), concurrent on one stream per hardware queue, timed fork to join",
                 "algorithmic_bytes_per_launch": bytes_frame * args.steps,
                 "launch_ms": round(kernel_ms / max(n_launch, 1), 3)}
-    ctx.close()
     diff = None if args.no_diff else differentiation_run()
     large = None
     if not args.no_large:
@@ -372,8 +387,9 @@ def main():
                    "constraints": int(sum(s.num_constraints() for s in settings)),
                    "solves_per_frame": int(solves), "collision_steps": scene.collision_steps,
                    "iterations": int(scene.bodies[0].params.num_iterations), "dt": scene.dt,
-                   "parity": parity},
+                   "schedule": args.schedule, "parity": parity},
         "roofline": roofline, "cpu_baseline": cpu, "differentiation": diff, "large_body": large,
+        "throughput_schedule": throughput,
     }
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 2)

@@ -164,12 +164,31 @@ typedef struct xpbd_settings_desc {
 } xpbd_settings_desc;
 
 int xpbd_settings_create(xpbd_context *ctx, const xpbd_settings_desc *desc, xpbd_settings **out_settings);
+/*
+ * Throughput schedules.  XPBD_SCHEDULE_EXACT is xpbd_settings_create's: every update group's
+ * constraints in the reference's order (SoftBodyMotionProperties.cpp:1030-1055).  The coloured
+ * schedules project the constraints of each run of one type in a group (all groups, or only the
+ * non-parallel last one) colour by colour: a constraint takes the lowest colour that no earlier
+ * constraint of the run holds on a shared vertex either of them writes.  The result equals the
+ * reference's on settings whose constraints are sorted (stably) by that colour within each run
+ * of each group, a reordering SoftBodySharedSettings::Optimize does not fix either; it differs
+ * from the reference on the given order by the rounding of the reordered Gauss-Seidel sweep.
+ * Groups with rod constraints keep the exact order.  No reference counterpart: for callers that
+ * trade the reference's constraint order for fewer level passes (many more constraints per pass).
+ */
+#define XPBD_SCHEDULE_EXACT 0
+#define XPBD_SCHEDULE_COLOURED_TAIL 1
+#define XPBD_SCHEDULE_COLOURED 2
+int xpbd_settings_create_scheduled(xpbd_context *ctx, const xpbd_settings_desc *desc, int32_t schedule,
+                                   xpbd_settings **out_settings);
 int xpbd_settings_release(xpbd_settings *settings);
 /* Schedule statistics: out[0]=parallel groups, out[1]=max levels of a parallel
  * group, out[2]=levels of the non-parallel group, out[3]=constraints of the
  * non-parallel group, out[4]=total constraints, out[5]=1 if the body keeps all
  * positions in LDS, out[6]/out[7]=parallel / non-parallel windows. */
 int xpbd_settings_stats(const xpbd_settings *settings, uint32_t out[8]);
+/* xpbd_schedule_build (below) with one of the XPBD_SCHEDULE_* schedules.  No reference counterpart. */
+int xpbd_schedule_build_scheduled(const xpbd_settings_desc *desc, int32_t schedule, xpbd_settings **out);
 /*
  * The level programs of settings built on the host alone, as xpbd_settings_create would
  * upload them (no device; for checks of the schedule).  xpbd_schedule_build returns settings

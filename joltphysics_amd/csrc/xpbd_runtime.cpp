@@ -367,6 +367,85 @@ std::vector<std::vector<uint32_t>> LevelGroup(const std::vector<Item> &seq, std:
     return out;
 }
 
+// The throughput schedules (XPBD_SCHEDULE_COLOURED, XPBD_SCHEDULE_COLOURED_TAIL): within each
+// run of one constraint type (ProcessGroup's order of the types is kept), a constraint takes the
+// lowest colour that no earlier constraint of the run holds on a vertex they share where either
+// of them writes (an LRA reads its vertex 0); each colour becomes one level, colours in order.
+// Constraints of one colour touch disjoint written vertices, so the levels give exactly what the
+// reference computes for settings whose constraints are sorted by colour (stably) within each
+// run of each update group.  False for a group with rod constraints or a run needing more than
+// 64 * kColourBanks colours (the caller keeps the exact levels).  wmask / rmask: nv * kColourBanks
+// words, zero on entry and on return.
+constexpr uint32_t kColourBanks = 4;
+bool ColourLevels(const std::vector<Item> &seq, std::vector<uint64_t> &wmask, std::vector<uint64_t> &rmask,
+                  std::vector<std::vector<uint32_t>> &out)
+{
+    out.clear();
+    std::vector<uint32_t> touched;
+    bool ok = true;
+    size_t k = 0;
+    while (k < seq.size() && ok)
+    {
+        const uint32_t type = seq[k].type;
+        if (type == kItemRodSS || type == kItemRodBT)
+        {
+            ok = false;
+            break;
+        }
+        size_t e = k;
+        while (e < seq.size() && seq[e].type == type)
+            ++e;
+        const bool lra = type == kTypeLRA;
+        const size_t base = out.size();
+        for (size_t i = k; i < e && ok; ++i)
+        {
+            const Item &it = seq[i];
+            uint32_t c = ~0u;
+            for (uint32_t bank = 0; bank < kColourBanks && c == ~0u; ++bank)
+            {
+                uint64_t forbidden = 0;
+                for (uint32_t j = 0; j < it.nv; ++j)
+                {
+                    const size_t m = (size_t)it.v[j] * kColourBanks + bank;
+                    forbidden |= wmask[m];
+                    if (!lra || j == 1)
+                        forbidden |= rmask[m];
+                }
+                if (~forbidden)
+                    c = bank * 64u + (uint32_t)__builtin_ctzll(~forbidden);
+            }
+            if (c == ~0u)
+            {
+                ok = false;
+                break;
+            }
+            for (uint32_t j = 0; j < it.nv; ++j)
+            {
+                const size_t m = (size_t)it.v[j] * kColourBanks + c / 64u;
+                ((!lra || j == 1) ? wmask : rmask)[m] |= 1ull << (c & 63u);
+                touched.push_back(it.v[j]);
+            }
+            if (out.size() <= base + c)
+                out.resize(base + c + 1);
+            out[base + c].push_back((uint32_t)i);
+        }
+        for (uint32_t v : touched)
+            for (uint32_t bank = 0; bank < kColourBanks; ++bank)
+                wmask[(size_t)v * kColourBanks + bank] = rmask[(size_t)v * kColourBanks + bank] = 0;
+        touched.clear();
+        k = e;
+    }
+    if (!ok)
+    {
+        for (const Item &it : seq)
+            for (uint32_t j = 0; j < it.nv; ++j)
+                for (uint32_t bank = 0; bank < kColourBanks; ++bank)
+                    wmask[(size_t)it.v[j] * kColourBanks + bank] = rmask[(size_t)it.v[j] * kColourBanks + bank] = 0;
+        out.clear();
+    }
+    return ok;
+}
+
 uint4 Pack(const Item &it, const uint32_t *local)
 {
     auto m = [&](uint32_t j) { return j < it.nv ? (local ? local[it.v[j]] : it.v[j]) : 0u; };
@@ -587,8 +666,10 @@ bool ColourGroup(const std::vector<Item> &seq, uint32_t nv, std::vector<uint4> &
     return true;
 }
 
-int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
+int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s, int32_t schedule)
 {
+    if (schedule < XPBD_SCHEDULE_EXACT || schedule > XPBD_SCHEDULE_COLOURED)
+        return Fail(XPBD_E_INVALID, "unknown schedule");
     const uint32_t nv = d.num_vertices;
     if (nv == 0 || d.vertices == nullptr)
         return Fail(XPBD_E_INVALID, "settings have no vertices");
@@ -665,6 +746,12 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
     const uint32_t ng = d.num_update_groups;
     std::vector<GroupLevels> gl(ng);
     std::vector<uint32_t> last_write(nv, 0), last_read(nv, 0), rod_last(d.num_rod_stretch_shear, 0);
+    std::vector<uint64_t> colour_w, colour_r;
+    if (schedule != XPBD_SCHEDULE_EXACT)
+    {
+        colour_w.assign((size_t)nv * kColourBanks, 0);
+        colour_r.assign((size_t)nv * kColourBanks, 0);
+    }
     prev = GroupRec{ 0, 0, 0, 0, 0, 0, 0 };
     for (uint32_t g = 0; g < ng; ++g)
     {
@@ -703,7 +790,9 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
                 continue;       // an LRA of a vertex to itself never moves it (that encoding marks skin records)
             seq.push_back({ kTypeLRA, { lras[k].v[0], lras[k].v[1], 0, 0 }, 2, lras[k].max_distance, 0.0f, k });
         }
-        gl[g].levels = LevelGroup(seq, last_write, last_read, rod_last);
+        const bool coloured = schedule == XPBD_SCHEDULE_COLOURED || (schedule == XPBD_SCHEDULE_COLOURED_TAIL && g + 1 == ng);
+        if (!coloured || !ColourLevels(seq, colour_w, colour_r, gl[g].levels))
+            gl[g].levels = LevelGroup(seq, last_write, last_read, rod_last);
         prev = c;
     }
     const uint32_t npar = ng - 1;
@@ -791,7 +880,39 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
             g = g_end;
         }
         s.dev.num_par_windows = npw;
-        // tail: consecutive levels per window
+        // tail: consecutive levels per window.  A coloured level may touch more vertices than a
+        // window holds: its constraints are independent, so it is cut into consecutive levels
+        // that fit.
+        std::vector<uint32_t> fresh_mark(nv, 0);
+        uint32_t fresh_stamp = 0;
+        if (schedule != XPBD_SCHEDULE_EXACT && !tail.levels.empty())
+        {
+            std::vector<std::vector<uint32_t>> cut;
+            for (const std::vector<uint32_t> &lv : tail.levels)
+            {
+                cut.emplace_back();
+                uint32_t count = 0;
+                ++fresh_stamp;
+                for (uint32_t k : lv)
+                {
+                    uint32_t add = 0;
+                    for (uint32_t j = 0; j < tail.seq[k].nv; ++j)
+                        add += fresh_mark[tail.seq[k].v[j]] != fresh_stamp ? 1u : 0u;
+                    if (count + add > kWindowCapacity)
+                    {
+                        cut.emplace_back();
+                        count = 0;
+                        ++fresh_stamp;
+                        add = tail.seq[k].nv;
+                    }
+                    for (uint32_t j = 0; j < tail.seq[k].nv; ++j)
+                        fresh_mark[tail.seq[k].v[j]] = fresh_stamp;
+                    count += add;
+                    cut.back().push_back(k);
+                }
+            }
+            gl[npar].levels = std::move(cut);
+        }
         uint32_t ntw = 0;
         uint32_t L = 0;
         const uint32_t ntl = (uint32_t)tail.levels.size();
@@ -802,16 +923,19 @@ int BuildSchedule(const xpbd_settings_desc &d, xpbd_settings &s)
             uint32_t L_end = L;
             while (L_end < ntl)
             {
-                std::vector<uint32_t> lv;
                 uint32_t sub = stamp;
                 // count the new vertices of this level without committing them
                 std::vector<uint32_t> fresh;
+                ++fresh_stamp;
                 for (uint32_t k : tail.levels[L_end])
                     for (uint32_t j = 0; j < tail.seq[k].nv; ++j)
                     {
                         uint32_t v = tail.seq[k].v[j];
-                        if (mark[v] != sub && std::find(fresh.begin(), fresh.end(), v) == fresh.end())
+                        if (mark[v] != sub && fresh_mark[v] != fresh_stamp)
+                        {
+                            fresh_mark[v] = fresh_stamp;
                             fresh.push_back(v);
+                        }
                     }
                 if (!verts.empty() && verts.size() + fresh.size() > kWindowCapacity)
                     break;
@@ -1611,7 +1735,7 @@ int xpbd_context_destroy(xpbd_context *ctx)
 
 void *xpbd_context_stream(xpbd_context *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
-int xpbd_settings_create(xpbd_context *ctx, const xpbd_settings_desc *desc, xpbd_settings **out_settings)
+int xpbd_settings_create_scheduled(xpbd_context *ctx, const xpbd_settings_desc *desc, int32_t schedule, xpbd_settings **out_settings)
 {
     if (!ctx || !desc || !out_settings)
         return Fail(XPBD_E_INVALID, "null argument");
@@ -1619,25 +1743,35 @@ int xpbd_settings_create(xpbd_context *ctx, const xpbd_settings_desc *desc, xpbd
     HIP_TRY(hipSetDevice(ctx->device));
     std::unique_ptr<xpbd_settings> s(new xpbd_settings);
     s->ctx = ctx;
-    int rc = BuildSchedule(*desc, *s);
+    int rc = BuildSchedule(*desc, *s, schedule);
     if (rc != XPBD_OK)
         return rc;
     *out_settings = s.release();
     return XPBD_OK;
 }
 
-int xpbd_schedule_build(const xpbd_settings_desc *desc, xpbd_settings **out)
+int xpbd_settings_create(xpbd_context *ctx, const xpbd_settings_desc *desc, xpbd_settings **out_settings)
+{
+    return xpbd_settings_create_scheduled(ctx, desc, XPBD_SCHEDULE_EXACT, out_settings);
+}
+
+int xpbd_schedule_build_scheduled(const xpbd_settings_desc *desc, int32_t schedule, xpbd_settings **out)
 {
     if (!desc || !out)
         return Fail(XPBD_E_INVALID, "null argument");
     *out = nullptr;
     std::unique_ptr<xpbd_settings> s(new xpbd_settings);
     s->host_schedule.reset(new xpbd_settings::HostSchedule);
-    int rc = BuildSchedule(*desc, *s);
+    int rc = BuildSchedule(*desc, *s, schedule);
     if (rc != XPBD_OK)
         return rc;
     *out = s.release();
     return XPBD_OK;
+}
+
+int xpbd_schedule_build(const xpbd_settings_desc *desc, xpbd_settings **out)
+{
+    return xpbd_schedule_build_scheduled(desc, XPBD_SCHEDULE_EXACT, out);
 }
 
 int xpbd_schedule_sizes(const xpbd_settings *s, uint32_t out[10])
@@ -2217,10 +2351,11 @@ int xpbd_update(xpbd_context *ctx, float dt, int32_t collision_steps, int32_t fr
     if (frames == 0 || ctx->bodies.empty())
         return XPBD_OK;
     HIP_TRY(hipSetDevice(ctx->device));
-    const uint32_t nc = (uint32_t)ctx->colliders.size();
+    const uint32_t nc = (uint32_t)ctx->colliders.size();     // leaf shapes
+    const uint32_t num_bodies = (uint32_t)ctx->collider_first.size();
     int rc;
     for (size_t i = 0; i < ctx->bodies.size(); ++i)
-        if (ctx->bodies[i]->pair_n && (ctx->bodies[i]->pair_n != nc || ctx->bodies[i]->pair_sig != ctx->collider_sig))
+        if (ctx->bodies[i]->pair_n && (ctx->bodies[i]->pair_n != num_bodies || ctx->bodies[i]->pair_sig != ctx->collider_sig))
             return Fail(XPBD_E_INVALID, "body " + std::to_string(i) + " has contact settings for another collider list");
     // skin states go up before the layout, whose bundles point at them
     if ((rc = PushSkin(ctx)) != XPBD_OK)

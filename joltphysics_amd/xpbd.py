@@ -126,7 +126,8 @@ class _BuildDesc(C.Structure):
 # Exported symbols of the C ABI, in declaration order of include/jolt_xpbd.h.
 EXPORTED_SYMBOLS = (
     "xpbd_last_error", "xpbd_context_create", "xpbd_context_destroy", "xpbd_context_stream",
-    "xpbd_settings_create", "xpbd_settings_release", "xpbd_settings_stats", "xpbd_schedule_build",
+    "xpbd_settings_create", "xpbd_settings_create_scheduled", "xpbd_settings_release", "xpbd_settings_stats",
+    "xpbd_schedule_build_scheduled", "xpbd_schedule_build",
     "xpbd_schedule_sizes", "xpbd_schedule_read", "xpbd_settings_team_stats",
     "xpbd_settings_build", "xpbd_built_counts", "xpbd_built_read", "xpbd_built_free",
     "xpbd_build_last_error", "xpbd_body_create",
@@ -171,6 +172,8 @@ def load_library(path=LIB_PATH):
     lib.xpbd_settings_stats.argtypes = [vp, C.POINTER(u32)]
     lib.xpbd_settings_team_stats.argtypes = [vp, C.POINTER(u32)]
     lib.xpbd_schedule_build.argtypes = [C.POINTER(_SettingsDesc), C.POINTER(vp)]
+    lib.xpbd_schedule_build_scheduled.argtypes = [C.POINTER(_SettingsDesc), i32, C.POINTER(vp)]
+    lib.xpbd_settings_create_scheduled.argtypes = [vp, C.POINTER(_SettingsDesc), i32, C.POINTER(vp)]
     lib.xpbd_schedule_sizes.argtypes = [vp, C.POINTER(u32)]
     lib.xpbd_schedule_read.argtypes = [vp] * 11
     lib.xpbd_context_set_team_min_windows.argtypes = [vp, u32]
@@ -251,15 +254,18 @@ def _settings_desc(s: SettingsArrays):
     return d, (v, f, e, b, t, lr, g, sk, skn, ib, rs, rb)
 
 
-def host_schedule(s: SettingsArrays) -> dict:
+SCHEDULES = {"exact": 0, "coloured_tail": 1, "coloured": 2}     # XPBD_SCHEDULE_*
+
+
+def host_schedule(s: SettingsArrays, schedule="exact") -> dict:
     """The level programs the device would run for these settings, built on the host
-    (xpbd_schedule_build; no device needed): records (n, 4) uint32, their constraint
+    (xpbd_schedule_build_scheduled; no device needed): records (n, 4) uint32, their constraint
     indices, level offsets, windows (n, 4), the gathered windows' vertex list, the coloured
     non-parallel group and the identity program bundles merge."""
     lib = load_library()
     d, _keep = _settings_desc(s)
     h = C.c_void_p()
-    rc = lib.xpbd_schedule_build(C.byref(d), C.byref(h))
+    rc = lib.xpbd_schedule_build_scheduled(C.byref(d), SCHEDULES[schedule], C.byref(h))
     if rc != XPBD_OK:
         raise XpbdError(f"xpbd error {rc}: {lib.xpbd_last_error().decode()}")
     try:
@@ -502,10 +508,12 @@ class Context:
                                                   upd.ctypes.data))
         return lin[:n], ang[:n], upd[:n]
 
-    def create_settings(self, s: SettingsArrays) -> Settings:
+    def create_settings(self, s: SettingsArrays, schedule="exact") -> Settings:
+        """Uploads settings; `schedule` "exact" (the reference's constraint order), "coloured"
+        or "coloured_tail" (the throughput schedules of xpbd_settings_create_scheduled)."""
         d, _keep = _settings_desc(s)
         h = C.c_void_p()
-        _check(self.lib.xpbd_settings_create(self.h, C.byref(d), C.byref(h)))
+        _check(self.lib.xpbd_settings_create_scheduled(self.h, C.byref(d), SCHEDULES[schedule], C.byref(h)))
         st = Settings(self, h, s)
         self._keep.append(st)
         return st

@@ -67,10 +67,12 @@ def save_config1():
     print("wrote", path, os.path.getsize(path), "bytes")
 
 
-def save_batched_digests():
+def save_batched_digests(coloured=False):
     """The batched 2048-body scene: digests of 64 sampled bodies (scenes.batched_check_sample)
     after each of the first 8 frames.  Settings are the reference's own; the bodies run in
-    a world of their own (soft bodies never interact)."""
+    a world of their own (soft bodies never interact).  coloured: the reference runs each
+    sampled body's settings with every run of every update group sorted by colour (the order
+    the coloured schedule runs, tests/helpers.py), the sample chosen on the given settings."""
     sc = scenes.config3_batched(frames=max(BATCHED_FRAMES))
     ref = JoltReference("nofma")
     cache = {}
@@ -82,6 +84,11 @@ def save_batched_digests():
         b.post(s)
         settings.append(s)
     sample = scenes.batched_check_sample(sc, settings)
+    if coloured:
+        sys.path.insert(0, os.path.dirname(HERE))
+        from helpers import recoloured_settings
+        for k in sample:
+            settings[k] = recoloured_settings(settings[k])[0]
     world = JoltWorld(ref, sc.gravity, num_threads=max(0, min(os.cpu_count() or 1, 16) - 1))
     for c in sc.colliders:
         world.add_collider(c)
@@ -94,9 +101,10 @@ def save_batched_digests():
         digests[str(f)] = [scenes.state_digest(world.read(i)) for i in range(len(sample))]
     world.close()
     out = {"scene": sc.name, "bodies": len(sc.bodies), "sample": [int(k) for k in sample],
-           "reference": "PhysicsSystem::Update of the reference built with -ffp-contract=off",
+           "reference": "PhysicsSystem::Update of the reference built with -ffp-contract=off"
+                        + (", settings recoloured" if coloured else ""),
            "digest": "sha256 of float32 pos, vel, body_pos", "frames": digests}
-    path = os.path.join(HERE, "batched2048_sample_digests.json")
+    path = os.path.join(HERE, "batched2048_coloured_sample_digests.json" if coloured else "batched2048_sample_digests.json")
     with open(path, "w") as f:
         json.dump(out, f, indent=0)
     print("wrote", path, os.path.getsize(path), "bytes")
@@ -108,3 +116,5 @@ if __name__ == "__main__":
         save_config1()
     if "batched" in what:
         save_batched_digests()
+    if "batched_coloured" in what:
+        save_batched_digests(coloured=True)

@@ -20,7 +20,7 @@ def load_config1_fixture():
     return fx, s
 
 
-def run_gpu(scene, settings_list, bounds=None, frames_to_keep=(1,), device=0):
+def run_gpu(scene, settings_list, bounds=None, frames_to_keep=(1,), device=0, schedule="exact"):
     """Runs a scene on the GPU through the C ABI; returns {frame: [read(body)...]}."""
     from joltphysics_amd.xpbd import Context
     ctx = Context(device)
@@ -33,7 +33,7 @@ def run_gpu(scene, settings_list, bounds=None, frames_to_keep=(1,), device=0):
     ctx.set_colliders(cols)
     ids = []
     for b, s in zip(scene.bodies, settings_list):
-        st = ctx.create_settings(s)
+        st = ctx.create_settings(s, schedule)
         ids.append(ctx.create_body(st, b.position, b.rotation, b.params))
     out = {}
     done = 0
@@ -257,3 +257,50 @@ class GpuRunner:
 
     def close(self):
         self.ctx.close()
+
+
+def colour_permutation(s):
+    """The constraint order XPBD_SCHEDULE_COLOURED runs: within each update group, each type's
+    constraints (volumes, bends, edges, LRAs: ProcessGroup's runs) sorted stably by a greedy
+    colour, a constraint taking the lowest colour that no earlier constraint of the run holds on
+    a vertex they share where either writes (an LRA reads its vertex 0 and writes vertex 1; an
+    LRA of a vertex to itself, never projected, goes last).  Returns {array name: permutation}."""
+    g = s.groups
+    names = (("volumes", "volume_end", 4), ("bends", "bend_end", 4), ("edges", "edge_end", 2), ("lras", "lra_end", 2))
+    perm = {n: [] for n, _, _ in names}
+    prev = {key: 0 for _, key, _ in names}
+    for row in g:
+        for name, key, nvert in names:
+            arr = getattr(s, name)
+            lra = name == "lras"
+            wcol, rcol = {}, {}
+            coloured, idle = [], []
+            for k in range(prev[key], int(row[key])):
+                v = [int(x) for x in arr["v"][k][:nvert]]
+                if lra and v[0] == v[1]:
+                    idle.append(k)
+                    continue
+                forbidden = set()
+                for j, x in enumerate(v):
+                    forbidden |= wcol.get(x, set())
+                    if not lra or j == 1:
+                        forbidden |= rcol.get(x, set())
+                c = 0
+                while c in forbidden:
+                    c += 1
+                for j, x in enumerate(v):
+                    ((wcol if not lra or j == 1 else rcol).setdefault(x, set())).add(c)
+                coloured.append((c, k))
+            perm[name].extend([k for _, k in sorted(coloured, key=lambda ck: ck[0])] + idle)
+            prev[key] = int(row[key])
+    return {n: np.array(p, np.int64) for n, p in perm.items()}
+
+
+def recoloured_settings(s):
+    """Settings whose constraints are in the order XPBD_SCHEDULE_COLOURED runs them (same update
+    groups): the reference on these equals the coloured schedule on the original settings."""
+    p = colour_permutation(s)
+    r = s.copy()
+    for name, order in p.items():
+        setattr(r, name, getattr(s, name)[order])
+    return r, p

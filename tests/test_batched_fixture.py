@@ -81,3 +81,24 @@ def test_plane_world_bounds_are_the_references():
         assert scenes.plane_world_bounds(c) == world.collider_bounds(world.add_collider(c))
     finally:
         world.close()
+
+
+def test_restatement_reproduces_coloured_digests(batched):
+    """The coloured-schedule digests (batched2048_coloured_sample_digests.json, the reference on
+    recoloured settings): the restatement on the same recoloured settings reproduces a spread of
+    them, and they differ from the exact order's digests."""
+    from helpers import recoloured_settings
+    fx, sc, settings = batched
+    with open(os.path.join(GOLDEN, "batched2048_coloured_sample_digests.json")) as f:
+        cx = json.load(f)
+    assert cx["sample"] == fx["sample"]
+    sample = cx["sample"]
+    pick = sorted(set(sample[1::8]))
+    frames = (1, 2, 3)
+    sub = scenes.Scene(sc.name, [sc.bodies[k] for k in pick], sc.colliders, sc.gravity, sc.dt, sc.collision_steps, 3)
+    bounds = np.array([c.bounds for c in sc.colliders], np.float32)
+    out = run_restatement(sub, [recoloured_settings(settings[k])[0] for k in pick], bounds, frames_to_keep=frames)
+    for f in frames:
+        for j, k in enumerate(pick):
+            assert scenes.state_digest(out[f][j]) == cx["frames"][str(f)][sample.index(k)], (f, k)
+    assert sum(a != b for a, b in zip(cx["frames"]["3"], fx["frames"]["3"])) > len(sample) // 2

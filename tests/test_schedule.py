@@ -208,3 +208,44 @@ def test_checker_catches_a_reordered_level():
     keep = np.arange(len(t)) != 5
     with pytest.raises(AssertionError, match="missing"):
         check_program(s, t[keep], idx[keep], level[keep], verts[keep], p0[keep], p1[keep])
+
+
+@pytest.mark.parametrize("name", list(MESHES))
+def test_coloured_schedule_is_the_reference_order_of_recoloured_settings(name):
+    """XPBD_SCHEDULE_COLOURED: the program follows, as checked above, the sequential order of
+    the same settings with each run of each group sorted by colour (tests/helpers.py), so the
+    device's results under it are the reference's on those settings bit for bit.  It takes far
+    fewer levels than the exact program."""
+    from helpers import recoloured_settings
+    s = build_settings(MESHES[name]())
+    r, perm = recoloured_settings(s)
+    inverse = {}
+    for ty, key in ((TYPE_VOLUME, "volumes"), (TYPE_BEND, "bends"), (TYPE_EDGE, "edges"), (TYPE_LRA, "lras")):
+        inv = np.zeros(len(perm[key]), np.int64)
+        inv[perm[key]] = np.arange(len(perm[key]))
+        inverse[ty] = inv
+    h = host_schedule(s, "coloured")
+
+    def remap(t, idx, level, verts, p0, p1):
+        idx = idx.copy()
+        for ty, inv in inverse.items():
+            sel = t == ty
+            idx[sel] = inv[idx[sel]]
+        return t, idx, level, verts, p0, p1
+
+    n, nlev = check_program(r, *remap(*windowed_program(h)))
+    _, nlev_exact = check_program(s, *windowed_program(host_schedule(s)))
+    print(f"{name}: {n} constraints in {nlev} coloured levels ({nlev_exact} exact)", flush=True)
+    assert nlev * 3 < nlev_exact
+    if len(h["identity_level_begin"]):
+        ident = levels_of(h["identity_recs"], h["identity_rec_index"], [(0, h["identity_level_begin"])], [lambda v: v])
+        check_program(r, *remap(*ident))
+
+
+def test_coloured_tail_schedule_colours_only_the_last_group():
+    """XPBD_SCHEDULE_COLOURED_TAIL keeps the parallel groups' levels and colours the last group."""
+    s = build_settings(MESHES["cloth127_streamed"]())
+    exact, tail = host_schedule(s), host_schedule(s, "coloured_tail")
+    npw = exact["num_par_windows"]
+    assert np.array_equal(exact["windows"][:npw, 3], tail["windows"][:npw, 3])
+    assert tail["windows"][npw:, 3].sum() * 10 < exact["windows"][npw:, 3].sum()
