@@ -353,8 +353,11 @@ def main():
     if args.schedule == "exact" and not args.no_throughput:
         # the coloured schedule on the same workload: fewer, wider level passes
         t_wall, t_launch, t_kernel_ms, t_parity = timed_run(scene, settings, "coloured", args.warmup, args.steps)
+        t_achieved = algorithmic_bytes_per_frame(scene, settings) * args.steps / (t_kernel_ms / 1000.0) / 1e9
         throughput = {"schedule": "coloured", "value": solves * args.steps / t_wall, "unit": "constraint-solves/s",
                       "ms_per_step": 1000.0 * t_wall / args.steps, "launch_ms": round(t_kernel_ms / max(t_launch, 1), 3),
+                      "roofline": {"bound": "hbm", "achieved": round(t_achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": round(t_achieved / HBM_PEAK_GBS, 4)},
                       "parity": t_parity}
     bytes_frame = algorithmic_bytes_per_frame(scene, settings)
     achieved = bytes_frame * args.steps / (kernel_ms / 1000.0) / 1e9
@@ -393,6 +396,8 @@ def main():
     }
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 2)
+        if throughput:
+            throughput["gpu_over_cpu"] = round(throughput["value"] / cpu["value"], 2)
     print(json.dumps(out), flush=True)
 
 
