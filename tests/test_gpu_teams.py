@@ -28,7 +28,7 @@ TEAM_SCENES = ("cloth100_sphere", "cloth_hulls_streamed", "block_lra_boxes_strea
                "cloth_on_height_field_streamed", "cloth_on_compounds_streamed")
 
 
-def _run_team(sc, settings, bounds, frames, coloured=False):
+def _run_team(sc, settings, bounds, frames, coloured=False, schedule="exact"):
     ctx = Context(0)
     try:
         ctx.set_gravity(sc.gravity)
@@ -36,7 +36,7 @@ def _run_team(sc, settings, bounds, frames, coloured=False):
             c.bounds = tuple(float(x) for x in b)
         ctx.set_colliders(sc.colliders)
         ctx.set_team_min_windows(1)
-        ids = [ctx.create_body(ctx.create_settings(s), b.position, b.rotation, b.params)
+        ids = [ctx.create_body(ctx.create_settings(s, schedule), b.position, b.rotation, b.params)
                for b, s in zip(sc.bodies, settings)]
         if coloured:
             for i in ids:
@@ -137,6 +137,25 @@ def test_team_coloured_equals_reference_on_recoloured_settings():
     finally:
         ctx.close()
     assert st["tail_colourable"] and st["tail_colours"] == colours[0], (st, colours)
+    for f in frames:
+        g, w = gpu[f][0], want[f][0]
+        print(f"  frame {f} {g['placement']}: max |dpos| {max_abs(g['pos'], w['pos']):.3e}", flush=True)
+        assert g["placement"][0] == "team"
+        assert float_equal(g["pos"], w["pos"]) and float_equal(g["vel"], w["vel"]), f"frame {f}"
+        assert float_equal(g["body_pos"], w["body_pos"]), f"frame {f}"
+
+
+def test_team_coloured_schedule_equals_reference_on_recoloured_settings():
+    """Settings under XPBD_SCHEDULE_COLOURED on a team with the coloured non-parallel group
+    (the fastest path for a large body): every group coloured, the last one as the team's colour
+    passes; equal to the reference on the settings with every run of every group sorted by
+    colour."""
+    from helpers import recoloured_settings as recolour_all
+    sc = scene_zoo.block_lra_boxes_streamed(frames=2)
+    frames = (1, 2)
+    settings, bounds, _ = reference_run(sc, (1,))
+    _, _, want = reference_run(sc, frames, settings=[recolour_all(s)[0] for s in settings])
+    gpu = _run_team(sc, settings, bounds, frames, coloured=True, schedule="coloured")
     for f in frames:
         g, w = gpu[f][0], want[f][0]
         print(f"  frame {f} {g['placement']}: max |dpos| {max_abs(g['pos'], w['pos']):.3e}", flush=True)
