@@ -280,9 +280,9 @@ inline bool FlattenHeightField(const JPH::HeightFieldShape &hf, ShapeData &out, 
         return false;
     }
     const uint32_t num_blocks = sample_count / block_size;
-    if (num_blocks < 2 || (num_blocks & (num_blocks - 1)) != 0)
+    if (num_blocks < 2)
     {
-        err = "height field with a block count that is not a power of two";
+        err = "height field of one block";
         return false;
     }
     const uint32_t max_level = 32 - CountLeadingZeros(num_blocks - 1);
@@ -334,9 +334,23 @@ inline bool FlattenHeightField(const JPH::HeightFieldShape &hf, ShapeData &out, 
             {
                 n.min_x[k] = bounds_minx[k]; n.min_y[k] = bounds_miny[k]; n.min_z[k] = bounds_minz[k];
                 n.max_x[k] = bounds_maxx[k]; n.max_y[k] = bounds_maxy[k]; n.max_z[k] = bounds_maxz[k];
+                // With a block count that is not a power of two, the last range blocks have
+                // children past the last block: the shape stores them with min > max, so the
+                // walk never enters them (their distance is FLT_MAX); they get no node.
+                const uint32_t cx = 2 * it.x + (k & 1), cy = 2 * it.y + (k >> 1), shift = max_level - (it.level + 1);
+                if ((cx << shift) >= num_blocks || (cy << shift) >= num_blocks)
+                {
+                    if (mm[k] <= mm[4 + k])
+                    {
+                        err = "height field range block past the last block with valid bounds";
+                        return false;
+                    }
+                    n.child[k] = XPBD_TREE_EMPTY;
+                    continue;
+                }
                 const uint32_t child = new_node();
                 n.child[k] = XPBD_TREE_NODE | child;
-                todo.push_back({ it.level + 1, 2 * it.x + (k & 1), 2 * it.y + (k >> 1), child });
+                todo.push_back({ it.level + 1, cx, cy, child });
             }
         }
         else

@@ -260,6 +260,14 @@ def cloth_on_height_field(n=40, frames=30, spacing=0.05, height=0.25, iterations
                  collision_steps=collision_steps, frames=frames)
 
 
+def cloth_on_height_field_uneven():
+    """The height field scene on a terrain of 40 x 40 samples in blocks of 4: 10 blocks a side,
+    not a power of two (the range-block hierarchy has children past the last block)."""
+    sc = cloth_on_height_field(samples=40, block=4)
+    sc.name = "cloth_on_height_field_uneven"
+    return sc
+
+
 def cloth_on_height_field_streamed():
     """The height field scene with a 120 x 120 cloth (14,400 vertices), a few frames."""
     return cloth_on_height_field(n=120, frames=8, spacing=0.02, height=0.05, iterations=4, collision_steps=1)
@@ -360,6 +368,7 @@ ALL = {
     "cloth_on_mesh_streamed": cloth_on_mesh_streamed,
     "cloth_on_height_field": cloth_on_height_field,
     "cloth_on_height_field_streamed": cloth_on_height_field_streamed,
+    "cloth_on_height_field_uneven": cloth_on_height_field_uneven,
     "cloth_on_compounds": cloth_on_compounds,
     "cloth_on_compounds_streamed": cloth_on_compounds_streamed,
 }

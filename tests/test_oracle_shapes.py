@@ -284,7 +284,7 @@ def test_flattened_mesh_holds_every_triangle_once(cells, per_leaf):
 
 
 @pytest.mark.skipif(not _have_reference(), reason="reference library not built (needs the reference sources)")
-@pytest.mark.parametrize("samples,block", [(16, 2), (32, 4), (64, 8)])
+@pytest.mark.parametrize("samples,block", [(16, 2), (32, 4), (64, 8), (24, 4), (40, 4), (36, 2)])
 def test_flattened_height_field_keeps_the_cells_with_collision(samples, block):
     # per cell (x, y) the walk emits its first triangle when samples (x, y), (x+1, y+1) and
     # (x, y+1) have collision, its second when (x, y), (x+1, y+1) and (x+1, y) do
