@@ -1,4 +1,4 @@
-// Differentiable rollout of the soft-body update for bodies with edge
+// Differentiable rollout of the soft-body update for bodies with edge and LRA
 // constraints: float and double forward passes and the hand-written adjoint.
 //
 // The forward pass follows the update of the main kernels step for step
@@ -102,6 +102,44 @@ __device__ __forceinline__ void edge_project(R4<T> *X, uint32_t i0, uint32_t i1,
     X[i1] = r4(x1 - w1 * correction, w1);
 }
 
+// ApplyLRAConstraints body (SoftBodyMotionProperties.cpp:685-697): only vertex 1 moves
+template <class T>
+__device__ __forceinline__ void lra_project(R4<T> *X, uint32_t i0, uint32_t i1, T max_distance)
+{
+    const R3<T> x0 = xyz3(X[i0]);
+    const R4<T> p1 = X[i1];
+    const R3<T> delta = xyz3(p1) - x0;
+    const T delta_len_sq = rdot(delta, delta);
+    if (delta_len_sq > max_distance * max_distance)
+        X[i1] = r4(x0 + (delta * max_distance) / rsqrt_exact(delta_len_sq), p1.w);
+}
+
+// Adjoint of lra_project from its pre-projection positions q0, q1: when the constraint acts,
+// x1' = x0 + m u with u = delta / |delta|, so with a1 = x1'bar
+//   x1bar = (m / |delta|) (a1 - u (u . a1)),  x0bar = x0'bar + a1 - x1bar.
+// Vertex 0 is only read in the forward pass, so the LRAs of one level may share it (all LRAs
+// of a pinned corner do): their x0bar terms are added atomically.  Nothing else of the level
+// touches it.
+__device__ __forceinline__ void lra_adjoint(float4 *A, uint32_t i0, uint32_t i1, float4 q0, float4 q1, float max_distance)
+{
+    const R3<float> delta = r3(q1.x - q0.x, q1.y - q0.y, q1.z - q0.z);
+    const float delta_len_sq = rdot(delta, delta);
+    if (!(delta_len_sq > max_distance * max_distance))
+        return;
+    const float length = sqrtf(delta_len_sq);
+    const R3<float> u = delta / length;
+    const float4 a1 = A[i1];
+    const R3<float> ab1 = r3(a1.x, a1.y, a1.z);
+    const R3<float> g = (max_distance / length) * (ab1 - rdot(u, ab1) * u);
+    A[i1] = make_float4(g.x, g.y, g.z, a1.w);
+    atomicAdd(&A[i0].x, ab1.x - g.x);
+    atomicAdd(&A[i0].y, ab1.y - g.y);
+    atomicAdd(&A[i0].z, ab1.z - g.z);
+}
+
+// Record type of the level programs (bits 14-15 of the first word): 3 for an LRA
+__device__ __forceinline__ bool is_lra_record(const uint4 &q) { return ((q.x >> 14) & 3u) == 3u; }
+
 // Adjoint of edge_project from its pre-projection positions q0, q1.
 __device__ __forceinline__ void edge_adjoint(float4 *A, uint32_t i0, uint32_t i1, float4 q0, float4 q1, float rest,
                                              float compliance, float inv_dt_sq, float &alpha_bar)
@@ -154,7 +192,10 @@ __device__ __forceinline__ void forward_substep(const RolloutBodyDev &B, const S
     run_program_global<kRolloutBlock, false>(B.recs, B.lvl, B.nlev, tid, [&](const uint4 &q, uint32_t) {
         if (q.x == kNopRecord)
             return;
-        edge_project<T>(X, q.x & 0x3fffu, q.x >> 16, (T)__uint_as_float(q.z), comp(q), K.inv_dt_sq);
+        if (is_lra_record(q))
+            lra_project<T>(X, q.x & 0x3fffu, q.x >> 16, (T)__uint_as_float(q.z));
+        else
+            edge_project<T>(X, q.x & 0x3fffu, q.x >> 16, (T)__uint_as_float(q.z), comp(q), K.inv_dt_sq);
     });
     for (uint32_t i = tid; i < nv; i += kRolloutBlock)
     {
@@ -194,7 +235,7 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_forward_kernel(RolloutD
         for (uint32_t r = tid; r < B.num_records; r += kRolloutBlock)
         {
             uint4 q = recs[r];
-            if (q.x != kNopRecord)
+            if (q.x != kNopRecord && !is_lra_record(q))
             {
                 q.w = __float_as_uint((float)compliance[B.eoff + q.y]);
                 recs[r] = q;
@@ -313,7 +354,10 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_backward_kernel(Rollout
                 const uint32_t i0 = q.x & 0x3fffu, i1 = q.x >> 16;
                 tape[2u * r] = A[i0];
                 tape[2u * r + 1u] = A[i1];
-                edge_project<float>(X, i0, i1, __uint_as_float(q.z), __uint_as_float(q.w), K.inv_dt_sq);
+                if (is_lra_record(q))
+                    lra_project<float>(X, i0, i1, __uint_as_float(q.z));
+                else
+                    edge_project<float>(X, i0, i1, __uint_as_float(q.z), __uint_as_float(q.w), K.inv_dt_sq);
             });
             // adjoint of the velocity update (and of the clamp after the last iteration)
             for (uint32_t i = tid; i < nv; i += kRolloutBlock)
@@ -346,6 +390,11 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_backward_kernel(Rollout
             run_program_global<kRolloutBlock, true>(B.recs, B.lvl, B.nlev, tid, [&](const uint4 &q, uint32_t r) {
                 if (q.x == kNopRecord)
                     return;
+                if (is_lra_record(q))
+                {
+                    lra_adjoint(A, q.x & 0x3fffu, q.x >> 16, tape[2u * r], tape[2u * r + 1u], __uint_as_float(q.z));
+                    return;
+                }
                 edge_adjoint(A, q.x & 0x3fffu, q.x >> 16, tape[2u * r], tape[2u * r + 1u], __uint_as_float(q.z),
                              __uint_as_float(q.w), K.inv_dt_sq, alpha_bar[q.y]);
             });

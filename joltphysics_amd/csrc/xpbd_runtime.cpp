@@ -3141,8 +3141,8 @@ int xpbd_rollout_create(xpbd_context *ctx, const xpbd_rollout_desc *d, xpbd_roll
         xpbd_settings *s = d->settings[b];
         if (!s || s->ctx != ctx)
             return Fail(XPBD_E_INVALID, "settings " + std::to_string(b) + " missing or from another context");
-        if (s->num_bends || s->num_volumes || s->num_lras)
-            return Fail(XPBD_E_UNSUPPORTED, "rollout bodies may have edge constraints only (body " + std::to_string(b) + ")");
+        if (s->num_bends || s->num_volumes)
+            return Fail(XPBD_E_UNSUPPORTED, "rollout bodies may have edge and LRA constraints only (body " + std::to_string(b) + ")");
         if (s->host_level_begin.empty())
             return Fail(XPBD_E_CAPACITY, "rollout body " + std::to_string(b) + " does not fit in LDS");
         const uint32_t nv = s->num_vertices, ne = s->num_edges;

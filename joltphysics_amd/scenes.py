@@ -18,7 +18,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .records import (BEND_DIHEDRAL, BEND_NONE, EDGE, LRA, LRA_NONE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_COMPOUND,
+from .records import (BEND_DIHEDRAL, BEND_NONE, EDGE, LRA, LRA_EUCLIDEAN, LRA_NONE, SHAPE_BOX, SHAPE_CAPSULE, SHAPE_COMPOUND,
                       SHAPE_CYLINDER, SHAPE_CONVEX_HULL, SHAPE_MESH, SHAPE_PLANE, SHAPE_SPHERE, SHAPE_TAPERED_CAPSULE,
                       SHAPE_TAPERED_CYLINDER, VERTEX, VOLUME, BodyParams, Collider, tapered_capsule_device_params)
 
@@ -529,13 +529,16 @@ class RolloutCase:
     checkpoint_interval: int = 10
 
 
-def config4_rollout(num_bodies=64, n=64, frames=60, seed=4, compliance=1e-3):
+def config4_rollout(num_bodies=64, n=64, frames=60, seed=4, compliance=1e-3, lra=False):
     """64 cloths of 64 x 64 vertices (spacing 0.05 m, inv-mass 100, two top corners
     pinned), structural + shear edges only (compliance 1e-3), gravity -9.81 z, dt 1/60,
     one collision step x 5 iterations, 60 frames, no colliders.  Initial velocities
-    U(-0.1, 0.1) m/s and target compliance multipliers U(0.5, 2.0) per edge, from `seed`."""
+    U(-0.1, 0.1) m/s and target compliance multipliers U(0.5, 2.0) per edge, from `seed`.
+    lra: also Euclidean LRA constraints from the pinned corners (max distance 1.0 x the rest
+    distance), which act once the sheet stretches."""
     v, f = cloth(n, 0.05, 100.0, pinned=((n - 1) * n, n * n - 1))
-    mesh = MeshSpec(vertices=v, faces=f, attrs=attrs(compliance, compliance, 0.0), bend_type=BEND_NONE)
+    a = attrs(compliance, compliance, 0.0, lra_type=LRA_EUCLIDEAN, lra_mult=1.0) if lra else attrs(compliance, compliance, 0.0)
+    mesh = MeshSpec(vertices=v, faces=f, attrs=a, bend_type=BEND_NONE)
     rng = np.random.Generator(np.random.PCG64(seed))
     num_edges = 2 * n * (n - 1) + 2 * (n - 1) * (n - 1)     # structural + shear
     comp = np.full(num_bodies * num_edges, compliance, np.float32)

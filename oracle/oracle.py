@@ -723,7 +723,7 @@ class Restatement:
 
 
 class RolloutReference:
-    """CPU restatement of the differentiable edge-only rollout (oracle/xpbd_adjoint.cpp):
+    """CPU restatement of the differentiable rollout of edge and LRA bodies (oracle/xpbd_adjoint.cpp):
     FP32 or FP64 forward pass and the FP64 adjoint from a full tape."""
 
     _lib = None
@@ -737,10 +737,21 @@ class RolloutReference:
             lib.xa_rollout_forward.argtypes = [_i] + common + [_vp, _vp, _vp]
             lib.xa_rollout_gradient.restype = _i
             lib.xa_rollout_gradient.argtypes = [_i] + common + [_vp] * 6
+            common_lra = [_u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _u32, _u32, _f, _f, _f, _vp, _f]
+            lib.xa_rollout_forward_lra.restype = _i
+            lib.xa_rollout_forward_lra.argtypes = [_i] + common_lra + [_vp, _vp, _vp]
+            lib.xa_rollout_gradient_lra.restype = _i
+            lib.xa_rollout_gradient_lra.argtypes = [_i] + common_lra + [_vp] * 6
             RolloutReference._lib = lib
         self.lib = RolloutReference._lib
-        if len(settings.bends) or len(settings.volumes) or len(settings.lras):
-            raise ValueError("the rollout oracle covers edge constraints only")
+        if len(settings.bends) or len(settings.volumes):
+            raise ValueError("the rollout oracle covers edge and LRA constraints only")
+        self.nl = len(settings.lras)
+        self.lv = np.ascontiguousarray(settings.lras["v"], np.uint32).reshape(-1) if self.nl else np.zeros(2, np.uint32)
+        self.maxd = np.ascontiguousarray(settings.lras["max_distance"], np.float32) if self.nl else np.zeros(1, np.float32)
+        g = settings.groups
+        self.groups = np.ascontiguousarray(np.stack([g["edge_end"], g["lra_end"]], axis=1), np.uint32).reshape(-1)
+        self.ng = len(g)
         v = settings.vertices
         self.nv, self.ne = len(v), len(settings.edges)
         self.pos = np.ascontiguousarray(v["pos"], np.float32).reshape(-1)
@@ -752,14 +763,14 @@ class RolloutReference:
         self.gravity_factor = float(gravity_factor)
 
     def _common(self):
-        return [self.nv, _ptr(self.pos), _ptr(self.inv_mass), self.ne, _ptr(self.ev), _ptr(self.rest),
-                *self.args, _ptr(self.gravity), self.gravity_factor]
+        return [self.nv, _ptr(self.pos), _ptr(self.inv_mass), self.ne, _ptr(self.ev), _ptr(self.rest), self.nl, _ptr(self.lv),
+                _ptr(self.maxd), self.ng, _ptr(self.groups), *self.args, _ptr(self.gravity), self.gravity_factor]
 
     def forward(self, compliance, velocity, precision=64):
         c = np.ascontiguousarray(compliance, np.float64).reshape(-1)
         vel = np.ascontiguousarray(velocity, np.float64).reshape(-1)
         out = np.zeros(3 * self.nv, np.float64)
-        rc = self.lib.xa_rollout_forward(int(precision), *self._common(), _ptr(c), _ptr(vel), _ptr(out))
+        rc = self.lib.xa_rollout_forward_lra(int(precision), *self._common(), _ptr(c), _ptr(vel), _ptr(out))
         if rc != 0:
             raise RuntimeError("xa_rollout_forward failed")
         return out.reshape(-1, 3)
@@ -772,7 +783,7 @@ class RolloutReference:
         gc = np.zeros(self.ne, np.float64)
         gv = np.zeros(3 * self.nv, np.float64)
         out = np.zeros(3 * self.nv, np.float64)
-        rc = self.lib.xa_rollout_gradient(int(forward_precision), *self._common(), _ptr(c), _ptr(vel), _ptr(g), _ptr(gc),
+        rc = self.lib.xa_rollout_gradient_lra(int(forward_precision), *self._common(), _ptr(c), _ptr(vel), _ptr(g), _ptr(gc),
                                           _ptr(gv), _ptr(out))
         if rc != 0:
             raise RuntimeError("xa_rollout_gradient failed")

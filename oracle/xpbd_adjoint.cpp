@@ -4,14 +4,14 @@
 // (include/jolt_xpbd.h, xpbd_rollout_*).  It is loaded by tests/ as the
 // checker; the product never links, loads or calls it.
 //
-// Scope: one body with edge constraints only, no colliders, no pressure, one
+// Scope: one body with edge and LRA constraints, no colliders, no pressure, one
 // collision step per frame, identity rotation.  The update is restated from
 // the reference (paths relative to its Jolt/ directory):
 //   InitializeUpdateContext        Physics/SoftBody/SoftBodyMotionProperties.cpp:937-960
 //   IntegratePositions             :325-350
-//   ProcessGroup / ApplyEdgeConstraints :1030-1060, :573-601 -- with edge
-//       constraints only, the update groups in order, each in array order, visit
-//       the edges in array order: a plain sequential Gauss-Seidel sweep
+//   ProcessGroup :1030-1060 -- the update groups in order, in each its edges
+//       (ApplyEdgeConstraints :573-601) then its LRAs (ApplyLRAConstraints
+//       :679-698), each in array order: a sequential Gauss-Seidel sweep
 //   velocity update                :700-710 (no colliding shape)
 //   velocity clamp                 UpdateSoftBodyState :852-857
 // The forward pass is templated (float: the FP32 update, double: the FP64
@@ -47,6 +47,10 @@ struct Spec
     const float *rest;
     uint32_t iterations, frames;
     float dt, linear_damping, max_linear_velocity, gravity[3], gravity_factor;
+    const uint32_t *lv;     // 2 per LRA
+    const float *max_distance;
+    // processing order: (0, edge index) or (1, LRA index) per projection
+    std::vector<std::pair<int, uint32_t>> seq;
 };
 
 template <class F>
@@ -81,7 +85,9 @@ struct Consts
 
 struct Tape
 {
-    std::vector<double> x0, v0, pre, y;     // per sub-step: 3 nv, 3 nv, 6 ne, 3 nv
+    std::vector<double> x0, v0, pre, y;     // per sub-step: 3 nv, 3 nv, 6 per projection, 3 nv
+    std::vector<uint8_t> acted;             // per sub-step and projection: an LRA moved its vertex (decided
+                                            // in the forward pass's precision)
 };
 
 // Forward pass; with a tape, records every sub-step (double only).
@@ -114,14 +120,29 @@ void Forward(const Spec &s, const F *compliance, const F *velocity, F *out, Tape
                 prev[i] = x[i];
                 x[i] = x[i] + v[i] * K.dt_sub;
             }
-            for (uint32_t e = 0; e < ne; ++e)                  // ApplyEdgeConstraints :583-600
+            for (const auto &op : s.seq)
             {
-                const uint32_t i0 = s.ev[2 * e], i1 = s.ev[2 * e + 1];
+                const uint32_t e = op.second;
+                const uint32_t i0 = op.first ? s.lv[2 * e] : s.ev[2 * e], i1 = op.first ? s.lv[2 * e + 1] : s.ev[2 * e + 1];
                 if (tape)
                 {
                     const double a[6] = { (double)x[i0].x, (double)x[i0].y, (double)x[i0].z, (double)x[i1].x, (double)x[i1].y, (double)x[i1].z };
                     tape->pre.insert(tape->pre.end(), a, a + 6);
                 }
+                if (op.first)                                   // ApplyLRAConstraints :685-697
+                {
+                    const V<F> d = x[i1] - x[i0];
+                    const F d_sq = dot(d, d), m = (F)s.max_distance[e];
+                    const bool act = d_sq > m * m;
+                    if (tape)
+                        tape->acted.push_back(act ? 1 : 0);
+                    if (act)
+                        x[i1] = x[i0] + (d * m) / std::sqrt(d_sq);
+                    continue;
+                }
+                if (tape)
+                    tape->acted.push_back(0);
+                // ApplyEdgeConstraints :583-600
                 const V<F> delta = x[i1] - x[i0];
                 const F length = std::sqrt(dot(delta, delta));
                 const F denom = length * ((w[i0] + w[i1]) + compliance[e] * K.inv_dt_sq);
@@ -188,7 +209,7 @@ void Adjoint(const Spec &s, const double *compliance, const double *velocity, co
     {
         const bool last = sub % I == I - 1;
         const double *x0 = &tape.x0[(size_t)sub * 3 * nv], *v0 = &tape.v0[(size_t)sub * 3 * nv];
-        const double *y = &tape.y[(size_t)sub * 3 * nv], *pre = &tape.pre[(size_t)sub * 6 * ne];
+        const double *y = &tape.y[(size_t)sub * 3 * nv], *pre = &tape.pre[(size_t)sub * 6 * s.seq.size()];
         for (uint32_t i = 0; i < nv; ++i)
         {
             const double w = s.inv_mass[i];
@@ -207,11 +228,28 @@ void Adjoint(const Spec &s, const double *compliance, const double *velocity, co
             }
             A[i] = w > 0.0 ? xb[i] + vb[i] / h : xb[i];
         }
-        for (uint32_t e = ne; e-- > 0;)
+        for (size_t k = s.seq.size(); k-- > 0;)
         {
+            const uint32_t e = s.seq[k].second;
+            const double *q = pre + 6 * k;
+            const V<double> p0 = mk(q[0], q[1], q[2]), p1 = mk(q[3], q[4], q[5]);
+            if (s.seq[k].first)
+            {
+                // x1' = x0 + m u, u = d / |d|:  x1bar = (m / |d|)(a1 - u (u . a1)),  x0bar += a1 - x1bar
+                const uint32_t j0 = s.lv[2 * e], j1 = s.lv[2 * e + 1];
+                const V<double> d = p1 - p0;
+                const double d_sq = dot(d, d), m = (double)s.max_distance[e];
+                if (!tape.acted[(size_t)sub * s.seq.size() + k])
+                    continue;
+                const double L = std::sqrt(d_sq);
+                const V<double> u = d / L;
+                const V<double> g = (m / L) * (A[j1] - dot(u, A[j1]) * u);
+                A[j0] = A[j0] + A[j1] - g;
+                A[j1] = g;
+                continue;
+            }
             const uint32_t i0 = s.ev[2 * e], i1 = s.ev[2 * e + 1];
             const double w0 = s.inv_mass[i0], w1 = s.inv_mass[i1];
-            const V<double> p0 = mk(pre[6 * e], pre[6 * e + 1], pre[6 * e + 2]), p1 = mk(pre[6 * e + 3], pre[6 * e + 4], pre[6 * e + 5]);
             const V<double> delta = p1 - p0;
             const double L = std::sqrt(dot(delta, delta));
             const double sc = (w0 + w1) + compliance[e] * K.inv_dt_sq;
@@ -244,12 +282,30 @@ void Adjoint(const Spec &s, const double *compliance, const double *velocity, co
     }
 }
 
+// groups: ng update groups' end indices (edge end, LRA end per group); ng 0: the edges in
+// array order, no LRA
 Spec MakeSpec(uint32_t nv, const float *pos, const float *inv_mass, uint32_t ne, const uint32_t *ev, const float *rest,
               uint32_t iterations, uint32_t frames, float dt, float linear_damping, float max_linear_velocity,
-              const float *gravity, float gravity_factor)
+              const float *gravity, float gravity_factor, uint32_t nl = 0, const uint32_t *lv = nullptr,
+              const float *max_distance = nullptr, uint32_t ng = 0, const uint32_t *groups = nullptr)
 {
     Spec s;
     s.nv = nv; s.ne = ne; s.pos = pos; s.inv_mass = inv_mass; s.ev = ev; s.rest = rest;
+    s.lv = lv; s.max_distance = max_distance;
+    if (ng == 0)
+        for (uint32_t e = 0; e < ne; ++e)
+            s.seq.push_back({ 0, e });
+    uint32_t e0 = 0, l0 = 0;
+    for (uint32_t g = 0; g < ng; ++g)
+    {
+        for (uint32_t e = e0; e < groups[2 * g] && e < ne; ++e)
+            s.seq.push_back({ 0, e });
+        for (uint32_t l = l0; l < groups[2 * g + 1] && l < nl; ++l)
+            if (lv[2 * l] != lv[2 * l + 1])         // an LRA of a vertex to itself never moves it
+                s.seq.push_back({ 1, l });
+        e0 = groups[2 * g];
+        l0 = groups[2 * g + 1];
+    }
     s.iterations = iterations; s.frames = frames; s.dt = dt; s.linear_damping = linear_damping;
     s.max_linear_velocity = max_linear_velocity;
     s.gravity[0] = gravity[0]; s.gravity[1] = gravity[1]; s.gravity[2] = gravity[2];
@@ -293,6 +349,45 @@ int xa_rollout_gradient(int forward_precision, uint32_t nv, const float *pos, co
         return -1;
     const Spec s = MakeSpec(nv, pos, inv_mass, ne, ev, rest, iterations, frames, dt, linear_damping, max_linear_velocity,
                             gravity, gravity_factor);
+    Adjoint(s, compliance, velocity, grad_positions, grad_compliance, grad_velocity, positions, forward_precision);
+    return 0;
+}
+
+// The two entry points above for a body with LRA constraints: nl LRAs (2 vertices each, max
+// distance), ng update groups (edge end, LRA end each).
+int xa_rollout_forward_lra(int precision, uint32_t nv, const float *pos, const float *inv_mass, uint32_t ne, const uint32_t *ev,
+                           const float *rest, uint32_t nl, const uint32_t *lv, const float *max_distance, uint32_t ng,
+                           const uint32_t *groups, uint32_t iterations, uint32_t frames, float dt, float linear_damping,
+                           float max_linear_velocity, const float *gravity, float gravity_factor, const double *compliance,
+                           const double *velocity, double *positions)
+{
+    const Spec s = MakeSpec(nv, pos, inv_mass, ne, ev, rest, iterations, frames, dt, linear_damping, max_linear_velocity,
+                            gravity, gravity_factor, nl, lv, max_distance, ng, groups);
+    if (precision == 64)
+    {
+        Forward<double>(s, compliance, velocity, positions, nullptr);
+        return 0;
+    }
+    if (precision != 32)
+        return -1;
+    std::vector<float> c(compliance, compliance + ne), v(velocity, velocity + 3 * (size_t)nv), out(3 * (size_t)nv);
+    Forward<float>(s, c.data(), v.data(), out.data(), nullptr);
+    for (size_t k = 0; k < out.size(); ++k)
+        positions[k] = out[k];
+    return 0;
+}
+
+int xa_rollout_gradient_lra(int forward_precision, uint32_t nv, const float *pos, const float *inv_mass, uint32_t ne,
+                            const uint32_t *ev, const float *rest, uint32_t nl, const uint32_t *lv, const float *max_distance,
+                            uint32_t ng, const uint32_t *groups, uint32_t iterations, uint32_t frames, float dt,
+                            float linear_damping, float max_linear_velocity, const float *gravity, float gravity_factor,
+                            const double *compliance, const double *velocity, const double *grad_positions,
+                            double *grad_compliance, double *grad_velocity, double *positions)
+{
+    if (forward_precision != 32 && forward_precision != 64)
+        return -1;
+    const Spec s = MakeSpec(nv, pos, inv_mass, ne, ev, rest, iterations, frames, dt, linear_damping, max_linear_velocity,
+                            gravity, gravity_factor, nl, lv, max_distance, ng, groups);
     Adjoint(s, compliance, velocity, grad_positions, grad_compliance, grad_velocity, positions, forward_precision);
     return 0;
 }

@@ -27,8 +27,8 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 
-def _make(frames, interval, compliance):
-    case = scenes.config4_rollout(num_bodies=2, n=16, frames=frames, compliance=compliance)
+def _make(frames, interval, compliance, lra=False):
+    case = scenes.config4_rollout(num_bodies=2, n=16, frames=frames, compliance=compliance, lra=lra)
     s = build_settings(case.mesh)
     ctx = Context(0)
     ctx.set_gravity(case.gravity)
@@ -38,10 +38,11 @@ def _make(frames, interval, compliance):
     return case, s, ctx, r
 
 
-@pytest.fixture
-def setup():
-    """2 cloths 16 x 16, 22 frames, checkpoints every 5 frames (the last segment is partial)."""
-    case, s, ctx, r = _make(22, 5, 1e-4)
+@pytest.fixture(params=[False, True], ids=["edges", "edges_lra"])
+def setup(request):
+    """2 cloths 16 x 16, 22 frames, checkpoints every 5 frames (the last segment is partial);
+    edge-only, and with LRA constraints from the pinned corners."""
+    case, s, ctx, r = _make(22, 5, 1e-4, lra=request.param)
     try:
         yield case, s, ctx, r
     finally:
@@ -50,13 +51,13 @@ def setup():
         ctx.close()
 
 
-@pytest.fixture
-def setup_fd():
+@pytest.fixture(params=[False, True], ids=["edges", "edges_lra"])
+def setup_fd(request):
     """2 cloths 16 x 16, compliance 1e-3, 22 frames, checkpoints every 5 frames.  The
     finite-difference check compares the FP32 program's gradient with the FP64 one; on
     this case the two differ by 1e-4 to 3e-4 (norm over all parameters, measured with
     the oracle)."""
-    case, s, ctx, r = _make(22, 5, 1e-3)
+    case, s, ctx, r = _make(22, 5, 1e-3, lra=request.param)
     try:
         yield case, s, ctx, r
     finally:

@@ -20,8 +20,8 @@ from oracle import RolloutReference, have_restatement
 pytestmark = pytest.mark.skipif(not have_restatement(), reason="oracle restatement not built (make -C oracle restatement)")
 
 
-def _case(n=6, frames=8):
-    case = scenes.config4_rollout(num_bodies=1, n=n, frames=frames)
+def _case(n=6, frames=8, lra=False):
+    case = scenes.config4_rollout(num_bodies=1, n=n, frames=frames, lra=lra)
     s = build_settings(case.mesh)
     ref = RolloutReference(s, iterations=case.iterations, frames=frames, dt=case.dt, gravity=case.gravity)
     comp = case.compliance.astype(np.float64)
@@ -35,8 +35,19 @@ def _loss(ref, target, c, v):
     return float(np.mean(np.sum((x - target) ** 2, axis=1)))
 
 
-def test_adjoint_matches_central_differences_fp64():
-    case, s, ref, comp, vel, target = _case()
+@pytest.mark.parametrize("lra", [False, True])
+def test_adjoint_matches_central_differences_fp64(lra):
+    """The FP64 adjoint against central differences of the FP64 forward pass, edge-only and with
+    LRA constraints (some of which act: checked)."""
+    case, s, ref, comp, vel, target = _case(lra=lra)
+    if lra:
+        assert len(s.lras) > 0
+        stretched = ref.forward(comp, vel)
+        lv = s.lras["v"].astype(np.int64)
+        d = np.linalg.norm(stretched[lv[:, 1]] - stretched[lv[:, 0]], axis=1)
+        assert (d >= s.lras["max_distance"] * (1 - 1e-6)).any(), "no LRA acts at the end"
+        assert np.array_equal(ref.forward(comp, vel), RolloutReference(s, iterations=case.iterations, frames=case.frames, dt=case.dt,
+                                                                         gravity=case.gravity).forward(comp, vel))
     x = ref.forward(comp, vel)
     g = 2.0 * (x - target) / ref.nv
     x2, gc, gv = ref.gradient(comp, vel, g)
@@ -61,11 +72,12 @@ def test_adjoint_matches_central_differences_fp64():
         assert abs(fd - gvf[k]) <= 1e-5 * max(abs(fd), 1e-3 * scale_v), (k, fd, gvf[k])
 
 
-def test_fp32_forward_matches_update_restatement():
+@pytest.mark.parametrize("lra", [False, True])
+def test_fp32_forward_matches_update_restatement(lra):
     """The rollout oracle's FP32 pass equals the parity restatement of the full update
-    on the same body (no colliders, identity rotation, no recentring)."""
+    on the same body (no colliders, identity rotation, no recentring), with LRAs too."""
     from oracle import Restatement
-    case, s, ref, comp, vel, target = _case(n=8, frames=6)
+    case, s, ref, comp, vel, target = _case(n=8, frames=6, lra=lra)
     x32 = ref.forward(comp, vel, precision=32)
     st = s.copy()
     st.edges["compliance"] = comp.astype(np.float32)
