@@ -1,4 +1,4 @@
-// Differentiable rollout of the soft-body update for bodies with edge and LRA
+// Differentiable rollout of the soft-body update for bodies with edge, volume and LRA
 // constraints: float and double forward passes and the hand-written adjoint.
 //
 // The forward pass follows the update of the main kernels step for step
@@ -137,8 +137,84 @@ __device__ __forceinline__ void lra_adjoint(float4 *A, uint32_t i0, uint32_t i1,
     atomicAdd(&A[i0].z, ab1.z - g.z);
 }
 
-// Record type of the level programs (bits 14-15 of the first word): 3 for an LRA
+// Record type of the level programs (bits 14-15 of the first word): 3 for an LRA, 0 for a
+// volume (vertices 2 and 3 in the second word, as the update kernels unpack them)
 __device__ __forceinline__ bool is_lra_record(const uint4 &q) { return ((q.x >> 14) & 3u) == 3u; }
+__device__ __forceinline__ bool is_volume_record(const uint4 &q) { return ((q.x >> 14) & 3u) == 0u; }
+
+template <class T> __device__ __forceinline__ R3<T> rcross(R3<T> a, R3<T> b)
+{
+    return r3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);       // Vec3.inl:855-863
+}
+
+// ApplyVolumeConstraints body (SoftBodyMotionProperties.cpp:467-506)
+template <class T>
+__device__ __forceinline__ void volume_project(R4<T> *X, uint32_t i1, uint32_t i2, uint32_t i3, uint32_t i4, T six_rest_volume,
+                                               T compliance, T inv_dt_sq)
+{
+    const R4<T> p1 = X[i1], p2 = X[i2], p3 = X[i3], p4 = X[i4];
+    const R3<T> x1 = xyz3(p1), x2 = xyz3(p2), x3 = xyz3(p3), x4 = xyz3(p4);
+    const R3<T> x1x2 = x2 - x1, x1x3 = x3 - x1, x1x4 = x4 - x1;
+    const T six = rdot(rcross(x1x2, x1x3), x1x4);
+    const T c = (six < (T)0 ? -six : six) - six_rest_volume;
+    const R3<T> d1c = rcross(x4 - x2, x3 - x2), d2c = rcross(x1x3, x1x4), d3c = rcross(x1x4, x1x2), d4c = rcross(x1x2, x1x3);
+    const T w1 = p1.w, w2 = p2.w, w3 = p3.w, w4 = p4.w;
+    const T denom = (((w1 * rdot(d1c, d1c) + w2 * rdot(d2c, d2c)) + w3 * rdot(d3c, d3c)) + w4 * rdot(d4c, d4c)) + compliance * inv_dt_sq;
+    if (denom < (T)1.0e-12f)
+        return;
+    const T minus_lambda = c / denom;
+    X[i1] = r4(x1 - (minus_lambda * w1) * d1c, w1);
+    X[i2] = r4(x2 - (minus_lambda * w2) * d2c, w2);
+    X[i3] = r4(x3 - (minus_lambda * w3) * d3c, w3);
+    X[i4] = r4(x4 - (minus_lambda * w4) * d4c, w4);
+}
+
+// Adjoint of volume_project from its pre-projection positions q1..q4 (outputs y_i = x_i - mu w_i d_i,
+// mu = c / D, c = |6V| - 6V0, D = sum w_i |d_i|^2 + alpha / h^2, d_i = grad_i 6V):
+//   s = sum w_i d_i . ybar_i,  mubar = -s,  cbar = mubar / D,  Dbar = -mubar mu / D,
+//   dbar_i = -mu w_i ybar_i + 2 w_i Dbar d_i,  xbar_j = ybar_j + cbar sign(6V) d_j, and dbar_i back
+//   through the cross products that form d_i (for a x b: abar = b x dbar, bbar = dbar x a).
+// The volume's compliance is not a parameter of the rollout.
+__device__ __forceinline__ void volume_adjoint(float4 *A, uint32_t i1, uint32_t i2, uint32_t i3, uint32_t i4, const float4 *q,
+                                               float six_rest_volume, float compliance, float inv_dt_sq)
+{
+    const R3<float> x1 = xyz3(to_r4<float>(q[0])), x2 = xyz3(to_r4<float>(q[1])), x3 = xyz3(to_r4<float>(q[2])),
+                    x4 = xyz3(to_r4<float>(q[3]));
+    const float w1 = q[0].w, w2 = q[1].w, w3 = q[2].w, w4 = q[3].w;
+    const R3<float> x1x2 = x2 - x1, x1x3 = x3 - x1, x1x4 = x4 - x1, x2x4 = x4 - x2, x2x3 = x3 - x2;
+    const float six = rdot(rcross(x1x2, x1x3), x1x4);
+    const float c = fabsf(six) - six_rest_volume;
+    const R3<float> d1 = rcross(x2x4, x2x3), d2 = rcross(x1x3, x1x4), d3 = rcross(x1x4, x1x2), d4 = rcross(x1x2, x1x3);
+    const float denom = (((w1 * rdot(d1, d1) + w2 * rdot(d2, d2)) + w3 * rdot(d3, d3)) + w4 * rdot(d4, d4)) + compliance * inv_dt_sq;
+    if (denom < 1.0e-12f)
+        return;
+    const float mu = c / denom;
+    const float4 a1 = A[i1], a2 = A[i2], a3 = A[i3], a4 = A[i4];
+    const R3<float> y1 = xyz3(to_r4<float>(a1)), y2 = xyz3(to_r4<float>(a2)), y3 = xyz3(to_r4<float>(a3)), y4 = xyz3(to_r4<float>(a4));
+    const float s = ((w1 * rdot(d1, y1) + w2 * rdot(d2, y2)) + w3 * rdot(d3, y3)) + w4 * rdot(d4, y4);
+    const float mubar = -s;
+    const float cbar = (mubar / denom) * (six < 0.0f ? -1.0f : 1.0f);
+    const float dbar_den = 2.0f * (-mubar * mu / denom);
+    const R3<float> e1 = (-mu * w1) * y1 + (dbar_den * w1) * d1, e2 = (-mu * w2) * y2 + (dbar_den * w2) * d2;
+    const R3<float> e3 = (-mu * w3) * y3 + (dbar_den * w3) * d3, e4 = (-mu * w4) * y4 + (dbar_den * w4) * d4;
+    R3<float> b1 = y1 + cbar * d1, b2 = y2 + cbar * d2, b3 = y3 + cbar * d3, b4 = y4 + cbar * d4;
+    // d1 = (x4 - x2) x (x3 - x2)
+    R3<float> g = rcross(x2x3, e1), h = rcross(e1, x2x4);
+    b4 = b4 + g; b3 = b3 + h; b2 = b2 - g - h;
+    // d2 = (x3 - x1) x (x4 - x1)
+    g = rcross(x1x4, e2); h = rcross(e2, x1x3);
+    b3 = b3 + g; b4 = b4 + h; b1 = b1 - g - h;
+    // d3 = (x4 - x1) x (x2 - x1)
+    g = rcross(x1x2, e3); h = rcross(e3, x1x4);
+    b4 = b4 + g; b2 = b2 + h; b1 = b1 - g - h;
+    // d4 = (x2 - x1) x (x3 - x1)
+    g = rcross(x1x3, e4); h = rcross(e4, x1x2);
+    b2 = b2 + g; b3 = b3 + h; b1 = b1 - g - h;
+    A[i1] = make_float4(b1.x, b1.y, b1.z, a1.w);
+    A[i2] = make_float4(b2.x, b2.y, b2.z, a2.w);
+    A[i3] = make_float4(b3.x, b3.y, b3.z, a3.w);
+    A[i4] = make_float4(b4.x, b4.y, b4.z, a4.w);
+}
 
 // Adjoint of edge_project from its pre-projection positions q0, q1.
 __device__ __forceinline__ void edge_adjoint(float4 *A, uint32_t i0, uint32_t i1, float4 q0, float4 q1, float rest,
@@ -194,6 +270,9 @@ __device__ __forceinline__ void forward_substep(const RolloutBodyDev &B, const S
             return;
         if (is_lra_record(q))
             lra_project<T>(X, q.x & 0x3fffu, q.x >> 16, (T)__uint_as_float(q.z));
+        else if (is_volume_record(q))
+            volume_project<T>(X, q.x & 0x3fffu, q.x >> 16, q.y & 0x7fffu, q.y >> 16, (T)__uint_as_float(q.z),
+                              (T)__uint_as_float(q.w), K.inv_dt_sq);
         else
             edge_project<T>(X, q.x & 0x3fffu, q.x >> 16, (T)__uint_as_float(q.z), comp(q), K.inv_dt_sq);
     });
@@ -235,7 +314,7 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_forward_kernel(RolloutD
         for (uint32_t r = tid; r < B.num_records; r += kRolloutBlock)
         {
             uint4 q = recs[r];
-            if (q.x != kNopRecord && !is_lra_record(q))
+            if (q.x != kNopRecord && !is_lra_record(q) && !is_volume_record(q))
             {
                 q.w = __float_as_uint((float)compliance[B.eoff + q.y]);
                 recs[r] = q;
@@ -294,7 +373,7 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_backward_kernel(Rollout
     R4<float> *V = state_vel<float>(R) + B.voff;
     R4<float> *P = state_prev<float>(R) + B.voff;
     float4 *adj_x = R.adj_x + B.voff, *adj_v = R.adj_v + B.voff;
-    float4 *tape = R.rec_tape + 2u * (size_t)B.rbase;
+    float4 *tape = R.rec_tape + 4u * (size_t)B.rbase;
 
     for (uint32_t e = tid; e < ne; e += kRolloutBlock)
         alpha_bar[e] = 0.0f;
@@ -352,9 +431,16 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_backward_kernel(Rollout
                 if (q.x == kNopRecord)
                     return;
                 const uint32_t i0 = q.x & 0x3fffu, i1 = q.x >> 16;
-                tape[2u * r] = A[i0];
-                tape[2u * r + 1u] = A[i1];
-                if (is_lra_record(q))
+                tape[4u * r] = A[i0];
+                tape[4u * r + 1u] = A[i1];
+                if (is_volume_record(q))
+                {
+                    const uint32_t i2 = q.y & 0x7fffu, i3 = q.y >> 16;
+                    tape[4u * r + 2u] = A[i2];
+                    tape[4u * r + 3u] = A[i3];
+                    volume_project<float>(X, i0, i1, i2, i3, __uint_as_float(q.z), __uint_as_float(q.w), K.inv_dt_sq);
+                }
+                else if (is_lra_record(q))
                     lra_project<float>(X, i0, i1, __uint_as_float(q.z));
                 else
                     edge_project<float>(X, i0, i1, __uint_as_float(q.z), __uint_as_float(q.w), K.inv_dt_sq);
@@ -392,10 +478,16 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_backward_kernel(Rollout
                     return;
                 if (is_lra_record(q))
                 {
-                    lra_adjoint(A, q.x & 0x3fffu, q.x >> 16, tape[2u * r], tape[2u * r + 1u], __uint_as_float(q.z));
+                    lra_adjoint(A, q.x & 0x3fffu, q.x >> 16, tape[4u * r], tape[4u * r + 1u], __uint_as_float(q.z));
                     return;
                 }
-                edge_adjoint(A, q.x & 0x3fffu, q.x >> 16, tape[2u * r], tape[2u * r + 1u], __uint_as_float(q.z),
+                if (is_volume_record(q))
+                {
+                    volume_adjoint(A, q.x & 0x3fffu, q.x >> 16, q.y & 0x7fffu, q.y >> 16, tape + 4u * r, __uint_as_float(q.z),
+                                   __uint_as_float(q.w), K.inv_dt_sq);
+                    return;
+                }
+                edge_adjoint(A, q.x & 0x3fffu, q.x >> 16, tape[4u * r], tape[4u * r + 1u], __uint_as_float(q.z),
                              __uint_as_float(q.w), K.inv_dt_sq, alpha_bar[q.y]);
             });
             // adjoint of prev = x0, y0 = x0 + v1 h and the integration of v

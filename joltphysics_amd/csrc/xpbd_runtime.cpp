@@ -3141,8 +3141,8 @@ int xpbd_rollout_create(xpbd_context *ctx, const xpbd_rollout_desc *d, xpbd_roll
         xpbd_settings *s = d->settings[b];
         if (!s || s->ctx != ctx)
             return Fail(XPBD_E_INVALID, "settings " + std::to_string(b) + " missing or from another context");
-        if (s->num_bends || s->num_volumes)
-            return Fail(XPBD_E_UNSUPPORTED, "rollout bodies may have edge and LRA constraints only (body " + std::to_string(b) + ")");
+        if (s->num_bends)
+            return Fail(XPBD_E_UNSUPPORTED, "rollout bodies may have edge, volume and LRA constraints only (body " + std::to_string(b) + ")");
         if (s->host_level_begin.empty())
             return Fail(XPBD_E_CAPACITY, "rollout body " + std::to_string(b) + " does not fit in LDS");
         const uint32_t nv = s->num_vertices, ne = s->num_edges;
@@ -3164,10 +3164,13 @@ int xpbd_rollout_create(xpbd_context *ctx, const xpbd_rollout_desc *d, xpbd_roll
         for (size_t k = 0; k < s->host_recs.size(); ++k)
         {
             uint4 q = s->host_recs[k];
-            if (q.x != kNopRecordHost)
+            if (q.x != kNopRecordHost && ((q.x >> 14) & 3u) != kTypeVolume)
             {
-                q.y = s->host_rec_index[k];         // edge index in the body
-                q.w = 0u;                           // compliance, written by each float forward pass
+                // edges: their index in the body (for the compliance parameters) and a compliance
+                // written by each float forward pass; LRAs: their index.  Volume records keep their
+                // vertices 2-3 and compliance.
+                q.y = s->host_rec_index[k];
+                q.w = 0u;
             }
             recs.push_back(q);
         }
@@ -3197,7 +3200,7 @@ int xpbd_rollout_create(xpbd_context *ctx, const xpbd_rollout_desc *d, xpbd_roll
     HIP_TRY(r->seg_x.alloc(seg_sub * nv)); HIP_TRY(r->seg_v.alloc(seg_sub * nv));
     HIP_TRY(r->vel.alloc(nv)); HIP_TRY(r->prev.alloc(nv));
     HIP_TRY(r->vel64.alloc(4 * nv)); HIP_TRY(r->prev64.alloc(4 * nv));
-    HIP_TRY(r->rec_tape.alloc(2 * std::max<size_t>(recs.size(), 1)));
+    HIP_TRY(r->rec_tape.alloc(4 * std::max<size_t>(recs.size(), 1)));
     HIP_TRY(r->adj_x.alloc(nv)); HIP_TRY(r->adj_v.alloc(nv));
     RolloutDev &R = r->dev;
     R.bodies = r->bodies.p;

@@ -283,7 +283,7 @@ struct RolloutDev
     float4 *seg_x, *seg_v;      // state at every sub-step start of the segment being reversed
     float4 *vel, *prev;         // float forward state
     double *vel64, *prev64;     // double forward state (4 per vertex)
-    float4 *rec_tape;           // 2 per record: pre-projection positions (+ inverse mass)
+    float4 *rec_tape;           // 4 per record: pre-projection positions (+ inverse mass) of its vertices
     float4 *adj_x, *adj_v;      // adjoint state
 };
 

@@ -548,6 +548,25 @@ def config4_rollout(num_bodies=64, n=64, frames=60, seed=4, compliance=1e-3, lra
                        (0.0, 0.0, -9.81), comp, vel, scale)
 
 
+def rollout_tet_block(num_bodies=2, cells=3, frames=8, seed=6, compliance=1e-4, volume_compliance=1e-5):
+    """Differentiation workload with volume constraints: `num_bodies` copies of a (cells+1)^3
+    5-tet block (spacing 0.1 m) hanging from its top layer (inverse mass 0), edges of
+    compliance `compliance` (the parameters) and volumes of compliance `volume_compliance`,
+    gravity -9.81 z, dt 1/60, 5 iterations; initial velocities U(-0.1, 0.1) m/s and target
+    compliance multipliers U(0.5, 2.0) per edge, from `seed`."""
+    v, e, t = tet_cube(cells, 0.1, inv_mass=1.0, compliance=compliance)
+    t["compliance"] = volume_compliance
+    top = v["pos"][:, 2] > v["pos"][:, 2].max() - 1e-6
+    v["inv_mass"][top] = 0.0
+    mesh = MeshSpec(vertices=v, edges=e, volumes=t, flags=CALC_EDGE_LENGTHS | CALC_VOLUMES | OPTIMIZE)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    comp = np.full(num_bodies * len(e), compliance, np.float32)
+    vel = rng.uniform(-0.1, 0.1, size=(num_bodies * len(v), 3)).astype(np.float32)
+    scale = rng.uniform(0.5, 2.0, size=num_bodies * len(e)).astype(np.float32)
+    return RolloutCase(f"tet{cells}x{num_bodies}_rollout", mesh, num_bodies, frames, 5, 1.0 / 60.0, (0.0, 0.0, -9.81),
+                       comp, vel, scale)
+
+
 def mesh_for_key(key):
     """MeshSpec of a shape key used by config3_batched (meshes shared between bodies)."""
     kind, n = key

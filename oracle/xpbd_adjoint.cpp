@@ -4,14 +4,15 @@
 // (include/jolt_xpbd.h, xpbd_rollout_*).  It is loaded by tests/ as the
 // checker; the product never links, loads or calls it.
 //
-// Scope: one body with edge and LRA constraints, no colliders, no pressure, one
+// Scope: one body with edge, volume and LRA constraints, no colliders, no pressure, one
 // collision step per frame, identity rotation.  The update is restated from
 // the reference (paths relative to its Jolt/ directory):
 //   InitializeUpdateContext        Physics/SoftBody/SoftBodyMotionProperties.cpp:937-960
 //   IntegratePositions             :325-350
-//   ProcessGroup :1030-1060 -- the update groups in order, in each its edges
-//       (ApplyEdgeConstraints :573-601) then its LRAs (ApplyLRAConstraints
-//       :679-698), each in array order: a sequential Gauss-Seidel sweep
+//   ProcessGroup :1030-1060 -- the update groups in order, in each its volumes
+//       (ApplyVolumeConstraints :458-507), its edges (ApplyEdgeConstraints :573-601),
+//       then its LRAs (ApplyLRAConstraints :679-698), each in array order: a
+//       sequential Gauss-Seidel sweep
 //   velocity update                :700-710 (no colliding shape)
 //   velocity clamp                 UpdateSoftBodyState :852-857
 // The forward pass is templated (float: the FP32 update, double: the FP64
@@ -37,6 +38,7 @@ template <class F> inline V<F> operator*(V<F> a, F s) { return mk(a.x * s, a.y *
 template <class F> inline V<F> operator*(F s, V<F> a) { return mk(s * a.x, s * a.y, s * a.z); }
 template <class F> inline V<F> operator/(V<F> a, F s) { return mk(a.x / s, a.y / s, a.z / s); }
 template <class F> inline F dot(V<F> a, V<F> b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }   // Vec3.inl:901-934
+template <class F> inline V<F> cross(V<F> a, V<F> b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
 
 struct Spec
 {
@@ -49,7 +51,9 @@ struct Spec
     float dt, linear_damping, max_linear_velocity, gravity[3], gravity_factor;
     const uint32_t *lv;     // 2 per LRA
     const float *max_distance;
-    // processing order: (0, edge index) or (1, LRA index) per projection
+    const uint32_t *tv;     // 4 per volume
+    const float *six_rest, *vol_compliance;
+    // processing order: (0, edge index), (1, LRA index) or (2, volume index) per projection
     std::vector<std::pair<int, uint32_t>> seq;
 };
 
@@ -85,7 +89,7 @@ struct Consts
 
 struct Tape
 {
-    std::vector<double> x0, v0, pre, y;     // per sub-step: 3 nv, 3 nv, 6 per projection, 3 nv
+    std::vector<double> x0, v0, pre, y;     // per sub-step: 3 nv, 3 nv, 12 per projection, 3 nv
     std::vector<uint8_t> acted;             // per sub-step and projection: an LRA moved its vertex (decided
                                             // in the forward pass's precision)
 };
@@ -123,11 +127,34 @@ void Forward(const Spec &s, const F *compliance, const F *velocity, F *out, Tape
             for (const auto &op : s.seq)
             {
                 const uint32_t e = op.second;
-                const uint32_t i0 = op.first ? s.lv[2 * e] : s.ev[2 * e], i1 = op.first ? s.lv[2 * e + 1] : s.ev[2 * e + 1];
+                const uint32_t *vs = op.first == 2 ? s.tv + 4 * e : op.first == 1 ? s.lv + 2 * e : s.ev + 2 * e;
+                const uint32_t i0 = vs[0], i1 = vs[1];
                 if (tape)
+                    for (int j = 0; j < 4; ++j)
+                    {
+                        const uint32_t vj = j < (op.first == 2 ? 4 : 2) ? vs[j] : vs[0];
+                        const double a[3] = { (double)x[vj].x, (double)x[vj].y, (double)x[vj].z };
+                        tape->pre.insert(tape->pre.end(), a, a + 3);
+                    }
+                if (op.first == 2)                              // ApplyVolumeConstraints :467-506
                 {
-                    const double a[6] = { (double)x[i0].x, (double)x[i0].y, (double)x[i0].z, (double)x[i1].x, (double)x[i1].y, (double)x[i1].z };
-                    tape->pre.insert(tape->pre.end(), a, a + 6);
+                    if (tape)
+                        tape->acted.push_back(0);
+                    const uint32_t i2 = vs[2], i3 = vs[3];
+                    const V<F> x1x2 = x[i1] - x[i0], x1x3 = x[i2] - x[i0], x1x4 = x[i3] - x[i0];
+                    const F six = dot(cross(x1x2, x1x3), x1x4);
+                    const F c = std::abs(six) - (F)s.six_rest[e];
+                    const V<F> d1 = cross(x[i3] - x[i1], x[i2] - x[i1]), d2 = cross(x1x3, x1x4), d3 = cross(x1x4, x1x2), d4 = cross(x1x2, x1x3);
+                    const F denom = (((w[i0] * dot(d1, d1) + w[i1] * dot(d2, d2)) + w[i2] * dot(d3, d3)) + w[i3] * dot(d4, d4))
+                                    + (F)s.vol_compliance[e] * K.inv_dt_sq;
+                    if (denom < (F)1.0e-12f)
+                        continue;
+                    const F ml = c / denom;
+                    x[i0] = x[i0] - (ml * w[i0]) * d1;
+                    x[i1] = x[i1] - (ml * w[i1]) * d2;
+                    x[i2] = x[i2] - (ml * w[i2]) * d3;
+                    x[i3] = x[i3] - (ml * w[i3]) * d4;
+                    continue;
                 }
                 if (op.first)                                   // ApplyLRAConstraints :685-697
                 {
@@ -209,7 +236,7 @@ void Adjoint(const Spec &s, const double *compliance, const double *velocity, co
     {
         const bool last = sub % I == I - 1;
         const double *x0 = &tape.x0[(size_t)sub * 3 * nv], *v0 = &tape.v0[(size_t)sub * 3 * nv];
-        const double *y = &tape.y[(size_t)sub * 3 * nv], *pre = &tape.pre[(size_t)sub * 6 * s.seq.size()];
+        const double *y = &tape.y[(size_t)sub * 3 * nv], *pre = &tape.pre[(size_t)sub * 12 * s.seq.size()];
         for (uint32_t i = 0; i < nv; ++i)
         {
             const double w = s.inv_mass[i];
@@ -231,8 +258,39 @@ void Adjoint(const Spec &s, const double *compliance, const double *velocity, co
         for (size_t k = s.seq.size(); k-- > 0;)
         {
             const uint32_t e = s.seq[k].second;
-            const double *q = pre + 6 * k;
+            const double *q = pre + 12 * k;
             const V<double> p0 = mk(q[0], q[1], q[2]), p1 = mk(q[3], q[4], q[5]);
+            if (s.seq[k].first == 2)
+            {
+                // y_i = x_i - mu w_i d_i, mu = c / D, c = |6V| - 6V0, d_i = grad_i 6V: see the device's volume_adjoint
+                const uint32_t *vs = s.tv + 4 * e;
+                const V<double> p2 = mk(q[6], q[7], q[8]), p3 = mk(q[9], q[10], q[11]);
+                const double w0 = s.inv_mass[vs[0]], w1 = s.inv_mass[vs[1]], w2 = s.inv_mass[vs[2]], w3 = s.inv_mass[vs[3]];
+                const V<double> a12 = p1 - p0, a13 = p2 - p0, a14 = p3 - p0, a24 = p3 - p1, a23 = p2 - p1;
+                const double six = dot(cross(a12, a13), a14);
+                const V<double> d1 = cross(a24, a23), d2 = cross(a13, a14), d3 = cross(a14, a12), d4 = cross(a12, a13);
+                const double D = w0 * dot(d1, d1) + w1 * dot(d2, d2) + w2 * dot(d3, d3) + w3 * dot(d4, d4)
+                                 + (double)s.vol_compliance[e] * K.inv_dt_sq;
+                if (D < (double)1.0e-12f)
+                    continue;
+                const double mu = (std::abs(six) - (double)s.six_rest[e]) / D;
+                const V<double> y0 = A[vs[0]], y1 = A[vs[1]], y2 = A[vs[2]], y3 = A[vs[3]];
+                const double mubar = -(w0 * dot(d1, y0) + w1 * dot(d2, y1) + w2 * dot(d3, y2) + w3 * dot(d4, y3));
+                const double cbar = mubar / D * (six < 0.0 ? -1.0 : 1.0), Dbar2 = 2.0 * (-mubar * mu / D);
+                const V<double> e1 = (-mu * w0) * y0 + (Dbar2 * w0) * d1, e2 = (-mu * w1) * y1 + (Dbar2 * w1) * d2;
+                const V<double> e3 = (-mu * w2) * y2 + (Dbar2 * w2) * d3, e4 = (-mu * w3) * y3 + (Dbar2 * w3) * d4;
+                V<double> b0 = y0 + cbar * d1, b1 = y1 + cbar * d2, b2 = y2 + cbar * d3, b3 = y3 + cbar * d4;
+                V<double> g = cross(a23, e1), hh = cross(e1, a24);          // d1 = (x4 - x2) x (x3 - x2)
+                b3 = b3 + g; b2 = b2 + hh; b1 = b1 - g - hh;
+                g = cross(a14, e2); hh = cross(e2, a13);                    // d2 = (x3 - x1) x (x4 - x1)
+                b2 = b2 + g; b3 = b3 + hh; b0 = b0 - g - hh;
+                g = cross(a12, e3); hh = cross(e3, a14);                    // d3 = (x4 - x1) x (x2 - x1)
+                b3 = b3 + g; b1 = b1 + hh; b0 = b0 - g - hh;
+                g = cross(a13, e4); hh = cross(e4, a12);                    // d4 = (x2 - x1) x (x3 - x1)
+                b1 = b1 + g; b2 = b2 + hh; b0 = b0 - g - hh;
+                A[vs[0]] = b0; A[vs[1]] = b1; A[vs[2]] = b2; A[vs[3]] = b3;
+                continue;
+            }
             if (s.seq[k].first)
             {
                 // x1' = x0 + m u, u = d / |d|:  x1bar = (m / |d|)(a1 - u (u . a1)),  x0bar += a1 - x1bar
@@ -287,24 +345,29 @@ void Adjoint(const Spec &s, const double *compliance, const double *velocity, co
 Spec MakeSpec(uint32_t nv, const float *pos, const float *inv_mass, uint32_t ne, const uint32_t *ev, const float *rest,
               uint32_t iterations, uint32_t frames, float dt, float linear_damping, float max_linear_velocity,
               const float *gravity, float gravity_factor, uint32_t nl = 0, const uint32_t *lv = nullptr,
-              const float *max_distance = nullptr, uint32_t ng = 0, const uint32_t *groups = nullptr)
+              const float *max_distance = nullptr, uint32_t nt = 0, const uint32_t *tv = nullptr, const float *six_rest = nullptr,
+              const float *vol_compliance = nullptr, uint32_t ng = 0, const uint32_t *groups = nullptr)
 {
     Spec s;
     s.nv = nv; s.ne = ne; s.pos = pos; s.inv_mass = inv_mass; s.ev = ev; s.rest = rest;
     s.lv = lv; s.max_distance = max_distance;
+    s.tv = tv; s.six_rest = six_rest; s.vol_compliance = vol_compliance;
     if (ng == 0)
         for (uint32_t e = 0; e < ne; ++e)
             s.seq.push_back({ 0, e });
-    uint32_t e0 = 0, l0 = 0;
+    uint32_t t0 = 0, e0 = 0, l0 = 0;
     for (uint32_t g = 0; g < ng; ++g)
     {
-        for (uint32_t e = e0; e < groups[2 * g] && e < ne; ++e)
+        for (uint32_t t = t0; t < groups[3 * g] && t < nt; ++t)
+            s.seq.push_back({ 2, t });
+        for (uint32_t e = e0; e < groups[3 * g + 1] && e < ne; ++e)
             s.seq.push_back({ 0, e });
-        for (uint32_t l = l0; l < groups[2 * g + 1] && l < nl; ++l)
+        for (uint32_t l = l0; l < groups[3 * g + 2] && l < nl; ++l)
             if (lv[2 * l] != lv[2 * l + 1])         // an LRA of a vertex to itself never moves it
                 s.seq.push_back({ 1, l });
-        e0 = groups[2 * g];
-        l0 = groups[2 * g + 1];
+        t0 = groups[3 * g];
+        e0 = groups[3 * g + 1];
+        l0 = groups[3 * g + 2];
     }
     s.iterations = iterations; s.frames = frames; s.dt = dt; s.linear_damping = linear_damping;
     s.max_linear_velocity = max_linear_velocity;
@@ -353,16 +416,18 @@ int xa_rollout_gradient(int forward_precision, uint32_t nv, const float *pos, co
     return 0;
 }
 
-// The two entry points above for a body with LRA constraints: nl LRAs (2 vertices each, max
-// distance), ng update groups (edge end, LRA end each).
+// The two entry points above for a body with volume and LRA constraints: nl LRAs (2 vertices
+// each, max distance), nt volumes (4 vertices, six rest volume, compliance each), ng update
+// groups (volume end, edge end, LRA end each).
 int xa_rollout_forward_lra(int precision, uint32_t nv, const float *pos, const float *inv_mass, uint32_t ne, const uint32_t *ev,
-                           const float *rest, uint32_t nl, const uint32_t *lv, const float *max_distance, uint32_t ng,
+                           const float *rest, uint32_t nl, const uint32_t *lv, const float *max_distance, uint32_t nt,
+                           const uint32_t *tv, const float *six_rest, const float *vol_compliance, uint32_t ng,
                            const uint32_t *groups, uint32_t iterations, uint32_t frames, float dt, float linear_damping,
                            float max_linear_velocity, const float *gravity, float gravity_factor, const double *compliance,
                            const double *velocity, double *positions)
 {
     const Spec s = MakeSpec(nv, pos, inv_mass, ne, ev, rest, iterations, frames, dt, linear_damping, max_linear_velocity,
-                            gravity, gravity_factor, nl, lv, max_distance, ng, groups);
+                            gravity, gravity_factor, nl, lv, max_distance, nt, tv, six_rest, vol_compliance, ng, groups);
     if (precision == 64)
     {
         Forward<double>(s, compliance, velocity, positions, nullptr);
@@ -379,6 +444,7 @@ int xa_rollout_forward_lra(int precision, uint32_t nv, const float *pos, const f
 
 int xa_rollout_gradient_lra(int forward_precision, uint32_t nv, const float *pos, const float *inv_mass, uint32_t ne,
                             const uint32_t *ev, const float *rest, uint32_t nl, const uint32_t *lv, const float *max_distance,
+                            uint32_t nt, const uint32_t *tv, const float *six_rest, const float *vol_compliance,
                             uint32_t ng, const uint32_t *groups, uint32_t iterations, uint32_t frames, float dt,
                             float linear_damping, float max_linear_velocity, const float *gravity, float gravity_factor,
                             const double *compliance, const double *velocity, const double *grad_positions,
@@ -387,7 +453,7 @@ int xa_rollout_gradient_lra(int forward_precision, uint32_t nv, const float *pos
     if (forward_precision != 32 && forward_precision != 64)
         return -1;
     const Spec s = MakeSpec(nv, pos, inv_mass, ne, ev, rest, iterations, frames, dt, linear_damping, max_linear_velocity,
-                            gravity, gravity_factor, nl, lv, max_distance, ng, groups);
+                            gravity, gravity_factor, nl, lv, max_distance, nt, tv, six_rest, vol_compliance, ng, groups);
     Adjoint(s, compliance, velocity, grad_positions, grad_compliance, grad_velocity, positions, forward_precision);
     return 0;
 }

@@ -27,8 +27,11 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 
-def _make(frames, interval, compliance, lra=False):
-    case = scenes.config4_rollout(num_bodies=2, n=16, frames=frames, compliance=compliance, lra=lra)
+def _make(frames, interval, compliance, kind="edges"):
+    if kind == "tet_volumes":
+        case = scenes.rollout_tet_block(num_bodies=2, cells=3, frames=frames, compliance=compliance)
+    else:
+        case = scenes.config4_rollout(num_bodies=2, n=16, frames=frames, compliance=compliance, lra=kind == "edges_lra")
     s = build_settings(case.mesh)
     ctx = Context(0)
     ctx.set_gravity(case.gravity)
@@ -38,11 +41,15 @@ def _make(frames, interval, compliance, lra=False):
     return case, s, ctx, r
 
 
-@pytest.fixture(params=[False, True], ids=["edges", "edges_lra"])
+KINDS = ["edges", "edges_lra", "tet_volumes"]
+
+
+@pytest.fixture(params=KINDS)
 def setup(request):
     """2 cloths 16 x 16, 22 frames, checkpoints every 5 frames (the last segment is partial);
-    edge-only, and with LRA constraints from the pinned corners."""
-    case, s, ctx, r = _make(22, 5, 1e-4, lra=request.param)
+    edge-only, and with LRA constraints from the pinned corners; and 2 hanging 3^3-cell tet
+    blocks with volume constraints."""
+    case, s, ctx, r = _make(22, 5, 1e-4, kind=request.param)
     try:
         yield case, s, ctx, r
     finally:
@@ -51,13 +58,13 @@ def setup(request):
         ctx.close()
 
 
-@pytest.fixture(params=[False, True], ids=["edges", "edges_lra"])
+@pytest.fixture(params=KINDS)
 def setup_fd(request):
     """2 cloths 16 x 16, compliance 1e-3, 22 frames, checkpoints every 5 frames.  The
     finite-difference check compares the FP32 program's gradient with the FP64 one; on
     this case the two differ by 1e-4 to 3e-4 (norm over all parameters, measured with
     the oracle)."""
-    case, s, ctx, r = _make(22, 5, 1e-3, lra=request.param)
+    case, s, ctx, r = _make(22, 5, 1e-3, kind=request.param)
     try:
         yield case, s, ctx, r
     finally:
