@@ -1,5 +1,5 @@
-// Differentiable rollout of the soft-body update for bodies with edge, volume and LRA
-// constraints: float and double forward passes and the hand-written adjoint.
+// Differentiable rollout of the soft-body update for bodies with edge, dihedral bend, volume and
+// LRA constraints: float and double forward passes and the adjoint.
 //
 // The forward pass follows the update of the main kernels step for step
 // (SoftBodyMotionProperties.cpp: InitializeUpdateContext :937-960,
@@ -147,6 +147,151 @@ template <class T> __device__ __forceinline__ R3<T> rcross(R3<T> a, R3<T> b)
     return r3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);       // Vec3.inl:855-863
 }
 
+// Forward-mode dual numbers: a value and its derivative along one input direction.  The
+// adjoint of a dihedral bend is Jacobian-transpose times the output adjoints, its Jacobian
+// built column by column by running the projection on duals.
+template <class T> struct Dual { T v, d; };
+template <class T> __device__ __forceinline__ Dual<T> dual(T v, T d = (T)0) { Dual<T> r; r.v = v; r.d = d; return r; }
+template <class T> __device__ __forceinline__ Dual<T> operator+(Dual<T> a, Dual<T> b) { return dual(a.v + b.v, a.d + b.d); }
+template <class T> __device__ __forceinline__ Dual<T> operator-(Dual<T> a, Dual<T> b) { return dual(a.v - b.v, a.d - b.d); }
+template <class T> __device__ __forceinline__ Dual<T> operator*(Dual<T> a, Dual<T> b) { return dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+template <class T> __device__ __forceinline__ Dual<T> operator/(Dual<T> a, Dual<T> b)
+{
+    return dual(a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v));
+}
+// sqrt on duals; at 0 (a hinge whose cosine rounds to exactly 1, the cusp of ACosApproximate)
+// the derivative is taken as 0, a subgradient, where the formula would give 0 / 0.
+template <class T> __device__ __forceinline__ Dual<T> rsqrt_exact(Dual<T> a)
+{
+    const T s = rsqrt_exact(a.v);
+    return dual(s, s > (T)0 ? a.d / ((T)2 * s) : (T)0);
+}
+template <class T> __device__ __forceinline__ T value_of(T a) { return a; }
+template <class T> __device__ __forceinline__ T value_of(Dual<T> a) { return a.v; }
+// A constant in the arithmetic of S (a float literal converts exactly; pi rounds to the type)
+template <class S> struct Constant;
+template <> struct Constant<float> { static __device__ __forceinline__ float of(double c) { return (float)c; } };
+template <> struct Constant<double> { static __device__ __forceinline__ double of(double c) { return c; } };
+template <> struct Constant<Dual<double>> { static __device__ __forceinline__ Dual<double> of(double c) { return dual(c); } };
+template <class S> __device__ __forceinline__ S constant(double c) { return Constant<S>::of(c); }
+
+// ACosApproximate (Math/Trigonometry.h:48-65)
+template <class S> __device__ __forceinline__ S acos_approx_t(S x)
+{
+    const S one = constant<S>(1.0f);
+    const S ax = value_of(x) < 0.0f ? constant<S>(0.0f) - x : x;
+    const S abs_x = value_of(ax) < 1.0f ? ax : one;
+    const S half_pi = constant<S>(3.14159265358979323846) / constant<S>(2.0f);
+    const S v = rsqrt_exact(one - abs_x) * (half_pi - constant<S>(0.175394f) * abs_x);
+    return value_of(x) < 0.0f ? constant<S>(3.14159265358979323846) - v : v;
+}
+
+// ApplyDihedralBendConstraints body (SoftBodyMotionProperties.cpp:367-456) on positions x[4] of
+// scalar type S (float, or duals for the adjoint), writing the projected positions to y[4];
+// returns false where the constraint does nothing (y = x then).
+template <class S, class P>
+__device__ __forceinline__ bool bend_generic(const R3<S> *x, const float *w, float compliance, float initial_angle, P inv_dt_sq,
+                                             R3<S> *y)
+{
+    for (int k = 0; k < 4; ++k)
+        y[k] = x[k];
+    const R3<S> e = x[1] - x[0];
+    const S e_len = rsqrt_exact(rdot(e, e));
+    if (value_of(e_len) < 1.0e-6f)
+        return false;
+    const R3<S> x1x2 = x[2] - x[1], x1x3 = x[3] - x[1];
+    R3<S> n1 = rcross(x[2] - x[0], x1x2);
+    R3<S> n2 = rcross(x1x3, x[3] - x[0]);
+    const S n1sq = rdot(n1, n1), n2sq = rdot(n2, n2);
+    const S n1n2 = n1sq * n2sq;
+    if (value_of(n1n2) < 1.0e-24f)
+        return false;
+    const S sign = constant<S>(value_of(rdot(rcross(n2, n1), e)) < 0.0f ? -1.0f : 1.0f);
+    const S d = rdot(n1, n2) / rsqrt_exact(n1n2);
+    S c = sign * acos_approx_t(d) - constant<S>(initial_angle);
+    const S pi = constant<S>(3.14159265358979323846);
+    if (value_of(c) > value_of(pi))
+        c = c - constant<S>(2.0f) * pi;
+    else if (value_of(c) < -value_of(pi))
+        c = c + constant<S>(2.0f) * pi;
+    n1 = n1 / n1sq;
+    n2 = n2 / n2sq;
+    const R3<S> d0c = (rdot(x1x2, e) * n1 + rdot(x1x3, e) * n2) / e_len;
+    const R3<S> d2c = e_len * n1;
+    const R3<S> d3c = e_len * n2;
+    const R3<S> d1c = ((r3(constant<S>(0.0f), constant<S>(0.0f), constant<S>(0.0f)) - d0c) - d2c) - d3c;
+    const S w0 = constant<S>(w[0]), w1 = constant<S>(w[1]), w2 = constant<S>(w[2]), w3 = constant<S>(w[3]);
+    const S denom = (((w0 * rdot(d0c, d0c) + w1 * rdot(d1c, d1c)) + w2 * rdot(d2c, d2c)) + w3 * rdot(d3c, d3c))
+                    + constant<S>(compliance) * constant<S>(inv_dt_sq);
+    if (value_of(denom) < 1.0e-12f)
+        return false;
+    const S ml = c / denom;
+    y[0] = x[0] - (ml * w0) * d0c;
+    y[1] = x[1] - (ml * w1) * d1c;
+    y[2] = x[2] - (ml * w2) * d2c;
+    y[3] = x[3] - (ml * w3) * d3c;
+    return true;
+}
+
+template <class T>
+__device__ __forceinline__ void bend_project(R4<T> *X, uint32_t i0, uint32_t i1, uint32_t i2, uint32_t i3, float compliance,
+                                             float initial_angle, T inv_dt_sq)
+{
+    const uint32_t idx[4] = { i0, i1, i2, i3 };
+    R3<T> x[4], y[4];
+    float w[4];
+    for (int k = 0; k < 4; ++k)
+    {
+        x[k] = xyz3(X[idx[k]]);
+        w[k] = (float)X[idx[k]].w;
+    }
+    if (bend_generic<T, T>(x, w, compliance, initial_angle, inv_dt_sq, y))
+        for (int k = 0; k < 4; ++k)
+            X[idx[k]] = r4(y[k], X[idx[k]].w);
+}
+
+// Adjoint of bend_project from its pre-projection positions q[4]: xbar = J^T ybar, the
+// Jacobian J of the projection taken one input component at a time on duals.  The Jacobian is
+// evaluated in double at the float positions: near a flat hinge the derivative of
+// ACosApproximate, 1 / (2 sqrt(1 - |x|)), magnifies the rounding of 1 - |x| and float duals lose
+// a few digits there.
+__device__ __forceinline__ void bend_adjoint(float4 *A, uint32_t i0, uint32_t i1, uint32_t i2, uint32_t i3, const float4 *q,
+                                             float compliance, float initial_angle, float inv_dt_sq)
+{
+    const uint32_t idx[4] = { i0, i1, i2, i3 };
+    float w[4];
+    R3<float> ybar[4];
+    for (int k = 0; k < 4; ++k)
+    {
+        w[k] = q[k].w;
+        const float4 a = A[idx[k]];
+        ybar[k] = r3(a.x, a.y, a.z);
+    }
+    double xbar[12];
+    for (int col = 0; col < 12; ++col)
+    {
+        R3<Dual<double>> x[4], y[4];
+        for (int k = 0; k < 4; ++k)
+            x[k] = r3(dual((double)q[k].x, col == 3 * k ? 1.0 : 0.0), dual((double)q[k].y, col == 3 * k + 1 ? 1.0 : 0.0),
+                      dual((double)q[k].z, col == 3 * k + 2 ? 1.0 : 0.0));
+        bend_generic<Dual<double>, double>(x, w, compliance, initial_angle, (double)inv_dt_sq, y);
+        double acc = 0.0;
+        for (int k = 0; k < 4; ++k)
+            acc += (y[k].x.d * (double)ybar[k].x + y[k].y.d * (double)ybar[k].y) + y[k].z.d * (double)ybar[k].z;
+        xbar[col] = acc;
+    }
+    for (int k = 0; k < 4; ++k)
+    {
+        const float4 a = A[idx[k]];
+        A[idx[k]] = make_float4((float)xbar[3 * k], (float)xbar[3 * k + 1], (float)xbar[3 * k + 2], a.w);
+    }
+}
+
+// Record type: 1 for a dihedral bend, whose second-lane half (bit 15 of the second word) the
+// rollout skips (it runs each bend on one lane)
+__device__ __forceinline__ bool is_bend_record(const uint4 &q) { return ((q.x >> 14) & 3u) == 1u; }
+__device__ __forceinline__ bool is_second_half(const uint4 &q) { return (q.y & 0x8000u) != 0u; }
+
 // ApplyVolumeConstraints body (SoftBodyMotionProperties.cpp:467-506)
 template <class T>
 __device__ __forceinline__ void volume_project(R4<T> *X, uint32_t i1, uint32_t i2, uint32_t i3, uint32_t i4, T six_rest_volume,
@@ -270,6 +415,12 @@ __device__ __forceinline__ void forward_substep(const RolloutBodyDev &B, const S
             return;
         if (is_lra_record(q))
             lra_project<T>(X, q.x & 0x3fffu, q.x >> 16, (T)__uint_as_float(q.z));
+        else if (is_bend_record(q))
+        {
+            if (!is_second_half(q))
+                bend_project<T>(X, q.x & 0x3fffu, q.x >> 16, q.y & 0x7fffu, q.y >> 16, __uint_as_float(q.z),
+                                __uint_as_float(q.w), K.inv_dt_sq);
+        }
         else if (is_volume_record(q))
             volume_project<T>(X, q.x & 0x3fffu, q.x >> 16, q.y & 0x7fffu, q.y >> 16, (T)__uint_as_float(q.z),
                               (T)__uint_as_float(q.w), K.inv_dt_sq);
@@ -314,7 +465,7 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_forward_kernel(RolloutD
         for (uint32_t r = tid; r < B.num_records; r += kRolloutBlock)
         {
             uint4 q = recs[r];
-            if (q.x != kNopRecord && !is_lra_record(q) && !is_volume_record(q))
+            if (q.x != kNopRecord && !is_lra_record(q) && !is_volume_record(q) && !is_bend_record(q))
             {
                 q.w = __float_as_uint((float)compliance[B.eoff + q.y]);
                 recs[r] = q;
@@ -440,6 +591,16 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_backward_kernel(Rollout
                     tape[4u * r + 3u] = A[i3];
                     volume_project<float>(X, i0, i1, i2, i3, __uint_as_float(q.z), __uint_as_float(q.w), K.inv_dt_sq);
                 }
+                else if (is_bend_record(q))
+                {
+                    if (!is_second_half(q))
+                    {
+                        const uint32_t i2 = q.y & 0x7fffu, i3 = q.y >> 16;
+                        tape[4u * r + 2u] = A[i2];
+                        tape[4u * r + 3u] = A[i3];
+                        bend_project<float>(X, i0, i1, i2, i3, __uint_as_float(q.z), __uint_as_float(q.w), K.inv_dt_sq);
+                    }
+                }
                 else if (is_lra_record(q))
                     lra_project<float>(X, i0, i1, __uint_as_float(q.z));
                 else
@@ -479,6 +640,13 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_backward_kernel(Rollout
                 if (is_lra_record(q))
                 {
                     lra_adjoint(A, q.x & 0x3fffu, q.x >> 16, tape[4u * r], tape[4u * r + 1u], __uint_as_float(q.z));
+                    return;
+                }
+                if (is_bend_record(q))
+                {
+                    if (!is_second_half(q))
+                        bend_adjoint(A, q.x & 0x3fffu, q.x >> 16, q.y & 0x7fffu, q.y >> 16, tape + 4u * r, __uint_as_float(q.z),
+                                     __uint_as_float(q.w), K.inv_dt_sq);
                     return;
                 }
                 if (is_volume_record(q))

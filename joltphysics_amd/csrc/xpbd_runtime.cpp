@@ -3141,8 +3141,6 @@ int xpbd_rollout_create(xpbd_context *ctx, const xpbd_rollout_desc *d, xpbd_roll
         xpbd_settings *s = d->settings[b];
         if (!s || s->ctx != ctx)
             return Fail(XPBD_E_INVALID, "settings " + std::to_string(b) + " missing or from another context");
-        if (s->num_bends)
-            return Fail(XPBD_E_UNSUPPORTED, "rollout bodies may have edge, volume and LRA constraints only (body " + std::to_string(b) + ")");
         if (s->host_level_begin.empty())
             return Fail(XPBD_E_CAPACITY, "rollout body " + std::to_string(b) + " does not fit in LDS");
         const uint32_t nv = s->num_vertices, ne = s->num_edges;
@@ -3164,11 +3162,11 @@ int xpbd_rollout_create(xpbd_context *ctx, const xpbd_rollout_desc *d, xpbd_roll
         for (size_t k = 0; k < s->host_recs.size(); ++k)
         {
             uint4 q = s->host_recs[k];
-            if (q.x != kNopRecordHost && ((q.x >> 14) & 3u) != kTypeVolume)
+            if (q.x != kNopRecordHost && ((q.x >> 14) & 3u) != kTypeVolume && ((q.x >> 14) & 3u) != kTypeBend)
             {
                 // edges: their index in the body (for the compliance parameters) and a compliance
-                // written by each float forward pass; LRAs: their index.  Volume records keep their
-                // vertices 2-3 and compliance.
+                // written by each float forward pass; LRAs: their index.  Volume and bend records
+                // keep their vertices 2-3 and parameters.
                 q.y = s->host_rec_index[k];
                 q.w = 0u;
             }

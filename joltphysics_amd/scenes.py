@@ -529,18 +529,23 @@ class RolloutCase:
     checkpoint_interval: int = 10
 
 
-def config4_rollout(num_bodies=64, n=64, frames=60, seed=4, compliance=1e-3, lra=False):
+def config4_rollout(num_bodies=64, n=64, frames=60, seed=4, compliance=1e-3, lra=False, bends=False):
     """64 cloths of 64 x 64 vertices (spacing 0.05 m, inv-mass 100, two top corners
     pinned), structural + shear edges only (compliance 1e-3), gravity -9.81 z, dt 1/60,
     one collision step x 5 iterations, 60 frames, no colliders.  Initial velocities
     U(-0.1, 0.1) m/s and target compliance multipliers U(0.5, 2.0) per edge, from `seed`.
     lra: also Euclidean LRA constraints from the pinned corners (max distance 1.0 x the rest
-    distance), which act once the sheet stretches."""
-    v, f = cloth(n, 0.05, 100.0, pinned=((n - 1) * n, n * n - 1))
-    a = attrs(compliance, compliance, 0.0, lra_type=LRA_EUCLIDEAN, lra_mult=1.0) if lra else attrs(compliance, compliance, 0.0)
-    mesh = MeshSpec(vertices=v, faces=f, attrs=a, bend_type=BEND_NONE)
+    distance), which act once the sheet stretches.  bends: also dihedral bends (compliance
+    1e-3) on a sheet jittered by up to 1 cm, so that no hinge starts flat."""
+    v, f = cloth(n, 0.05, 100.0, pinned=((n - 1) * n, n * n - 1), jitter=0.01 if bends else 0.0, seed=seed)
+    a = attrs(compliance, compliance, 1e-3 if bends else 0.0, lra_type=LRA_EUCLIDEAN if lra else LRA_NONE, lra_mult=1.0)
+    mesh = MeshSpec(vertices=v, faces=f, attrs=a, bend_type=BEND_DIHEDRAL if bends else BEND_NONE)
     rng = np.random.Generator(np.random.PCG64(seed))
     num_edges = 2 * n * (n - 1) + 2 * (n - 1) * (n - 1)     # structural + shear
+    if bends:
+        # with dihedral bends CreateConstraints makes another edge set: count it
+        from .xpbd import build_settings
+        num_edges = len(build_settings(mesh).edges)
     comp = np.full(num_bodies * num_edges, compliance, np.float32)
     vel = rng.uniform(-0.1, 0.1, size=(num_bodies * n * n, 3)).astype(np.float32)
     scale = rng.uniform(0.5, 2.0, size=num_bodies * num_edges).astype(np.float32)

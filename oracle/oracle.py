@@ -723,7 +723,7 @@ class Restatement:
 
 
 class RolloutReference:
-    """CPU restatement of the differentiable rollout of edge, volume and LRA bodies (oracle/xpbd_adjoint.cpp):
+    """CPU restatement of the differentiable rollout of edge, bend, volume and LRA bodies (oracle/xpbd_adjoint.cpp):
     FP32 or FP64 forward pass and the FP64 adjoint from a full tape."""
 
     _lib = None
@@ -737,16 +737,18 @@ class RolloutReference:
             lib.xa_rollout_forward.argtypes = [_i] + common + [_vp, _vp, _vp]
             lib.xa_rollout_gradient.restype = _i
             lib.xa_rollout_gradient.argtypes = [_i] + common + [_vp] * 6
-            common_lra = [_u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _vp, _u32, _vp, _u32, _u32, _f, _f, _f,
-                          _vp, _f]
+            common_lra = [_u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _u32, _vp,
+                          _u32, _u32, _f, _f, _f, _vp, _f]
             lib.xa_rollout_forward_lra.restype = _i
             lib.xa_rollout_forward_lra.argtypes = [_i] + common_lra + [_vp, _vp, _vp]
             lib.xa_rollout_gradient_lra.restype = _i
             lib.xa_rollout_gradient_lra.argtypes = [_i] + common_lra + [_vp] * 6
             RolloutReference._lib = lib
         self.lib = RolloutReference._lib
-        if len(settings.bends):
-            raise ValueError("the rollout oracle covers edge, volume and LRA constraints only")
+        self.nb = len(settings.bends)
+        self.bv = np.ascontiguousarray(settings.bends["v"], np.uint32).reshape(-1) if self.nb else np.zeros(4, np.uint32)
+        self.bend_comp = np.ascontiguousarray(settings.bends["compliance"], np.float32) if self.nb else np.zeros(1, np.float32)
+        self.angle = np.ascontiguousarray(settings.bends["initial_angle"], np.float32) if self.nb else np.zeros(1, np.float32)
         self.nt = len(settings.volumes)
         self.tv = np.ascontiguousarray(settings.volumes["v"], np.uint32).reshape(-1) if self.nt else np.zeros(4, np.uint32)
         self.six_rest = np.ascontiguousarray(settings.volumes["six_rest_volume"], np.float32) if self.nt else np.zeros(1, np.float32)
@@ -755,7 +757,8 @@ class RolloutReference:
         self.lv = np.ascontiguousarray(settings.lras["v"], np.uint32).reshape(-1) if self.nl else np.zeros(2, np.uint32)
         self.maxd = np.ascontiguousarray(settings.lras["max_distance"], np.float32) if self.nl else np.zeros(1, np.float32)
         g = settings.groups
-        self.groups = np.ascontiguousarray(np.stack([g["volume_end"], g["edge_end"], g["lra_end"]], axis=1), np.uint32).reshape(-1)
+        self.groups = np.ascontiguousarray(np.stack([g["volume_end"], g["bend_end"], g["edge_end"], g["lra_end"]], axis=1),
+                                           np.uint32).reshape(-1)
         self.ng = len(g)
         v = settings.vertices
         self.nv, self.ne = len(v), len(settings.edges)
@@ -769,7 +772,8 @@ class RolloutReference:
 
     def _common(self):
         return [self.nv, _ptr(self.pos), _ptr(self.inv_mass), self.ne, _ptr(self.ev), _ptr(self.rest), self.nl, _ptr(self.lv),
-                _ptr(self.maxd), self.nt, _ptr(self.tv), _ptr(self.six_rest), _ptr(self.vol_comp), self.ng, _ptr(self.groups),
+                _ptr(self.maxd), self.nt, _ptr(self.tv), _ptr(self.six_rest), _ptr(self.vol_comp), self.nb, _ptr(self.bv),
+                _ptr(self.bend_comp), _ptr(self.angle), self.ng, _ptr(self.groups),
                 *self.args, _ptr(self.gravity), self.gravity_factor]
 
     def forward(self, compliance, velocity, precision=64):

@@ -4,13 +4,14 @@
 // (include/jolt_xpbd.h, xpbd_rollout_*).  It is loaded by tests/ as the
 // checker; the product never links, loads or calls it.
 //
-// Scope: one body with edge, volume and LRA constraints, no colliders, no pressure, one
+// Scope: one body with edge, dihedral bend, volume and LRA constraints, no colliders, no pressure, one
 // collision step per frame, identity rotation.  The update is restated from
 // the reference (paths relative to its Jolt/ directory):
 //   InitializeUpdateContext        Physics/SoftBody/SoftBodyMotionProperties.cpp:937-960
 //   IntegratePositions             :325-350
 //   ProcessGroup :1030-1060 -- the update groups in order, in each its volumes
-//       (ApplyVolumeConstraints :458-507), its edges (ApplyEdgeConstraints :573-601),
+//       (ApplyVolumeConstraints :458-507), its bends (ApplyDihedralBendConstraints
+//       :367-456), its edges (ApplyEdgeConstraints :573-601),
 //       then its LRAs (ApplyLRAConstraints :679-698), each in array order: a
 //       sequential Gauss-Seidel sweep
 //   velocity update                :700-710 (no colliding shape)
@@ -40,6 +41,72 @@ template <class F> inline V<F> operator/(V<F> a, F s) { return mk(a.x / s, a.y /
 template <class F> inline F dot(V<F> a, V<F> b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }   // Vec3.inl:901-934
 template <class F> inline V<F> cross(V<F> a, V<F> b) { return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
 
+// Forward-mode dual numbers (value, derivative) for the bend's Jacobian in the adjoint
+struct D { double v, d; };
+inline D operator+(D a, D b) { return { a.v + b.v, a.d + b.d }; }
+inline D operator-(D a, D b) { return { a.v - b.v, a.d - b.d }; }
+inline D operator*(D a, D b) { return { a.v * b.v, a.d * b.v + a.v * b.d }; }
+inline D operator/(D a, D b) { return { a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v) }; }
+inline D Sqrt(D a) { const double r = std::sqrt(a.v); return { r, r > 0.0 ? a.d / (2.0 * r) : 0.0 }; }   // 0 at the cusp
+inline float Sqrt(float a) { return std::sqrt(a); }
+inline double Sqrt(double a) { return std::sqrt(a); }
+inline double Val(D a) { return a.v; }
+inline double Val(double a) { return a; }
+inline float Val(float a) { return a; }
+template <class S> inline S Cst(double c);
+template <> inline float Cst<float>(double c) { return (float)c; }
+template <> inline double Cst<double>(double c) { return c; }
+template <> inline D Cst<D>(double c) { return { c, 0.0 }; }
+
+// ACosApproximate (Math/Trigonometry.h:48-65)
+template <class S> S ACosApprox(S x)
+{
+    const S ax = Val(x) < 0.0 ? Cst<S>(0.0) - x : x;
+    const S abs_x = Val(ax) < 1.0 ? ax : Cst<S>(1.0);
+    const S val = Sqrt(Cst<S>(1.0) - abs_x) * (Cst<S>(3.14159265358979323846) / Cst<S>(2.0) - Cst<S>(0.175394f) * abs_x);
+    return Val(x) < 0.0 ? Cst<S>(3.14159265358979323846) - val : val;
+}
+
+// ApplyDihedralBendConstraints body (SoftBodyMotionProperties.cpp:376-455) on x[4], into y[4]
+template <class S, class P>
+void Bend(const V<S> *x, const float *w, float compliance, float initial_angle, P inv_dt_sq, V<S> *y)
+{
+    for (int k = 0; k < 4; ++k)
+        y[k] = x[k];
+    const V<S> e = x[1] - x[0];
+    const S e_len = Sqrt(dot(e, e));
+    if (Val(e_len) < 1.0e-6f)
+        return;
+    const V<S> x1x2 = x[2] - x[1], x1x3 = x[3] - x[1];
+    V<S> n1 = cross(x[2] - x[0], x1x2), n2 = cross(x1x3, x[3] - x[0]);
+    const S n1sq = dot(n1, n1), n2sq = dot(n2, n2), n1n2 = n1sq * n2sq;
+    if (Val(n1n2) < 1.0e-24f)
+        return;
+    const S sign = Cst<S>(Val(dot(cross(n2, n1), e)) < 0.0 ? -1.0 : 1.0);
+    const S dd = dot(n1, n2) / Sqrt(n1n2);
+    S c = sign * ACosApprox(dd) - Cst<S>(initial_angle);
+    const S pi = Cst<S>(3.14159265358979323846);
+    if (Val(c) > Val(pi))
+        c = c - Cst<S>(2.0) * pi;
+    else if (Val(c) < -Val(pi))
+        c = c + Cst<S>(2.0) * pi;
+    n1 = n1 / n1sq;
+    n2 = n2 / n2sq;
+    const V<S> d0c = (dot(x1x2, e) * n1 + dot(x1x3, e) * n2) / e_len, d2c = e_len * n1, d3c = e_len * n2;
+    const V<S> zero = mk(Cst<S>(0.0), Cst<S>(0.0), Cst<S>(0.0));
+    const V<S> d1c = ((zero - d0c) - d2c) - d3c;
+    const S w0 = Cst<S>(w[0]), w1 = Cst<S>(w[1]), w2 = Cst<S>(w[2]), w3 = Cst<S>(w[3]);
+    const S denom = (((w0 * dot(d0c, d0c) + w1 * dot(d1c, d1c)) + w2 * dot(d2c, d2c)) + w3 * dot(d3c, d3c))
+                    + Cst<S>(compliance) * Cst<S>((double)inv_dt_sq);
+    if (Val(denom) < 1.0e-12f)
+        return;
+    const S ml = c / denom;
+    y[0] = x[0] - (ml * w0) * d0c;
+    y[1] = x[1] - (ml * w1) * d1c;
+    y[2] = x[2] - (ml * w2) * d2c;
+    y[3] = x[3] - (ml * w3) * d3c;
+}
+
 struct Spec
 {
     uint32_t nv, ne;
@@ -53,7 +120,10 @@ struct Spec
     const float *max_distance;
     const uint32_t *tv;     // 4 per volume
     const float *six_rest, *vol_compliance;
-    // processing order: (0, edge index), (1, LRA index) or (2, volume index) per projection
+    const uint32_t *bv;     // 4 per dihedral bend
+    const float *bend_compliance, *initial_angle;
+    // processing order: (0, edge index), (1, LRA index), (2, volume index) or (3, bend index)
+    // per projection
     std::vector<std::pair<int, uint32_t>> seq;
 };
 
@@ -127,15 +197,27 @@ void Forward(const Spec &s, const F *compliance, const F *velocity, F *out, Tape
             for (const auto &op : s.seq)
             {
                 const uint32_t e = op.second;
-                const uint32_t *vs = op.first == 2 ? s.tv + 4 * e : op.first == 1 ? s.lv + 2 * e : s.ev + 2 * e;
+                const uint32_t *vs = op.first == 3 ? s.bv + 4 * e : op.first == 2 ? s.tv + 4 * e : op.first == 1 ? s.lv + 2 * e : s.ev + 2 * e;
                 const uint32_t i0 = vs[0], i1 = vs[1];
                 if (tape)
                     for (int j = 0; j < 4; ++j)
                     {
-                        const uint32_t vj = j < (op.first == 2 ? 4 : 2) ? vs[j] : vs[0];
+                        const uint32_t vj = j < (op.first >= 2 ? 4 : 2) ? vs[j] : vs[0];
                         const double a[3] = { (double)x[vj].x, (double)x[vj].y, (double)x[vj].z };
                         tape->pre.insert(tape->pre.end(), a, a + 3);
                     }
+                if (op.first == 3)                              // ApplyDihedralBendConstraints :376-455
+                {
+                    if (tape)
+                        tape->acted.push_back(0);
+                    const V<F> xs[4] = { x[vs[0]], x[vs[1]], x[vs[2]], x[vs[3]] };
+                    const float ws[4] = { (float)w[vs[0]], (float)w[vs[1]], (float)w[vs[2]], (float)w[vs[3]] };
+                    V<F> ys[4];
+                    Bend<F, F>(xs, ws, s.bend_compliance[e], s.initial_angle[e], K.inv_dt_sq, ys);
+                    for (int j = 0; j < 4; ++j)
+                        x[vs[j]] = ys[j];
+                    continue;
+                }
                 if (op.first == 2)                              // ApplyVolumeConstraints :467-506
                 {
                     if (tape)
@@ -260,6 +342,28 @@ void Adjoint(const Spec &s, const double *compliance, const double *velocity, co
             const uint32_t e = s.seq[k].second;
             const double *q = pre + 12 * k;
             const V<double> p0 = mk(q[0], q[1], q[2]), p1 = mk(q[3], q[4], q[5]);
+            if (s.seq[k].first == 3)
+            {
+                // xbar = J^T ybar, J column by column on duals from the pre-projection positions
+                const uint32_t *vs = s.bv + 4 * e;
+                const float ws[4] = { (float)s.inv_mass[vs[0]], (float)s.inv_mass[vs[1]], (float)s.inv_mass[vs[2]], (float)s.inv_mass[vs[3]] };
+                double xbar[12];
+                for (int col = 0; col < 12; ++col)
+                {
+                    V<D> xs[4], ys[4];
+                    for (int j = 0; j < 4; ++j)
+                        xs[j] = mk(D{ q[3 * j], col == 3 * j ? 1.0 : 0.0 }, D{ q[3 * j + 1], col == 3 * j + 1 ? 1.0 : 0.0 },
+                                   D{ q[3 * j + 2], col == 3 * j + 2 ? 1.0 : 0.0 });
+                    Bend<D, double>(xs, ws, s.bend_compliance[e], s.initial_angle[e], K.inv_dt_sq, ys);
+                    double acc = 0.0;
+                    for (int j = 0; j < 4; ++j)
+                        acc += ys[j].x.d * A[vs[j]].x + ys[j].y.d * A[vs[j]].y + ys[j].z.d * A[vs[j]].z;
+                    xbar[col] = acc;
+                }
+                for (int j = 0; j < 4; ++j)
+                    A[vs[j]] = mk(xbar[3 * j], xbar[3 * j + 1], xbar[3 * j + 2]);
+                continue;
+            }
             if (s.seq[k].first == 2)
             {
                 // y_i = x_i - mu w_i d_i, mu = c / D, c = |6V| - 6V0, d_i = grad_i 6V: see the device's volume_adjoint
@@ -346,28 +450,34 @@ Spec MakeSpec(uint32_t nv, const float *pos, const float *inv_mass, uint32_t ne,
               uint32_t iterations, uint32_t frames, float dt, float linear_damping, float max_linear_velocity,
               const float *gravity, float gravity_factor, uint32_t nl = 0, const uint32_t *lv = nullptr,
               const float *max_distance = nullptr, uint32_t nt = 0, const uint32_t *tv = nullptr, const float *six_rest = nullptr,
-              const float *vol_compliance = nullptr, uint32_t ng = 0, const uint32_t *groups = nullptr)
+              const float *vol_compliance = nullptr, uint32_t nb = 0, const uint32_t *bv = nullptr,
+              const float *bend_compliance = nullptr, const float *initial_angle = nullptr, uint32_t ng = 0,
+              const uint32_t *groups = nullptr)
 {
     Spec s;
     s.nv = nv; s.ne = ne; s.pos = pos; s.inv_mass = inv_mass; s.ev = ev; s.rest = rest;
     s.lv = lv; s.max_distance = max_distance;
     s.tv = tv; s.six_rest = six_rest; s.vol_compliance = vol_compliance;
+    s.bv = bv; s.bend_compliance = bend_compliance; s.initial_angle = initial_angle;
     if (ng == 0)
         for (uint32_t e = 0; e < ne; ++e)
             s.seq.push_back({ 0, e });
-    uint32_t t0 = 0, e0 = 0, l0 = 0;
+    uint32_t t0 = 0, b0 = 0, e0 = 0, l0 = 0;
     for (uint32_t g = 0; g < ng; ++g)
     {
-        for (uint32_t t = t0; t < groups[3 * g] && t < nt; ++t)
+        for (uint32_t t = t0; t < groups[4 * g] && t < nt; ++t)
             s.seq.push_back({ 2, t });
-        for (uint32_t e = e0; e < groups[3 * g + 1] && e < ne; ++e)
+        for (uint32_t b = b0; b < groups[4 * g + 1] && b < nb; ++b)
+            s.seq.push_back({ 3, b });
+        for (uint32_t e = e0; e < groups[4 * g + 2] && e < ne; ++e)
             s.seq.push_back({ 0, e });
-        for (uint32_t l = l0; l < groups[3 * g + 2] && l < nl; ++l)
+        for (uint32_t l = l0; l < groups[4 * g + 3] && l < nl; ++l)
             if (lv[2 * l] != lv[2 * l + 1])         // an LRA of a vertex to itself never moves it
                 s.seq.push_back({ 1, l });
-        t0 = groups[3 * g];
-        e0 = groups[3 * g + 1];
-        l0 = groups[3 * g + 2];
+        t0 = groups[4 * g];
+        b0 = groups[4 * g + 1];
+        e0 = groups[4 * g + 2];
+        l0 = groups[4 * g + 3];
     }
     s.iterations = iterations; s.frames = frames; s.dt = dt; s.linear_damping = linear_damping;
     s.max_linear_velocity = max_linear_velocity;
@@ -416,18 +526,20 @@ int xa_rollout_gradient(int forward_precision, uint32_t nv, const float *pos, co
     return 0;
 }
 
-// The two entry points above for a body with volume and LRA constraints: nl LRAs (2 vertices
-// each, max distance), nt volumes (4 vertices, six rest volume, compliance each), ng update
-// groups (volume end, edge end, LRA end each).
+// The two entry points above for a body with bend, volume and LRA constraints: nl LRAs (2
+// vertices each, max distance), nt volumes (4 vertices, six rest volume, compliance each), nb
+// dihedral bends (4 vertices, compliance, initial angle each), ng update groups (volume end,
+// bend end, edge end, LRA end each).
 int xa_rollout_forward_lra(int precision, uint32_t nv, const float *pos, const float *inv_mass, uint32_t ne, const uint32_t *ev,
                            const float *rest, uint32_t nl, const uint32_t *lv, const float *max_distance, uint32_t nt,
-                           const uint32_t *tv, const float *six_rest, const float *vol_compliance, uint32_t ng,
+                           const uint32_t *tv, const float *six_rest, const float *vol_compliance, uint32_t nb,
+                           const uint32_t *bv, const float *bend_compliance, const float *initial_angle, uint32_t ng,
                            const uint32_t *groups, uint32_t iterations, uint32_t frames, float dt, float linear_damping,
                            float max_linear_velocity, const float *gravity, float gravity_factor, const double *compliance,
                            const double *velocity, double *positions)
 {
     const Spec s = MakeSpec(nv, pos, inv_mass, ne, ev, rest, iterations, frames, dt, linear_damping, max_linear_velocity,
-                            gravity, gravity_factor, nl, lv, max_distance, nt, tv, six_rest, vol_compliance, ng, groups);
+                            gravity, gravity_factor, nl, lv, max_distance, nt, tv, six_rest, vol_compliance, nb, bv, bend_compliance, initial_angle, ng, groups);
     if (precision == 64)
     {
         Forward<double>(s, compliance, velocity, positions, nullptr);
@@ -445,6 +557,7 @@ int xa_rollout_forward_lra(int precision, uint32_t nv, const float *pos, const f
 int xa_rollout_gradient_lra(int forward_precision, uint32_t nv, const float *pos, const float *inv_mass, uint32_t ne,
                             const uint32_t *ev, const float *rest, uint32_t nl, const uint32_t *lv, const float *max_distance,
                             uint32_t nt, const uint32_t *tv, const float *six_rest, const float *vol_compliance,
+                            uint32_t nb, const uint32_t *bv, const float *bend_compliance, const float *initial_angle,
                             uint32_t ng, const uint32_t *groups, uint32_t iterations, uint32_t frames, float dt,
                             float linear_damping, float max_linear_velocity, const float *gravity, float gravity_factor,
                             const double *compliance, const double *velocity, const double *grad_positions,
@@ -453,7 +566,7 @@ int xa_rollout_gradient_lra(int forward_precision, uint32_t nv, const float *pos
     if (forward_precision != 32 && forward_precision != 64)
         return -1;
     const Spec s = MakeSpec(nv, pos, inv_mass, ne, ev, rest, iterations, frames, dt, linear_damping, max_linear_velocity,
-                            gravity, gravity_factor, nl, lv, max_distance, nt, tv, six_rest, vol_compliance, ng, groups);
+                            gravity, gravity_factor, nl, lv, max_distance, nt, tv, six_rest, vol_compliance, nb, bv, bend_compliance, initial_angle, ng, groups);
     Adjoint(s, compliance, velocity, grad_positions, grad_compliance, grad_velocity, positions, forward_precision);
     return 0;
 }

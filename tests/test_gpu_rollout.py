@@ -31,7 +31,8 @@ def _make(frames, interval, compliance, kind="edges"):
     if kind == "tet_volumes":
         case = scenes.rollout_tet_block(num_bodies=2, cells=3, frames=frames, compliance=compliance)
     else:
-        case = scenes.config4_rollout(num_bodies=2, n=16, frames=frames, compliance=compliance, lra=kind == "edges_lra")
+        case = scenes.config4_rollout(num_bodies=2, n=16, frames=frames, compliance=compliance, lra=kind == "edges_lra",
+                                      bends=kind == "edges_bends")
     s = build_settings(case.mesh)
     ctx = Context(0)
     ctx.set_gravity(case.gravity)
@@ -41,14 +42,14 @@ def _make(frames, interval, compliance, kind="edges"):
     return case, s, ctx, r
 
 
-KINDS = ["edges", "edges_lra", "tet_volumes"]
+KINDS = ["edges", "edges_lra", "tet_volumes", "edges_bends"]
 
 
 @pytest.fixture(params=KINDS)
 def setup(request):
     """2 cloths 16 x 16, 22 frames, checkpoints every 5 frames (the last segment is partial);
-    edge-only, and with LRA constraints from the pinned corners; and 2 hanging 3^3-cell tet
-    blocks with volume constraints."""
+    edge-only, with LRA constraints from the pinned corners, and with dihedral bends (a
+    jittered sheet); and 2 hanging 3^3-cell tet blocks with volume constraints."""
     case, s, ctx, r = _make(22, 5, 1e-4, kind=request.param)
     try:
         yield case, s, ctx, r

@@ -20,9 +20,9 @@ from oracle import RolloutReference, have_restatement
 pytestmark = pytest.mark.skipif(not have_restatement(), reason="oracle restatement not built (make -C oracle restatement)")
 
 
-def _case(n=6, frames=8, lra=False, tet=False):
+def _case(n=6, frames=8, lra=False, tet=False, bends=False):
     case = scenes.rollout_tet_block(num_bodies=1, cells=2, frames=frames) if tet else \
-        scenes.config4_rollout(num_bodies=1, n=n, frames=frames, lra=lra)
+        scenes.config4_rollout(num_bodies=1, n=n, frames=frames, lra=lra, bends=bends)
     s = build_settings(case.mesh)
     ref = RolloutReference(s, iterations=case.iterations, frames=frames, dt=case.dt, gravity=case.gravity)
     comp = case.compliance.astype(np.float64)
@@ -36,13 +36,16 @@ def _loss(ref, target, c, v):
     return float(np.mean(np.sum((x - target) ** 2, axis=1)))
 
 
-@pytest.mark.parametrize("lra,tet", [(False, False), (True, False), (False, True)], ids=["edges", "edges_lra", "tet_volumes"])
-def test_adjoint_matches_central_differences_fp64(lra, tet):
+@pytest.mark.parametrize("lra,tet,bends", [(False, False, False), (True, False, False), (False, True, False), (False, False, True)],
+                         ids=["edges", "edges_lra", "tet_volumes", "edges_bends"])
+def test_adjoint_matches_central_differences_fp64(lra, tet, bends):
     """The FP64 adjoint against central differences of the FP64 forward pass: edge-only, with
     LRA constraints (some of which act: checked), and a hanging tet block with volumes."""
-    case, s, ref, comp, vel, target = _case(lra=lra, tet=tet)
+    case, s, ref, comp, vel, target = _case(lra=lra, tet=tet, bends=bends)
     if tet:
         assert len(s.volumes) > 0
+    if bends:
+        assert len(s.bends) > 0
     if lra:
         assert len(s.lras) > 0
         stretched = ref.forward(comp, vel)
@@ -57,7 +60,7 @@ def test_adjoint_matches_central_differences_fp64(lra, tet):
     assert np.array_equal(x, x2)
     scale_c = np.abs(gc).max()
     for e in range(0, ref.ne, 5):
-        h = 1e-3 * comp[e]
+        h = 1e-4 * comp[e]
         cp, cm = comp.copy(), comp.copy()
         cp[e] += h
         cm[e] -= h
@@ -75,12 +78,13 @@ def test_adjoint_matches_central_differences_fp64(lra, tet):
         assert abs(fd - gvf[k]) <= 1e-5 * max(abs(fd), 1e-3 * scale_v), (k, fd, gvf[k])
 
 
-@pytest.mark.parametrize("lra,tet", [(False, False), (True, False), (False, True)], ids=["edges", "edges_lra", "tet_volumes"])
-def test_fp32_forward_matches_update_restatement(lra, tet):
+@pytest.mark.parametrize("lra,tet,bends", [(False, False, False), (True, False, False), (False, True, False), (False, False, True)],
+                         ids=["edges", "edges_lra", "tet_volumes", "edges_bends"])
+def test_fp32_forward_matches_update_restatement(lra, tet, bends):
     """The rollout oracle's FP32 pass equals the parity restatement of the full update
     on the same body (no colliders, identity rotation, no recentring), with LRAs and volumes too."""
     from oracle import Restatement
-    case, s, ref, comp, vel, target = _case(n=8, frames=6, lra=lra, tet=tet)
+    case, s, ref, comp, vel, target = _case(n=8, frames=6, lra=lra, tet=tet, bends=bends)
     x32 = ref.forward(comp, vel, precision=32)
     st = s.copy()
     st.edges["compliance"] = comp.astype(np.float32)
