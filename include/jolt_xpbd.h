@@ -555,12 +555,14 @@ int xpbd_profile_read(xpbd_context *ctx, int32_t *num_launches, double *total_ms
 
 /*
  * Differentiable rollout.  No counterpart in the reference (Jolt computes no
- * gradients); it differentiates exactly the update above for bodies with edge
- * constraints only, no colliders, no pressure, no sleeping and
- * mUpdatePosition false, one collision step per frame.  Each frame runs
- * InitializeUpdateContext (SoftBodyMotionProperties.cpp:937-960), then
- * num_iterations x (IntegratePositions :325-350, the update groups' edge
- * projections :573-601, the velocity update :700-710) and the velocity clamp
+ * gradients); it differentiates exactly the update above for bodies with edge,
+ * volume, dihedral-bend and LRA constraints and optional pressure, no colliders,
+ * no skinning, no sleeping and mUpdatePosition false, one collision step per
+ * frame.  Each frame runs InitializeUpdateContext
+ * (SoftBodyMotionProperties.cpp:937-960), then num_iterations x (ApplyPressure
+ * :291-323, IntegratePositions :325-350, each update group's volume, bend, edge
+ * and LRA projections :367-507, :573-601, :679-698, the velocity update
+ * :700-710) and the velocity clamp
  * of UpdateSoftBodyState (:852-857); the float forward is bit-identical to
  * xpbd_update on such a body with identity rotation.
  *
@@ -584,7 +586,7 @@ int xpbd_profile_read(xpbd_context *ctx, int32_t *num_launches, double *total_ms
 typedef struct xpbd_rollout xpbd_rollout;
 
 typedef struct xpbd_rollout_desc {
-    xpbd_settings *const *settings;     /* one per body, edge constraints only */
+    xpbd_settings *const *settings;     /* one per body: edges, volumes, dihedral bends, LRAs, faces */
     uint32_t num_bodies;
     uint32_t num_iterations;            /* SoftBodyCreationSettings::mNumIterations */
     float linear_damping;               /* mLinearDamping */
@@ -593,6 +595,8 @@ typedef struct xpbd_rollout_desc {
     float dt;                           /* frame time step (one collision step per frame) */
     uint32_t frames;
     uint32_t checkpoint_interval;       /* frames between (x, v) checkpoints, >= 1 */
+    float pressure;                     /* mPressure (n R T) of every body; > 0: ApplyPressure on each
+                                           body's faces every sub-step (SoftBodyMotionProperties.cpp:291-323) */
 } xpbd_rollout_desc;
 
 int xpbd_rollout_create(xpbd_context *ctx, const xpbd_rollout_desc *desc, xpbd_rollout **out_rollout);

@@ -392,6 +392,126 @@ template <class T> __device__ __forceinline__ R4<T> *state_prev(const RolloutDev
 template <> __device__ __forceinline__ R4<float> *state_prev<float>(const RolloutDev &R) { return reinterpret_cast<R4<float> *>(R.prev); }
 template <> __device__ __forceinline__ R4<double> *state_prev<double>(const RolloutDev &R) { return reinterpret_cast<R4<double> *>(R.prev64); }
 
+// ApplyPressure (SoftBodyMotionProperties.cpp:291-323) on the sub-step start positions X (LDS)
+// and the velocities V, with the operation order of the update kernel: GetVolumeTimesSix
+// (:107-118) summed in face order by one lane from per-face terms (terms: num_faces + 2 slots),
+// then each vertex adds its faces' impulses w * c * (x2 - x1) x (x3 - x1) in face order.
+template <class T>
+__device__ __forceinline__ void pressure_apply(const RolloutBodyDev &B, const StepConstants<T> &K, const R4<T> *X, R4<T> *V,
+                                               T *terms, uint32_t tid)
+{
+    const uint32_t nf = B.num_faces, nv = B.nv;
+    for (uint32_t f = tid; f < nf; f += kRolloutBlock)
+    {
+        const uint32_t *t = B.faces + 3u * f;
+        terms[f] = rdot(rcross(xyz3(X[t[0]]), xyz3(X[t[1]])), xyz3(X[t[2]]));
+    }
+    __syncthreads();
+    if (tid == 0)
+    {
+        T six_volume = (T)0;
+        for (uint32_t f = 0; f < nf; ++f)
+            six_volume += terms[f];
+        terms[nf] = six_volume;
+    }
+    __syncthreads();
+    const T six_volume = terms[nf];
+    if (six_volume > (T)0)
+    {
+        const T coefficient = (T)B.pressure * K.dt_sub / six_volume;
+        for (uint32_t i = tid; i < nv; i += kRolloutBlock)
+        {
+            const uint32_t fb = B.vf_begin[i], fe = B.vf_begin[i + 1];
+            if (fb == fe)
+                continue;
+            const T w = X[i].w;
+            R3<T> v = xyz3(V[i]);
+            for (uint32_t k = fb; k < fe; ++k)
+            {
+                const uint32_t *t = B.faces + 3u * B.vf_list[k];
+                const R3<T> x1 = xyz3(X[t[0]]);
+                v = v + w * (coefficient * rcross(xyz3(X[t[1]]) - x1, xyz3(X[t[2]]) - x1));
+            }
+            V[i] = r4(v, (T)0);
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void atomic_add3(float4 *A, uint32_t i, const R3<float> &d)
+{
+    atomicAdd(&A[i].x, d.x);
+    atomicAdd(&A[i].y, d.y);
+    atomicAdd(&A[i].z, d.z);
+}
+
+// Adjoint of pressure_apply at the sub-step start positions X0 (+ inverse mass).  The
+// velocities pass through unchanged (vbar: their adjoints after the pressure step); with
+// c = p h / S, S = sum_f x1 . (x2 x x3), n_f = (x2 - x1) x (x3 - x1) and
+// ibar_f = sum_{i in f} w_i vbar_i: cbar = sum_f ibar_f . n_f, nbar_f = c ibar_f (into
+// x1, x2, x3 through n_f = a x b: abar = b x nbar, bbar = nbar x a), Sbar = -cbar c / S (into
+// x1, x2, x3 through S: x2 x x3, x3 x x1, x1 x x2).  A (LDS) holds the position adjoints; faces
+// sharing a vertex add to it atomically.
+__device__ __forceinline__ void pressure_adjoint(const RolloutBodyDev &B, float h, float4 *A, const float4 *vbar, const float4 *X0,
+                                                 float *terms, uint32_t tid)
+{
+    const uint32_t nf = B.num_faces;
+    auto x_of = [&](uint32_t i) { const float4 q = X0[i]; return r3(q.x, q.y, q.z); };
+    for (uint32_t f = tid; f < nf; f += kRolloutBlock)
+    {
+        const uint32_t *t = B.faces + 3u * f;
+        terms[f] = rdot(rcross(x_of(t[0]), x_of(t[1])), x_of(t[2]));
+    }
+    if (tid == 0)
+        terms[nf + 1] = 0.0f;
+    __syncthreads();
+    if (tid == 0)
+    {
+        float six_volume = 0.0f;
+        for (uint32_t f = 0; f < nf; ++f)
+            six_volume += terms[f];
+        terms[nf] = six_volume;
+    }
+    __syncthreads();
+    const float S = terms[nf];
+    if (S > 0.0f)
+    {
+        const float c = B.pressure * h / S;
+        float cb = 0.0f;
+        for (uint32_t f = tid; f < nf; f += kRolloutBlock)
+        {
+            const uint32_t *t = B.faces + 3u * f;
+            const R3<float> x1 = x_of(t[0]), a = x_of(t[1]) - x1, b = x_of(t[2]) - x1;
+            R3<float> ib = r3(0.0f, 0.0f, 0.0f);
+            for (int j = 0; j < 3; ++j)
+            {
+                const float4 vb = vbar[t[j]];
+                ib = ib + X0[t[j]].w * r3(vb.x, vb.y, vb.z);
+            }
+            cb += rdot(ib, rcross(a, b));
+            const R3<float> nb = c * ib, ab = rcross(b, nb), bb = rcross(nb, a);
+            atomic_add3(A, t[0], r3(0.0f, 0.0f, 0.0f) - (ab + bb));
+            atomic_add3(A, t[1], ab);
+            atomic_add3(A, t[2], bb);
+        }
+        atomicAdd(&terms[nf + 1], cb);
+        __syncthreads();
+        const float Sb = -terms[nf + 1] * c / S;
+        for (uint32_t f = tid; f < nf; f += kRolloutBlock)
+        {
+            const uint32_t *t = B.faces + 3u * f;
+            const R3<float> x1 = x_of(t[0]), x2 = x_of(t[1]), x3 = x_of(t[2]);
+            atomic_add3(A, t[0], Sb * rcross(x2, x3));
+            atomic_add3(A, t[1], Sb * rcross(x3, x1));
+            atomic_add3(A, t[2], Sb * rcross(x1, x2));
+        }
+    }
+    __syncthreads();
+}
+
+// Pressure scratch after the LDS positions (and edge adjoints): num_faces + 2 slots of T
+template <class T> __device__ __forceinline__ T *pressure_terms(unsigned char *lds, uint32_t offset) { return reinterpret_cast<T *>(lds + offset); }
+
 // One sub-step of the forward pass on the LDS positions X; V / P are the
 // body's velocity / previous-position arrays.  The compliance of record r is
 // comp(r).
@@ -456,6 +576,7 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_forward_kernel(RolloutD
     const uint32_t tid = threadIdx.x;
     const RolloutBodyDev B = R.bodies[blockIdx.x];
     const uint32_t nv = B.nv;
+    T *terms = pressure_terms<T>(lds_raw, nv * (uint32_t)sizeof(R4<T>));
     R4<T> *V = state_vel<T>(R) + B.voff;
     R4<T> *P = state_prev<T>(R) + B.voff;
     constexpr bool kFloat = sizeof(T) == 4;
@@ -499,7 +620,11 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_forward_kernel(RolloutD
             }
         }
         for (uint32_t it = 0; it < R.iterations; ++it)
+        {
+            if (B.num_faces > 0)
+                pressure_apply<T>(B, K, X, V, terms, tid);
             forward_substep<T>(B, K, X, V, P, it + 1 == R.iterations, tid, comp);
+        }
     }
     for (uint32_t i = tid; i < nv; i += kRolloutBlock)
     {
@@ -521,6 +646,8 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_backward_kernel(Rollout
     float4 *A = reinterpret_cast<float4 *>(lds_raw);                 // positions, then their adjoints
     R4<float> *X = reinterpret_cast<R4<float> *>(lds_raw);
     float *alpha_bar = reinterpret_cast<float *>(lds_raw + 16u * nv);
+    float *terms = pressure_terms<float>(lds_raw, 16u * nv + ((4u * ne + 15u) & ~15u));
+    const bool has_pressure = B.num_faces > 0;
     R4<float> *V = state_vel<float>(R) + B.voff;
     R4<float> *P = state_prev<float>(R) + B.voff;
     float4 *adj_x = R.adj_x + B.voff, *adj_v = R.adj_v + B.voff;
@@ -558,6 +685,9 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_backward_kernel(Rollout
         for (uint32_t s = 0; s < nsub; ++s)
         {
             const size_t sbase = (size_t)s * R.total_nv + B.voff;
+            // the tape keeps the velocity after the pressure step
+            if (has_pressure)
+                pressure_apply<float>(B, K, X, V, terms, tid);
             for (uint32_t i = tid; i < nv; i += kRolloutBlock)
             {
                 R.seg_x[sbase + i] = A[i];
@@ -670,10 +800,23 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_backward_kernel(Rollout
                 else
                     v1b = v1b + vb;
                 const R3<float> v0b = w > 0.0f ? v1b * K.damping : v1b;
-                adj_x[i] = make_float4(x0b.x, x0b.y, x0b.z, 0.0f);
+                if (has_pressure)
+                    A[i] = make_float4(x0b.x, x0b.y, x0b.z, w);
+                else
+                    adj_x[i] = make_float4(x0b.x, x0b.y, x0b.z, 0.0f);
                 adj_v[i] = make_float4(v0b.x, v0b.y, v0b.z, 0.0f);
             }
             __syncthreads();
+            if (has_pressure)
+            {
+                pressure_adjoint(B, h, A, adj_v, R.seg_x + sbase, terms, tid);
+                for (uint32_t i = tid; i < nv; i += kRolloutBlock)
+                {
+                    const float4 a = A[i];
+                    adj_x[i] = make_float4(a.x, a.y, a.z, 0.0f);
+                }
+                __syncthreads();
+            }
         }
     }
     for (uint32_t i = tid; i < nv; i += kRolloutBlock)
@@ -686,8 +829,15 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_backward_kernel(Rollout
         grad_compliance[B.eoff + e] = alpha_bar[e];
 }
 
-uint32_t rollout_forward_lds_bytes(uint32_t nv, bool f64) { return nv * (f64 ? 32u : 16u); }
-uint32_t rollout_backward_lds_bytes(uint32_t nv, uint32_t ne) { return 16u * nv + ((4u * ne + 15u) & ~15u); }
+// nf: faces under pressure (0: none), which add num_faces + 2 scratch slots
+uint32_t rollout_forward_lds_bytes(uint32_t nv, uint32_t nf, bool f64)
+{
+    return nv * (f64 ? 32u : 16u) + (nf ? (nf + 2u) * (f64 ? 8u : 4u) : 0u);
+}
+uint32_t rollout_backward_lds_bytes(uint32_t nv, uint32_t ne, uint32_t nf)
+{
+    return 16u * nv + ((4u * ne + 15u) & ~15u) + (nf ? 4u * (nf + 2u) : 0u);
+}
 
 template <class K>
 static hipError_t allow_lds(K kernel)

@@ -37,8 +37,8 @@ hipError_t launch_team(bool ext, const BodyDev *bodies, BodyStateDev *states, co
 hipError_t launch_skin(const float *joints, const SkinInvBindDev *inv_binds, uint32_t num_inv_binds, float *skin,
                        const SkinnedDev *recs, uint32_t num_recs, const uint32_t *normal_faces, const uint32_t *faces,
                        float4 *states, uint32_t nv, float4 *x, float4 *v, int32_t hard_skin_all, hipStream_t stream);
-uint32_t rollout_forward_lds_bytes(uint32_t nv, bool f64);
-uint32_t rollout_backward_lds_bytes(uint32_t nv, uint32_t ne);
+uint32_t rollout_forward_lds_bytes(uint32_t nv, uint32_t nf, bool f64);
+uint32_t rollout_backward_lds_bytes(uint32_t nv, uint32_t ne, uint32_t nf);
 hipError_t launch_rollout_forward_f32(const RolloutDev &R, const float *compliance, const float *velocity, float *positions,
                                       uint32_t lds, hipStream_t stream);
 hipError_t launch_rollout_forward_f64(const RolloutDev &R, const double *compliance, const double *velocity, double *positions,
@@ -3135,7 +3135,6 @@ int xpbd_rollout_create(xpbd_context *ctx, const xpbd_rollout_desc *d, xpbd_roll
     std::vector<uint32_t> lvl;
     std::vector<float4> x0;
     std::vector<RolloutBodyDev> bodies(d->num_bodies);
-    uint32_t max_nv = 0;
     for (uint32_t b = 0; b < d->num_bodies; ++b)
     {
         xpbd_settings *s = d->settings[b];
@@ -3144,8 +3143,9 @@ int xpbd_rollout_create(xpbd_context *ctx, const xpbd_rollout_desc *d, xpbd_roll
         if (s->host_level_begin.empty())
             return Fail(XPBD_E_CAPACITY, "rollout body " + std::to_string(b) + " does not fit in LDS");
         const uint32_t nv = s->num_vertices, ne = s->num_edges;
-        if (rollout_backward_lds_bytes(nv, ne) > kMaxLdsBytes)
-            return Fail(XPBD_E_CAPACITY, "rollout body " + std::to_string(b) + ": positions and edge adjoints exceed LDS");
+        const uint32_t nf = d->pressure > 0.0f ? s->num_faces : 0u;
+        if (rollout_backward_lds_bytes(nv, ne, nf) > kMaxLdsBytes)
+            return Fail(XPBD_E_CAPACITY, "rollout body " + std::to_string(b) + ": positions, edge adjoints and face terms exceed LDS");
         ++s->refcount;
         r->settings.push_back(s);
         RolloutBodyDev &B = bodies[b];
@@ -3157,6 +3157,11 @@ int xpbd_rollout_create(xpbd_context *ctx, const xpbd_rollout_desc *d, xpbd_roll
         B.nlev = (uint32_t)s->host_level_begin.size() - 1;
         B.num_records = (uint32_t)s->host_recs.size();
         B.lvl = reinterpret_cast<const uint32_t *>((uintptr_t)lvl.size());     // index until uploaded
+        B.num_faces = nf;
+        B.pressure = d->pressure;
+        B.faces = s->faces.p;           // the settings' device arrays (the rollout holds a reference)
+        B.vf_begin = s->vf_begin.p;
+        B.vf_list = s->vf_list.p;
         for (uint32_t L : s->host_level_begin)
             lvl.push_back(L);                       // relative to the body's first record
         for (size_t k = 0; k < s->host_recs.size(); ++k)
@@ -3176,11 +3181,10 @@ int xpbd_rollout_create(xpbd_context *ctx, const xpbd_rollout_desc *d, xpbd_roll
             x0.push_back(make_float4(v.pos[0], v.pos[1], v.pos[2], v.inv_mass));
         r->total_nv += nv;
         r->total_ne += ne;
-        max_nv = std::max(max_nv, nv);
-        r->lds_bwd = std::max(r->lds_bwd, rollout_backward_lds_bytes(nv, ne));
+        r->lds_bwd = std::max(r->lds_bwd, rollout_backward_lds_bytes(nv, ne, nf));
+        r->lds_f32 = std::max(r->lds_f32, rollout_forward_lds_bytes(nv, nf, false));
+        r->lds_f64 = std::max(r->lds_f64, rollout_forward_lds_bytes(nv, nf, true));
     }
-    r->lds_f32 = rollout_forward_lds_bytes(max_nv, false);
-    r->lds_f64 = rollout_forward_lds_bytes(max_nv, true);
     HIP_TRY(r->recs.upload(recs));
     HIP_TRY(r->lvl.upload(lvl));
     HIP_TRY(r->x0.upload(x0));

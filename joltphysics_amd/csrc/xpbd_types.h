@@ -270,6 +270,10 @@ struct RolloutBodyDev
     uint32_t voff, eoff;        // first vertex / edge in the concatenated arrays
     uint32_t rbase;             // first record slot in the record tape
     const float4 *x0;           // initial positions + inverse mass
+    // pressure (num_faces 0: none): faces (3 vertices each) and vertex -> face lists in face order
+    const uint32_t *faces, *vf_begin, *vf_list;
+    uint32_t num_faces;
+    float pressure;
 };
 
 struct RolloutDev

@@ -527,6 +527,7 @@ class RolloutCase:
     velocity: np.ndarray            # (num_bodies * V, 3) float32
     target_scale: np.ndarray        # (num_bodies * E,) float32: compliance multiplier of the target run
     checkpoint_interval: int = 10
+    pressure: float = 0.0           # mPressure of every body (ApplyPressure on its faces)
 
 
 def config4_rollout(num_bodies=64, n=64, frames=60, seed=4, compliance=1e-3, lra=False, bends=False):
@@ -570,6 +571,25 @@ def rollout_tet_block(num_bodies=2, cells=3, frames=8, seed=6, compliance=1e-4, 
     scale = rng.uniform(0.5, 2.0, size=num_bodies * len(e)).astype(np.float32)
     return RolloutCase(f"tet{cells}x{num_bodies}_rollout", mesh, num_bodies, frames, 5, 1.0 / 60.0, (0.0, 0.0, -9.81),
                        comp, vel, scale)
+
+
+def rollout_balloon(num_bodies=2, subdivisions=2, frames=8, seed=8, compliance=1e-4, pressure=1000.0):
+    """Differentiation workload with pressure: `num_bodies` copies of an icosphere of
+    radius 0.5 m (`subdivisions` subdivisions, inverse mass 1, its highest vertex pinned),
+    edges of compliance `compliance` (the parameters), no bends, pressure `pressure` on its
+    faces, gravity -9.81 z, dt 1/60, 5 iterations; initial velocities U(-0.1, 0.1) m/s and
+    target compliance multipliers U(0.5, 2.0) per edge, from `seed`."""
+    v, f = icosphere(subdivisions, radius=0.5, inv_mass=1.0)
+    v["inv_mass"][int(np.argmax(v["pos"][:, 2]))] = 0.0
+    mesh = MeshSpec(vertices=v, faces=f, attrs=attrs(compliance, compliance, 0.0), bend_type=BEND_NONE)
+    from .xpbd import build_settings
+    num_edges = len(build_settings(mesh).edges)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    comp = np.full(num_bodies * num_edges, compliance, np.float32)
+    vel = rng.uniform(-0.1, 0.1, size=(num_bodies * len(v), 3)).astype(np.float32)
+    scale = rng.uniform(0.5, 2.0, size=num_bodies * num_edges).astype(np.float32)
+    return RolloutCase(f"balloon{subdivisions}x{num_bodies}_rollout", mesh, num_bodies, frames, 5, 1.0 / 60.0,
+                       (0.0, 0.0, -9.81), comp, vel, scale, pressure=float(pressure))
 
 
 def mesh_for_key(key):

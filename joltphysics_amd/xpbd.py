@@ -104,7 +104,7 @@ def _tree_struct(t, keep):
 class _RolloutDesc(C.Structure):
     _fields_ = [("settings", C.c_void_p), ("num_bodies", C.c_uint32), ("num_iterations", C.c_uint32),
                 ("linear_damping", C.c_float), ("max_linear_velocity", C.c_float), ("gravity_factor", C.c_float),
-                ("dt", C.c_float), ("frames", C.c_uint32), ("checkpoint_interval", C.c_uint32)]
+                ("dt", C.c_float), ("frames", C.c_uint32), ("checkpoint_interval", C.c_uint32), ("pressure", C.c_float)]
 
 
 class _VertexAttributes(C.Structure):
@@ -592,10 +592,10 @@ class Context:
                     bounds_max=out[9:12], active=bool(out[12]), sleep_timer=float(out[13]))
 
     def create_rollout(self, settings, num_iterations=5, dt=1.0 / 60.0, frames=60, checkpoint_interval=10,
-                       linear_damping=0.1, max_linear_velocity=500.0, gravity_factor=1.0) -> "Rollout":
+                       linear_damping=0.1, max_linear_velocity=500.0, gravity_factor=1.0, pressure=0.0) -> "Rollout":
         """Differentiable rollout of bodies with edge constraints only (see Rollout)."""
         return Rollout(self, settings, num_iterations, dt, frames, checkpoint_interval, linear_damping,
-                       max_linear_velocity, gravity_factor)
+                       max_linear_velocity, gravity_factor, pressure)
 
     def profile_enable(self, on=True):
         _check(self.lib.xpbd_profile_enable(self.h, int(on)))
@@ -620,14 +620,14 @@ class Rollout:
     """
 
     def __init__(self, ctx, settings, num_iterations, dt, frames, checkpoint_interval, linear_damping,
-                 max_linear_velocity, gravity_factor):
+                 max_linear_velocity, gravity_factor, pressure=0.0):
         self.ctx = ctx
         self.lib = ctx.lib
         self.settings = list(settings)
         handles = (C.c_void_p * len(self.settings))(*[s.handle.value for s in self.settings])
         d = _RolloutDesc(C.cast(handles, C.c_void_p), len(self.settings), int(num_iterations), float(linear_damping),
                          float(max_linear_velocity), float(gravity_factor), float(dt), int(frames),
-                         int(checkpoint_interval))
+                         int(checkpoint_interval), float(pressure))
         h = C.c_void_p()
         _check(self.lib.xpbd_rollout_create(ctx.h, C.byref(d), C.byref(h)))
         self.h = h

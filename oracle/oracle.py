@@ -723,13 +723,14 @@ class Restatement:
 
 
 class RolloutReference:
-    """CPU restatement of the differentiable rollout of edge, bend, volume and LRA bodies (oracle/xpbd_adjoint.cpp):
-    FP32 or FP64 forward pass and the FP64 adjoint from a full tape."""
+    """CPU restatement of the differentiable rollout of edge, bend, volume and LRA bodies, with pressure on their
+    faces when `pressure` > 0 (oracle/xpbd_adjoint.cpp): FP32 or FP64 forward pass and the FP64 adjoint from a
+    full tape."""
 
     _lib = None
 
     def __init__(self, settings: SettingsArrays, iterations=5, frames=60, dt=1.0 / 60.0, linear_damping=0.1,
-                 max_linear_velocity=500.0, gravity=(0.0, 0.0, -9.81), gravity_factor=1.0):
+                 max_linear_velocity=500.0, gravity=(0.0, 0.0, -9.81), gravity_factor=1.0, pressure=0.0):
         if RolloutReference._lib is None:
             lib = C.CDLL(restatement_path())
             common = [_u32, _vp, _vp, _u32, _vp, _vp, _u32, _u32, _f, _f, _f, _vp, _f]
@@ -738,7 +739,7 @@ class RolloutReference:
             lib.xa_rollout_gradient.restype = _i
             lib.xa_rollout_gradient.argtypes = [_i] + common + [_vp] * 6
             common_lra = [_u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _u32, _vp, _vp, _vp, _u32, _vp, _vp, _vp, _u32, _vp,
-                          _u32, _u32, _f, _f, _f, _vp, _f]
+                          _u32, _vp, _f, _u32, _u32, _f, _f, _f, _vp, _f]
             lib.xa_rollout_forward_lra.restype = _i
             lib.xa_rollout_forward_lra.argtypes = [_i] + common_lra + [_vp, _vp, _vp]
             lib.xa_rollout_gradient_lra.restype = _i
@@ -760,6 +761,9 @@ class RolloutReference:
         self.groups = np.ascontiguousarray(np.stack([g["volume_end"], g["bend_end"], g["edge_end"], g["lra_end"]], axis=1),
                                            np.uint32).reshape(-1)
         self.ng = len(g)
+        self.nf = len(settings.faces) if pressure > 0.0 else 0
+        self.fv = np.ascontiguousarray(settings.faces["v"], np.uint32).reshape(-1) if self.nf else np.zeros(3, np.uint32)
+        self.pressure = float(pressure)
         v = settings.vertices
         self.nv, self.ne = len(v), len(settings.edges)
         self.pos = np.ascontiguousarray(v["pos"], np.float32).reshape(-1)
@@ -773,7 +777,7 @@ class RolloutReference:
     def _common(self):
         return [self.nv, _ptr(self.pos), _ptr(self.inv_mass), self.ne, _ptr(self.ev), _ptr(self.rest), self.nl, _ptr(self.lv),
                 _ptr(self.maxd), self.nt, _ptr(self.tv), _ptr(self.six_rest), _ptr(self.vol_comp), self.nb, _ptr(self.bv),
-                _ptr(self.bend_comp), _ptr(self.angle), self.ng, _ptr(self.groups),
+                _ptr(self.bend_comp), _ptr(self.angle), self.ng, _ptr(self.groups), self.nf, _ptr(self.fv), self.pressure,
                 *self.args, _ptr(self.gravity), self.gravity_factor]
 
     def forward(self, compliance, velocity, precision=64):

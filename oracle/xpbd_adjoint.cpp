@@ -122,6 +122,9 @@ struct Spec
     const float *six_rest, *vol_compliance;
     const uint32_t *bv;     // 4 per dihedral bend
     const float *bend_compliance, *initial_angle;
+    uint32_t nf = 0;        // faces (3 vertices each) of a body under pressure
+    const uint32_t *fv = nullptr;
+    float pressure = 0.0f;
     // processing order: (0, edge index), (1, LRA index), (2, volume index) or (3, bend index)
     // per projection
     std::vector<std::pair<int, uint32_t>> seq;
@@ -157,9 +160,73 @@ struct Consts
     }
 };
 
+// ApplyPressure (SoftBodyMotionProperties.cpp:291-323) with GetVolumeTimesSix (:107-118): the
+// volume sums face by face in face order, and each face's impulse adds to its vertices'
+// velocities in face order.
+template <class F>
+void Pressure(const Spec &s, const Consts<F> &K, const std::vector<V<F>> &x, const std::vector<F> &w, std::vector<V<F>> &v)
+{
+    if (!(s.pressure > 0.0f) || s.nf == 0)
+        return;
+    F six_volume = (F)0;
+    for (uint32_t f = 0; f < s.nf; ++f)
+        six_volume += dot(cross(x[s.fv[3 * f]], x[s.fv[3 * f + 1]]), x[s.fv[3 * f + 2]]);
+    if (!(six_volume > (F)0))
+        return;
+    const F coefficient = (F)s.pressure * K.dt_sub / six_volume;
+    for (uint32_t f = 0; f < s.nf; ++f)
+    {
+        const uint32_t *t = s.fv + 3 * f;
+        const V<F> impulse = coefficient * cross(x[t[1]] - x[t[0]], x[t[2]] - x[t[0]]);
+        for (int j = 0; j < 3; ++j)
+            v[t[j]] = v[t[j]] + w[t[j]] * impulse;
+    }
+}
+
+// Adjoint of Pressure at positions x0: the velocities pass through unchanged; with
+// c = p h / S, S = sum_f x1 . (x2 x x3), n_f = (x2 - x1) x (x3 - x1) and
+// ibar_f = sum_{i in f} w_i vbar_i: cbar = sum_f ibar_f . n_f, nbar_f = c ibar_f,
+// Sbar = -cbar c / S, and xbar gains nbar_f through n_f and Sbar through S.
+void PressureAdjoint(const Spec &s, const Consts<double> &K, const double *x0, const std::vector<V<double>> &vb,
+                     std::vector<V<double>> &xb)
+{
+    if (!(s.pressure > 0.0f) || s.nf == 0)
+        return;
+    auto X = [&](uint32_t i) { return mk(x0[3 * i], x0[3 * i + 1], x0[3 * i + 2]); };
+    double S = 0.0;
+    for (uint32_t f = 0; f < s.nf; ++f)
+        S += dot(cross(X(s.fv[3 * f]), X(s.fv[3 * f + 1])), X(s.fv[3 * f + 2]));
+    if (!(S > 0.0))
+        return;
+    const double c = (double)s.pressure * K.dt_sub / S;
+    double cbar = 0.0;
+    for (uint32_t f = 0; f < s.nf; ++f)
+    {
+        const uint32_t *t = s.fv + 3 * f;
+        const V<double> a = X(t[1]) - X(t[0]), b = X(t[2]) - X(t[0]);
+        V<double> ib = mk(0.0, 0.0, 0.0);
+        for (int j = 0; j < 3; ++j)
+            ib = ib + (double)s.inv_mass[t[j]] * vb[t[j]];
+        cbar += dot(ib, cross(a, b));
+        const V<double> nb = c * ib, ab = cross(b, nb), bb = cross(nb, a);
+        xb[t[0]] = xb[t[0]] - (ab + bb);
+        xb[t[1]] = xb[t[1]] + ab;
+        xb[t[2]] = xb[t[2]] + bb;
+    }
+    const double Sb = -cbar * c / S;
+    for (uint32_t f = 0; f < s.nf; ++f)
+    {
+        const uint32_t *t = s.fv + 3 * f;
+        const V<double> x1 = X(t[0]), x2 = X(t[1]), x3 = X(t[2]);
+        xb[t[0]] = xb[t[0]] + Sb * cross(x2, x3);
+        xb[t[1]] = xb[t[1]] + Sb * cross(x3, x1);
+        xb[t[2]] = xb[t[2]] + Sb * cross(x1, x2);
+    }
+}
+
 struct Tape
 {
-    std::vector<double> x0, v0, pre, y;     // per sub-step: 3 nv, 3 nv, 12 per projection, 3 nv
+    std::vector<double> x0, v0, pre, y;     // per sub-step: 3 nv, 3 nv (after pressure), 12 per projection, 3 nv
     std::vector<uint8_t> acted;             // per sub-step and projection: an LRA moved its vertex (decided
                                             // in the forward pass's precision)
 };
@@ -181,6 +248,7 @@ void Forward(const Spec &s, const F *compliance, const F *velocity, F *out, Tape
     for (uint32_t frame = 0; frame < s.frames; ++frame)
         for (uint32_t it = 0; it < s.iterations; ++it)
         {
+            Pressure<F>(s, K, x, w, v);
             if (tape)
                 for (uint32_t i = 0; i < nv; ++i)
                 {
@@ -437,6 +505,7 @@ void Adjoint(const Spec &s, const double *compliance, const double *velocity, co
             xb[i] = x0b;
             vb[i] = w > 0.0 ? v1b * K.damping : v1b;
         }
+        PressureAdjoint(s, K, x0, vb, xb);
     }
     for (uint32_t i = 0; i < nv; ++i)
     {
@@ -452,9 +521,10 @@ Spec MakeSpec(uint32_t nv, const float *pos, const float *inv_mass, uint32_t ne,
               const float *max_distance = nullptr, uint32_t nt = 0, const uint32_t *tv = nullptr, const float *six_rest = nullptr,
               const float *vol_compliance = nullptr, uint32_t nb = 0, const uint32_t *bv = nullptr,
               const float *bend_compliance = nullptr, const float *initial_angle = nullptr, uint32_t ng = 0,
-              const uint32_t *groups = nullptr)
+              const uint32_t *groups = nullptr, uint32_t nf = 0, const uint32_t *fv = nullptr, float pressure = 0.0f)
 {
     Spec s;
+    s.nf = nf; s.fv = fv; s.pressure = pressure;
     s.nv = nv; s.ne = ne; s.pos = pos; s.inv_mass = inv_mass; s.ev = ev; s.rest = rest;
     s.lv = lv; s.max_distance = max_distance;
     s.tv = tv; s.six_rest = six_rest; s.vol_compliance = vol_compliance;
@@ -534,12 +604,12 @@ int xa_rollout_forward_lra(int precision, uint32_t nv, const float *pos, const f
                            const float *rest, uint32_t nl, const uint32_t *lv, const float *max_distance, uint32_t nt,
                            const uint32_t *tv, const float *six_rest, const float *vol_compliance, uint32_t nb,
                            const uint32_t *bv, const float *bend_compliance, const float *initial_angle, uint32_t ng,
-                           const uint32_t *groups, uint32_t iterations, uint32_t frames, float dt, float linear_damping,
+                           const uint32_t *groups, uint32_t nf, const uint32_t *fv, float pressure, uint32_t iterations, uint32_t frames, float dt, float linear_damping,
                            float max_linear_velocity, const float *gravity, float gravity_factor, const double *compliance,
                            const double *velocity, double *positions)
 {
     const Spec s = MakeSpec(nv, pos, inv_mass, ne, ev, rest, iterations, frames, dt, linear_damping, max_linear_velocity,
-                            gravity, gravity_factor, nl, lv, max_distance, nt, tv, six_rest, vol_compliance, nb, bv, bend_compliance, initial_angle, ng, groups);
+                            gravity, gravity_factor, nl, lv, max_distance, nt, tv, six_rest, vol_compliance, nb, bv, bend_compliance, initial_angle, ng, groups, nf, fv, pressure);
     if (precision == 64)
     {
         Forward<double>(s, compliance, velocity, positions, nullptr);
@@ -558,7 +628,7 @@ int xa_rollout_gradient_lra(int forward_precision, uint32_t nv, const float *pos
                             const uint32_t *ev, const float *rest, uint32_t nl, const uint32_t *lv, const float *max_distance,
                             uint32_t nt, const uint32_t *tv, const float *six_rest, const float *vol_compliance,
                             uint32_t nb, const uint32_t *bv, const float *bend_compliance, const float *initial_angle,
-                            uint32_t ng, const uint32_t *groups, uint32_t iterations, uint32_t frames, float dt,
+                            uint32_t ng, const uint32_t *groups, uint32_t nf, const uint32_t *fv, float pressure, uint32_t iterations, uint32_t frames, float dt,
                             float linear_damping, float max_linear_velocity, const float *gravity, float gravity_factor,
                             const double *compliance, const double *velocity, const double *grad_positions,
                             double *grad_compliance, double *grad_velocity, double *positions)
@@ -566,7 +636,7 @@ int xa_rollout_gradient_lra(int forward_precision, uint32_t nv, const float *pos
     if (forward_precision != 32 && forward_precision != 64)
         return -1;
     const Spec s = MakeSpec(nv, pos, inv_mass, ne, ev, rest, iterations, frames, dt, linear_damping, max_linear_velocity,
-                            gravity, gravity_factor, nl, lv, max_distance, nt, tv, six_rest, vol_compliance, nb, bv, bend_compliance, initial_angle, ng, groups);
+                            gravity, gravity_factor, nl, lv, max_distance, nt, tv, six_rest, vol_compliance, nb, bv, bend_compliance, initial_angle, ng, groups, nf, fv, pressure);
     Adjoint(s, compliance, velocity, grad_positions, grad_compliance, grad_velocity, positions, forward_precision);
     return 0;
 }

@@ -30,6 +30,8 @@ torch = pytest.importorskip("torch")
 def _make(frames, interval, compliance, kind="edges"):
     if kind == "tet_volumes":
         case = scenes.rollout_tet_block(num_bodies=2, cells=3, frames=frames, compliance=compliance)
+    elif kind == "balloon_pressure":
+        case = scenes.rollout_balloon(num_bodies=2, subdivisions=2, frames=frames, compliance=compliance)
     else:
         case = scenes.config4_rollout(num_bodies=2, n=16, frames=frames, compliance=compliance, lra=kind == "edges_lra",
                                       bends=kind == "edges_bends")
@@ -38,18 +40,19 @@ def _make(frames, interval, compliance, kind="edges"):
     ctx.set_gravity(case.gravity)
     st = ctx.create_settings(s)
     r = ctx.create_rollout([st] * case.num_bodies, num_iterations=case.iterations, dt=case.dt, frames=case.frames,
-                           checkpoint_interval=interval)
+                           checkpoint_interval=interval, pressure=case.pressure)
     return case, s, ctx, r
 
 
-KINDS = ["edges", "edges_lra", "tet_volumes", "edges_bends"]
+KINDS = ["edges", "edges_lra", "tet_volumes", "edges_bends", "balloon_pressure"]
 
 
 @pytest.fixture(params=KINDS)
 def setup(request):
     """2 cloths 16 x 16, 22 frames, checkpoints every 5 frames (the last segment is partial);
     edge-only, with LRA constraints from the pinned corners, and with dihedral bends (a
-    jittered sheet); and 2 hanging 3^3-cell tet blocks with volume constraints."""
+    jittered sheet); 2 hanging 3^3-cell tet blocks with volume constraints; and 2 pinned
+    icosphere balloons (162 vertices, 320 faces) under pressure."""
     case, s, ctx, r = _make(22, 5, 1e-4, kind=request.param)
     try:
         yield case, s, ctx, r
@@ -108,7 +111,8 @@ def test_float_forward_bit_identical_to_update(setup):
         ctx2 = Context(0)
         ctx2.set_gravity(case.gravity)
         body = ctx2.create_body(ctx2.create_settings(sb), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0),
-                                BodyParams(num_iterations=case.iterations, allow_sleeping=False, update_position=False))
+                                BodyParams(num_iterations=case.iterations, allow_sleeping=False, update_position=False,
+                                           pressure=case.pressure))
         try:
             ctx2.update(case.dt, 1, case.frames)
             ref = ctx2.read(body)["pos"]
@@ -122,7 +126,8 @@ def test_double_forward_matches_fp64_oracle(setup):
     print("rollout double forward vs FP64 oracle", flush=True)
     case, s, ctx, r = setup
     x = _forward64(r, case.compliance.astype(np.float64), case.velocity.astype(np.float64))
-    ref = RolloutReference(s, iterations=case.iterations, frames=case.frames, dt=case.dt, gravity=case.gravity)
+    ref = RolloutReference(s, iterations=case.iterations, frames=case.frames, dt=case.dt, gravity=case.gravity,
+                           pressure=case.pressure)
     nv, ne = len(s.vertices), len(s.edges)
     for b in range(case.num_bodies):
         xr = ref.forward(case.compliance[b * ne:(b + 1) * ne].astype(np.float64),
@@ -156,7 +161,8 @@ def test_adjoint_matches_fp64_oracle_adjoint(setup):
     target64 = torch.from_numpy(_forward64(r, (case.compliance * case.target_scale).astype(np.float64),
                                            case.velocity.astype(np.float64))).to("cuda")
     gx, gc, gv = _grads(case, r, target64)
-    ref = RolloutReference(s, iterations=case.iterations, frames=case.frames, dt=case.dt, gravity=case.gravity)
+    ref = RolloutReference(s, iterations=case.iterations, frames=case.frames, dt=case.dt, gravity=case.gravity,
+                           pressure=case.pressure)
     nv, ne = len(s.vertices), len(s.edges)
     for b in range(case.num_bodies):
         c = case.compliance[b * ne:(b + 1) * ne].astype(np.float64)

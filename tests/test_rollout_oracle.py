@@ -20,11 +20,19 @@ from oracle import RolloutReference, have_restatement
 pytestmark = pytest.mark.skipif(not have_restatement(), reason="oracle restatement not built (make -C oracle restatement)")
 
 
-def _case(n=6, frames=8, lra=False, tet=False, bends=False):
-    case = scenes.rollout_tet_block(num_bodies=1, cells=2, frames=frames) if tet else \
-        scenes.config4_rollout(num_bodies=1, n=n, frames=frames, lra=lra, bends=bends)
+KINDS = ["edges", "edges_lra", "tet_volumes", "edges_bends", "balloon_pressure"]
+
+
+def _case(n=6, frames=8, kind="edges"):
+    if kind == "tet_volumes":
+        case = scenes.rollout_tet_block(num_bodies=1, cells=2, frames=frames)
+    elif kind == "balloon_pressure":
+        case = scenes.rollout_balloon(num_bodies=1, subdivisions=1, frames=frames)
+    else:
+        case = scenes.config4_rollout(num_bodies=1, n=n, frames=frames, lra=kind == "edges_lra", bends=kind == "edges_bends")
     s = build_settings(case.mesh)
-    ref = RolloutReference(s, iterations=case.iterations, frames=frames, dt=case.dt, gravity=case.gravity)
+    ref = RolloutReference(s, iterations=case.iterations, frames=frames, dt=case.dt, gravity=case.gravity,
+                           pressure=case.pressure)
     comp = case.compliance.astype(np.float64)
     vel = case.velocity.astype(np.float64)
     target = ref.forward(comp * case.target_scale, vel)
@@ -36,17 +44,23 @@ def _loss(ref, target, c, v):
     return float(np.mean(np.sum((x - target) ** 2, axis=1)))
 
 
-@pytest.mark.parametrize("lra,tet,bends", [(False, False, False), (True, False, False), (False, True, False), (False, False, True)],
-                         ids=["edges", "edges_lra", "tet_volumes", "edges_bends"])
-def test_adjoint_matches_central_differences_fp64(lra, tet, bends):
+@pytest.mark.parametrize("kind", KINDS)
+def test_adjoint_matches_central_differences_fp64(kind):
     """The FP64 adjoint against central differences of the FP64 forward pass: edge-only, with
-    LRA constraints (some of which act: checked), and a hanging tet block with volumes."""
-    case, s, ref, comp, vel, target = _case(lra=lra, tet=tet, bends=bends)
-    if tet:
+    LRA constraints (some of which act: checked), a hanging tet block with volumes, a cloth
+    with dihedral bends, and a pinned balloon under pressure (whose volume grows: checked)."""
+    case, s, ref, comp, vel, target = _case(kind=kind)
+    if kind == "tet_volumes":
         assert len(s.volumes) > 0
-    if bends:
+    if kind == "edges_bends":
         assert len(s.bends) > 0
-    if lra:
+    if kind == "balloon_pressure":
+        x = ref.forward(comp, vel)
+        f = s.faces["v"].astype(np.int64)
+        six = lambda p: float(np.einsum("ij,ij->i", np.cross(p[f[:, 0]], p[f[:, 1]]), p[f[:, 2]]).sum())
+        p0 = s.vertices["pos"].astype(np.float64)
+        assert six(x - x[f].reshape(-1, 3).mean(0)) > six(p0 - p0.mean(0)) * 1.0001, "pressure does not inflate"
+    if kind == "edges_lra":
         assert len(s.lras) > 0
         stretched = ref.forward(comp, vel)
         lv = s.lras["v"].astype(np.int64)
@@ -78,20 +92,20 @@ def test_adjoint_matches_central_differences_fp64(lra, tet, bends):
         assert abs(fd - gvf[k]) <= 1e-5 * max(abs(fd), 1e-3 * scale_v), (k, fd, gvf[k])
 
 
-@pytest.mark.parametrize("lra,tet,bends", [(False, False, False), (True, False, False), (False, True, False), (False, False, True)],
-                         ids=["edges", "edges_lra", "tet_volumes", "edges_bends"])
-def test_fp32_forward_matches_update_restatement(lra, tet, bends):
+@pytest.mark.parametrize("kind", KINDS)
+def test_fp32_forward_matches_update_restatement(kind):
     """The rollout oracle's FP32 pass equals the parity restatement of the full update
     on the same body (no colliders, identity rotation, no recentring), with LRAs and volumes too."""
     from oracle import Restatement
-    case, s, ref, comp, vel, target = _case(n=8, frames=6, lra=lra, tet=tet, bends=bends)
+    case, s, ref, comp, vel, target = _case(n=8, frames=6, kind=kind)
     x32 = ref.forward(comp, vel, precision=32)
     st = s.copy()
     st.edges["compliance"] = comp.astype(np.float32)
     st.vertices["vel"] = vel.astype(np.float32)
     r = Restatement(case.gravity, 32)
     r.add_soft_body(st, (0.0, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0),
-                    BodyParams(num_iterations=case.iterations, allow_sleeping=False, update_position=False))
+                    BodyParams(num_iterations=case.iterations, allow_sleeping=False, update_position=False,
+                               pressure=case.pressure))
     r.update(case.dt, 1, 6)
     out = r.read(0)
     r.close()
