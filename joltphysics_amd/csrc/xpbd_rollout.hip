@@ -530,6 +530,12 @@ __device__ __forceinline__ void forward_substep(const RolloutBodyDev &B, const S
         V[i] = r4(v, (T)0);
     }
     __syncthreads();
+    if (B.edges_only)
+        run_program_global<kRolloutBlock, false>(B.recs, B.lvl, B.nlev, tid, [&](const uint4 &q, uint32_t) {
+            if (q.x != kNopRecord)
+                edge_project<T>(X, q.x & 0x3fffu, q.x >> 16, (T)__uint_as_float(q.z), comp(q), K.inv_dt_sq);
+        });
+    else
     run_program_global<kRolloutBlock, false>(B.recs, B.lvl, B.nlev, tid, [&](const uint4 &q, uint32_t) {
         if (q.x == kNopRecord)
             return;
@@ -708,6 +714,16 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_backward_kernel(Rollout
                 X[i] = r4(r3(p.x, p.y, p.z) + v1 * h, p.w);
             }
             __syncthreads();
+            if (B.edges_only)
+                run_program_global<kRolloutBlock, false>(B.recs, B.lvl, B.nlev, tid, [&](const uint4 &q, uint32_t r) {
+                    if (q.x == kNopRecord)
+                        return;
+                    const uint32_t i0 = q.x & 0x3fffu, i1 = q.x >> 16;
+                    tape[4u * r] = A[i0];
+                    tape[4u * r + 1u] = A[i1];
+                    edge_project<float>(X, i0, i1, __uint_as_float(q.z), __uint_as_float(q.w), K.inv_dt_sq);
+                });
+            else
             run_program_global<kRolloutBlock, false>(B.recs, B.lvl, B.nlev, tid, [&](const uint4 &q, uint32_t r) {
                 if (q.x == kNopRecord)
                     return;
@@ -764,6 +780,13 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_backward_kernel(Rollout
                 A[i] = make_float4(yb.x, yb.y, yb.z, w);
             }
             __syncthreads();
+            if (B.edges_only)
+                run_program_global<kRolloutBlock, true>(B.recs, B.lvl, B.nlev, tid, [&](const uint4 &q, uint32_t r) {
+                    if (q.x != kNopRecord)
+                        edge_adjoint(A, q.x & 0x3fffu, q.x >> 16, tape[4u * r], tape[4u * r + 1u], __uint_as_float(q.z),
+                                     __uint_as_float(q.w), K.inv_dt_sq, alpha_bar[q.y]);
+                });
+            else
             run_program_global<kRolloutBlock, true>(B.recs, B.lvl, B.nlev, tid, [&](const uint4 &q, uint32_t r) {
                 if (q.x == kNopRecord)
                     return;

@@ -3164,9 +3164,12 @@ int xpbd_rollout_create(xpbd_context *ctx, const xpbd_rollout_desc *d, xpbd_roll
         B.vf_list = s->vf_list.p;
         for (uint32_t L : s->host_level_begin)
             lvl.push_back(L);                       // relative to the body's first record
+        B.edges_only = 1u;
         for (size_t k = 0; k < s->host_recs.size(); ++k)
         {
             uint4 q = s->host_recs[k];
+            if (q.x != kNopRecordHost && ((q.x >> 14) & 3u) != kTypeEdge)
+                B.edges_only = 0u;
             if (q.x != kNopRecordHost && ((q.x >> 14) & 3u) != kTypeVolume && ((q.x >> 14) & 3u) != kTypeBend)
             {
                 // edges: their index in the body (for the compliance parameters) and a compliance

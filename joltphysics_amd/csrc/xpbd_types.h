@@ -274,6 +274,7 @@ struct RolloutBodyDev
     const uint32_t *faces, *vf_begin, *vf_list;
     uint32_t num_faces;
     float pressure;
+    uint32_t edges_only;        // 1: every record is an edge (the level passes skip the type dispatch)
 };
 
 struct RolloutDev

@@ -219,3 +219,61 @@ def test_adjoint_matches_gpu_fp64_central_differences(setup_fd):
         assert normwise < 2e-3, (name, normwise)
     print(f"worst per-parameter relative error (floor 5% of the largest gradient): {worst:.2e}", flush=True)
     assert worst < 1e-3
+
+
+def _sum_grads(r, comp, vel, target64):
+    from joltphysics_amd.autograd import rollout
+    c = _dev(comp).requires_grad_(True)
+    v = _dev(vel).requires_grad_(True)
+    x = rollout(c, v, r)
+    ((x.double() - target64) ** 2).sum().backward()
+    torch.cuda.synchronize()
+    return c.grad.cpu().numpy().astype(np.float64), v.grad.cpu().numpy().astype(np.float64)
+
+
+def test_mixed_bodies_match_separate_rollouts():
+    """One rollout over bodies of different kinds and sizes (a 12 x 12 cloth with dihedral
+    bends, a hanging 3^3-cell tet block with volumes, a 16 x 16 cloth with LRAs) gives each
+    body the positions of a rollout of that body alone, bit for bit, and its gradients to
+    1e-5 relative (LRA anchors add their adjoints atomically, in any order)."""
+    print("rollout of mixed bodies vs one rollout per body: bend cloth, tet block, LRA cloth, 10 frames", flush=True)
+    cases = [scenes.config4_rollout(num_bodies=1, n=12, frames=10, compliance=1e-4, bends=True),
+             scenes.rollout_tet_block(num_bodies=1, cells=3, frames=10),
+             scenes.config4_rollout(num_bodies=1, n=16, frames=10, compliance=1e-4, lra=True)]
+    ctx = Context(0)
+    ctx.set_gravity(cases[0].gravity)
+    rollouts = []
+    try:
+        sts = [ctx.create_settings(build_settings(c.mesh)) for c in cases]
+
+        def make(bodies):
+            r = ctx.create_rollout(bodies, num_iterations=5, dt=1.0 / 60.0, frames=10, checkpoint_interval=4)
+            rollouts.append(r)
+            return r
+
+        mixed = make(sts)
+        comp = np.concatenate([c.compliance for c in cases])
+        vel = np.concatenate([c.velocity for c in cases])
+        x = _forward32(mixed, comp, vel)
+        target = x.astype(np.float64) + 0.01
+        gc, gv = _sum_grads(mixed, comp, vel, torch.from_numpy(target).to("cuda"))
+        vo = eo = 0
+        for c, st in zip(cases, sts):
+            single = make([st])
+            nv, ne = single.num_vertices, single.num_edges
+            xs = _forward32(single, c.compliance, c.velocity)
+            assert np.array_equal(xs, x[vo:vo + nv]), (c.name, np.abs(xs - x[vo:vo + nv]).max())
+            sc, sv = _sum_grads(single, c.compliance, c.velocity, torch.from_numpy(target[vo:vo + nv]).to("cuda"))
+            ec = np.linalg.norm(gc[eo:eo + ne] - sc) / np.linalg.norm(sc)
+            ev = np.linalg.norm(gv[vo:vo + nv] - sv) / np.linalg.norm(sv)
+            print(f"{c.name}: positions bit-identical; gradient relative difference compliance {ec:.2e} velocity {ev:.2e}",
+                  flush=True)
+            assert ec < 1e-5 and ev < 1e-5
+            vo += nv
+            eo += ne
+        assert vo == mixed.num_vertices and eo == mixed.num_edges
+    finally:
+        torch.cuda.synchronize()
+        for r in rollouts:
+            r.close()
+        ctx.close()
