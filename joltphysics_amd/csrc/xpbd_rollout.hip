@@ -536,23 +536,23 @@ __device__ __forceinline__ void forward_substep(const RolloutBodyDev &B, const S
                 edge_project<T>(X, q.x & 0x3fffu, q.x >> 16, (T)__uint_as_float(q.z), comp(q), K.inv_dt_sq);
         });
     else
-    run_program_global<kRolloutBlock, false>(B.recs, B.lvl, B.nlev, tid, [&](const uint4 &q, uint32_t) {
-        if (q.x == kNopRecord)
-            return;
-        if (is_lra_record(q))
-            lra_project<T>(X, q.x & 0x3fffu, q.x >> 16, (T)__uint_as_float(q.z));
-        else if (is_bend_record(q))
-        {
-            if (!is_second_half(q))
-                bend_project<T>(X, q.x & 0x3fffu, q.x >> 16, q.y & 0x7fffu, q.y >> 16, __uint_as_float(q.z),
-                                __uint_as_float(q.w), K.inv_dt_sq);
-        }
-        else if (is_volume_record(q))
-            volume_project<T>(X, q.x & 0x3fffu, q.x >> 16, q.y & 0x7fffu, q.y >> 16, (T)__uint_as_float(q.z),
-                              (T)__uint_as_float(q.w), K.inv_dt_sq);
-        else
-            edge_project<T>(X, q.x & 0x3fffu, q.x >> 16, (T)__uint_as_float(q.z), comp(q), K.inv_dt_sq);
-    });
+        run_program_global<kRolloutBlock, false>(B.recs, B.lvl, B.nlev, tid, [&](const uint4 &q, uint32_t) {
+            if (q.x == kNopRecord)
+                return;
+            if (is_lra_record(q))
+                lra_project<T>(X, q.x & 0x3fffu, q.x >> 16, (T)__uint_as_float(q.z));
+            else if (is_bend_record(q))
+            {
+                if (!is_second_half(q))
+                    bend_project<T>(X, q.x & 0x3fffu, q.x >> 16, q.y & 0x7fffu, q.y >> 16, __uint_as_float(q.z),
+                                    __uint_as_float(q.w), K.inv_dt_sq);
+            }
+            else if (is_volume_record(q))
+                volume_project<T>(X, q.x & 0x3fffu, q.x >> 16, q.y & 0x7fffu, q.y >> 16, (T)__uint_as_float(q.z),
+                                  (T)__uint_as_float(q.w), K.inv_dt_sq);
+            else
+                edge_project<T>(X, q.x & 0x3fffu, q.x >> 16, (T)__uint_as_float(q.z), comp(q), K.inv_dt_sq);
+        });
     for (uint32_t i = tid; i < nv; i += kRolloutBlock)
     {
         const R4<T> p = X[i];
@@ -724,34 +724,34 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_backward_kernel(Rollout
                     edge_project<float>(X, i0, i1, __uint_as_float(q.z), __uint_as_float(q.w), K.inv_dt_sq);
                 });
             else
-            run_program_global<kRolloutBlock, false>(B.recs, B.lvl, B.nlev, tid, [&](const uint4 &q, uint32_t r) {
-                if (q.x == kNopRecord)
-                    return;
-                const uint32_t i0 = q.x & 0x3fffu, i1 = q.x >> 16;
-                tape[4u * r] = A[i0];
-                tape[4u * r + 1u] = A[i1];
-                if (is_volume_record(q))
-                {
-                    const uint32_t i2 = q.y & 0x7fffu, i3 = q.y >> 16;
-                    tape[4u * r + 2u] = A[i2];
-                    tape[4u * r + 3u] = A[i3];
-                    volume_project<float>(X, i0, i1, i2, i3, __uint_as_float(q.z), __uint_as_float(q.w), K.inv_dt_sq);
-                }
-                else if (is_bend_record(q))
-                {
-                    if (!is_second_half(q))
+                run_program_global<kRolloutBlock, false>(B.recs, B.lvl, B.nlev, tid, [&](const uint4 &q, uint32_t r) {
+                    if (q.x == kNopRecord)
+                        return;
+                    const uint32_t i0 = q.x & 0x3fffu, i1 = q.x >> 16;
+                    tape[4u * r] = A[i0];
+                    tape[4u * r + 1u] = A[i1];
+                    if (is_volume_record(q))
                     {
                         const uint32_t i2 = q.y & 0x7fffu, i3 = q.y >> 16;
                         tape[4u * r + 2u] = A[i2];
                         tape[4u * r + 3u] = A[i3];
-                        bend_project<float>(X, i0, i1, i2, i3, __uint_as_float(q.z), __uint_as_float(q.w), K.inv_dt_sq);
+                        volume_project<float>(X, i0, i1, i2, i3, __uint_as_float(q.z), __uint_as_float(q.w), K.inv_dt_sq);
                     }
-                }
-                else if (is_lra_record(q))
-                    lra_project<float>(X, i0, i1, __uint_as_float(q.z));
-                else
-                    edge_project<float>(X, i0, i1, __uint_as_float(q.z), __uint_as_float(q.w), K.inv_dt_sq);
-            });
+                    else if (is_bend_record(q))
+                    {
+                        if (!is_second_half(q))
+                        {
+                            const uint32_t i2 = q.y & 0x7fffu, i3 = q.y >> 16;
+                            tape[4u * r + 2u] = A[i2];
+                            tape[4u * r + 3u] = A[i3];
+                            bend_project<float>(X, i0, i1, i2, i3, __uint_as_float(q.z), __uint_as_float(q.w), K.inv_dt_sq);
+                        }
+                    }
+                    else if (is_lra_record(q))
+                        lra_project<float>(X, i0, i1, __uint_as_float(q.z));
+                    else
+                        edge_project<float>(X, i0, i1, __uint_as_float(q.z), __uint_as_float(q.w), K.inv_dt_sq);
+                });
             // adjoint of the velocity update (and of the clamp after the last iteration)
             for (uint32_t i = tid; i < nv; i += kRolloutBlock)
             {
@@ -787,30 +787,30 @@ __global__ __launch_bounds__(kRolloutBlock) void rollout_backward_kernel(Rollout
                                      __uint_as_float(q.w), K.inv_dt_sq, alpha_bar[q.y]);
                 });
             else
-            run_program_global<kRolloutBlock, true>(B.recs, B.lvl, B.nlev, tid, [&](const uint4 &q, uint32_t r) {
-                if (q.x == kNopRecord)
-                    return;
-                if (is_lra_record(q))
-                {
-                    lra_adjoint(A, q.x & 0x3fffu, q.x >> 16, tape[4u * r], tape[4u * r + 1u], __uint_as_float(q.z));
-                    return;
-                }
-                if (is_bend_record(q))
-                {
-                    if (!is_second_half(q))
-                        bend_adjoint(A, q.x & 0x3fffu, q.x >> 16, q.y & 0x7fffu, q.y >> 16, tape + 4u * r, __uint_as_float(q.z),
-                                     __uint_as_float(q.w), K.inv_dt_sq);
-                    return;
-                }
-                if (is_volume_record(q))
-                {
-                    volume_adjoint(A, q.x & 0x3fffu, q.x >> 16, q.y & 0x7fffu, q.y >> 16, tape + 4u * r, __uint_as_float(q.z),
-                                   __uint_as_float(q.w), K.inv_dt_sq);
-                    return;
-                }
-                edge_adjoint(A, q.x & 0x3fffu, q.x >> 16, tape[4u * r], tape[4u * r + 1u], __uint_as_float(q.z),
-                             __uint_as_float(q.w), K.inv_dt_sq, alpha_bar[q.y]);
-            });
+                run_program_global<kRolloutBlock, true>(B.recs, B.lvl, B.nlev, tid, [&](const uint4 &q, uint32_t r) {
+                    if (q.x == kNopRecord)
+                        return;
+                    if (is_lra_record(q))
+                    {
+                        lra_adjoint(A, q.x & 0x3fffu, q.x >> 16, tape[4u * r], tape[4u * r + 1u], __uint_as_float(q.z));
+                        return;
+                    }
+                    if (is_bend_record(q))
+                    {
+                        if (!is_second_half(q))
+                            bend_adjoint(A, q.x & 0x3fffu, q.x >> 16, q.y & 0x7fffu, q.y >> 16, tape + 4u * r, __uint_as_float(q.z),
+                                         __uint_as_float(q.w), K.inv_dt_sq);
+                        return;
+                    }
+                    if (is_volume_record(q))
+                    {
+                        volume_adjoint(A, q.x & 0x3fffu, q.x >> 16, q.y & 0x7fffu, q.y >> 16, tape + 4u * r, __uint_as_float(q.z),
+                                       __uint_as_float(q.w), K.inv_dt_sq);
+                        return;
+                    }
+                    edge_adjoint(A, q.x & 0x3fffu, q.x >> 16, tape[4u * r], tape[4u * r + 1u], __uint_as_float(q.z),
+                                 __uint_as_float(q.w), K.inv_dt_sq, alpha_bar[q.y]);
+                });
             // adjoint of prev = x0, y0 = x0 + v1 h and the integration of v
             for (uint32_t i = tid; i < nv; i += kRolloutBlock)
             {
