@@ -348,6 +348,32 @@ def cloth_on_compounds_streamed():
     return sc
 
 
+def degenerate_bodies(frames=20):
+    """Degenerate but finite inputs, each of which the reference's guards skip: a 12 x 12 cloth
+    whose first row has pairs of coincident vertices (zero-length edges, rest length 0:
+    denominator 0 in ApplyEdgeConstraints) and a pinned second row (edges between two vertices
+    of inverse mass 0), with distance bends; and a 2^3-cell tet block whose top layer is
+    squashed onto the middle one (coincident vertices, zero-volume tetrahedra: six rest volume 0).
+    Both fall onto a plane and a sphere."""
+    n = 12
+    v, f = scenes.cloth(n, 0.05, 100.0, jitter=0.0)
+    for i in range(0, n - 1, 2):
+        v["pos"][i + 1] = v["pos"][i]
+    v["inv_mass"][n:2 * n] = 0.0
+    cloth = MeshSpec(vertices=v, faces=f, attrs=scenes.attrs(1e-5, 1e-5, 1e-4), bend_type=BEND_DISTANCE)
+    tv, te, tt = scenes.tet_cube(2, 0.1, inv_mass=2.0, compliance=1e-5)
+    z = tv["pos"][:, 2]
+    levels = np.unique(z)
+    tv["pos"][z == levels[-1], 2] = levels[-2]
+    block = MeshSpec(vertices=tv, edges=te, volumes=tt, flags=CALC_EDGE_LENGTHS | CALC_VOLUMES | OPTIMIZE)
+    bodies = [BodySpec(mesh=cloth, position=(0.0, 0.0, 0.3), params=BodyParams(num_iterations=5, allow_sleeping=False)),
+              BodySpec(mesh=block, position=(0.2, 0.1, 0.6), rotation=_quat((1, 1, 0), 0.3),
+                       params=BodyParams(num_iterations=5, allow_sleeping=False))]
+    cols = _bounded([Collider(SHAPE_PLANE, (0.0, 0.0, 1.0, 0.0, 50.0)),
+                     Collider(SHAPE_SPHERE, (0.15,), pos=(0.25, 0.25, 0.1))])
+    return Scene("degenerate_bodies", bodies, cols, gravity=(0.0, 0.0, -9.81), dt=1 / 60, collision_steps=1, frames=frames)
+
+
 ALL = {
     "cloth100_sphere": big_cloth_on_sphere,
     "tetcube_plane_box": tet_cube_on_plane,
@@ -371,6 +397,7 @@ ALL = {
     "cloth_on_height_field_uneven": cloth_on_height_field_uneven,
     "cloth_on_compounds": cloth_on_compounds,
     "cloth_on_compounds_streamed": cloth_on_compounds_streamed,
+    "degenerate_bodies": degenerate_bodies,
 }
 
 
